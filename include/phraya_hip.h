@@ -1,0 +1,169 @@
+/*
+ * phraya_hip.h -- C ABI of libphraya_hip.so, the MI355X (gfx950) extension
+ * engines of phraya-align.
+ *
+ * The per-pair entry points replace phraya-align's extension API one for one
+ * (paths relative to the upstream repository root):
+ *   phx_wfa_extend           <- wfa_extend / wfa_extend_naive    phraya-align/src/lib.rs:63-76
+ *   phx_wfa_extend_affine    <- wfa_extend_affine                phraya-align/src/lib.rs:139-167
+ *   phx_myers_extend         <- myers_extend                     phraya-align/src/lib.rs:106-127
+ *   phx_myers_edit_distance  <- myers_edit_distance              phraya-align/src/lib.rs:94-96
+ * and phx_align_batch is the batched form of all four that a batched
+ * extend_chains (phraya-align/src/executor.rs:528-567) calls once per read
+ * batch.  Results are bit-identical to the upstream engines: score, edit
+ * distance, query/target end and CIGAR, including tie-breaking.
+ *
+ * CIGAR convention (phraya-align/src/wfa_simd.rs:502-506): M = exact match,
+ * X = mismatch, I = target-only base, D = query-only base.  Runs are returned
+ * as u32 words (length << 4 | op) with op 0 M, 1 X, 2 I, 3 D;
+ * phx_format_cigar renders them as phraya's "<count><op>" text.
+ *
+ * Threading: a context serialises its own submissions (internal lock), so one
+ * context may be shared by many host threads; results never depend on what
+ * else runs on the device.  No C++ exception crosses this boundary.
+ */
+#ifndef PHRAYA_HIP_H
+#define PHRAYA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes, per pair and per call. */
+enum {
+  PHX_OK = 0,
+  PHX_INVALID_INPUT = 1,    /* WfaError::InvalidInput: seed beyond a sequence (lib.rs:107-111, :145-149) */
+  PHX_ALIGNMENT_FAILED = 2, /* WfaError::AlignmentFailed: cap / search bound exhausted (lib.rs:163-165) */
+  PHX_E_ARG = 32,           /* bad argument to the call itself */
+  PHX_E_DEVICE = 33,        /* HIP runtime error */
+  PHX_E_NOMEM = 34,         /* device memory exhausted */
+  PHX_E_CAPACITY = 35,      /* caller's CIGAR arena too small; run_off[n] holds the size needed */
+  PHX_E_INTERNAL = 36       /* an alignment could not be completed by any kernel tier */
+};
+
+/* Engines (what phraya-align's extend_anchor dispatches to, executor.rs:379-404). */
+enum {
+  PHX_ENGINE_LINEAR = 0,       /* wfa_extend: unit-cost WFA, fitting iff tn > qn + qn/2 + 10 */
+  PHX_ENGINE_AFFINE = 1,       /* wfa_extend_affine: gap-affine WFA */
+  PHX_ENGINE_MYERS = 2,        /* myers_extend: Myers, fitting iff tn > qn + qn/2 + 10 */
+  PHX_ENGINE_MYERS_GLOBAL = 3, /* myers_edit_distance: global Myers (seeds ignored) */
+  PHX_ENGINE_MYERS_DIST = 4    /* myers_extend without the backtrace: distance + end column */
+};
+
+/* AffineCosts (phraya-align/src/wfa_simd.rs:520-525): an L-base gap costs
+ * gap_open + L * gap_extend.  The device path requires mismatch >= 1,
+ * gap_extend >= 1, gap_open >= 0 (the upstream loop cannot run otherwise). */
+typedef struct phx_costs {
+  int32_t mismatch;
+  int32_t gap_open;
+  int32_t gap_extend;
+} phx_costs;
+
+/* Alignment (phraya-align/src/lib.rs:31-39) plus the engine's search score. */
+typedef struct phx_alignment {
+  int32_t status;
+  int32_t reserved;
+  int64_t score;          /* linear s, affine s, or Myers distance */
+  int64_t edit_distance;  /* what upstream reports as edit_distance */
+  int64_t query_start, query_end, target_start, target_end;
+} phx_alignment;
+
+/* A batch of pairs in host memory: sequence i is seq[off[i] .. off[i+1]). */
+typedef struct phx_batch {
+  uint32_t n;
+  int32_t engine;
+  phx_costs costs;        /* PHX_ENGINE_AFFINE only */
+  int64_t cap;            /* max search score (upstream max_s_cap); < 0 = none */
+  const uint8_t* q_seq;
+  const uint64_t* q_off;  /* n + 1 entries */
+  const uint8_t* t_seq;
+  const uint64_t* t_off;  /* n + 1 entries */
+  const uint64_t* seed_q; /* may be NULL (all zero) */
+  const uint64_t* seed_t; /* may be NULL (all zero) */
+} phx_batch;
+
+/* Caller-owned result arrays (n entries each; `runs` holds runs_cap words).
+ * Pair i's CIGAR runs are runs[run_off[i] .. run_off[i+1]).  For
+ * PHX_ENGINE_MYERS_DIST no runs are produced and runs may be NULL. */
+typedef struct phx_results {
+  phx_alignment* aln;
+  uint32_t* runs;
+  uint64_t runs_cap;
+  uint64_t* run_off;      /* n + 1 entries */
+} phx_results;
+
+typedef struct phx_ctx phx_ctx;
+typedef struct phx_plan phx_plan;
+
+/* Tuning knobs; NULL selects the built-in defaults. */
+typedef struct phx_tuning {
+  uint32_t wave_wcap;        /* ring width (diagonals) of the one-wave tier; power of two */
+  uint32_t wave_hist_kib;    /* history per workgroup slot, one-wave tier */
+  uint32_t wave_slots;       /* workgroups of the one-wave tier */
+  uint32_t block_wcap;       /* ring width of the 256-thread LDS tier */
+  uint32_t block_hist_kib;
+  uint32_t block_slots;
+  uint64_t hbm_budget_mib;   /* device scratch budget for the HBM tiers */
+} phx_tuning;
+
+/* Context: owns the stream, device buffers and the submission lock.
+ * device < 0 selects the current device. Returns NULL on failure. */
+phx_ctx* phx_ctx_create(int device, uint32_t flags, const phx_tuning* tuning);
+void phx_ctx_destroy(phx_ctx* ctx);
+/* Last error text of this context (static storage owned by the context). */
+const char* phx_ctx_last_error(const phx_ctx* ctx);
+
+/* Upper bound of CIGAR run words a batch can need (sum of qn + tn + 1). */
+uint64_t phx_cigar_bound(const phx_batch* batch);
+
+/* Align a batch end to end (pack, upload, run, download). Returns PHX_OK when
+ * every pair got a final status (PHX_OK / PHX_INVALID_INPUT /
+ * PHX_ALIGNMENT_FAILED in aln[i].status), else a PHX_E_* code. */
+int phx_align_batch(phx_ctx* ctx, const phx_batch* batch, phx_results* out);
+
+/* Split form of phx_align_batch for callers that keep inputs resident in HBM
+ * and launch repeatedly: create packs and uploads, run launches the kernels
+ * on the resident inputs (results stay in HBM), results downloads. */
+phx_plan* phx_plan_create(phx_ctx* ctx, const phx_batch* batch);
+int phx_plan_run(phx_plan* plan);
+int phx_plan_results(phx_plan* plan, phx_results* out);
+/* Device time of the last phx_plan_run, in milliseconds (HIP events around
+ * the whole run), and the summed time of its alignment kernel launches alone
+ * (events recorded on the launch stream right before and after each launch). */
+double phx_plan_last_ms(const phx_plan* plan);
+double phx_plan_last_kernel_ms(const phx_plan* plan);
+uint32_t phx_plan_last_launches(const phx_plan* plan);
+/* Algorithmic HBM bytes of one run: packed inputs + results + CIGAR runs. */
+uint64_t phx_plan_algorithmic_bytes(const phx_plan* plan);
+void phx_plan_destroy(phx_plan* plan);
+
+/* Render runs as phraya CIGAR text. Writes at most cap bytes including the
+ * NUL; returns the full text length (excluding the NUL). */
+size_t phx_format_cigar(const uint32_t* runs, uint64_t nruns, char* buf, size_t cap);
+
+/* Per-pair forms of the upstream API.  The CIGAR text is written to cigar
+ * (NUL-terminated, truncated to cigar_cap); the return value is the pair
+ * status or a PHX_E_* code.  cigar_len (may be NULL) gets the full length. */
+int phx_wfa_extend(phx_ctx* ctx, const uint8_t* q, size_t qlen, const uint8_t* t, size_t tlen,
+                   uint64_t seed_q, uint64_t seed_t, phx_alignment* out, char* cigar,
+                   size_t cigar_cap, size_t* cigar_len);
+int phx_wfa_extend_affine(phx_ctx* ctx, const uint8_t* q, size_t qlen, const uint8_t* t,
+                          size_t tlen, uint64_t seed_q, uint64_t seed_t, phx_costs costs,
+                          phx_alignment* out, char* cigar, size_t cigar_cap, size_t* cigar_len);
+int phx_myers_extend(phx_ctx* ctx, const uint8_t* q, size_t qlen, const uint8_t* t, size_t tlen,
+                     uint64_t seed_q, uint64_t seed_t, phx_alignment* out, char* cigar,
+                     size_t cigar_cap, size_t* cigar_len);
+int phx_myers_edit_distance(phx_ctx* ctx, const uint8_t* q, size_t qlen, const uint8_t* t,
+                            size_t tlen, int64_t* edit_distance, char* cigar, size_t cigar_cap,
+                            size_t* cigar_len);
+
+/* Library version string. */
+const char* phx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,184 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg, and there only as the checker or the timed CPU
+baseline.  The product package (phraya_amd) never imports this module.
+
+The oracle restates phraya-align's engines (phraya-align/src/wfa_simd.rs,
+phraya-align/src/lib.rs); see phraya_oracle.cpp for the per-function
+citations.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+OK, INVALID_INPUT, ALIGNMENT_FAILED, BAD_COSTS = 0, 1, 2, 3
+LINEAR, AFFINE, MYERS, MYERS_GLOBAL, MYERS_FWD = 0, 1, 2, 3, 4
+
+
+class OrcResult(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("score", ctypes.c_int64),
+        ("edit_distance", ctypes.c_int64),
+        ("query_start", ctypes.c_int64),
+        ("query_end", ctypes.c_int64),
+        ("target_start", ctypes.c_int64),
+        ("target_end", ctypes.c_int64),
+        ("cigar_len", ctypes.c_int64),
+    ]
+
+
+RESULT_DTYPE = np.dtype(
+    [
+        ("status", np.int32),
+        ("reserved", np.int32),
+        ("score", np.int64),
+        ("edit_distance", np.int64),
+        ("query_start", np.int64),
+        ("query_end", np.int64),
+        ("target_start", np.int64),
+        ("target_end", np.int64),
+        ("cigar_len", np.int64),
+    ]
+)
+assert RESULT_DTYPE.itemsize == ctypes.sizeof(OrcResult)
+
+
+def build() -> str:
+    """Compile liboracle.so with the Makefile next to this file."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.c_void_p
+        sz = ctypes.c_size_t
+        i32, i64, u32 = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint32
+        L.orc_lcp.argtypes = [P, sz, P, sz]
+        L.orc_lcp.restype = sz
+        L.orc_lcp_scalar.argtypes = [P, sz, P, sz]
+        L.orc_lcp_scalar.restype = sz
+        L.orc_lcp_u64.argtypes = [P, sz, P, sz]
+        L.orc_lcp_u64.restype = sz
+        R = ctypes.POINTER(OrcResult)
+        L.orc_wfa_extend.argtypes = [P, sz, P, sz, sz, sz, i64, R, P, sz]
+        L.orc_wfa_extend_affine.argtypes = [P, sz, P, sz, sz, sz, i32, i32, i32, i64, R, P, sz]
+        L.orc_myers_extend.argtypes = [P, sz, P, sz, sz, sz, R, P, sz]
+        L.orc_myers_edit_distance.argtypes = [P, sz, P, sz, R, P, sz]
+        L.orc_myers_extend_distance.argtypes = [P, sz, P, sz, sz, sz, R]
+        L.orc_batch_run.argtypes = [ctypes.c_int, u32, P, P, P, P, P, P, i32, i32, i32, i64,
+                                    ctypes.c_int, P]
+        L.orc_batch_run.restype = P
+        L.orc_batch_cigar_bytes.argtypes = [P]
+        L.orc_batch_cigar_bytes.restype = ctypes.c_uint64
+        L.orc_batch_cigar_copy.argtypes = [P, P, P]
+        L.orc_batch_free.argtypes = [P]
+        L.orc_hardware_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+@dataclass
+class Result:
+    status: int
+    cigar: str
+    score: int
+    edit_distance: int
+    query_start: int
+    query_end: int
+    target_start: int
+    target_end: int
+
+
+def _buf(b: bytes):
+    return ctypes.c_char_p(b) if b else ctypes.c_char_p(b"\0")
+
+
+def _call(fn, *args) -> Result:
+    cap = 1 << 12
+    while True:
+        out = ctypes.create_string_buffer(cap)
+        r = OrcResult()
+        fn(*args, ctypes.byref(r), out, cap)
+        if r.status != OK or r.cigar_len < cap:
+            return Result(r.status, out.value.decode() if r.status == OK else "", r.score,
+                          r.edit_distance, r.query_start, r.query_end, r.target_start, r.target_end)
+        cap = int(r.cigar_len) + 1
+
+
+def wfa_extend(q: bytes, t: bytes, seed=(0, 0), cap: int = -1) -> Result:
+    return _call(lib().orc_wfa_extend, _buf(q), len(q), _buf(t), len(t), seed[0], seed[1], cap)
+
+
+def wfa_extend_affine(q: bytes, t: bytes, seed=(0, 0), costs=(2, 4, 1), cap: int = -1) -> Result:
+    x, o, e = costs
+    return _call(lib().orc_wfa_extend_affine, _buf(q), len(q), _buf(t), len(t), seed[0], seed[1],
+                 x, o, e, cap)
+
+
+def myers_extend(q: bytes, t: bytes, seed=(0, 0)) -> Result:
+    return _call(lib().orc_myers_extend, _buf(q), len(q), _buf(t), len(t), seed[0], seed[1])
+
+
+def myers_edit_distance(q: bytes, t: bytes) -> tuple[int, str]:
+    r = _call(lib().orc_myers_edit_distance, _buf(q), len(q), _buf(t), len(t))
+    return r.edit_distance, r.cigar
+
+
+def lcp(a: bytes, b: bytes, tier: str = "sse2") -> int:
+    fn = {"sse2": lib().orc_lcp, "scalar": lib().orc_lcp_scalar, "u64": lib().orc_lcp_u64}[tier]
+    return fn(_buf(a), len(a), _buf(b), len(b))
+
+
+def hardware_threads() -> int:
+    return lib().orc_hardware_threads()
+
+
+def batch(engine: int, q_seq: np.ndarray, q_off: np.ndarray, t_seq: np.ndarray, t_off: np.ndarray,
+          seed_q=None, seed_t=None, costs=(2, 4, 1), cap: int = -1, threads: int = 1,
+          want_cigars: bool = True):
+    """Run the oracle over a concatenated batch.  Returns (results, cigars or None)."""
+    L = lib()
+    n = len(q_off) - 1
+    q_seq = np.ascontiguousarray(q_seq, dtype=np.uint8)
+    t_seq = np.ascontiguousarray(t_seq, dtype=np.uint8)
+    q_off = np.ascontiguousarray(q_off, dtype=np.uint64)
+    t_off = np.ascontiguousarray(t_off, dtype=np.uint64)
+    sq = None if seed_q is None else np.ascontiguousarray(seed_q, dtype=np.uint64)
+    st = None if seed_t is None else np.ascontiguousarray(seed_t, dtype=np.uint64)
+    res = np.zeros(n, dtype=RESULT_DTYPE)
+    x, o, e = costs
+    h = L.orc_batch_run(engine, n, q_seq.ctypes.data, q_off.ctypes.data, t_seq.ctypes.data,
+                        t_off.ctypes.data, None if sq is None else sq.ctypes.data,
+                        None if st is None else st.ctypes.data, x, o, e, cap, threads,
+                        res.ctypes.data)
+    try:
+        cigars = None
+        if want_cigars:
+            total = L.orc_batch_cigar_bytes(h)
+            arena = np.zeros(max(int(total), 1), dtype=np.uint8)
+            off = np.zeros(n + 1, dtype=np.uint64)
+            L.orc_batch_cigar_copy(h, arena.ctypes.data, off.ctypes.data)
+            raw = arena.tobytes()
+            cigars = [raw[int(off[i]):int(off[i + 1])].decode() for i in range(n)]
+    finally:
+        L.orc_batch_free(h)
+    return res, cigars

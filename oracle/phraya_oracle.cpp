@@ -1,0 +1,884 @@
+// phraya_oracle.cpp -- CPU restatement of phraya-align's extension engines.
+//
+// TEST INFRASTRUCTURE ONLY: the parity checker for the HIP path and the CPU
+// baseline timed beside it.  The product never links or calls this file.
+//
+// Each function below follows the upstream Rust function named in its comment
+// (paths relative to the upstream repository root).  The restatement keeps
+// the upstream's data layout where it shapes cost (full-width frames pushed
+// per score, an O(m) column rebuild per Myers backtrace step) so that the
+// CPU baseline times the same algorithm the upstream runs.
+#include "phraya_oracle.h"
+
+#include <emmintrin.h>
+
+#include <algorithm>
+#include <atomic>
+#include <climits>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Longest common prefix (phraya-align/src/wfa_simd.rs:52-155)
+// ---------------------------------------------------------------------------
+
+// Byte-by-byte tier (wfa_simd.rs:123-130).
+inline size_t lcp_scalar(const uint8_t* a, size_t an, const uint8_t* b, size_t bn) {
+  size_t n = std::min(an, bn), i = 0;
+  while (i < n && a[i] == b[i]) ++i;
+  return i;
+}
+
+// 16-byte SSE2 tier used in production on x86_64 (wfa_simd.rs:69-90): compare
+// 16 bytes, movemask, first zero bit of the mask is the first mismatch; the
+// sub-16-byte tail goes to the scalar tier.
+inline size_t lcp_sse2(const uint8_t* a, size_t an, const uint8_t* b, size_t bn) {
+  size_t n = std::min(an, bn), i = 0;
+  while (i + 16 <= n) {
+    __m128i va = _mm_loadu_si128(reinterpret_cast<const __m128i*>(a + i));
+    __m128i vb = _mm_loadu_si128(reinterpret_cast<const __m128i*>(b + i));
+    unsigned mask = static_cast<unsigned>(_mm_movemask_epi8(_mm_cmpeq_epi8(va, vb))) & 0xFFFFu;
+    if (mask != 0xFFFFu) return i + static_cast<size_t>(__builtin_ctz(mask ^ 0xFFFFu));
+    i += 16;
+  }
+  return i + lcp_scalar(a + i, n - i, b + i, n - i);
+}
+
+// 8-byte little-endian XOR tier (wfa_simd.rs:139-155).
+inline size_t lcp_u64(const uint8_t* a, size_t an, const uint8_t* b, size_t bn) {
+  size_t n = std::min(an, bn), i = 0;
+  while (i + 8 <= n) {
+    uint64_t aw, bw;
+    std::memcpy(&aw, a + i, 8);
+    std::memcpy(&bw, b + i, 8);
+    uint64_t diff = aw ^ bw;
+    if (diff) return i + static_cast<size_t>(__builtin_ctzll(diff)) / 8;
+    i += 8;
+  }
+  while (i < n && a[i] == b[i]) ++i;
+  return i;
+}
+
+inline size_t lcp(const uint8_t* a, size_t an, const uint8_t* b, size_t bn) {
+  return lcp_sse2(a, an, b, bn);
+}
+
+// ---------------------------------------------------------------------------
+// CIGAR helpers
+// ---------------------------------------------------------------------------
+
+// compact_cigar (wfa_simd.rs:1200-1222): run-length encode op characters.
+std::string compact_cigar(const std::vector<char>& ops) {
+  std::string out;
+  if (ops.empty()) return out;
+  char cur = ops[0];
+  size_t count = 1;
+  for (size_t i = 1; i < ops.size(); ++i) {
+    if (ops[i] == cur) {
+      ++count;
+    } else {
+      out += std::to_string(count);
+      out.push_back(cur);
+      cur = ops[i];
+      count = 1;
+    }
+  }
+  out += std::to_string(count);
+  out.push_back(cur);
+  return out;
+}
+
+// levenshtein_from_cigar (wfa_simd.rs:893-908): sum of all non-M run lengths.
+size_t levenshtein_from_cigar(const std::string& cigar) {
+  size_t total = 0, num = 0;
+  for (char ch : cigar) {
+    if (ch >= '0' && ch <= '9') {
+      num = num * 10 + static_cast<size_t>(ch - '0');
+    } else {
+      if (ch != 'M') total += num;
+      num = 0;
+    }
+  }
+  return total;
+}
+
+// ---------------------------------------------------------------------------
+// Linear (unit-cost) WFA (wfa_simd.rs:231-488, :940-1130)
+// ---------------------------------------------------------------------------
+
+constexpr int32_t UNSET = INT32_MIN;
+
+struct LinFrame {
+  std::vector<int32_t> wf;
+  std::vector<uint8_t> op;  // 0 none, 1 X (same k), 2 query-only 'D' (from k-1), 3 target-only 'I' (from k+1)
+};
+
+struct Seqs {
+  const uint8_t* q;
+  int32_t qn;
+  const uint8_t* t;
+  int32_t tn;
+  // Greedy diagonal extension (wfa_simd.rs:252-262): unchanged outside the
+  // matrix, else position + LCP of the two suffixes.
+  int32_t extend(int32_t qp, int32_t k) const {
+    int32_t j = qp - k;
+    if (qp < 0 || qp >= qn || j < 0 || j >= tn) return qp;
+    return qp + static_cast<int32_t>(lcp(q + qp, static_cast<size_t>(qn - qp),
+                                         t + j, static_cast<size_t>(tn - j)));
+  }
+};
+
+// One score step of the linear recurrence, identical in the global
+// (wfa_simd.rs:283-330) and fitting (:430-469) loops.
+void linear_step(const Seqs& S, int32_t s, const std::vector<int32_t>& prev,
+                 std::vector<int32_t>& next, std::vector<uint8_t>& ops) {
+  const int32_t qn = S.qn, tn = S.tn;
+  const int32_t lo = -std::min(s, tn);
+  const int32_t hi = std::min(s, qn);
+  for (int32_t k = lo; k <= hi; ++k) {
+    const size_t ki = static_cast<size_t>(k + tn);
+    // Candidates in upstream order [mismatch, ins, del]; max_by_key keeps the
+    // LAST maximum, so a later candidate wins ties.
+    int32_t best = UNSET;
+    uint8_t best_op = 0;
+    bool have = false;
+    auto offer = [&](int32_t p, uint8_t op) {
+      if (!have || p >= best) {
+        best = p;
+        best_op = op;
+        have = true;
+      }
+    };
+    if (prev[ki] != UNSET) offer(prev[ki] + 1, 1);
+    if (k > -tn) {
+      size_t pk = static_cast<size_t>(k - 1 + tn);
+      if (prev[pk] != UNSET) offer(prev[pk] + 1, 2);
+    }
+    if (k < qn) {
+      size_t pk = static_cast<size_t>(k + 1 + tn);
+      if (prev[pk] != UNSET) offer(prev[pk], 3);
+    }
+    if (!have) continue;
+    // Bounds check after the choice: an out-of-range winner leaves the cell
+    // unset, with no fallback to another candidate (wfa_simd.rs:324-326).
+    int32_t j = best - k;
+    if (best < 0 || best > qn || j < 0 || j > tn) continue;
+    next[ki] = S.extend(best, k);
+    ops[ki] = best_op;
+  }
+}
+
+// traceback_wfa / traceback_wfa_with_tend (wfa_simd.rs:959-1130).  The two
+// upstream functions differ only in the starting target index.
+std::string linear_traceback(const std::vector<LinFrame>& hist, const Seqs& S,
+                             int32_t edit_dist, int32_t t_end) {
+  const int32_t tn = S.tn;
+  std::vector<char> ops;
+  int32_t qi = S.qn, ti = t_end;
+  int32_t k = qi - ti;
+  int32_t s = edit_dist;
+  auto prev_at = [&](const std::vector<int32_t>& wf, int64_t pk) -> int32_t {
+    return (pk >= 0 && pk < static_cast<int64_t>(wf.size())) ? wf[static_cast<size_t>(pk)] : 0;
+  };
+  while (qi > 0 || ti > 0) {
+    if (s < 0) break;
+    int64_t ki = static_cast<int64_t>(k) + tn;
+    if (s == 0) {
+      while (qi > 0 && ti > 0) {
+        ops.push_back('M');
+        --qi;
+        --ti;
+      }
+      break;
+    }
+    const LinFrame& cur = hist[static_cast<size_t>(s)];
+    uint8_t op = (ki >= 0 && ki < static_cast<int64_t>(cur.op.size())) ? cur.op[static_cast<size_t>(ki)] : 0;
+    const std::vector<int32_t>& pw = hist[static_cast<size_t>(s) - 1].wf;
+    int32_t pred_pos, pred_k;
+    switch (op) {
+      case 1: pred_pos = prev_at(pw, ki); pred_k = k; break;
+      case 2: pred_pos = prev_at(pw, ki - 1); pred_k = k - 1; break;
+      case 3: pred_pos = prev_at(pw, ki + 1); pred_k = k + 1; break;
+      default: pred_pos = prev_at(pw, ki); pred_k = k; break;
+    }
+    int32_t edit_start = (op == 1 || op == 2) ? pred_pos + 1 : pred_pos;
+    int32_t match_count = qi - edit_start;
+    for (int32_t i = 0; i < match_count; ++i) {
+      ops.push_back('M');
+      --qi;
+      --ti;
+    }
+    if (op == 1) {
+      ops.push_back('X');
+      --qi;
+      --ti;
+    } else if (op == 2) {
+      ops.push_back('D');
+      --qi;
+    } else if (op == 3) {
+      ops.push_back('I');
+      --ti;
+    }
+    k = pred_k;
+    --s;
+  }
+  std::reverse(ops.begin(), ops.end());
+  return compact_cigar(ops);
+}
+
+// fill_wfa (wfa_simd.rs:231-345): global unit-cost WFA.
+std::pair<std::string, int32_t> fill_wfa(const Seqs& S) {
+  const int32_t qn = S.qn, tn = S.tn;
+  const size_t size = static_cast<size_t>(qn + tn + 1);
+  std::vector<LinFrame> hist;
+  LinFrame f0{std::vector<int32_t>(size, UNSET), std::vector<uint8_t>(size, 0)};
+  f0.wf[static_cast<size_t>(tn)] = S.extend(0, 0);
+  const size_t target_idx = static_cast<size_t>(qn);  // diagonal qn - tn, index (qn - tn) + tn
+  if (f0.wf[target_idx] == qn) return {linear_traceback(hist, S, 0, tn), 0};
+  hist.push_back(std::move(f0));
+  const int32_t max_s = qn + tn;
+  for (int32_t s = 1; s <= max_s; ++s) {
+    LinFrame nf{std::vector<int32_t>(size, UNSET), std::vector<uint8_t>(size, 0)};
+    linear_step(S, s, hist[static_cast<size_t>(s) - 1].wf, nf.wf, nf.op);
+    bool done = nf.wf[target_idx] >= qn;
+    hist.push_back(std::move(nf));
+    if (done) return {linear_traceback(hist, S, s, tn), s};
+  }
+  return {std::string(), 0};  // upstream fallback (wfa_simd.rs:343-344)
+}
+
+// fitting_end_k (wfa_simd.rs:940-955): smallest k (largest t_end) with wf >= qn.
+bool fitting_end_k(const std::vector<int32_t>& wf, int32_t qn, int32_t tn, int32_t* t_end) {
+  bool found = false;
+  int32_t best = 0;
+  for (int32_t k = qn - tn; k <= qn; ++k) {
+    int32_t te = qn - k;
+    if (te < 0 || te > tn) continue;
+    size_t ki = static_cast<size_t>(k + tn);
+    if (ki >= wf.size()) continue;
+    if (wf[ki] >= qn && (!found || te > best)) {
+      best = te;
+      found = true;
+    }
+  }
+  *t_end = best;
+  return found;
+}
+
+struct EngineOut {
+  bool some;
+  std::string cigar;
+  int64_t score;          // search score
+  int64_t edit_distance;  // reported distance
+  int64_t t_consumed;
+};
+
+// fill_wfa_fitting_impl (wfa_simd.rs:360-482).  cap < 0 = None.
+EngineOut fill_wfa_fitting_impl(const Seqs& S, int64_t cap) {
+  const int32_t qn = S.qn, tn = S.tn;
+  if (qn == 0) return {true, std::string(), 0, 0, 0};
+  if (tn <= qn + qn / 2 + 10) {
+    auto r = fill_wfa(S);
+    if (cap >= 0 && static_cast<int64_t>(r.second) > cap) return {false, {}, 0, 0, 0};
+    return {true, r.first, r.second, r.second, tn};
+  }
+  const size_t size = static_cast<size_t>(qn + tn + 1);
+  std::vector<LinFrame> hist;
+  LinFrame f0{std::vector<int32_t>(size, UNSET), std::vector<uint8_t>(size, 0)};
+  f0.wf[static_cast<size_t>(tn)] = S.extend(0, 0);
+  const int32_t max_s = qn + tn;
+  // `(cap as i32).min(max_s)`: caps in this API are non-negative i32 values.
+  const int32_t explore = cap >= 0 ? static_cast<int32_t>(std::min<int64_t>(cap, max_s)) : max_s;
+  int32_t t_end;
+  if (fitting_end_k(f0.wf, qn, tn, &t_end)) {
+    hist.push_back(std::move(f0));
+    return {true, linear_traceback(hist, S, 0, t_end), 0, 0, t_end};
+  }
+  hist.push_back(std::move(f0));
+  for (int32_t s = 1; s <= explore; ++s) {
+    LinFrame nf{std::vector<int32_t>(size, UNSET), std::vector<uint8_t>(size, 0)};
+    linear_step(S, s, hist[static_cast<size_t>(s) - 1].wf, nf.wf, nf.op);
+    bool done = fitting_end_k(nf.wf, qn, tn, &t_end);
+    hist.push_back(std::move(nf));
+    if (done) return {true, linear_traceback(hist, S, s, t_end), s, s, t_end};
+  }
+  return {false, {}, 0, 0, 0};
+}
+
+// ---------------------------------------------------------------------------
+// Gap-affine WFA (wfa_simd.rs:542-935)
+// ---------------------------------------------------------------------------
+
+struct AffFrame {
+  std::vector<int32_t> m, ins, del;
+  std::vector<uint8_t> m_op, ins_op, del_op;
+  explicit AffFrame(size_t n)
+      : m(n, UNSET), ins(n, UNSET), del(n, UNSET), m_op(n, 0), ins_op(n, 0), del_op(n, 0) {}
+};
+
+struct Costs {
+  int32_t x, o, e;
+};
+
+// affine_pos_valid (wfa_simd.rs:563-569).
+inline bool pos_valid(int32_t pos, int32_t k, int32_t qn, int32_t tn) {
+  if (pos == UNSET || pos < 0 || pos > qn) return false;
+  int32_t tp = pos - k;
+  return tp >= 0 && tp <= tn;
+}
+
+// affine_traceback (wfa_simd.rs:785-886).
+std::string affine_traceback(const std::vector<AffFrame>& hist, const Seqs& S, int32_t s_end,
+                             int32_t t_end, int32_t k_end, uint8_t start_layer, Costs c) {
+  const int32_t tn = S.tn;
+  std::vector<char> ops;
+  int32_t qi = S.qn, ti = t_end, k = k_end, s = s_end;
+  uint8_t layer = start_layer;
+  auto emit_m = [&](int32_t n) {
+    for (int32_t i = 0; i < n; ++i) {
+      ops.push_back('M');
+      --qi;
+      --ti;
+    }
+  };
+  while (qi > 0 || ti > 0) {
+    if (s == 0) {
+      while (qi > 0 && ti > 0) {
+        ops.push_back('M');
+        --qi;
+        --ti;
+      }
+      break;
+    }
+    const size_t ki = static_cast<size_t>(k + tn);
+    const AffFrame& f = hist[static_cast<size_t>(s)];
+    if (layer == 0) {
+      uint8_t op = f.m_op[ki];
+      if (op == 0) {
+        int32_t s_mm = s - c.x;
+        int32_t raw = hist[static_cast<size_t>(s_mm)].m[ki] + 1;
+        emit_m(std::max(qi - raw, 0));
+        ops.push_back('X');
+        --qi;
+        --ti;
+        s = s_mm;
+      } else if (op == 1) {
+        emit_m(std::max(qi - f.ins[ki], 0));
+        layer = 1;
+      } else {
+        emit_m(std::max(qi - f.del[ki], 0));
+        layer = 2;
+      }
+    } else if (layer == 1) {
+      ops.push_back('D');  // query-extra
+      --qi;
+      if (f.ins_op[ki] == 0) {
+        s -= c.o + c.e;
+        layer = 0;
+      } else {
+        s -= c.e;
+      }
+      --k;
+    } else {
+      ops.push_back('I');  // target-extra
+      --ti;
+      if (f.del_op[ki] == 0) {
+        s -= c.o + c.e;
+        layer = 0;
+      } else {
+        s -= c.e;
+      }
+      ++k;
+    }
+  }
+  std::reverse(ops.begin(), ops.end());
+  return compact_cigar(ops);
+}
+
+// fill_wfa_affine_generic (wfa_simd.rs:581-779).  cap < 0 = None.
+EngineOut fill_wfa_affine_generic(const Seqs& S, Costs c, bool fitting, int64_t cap) {
+  const int32_t qn = S.qn, tn = S.tn;
+  if (qn == 0 && tn == 0) return {true, std::string(), 0, 0, 0};
+  if (qn == 0) {
+    if (fitting) return {true, std::string(), 0, 0, 0};
+    return {true, std::to_string(tn) + "I", c.o + static_cast<int64_t>(tn) * c.e, 0, tn};
+  }
+  const size_t size = static_cast<size_t>(qn + tn + 1);
+  const int32_t diag_lo = -tn, diag_hi = qn;
+
+  // Endpoint scan (wfa_simd.rs:618-650): layer order m, ins, del; fitting takes
+  // the largest t_end, i.e. the smallest diagonal.
+  auto endpoint = [&](const AffFrame& f, int32_t* t_end, int32_t* k_out, uint8_t* layer) -> bool {
+    auto check = [&](size_t ki, uint8_t* l) -> bool {
+      if (f.m[ki] == qn) { *l = 0; return true; }
+      if (f.ins[ki] == qn) { *l = 1; return true; }
+      if (f.del[ki] == qn) { *l = 2; return true; }
+      return false;
+    };
+    if (!fitting) {
+      int32_t k = qn - tn;
+      uint8_t l;
+      if (check(static_cast<size_t>(k + tn), &l)) {
+        *t_end = tn;
+        *k_out = k;
+        *layer = l;
+        return true;
+      }
+      return false;
+    }
+    bool found = false;
+    for (int32_t k = qn - tn; k <= qn; ++k) {
+      int32_t te = qn - k;
+      if (te < 0 || te > tn) continue;
+      uint8_t l;
+      if (check(static_cast<size_t>(k + tn), &l) && (!found || te > *t_end)) {
+        *t_end = te;
+        *k_out = k;
+        *layer = l;
+        found = true;
+      }
+    }
+    return found;
+  };
+
+  std::vector<AffFrame> hist;
+  hist.emplace_back(size);
+  hist[0].m[static_cast<size_t>(tn)] = S.extend(0, 0);
+  int32_t t_end = 0, k_end = 0;
+  uint8_t layer = 0;
+  if (endpoint(hist[0], &t_end, &k_end, &layer)) {
+    return {true, affine_traceback(hist, S, 0, t_end, k_end, layer, c), 0, 0, t_end};
+  }
+  const int64_t hard_max_s =
+      static_cast<int64_t>(qn + tn) * std::max(c.x, 1) + c.o + c.e + 4;
+  const int64_t explore_to = cap >= 0 ? std::min<int64_t>(cap, hard_max_s) : hard_max_s;
+
+  for (int64_t s64 = 1; s64 <= explore_to; ++s64) {
+    const int32_t s = static_cast<int32_t>(s64);
+    AffFrame f(size);
+    const int32_t s_open = s - c.o - c.e;
+    const int32_t s_ext = s - c.e;
+    for (int32_t k = diag_lo; k <= diag_hi; ++k) {
+      const size_t ki = static_cast<size_t>(k + tn);
+      // ins (query-extra): open from m[s_open][k-1]+1, extend from ins[s_ext][k-1]+1.
+      {
+        int32_t best = UNSET;
+        uint8_t bop = 0;
+        if (k - 1 >= diag_lo) {
+          const size_t pk = ki - 1;
+          if (s_open >= 0) {
+            int32_t p = hist[static_cast<size_t>(s_open)].m[pk];
+            if (p != UNSET) {
+              int32_t cand = p + 1;
+              if (pos_valid(cand, k, qn, tn) && cand > best) { best = cand; bop = 0; }
+            }
+          }
+          if (s_ext >= 0) {
+            int32_t p = hist[static_cast<size_t>(s_ext)].ins[pk];
+            if (p != UNSET) {
+              int32_t cand = p + 1;
+              if (pos_valid(cand, k, qn, tn) && cand > best) { best = cand; bop = 1; }
+            }
+          }
+        }
+        if (best != UNSET) { f.ins[ki] = best; f.ins_op[ki] = bop; }
+      }
+      // del (target-extra): open from m[s_open][k+1], extend from del[s_ext][k+1].
+      {
+        int32_t best = UNSET;
+        uint8_t bop = 0;
+        if (k + 1 <= diag_hi) {
+          const size_t pk = ki + 1;
+          if (s_open >= 0) {
+            int32_t p = hist[static_cast<size_t>(s_open)].m[pk];
+            if (p != UNSET && pos_valid(p, k, qn, tn) && p > best) { best = p; bop = 0; }
+          }
+          if (s_ext >= 0) {
+            int32_t p = hist[static_cast<size_t>(s_ext)].del[pk];
+            if (p != UNSET && pos_valid(p, k, qn, tn) && p > best) { best = p; bop = 1; }
+          }
+        }
+        if (best != UNSET) { f.del[ki] = best; f.del_op[ki] = bop; }
+      }
+    }
+    const int32_t s_mm = s - c.x;
+    for (int32_t k = diag_lo; k <= diag_hi; ++k) {
+      const size_t ki = static_cast<size_t>(k + tn);
+      int32_t best = UNSET;
+      uint8_t bop = 0;
+      if (s_mm >= 0) {
+        int32_t p = hist[static_cast<size_t>(s_mm)].m[ki];
+        if (p != UNSET) {
+          int32_t cand = p + 1;
+          if (pos_valid(cand, k, qn, tn) && cand > best) { best = cand; bop = 0; }
+        }
+      }
+      if (f.ins[ki] != UNSET && pos_valid(f.ins[ki], k, qn, tn) && f.ins[ki] > best) { best = f.ins[ki]; bop = 1; }
+      if (f.del[ki] != UNSET && pos_valid(f.del[ki], k, qn, tn) && f.del[ki] > best) { best = f.del[ki]; bop = 2; }
+      if (best != UNSET) {
+        f.m[ki] = S.extend(best, k);
+        f.m_op[ki] = bop;
+      }
+    }
+    hist.push_back(std::move(f));
+    if (endpoint(hist[static_cast<size_t>(s)], &t_end, &k_end, &layer)) {
+      return {true, affine_traceback(hist, S, s, t_end, k_end, layer, c), s, 0, t_end};
+    }
+  }
+  return {false, {}, 0, 0, 0};
+}
+
+// fill_wfa_affine_fitting_impl (wfa_simd.rs:919-935).
+EngineOut fill_wfa_affine_fitting_impl(const Seqs& S, Costs c, int64_t cap) {
+  if (S.qn == 0) return {true, std::string(), 0, 0, 0};
+  const bool fitting = S.tn > S.qn + S.qn / 2 + 10;
+  EngineOut r = fill_wfa_affine_generic(S, c, fitting, cap);
+  if (r.some) r.edit_distance = static_cast<int64_t>(levenshtein_from_cigar(r.cigar));
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Myers / Hyyrö bit-parallel edit distance (wfa_simd.rs:3507-3814)
+// ---------------------------------------------------------------------------
+
+struct MyersColumns {
+  std::vector<uint64_t> data;  // per column: vp[B] then vn[B]
+  std::vector<size_t> scores;
+  size_t nb = 0;
+  size_t ncols = 0;
+};
+
+// myers_forward_flat_with_cutoff (wfa_simd.rs:3552-3648).  margin < 0 = None.
+MyersColumns myers_forward(const uint8_t* q, size_t m, const uint8_t* t, size_t n, int64_t margin,
+                           bool keep_columns) {
+  constexpr size_t W = 64;
+  MyersColumns out;
+  const size_t nb = (m + W - 1) / W;
+  const size_t last = nb - 1;
+  const size_t last_bits = (m % W == 0) ? W : m % W;
+  const uint64_t last_mask = last_bits == W ? ~0ull : ((1ull << last_bits) - 1);
+  const size_t score_bit = (m - 1) % W;
+  std::vector<uint64_t> pm(nb * 256, 0);
+  for (size_t i = 0; i < m; ++i) pm[(i / W) * 256 + q[i]] |= 1ull << (i % W);
+  std::vector<uint64_t> vp(nb, ~0ull), vn(nb, 0), nvp(nb, 0), nvn(nb, 0);
+  vp[last] = last_mask;
+  size_t score = m, best_seen = m;
+  const size_t stride = nb * 2;
+  out.nb = nb;
+  if (keep_columns) out.data.assign(n * stride, 0);
+  out.scores.reserve(n);
+  for (size_t col = 0; col < n; ++col) {
+    const uint8_t tb = t[col];
+    uint64_t add_carry = 0, hp_carry = 1, hn_carry = 0, last_hp = 0, last_hn = 0;
+    for (size_t b = 0; b < nb; ++b) {
+      const uint64_t eq = pm[b * 256 + tb];
+      const uint64_t xh = eq | vn[b];
+      const uint64_t xhvp = xh & vp[b];
+      uint64_t s1 = vp[b] + xhvp;
+      uint64_t c1 = s1 < vp[b];
+      uint64_t s2 = s1 + add_carry;
+      uint64_t c2 = s2 < s1;
+      add_carry = c1 | c2;
+      uint64_t d0 = (s2 ^ vp[b]) | xh;
+      uint64_t hn = vp[b] & d0;
+      uint64_t hp = vn[b] | ~(vp[b] | d0);
+      if (b == last && last_bits < W) {
+        d0 &= last_mask;
+        hn &= last_mask;
+        hp &= last_mask;
+      }
+      if (b == last) {
+        last_hp = hp;
+        last_hn = hn;
+      }
+      const uint64_t nhp = hp >> (W - 1);
+      const uint64_t x = (hp << 1) | hp_carry;
+      hp_carry = nhp;
+      const uint64_t nhn = hn >> (W - 1);
+      const uint64_t hns = (hn << 1) | hn_carry;
+      hn_carry = nhn;
+      nvn[b] = x & d0;
+      uint64_t v2 = hns | ~(d0 | x);
+      nvp[b] = (b == last && last_bits < W) ? (v2 & last_mask) : v2;
+    }
+    if ((last_hp >> score_bit) & 1) score += 1;
+    if ((last_hn >> score_bit) & 1) score = score > 0 ? score - 1 : 0;
+    vp.swap(nvp);
+    vn.swap(nvn);
+    if (keep_columns) {
+      std::copy(vp.begin(), vp.end(), out.data.begin() + static_cast<ptrdiff_t>(col * stride));
+      std::copy(vn.begin(), vn.end(), out.data.begin() + static_cast<ptrdiff_t>(col * stride + nb));
+    }
+    out.scores.push_back(score);
+    if (score < best_seen) best_seen = score;
+    if (margin >= 0 && score > best_seen + static_cast<size_t>(margin)) {
+      out.ncols = col + 1;
+      if (keep_columns) out.data.resize(out.ncols * stride);
+      return out;
+    }
+  }
+  out.ncols = n;
+  return out;
+}
+
+// fill_dp_column_flat + reconstruct_into (wfa_simd.rs:3507-3516, :3687-3699).
+void dp_column(const MyersColumns& cols, size_t m, size_t j, std::vector<size_t>& buf) {
+  if (j == 0) {
+    for (size_t i = 0; i <= m; ++i) buf[i] = i;
+    return;
+  }
+  const uint64_t* vp = &cols.data[(j - 1) * cols.nb * 2];
+  const uint64_t* vn = vp + cols.nb;
+  buf[m] = cols.scores[j - 1];
+  for (size_t i = m; i-- > 0;) {
+    const int64_t pb = static_cast<int64_t>((vp[i / 64] >> (i % 64)) & 1);
+    const int64_t nb = static_cast<int64_t>((vn[i / 64] >> (i % 64)) & 1);
+    buf[i] = static_cast<size_t>(static_cast<int64_t>(buf[i + 1]) - pb + nb);
+  }
+}
+
+// myers_backtrace_flat (wfa_simd.rs:3702-3781): priority D, then I, then diagonal.
+std::string myers_backtrace(const uint8_t* q, size_t m, const uint8_t* t, const MyersColumns& cols,
+                            size_t end_ti) {
+  std::vector<char> ops;
+  ops.reserve(m + end_ti);
+  size_t qi = m, ti = end_ti;
+  std::vector<size_t> cur(m + 1, 0), prev(m + 1, 0);
+  if (ti > 0) {
+    dp_column(cols, m, ti, cur);
+    dp_column(cols, m, ti - 1, prev);
+  }
+  while (qi > 0 || ti > 0) {
+    if (qi == 0) {
+      ops.push_back('I');
+      --ti;
+      continue;
+    }
+    if (ti == 0) {
+      ops.push_back('D');
+      --qi;
+      continue;
+    }
+    const size_t c = cur[qi];
+    const size_t diag_cost = q[qi - 1] == t[ti - 1] ? 0 : 1;
+    if (c == cur[qi - 1] + 1) {
+      ops.push_back('D');
+      --qi;
+    } else if (c == prev[qi] + 1) {
+      ops.push_back('I');
+      --ti;
+      cur.swap(prev);
+      if (ti > 0) dp_column(cols, m, ti - 1, prev);
+    } else {
+      ops.push_back(diag_cost == 0 ? 'M' : 'X');
+      --qi;
+      --ti;
+      cur.swap(prev);
+      if (ti > 0) dp_column(cols, m, ti - 1, prev);
+    }
+  }
+  std::reverse(ops.begin(), ops.end());
+  return compact_cigar(ops);
+}
+
+// myers_edit_distance_impl (wfa_simd.rs:3657-3673).
+EngineOut myers_global(const uint8_t* q, size_t m, const uint8_t* t, size_t n, bool with_cigar) {
+  if (m == 0) return {true, n > 0 ? std::to_string(n) + "I" : std::string(), (int64_t)n, (int64_t)n, (int64_t)n};
+  if (n == 0) return {true, std::to_string(m) + "D", (int64_t)m, (int64_t)m, 0};
+  MyersColumns cols = myers_forward(q, m, t, n, -1, with_cigar);
+  const size_t ed = cols.ncols > 0 ? cols.scores[cols.ncols - 1] : m;
+  std::string cigar = with_cigar ? myers_backtrace(q, m, t, cols, n) : std::string();
+  return {true, cigar, (int64_t)ed, (int64_t)ed, (int64_t)n};
+}
+
+// myers_fitting_impl (wfa_simd.rs:3783-3814).
+EngineOut myers_fitting(const uint8_t* q, size_t m, const uint8_t* t, size_t n, bool with_cigar) {
+  if (m == 0) return {true, std::string(), 0, 0, 0};
+  if (n == 0) return {true, std::to_string(m) + "D", (int64_t)m, (int64_t)m, 0};
+  if (n <= m + m / 2 + 10) return myers_global(q, m, t, n, with_cigar);
+  const int64_t margin = static_cast<int64_t>(m / 10 + 1);
+  MyersColumns cols = myers_forward(q, m, t, n, margin, with_cigar);
+  size_t best_consumed = 0, best_score = m;
+  for (size_t c = 0; c < cols.ncols; ++c) {
+    if (cols.scores[c] < best_score) {
+      best_score = cols.scores[c];
+      best_consumed = c + 1;
+    }
+  }
+  std::string cigar = with_cigar ? myers_backtrace(q, m, t, cols, best_consumed) : std::string();
+  return {true, cigar, (int64_t)best_score, (int64_t)best_score, (int64_t)best_consumed};
+}
+
+// ---------------------------------------------------------------------------
+// Wrappers (phraya-align/src/lib.rs:63-167, wfa_simd.rs:173-223)
+// ---------------------------------------------------------------------------
+
+int finish(const EngineOut& e, size_t seed_q, size_t seed_t, size_t q_suffix_len, orc_result* r,
+           char* cigar, size_t cap, std::string* keep) {
+  if (!e.some) {
+    r->status = ORC_ALIGNMENT_FAILED;
+    r->cigar_len = 0;
+    if (cigar && cap) cigar[0] = 0;
+    return r->status;
+  }
+  r->status = ORC_OK;
+  r->score = e.score;
+  r->edit_distance = e.edit_distance;
+  r->query_start = static_cast<int64_t>(seed_q);
+  r->query_end = static_cast<int64_t>(seed_q + q_suffix_len);
+  r->target_start = static_cast<int64_t>(seed_t);
+  r->target_end = static_cast<int64_t>(seed_t) + e.t_consumed;
+  r->cigar_len = static_cast<int64_t>(e.cigar.size());
+  if (cigar && cap) {
+    size_t nc = std::min(cap - 1, e.cigar.size());
+    std::memcpy(cigar, e.cigar.data(), nc);
+    cigar[nc] = 0;
+  }
+  if (keep) *keep = e.cigar;
+  return ORC_OK;
+}
+
+int invalid(orc_result* r, char* cigar, size_t cap) {
+  std::memset(r, 0, sizeof(*r));
+  r->status = ORC_INVALID_INPUT;
+  if (cigar && cap) cigar[0] = 0;
+  return ORC_INVALID_INPUT;
+}
+
+int run_engine(int engine, const uint8_t* q, size_t qn, const uint8_t* t, size_t tn, size_t sq,
+               size_t st, Costs c, int64_t cap, orc_result* r, char* cigar, size_t ccap,
+               std::string* keep) {
+  std::memset(r, 0, sizeof(*r));
+  if (engine == ORC_ENGINE_MYERS_GLOBAL) {
+    EngineOut e = myers_global(q, qn, t, tn, true);
+    return finish(e, 0, 0, qn, r, cigar, ccap, keep);
+  }
+  if (sq > qn || st > tn) return invalid(r, cigar, ccap);
+  const uint8_t* qs = q + sq;
+  const uint8_t* ts = t + st;
+  const size_t qsn = qn - sq, tsn = tn - st;
+  if (qsn > static_cast<size_t>(INT32_MAX / 4) || tsn > static_cast<size_t>(INT32_MAX / 4))
+    return invalid(r, cigar, ccap);
+  Seqs S{qs, static_cast<int32_t>(qsn), ts, static_cast<int32_t>(tsn)};
+  switch (engine) {
+    case ORC_ENGINE_LINEAR: {
+      if (qsn == 0 && tsn == 0) return finish({true, {}, 0, 0, 0}, sq, st, 0, r, cigar, ccap, keep);
+      return finish(fill_wfa_fitting_impl(S, cap), sq, st, qsn, r, cigar, ccap, keep);
+    }
+    case ORC_ENGINE_AFFINE: {
+      if (c.x < 1 || c.e < 1 || c.o < 0) {
+        r->status = ORC_BAD_COSTS;
+        return ORC_BAD_COSTS;
+      }
+      return finish(fill_wfa_affine_fitting_impl(S, c, cap), sq, st, qsn, r, cigar, ccap, keep);
+    }
+    case ORC_ENGINE_MYERS:
+      return finish(myers_fitting(qs, qsn, ts, tsn, true), sq, st, qsn, r, cigar, ccap, keep);
+    case ORC_ENGINE_MYERS_FWD:
+      return finish(myers_fitting(qs, qsn, ts, tsn, false), sq, st, qsn, r, cigar, ccap, keep);
+    default:
+      return invalid(r, cigar, ccap);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t orc_lcp(const uint8_t* a, size_t an, const uint8_t* b, size_t bn) { return lcp_sse2(a, an, b, bn); }
+size_t orc_lcp_scalar(const uint8_t* a, size_t an, const uint8_t* b, size_t bn) { return lcp_scalar(a, an, b, bn); }
+size_t orc_lcp_u64(const uint8_t* a, size_t an, const uint8_t* b, size_t bn) { return lcp_u64(a, an, b, bn); }
+
+int orc_wfa_extend(const uint8_t* q, size_t qn, const uint8_t* t, size_t tn, size_t seed_q,
+                   size_t seed_t, int64_t cap, orc_result* r, char* cigar, size_t cigar_cap) {
+  return run_engine(ORC_ENGINE_LINEAR, q, qn, t, tn, seed_q, seed_t, {0, 0, 0}, cap, r, cigar,
+                    cigar_cap, nullptr);
+}
+
+int orc_wfa_extend_affine(const uint8_t* q, size_t qn, const uint8_t* t, size_t tn, size_t seed_q,
+                          size_t seed_t, int32_t mismatch, int32_t gap_open, int32_t gap_extend,
+                          int64_t cap, orc_result* r, char* cigar, size_t cigar_cap) {
+  return run_engine(ORC_ENGINE_AFFINE, q, qn, t, tn, seed_q, seed_t,
+                    {mismatch, gap_open, gap_extend}, cap, r, cigar, cigar_cap, nullptr);
+}
+
+int orc_myers_extend(const uint8_t* q, size_t qn, const uint8_t* t, size_t tn, size_t seed_q,
+                     size_t seed_t, orc_result* r, char* cigar, size_t cigar_cap) {
+  return run_engine(ORC_ENGINE_MYERS, q, qn, t, tn, seed_q, seed_t, {0, 0, 0}, -1, r, cigar,
+                    cigar_cap, nullptr);
+}
+
+int orc_myers_edit_distance(const uint8_t* q, size_t qn, const uint8_t* t, size_t tn,
+                            orc_result* r, char* cigar, size_t cigar_cap) {
+  return run_engine(ORC_ENGINE_MYERS_GLOBAL, q, qn, t, tn, 0, 0, {0, 0, 0}, -1, r, cigar,
+                    cigar_cap, nullptr);
+}
+
+int orc_myers_extend_distance(const uint8_t* q, size_t qn, const uint8_t* t, size_t tn,
+                              size_t seed_q, size_t seed_t, orc_result* r) {
+  return run_engine(ORC_ENGINE_MYERS_FWD, q, qn, t, tn, seed_q, seed_t, {0, 0, 0}, -1, r,
+                    nullptr, 0, nullptr);
+}
+
+struct orc_batch {
+  std::vector<std::string> cigars;
+};
+
+orc_batch* orc_batch_run(int engine, uint32_t n, const uint8_t* q_seq, const uint64_t* q_off,
+                         const uint8_t* t_seq, const uint64_t* t_off, const uint64_t* seed_q,
+                         const uint64_t* seed_t, int32_t mismatch, int32_t gap_open,
+                         int32_t gap_extend, int64_t cap, int threads, orc_result* res) {
+  orc_batch* b = new orc_batch;
+  b->cigars.resize(n);
+  std::atomic<uint32_t> next{0};
+  auto worker = [&]() {
+    for (;;) {
+      uint32_t i = next.fetch_add(1, std::memory_order_relaxed);
+      if (i >= n) break;
+      const uint8_t* q = q_seq + q_off[i];
+      const uint8_t* t = t_seq + t_off[i];
+      size_t qn = static_cast<size_t>(q_off[i + 1] - q_off[i]);
+      size_t tn = static_cast<size_t>(t_off[i + 1] - t_off[i]);
+      run_engine(engine, q, qn, t, tn, seed_q ? seed_q[i] : 0, seed_t ? seed_t[i] : 0,
+                 {mismatch, gap_open, gap_extend}, cap, &res[i], nullptr, 0, &b->cigars[i]);
+    }
+  };
+  if (threads <= 1) {
+    worker();
+  } else {
+    std::vector<std::thread> pool;
+    pool.reserve(static_cast<size_t>(threads));
+    for (int i = 0; i < threads; ++i) pool.emplace_back(worker);
+    for (auto& th : pool) th.join();
+  }
+  return b;
+}
+
+uint64_t orc_batch_cigar_bytes(const orc_batch* b) {
+  uint64_t total = 0;
+  for (const auto& s : b->cigars) total += s.size();
+  return total;
+}
+
+void orc_batch_cigar_copy(const orc_batch* b, char* arena, uint64_t* off) {
+  uint64_t pos = 0;
+  for (size_t i = 0; i < b->cigars.size(); ++i) {
+    off[i] = pos;
+    std::memcpy(arena + pos, b->cigars[i].data(), b->cigars[i].size());
+    pos += b->cigars[i].size();
+  }
+  off[b->cigars.size()] = pos;
+}
+
+void orc_batch_free(orc_batch* b) { delete b; }
+
+int orc_hardware_threads(void) {
+  unsigned h = std::thread::hardware_concurrency();
+  return h ? static_cast<int>(h) : 1;
+}
+
+}  // extern "C"

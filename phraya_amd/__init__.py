@@ -1,0 +1,310 @@
+"""phraya_amd -- MI355X (gfx950) extension engines of phraya-align.
+
+Host mirror of phraya-align's extension API (phraya-align/src/lib.rs:23-167)
+over the C ABI of libphraya_hip.so:
+
+    wfa_extend(query, target, seed)               lib.rs:74-76
+    wfa_extend_naive(query, target, seed)         lib.rs:63-65
+    wfa_extend_affine(query, target, seed, costs) lib.rs:139-167
+    myers_extend(query, target, seed)             lib.rs:106-127
+    myers_edit_distance(query, target)            lib.rs:94-96
+
+Each returns what the upstream returns (an Alignment, or raises WfaError's
+InvalidInput / AlignmentFailed where the upstream returns Err), computed on
+the GPU.  align_batch() is the batched form used by a batched executor; Plan
+keeps a batch resident in HBM for repeated launches.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+from dataclasses import dataclass
+from typing import Iterable, Optional, Sequence
+
+import numpy as np
+
+from . import _lib as L
+
+__all__ = [
+    "SeedAnchor", "Alignment", "WfaError", "InvalidInput", "AlignmentFailed", "AffineCosts",
+    "Context", "wfa_extend", "wfa_extend_naive", "wfa_extend_affine", "myers_extend",
+    "myers_edit_distance", "align_batch", "BatchResult", "Plan", "pack_batch", "format_cigar",
+    "ENGINE_LINEAR", "ENGINE_AFFINE", "ENGINE_MYERS", "ENGINE_MYERS_GLOBAL", "ENGINE_MYERS_DIST",
+]
+
+ENGINE_LINEAR = L.ENGINE_LINEAR
+ENGINE_AFFINE = L.ENGINE_AFFINE
+ENGINE_MYERS = L.ENGINE_MYERS
+ENGINE_MYERS_GLOBAL = L.ENGINE_MYERS_GLOBAL
+ENGINE_MYERS_DIST = L.ENGINE_MYERS_DIST
+
+_OPS = "MXID"
+
+
+@dataclass(frozen=True)
+class SeedAnchor:
+    """SeedAnchor (phraya-align/src/lib.rs:24-28)."""
+    query_pos: int = 0
+    target_pos: int = 0
+
+
+@dataclass
+class Alignment:
+    """Alignment (phraya-align/src/lib.rs:31-39); `score` is the engine's search score."""
+    cigar: str
+    edit_distance: int
+    query_start: int
+    query_end: int
+    target_start: int
+    target_end: int
+    score: int = 0
+
+
+class WfaError(Exception):
+    """WfaError (phraya-align/src/lib.rs:45-49)."""
+
+
+class InvalidInput(WfaError):
+    pass
+
+
+class AlignmentFailed(WfaError):
+    pass
+
+
+@dataclass(frozen=True)
+class AffineCosts:
+    """AffineCosts (phraya-align/src/wfa_simd.rs:520-540); defaults are upstream's 2/4/1."""
+    mismatch: int = 2
+    gap_open: int = 4
+    gap_extend: int = 1
+
+
+def format_cigar(runs: np.ndarray) -> str:
+    """Runs (length << 4 | op) -> phraya CIGAR text."""
+    return "".join(f"{int(r) >> 4}{_OPS[int(r) & 3]}" for r in runs)
+
+
+class Context:
+    """A device context (phx_ctx): stream, buffers, submission lock."""
+
+    def __init__(self, device: int = 0, tuning: Optional[dict] = None):
+        lib = L.lib()
+        tun = None
+        if tuning:
+            t = L.CTuning(**tuning)
+            tun = ctypes.byref(t)
+        self._h = lib.phx_ctx_create(device, 0, tun)
+        if not self._h:
+            raise RuntimeError(f"phx_ctx_create({device}) failed: no usable GPU")
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def last_error(self) -> str:
+        return L.lib().phx_ctx_last_error(self._h).decode()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().phx_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx: Optional[Context] = None
+_ctx_lock = threading.Lock()
+
+
+def default_context() -> Context:
+    global _default_ctx
+    with _ctx_lock:
+        if _default_ctx is None:
+            _default_ctx = Context(0)
+        return _default_ctx
+
+
+# --------------------------------------------------------------------------- batches
+
+@dataclass
+class PackedBatch:
+    """Concatenated host arrays in the phx_batch layout."""
+    q_seq: np.ndarray
+    q_off: np.ndarray
+    t_seq: np.ndarray
+    t_off: np.ndarray
+    seed_q: np.ndarray
+    seed_t: np.ndarray
+
+    @property
+    def n(self) -> int:
+        return len(self.q_off) - 1
+
+
+def _concat(seqs: Sequence[bytes]) -> tuple[np.ndarray, np.ndarray]:
+    off = np.zeros(len(seqs) + 1, dtype=np.uint64)
+    off[1:] = np.cumsum([len(s) for s in seqs], dtype=np.uint64)
+    buf = np.frombuffer(b"".join(seqs), dtype=np.uint8) if off[-1] else np.zeros(1, np.uint8)
+    return np.ascontiguousarray(buf), off
+
+
+def pack_batch(queries: Sequence[bytes], targets: Sequence[bytes],
+               seeds: Optional[Iterable] = None) -> PackedBatch:
+    if len(queries) != len(targets):
+        raise ValueError("queries and targets differ in length")
+    q_seq, q_off = _concat(queries)
+    t_seq, t_off = _concat(targets)
+    n = len(queries)
+    sq = np.zeros(n, dtype=np.uint64)
+    st = np.zeros(n, dtype=np.uint64)
+    if seeds is not None:
+        for i, s in enumerate(seeds):
+            if isinstance(s, SeedAnchor):
+                sq[i], st[i] = s.query_pos, s.target_pos
+            else:
+                sq[i], st[i] = s
+    return PackedBatch(q_seq, q_off, t_seq, t_off, sq, st)
+
+
+def _cbatch(pb: PackedBatch, engine: int, costs: AffineCosts, cap: Optional[int]) -> L.CBatch:
+    return L.CBatch(
+        pb.n, engine, L.Costs(costs.mismatch, costs.gap_open, costs.gap_extend),
+        -1 if cap is None else int(cap),
+        pb.q_seq.ctypes.data, pb.q_off.ctypes.data, pb.t_seq.ctypes.data, pb.t_off.ctypes.data,
+        pb.seed_q.ctypes.data, pb.seed_t.ctypes.data)
+
+
+@dataclass
+class BatchResult:
+    """Per-pair results: `aln` is a structured array (status, score, edit_distance,
+    query/target start/end); pair i's CIGAR runs are runs[run_off[i]:run_off[i+1]]."""
+    aln: np.ndarray
+    runs: np.ndarray
+    run_off: np.ndarray
+
+    def cigar(self, i: int) -> str:
+        return format_cigar(self.runs[int(self.run_off[i]):int(self.run_off[i + 1])])
+
+    def cigars(self) -> list[str]:
+        return [self.cigar(i) for i in range(len(self.aln))]
+
+    def alignment(self, i: int) -> Alignment:
+        a = self.aln[i]
+        st = int(a["status"])
+        if st == L.PHX_INVALID_INPUT:
+            raise InvalidInput("Seed position beyond sequence length")
+        if st == L.PHX_ALIGNMENT_FAILED:
+            raise AlignmentFailed("alignment abandoned: max edit distance exceeded")
+        if st != L.PHX_OK:
+            raise RuntimeError(f"pair {i}: device status {st}")
+        return Alignment(self.cigar(i), int(a["edit_distance"]), int(a["query_start"]),
+                         int(a["query_end"]), int(a["target_start"]), int(a["target_end"]),
+                         int(a["score"]))
+
+
+class Plan:
+    """A batch resident in HBM (phx_plan): run() launches the kernels only."""
+
+    def __init__(self, pb: PackedBatch, engine: int, costs: AffineCosts = AffineCosts(),
+                 cap: Optional[int] = None, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self.pb = pb
+        self.engine = engine
+        self._cb = _cbatch(pb, engine, costs, cap)
+        self._h = L.lib().phx_plan_create(self.ctx.handle, ctypes.byref(self._cb))
+        if not self._h:
+            raise RuntimeError(f"phx_plan_create failed: {self.ctx.last_error()}")
+
+    def run(self) -> None:
+        rc = L.lib().phx_plan_run(self._h)
+        if rc != L.PHX_OK:
+            raise RuntimeError(f"phx_plan_run failed ({rc}): {self.ctx.last_error()}")
+
+    def last_ms(self) -> float:
+        return L.lib().phx_plan_last_ms(self._h)
+
+    def last_kernel_ms(self) -> float:
+        return L.lib().phx_plan_last_kernel_ms(self._h)
+
+    def last_launches(self) -> int:
+        return int(L.lib().phx_plan_last_launches(self._h))
+
+    def algorithmic_bytes(self) -> int:
+        return int(L.lib().phx_plan_algorithmic_bytes(self._h))
+
+    def results(self) -> BatchResult:
+        n = self.pb.n
+        aln = np.zeros(n, dtype=L.ALN_DTYPE)
+        cap = int(L.lib().phx_cigar_bound(ctypes.byref(self._cb)))
+        runs = np.zeros(max(cap, 1), dtype=np.uint32)
+        off = np.zeros(n + 1, dtype=np.uint64)
+        res = L.CResults(aln.ctypes.data, runs.ctypes.data, cap, off.ctypes.data)
+        rc = L.lib().phx_plan_results(self._h, ctypes.byref(res))
+        if rc != L.PHX_OK:
+            raise RuntimeError(f"phx_plan_results failed ({rc}): {self.ctx.last_error()}")
+        return BatchResult(aln, runs[: int(off[-1])], off)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().phx_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def align_batch(engine: int, queries: Sequence[bytes], targets: Sequence[bytes],
+                seeds: Optional[Iterable] = None, costs: AffineCosts = AffineCosts(),
+                cap: Optional[int] = None, ctx: Optional[Context] = None) -> BatchResult:
+    """Align many pairs in one submission (phx_align_batch)."""
+    pb = pack_batch(queries, targets, seeds)
+    plan = Plan(pb, engine, costs, cap, ctx)
+    try:
+        plan.run()
+        return plan.results()
+    finally:
+        plan.close()
+
+
+# ------------------------------------------------------------------ per-pair API
+
+def _one(engine: int, query: bytes, target: bytes, seed: SeedAnchor,
+         costs: AffineCosts = AffineCosts(), cap: Optional[int] = None) -> Alignment:
+    res = align_batch(engine, [bytes(query)], [bytes(target)], [seed], costs, cap)
+    return res.alignment(0)
+
+
+def wfa_extend(query: bytes, target: bytes, seed: SeedAnchor = SeedAnchor(),
+               cap: Optional[int] = None) -> Alignment:
+    """wfa_extend (lib.rs:74-76): linear WFA, fitting iff tn > qn + qn/2 + 10."""
+    return _one(ENGINE_LINEAR, query, target, seed, cap=cap)
+
+
+wfa_extend_naive = wfa_extend
+
+
+def wfa_extend_affine(query: bytes, target: bytes, seed: SeedAnchor = SeedAnchor(),
+                      costs: AffineCosts = AffineCosts(), cap: Optional[int] = None) -> Alignment:
+    """wfa_extend_affine (lib.rs:139-167)."""
+    return _one(ENGINE_AFFINE, query, target, seed, costs, cap)
+
+
+def myers_extend(query: bytes, target: bytes, seed: SeedAnchor = SeedAnchor()) -> Alignment:
+    """myers_extend (lib.rs:106-127)."""
+    return _one(ENGINE_MYERS, query, target, seed)
+
+
+def myers_edit_distance(query: bytes, target: bytes) -> tuple[int, str]:
+    """myers_edit_distance (lib.rs:94-96): global distance and CIGAR."""
+    res = align_batch(ENGINE_MYERS_GLOBAL, [bytes(query)], [bytes(target)])
+    a = res.alignment(0)
+    return a.edit_distance, a.cigar

@@ -1,0 +1,124 @@
+"""ctypes binding of libphraya_hip.so (include/phraya_hip.h).
+
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libphraya_hip.so")
+
+PHX_OK = 0
+PHX_INVALID_INPUT = 1
+PHX_ALIGNMENT_FAILED = 2
+PHX_E_ARG = 32
+PHX_E_DEVICE = 33
+PHX_E_NOMEM = 34
+PHX_E_CAPACITY = 35
+PHX_E_INTERNAL = 36
+
+ENGINE_LINEAR = 0
+ENGINE_AFFINE = 1
+ENGINE_MYERS = 2
+ENGINE_MYERS_GLOBAL = 3
+ENGINE_MYERS_DIST = 4
+
+# Every symbol include/phraya_hip.h declares.
+EXPORTS = (
+    "phx_ctx_create", "phx_ctx_destroy", "phx_ctx_last_error", "phx_cigar_bound",
+    "phx_align_batch", "phx_plan_create", "phx_plan_run", "phx_plan_results",
+    "phx_plan_last_ms", "phx_plan_last_kernel_ms", "phx_plan_last_launches",
+    "phx_plan_algorithmic_bytes", "phx_plan_destroy", "phx_format_cigar",
+    "phx_wfa_extend", "phx_wfa_extend_affine", "phx_myers_extend", "phx_myers_edit_distance",
+    "phx_version",
+)
+
+
+class Costs(ctypes.Structure):
+    _fields_ = [("mismatch", ctypes.c_int32), ("gap_open", ctypes.c_int32),
+                ("gap_extend", ctypes.c_int32)]
+
+
+class CAlignment(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("score", ctypes.c_int64), ("edit_distance", ctypes.c_int64),
+                ("query_start", ctypes.c_int64), ("query_end", ctypes.c_int64),
+                ("target_start", ctypes.c_int64), ("target_end", ctypes.c_int64)]
+
+
+ALN_DTYPE = np.dtype([("status", np.int32), ("reserved", np.int32), ("score", np.int64),
+                      ("edit_distance", np.int64), ("query_start", np.int64),
+                      ("query_end", np.int64), ("target_start", np.int64),
+                      ("target_end", np.int64)])
+assert ALN_DTYPE.itemsize == ctypes.sizeof(CAlignment)
+
+
+class CBatch(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint32), ("engine", ctypes.c_int32), ("costs", Costs),
+                ("cap", ctypes.c_int64),
+                ("q_seq", ctypes.c_void_p), ("q_off", ctypes.c_void_p),
+                ("t_seq", ctypes.c_void_p), ("t_off", ctypes.c_void_p),
+                ("seed_q", ctypes.c_void_p), ("seed_t", ctypes.c_void_p)]
+
+
+class CResults(ctypes.Structure):
+    _fields_ = [("aln", ctypes.c_void_p), ("runs", ctypes.c_void_p),
+                ("runs_cap", ctypes.c_uint64), ("run_off", ctypes.c_void_p)]
+
+
+class CTuning(ctypes.Structure):
+    _fields_ = [("wave_wcap", ctypes.c_uint32), ("wave_hist_kib", ctypes.c_uint32),
+                ("wave_slots", ctypes.c_uint32), ("block_wcap", ctypes.c_uint32),
+                ("block_hist_kib", ctypes.c_uint32), ("block_slots", ctypes.c_uint32),
+                ("hbm_budget_mib", ctypes.c_uint64)]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libphraya_hip.so; raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m phraya_amd.build` "
+            "(the HIP path has no fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    L.phx_ctx_create.argtypes = [ctypes.c_int, ctypes.c_uint32, P]
+    L.phx_ctx_create.restype = P
+    L.phx_ctx_destroy.argtypes = [P]
+    L.phx_ctx_last_error.argtypes = [P]
+    L.phx_ctx_last_error.restype = ctypes.c_char_p
+    L.phx_cigar_bound.argtypes = [P]
+    L.phx_cigar_bound.restype = ctypes.c_uint64
+    L.phx_align_batch.argtypes = [P, P, P]
+    L.phx_plan_create.argtypes = [P, P]
+    L.phx_plan_create.restype = P
+    L.phx_plan_run.argtypes = [P]
+    L.phx_plan_results.argtypes = [P, P]
+    L.phx_plan_last_ms.argtypes = [P]
+    L.phx_plan_last_ms.restype = ctypes.c_double
+    L.phx_plan_last_kernel_ms.argtypes = [P]
+    L.phx_plan_last_kernel_ms.restype = ctypes.c_double
+    L.phx_plan_last_launches.argtypes = [P]
+    L.phx_plan_last_launches.restype = ctypes.c_uint32
+    L.phx_plan_algorithmic_bytes.argtypes = [P]
+    L.phx_plan_algorithmic_bytes.restype = ctypes.c_uint64
+    L.phx_plan_destroy.argtypes = [P]
+    L.phx_format_cigar.argtypes = [P, ctypes.c_uint64, P, ctypes.c_size_t]
+    L.phx_format_cigar.restype = ctypes.c_size_t
+    sz, u64 = ctypes.c_size_t, ctypes.c_uint64
+    L.phx_wfa_extend.argtypes = [P, P, sz, P, sz, u64, u64, P, P, sz, P]
+    L.phx_wfa_extend_affine.argtypes = [P, P, sz, P, sz, u64, u64, Costs, P, P, sz, P]
+    L.phx_myers_extend.argtypes = [P, P, sz, P, sz, u64, u64, P, P, sz, P]
+    L.phx_myers_edit_distance.argtypes = [P, P, sz, P, sz, P, P, sz, P]
+    L.phx_version.restype = ctypes.c_char_p
+    _lib = L
+    return L

@@ -1,0 +1,745 @@
+// phx_api.hip -- host runtime behind include/phraya_hip.h.
+//
+// Packs sequences (2-bit when a pair is pure uppercase ACGT, raw bytes
+// otherwise), uploads them into one HBM word arena, routes pairs to kernel
+// tiers, re-routes the few pairs a tier cannot hold (wider wavefront or longer
+// history than its slots) to the next tier, and returns results in the
+// upstream's units.  Seed handling and the InvalidInput rule follow
+// phraya-align/src/lib.rs:107-111 and :145-149.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "phraya_hip.h"
+#include "phx_device.h"
+
+namespace phx {
+hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t st);
+hipError_t launch_wfa(int mode, int variant, const WfaArgs& a, uint32_t grid, size_t lds, hipStream_t st);
+hipError_t launch_myers(int nb_template, const MyersArgs& a, hipStream_t st);
+}  // namespace phx
+
+using namespace phx;
+
+namespace {
+
+constexpr int kRingSlotsMax = 128;  // WfaShared metadata capacity
+constexpr size_t kRingLdsMax = 56 * 1024;  // dynamic LDS ring per workgroup (plus ~3 KB static)
+constexpr int32_t kMaxLen = (1 << 27) - 2;  // history cells hold offset + 1 in 28 bits
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+uint8_t acgt_code[256];
+bool acgt_init = [] {
+  std::memset(acgt_code, 0xFF, sizeof(acgt_code));
+  acgt_code['A'] = 0;
+  acgt_code['C'] = 1;
+  acgt_code['G'] = 2;
+  acgt_code['T'] = 3;
+  return true;
+}();
+
+bool is_acgt(const uint8_t* s, size_t n) {
+  for (size_t i = 0; i < n; ++i)
+    if (acgt_code[s[i]] == 0xFF) return false;
+  return true;
+}
+
+uint64_t words_for(size_t n, bool b8) { return (b8 ? (n + 7) / 8 : (n + 31) / 32) + 1; }
+
+void pack_into(uint64_t* w, const uint8_t* s, size_t n, bool b8) {
+  const uint64_t nw = words_for(n, b8);
+  std::memset(w, 0, nw * 8);
+  if (b8) {
+    std::memcpy(w, s, n);
+    return;
+  }
+  for (size_t i = 0; i < n; ++i) w[i >> 5] |= (uint64_t)acgt_code[s[i]] << ((i & 31) * 2);
+}
+
+uint32_t next_pow2(uint64_t v) {
+  uint32_t p = 1;
+  while (p < v && p < (1u << 30)) p <<= 1;
+  return p;
+}
+
+uint32_t log2u(uint32_t v) {
+  uint32_t l = 0;
+  while ((1u << l) < v) ++l;
+  return l;
+}
+
+int32_t gcd3(int32_t a, int32_t b, int32_t c) {
+  auto g2 = [](int32_t x, int32_t y) {
+    while (y) {
+      int32_t r = x % y;
+      x = y;
+      y = r;
+    }
+    return x;
+  };
+  return g2(g2(a, b), c);
+}
+
+}  // namespace
+
+struct phx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<hipEvent_t> kev;  // pairs of events around each kernel launch of a run
+  std::mutex mu;
+  phx_tuning tun{};
+  std::string err;
+  int cu_count = 256;
+};
+
+struct Tier {
+  int variant;
+  uint32_t wcap;
+  uint64_t hist_stride;
+  uint32_t grid;
+  size_t lds;
+};
+
+struct phx_plan {
+  phx_ctx* ctx = nullptr;
+  int engine = 0;
+  phx_costs costs{};
+  int64_t cap = -1;
+  uint32_t n = 0;
+  std::vector<uint64_t> seed_q, seed_t, qlen, tlen;  // suffix lengths
+  std::vector<int32_t> host_status;                   // final status of pairs never sent to the GPU
+  std::vector<PairDesc> pairs;
+  std::vector<uint32_t> tier_of;                      // WFA: initial tier per pair
+  uint64_t seq_words = 0, runs_total = 0;
+  uint64_t input_bytes = 0;                            // packed algorithmic input bytes
+  int32_t ring_slots = 2;
+  DevBuf d_seq, d_pairs, d_out, d_runs, d_work, d_ctr, d_hist, d_ring, d_scratch;
+  std::vector<std::vector<uint32_t>> work;             // initial work per tier / Myers group
+  std::vector<PairOut> out_h;
+  bool out_valid = false;
+  double last_ms = 0;
+  double last_kernel_ms = 0;
+  uint32_t last_launches = 0;
+};
+
+namespace {
+
+int fail(phx_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(ctx, e_ == hipErrorOutOfMemory ? PHX_E_NOMEM : PHX_E_DEVICE,          \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                   \
+  } while (0)
+
+int dev_alloc(phx_ctx* ctx, DevBuf& b, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  if (b.bytes >= bytes) return PHX_OK;
+  if (b.p) {
+    HIPCHK(hipFree(b.p));
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  HIPCHK(hipMalloc(&b.p, bytes));
+  b.bytes = bytes;
+  return PHX_OK;
+}
+
+void dev_free(DevBuf& b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+// Event pair number i of the context (created on first use).
+int kernel_events(phx_ctx* ctx, uint32_t i, hipEvent_t* a, hipEvent_t* b) {
+  while (ctx->kev.size() < 2 * (size_t)(i + 1)) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    ctx->kev.push_back(e);
+  }
+  *a = ctx->kev[2 * i];
+  *b = ctx->kev[2 * i + 1];
+  return PHX_OK;
+}
+
+bool is_wfa(int engine) { return engine == PHX_ENGINE_LINEAR || engine == PHX_ENGINE_AFFINE; }
+bool has_cigar(int engine) { return engine != PHX_ENGINE_MYERS_DIST; }
+
+// Tier geometry for a WFA launch.  Tier 0: one wave per pair, ring in LDS;
+// tier 1: 256 threads, ring in LDS; tiers 2-3: ring and history in HBM.
+Tier make_tier(const phx_plan* P, int t, uint64_t max_len_sum, uint32_t nwork) {
+  const phx_tuning& tu = P->ctx->tun;
+  const int R = P->ring_slots;
+  const int layers = P->engine == PHX_ENGINE_AFFINE ? 3 : 1;
+  Tier T{};
+  const uint64_t budget = tu.hbm_budget_mib << 20;
+  switch (t) {
+    case 0: {
+      T.variant = 0;
+      T.wcap = tu.wave_wcap;
+      while (T.wcap > 8 && (size_t)layers * R * T.wcap * 4 > kRingLdsMax) T.wcap >>= 1;
+      T.hist_stride = (uint64_t)tu.wave_hist_kib << 10;
+      T.grid = std::min<uint32_t>(nwork, tu.wave_slots);
+      T.lds = (size_t)layers * R * T.wcap * 4;
+      break;
+    }
+    case 1: {
+      T.variant = 1;
+      uint32_t wcap = tu.block_wcap;
+      while (wcap > 8 && (size_t)layers * R * wcap * 4 > kRingLdsMax) wcap >>= 1;
+      T.wcap = wcap;
+      T.hist_stride = (uint64_t)tu.block_hist_kib << 10;
+      T.grid = std::min<uint32_t>(nwork, tu.block_slots);
+      T.lds = (size_t)layers * R * T.wcap * 4;
+      break;
+    }
+    default: {
+      T.variant = t == 2 ? 2 : 3;
+      T.wcap = next_pow2(max_len_sum + 1);
+      const uint64_t ring_bytes = (uint64_t)layers * R * T.wcap * 4;
+      T.hist_stride = t == 2 ? (64ull << 20) : (1ull << 30) << (t - 3);
+      if (T.hist_stride > budget / 2) T.hist_stride = budget / 2;
+      const uint64_t per = T.hist_stride + ring_bytes;
+      const uint32_t max_slots = t == 2 ? 1024 : 128;
+      uint64_t g = std::max<uint64_t>(1, budget / per);
+      T.grid = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(g, max_slots), nwork);
+      T.lds = 0;
+      break;
+    }
+  }
+  if (T.grid == 0) T.grid = 1;
+  return T;
+}
+
+int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint32_t* d_list,
+                    uint32_t* d_counters) {
+  phx_ctx* ctx = P->ctx;
+  uint64_t max_sum = 1;
+  for (uint32_t i : work) max_sum = std::max<uint64_t>(max_sum, (uint64_t)P->pairs[i].qn + P->pairs[i].tn);
+  Tier T = make_tier(P, t, max_sum, (uint32_t)work.size());
+  const int layers = P->engine == PHX_ENGINE_AFFINE ? 3 : 1;
+  const int R = P->ring_slots;
+  if (dev_alloc(ctx, P->d_hist, (size_t)T.grid * T.hist_stride) != PHX_OK) return PHX_E_NOMEM;
+  uint64_t ring_stride = 0;
+  if (T.variant >= 2) {
+    ring_stride = (uint64_t)layers * R * T.wcap;
+    if (dev_alloc(ctx, P->d_ring, (size_t)T.grid * ring_stride * 4) != PHX_OK) return PHX_E_NOMEM;
+  }
+  WfaArgs a{};
+  a.seq = (const uint64_t*)P->d_seq.p;
+  a.pairs = (const PairDesc*)P->d_pairs.p;
+  a.work_list = d_list;
+  a.nwork = (uint32_t)work.size();
+  a.counters = d_counters;
+  a.cp = CostParams{P->costs.mismatch, P->costs.gap_open, P->costs.gap_extend,
+                    P->cap < 0 ? -1 : (int32_t)std::min<int64_t>(P->cap, INT32_MAX)};
+  a.out = (PairOut*)P->d_out.p;
+  a.runs = (uint32_t*)P->d_runs.p;
+  a.hist = (uint8_t*)P->d_hist.p;
+  a.hist_stride = T.hist_stride;
+  a.ring_g = (int32_t*)P->d_ring.p;
+  a.ring_stride = ring_stride;
+  a.wcap_log2 = log2u(T.wcap);
+  const int mode = P->engine == PHX_ENGINE_AFFINE ? ENG_AFFINE : ENG_LINEAR;
+  hipEvent_t e0, e1;
+  if (kernel_events(ctx, P->last_launches, &e0, &e1) != PHX_OK) return PHX_E_DEVICE;
+  HIPCHK(hipEventRecord(e0, ctx->stream));
+  HIPCHK(launch_wfa(mode, T.variant, a, T.grid, T.lds, ctx->stream));
+  HIPCHK(hipEventRecord(e1, ctx->stream));
+  ++P->last_launches;
+  return PHX_OK;
+}
+
+int run_wfa(phx_plan* P) {
+  phx_ctx* ctx = P->ctx;
+  const int ntiers = 6;
+  std::vector<std::vector<uint32_t>> lists = P->work;
+  lists.resize(ntiers);
+  uint32_t* d_list = (uint32_t*)P->d_work.p;
+  uint32_t* d_ctr = (uint32_t*)P->d_ctr.p;
+  uint32_t ctr_h[2] = {0, 0};
+  for (int t = 0; t < ntiers; ++t) {
+    if (lists[t].empty()) continue;
+    HIPCHK(hipMemcpyAsync(d_list, lists[t].data(), lists[t].size() * 4, hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(launch_zero(d_ctr, 2, ctx->stream));
+    int rc = launch_wfa_tier(P, t, lists[t], d_list, d_ctr);
+    if (rc != PHX_OK) return rc;
+    HIPCHK(hipMemcpyAsync(ctr_h, d_ctr, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ctr_h[1] == 0) continue;
+    // Some pairs did not fit this tier's slots: move them to the next tier.
+    std::vector<PairOut> outs(P->n);
+    HIPCHK(hipMemcpy(outs.data(), P->d_out.p, sizeof(PairOut) * P->n, hipMemcpyDeviceToHost));
+    if (t + 1 >= ntiers) break;
+    auto& nl = lists[t + 1];
+    for (uint32_t i : lists[t])
+      if (outs[i].status >= ST_OVERFLOW_HIST && outs[i].status <= ST_OVERFLOW_RUNS) nl.push_back(i);
+    std::stable_sort(nl.begin(), nl.end(), [&](uint32_t a, uint32_t b) {
+      return (uint64_t)P->pairs[a].qn + P->pairs[a].tn > (uint64_t)P->pairs[b].qn + P->pairs[b].tn;
+    });
+  }
+  return PHX_OK;
+}
+
+int run_myers(phx_plan* P) {
+  phx_ctx* ctx = P->ctx;
+  uint32_t* d_list_base = (uint32_t*)P->d_work.p;
+  const uint64_t budget = ctx->tun.hbm_budget_mib << 20;
+  uint64_t list_off = 0;
+  for (size_t g = 0; g < P->work.size(); ++g) {
+    const auto& wl = P->work[g];
+    if (wl.empty()) continue;
+    const int nbt = g <= 8 ? (int)g : 0;  // group index = block count (0: generic)
+    uint64_t stride = 0;
+    for (uint32_t i : wl) {
+      const uint64_t m = P->pairs[i].qn, n = P->pairs[i].tn;
+      const uint64_t nb = (m + 63) / 64;
+      uint64_t s = nbt == 0 ? 6 * nb * 8 : 0;
+      if (has_cigar(P->engine)) s += n * (16 * nb + 4);
+      stride = std::max<uint64_t>(stride, (s + 15) & ~(uint64_t)15);
+    }
+    uint64_t chunk = stride ? std::max<uint64_t>(1, budget / stride) : wl.size();
+    chunk = std::min<uint64_t>(chunk, wl.size());
+    if (stride && dev_alloc(ctx, P->d_scratch, (size_t)(chunk * stride)) != PHX_OK) return PHX_E_NOMEM;
+    for (uint64_t c0 = 0; c0 < wl.size(); c0 += chunk) {
+      const uint64_t cn = std::min<uint64_t>(chunk, wl.size() - c0);
+      MyersArgs a{};
+      a.seq = (const uint64_t*)P->d_seq.p;
+      a.pairs = (const PairDesc*)P->d_pairs.p;
+      a.work_list = d_list_base + list_off + c0;
+      a.nwork = (uint32_t)cn;
+      a.out = (PairOut*)P->d_out.p;
+      a.runs = (uint32_t*)P->d_runs.p;
+      a.scratch = (uint8_t*)P->d_scratch.p;
+      a.scratch_stride = stride;
+      a.engine = P->engine == PHX_ENGINE_MYERS ? ENG_MYERS
+               : P->engine == PHX_ENGINE_MYERS_GLOBAL ? ENG_MYERS_GLOBAL : ENG_MYERS_DIST;
+      hipEvent_t e0, e1;
+      if (kernel_events(ctx, P->last_launches, &e0, &e1) != PHX_OK) return PHX_E_DEVICE;
+      HIPCHK(hipEventRecord(e0, ctx->stream));
+      HIPCHK(launch_myers(nbt, a, ctx->stream));
+      HIPCHK(hipEventRecord(e1, ctx->stream));
+      ++P->last_launches;
+      if (c0 + cn < wl.size()) HIPCHK(hipStreamSynchronize(ctx->stream));  // scratch reuse
+    }
+    list_off += wl.size();
+  }
+  return PHX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* phx_version(void) { return "phraya_hip 0.1 gfx950"; }
+
+phx_ctx* phx_ctx_create(int device, uint32_t flags, const phx_tuning* tuning) {
+  (void)flags;
+  phx_ctx* c = new phx_ctx;
+  if (device >= 0) {
+    if (hipSetDevice(device) != hipSuccess) { delete c; return nullptr; }
+    c->device = device;
+  } else {
+    if (hipGetDevice(&c->device) != hipSuccess) { delete c; return nullptr; }
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return nullptr; }
+  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return nullptr;
+  }
+  phx_tuning d{};
+  d.wave_wcap = 64;
+  d.wave_hist_kib = 32;
+  d.wave_slots = (uint32_t)c->cu_count * 16;
+  d.block_wcap = 1024;
+  d.block_hist_kib = 4096;
+  d.block_slots = (uint32_t)c->cu_count * 2;
+  d.hbm_budget_mib = 32768;
+  if (tuning) {
+    if (tuning->wave_wcap) d.wave_wcap = next_pow2(tuning->wave_wcap);
+    if (tuning->wave_hist_kib) d.wave_hist_kib = tuning->wave_hist_kib;
+    if (tuning->wave_slots) d.wave_slots = tuning->wave_slots;
+    if (tuning->block_wcap) d.block_wcap = next_pow2(tuning->block_wcap);
+    if (tuning->block_hist_kib) d.block_hist_kib = tuning->block_hist_kib;
+    if (tuning->block_slots) d.block_slots = tuning->block_slots;
+    if (tuning->hbm_budget_mib) d.hbm_budget_mib = tuning->hbm_budget_mib;
+  }
+  c->tun = d;
+  return c;
+}
+
+void phx_ctx_destroy(phx_ctx* c) {
+  if (!c) return;
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipEventDestroy(c->ev0);
+  (void)hipEventDestroy(c->ev1);
+  for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* phx_ctx_last_error(const phx_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+uint64_t phx_cigar_bound(const phx_batch* b) {
+  if (!b || b->engine == PHX_ENGINE_MYERS_DIST) return 0;
+  uint64_t s = 0;
+  for (uint32_t i = 0; i < b->n; ++i) s += (b->q_off[i + 1] - b->q_off[i]) + (b->t_off[i + 1] - b->t_off[i]) + 1;
+  return s;
+}
+
+phx_plan* phx_plan_create(phx_ctx* ctx, const phx_batch* b) {
+  if (!ctx || !b || b->engine < 0 || b->engine > PHX_ENGINE_MYERS_DIST) {
+    if (ctx) ctx->err = "bad batch";
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (hipSetDevice(ctx->device) != hipSuccess) { ctx->err = "hipSetDevice failed"; return nullptr; }
+  phx_plan* P = new phx_plan;
+  P->ctx = ctx;
+  P->engine = b->engine;
+  P->costs = b->costs;
+  P->cap = b->cap;
+  P->n = b->n;
+  const uint32_t n = b->n;
+  if (b->engine == PHX_ENGINE_AFFINE) {
+    const phx_costs& c = b->costs;
+    if (c.mismatch < 1 || c.gap_extend < 1 || c.gap_open < 0) {
+      ctx->err = "affine costs need mismatch >= 1, gap_extend >= 1, gap_open >= 0";
+      delete P;
+      return nullptr;
+    }
+    const int32_t g = gcd3(c.mismatch, c.gap_open, c.gap_extend);
+    P->ring_slots = std::max(c.mismatch, std::max(c.gap_open + c.gap_extend, c.gap_extend)) / g + 1;
+    if (P->ring_slots > kRingSlotsMax) {
+      ctx->err = "affine costs span more score frames than the device ring holds";
+      delete P;
+      return nullptr;
+    }
+  }
+  P->seed_q.assign(n, 0);
+  P->seed_t.assign(n, 0);
+  P->qlen.assign(n, 0);
+  P->tlen.assign(n, 0);
+  P->host_status.assign(n, ST_PENDING);
+  P->pairs.assign(n, PairDesc{});
+  // Pass 1: seeds, lengths, encodings, word and run offsets.
+  std::vector<uint8_t> b8(n, 0);
+  uint64_t words = 0, runs = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint64_t qfull = b->q_off[i + 1] - b->q_off[i], tfull = b->t_off[i + 1] - b->t_off[i];
+    uint64_t sq = 0, st = 0;
+    if (b->engine != PHX_ENGINE_MYERS_GLOBAL) {
+      sq = b->seed_q ? b->seed_q[i] : 0;
+      st = b->seed_t ? b->seed_t[i] : 0;
+    }
+    P->seed_q[i] = sq;
+    P->seed_t[i] = st;
+    if (sq > qfull || st > tfull) {
+      P->host_status[i] = ST_INVALID_INPUT;  // lib.rs:107-111
+      continue;
+    }
+    const uint64_t qn = qfull - sq, tn = tfull - st;
+    if (qn > (uint64_t)kMaxLen || tn > (uint64_t)kMaxLen) {
+      ctx->err = "sequence longer than the device format supports (2^27 - 2)";
+      delete P;
+      return nullptr;
+    }
+    P->qlen[i] = qn;
+    P->tlen[i] = tn;
+    const uint8_t* qs = b->q_seq + b->q_off[i] + sq;
+    const uint8_t* ts = b->t_seq + b->t_off[i] + st;
+    b8[i] = !(is_acgt(qs, qn) && is_acgt(ts, tn));
+    PairDesc& pd = P->pairs[i];
+    pd.qn = (int32_t)qn;
+    pd.tn = (int32_t)tn;
+    pd.flags = b8[i] ? PF_BYTES : 0;
+    pd.q_word = words;
+    words += words_for(qn, b8[i]);
+    pd.t_word = words;
+    words += words_for(tn, b8[i]);
+    pd.run_cap = has_cigar(b->engine) ? (uint32_t)(qn + tn + 1) : 0;
+    pd.run_off = runs;
+    runs += pd.run_cap;
+    P->input_bytes += b8[i] ? qn + tn : (qn + 3) / 4 + (tn + 3) / 4;
+  }
+  P->seq_words = words;
+  P->runs_total = runs;
+  // Pass 2: pack.
+  std::vector<uint64_t> packed(words ? words : 1);
+  for (uint32_t i = 0; i < n; ++i) {
+    if (P->host_status[i] == ST_INVALID_INPUT) continue;
+    const PairDesc& pd = P->pairs[i];
+    pack_into(packed.data() + pd.q_word, b->q_seq + b->q_off[i] + P->seed_q[i], P->qlen[i], b8[i]);
+    pack_into(packed.data() + pd.t_word, b->t_seq + b->t_off[i] + P->seed_t[i], P->tlen[i], b8[i]);
+  }
+  // Work lists.
+  std::vector<uint32_t> order;
+  order.reserve(n);
+  for (uint32_t i = 0; i < n; ++i)
+    if (P->host_status[i] == ST_PENDING) order.push_back(i);
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) {
+    return P->qlen[a] + P->tlen[a] > P->qlen[c] + P->tlen[c];
+  });
+  if (is_wfa(b->engine)) {
+    P->work.assign(4, {});
+    for (uint32_t i : order) {
+      const uint64_t L = P->qlen[i] + P->tlen[i];
+      const int t = L <= 2048 ? 0 : (L <= 16384 ? 1 : 2);
+      P->work[t].push_back(i);
+    }
+  } else {
+    P->work.assign(9, {});
+    for (uint32_t i : order) {
+      const uint64_t m = P->qlen[i];
+      const uint64_t nb = m == 0 ? 1 : (m + 63) / 64;
+      P->work[nb <= 8 ? nb : 0].push_back(i);
+    }
+  }
+  // Device buffers.
+  auto fail_plan = [&](const char* what) -> phx_plan* {
+    ctx->err = std::string("device allocation failed: ") + what;
+    phx_plan_destroy(P);
+    return nullptr;
+  };
+  if (dev_alloc(ctx, P->d_seq, words * 8 + 8) != PHX_OK) return fail_plan("seq");
+  if (dev_alloc(ctx, P->d_pairs, sizeof(PairDesc) * (size_t)std::max<uint32_t>(n, 1)) != PHX_OK) return fail_plan("pairs");
+  if (dev_alloc(ctx, P->d_out, sizeof(PairOut) * (size_t)std::max<uint32_t>(n, 1)) != PHX_OK) return fail_plan("out");
+  if (dev_alloc(ctx, P->d_runs, runs * 4 + 4) != PHX_OK) return fail_plan("runs");
+  if (dev_alloc(ctx, P->d_work, (size_t)std::max<uint32_t>(n, 1) * 4) != PHX_OK) return fail_plan("work");
+  if (dev_alloc(ctx, P->d_ctr, 64) != PHX_OK) return fail_plan("counters");
+  if (hipMemcpy(P->d_seq.p, packed.data(), words * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      (n && hipMemcpy(P->d_pairs.p, P->pairs.data(), sizeof(PairDesc) * n, hipMemcpyHostToDevice) != hipSuccess))
+    return fail_plan("upload");
+  if (!is_wfa(b->engine)) {
+    // Myers work lists are static: upload once, grouped.
+    std::vector<uint32_t> flat;
+    for (auto& w : P->work) flat.insert(flat.end(), w.begin(), w.end());
+    if (!flat.empty() &&
+        hipMemcpy(P->d_work.p, flat.data(), flat.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+      return fail_plan("work upload");
+  }
+  return P;
+}
+
+int phx_plan_run(phx_plan* P) {
+  if (!P) return PHX_E_ARG;
+  phx_ctx* ctx = P->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  P->out_valid = false;
+  P->last_launches = 0;
+  HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+  int rc = is_wfa(P->engine) ? run_wfa(P) : run_myers(P);
+  if (rc != PHX_OK) return rc;
+  HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+  HIPCHK(hipEventSynchronize(ctx->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  P->last_ms = ms;
+  double kms = 0;
+  for (uint32_t i = 0; i < P->last_launches; ++i) {
+    float k = 0;
+    HIPCHK(hipEventElapsedTime(&k, ctx->kev[2 * i], ctx->kev[2 * i + 1]));
+    kms += k;
+  }
+  P->last_kernel_ms = kms;
+  return PHX_OK;
+}
+
+double phx_plan_last_ms(const phx_plan* P) { return P ? P->last_ms : 0.0; }
+double phx_plan_last_kernel_ms(const phx_plan* P) { return P ? P->last_kernel_ms : 0.0; }
+uint32_t phx_plan_last_launches(const phx_plan* P) { return P ? P->last_launches : 0; }
+
+static int fetch_out(phx_plan* P) {
+  phx_ctx* ctx = P->ctx;
+  if (P->out_valid) return PHX_OK;
+  P->out_h.resize(P->n);
+  if (P->n) HIPCHK(hipMemcpy(P->out_h.data(), P->d_out.p, sizeof(PairOut) * P->n, hipMemcpyDeviceToHost));
+  P->out_valid = true;
+  return PHX_OK;
+}
+
+uint64_t phx_plan_algorithmic_bytes(const phx_plan* Pc) {
+  phx_plan* P = const_cast<phx_plan*>(Pc);
+  if (!P) return 0;
+  std::lock_guard<std::mutex> lk(P->ctx->mu);
+  if (fetch_out(P) != PHX_OK) return 0;
+  uint64_t bytes = P->input_bytes;
+  for (uint32_t i = 0; i < P->n; ++i) {
+    if (P->host_status[i] != ST_PENDING) continue;
+    bytes += P->engine == PHX_ENGINE_MYERS_DIST ? 16 : 12 + 4ull * (uint64_t)P->out_h[i].nruns;
+  }
+  return bytes;
+}
+
+int phx_plan_results(phx_plan* P, phx_results* r) {
+  if (!P || !r || !r->aln || !r->run_off) return PHX_E_ARG;
+  phx_ctx* ctx = P->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  int rc = fetch_out(P);
+  if (rc != PHX_OK) return rc;
+  std::vector<uint32_t> runs_h;
+  if (has_cigar(P->engine) && P->runs_total) {
+    runs_h.resize(P->runs_total);
+    HIPCHK(hipMemcpy(runs_h.data(), P->d_runs.p, P->runs_total * 4, hipMemcpyDeviceToHost));
+  }
+  uint64_t pos = 0;
+  int ret = PHX_OK;
+  for (uint32_t i = 0; i < P->n; ++i) {
+    phx_alignment& a = r->aln[i];
+    std::memset(&a, 0, sizeof(a));
+    r->run_off[i] = pos;
+    if (P->host_status[i] != ST_PENDING) {
+      a.status = P->host_status[i];
+      continue;
+    }
+    const PairOut& o = P->out_h[i];
+    if (o.status != ST_OK && o.status != ST_ALIGNMENT_FAILED) {
+      a.status = PHX_E_INTERNAL;
+      ret = PHX_E_INTERNAL;
+      ctx->err = "pair " + std::to_string(i) + " ended with device status " + std::to_string(o.status);
+      continue;
+    }
+    a.status = o.status;
+    if (o.status != ST_OK) continue;
+    a.score = o.score;
+    a.edit_distance = o.edit_distance;
+    a.query_start = (int64_t)P->seed_q[i];
+    a.query_end = (int64_t)(P->seed_q[i] + P->qlen[i]);
+    a.target_start = (int64_t)P->seed_t[i];
+    a.target_end = (int64_t)P->seed_t[i] + o.t_end;
+    if (P->engine == PHX_ENGINE_MYERS_GLOBAL) {
+      a.query_start = a.target_start = 0;
+      a.query_end = (int64_t)P->qlen[i];
+      a.target_end = (int64_t)P->tlen[i];
+    }
+    if (has_cigar(P->engine) && o.nruns > 0) {
+      if (r->runs && pos + (uint64_t)o.nruns <= r->runs_cap)
+        std::memcpy(r->runs + pos, runs_h.data() + P->pairs[i].run_off + (uint64_t)o.run_start,
+                    (size_t)o.nruns * 4);
+      pos += (uint64_t)o.nruns;
+    }
+  }
+  r->run_off[P->n] = pos;
+  if (ret == PHX_OK && has_cigar(P->engine) && (pos > r->runs_cap || (!r->runs && pos > 0))) {
+    ctx->err = "CIGAR arena too small";
+    return PHX_E_CAPACITY;
+  }
+  return ret;
+}
+
+void phx_plan_destroy(phx_plan* P) {
+  if (!P) return;
+  (void)hipStreamSynchronize(P->ctx->stream);
+  for (DevBuf* b : {&P->d_seq, &P->d_pairs, &P->d_out, &P->d_runs, &P->d_work, &P->d_ctr, &P->d_hist,
+                    &P->d_ring, &P->d_scratch})
+    dev_free(*b);
+  delete P;
+}
+
+int phx_align_batch(phx_ctx* ctx, const phx_batch* b, phx_results* out) {
+  if (!ctx || !b || !out) return PHX_E_ARG;
+  phx_plan* P = phx_plan_create(ctx, b);
+  if (!P) return PHX_E_ARG;
+  int rc = phx_plan_run(P);
+  if (rc == PHX_OK) rc = phx_plan_results(P, out);
+  phx_plan_destroy(P);
+  return rc;
+}
+
+size_t phx_format_cigar(const uint32_t* runs, uint64_t nruns, char* buf, size_t cap) {
+  static const char kOps[4] = {'M', 'X', 'I', 'D'};
+  size_t len = 0;
+  char tmp[16];
+  for (uint64_t i = 0; i < nruns; ++i) {
+    const int k = snprintf(tmp, sizeof(tmp), "%u%c", runs[i] >> 4, kOps[runs[i] & 3]);
+    for (int j = 0; j < k; ++j) {
+      if (buf && len + 1 < cap) buf[len] = tmp[j];
+      ++len;
+    }
+  }
+  if (buf && cap) buf[std::min(len, cap - 1)] = 0;
+  return len;
+}
+
+static int one_pair(phx_ctx* ctx, int engine, const uint8_t* q, size_t qlen, const uint8_t* t,
+                    size_t tlen, uint64_t sq, uint64_t st, phx_costs costs, phx_alignment* out,
+                    char* cigar, size_t cigar_cap, size_t* cigar_len) {
+  if (!ctx || !out) return PHX_E_ARG;
+  static const uint8_t kEmpty = 0;
+  const uint64_t qo[2] = {0, qlen}, to[2] = {0, tlen};
+  phx_batch b{};
+  b.n = 1;
+  b.engine = engine;
+  b.costs = costs;
+  b.cap = -1;
+  b.q_seq = q ? q : &kEmpty;
+  b.q_off = qo;
+  b.t_seq = t ? t : &kEmpty;
+  b.t_off = to;
+  b.seed_q = &sq;
+  b.seed_t = &st;
+  std::vector<uint32_t> runs(qlen + tlen + 2);
+  uint64_t roff[2] = {0, 0};
+  phx_results r{out, runs.data(), runs.size(), roff};
+  const int rc = phx_align_batch(ctx, &b, &r);
+  if (rc != PHX_OK) return rc;
+  const size_t len = phx_format_cigar(runs.data(), roff[1], cigar, cigar_cap);
+  if (cigar_len) *cigar_len = len;
+  return out->status;
+}
+
+int phx_wfa_extend(phx_ctx* ctx, const uint8_t* q, size_t qlen, const uint8_t* t, size_t tlen,
+                   uint64_t seed_q, uint64_t seed_t, phx_alignment* out, char* cigar,
+                   size_t cigar_cap, size_t* cigar_len) {
+  return one_pair(ctx, PHX_ENGINE_LINEAR, q, qlen, t, tlen, seed_q, seed_t, phx_costs{0, 0, 0}, out,
+                  cigar, cigar_cap, cigar_len);
+}
+
+int phx_wfa_extend_affine(phx_ctx* ctx, const uint8_t* q, size_t qlen, const uint8_t* t,
+                          size_t tlen, uint64_t seed_q, uint64_t seed_t, phx_costs costs,
+                          phx_alignment* out, char* cigar, size_t cigar_cap, size_t* cigar_len) {
+  return one_pair(ctx, PHX_ENGINE_AFFINE, q, qlen, t, tlen, seed_q, seed_t, costs, out, cigar,
+                  cigar_cap, cigar_len);
+}
+
+int phx_myers_extend(phx_ctx* ctx, const uint8_t* q, size_t qlen, const uint8_t* t, size_t tlen,
+                     uint64_t seed_q, uint64_t seed_t, phx_alignment* out, char* cigar,
+                     size_t cigar_cap, size_t* cigar_len) {
+  return one_pair(ctx, PHX_ENGINE_MYERS, q, qlen, t, tlen, seed_q, seed_t, phx_costs{0, 0, 0}, out,
+                  cigar, cigar_cap, cigar_len);
+}
+
+int phx_myers_edit_distance(phx_ctx* ctx, const uint8_t* q, size_t qlen, const uint8_t* t,
+                            size_t tlen, int64_t* edit_distance, char* cigar, size_t cigar_cap,
+                            size_t* cigar_len) {
+  phx_alignment a{};
+  const int rc = one_pair(ctx, PHX_ENGINE_MYERS_GLOBAL, q, qlen, t, tlen, 0, 0, phx_costs{0, 0, 0},
+                          &a, cigar, cigar_cap, cigar_len);
+  if (edit_distance) *edit_distance = a.edit_distance;
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,104 @@
+// phx_device.h -- device-side data layout shared by the HIP kernels and the
+// host runtime of libphraya_hip.so.
+//
+// Sequences live in one u64 word arena in HBM.  A pair's query and target are
+// each either 2-bit packed (A=0 C=1 G=2 T=3, 32 bases per word, base i in
+// bits [2i, 2i+2) of word i/32) or, when either sequence holds a byte outside
+// uppercase ACGT, 8-bit raw (8 bytes per word, byte i in bits [8i, 8i+8)).
+// Both encodings compare exactly what upstream compares (raw bytes), because
+// the 2-bit code is a bijection on {A,C,G,T}.  Every sequence is followed by
+// one zero pad word so a two-word funnel read never leaves its allocation.
+#ifndef PHX_DEVICE_H
+#define PHX_DEVICE_H
+
+#include <stdint.h>
+
+namespace phx {
+
+enum Engine : int32_t {
+  ENG_LINEAR = 0,
+  ENG_AFFINE = 1,
+  ENG_MYERS = 2,         // myers_extend: fitting/global + CIGAR
+  ENG_MYERS_GLOBAL = 3,  // myers_edit_distance: global + CIGAR
+  ENG_MYERS_DIST = 4     // myers_extend distance + end column only
+};
+
+// Device status codes (the C-ABI reports the same numbers).
+enum Status : int32_t {
+  ST_OK = 0,
+  ST_INVALID_INPUT = 1,
+  ST_ALIGNMENT_FAILED = 2,
+  ST_PENDING = 15,          // not yet run
+  ST_OVERFLOW_HIST = 16,    // history region too small: rerun with more
+  ST_OVERFLOW_WIDTH = 17,   // wavefront wider than the kernel's ring
+  ST_OVERFLOW_RUNS = 18,    // CIGAR run capacity exceeded
+  ST_INTERNAL = 19          // traceback reached a cell that was never set
+};
+
+enum PairFlags : uint32_t {
+  PF_BYTES = 1u,  // 8-bit encoding
+};
+
+struct PairDesc {
+  uint64_t q_word;    // word offset of the query in the sequence arena
+  uint64_t t_word;    // word offset of the target
+  int32_t qn, tn;     // suffix lengths (the host applies the seed)
+  uint32_t flags;
+  uint32_t run_cap;   // CIGAR run capacity
+  uint64_t run_off;   // offset (in runs) of this pair's run slots
+  uint64_t hist_off;  // byte offset of this pair's history region
+  uint64_t hist_cap;  // bytes in the history region
+};
+
+struct PairOut {
+  int32_t status;
+  int32_t score;          // linear s / affine s / Myers distance
+  int32_t edit_distance;  // reported distance (affine: non-M bases of the CIGAR)
+  int32_t t_end;          // target bases consumed
+  int32_t nruns;          // CIGAR runs
+  int32_t run_start;      // first run slot (runs occupy [run_start, run_start + nruns))
+  int32_t steps;          // score steps explored (diagnostic / memory model)
+  int32_t cells;          // history cells written (diagnostic / memory model)
+};
+
+struct CostParams {
+  int32_t x, o, e;  // mismatch, gap open, gap extend (affine); unused for linear
+  int32_t cap;      // max score explored; -1 = none
+};
+
+// CIGAR run encoding: (length << 4) | op.
+enum RunOp : uint32_t { OP_M = 0, OP_X = 1, OP_I = 2, OP_D = 3 };
+
+// Kernel arguments of one WFA tier launch.
+struct WfaArgs {
+  const uint64_t* seq;
+  const PairDesc* pairs;
+  const uint32_t* work_list;
+  uint32_t nwork;
+  uint32_t* counters;    // [0] work queue head, [1] pairs that overflowed this tier
+  CostParams cp;
+  PairOut* out;
+  uint32_t* runs;
+  uint8_t* hist;         // one slot of hist_stride bytes per workgroup
+  uint64_t hist_stride;
+  int32_t* ring_g;       // one slot of ring_stride ints per workgroup (HBM ring tiers)
+  uint64_t ring_stride;
+  uint32_t wcap_log2;    // ring width in diagonals = 1 << wcap_log2
+};
+
+// Kernel arguments of one Myers launch.
+struct MyersArgs {
+  const uint64_t* seq;
+  const PairDesc* pairs;
+  const uint32_t* work_list;
+  uint32_t nwork;
+  PairOut* out;
+  uint32_t* runs;
+  uint8_t* scratch;        // one slot of scratch_stride bytes per work item
+  uint64_t scratch_stride;
+  int32_t engine;          // ENG_MYERS, ENG_MYERS_GLOBAL or ENG_MYERS_DIST
+};
+
+}  // namespace phx
+
+#endif

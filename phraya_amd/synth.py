@@ -1,0 +1,144 @@
+"""Seeded synthetic workloads (host side) for benchmarks and tests.
+
+Workloads (read = query, window = target; seed (0, 0)):
+  short150   10,000 reads of 150 bp from a 5 Mbp uniform-ACGT genome, 1%
+             substitutions + 0.1% indel events (length 1-3, geometric p=1/2),
+             against the 170 bp window starting at the read's source position.
+  myers150   1,000,000 reads of 150 bp, per-read event rate ~ U[0, 5%] (80%
+             substitutions, 20% indels of length 1-3), against 180 bp windows.
+  mixed      200,000 reads, lengths log-uniform in [100, 20000]; 70% class
+             "short" (1% events, 10% of them indels) and 30% class "long" (8%
+             events, 60% indels, geometric length mean 2, max 20); window =
+             ceil(1.1 * L).
+Event rates are per source site; see csrc/synth.cpp for the walk.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SYNTH_LIB = os.path.join(HERE, "libphx_synth.so")
+
+
+class SynthParams(ctypes.Structure):
+    _fields_ = [
+        ("genome_len", ctypes.c_uint64), ("genome_seed", ctypes.c_uint64),
+        ("read_seed", ctypes.c_uint64), ("n", ctypes.c_uint32),
+        ("len_lo", ctypes.c_uint32), ("len_hi", ctypes.c_uint32),
+        ("log_uniform", ctypes.c_int32), ("window_scale", ctypes.c_double),
+        ("window_add", ctypes.c_int32), ("class_a_frac", ctypes.c_double),
+        ("a_err_lo", ctypes.c_double), ("a_err_hi", ctypes.c_double),
+        ("a_indel_frac", ctypes.c_double), ("b_err_lo", ctypes.c_double),
+        ("b_err_hi", ctypes.c_double), ("b_indel_frac", ctypes.c_double),
+        ("indel_mean", ctypes.c_double), ("indel_max", ctypes.c_int32),
+    ]
+
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(SYNTH_LIB):
+            from .build import build_synth
+            build_synth()
+        L = ctypes.CDLL(SYNTH_LIB)
+        P = ctypes.c_void_p
+        L.synth_genome.argtypes = [ctypes.c_uint64, ctypes.c_uint64, P]
+        L.synth_pairs.argtypes = [P, P]
+        L.synth_pairs.restype = P
+        L.synth_q_bytes.argtypes = [P]
+        L.synth_q_bytes.restype = ctypes.c_uint64
+        L.synth_t_bytes.argtypes = [P]
+        L.synth_t_bytes.restype = ctypes.c_uint64
+        L.synth_copy.argtypes = [P, P, P, P, P, P]
+        L.synth_free.argtypes = [P]
+        _lib = L
+    return _lib
+
+
+@dataclass
+class Workload:
+    name: str
+    q_seq: np.ndarray
+    q_off: np.ndarray
+    t_seq: np.ndarray
+    t_off: np.ndarray
+    cls: np.ndarray
+    params: dict
+
+    @property
+    def n(self) -> int:
+        return len(self.q_off) - 1
+
+    def pair(self, i: int) -> tuple[bytes, bytes]:
+        return (self.q_seq[self.q_off[i]:self.q_off[i + 1]].tobytes(),
+                self.t_seq[self.t_off[i]:self.t_off[i + 1]].tobytes())
+
+    def subset(self, idx) -> "Workload":
+        idx = np.asarray(idx)
+        qs = [self.q_seq[self.q_off[i]:self.q_off[i + 1]] for i in idx]
+        ts = [self.t_seq[self.t_off[i]:self.t_off[i + 1]] for i in idx]
+        qo = np.zeros(len(idx) + 1, np.uint64)
+        to = np.zeros(len(idx) + 1, np.uint64)
+        qo[1:] = np.cumsum([len(x) for x in qs])
+        to[1:] = np.cumsum([len(x) for x in ts])
+        return Workload(self.name, np.concatenate(qs) if qs else np.zeros(0, np.uint8), qo,
+                        np.concatenate(ts) if ts else np.zeros(0, np.uint8), to, self.cls[idx],
+                        dict(self.params, subset=len(idx)))
+
+
+def genome(length: int = 5_000_000, seed: int = 0x5EED0001) -> np.ndarray:
+    g = np.empty(length, dtype=np.uint8)
+    _load().synth_genome(length, seed, g.ctypes.data)
+    return g
+
+
+def generate(p: dict, name: str = "custom") -> Workload:
+    L = _load()
+    g = genome(p.get("genome_len", 5_000_000), p.get("genome_seed", 0x5EED0001))
+    sp = SynthParams(**{k: v for k, v in p.items() if k in dict(SynthParams._fields_)})
+    sp.genome_len = len(g)
+    h = L.synth_pairs(ctypes.byref(sp), g.ctypes.data)
+    try:
+        n = sp.n
+        q = np.empty(max(L.synth_q_bytes(h), 1), np.uint8)
+        t = np.empty(max(L.synth_t_bytes(h), 1), np.uint8)
+        qo = np.empty(n + 1, np.uint64)
+        to = np.empty(n + 1, np.uint64)
+        cls = np.empty(max(n, 1), np.uint8)
+        L.synth_copy(h, q.ctypes.data, qo.ctypes.data, t.ctypes.data, to.ctypes.data, cls.ctypes.data)
+    finally:
+        L.synth_free(h)
+    return Workload(name, q, qo, t, to, cls[:n], dict(p))
+
+
+WORKLOADS = {
+    "short150": dict(genome_seed=0x5EED0001, read_seed=0xC0FFEE01, n=10_000, len_lo=150,
+                     len_hi=150, log_uniform=0, window_scale=1.0, window_add=20,
+                     class_a_frac=1.0, a_err_lo=0.011, a_err_hi=0.011,
+                     a_indel_frac=0.001 / 0.011, b_err_lo=0, b_err_hi=0, b_indel_frac=0,
+                     indel_mean=2.0, indel_max=3),
+    "myers150": dict(genome_seed=0x5EED0001, read_seed=0xC0FFEE02, n=1_000_000, len_lo=150,
+                     len_hi=150, log_uniform=0, window_scale=1.0, window_add=30,
+                     class_a_frac=1.0, a_err_lo=0.0, a_err_hi=0.05, a_indel_frac=0.2,
+                     b_err_lo=0, b_err_hi=0, b_indel_frac=0, indel_mean=2.0, indel_max=3),
+    "mixed": dict(genome_seed=0x5EED0001, read_seed=0xC0FFEE03, n=200_000, len_lo=100,
+                  len_hi=20_000, log_uniform=1, window_scale=1.1, window_add=0,
+                  class_a_frac=0.7, a_err_lo=0.01, a_err_hi=0.01, a_indel_frac=0.1,
+                  b_err_lo=0.08, b_err_hi=0.08, b_indel_frac=0.6, indel_mean=2.0, indel_max=20),
+}
+
+
+def workload(name: str, n: int | None = None, seed: int | None = None) -> Workload:
+    p = dict(WORKLOADS[name])
+    if n is not None:
+        p["n"] = n
+    if seed is not None:
+        p["read_seed"] = seed
+    return generate(p, name)

@@ -1,0 +1,54 @@
+"""CPU checks of the drop-in boundary: the library loads, exports every
+symbol include/phraya_hip.h declares, and its host-only helpers behave."""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+from phraya_amd import _lib as L
+from phraya_amd import format_cigar, pack_batch
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "phraya_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(phx_[a-z_0-9]+)\s*\(", src)))
+
+
+def test_header_declares_what_python_binds():
+    assert declared_symbols() == sorted(L.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.lib()
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+    assert lib.phx_version().decode().startswith("phraya_hip")
+
+
+def test_format_cigar_host_helper():
+    lib = L.lib()
+    runs = np.array([(150 << 4) | 0, (20 << 4) | 2, (1 << 4) | 1, (3 << 4) | 3], dtype=np.uint32)
+    buf = ctypes.create_string_buffer(64)
+    n = lib.phx_format_cigar(runs.ctypes.data, len(runs), buf, 64)
+    assert buf.value.decode() == "150M20I1X3D" and n == len("150M20I1X3D")
+    assert format_cigar(runs) == "150M20I1X3D"
+    # truncation keeps the NUL and reports the full length
+    small = ctypes.create_string_buffer(4)
+    assert lib.phx_format_cigar(runs.ctypes.data, len(runs), small, 4) == 11
+    assert small.value.decode() == "150"
+
+
+def test_cigar_bound_counts_worst_case_runs():
+    lib = L.lib()
+    pb = pack_batch([b"ACGT", b"", b"AC"], [b"ACG", b"TT", b""])
+    from phraya_amd import _cbatch, AffineCosts
+    cb = _cbatch(pb, L.ENGINE_AFFINE, AffineCosts(), None)
+    assert lib.phx_cigar_bound(ctypes.byref(cb)) == (4 + 3 + 1) + (0 + 2 + 1) + (2 + 0 + 1)
+    cb = _cbatch(pb, L.ENGINE_MYERS_DIST, AffineCosts(), None)
+    assert lib.phx_cigar_bound(ctypes.byref(cb)) == 0

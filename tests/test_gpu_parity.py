@@ -1,0 +1,243 @@
+"""HIP path vs the CPU oracle, bit-exact (scores, edit distances, ends, CIGARs).
+
+Every test goes through the C ABI (libphraya_hip.so via phraya_amd)."""
+from __future__ import annotations
+
+import random
+import sys
+
+import numpy as np
+import pytest
+
+import kat
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ENG = {"linear": (0, O.LINEAR), "affine": (1, O.AFFINE), "myers": (2, O.MYERS),
+       "myers_global": (3, O.MYERS_GLOBAL), "myers_dist": (4, O.MYERS_FWD)}
+
+
+def say(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def px():
+    import phraya_amd
+    return phraya_amd
+
+
+class GpuImpl(kat.Impl):
+    def __init__(self, px):
+        self.px = px
+
+    def _d(self, fn, *a, **kw):
+        try:
+            r = fn(*a, **kw)
+        except self.px.InvalidInput:
+            return {"status": "InvalidInput"}
+        except self.px.AlignmentFailed:
+            return {"status": "AlignmentFailed"}
+        return dict(status="ok", cigar=r.cigar, edit_distance=r.edit_distance,
+                    query_start=r.query_start, query_end=r.query_end,
+                    target_start=r.target_start, target_end=r.target_end, score=r.score)
+
+    def wfa_extend(self, q, t, seed, cap=None):
+        return self._d(self.px.wfa_extend, q, t, self.px.SeedAnchor(*seed), cap)
+
+    def wfa_extend_affine(self, q, t, seed, costs):
+        return self._d(self.px.wfa_extend_affine, q, t, self.px.SeedAnchor(*seed),
+                       self.px.AffineCosts(*costs))
+
+    def myers_extend(self, q, t, seed):
+        return self._d(self.px.myers_extend, q, t, self.px.SeedAnchor(*seed))
+
+    def myers_edit_distance(self, q, t):
+        return self.px.myers_edit_distance(q, t)
+
+    def lcp(self, a, b):
+        # The match-extend primitive is exercised through a perfect-prefix alignment:
+        # wfa_extend of a vs b (fitting off) starts with an M run of LCP length.
+        r = self.px.wfa_extend(a, b)
+        first = r.cigar
+        n = 0
+        num = ""
+        for ch in first:
+            if ch.isdigit():
+                num += ch
+            else:
+                n = int(num) if ch == "M" else 0
+                break
+        return n
+
+
+@pytest.mark.parametrize("k", kat.load(), ids=lambda k: k["src"].split("/")[-1])
+def test_gpu_kat(px, k):
+    say(f"kat {k['src']}")
+    kat.check(k, GpuImpl(px))
+
+
+# ------------------------------------------------------------------- batch compare
+
+def compare(px, engine, qs, ts, seeds=None, costs=(2, 4, 1), cap=None, ctx=None, label=""):
+    eng_gpu, eng_orc = ENG[engine]
+    say(f"compare {engine} {label} n={len(qs)} costs={costs} cap={cap}")
+    res = px.align_batch(eng_gpu, qs, ts, seeds, px.AffineCosts(*costs), cap, ctx)
+    pb = px.pack_batch(qs, ts, seeds)
+    ores, ocig = O.batch(eng_orc, pb.q_seq, pb.q_off, pb.t_seq, pb.t_off, pb.seed_q, pb.seed_t,
+                         costs=costs, cap=-1 if cap is None else cap, threads=8)
+    g = res.aln
+    bad = []
+    for i in range(len(qs)):
+        gs, os_ = int(g["status"][i]), int(ores["status"][i])
+        if gs != os_:
+            bad.append((i, "status", gs, os_))
+            continue
+        if gs != 0:
+            continue
+        for f in ("edit_distance", "query_start", "query_end", "target_start", "target_end"):
+            if int(g[f][i]) != int(ores[f][i]):
+                bad.append((i, f, int(g[f][i]), int(ores[f][i])))
+        if engine in ("linear", "affine") and int(g["score"][i]) != int(ores["score"][i]):
+            bad.append((i, "score", int(g["score"][i]), int(ores["score"][i])))
+        if engine != "myers_dist" and res.cigar(i) != ocig[i]:
+            bad.append((i, "cigar", res.cigar(i)[:80], ocig[i][:80]))
+    assert not bad, f"{len(bad)} mismatches, first: {bad[:5]}"
+    return res
+
+
+def rand_seq(rng, n, alphabet=b"ACGT"):
+    return bytes(rng.choice(alphabet) for _ in range(n))
+
+
+def mutate(rng, s, rate, alphabet=b"ACGT"):
+    out = bytearray()
+    i = 0
+    while i < len(s):
+        r = rng.random()
+        if r < rate * 0.25:
+            out += rand_seq(rng, rng.randint(1, 3), alphabet)
+            continue
+        if r < rate * 0.5:
+            i += rng.randint(1, 3)
+            continue
+        out.append(rng.choice([c for c in alphabet if c != s[i]]) if r < rate else s[i])
+        i += 1
+    return bytes(out)
+
+
+def random_batch(seed, n, qmax, alphabet=b"ACGT", fitting_frac=0.3, seeds=False):
+    rng = random.Random(seed)
+    qs, ts, ss = [], [], []
+    for _ in range(n):
+        qn = rng.randint(0, qmax)
+        q = rand_seq(rng, qn, alphabet)
+        t = mutate(rng, q, rng.choice([0.0, 0.01, 0.05, 0.15, 0.3]), alphabet)
+        if rng.random() < fitting_frac:
+            t += rand_seq(rng, qn // 2 + rng.randint(11, 40), alphabet)
+        if seeds:
+            sq = rng.randint(0, min(len(q), 5))
+            st = rng.randint(0, min(len(t), 5))
+            if rng.random() < 0.05:
+                sq = len(q) + 1  # InvalidInput
+            q = rand_seq(rng, sq, alphabet) + q if rng.random() < 0.5 else q
+            ss.append((sq, st))
+        qs.append(q)
+        ts.append(t)
+    return qs, ts, (ss if seeds else None)
+
+
+@pytest.mark.parametrize("engine", ["linear", "affine", "myers", "myers_global", "myers_dist"])
+def test_random_small(px, engine):
+    qs, ts, ss = random_batch(100 + len(engine), 600, 120, seeds=True)
+    compare(px, engine, qs, ts, ss, costs=(4, 6, 2), label="small")
+
+
+@pytest.mark.parametrize("costs", [(2, 4, 1), (4, 6, 2), (1, 0, 1), (3, 5, 2), (1, 30, 1)])
+def test_affine_cost_sets(px, costs):
+    qs, ts, _ = random_batch(7 + costs[0] * 10 + costs[1], 300, 90)
+    compare(px, "affine", qs, ts, costs=costs, label="costs")
+
+
+@pytest.mark.parametrize("engine", ["linear", "affine", "myers", "myers_global", "myers_dist"])
+def test_bytes_path(px, engine):
+    """Pairs with bytes outside uppercase ACGT take the 8-bit encoding."""
+    qs, ts, _ = random_batch(55, 300, 100, alphabet=b"ACGTNacgt")
+    compare(px, engine, qs, ts, costs=(4, 6, 2), label="bytes")
+    prot = b"ACDEFGHIKLMNPQRSTVWY"
+    qs, ts, _ = random_batch(56, 100, 80, alphabet=prot)
+    compare(px, engine, qs, ts, costs=(4, 6, 2), label="protein")
+
+
+@pytest.mark.parametrize("engine", ["linear", "affine"])
+def test_caps(px, engine):
+    rng = random.Random(9)
+    qs, ts, _ = random_batch(77, 300, 60)
+    for cap in (0, 1, 3, 8, 20):
+        compare(px, engine, qs, ts, costs=(2, 4, 1), cap=cap, label=f"cap{cap}")
+
+
+def test_long_pairs_escalate_tiers(px):
+    """Long and divergent pairs overflow the one-wave tier and are re-run on
+    the wider tiers; results stay bit-exact."""
+    rng = random.Random(1234)
+    qs, ts = [], []
+    for L in (300, 700, 1500, 2500, 4000, 6000):
+        for rate in (0.01, 0.08):
+            q = rand_seq(rng, L)
+            qs.append(q)
+            ts.append(mutate(rng, q, rate) + rand_seq(rng, L // 10))
+    for eng in ("affine", "linear"):
+        compare(px, eng, qs, ts, costs=(4, 6, 2), label="long")
+
+
+def test_tiny_tiers_force_every_escalation(px):
+    """A context with deliberately small tiers makes most pairs move through
+    several tiers; results must not depend on the route."""
+    ctx = px.Context(0, dict(wave_wcap=16, wave_hist_kib=1, wave_slots=64, block_wcap=64,
+                             block_hist_kib=8, block_slots=16, hbm_budget_mib=2048))
+    qs, ts, _ = random_batch(4321, 200, 400)
+    for eng in ("affine", "linear"):
+        compare(px, eng, qs, ts, costs=(4, 6, 2), ctx=ctx, label="tiny-tiers")
+    ctx.close()
+
+
+def test_short150_full_batch(px):
+    """The 150 bp / 170 bp affine workload (4/6/2), all 10,000 pairs."""
+    from phraya_amd import synth
+    w = synth.workload("short150")
+    qs = [w.pair(i)[0] for i in range(w.n)]
+    ts = [w.pair(i)[1] for i in range(w.n)]
+    compare(px, "affine", qs, ts, costs=(4, 6, 2), label="short150")
+
+
+def test_myers150_subset(px):
+    from phraya_amd import synth
+    w = synth.workload("myers150", n=50_000)
+    qs = [w.pair(i)[0] for i in range(w.n)]
+    ts = [w.pair(i)[1] for i in range(w.n)]
+    compare(px, "myers_dist", qs, ts, label="myers150")
+    compare(px, "myers", qs[:5000], ts[:5000], label="myers150-cigar")
+
+
+def test_mixed_subset(px):
+    from phraya_amd import synth
+    w = synth.workload("mixed", n=400)
+    idx = [i for i in range(w.n) if w.q_off[i + 1] - w.q_off[i] <= 2500][:120]
+    qs = [w.pair(i)[0] for i in idx]
+    ts = [w.pair(i)[1] for i in idx]
+    compare(px, "affine", qs, ts, costs=(4, 6, 2), label="mixed")
+
+
+def test_deterministic_and_isolated(px):
+    """Same batch twice gives identical bytes; a pair alone equals the same
+    pair inside a mixed batch."""
+    qs, ts, _ = random_batch(999, 200, 200)
+    a = px.align_batch(1, qs, ts, costs=px.AffineCosts(4, 6, 2))
+    b = px.align_batch(1, qs, ts, costs=px.AffineCosts(4, 6, 2))
+    assert np.array_equal(a.aln, b.aln) and np.array_equal(a.runs, b.runs)
+    for i in (0, 17, 123):
+        one = px.wfa_extend_affine(qs[i], ts[i], px.SeedAnchor(), px.AffineCosts(4, 6, 2))
+        assert one == a.alignment(i)
