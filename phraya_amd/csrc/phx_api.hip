@@ -369,7 +369,7 @@ phx_ctx* phx_ctx_create(int device, uint32_t flags, const phx_tuning* tuning) {
     return nullptr;
   }
   phx_tuning d{};
-  d.wave_wcap = 64;
+  d.wave_wcap = 128;
   d.wave_hist_kib = 32;
   d.wave_slots = (uint32_t)c->cu_count * 16;
   d.block_wcap = 1024;
