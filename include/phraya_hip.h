@@ -101,7 +101,7 @@ typedef struct phx_plan phx_plan;
 /* Tuning knobs; NULL selects the built-in defaults. */
 typedef struct phx_tuning {
   uint32_t wave_wcap;        /* ring width (diagonals) of the one-wave tier; power of two */
-  uint32_t wave_hist_kib;    /* history per workgroup slot, one-wave tier */
+  uint32_t wave_hist_kib;    /* LDS bytes (KiB) per wave of the wave-resident tier (ring + sequences + history) */
   uint32_t wave_slots;       /* workgroups of the one-wave tier */
   uint32_t block_wcap;       /* ring width of the 256-thread LDS tier */
   uint32_t block_hist_kib;

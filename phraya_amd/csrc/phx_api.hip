@@ -23,6 +23,8 @@ namespace phx {
 hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t st);
 hipError_t launch_wfa(int mode, int variant, const WfaArgs& a, uint32_t grid, size_t lds, hipStream_t st);
 hipError_t launch_myers(int nb_template, const MyersArgs& a, hipStream_t st);
+hipError_t launch_wfa_wave(int mode, const WfaArgs& a, const WaveGeom& G, uint32_t grid,
+                           hipStream_t st);
 }  // namespace phx
 
 using namespace phx;
@@ -32,6 +34,10 @@ namespace {
 constexpr int kRingSlotsMax = 128;  // WfaShared metadata capacity
 constexpr size_t kRingLdsMax = 56 * 1024;  // dynamic LDS ring per workgroup (plus ~3 KB static)
 constexpr int32_t kMaxLen = (1 << 27) - 2;  // history cells hold offset + 1 in 28 bits
+constexpr uint64_t kWaveMaxLen = 2048;     // qn + tn limit of the wave-resident tier
+constexpr int kWaveRingMax = 64;           // ring slots the wave tier keeps in lane registers
+constexpr uint32_t kWaveSteps = 128;       // step-table entries per wave
+constexpr int kWaveWaves = 4;              // waves per workgroup of the wave tier
 
 struct DevBuf {
   void* p = nullptr;
@@ -124,7 +130,9 @@ struct phx_plan {
   uint64_t seq_words = 0, runs_total = 0;
   uint64_t input_bytes = 0;                            // packed algorithmic input bytes
   int32_t ring_slots = 2;
-  DevBuf d_seq, d_pairs, d_out, d_runs, d_work, d_ctr, d_hist, d_ring, d_scratch;
+  DevBuf d_seq, d_pairs, d_out, d_runs, d_work, d_work2, d_ctr, d_scratch;
+  DevBuf d_hist[8], d_ring[8];  // per tier, so tiers in flight never share scratch
+  uint32_t* ctr_host = nullptr;  // pinned readback of the tier counters
   std::vector<std::vector<uint32_t>> work;             // initial work per tier / Myers group
   std::vector<PairOut> out_h;
   bool out_valid = false;
@@ -236,11 +244,11 @@ int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint3
   Tier T = make_tier(P, t, max_sum, (uint32_t)work.size());
   const int layers = P->engine == PHX_ENGINE_AFFINE ? 3 : 1;
   const int R = P->ring_slots;
-  if (dev_alloc(ctx, P->d_hist, (size_t)T.grid * T.hist_stride) != PHX_OK) return PHX_E_NOMEM;
+  if (t > 0 && dev_alloc(ctx, P->d_hist[t], (size_t)T.grid * T.hist_stride) != PHX_OK) return PHX_E_NOMEM;
   uint64_t ring_stride = 0;
   if (T.variant >= 2) {
     ring_stride = (uint64_t)layers * R * T.wcap;
-    if (dev_alloc(ctx, P->d_ring, (size_t)T.grid * ring_stride * 4) != PHX_OK) return PHX_E_NOMEM;
+    if (dev_alloc(ctx, P->d_ring[t], (size_t)T.grid * ring_stride * 4) != PHX_OK) return PHX_E_NOMEM;
   }
   WfaArgs a{};
   a.seq = (const uint64_t*)P->d_seq.p;
@@ -252,14 +260,47 @@ int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint3
                     P->cap < 0 ? -1 : (int32_t)std::min<int64_t>(P->cap, INT32_MAX)};
   a.out = (PairOut*)P->d_out.p;
   a.runs = (uint32_t*)P->d_runs.p;
-  a.hist = (uint8_t*)P->d_hist.p;
+  a.hist = (uint8_t*)P->d_hist[t].p;
   a.hist_stride = T.hist_stride;
-  a.ring_g = (int32_t*)P->d_ring.p;
+  a.ring_g = (int32_t*)P->d_ring[t].p;
   a.ring_stride = ring_stride;
   a.wcap_log2 = log2u(T.wcap);
   const int mode = P->engine == PHX_ENGINE_AFFINE ? ENG_AFFINE : ENG_LINEAR;
   hipEvent_t e0, e1;
   if (kernel_events(ctx, P->last_launches, &e0, &e1) != PHX_OK) return PHX_E_DEVICE;
+  if (t == 0) {
+    // Wave-resident tier: one pair per wave, everything in LDS.
+    uint32_t seq_words = 2;
+    for (uint32_t i : work) {
+      const PairDesc& pd = P->pairs[i];
+      const bool b8 = pd.flags & PF_BYTES;
+      const uint32_t w = (uint32_t)(words_for((size_t)pd.qn, b8) + words_for((size_t)pd.tn, b8));
+      seq_words = std::max(seq_words, w);
+    }
+    auto al = [](uint32_t v) { return (v + 15u) & ~15u; };
+    WaveGeom G{};
+    G.ring_off = 0;
+    const uint32_t ring_bytes = (uint32_t)(P->engine == PHX_ENGINE_AFFINE ? 3 * R : 2) * 128u * 2u;
+    G.seq_off = al(ring_bytes);
+    G.seq_cap_words = seq_words;
+    G.tab_off = G.seq_off + al(seq_words * 8);
+    G.steps_cap = kWaveSteps;
+    G.cell_off = G.tab_off + al(2 * G.steps_cap);
+    // History cells fill the wave's share of the LDS budget (wave_hist_kib
+    // per wave, default sized so four 4-wave workgroups fit a CU's 160 KB).
+    const uint32_t budget = std::max<uint32_t>(ctx->tun.wave_hist_kib * 1024u,
+                                               std::min<uint32_t>(G.cell_off + 4096u, 15u * 1024u));
+    const uint32_t cell_bytes = budget > G.cell_off + 256 ? (budget - G.cell_off) & ~15u : 256u;
+    G.cells_cap = std::min<uint32_t>(65535, cell_bytes / 2);
+    G.per_wave_bytes = G.cell_off + al(2 * G.cells_cap);
+    const uint32_t wgs = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)((work.size() + kWaveWaves - 1) / kWaveWaves),
+                                                                  std::max<uint32_t>(1, ctx->tun.wave_slots / kWaveWaves)));
+    HIPCHK(hipEventRecord(e0, ctx->stream));
+    HIPCHK(launch_wfa_wave(mode, a, G, wgs, ctx->stream));
+    HIPCHK(hipEventRecord(e1, ctx->stream));
+    ++P->last_launches;
+    return PHX_OK;
+  }
   HIPCHK(hipEventRecord(e0, ctx->stream));
   HIPCHK(launch_wfa(mode, T.variant, a, T.grid, T.lds, ctx->stream));
   HIPCHK(hipEventRecord(e1, ctx->stream));
@@ -267,34 +308,57 @@ int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint3
   return PHX_OK;
 }
 
+// Initial tier lists live in d_work (uploaded once); lists of pairs that a
+// tier could not hold are staged per run in d_work2.  Counters: two words per
+// tier launch (queue head, overflow count).
 int run_wfa(phx_plan* P) {
   phx_ctx* ctx = P->ctx;
   const int ntiers = 6;
-  std::vector<std::vector<uint32_t>> lists = P->work;
-  lists.resize(ntiers);
-  uint32_t* d_list = (uint32_t*)P->d_work.p;
   uint32_t* d_ctr = (uint32_t*)P->d_ctr.p;
-  uint32_t ctr_h[2] = {0, 0};
-  for (int t = 0; t < ntiers; ++t) {
-    if (lists[t].empty()) continue;
-    HIPCHK(hipMemcpyAsync(d_list, lists[t].data(), lists[t].size() * 4, hipMemcpyHostToDevice,
-                          ctx->stream));
+  uint32_t* ctr_h = P->ctr_host;
+  HIPCHK(launch_zero(d_ctr, 2 * ntiers, ctx->stream));
+  std::vector<int> launched;
+  uint64_t off = 0;
+  for (int t = 0; t < (int)P->work.size(); ++t) {
+    const auto& wl = P->work[t];
+    if (wl.empty()) continue;
+    int rc = launch_wfa_tier(P, t, wl, (uint32_t*)P->d_work.p + off, d_ctr + 2 * t);
+    if (rc != PHX_OK) return rc;
+    launched.push_back(t);
+    off += wl.size();
+  }
+  if (launched.empty()) return PHX_OK;
+  HIPCHK(hipMemcpyAsync(ctr_h, d_ctr, 8 * ntiers, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  bool any = false;
+  for (int t : launched) any |= ctr_h[2 * t + 1] != 0;
+  if (!any) return PHX_OK;
+  // Re-route pairs that overflowed their tier (rare path).
+  std::vector<std::vector<uint32_t>> lists(ntiers);
+  std::vector<PairOut> outs(P->n);
+  HIPCHK(hipMemcpy(outs.data(), P->d_out.p, sizeof(PairOut) * P->n, hipMemcpyDeviceToHost));
+  auto collect = [&](int t, const std::vector<uint32_t>& from) {
+    if (t + 1 >= ntiers) return;
+    for (uint32_t i : from)
+      if (outs[i].status >= ST_OVERFLOW_HIST && outs[i].status <= ST_OVERFLOW_RUNS) lists[t + 1].push_back(i);
+  };
+  for (int t : launched) collect(t, P->work[t]);
+  for (int t = 1; t < ntiers; ++t) {
+    auto& wl = lists[t];
+    if (wl.empty()) continue;
+    std::stable_sort(wl.begin(), wl.end(), [&](uint32_t a, uint32_t b) {
+      return (uint64_t)P->pairs[a].qn + P->pairs[a].tn > (uint64_t)P->pairs[b].qn + P->pairs[b].tn;
+    });
+    if (dev_alloc(ctx, P->d_work2, wl.size() * 4) != PHX_OK) return PHX_E_NOMEM;
+    HIPCHK(hipMemcpy(P->d_work2.p, wl.data(), wl.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(launch_zero(d_ctr, 2, ctx->stream));
-    int rc = launch_wfa_tier(P, t, lists[t], d_list, d_ctr);
+    int rc = launch_wfa_tier(P, t, wl, (uint32_t*)P->d_work2.p, d_ctr);
     if (rc != PHX_OK) return rc;
     HIPCHK(hipMemcpyAsync(ctr_h, d_ctr, 8, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (ctr_h[1] == 0) continue;
-    // Some pairs did not fit this tier's slots: move them to the next tier.
-    std::vector<PairOut> outs(P->n);
     HIPCHK(hipMemcpy(outs.data(), P->d_out.p, sizeof(PairOut) * P->n, hipMemcpyDeviceToHost));
-    if (t + 1 >= ntiers) break;
-    auto& nl = lists[t + 1];
-    for (uint32_t i : lists[t])
-      if (outs[i].status >= ST_OVERFLOW_HIST && outs[i].status <= ST_OVERFLOW_RUNS) nl.push_back(i);
-    std::stable_sort(nl.begin(), nl.end(), [&](uint32_t a, uint32_t b) {
-      return (uint64_t)P->pairs[a].qn + P->pairs[a].tn > (uint64_t)P->pairs[b].qn + P->pairs[b].tn;
-    });
+    collect(t, wl);
   }
   return PHX_OK;
 }
@@ -370,7 +434,7 @@ phx_ctx* phx_ctx_create(int device, uint32_t flags, const phx_tuning* tuning) {
   }
   phx_tuning d{};
   d.wave_wcap = 128;
-  d.wave_hist_kib = 32;
+  d.wave_hist_kib = 9;  // per-wave LDS budget of the wave tier: 4 x 4 waves x 9 KiB <= 160 KiB
   d.wave_slots = (uint32_t)c->cu_count * 16;
   d.block_wcap = 1024;
   d.block_hist_kib = 4096;
@@ -505,7 +569,7 @@ phx_plan* phx_plan_create(phx_ctx* ctx, const phx_batch* b) {
     P->work.assign(4, {});
     for (uint32_t i : order) {
       const uint64_t L = P->qlen[i] + P->tlen[i];
-      const int t = L <= 2048 ? 0 : (L <= 16384 ? 1 : 2);
+      const int t = (L <= kWaveMaxLen && P->ring_slots <= kWaveRingMax) ? 0 : (L <= 16384 ? 1 : 2);
       P->work[t].push_back(i);
     }
   } else {
@@ -527,12 +591,14 @@ phx_plan* phx_plan_create(phx_ctx* ctx, const phx_batch* b) {
   if (dev_alloc(ctx, P->d_out, sizeof(PairOut) * (size_t)std::max<uint32_t>(n, 1)) != PHX_OK) return fail_plan("out");
   if (dev_alloc(ctx, P->d_runs, runs * 4 + 4) != PHX_OK) return fail_plan("runs");
   if (dev_alloc(ctx, P->d_work, (size_t)std::max<uint32_t>(n, 1) * 4) != PHX_OK) return fail_plan("work");
-  if (dev_alloc(ctx, P->d_ctr, 64) != PHX_OK) return fail_plan("counters");
+  if (dev_alloc(ctx, P->d_ctr, 256) != PHX_OK) return fail_plan("counters");
   if (hipMemcpy(P->d_seq.p, packed.data(), words * 8, hipMemcpyHostToDevice) != hipSuccess ||
       (n && hipMemcpy(P->d_pairs.p, P->pairs.data(), sizeof(PairDesc) * n, hipMemcpyHostToDevice) != hipSuccess))
     return fail_plan("upload");
-  if (!is_wfa(b->engine)) {
-    // Myers work lists are static: upload once, grouped.
+  if (hipHostMalloc((void**)&P->ctr_host, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
+    return fail_plan("pinned counters");
+  {
+    // Work lists are static: upload once, grouped by tier / block count.
     std::vector<uint32_t> flat;
     for (auto& w : P->work) flat.insert(flat.end(), w.begin(), w.end());
     if (!flat.empty() &&
@@ -653,9 +719,14 @@ int phx_plan_results(phx_plan* P, phx_results* r) {
 void phx_plan_destroy(phx_plan* P) {
   if (!P) return;
   (void)hipStreamSynchronize(P->ctx->stream);
-  for (DevBuf* b : {&P->d_seq, &P->d_pairs, &P->d_out, &P->d_runs, &P->d_work, &P->d_ctr, &P->d_hist,
-                    &P->d_ring, &P->d_scratch})
+  for (DevBuf* b : {&P->d_seq, &P->d_pairs, &P->d_out, &P->d_runs, &P->d_work, &P->d_work2, &P->d_ctr,
+                    &P->d_scratch})
     dev_free(*b);
+  for (int t = 0; t < 8; ++t) {
+    dev_free(P->d_hist[t]);
+    dev_free(P->d_ring[t]);
+  }
+  if (P->ctr_host) (void)hipHostFree(P->ctr_host);
   delete P;
 }
 

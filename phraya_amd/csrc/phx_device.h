@@ -86,6 +86,19 @@ struct WfaArgs {
   uint32_t wcap_log2;    // ring width in diagonals = 1 << wcap_log2
 };
 
+// LDS layout of one wave of the wave-resident WFA tier (byte offsets inside
+// the wave's region; the workgroup's dynamic LDS holds WAVES regions).
+struct WaveGeom {
+  uint32_t per_wave_bytes;
+  uint32_t ring_off;       // int16 ring [layers][R][128]
+  uint32_t seq_off;        // u64 packed query then target
+  uint32_t tab_off;        // u16 first-cell offset of each step [steps_cap]
+  uint32_t cell_off;       // u16 history cells [cells_cap]
+  uint32_t seq_cap_words;
+  uint32_t steps_cap;
+  uint32_t cells_cap;
+};
+
 // Kernel arguments of one Myers launch.
 struct MyersArgs {
   const uint64_t* seq;

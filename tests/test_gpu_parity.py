@@ -155,7 +155,7 @@ def test_random_small(px, engine):
     compare(px, engine, qs, ts, ss, costs=(4, 6, 2), label="small")
 
 
-@pytest.mark.parametrize("costs", [(2, 4, 1), (4, 6, 2), (1, 0, 1), (3, 5, 2), (1, 30, 1)])
+@pytest.mark.parametrize("costs", [(2, 4, 1), (4, 6, 2), (1, 0, 1), (3, 5, 2), (1, 30, 1), (1, 70, 1)])
 def test_affine_cost_sets(px, costs):
     qs, ts, _ = random_batch(7 + costs[0] * 10 + costs[1], 300, 90)
     compare(px, "affine", qs, ts, costs=costs, label="costs")
