@@ -98,15 +98,13 @@ typedef struct phx_results {
 typedef struct phx_ctx phx_ctx;
 typedef struct phx_plan phx_plan;
 
-/* Tuning knobs; NULL selects the built-in defaults. */
+/* Tuning knobs; NULL (or a zero field) selects the built-in default. */
 typedef struct phx_tuning {
-  uint32_t wave_wcap;        /* ring width (diagonals) of the one-wave tier; power of two */
-  uint32_t wave_hist_kib;    /* LDS bytes (KiB) per wave of the wave-resident tier (ring + sequences + history) */
-  uint32_t wave_slots;       /* workgroups of the one-wave tier */
-  uint32_t block_wcap;       /* ring width of the 256-thread LDS tier */
-  uint32_t block_hist_kib;
-  uint32_t block_slots;
-  uint64_t hbm_budget_mib;   /* device scratch budget for the HBM tiers */
+  uint32_t wave_lds_kib;     /* LDS tier: LDS bytes (KiB) per wave (ring + sequences + history) */
+  uint32_t wave_slots;       /* LDS tier: pairs in flight */
+  uint32_t hbm_slots;        /* HBM tiers: pairs in flight */
+  uint32_t hbm_slot_mib;     /* HBM tier 1: history slot per pair (MiB) */
+  uint64_t hbm_budget_mib;   /* device scratch budget of the HBM tiers (MiB) */
 } phx_tuning;
 
 /* Context: owns the stream, device buffers and the submission lock.

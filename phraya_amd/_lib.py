@@ -71,9 +71,8 @@ class CResults(ctypes.Structure):
 
 
 class CTuning(ctypes.Structure):
-    _fields_ = [("wave_wcap", ctypes.c_uint32), ("wave_hist_kib", ctypes.c_uint32),
-                ("wave_slots", ctypes.c_uint32), ("block_wcap", ctypes.c_uint32),
-                ("block_hist_kib", ctypes.c_uint32), ("block_slots", ctypes.c_uint32),
+    _fields_ = [("wave_lds_kib", ctypes.c_uint32), ("wave_slots", ctypes.c_uint32),
+                ("hbm_slots", ctypes.c_uint32), ("hbm_slot_mib", ctypes.c_uint32),
                 ("hbm_budget_mib", ctypes.c_uint64)]
 
 

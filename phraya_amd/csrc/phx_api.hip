@@ -21,10 +21,10 @@
 
 namespace phx {
 hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t st);
-hipError_t launch_wfa(int mode, int variant, const WfaArgs& a, uint32_t grid, size_t lds, hipStream_t st);
 hipError_t launch_myers(int nb_template, const MyersArgs& a, hipStream_t st);
-hipError_t launch_wfa_wave(int mode, const WfaArgs& a, const WaveGeom& G, uint32_t grid,
-                           hipStream_t st);
+hipError_t launch_wfa_wave(int mode, bool lds, const WfaArgs& a, const WaveGeom& G,
+                           uint32_t grid, hipStream_t st);
+hipError_t launch_wfa_block(int mode, int nt, const WfaArgs& a, uint32_t grid, hipStream_t st);
 }  // namespace phx
 
 using namespace phx;
@@ -32,7 +32,6 @@ using namespace phx;
 namespace {
 
 constexpr int kRingSlotsMax = 128;  // WfaShared metadata capacity
-constexpr size_t kRingLdsMax = 56 * 1024;  // dynamic LDS ring per workgroup (plus ~3 KB static)
 constexpr int32_t kMaxLen = (1 << 27) - 2;  // history cells hold offset + 1 in 28 bits
 constexpr uint64_t kWaveMaxLen = 2048;     // qn + tn limit of the wave-resident tier
 constexpr int kWaveRingMax = 64;           // ring slots the wave tier keeps in lane registers
@@ -98,23 +97,19 @@ int32_t gcd3(int32_t a, int32_t b, int32_t c) {
 
 }  // namespace
 
+constexpr int kStreams = 4;  // launch groups of one run overlap on these
+
 struct phx_ctx {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;         // = streams[0]
+  hipStream_t streams[kStreams] = {};
+  hipEvent_t join[32] = {};             // cross-stream joins
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<hipEvent_t> kev;  // pairs of events around each kernel launch of a run
   std::mutex mu;
   phx_tuning tun{};
   std::string err;
   int cu_count = 256;
-};
-
-struct Tier {
-  int variant;
-  uint32_t wcap;
-  uint64_t hist_stride;
-  uint32_t grid;
-  size_t lds;
 };
 
 struct phx_plan {
@@ -131,9 +126,20 @@ struct phx_plan {
   uint64_t input_bytes = 0;                            // packed algorithmic input bytes
   int32_t ring_slots = 2;
   DevBuf d_seq, d_pairs, d_out, d_runs, d_work, d_work2, d_ctr, d_scratch;
-  DevBuf d_hist[8], d_ring[8];  // per tier, so tiers in flight never share scratch
+  DevBuf d_hist[16], d_ring[16];  // per launch group, so groups in flight never share scratch
   uint32_t* ctr_host = nullptr;  // pinned readback of the tier counters
-  std::vector<std::vector<uint32_t>> work;             // initial work per tier / Myers group
+  std::vector<std::vector<uint32_t>> work;             // initial work per launch group / Myers group
+  std::vector<int> group_tier;                         // WFA: 0 = LDS tier, 1 = HBM tier
+  std::vector<uint64_t> group_slot;                    // WFA: HBM slot bytes of each group
+  std::vector<double> group_share;                     // WFA: share of the HBM budget
+  uint64_t group_budget(int buf) const {
+    const uint64_t total = ctx->tun.hbm_budget_mib << 20;
+    if (buf >= 0 && buf < (int)group_share.size()) return (uint64_t)(total * group_share[buf]);
+    // Re-runs get what the initial groups leave of the budget.
+    uint64_t held = 0;
+    for (int i = 0; i < 15; ++i) held += d_hist[i].bytes + d_ring[i].bytes;
+    return total > held + (256ull << 20) ? total - held : (256ull << 20);
+  }
   std::vector<PairOut> out_h;
   bool out_valid = false;
   double last_ms = 0;
@@ -190,66 +196,48 @@ int kernel_events(phx_ctx* ctx, uint32_t i, hipEvent_t* a, hipEvent_t* b) {
 bool is_wfa(int engine) { return engine == PHX_ENGINE_LINEAR || engine == PHX_ENGINE_AFFINE; }
 bool has_cigar(int engine) { return engine != PHX_ENGINE_MYERS_DIST; }
 
-// Tier geometry for a WFA launch.  Tier 0: one wave per pair, ring in LDS;
-// tier 1: 256 threads, ring in LDS; tiers 2-3: ring and history in HBM.
-Tier make_tier(const phx_plan* P, int t, uint64_t max_len_sum, uint32_t nwork) {
-  const phx_tuning& tu = P->ctx->tun;
-  const int R = P->ring_slots;
-  const int layers = P->engine == PHX_ENGINE_AFFINE ? 3 : 1;
-  Tier T{};
-  const uint64_t budget = tu.hbm_budget_mib << 20;
-  switch (t) {
-    case 0: {
-      T.variant = 0;
-      T.wcap = tu.wave_wcap;
-      while (T.wcap > 8 && (size_t)layers * R * T.wcap * 4 > kRingLdsMax) T.wcap >>= 1;
-      T.hist_stride = (uint64_t)tu.wave_hist_kib << 10;
-      T.grid = std::min<uint32_t>(nwork, tu.wave_slots);
-      T.lds = (size_t)layers * R * T.wcap * 4;
-      break;
-    }
-    case 1: {
-      T.variant = 1;
-      uint32_t wcap = tu.block_wcap;
-      while (wcap > 8 && (size_t)layers * R * wcap * 4 > kRingLdsMax) wcap >>= 1;
-      T.wcap = wcap;
-      T.hist_stride = (uint64_t)tu.block_hist_kib << 10;
-      T.grid = std::min<uint32_t>(nwork, tu.block_slots);
-      T.lds = (size_t)layers * R * T.wcap * 4;
-      break;
-    }
-    default: {
-      T.variant = t == 2 ? 2 : 3;
-      T.wcap = next_pow2(max_len_sum + 1);
-      const uint64_t ring_bytes = (uint64_t)layers * R * T.wcap * 4;
-      T.hist_stride = t == 2 ? (64ull << 20) : (1ull << 30) << (t - 3);
-      if (T.hist_stride > budget / 2) T.hist_stride = budget / 2;
-      const uint64_t per = T.hist_stride + ring_bytes;
-      const uint32_t max_slots = t == 2 ? 1024 : 128;
-      uint64_t g = std::max<uint64_t>(1, budget / per);
-      T.grid = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(g, max_slots), nwork);
-      T.lds = 0;
-      break;
-    }
-  }
-  if (T.grid == 0) T.grid = 1;
-  return T;
+// WFA tiers.  Tier 0: wave per pair, everything in LDS (short pairs).
+// Tier 1: wave per pair, ring and history in an HBM slot of fixed size.
+// Tier 2: wave per pair, HBM slots sized exactly from the step count the pair
+// reported when it overflowed tier 1.
+
+// sum_{u=1..S} min(u, a)
+uint64_t sum_min(uint64_t S, uint64_t a) {
+  return S <= a ? S * (S + 1) / 2 : a * (a + 1) / 2 + (S - a) * a;
 }
 
+// History words a pair needs for `steps` score steps: cells of every step
+// (width min(u, qn) + min(u, tn) + 1) plus one offset word per step.
+uint64_t exact_hist_words(int32_t steps, int32_t qn, int32_t tn) {
+  const uint64_t S = (uint64_t)std::max(steps, 0);
+  return sum_min(S, (uint64_t)qn) + sum_min(S, (uint64_t)tn) + 2 * S + 16;
+}
+
+// Generous estimate of the score steps of a pair, used to size HBM history
+// slots before the run: an end gap for the length difference plus 9% events
+// per base (40% mismatches, 60% indels of mean length 2), +15%.
+int32_t estimate_steps(const phx_plan* P, int32_t qn, int32_t tn) {
+  const double m = std::max(qn, tn), d = std::abs(tn - qn);
+  double steps;
+  if (P->engine == PHX_ENGINE_AFFINE) {
+    const phx_costs& c = P->costs;
+    const int32_t g = gcd3(c.mismatch, c.gap_open, c.gap_extend);
+    const double score = c.gap_open + c.gap_extend * d +
+                         0.09 * m * (0.4 * c.mismatch + 0.6 * (c.gap_open + 2.0 * c.gap_extend));
+    steps = score / g;
+  } else {
+    steps = d + 0.09 * m;
+  }
+  return (int32_t)std::min(1.15 * steps + 16.0, (double)(qn + tn + 16));
+}
+
+// Launches one group: tier t (0 LDS, 1 HBM) over `work`, scratch buffers of
+// slot `buf`, on stream `st`.
 int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint32_t* d_list,
-                    uint32_t* d_counters) {
+                    uint32_t* d_counters, uint64_t slot_bytes, int buf, hipStream_t st) {
   phx_ctx* ctx = P->ctx;
-  uint64_t max_sum = 1;
-  for (uint32_t i : work) max_sum = std::max<uint64_t>(max_sum, (uint64_t)P->pairs[i].qn + P->pairs[i].tn);
-  Tier T = make_tier(P, t, max_sum, (uint32_t)work.size());
   const int layers = P->engine == PHX_ENGINE_AFFINE ? 3 : 1;
   const int R = P->ring_slots;
-  if (t > 0 && dev_alloc(ctx, P->d_hist[t], (size_t)T.grid * T.hist_stride) != PHX_OK) return PHX_E_NOMEM;
-  uint64_t ring_stride = 0;
-  if (T.variant >= 2) {
-    ring_stride = (uint64_t)layers * R * T.wcap;
-    if (dev_alloc(ctx, P->d_ring[t], (size_t)T.grid * ring_stride * 4) != PHX_OK) return PHX_E_NOMEM;
-  }
   WfaArgs a{};
   a.seq = (const uint64_t*)P->d_seq.p;
   a.pairs = (const PairDesc*)P->d_pairs.p;
@@ -260,25 +248,17 @@ int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint3
                     P->cap < 0 ? -1 : (int32_t)std::min<int64_t>(P->cap, INT32_MAX)};
   a.out = (PairOut*)P->d_out.p;
   a.runs = (uint32_t*)P->d_runs.p;
-  a.hist = (uint8_t*)P->d_hist[t].p;
-  a.hist_stride = T.hist_stride;
-  a.ring_g = (int32_t*)P->d_ring[t].p;
-  a.ring_stride = ring_stride;
-  a.wcap_log2 = log2u(T.wcap);
   const int mode = P->engine == PHX_ENGINE_AFFINE ? ENG_AFFINE : ENG_LINEAR;
-  hipEvent_t e0, e1;
-  if (kernel_events(ctx, P->last_launches, &e0, &e1) != PHX_OK) return PHX_E_DEVICE;
+  WaveGeom G{};
+  uint32_t waves = 0;
   if (t == 0) {
-    // Wave-resident tier: one pair per wave, everything in LDS.
     uint32_t seq_words = 2;
     for (uint32_t i : work) {
       const PairDesc& pd = P->pairs[i];
       const bool b8 = pd.flags & PF_BYTES;
-      const uint32_t w = (uint32_t)(words_for((size_t)pd.qn, b8) + words_for((size_t)pd.tn, b8));
-      seq_words = std::max(seq_words, w);
+      seq_words = std::max(seq_words, (uint32_t)(words_for((size_t)pd.qn, b8) + words_for((size_t)pd.tn, b8)));
     }
     auto al = [](uint32_t v) { return (v + 15u) & ~15u; };
-    WaveGeom G{};
     G.ring_off = 0;
     const uint32_t ring_bytes = (uint32_t)(P->engine == PHX_ENGINE_AFFINE ? 3 * R : 2) * 128u * 2u;
     G.seq_off = al(ring_bytes);
@@ -286,79 +266,134 @@ int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint3
     G.tab_off = G.seq_off + al(seq_words * 8);
     G.steps_cap = kWaveSteps;
     G.cell_off = G.tab_off + al(2 * G.steps_cap);
-    // History cells fill the wave's share of the LDS budget (wave_hist_kib
-    // per wave, default sized so four 4-wave workgroups fit a CU's 160 KB).
-    const uint32_t budget = std::max<uint32_t>(ctx->tun.wave_hist_kib * 1024u,
+    // History cells fill the wave's share of the LDS budget (wave_lds_kib).
+    const uint32_t budget = std::max<uint32_t>(ctx->tun.wave_lds_kib * 1024u,
                                                std::min<uint32_t>(G.cell_off + 4096u, 15u * 1024u));
     const uint32_t cell_bytes = budget > G.cell_off + 256 ? (budget - G.cell_off) & ~15u : 256u;
     G.cells_cap = std::min<uint32_t>(65535, cell_bytes / 2);
     G.per_wave_bytes = G.cell_off + al(2 * G.cells_cap);
-    const uint32_t wgs = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)((work.size() + kWaveWaves - 1) / kWaveWaves),
-                                                                  std::max<uint32_t>(1, ctx->tun.wave_slots / kWaveWaves)));
-    HIPCHK(hipEventRecord(e0, ctx->stream));
-    HIPCHK(launch_wfa_wave(mode, a, G, wgs, ctx->stream));
-    HIPCHK(hipEventRecord(e1, ctx->stream));
-    ++P->last_launches;
-    return PHX_OK;
+    waves = std::min<uint32_t>((uint32_t)work.size(), std::max<uint32_t>(kWaveWaves, ctx->tun.wave_slots));
+  } else {
+    uint64_t max_sum = 1;
+    for (uint32_t i : work) max_sum = std::max<uint64_t>(max_sum, (uint64_t)P->pairs[i].qn + P->pairs[i].tn);
+    const uint32_t wcap = next_pow2(max_sum + 1);
+    const uint64_t ring_words = (uint64_t)(P->engine == PHX_ENGINE_AFFINE ? 3 * R : 2) * wcap;
+    const uint64_t hist = slot_bytes ? slot_bytes : ((uint64_t)ctx->tun.hbm_slot_mib << 20);
+    const uint64_t per = hist + ring_words * 4;
+    const uint64_t budget = P->group_budget(buf);
+    // Pairs in flight: one per wave (tier 1) or per workgroup (tiers 2-3).
+    const uint64_t max_flight = t == 1 ? ctx->tun.hbm_slots
+                              : t == 2 ? (uint64_t)ctx->cu_count * 4 : (uint64_t)ctx->cu_count;
+    uint64_t slots = std::max<uint64_t>(1, budget / per);
+    slots = std::min<uint64_t>(std::min<uint64_t>(slots, max_flight), work.size());
+    const uint64_t per_wg = t == 1 ? kWaveWaves : 1;
+    const uint64_t wgs = (slots + per_wg - 1) / per_wg;
+    if (dev_alloc(ctx, P->d_hist[buf], (size_t)(wgs * per_wg * hist)) != PHX_OK) return PHX_E_NOMEM;
+    if (dev_alloc(ctx, P->d_ring[buf], (size_t)(wgs * per_wg * ring_words * 4)) != PHX_OK) return PHX_E_NOMEM;
+    a.hist = (uint8_t*)P->d_hist[buf].p;
+    a.hist_stride = hist;
+    a.ring_g = (int32_t*)P->d_ring[buf].p;
+    a.ring_stride = ring_words;
+    a.wcap_log2 = log2u(wcap);
+    waves = (uint32_t)(wgs * per_wg);
   }
-  HIPCHK(hipEventRecord(e0, ctx->stream));
-  HIPCHK(launch_wfa(mode, T.variant, a, T.grid, T.lds, ctx->stream));
-  HIPCHK(hipEventRecord(e1, ctx->stream));
+  (void)layers;
+  hipEvent_t e0, e1;
+  if (kernel_events(ctx, P->last_launches, &e0, &e1) != PHX_OK) return PHX_E_DEVICE;
+  HIPCHK(hipEventRecord(e0, st));
+  if (t <= 1) {
+    const uint32_t wgs = std::max<uint32_t>(1, (waves + kWaveWaves - 1) / kWaveWaves);
+    HIPCHK(launch_wfa_wave(mode, t == 0, a, G, wgs, st));
+  } else {
+    HIPCHK(launch_wfa_block(mode, t == 2 ? 256 : 1024, a, std::max<uint32_t>(1, waves), st));
+  }
+  HIPCHK(hipEventRecord(e1, st));
   ++P->last_launches;
   return PHX_OK;
 }
 
-// Initial tier lists live in d_work (uploaded once); lists of pairs that a
-// tier could not hold are staged per run in d_work2.  Counters: two words per
-// tier launch (queue head, overflow count).
+// Initial group lists live in d_work (uploaded once); lists of pairs that a
+// group could not hold are staged per run in d_work2.  Counters: two words per
+// launch (queue head, overflow count).  Groups run concurrently on the
+// context's streams and join on the main stream.
 int run_wfa(phx_plan* P) {
   phx_ctx* ctx = P->ctx;
-  const int ntiers = 6;
+  const int ngroups = (int)P->work.size();
   uint32_t* d_ctr = (uint32_t*)P->d_ctr.p;
   uint32_t* ctr_h = P->ctr_host;
-  HIPCHK(launch_zero(d_ctr, 2 * ntiers, ctx->stream));
+  HIPCHK(launch_zero(d_ctr, 2 * (uint32_t)ngroups, ctx->stream));
+  HIPCHK(hipEventRecord(ctx->join[0], ctx->stream));
   std::vector<int> launched;
   uint64_t off = 0;
-  for (int t = 0; t < (int)P->work.size(); ++t) {
-    const auto& wl = P->work[t];
+  bool used_stream[kStreams] = {true, false, false, false};
+  for (int gi = 0; gi < ngroups; ++gi) {
+    const auto& wl = P->work[gi];
     if (wl.empty()) continue;
-    int rc = launch_wfa_tier(P, t, wl, (uint32_t*)P->d_work.p + off, d_ctr + 2 * t);
+    const int si = (int)launched.size() % kStreams;
+    hipStream_t st = ctx->streams[si];
+    if (si != 0 && !used_stream[si]) {
+      HIPCHK(hipStreamWaitEvent(st, ctx->join[0], 0));
+      used_stream[si] = true;
+    }
+    int rc = launch_wfa_tier(P, P->group_tier[gi], wl, (uint32_t*)P->d_work.p + off, d_ctr + 2 * gi,
+                             P->group_slot[gi], gi, st);
     if (rc != PHX_OK) return rc;
-    launched.push_back(t);
+    launched.push_back(gi);
     off += wl.size();
   }
   if (launched.empty()) return PHX_OK;
-  HIPCHK(hipMemcpyAsync(ctr_h, d_ctr, 8 * ntiers, hipMemcpyDeviceToHost, ctx->stream));
+  for (int si = 1; si < kStreams; ++si) {
+    if (!used_stream[si]) continue;
+    HIPCHK(hipEventRecord(ctx->join[si], ctx->streams[si]));
+    HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->join[si], 0));
+  }
+  HIPCHK(hipMemcpyAsync(ctr_h, d_ctr, 8 * (size_t)ngroups, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   bool any = false;
-  for (int t : launched) any |= ctr_h[2 * t + 1] != 0;
+  for (int gi : launched) any |= ctr_h[2 * gi + 1] != 0;
   if (!any) return PHX_OK;
-  // Re-route pairs that overflowed their tier (rare path).
-  std::vector<std::vector<uint32_t>> lists(ntiers);
+  // Re-route pairs that overflowed their group (rare path): LDS-tier pairs go
+  // to an HBM slot sized by estimate, HBM pairs to a slot sized exactly from
+  // the step count they reported.
   std::vector<PairOut> outs(P->n);
   HIPCHK(hipMemcpy(outs.data(), P->d_out.p, sizeof(PairOut) * P->n, hipMemcpyDeviceToHost));
-  auto collect = [&](int t, const std::vector<uint32_t>& from) {
-    if (t + 1 >= ntiers) return;
-    for (uint32_t i : from)
-      if (outs[i].status >= ST_OVERFLOW_HIST && outs[i].status <= ST_OVERFLOW_RUNS) lists[t + 1].push_back(i);
+  auto overflowed = [&](uint32_t i) {
+    return outs[i].status >= ST_OVERFLOW_HIST && outs[i].status <= ST_OVERFLOW_RUNS;
   };
-  for (int t : launched) collect(t, P->work[t]);
-  for (int t = 1; t < ntiers; ++t) {
-    auto& wl = lists[t];
-    if (wl.empty()) continue;
+  std::vector<uint32_t> from_lds, exact;
+  for (int gi : launched)
+    for (uint32_t i : P->work[gi])
+      if (overflowed(i)) (P->group_tier[gi] == 0 ? from_lds : exact).push_back(i);
+  for (int round = 0; round < 8 && (!from_lds.empty() || !exact.empty()); ++round) {
+    std::vector<uint32_t>& wl = !from_lds.empty() ? from_lds : exact;
+    const bool est = &wl == &from_lds;
     std::stable_sort(wl.begin(), wl.end(), [&](uint32_t a, uint32_t b) {
       return (uint64_t)P->pairs[a].qn + P->pairs[a].tn > (uint64_t)P->pairs[b].qn + P->pairs[b].tn;
     });
+    uint64_t words = 0;
+    for (uint32_t i : wl) {
+      const int32_t steps = est ? estimate_steps(P, P->pairs[i].qn, P->pairs[i].tn) : outs[i].steps;
+      words = std::max(words, exact_hist_words(steps, P->pairs[i].qn, P->pairs[i].tn));
+    }
+    const uint64_t slot_bytes = (words * 4 + 255) & ~(uint64_t)255;
     if (dev_alloc(ctx, P->d_work2, wl.size() * 4) != PHX_OK) return PHX_E_NOMEM;
     HIPCHK(hipMemcpy(P->d_work2.p, wl.data(), wl.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(launch_zero(d_ctr, 2, ctx->stream));
-    int rc = launch_wfa_tier(P, t, wl, (uint32_t*)P->d_work2.p, d_ctr);
+    uint64_t maxL = 0;
+    for (uint32_t i : wl) maxL = std::max<uint64_t>(maxL, (uint64_t)P->pairs[i].qn + P->pairs[i].tn);
+    const int rt = maxL <= 8192 ? 1 : 3;
+    int rc = launch_wfa_tier(P, rt, wl, (uint32_t*)P->d_work2.p, d_ctr, slot_bytes, 15, ctx->stream);
     if (rc != PHX_OK) return rc;
     HIPCHK(hipMemcpyAsync(ctr_h, d_ctr, 8, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    if (ctr_h[1] == 0) continue;
-    HIPCHK(hipMemcpy(outs.data(), P->d_out.p, sizeof(PairOut) * P->n, hipMemcpyDeviceToHost));
-    collect(t, wl);
+    std::vector<uint32_t> again;
+    if (ctr_h[1] != 0) {
+      HIPCHK(hipMemcpy(outs.data(), P->d_out.p, sizeof(PairOut) * P->n, hipMemcpyDeviceToHost));
+      for (uint32_t i : wl)
+        if (overflowed(i)) again.push_back(i);
+    }
+    wl.clear();
+    exact.insert(exact.end(), again.begin(), again.end());
   }
   return PHX_OK;
 }
@@ -426,27 +461,33 @@ phx_ctx* phx_ctx_create(int device, uint32_t flags, const phx_tuning* tuning) {
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return nullptr; }
+  for (int i = 0; i < kStreams; ++i) {
+    if (hipStreamCreateWithFlags(&c->streams[i], hipStreamNonBlocking) != hipSuccess) { delete c; return nullptr; }
+  }
+  c->stream = c->streams[0];
+  for (hipEvent_t& e : c->join) {
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) { delete c; return nullptr; }
+  }
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
-    (void)hipStreamDestroy(c->stream);
+    for (hipEvent_t e : c->join) (void)hipEventDestroy(e);
+  for (int i = 0; i < kStreams; ++i) (void)hipStreamDestroy(c->streams[i]);
     delete c;
     return nullptr;
   }
   phx_tuning d{};
-  d.wave_wcap = 128;
-  d.wave_hist_kib = 9;  // per-wave LDS budget of the wave tier: 4 x 4 waves x 9 KiB <= 160 KiB
+  d.wave_lds_kib = 9;  // 4 workgroups x 4 waves x 9 KiB fit a CU's 160 KiB
   d.wave_slots = (uint32_t)c->cu_count * 16;
-  d.block_wcap = 1024;
-  d.block_hist_kib = 4096;
-  d.block_slots = (uint32_t)c->cu_count * 2;
-  d.hbm_budget_mib = 32768;
+  d.hbm_slots = (uint32_t)c->cu_count * 8;
+  d.hbm_slot_mib = 32;
+  size_t free_b = 0, total_b = 0;
+  d.hbm_budget_mib = 65536;
+  if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b > 0)
+    d.hbm_budget_mib = std::max<uint64_t>(1024, (uint64_t)(0.6 * (double)free_b) >> 20);
   if (tuning) {
-    if (tuning->wave_wcap) d.wave_wcap = next_pow2(tuning->wave_wcap);
-    if (tuning->wave_hist_kib) d.wave_hist_kib = tuning->wave_hist_kib;
+    if (tuning->wave_lds_kib) d.wave_lds_kib = tuning->wave_lds_kib;
     if (tuning->wave_slots) d.wave_slots = tuning->wave_slots;
-    if (tuning->block_wcap) d.block_wcap = next_pow2(tuning->block_wcap);
-    if (tuning->block_hist_kib) d.block_hist_kib = tuning->block_hist_kib;
-    if (tuning->block_slots) d.block_slots = tuning->block_slots;
+    if (tuning->hbm_slots) d.hbm_slots = tuning->hbm_slots;
+    if (tuning->hbm_slot_mib) d.hbm_slot_mib = tuning->hbm_slot_mib;
     if (tuning->hbm_budget_mib) d.hbm_budget_mib = tuning->hbm_budget_mib;
   }
   c->tun = d;
@@ -459,7 +500,8 @@ void phx_ctx_destroy(phx_ctx* c) {
   (void)hipEventDestroy(c->ev0);
   (void)hipEventDestroy(c->ev1);
   for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
-  (void)hipStreamDestroy(c->stream);
+  for (hipEvent_t e : c->join) (void)hipEventDestroy(e);
+  for (int i = 0; i < kStreams; ++i) (void)hipStreamDestroy(c->streams[i]);
   delete c;
 }
 
@@ -566,12 +608,41 @@ phx_plan* phx_plan_create(phx_ctx* ctx, const phx_batch* b) {
     return P->qlen[a] + P->tlen[a] > P->qlen[c] + P->tlen[c];
   });
   if (is_wfa(b->engine)) {
-    P->work.assign(4, {});
+    // Group 0: LDS tier.  Groups 1..5: HBM tier, length buckets with edges
+    // 4096, 8192, 16384, 32768 (qn + tn); each bucket's slot fits the largest
+    // estimate in it.
+    P->work.assign(6, {});
+    P->group_tier.assign(6, 1);
+    P->group_tier[0] = 0;
+    P->group_slot.assign(6, 0);
+    std::vector<uint64_t> est_words(6, 0);
     for (uint32_t i : order) {
       const uint64_t L = P->qlen[i] + P->tlen[i];
-      const int t = (L <= kWaveMaxLen && P->ring_slots <= kWaveRingMax) ? 0 : (L <= 16384 ? 1 : 2);
-      P->work[t].push_back(i);
+      int gi;
+      if (L <= kWaveMaxLen && P->ring_slots <= kWaveRingMax) gi = 0;
+      else gi = L <= 4096 ? 1 : L <= 8192 ? 2 : L <= 16384 ? 3 : L <= 32768 ? 4 : 5;
+      P->work[gi].push_back(i);
+      if (gi > 0) {
+        const int32_t qn = P->pairs[i].qn, tn = P->pairs[i].tn;
+        est_words[gi] = std::max(est_words[gi], exact_hist_words(estimate_steps(P, qn, tn), qn, tn));
+      }
     }
+    for (int gi = 1; gi < 6; ++gi) P->group_slot[gi] = (est_words[gi] * 4 + 255) & ~(uint64_t)255;
+    // Tiers: buckets up to 8192 run a wave per pair, 16384 a 256-thread
+    // workgroup per pair, longer a 1024-thread workgroup per pair.
+    P->group_tier = {0, 1, 1, 2, 3, 3};
+    // HBM budget shares follow each group's estimated history volume.
+    std::vector<double> vol(6, 0.0);
+    double tot = 0;
+    for (int gi = 1; gi < 6; ++gi) {
+      for (uint32_t i : P->work[gi]) {
+        const int32_t qn = P->pairs[i].qn, tn = P->pairs[i].tn;
+        vol[gi] += (double)exact_hist_words(estimate_steps(P, qn, tn), qn, tn);
+      }
+      tot += vol[gi];
+    }
+    P->group_share.assign(6, 0.0);
+    for (int gi = 1; gi < 6; ++gi) P->group_share[gi] = tot > 0 ? std::max(0.02, vol[gi] / tot) : 0.2;
   } else {
     P->work.assign(9, {});
     for (uint32_t i : order) {
@@ -722,7 +793,7 @@ void phx_plan_destroy(phx_plan* P) {
   for (DevBuf* b : {&P->d_seq, &P->d_pairs, &P->d_out, &P->d_runs, &P->d_work, &P->d_work2, &P->d_ctr,
                     &P->d_scratch})
     dev_free(*b);
-  for (int t = 0; t < 8; ++t) {
+  for (int t = 0; t < 16; ++t) {
     dev_free(P->d_hist[t]);
     dev_free(P->d_ring[t]);
   }

@@ -1,100 +1,164 @@
-// wfa_wave.hip -- wave-resident WFA tier for short pairs (qn + tn <= 2048).
+// wfa_wave.hip -- wave-per-pair WFA (gap-affine and gap-linear).
 //
 // Each wave64 of a 256-thread workgroup owns one pair at a time and pulls the
 // next pair from the device work queue on its own; the four waves never
-// synchronise with each other.  Everything a pair touches lives in the wave's
-// LDS region: the packed query and target, a ring of the last R score frames
-// (int16 offsets, 128 diagonals, two per lane), per-step cell offsets and
-// 16-bit history cells (bits 15..4 = pre-extension offset + 1, bits 3..0 =
-// ops).  Diagonal ranges are arithmetic functions of the step (see below).
-// The fitting/global endpoint is found with a wave ballot (first chunk,
-// lowest lane = smallest diagonal); the traceback runs wave-wide and resolves
-// affine gap runs 64 cells per ballot.  A pair whose
-// wavefront exceeds 128 diagonals or whose history exceeds the LDS budget
-// reports an overflow status and is re-run on the workgroup-per-pair tier.
+// synchronise with each other.  Two storage tiers share one implementation:
 //
-// Recurrences, candidate orders and tie rules are those of
-// phraya-align/src/wfa_simd.rs (linear :283-330, fitting end :940-955,
-// linear traceback :959-1130, affine :581-779 and :785-886), exactly as in
-// wfa_kernels.hip.
+//  * LDS tier (short pairs, qn + tn <= 2048): the packed query and target, a
+//    ring of the last R score frames (int16 offsets, 128 diagonals, two per
+//    lane), per-step cell offsets and 16-bit history cells live in the wave's
+//    LDS region.  A pair that outgrows it reports an overflow status.
+//  * HBM tier (any length): ring (int32, width a power of two >= qn + tn + 1)
+//    and history (32-bit cells) live in the wave's HBM slot; sequences are
+//    read through the caches.  When the history slot is full the wave keeps
+//    going without recording history, so the pair returns the exact number of
+//    score steps it needs and the host re-runs it in a slot of exactly that
+//    size.
+//
+// History cell: bits 31..4 (15..4 in LDS) = pre-extension offset + 1 (0 =
+// unset), bits 3..0 = ops (affine: m_op | ins_op << 2 | del_op << 3; linear:
+// 1 X, 2 query-only, 3 target-only).  Every M run of the upstream traceback is
+// "current offset - pre-extension offset of the cell", so nothing else is kept.
+//
+// Diagonal ranges are arithmetic: every diagonal shift costs at least one
+// gap-extend unit (one score step of g = gcd(x, o, e)), so after u steps all
+// set cells lie in |k| <= u.  Step u computes and writes every k in
+// [max(-u, -tn), min(u, qn)] (-1 = unset), so a ring read is valid exactly when
+// the predecessor step's range holds the diagonal; cells outside the reachable
+// set come out unset, as in upstream's full-width frames.
+//
+// Recurrences, candidate orders and tie rules follow
+// phraya-align/src/wfa_simd.rs: linear :283-330 (last maximum wins, bounds
+// check after the choice), fitting end :940-955, linear traceback :959-1130,
+// affine :581-779 (strict-greater first candidate), affine traceback :785-886.
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "phx_device.h"
 #include "wfa_common.h"
 
 namespace phx {
 
-constexpr int WV_W = 128;               // diagonals per ring frame
-constexpr int WV_WAVES = 4;             // waves (pairs) per workgroup
+constexpr int WV_W = 128;    // LDS tier ring width (diagonals)
+constexpr int WV_WAVES = 4;  // waves (pairs) per workgroup
 
+// Orders one wave's memory traffic between steps.  LDS accesses of a wave
+// complete in order, so a wavefront-scope fence (a compiler barrier) is
+// enough; the HBM tier also waits for its global stores (workgroup scope).
+template <bool LDS>
 __device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (LDS) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  }
+}
+
+// Synchronisation between the threads of one pair: a wave-level fence for a
+// wave-per-pair launch, a workgroup barrier when NT threads share the pair.
+template <bool LDS, int NT>
+__device__ __forceinline__ void pair_sync() {
+  if (NT == 64) wave_sync<LDS>();
+  else __syncthreads();
 }
 
 __device__ __forceinline__ int32_t readlane(int32_t v, int32_t lane) {
   return __builtin_amdgcn_readlane(v, lane);
 }
 
-struct WaveLds {
-  int16_t* ring;
-  uint64_t* seq;
-  uint16_t* toff;   // first history cell of each step
-  uint16_t* cells;
-};
-
-__device__ __forceinline__ WaveLds wave_lds(uint8_t* base, const WaveGeom& G) {
-  WaveLds L;
-  L.ring = reinterpret_cast<int16_t*>(base + G.ring_off);
-  L.seq = reinterpret_cast<uint64_t*>(base + G.seq_off);
-  L.toff = reinterpret_cast<uint16_t*>(base + G.tab_off);
-  L.cells = reinterpret_cast<uint16_t*>(base + G.cell_off);
-  return L;
-}
-
-// Copies the pair's packed words into LDS; returns the target's word offset.
-template <bool B8>
-__device__ __forceinline__ uint32_t stage_seqs(const WfaArgs& a, const PairDesc& pd, uint64_t* dst,
-                                               int lane) {
-  const uint32_t nq = B8 ? ((uint32_t)pd.qn + 7) / 8 + 1 : ((uint32_t)pd.qn + 31) / 32 + 1;
-  const uint32_t nt = B8 ? ((uint32_t)pd.tn + 7) / 8 + 1 : ((uint32_t)pd.tn + 31) / 32 + 1;
-  for (uint32_t i = (uint32_t)lane; i < nq + nt; i += 64)
-    dst[i] = i < nq ? a.seq[pd.q_word + i] : a.seq[pd.t_word + (i - nq)];
-  wave_sync();
-  return nq;
-}
-
-// --- affine --------------------------------------------------------------------
-//
-// Diagonal ranges are arithmetic: every diagonal shift costs at least one
-// gap-extend unit, so after u score steps (in units of g) all set cells lie in
-// |k| <= u.  Step u computes every k in [max(-u, -tn), min(u, qn)] and writes
-// all three layers there (-1 = unset), so a ring read is valid exactly when
-// the predecessor step's range contains the diagonal; no per-slot metadata is
-// kept.  Cells outside the reachable set simply come out unset, as upstream's
-// full-width frames do.
-
 __device__ __forceinline__ int32_t step_lo(int32_t u, int32_t tn) { return max(-u, -tn); }
 __device__ __forceinline__ int32_t step_hi(int32_t u, int32_t qn) { return min(u, qn); }
 
+// Storage of one wave's pair: LDS (short) or HBM (wide).
+template <bool LDS>
+struct WaveStore {
+  using ring_t = typename std::conditional<LDS, int16_t, int32_t>::type;
+  using cell_t = typename std::conditional<LDS, uint16_t, uint32_t>::type;
+  ring_t* ring;
+  cell_t* cells;
+  cell_t* toff_base;  // LDS: toff[step]; HBM: entries grow down from toff_base
+  uint64_t cap;       // LDS: cells; HBM: 32-bit words of the slot
+  uint32_t steps_cap; // LDS only
+  uint32_t wmask;
+  const uint64_t* qw;
+  const uint64_t* tw;
+
+  __device__ __forceinline__ cell_t& toff(int32_t step) const {
+    return LDS ? toff_base[step] : *(toff_base - 1 - step);
+  }
+  // Room for `width` more cells (and the step's offset entry)?
+  __device__ __forceinline__ bool fits(int32_t step, uint64_t used, int32_t width) const {
+    if (LDS) return (uint32_t)step < steps_cap && used + (uint64_t)width <= cap;
+    return used + (uint64_t)width + (uint64_t)step + 1 <= cap;
+  }
+};
+
+template <bool B8>
+__device__ __forceinline__ uint32_t seq_words(int32_t n) {
+  return B8 ? ((uint32_t)n + 7) / 8 + 1 : ((uint32_t)n + 31) / 32 + 1;
+}
+
+template <bool LDS, bool B8, int NT>
+__device__ __forceinline__ WaveStore<LDS> make_store(const WfaArgs& a, const WaveGeom& G,
+                                                     uint8_t* lds_base, const PairDesc& pd,
+                                                     int lane) {
+  using S_t = WaveStore<LDS>;
+  S_t S;
+  if (LDS) {
+    S.ring = reinterpret_cast<typename S_t::ring_t*>(lds_base + G.ring_off);
+    uint64_t* seq = reinterpret_cast<uint64_t*>(lds_base + G.seq_off);
+    const uint32_t nq = seq_words<B8>(pd.qn), nt = seq_words<B8>(pd.tn);
+    for (uint32_t i = (uint32_t)lane; i < nq + nt; i += 64)
+      seq[i] = i < nq ? a.seq[pd.q_word + i] : a.seq[pd.t_word + (i - nq)];
+    S.qw = seq;
+    S.tw = seq + nq;
+    S.toff_base = reinterpret_cast<typename S_t::cell_t*>(lds_base + G.tab_off);
+    S.cells = reinterpret_cast<typename S_t::cell_t*>(lds_base + G.cell_off);
+    S.cap = G.cells_cap;
+    S.steps_cap = G.steps_cap;
+    S.wmask = WV_W - 1;
+    wave_sync<true>();
+  } else {
+    // One HBM slot per pair in flight: per wave, or per workgroup when the
+    // whole workgroup shares a pair.
+    const uint64_t gw = NT == 64 ? (uint64_t)blockIdx.x * WV_WAVES + (threadIdx.x >> 6)
+                                 : (uint64_t)blockIdx.x;
+    S.ring = reinterpret_cast<typename S_t::ring_t*>(a.ring_g + gw * a.ring_stride);
+    uint8_t* h = a.hist + gw * a.hist_stride;
+    S.cells = reinterpret_cast<typename S_t::cell_t*>(h);
+    S.toff_base = reinterpret_cast<typename S_t::cell_t*>(h + a.hist_stride);
+    S.cap = a.hist_stride / 4;
+    S.steps_cap = 0;
+    S.wmask = (1u << a.wcap_log2) - 1;
+    S.qw = a.seq + pd.q_word;
+    S.tw = a.seq + pd.t_word;
+  }
+  return S;
+}
+
 // Parallel search along a gap: lane j looks at the cell of step s - j*en on
 // diagonal k + dir*j and reports whether the gap was opened there (op bit 0).
-// Returns the number of gap bases up to and including the opening cell, or
-// 64 + (no opening within the window) as 64 with *opened = false.
-__device__ __forceinline__ int32_t gap_run(const WaveLds& L, int32_t s, int32_t k, int32_t dir,
-                                           int32_t en, uint32_t bit, int32_t qn, int32_t tn,
-                                           int lane, bool* opened) {
+// Returns the gap bases up to and including the opening cell (64 and
+// *opened = false when the opening lies beyond this window).
+template <bool LDS>
+__device__ __forceinline__ int32_t gap_run(const WaveStore<LDS>& S, int32_t s, int32_t k,
+                                           int32_t dir, int32_t en, uint32_t bit, int32_t qn,
+                                           int32_t tn, int lane, bool* opened) {
   const int32_t sj = s - lane * en;
   const int32_t kj = k + dir * lane;
   bool open = false;
   if (sj >= 1) {
     const int32_t lo = step_lo(sj, tn), hi = step_hi(sj, qn);
     if (kj >= lo && kj <= hi) {
-      const uint32_t cell = L.cells[(uint32_t)L.toff[sj] + (uint32_t)(kj - lo)];
+      const uint32_t cell = S.cells[(uint64_t)S.toff(sj) + (uint64_t)(kj - lo)];
       open = ((cell >> bit) & 1u) == 0;
     } else {
-      open = true;  // unreachable for a valid path; ends the run (caught below)
+      open = true;  // not on a valid path; the caller's checks report it
     }
   }
   const uint64_t mask = __ballot(open);
@@ -102,18 +166,34 @@ __device__ __forceinline__ int32_t gap_run(const WaveLds& L, int32_t s, int32_t 
   return mask ? (int32_t)__builtin_ctzll(mask) + 1 : 64;
 }
 
-template <bool B8>
-__device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* base,
-                            const PairDesc& pd, PairOut* po, int lane) {
+// Thread 0 stores the pair's result and counts an overflow for the host (one
+// counter read tells the host whether any pair must be re-run).
+__device__ __forceinline__ void publish(const WfaArgs& a, PairOut* po, const PairOut& r, int lane) {
+  if (lane == 0) {
+    *po = r;
+    if (r.status >= ST_OVERFLOW_HIST) atomicAdd(&a.counters[1], 1u);
+  }
+}
+
+// Endpoint reduction for multi-wave pairs: smallest ((k + tn) << 2 | layer),
+// triple-buffered so a step can reset the next buffer before its barrier.
+struct PairShared {
+  int32_t key[3];
+  uint32_t work;
+};
+
+// --- affine --------------------------------------------------------------------
+
+template <bool LDS, bool B8, int NT>
+__device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_base,
+                            const PairDesc& pd, PairOut* po, int lane, PairShared* sh) {
+  using S_t = WaveStore<LDS>;
   const int32_t qn = pd.qn, tn = pd.tn;
   if (qn == 0) {  // wfa_simd.rs:925-927
     if (lane == 0) *po = PairOut{ST_OK, 0, 0, 0, 0, 0, 0, 0};
     return;
   }
-  WaveLds L = wave_lds(base, G);
-  const uint32_t tw_off = stage_seqs<B8>(a, pd, L.seq, lane);
-  const uint64_t* qw = L.seq;
-  const uint64_t* tw = L.seq + tw_off;
+  const S_t S = make_store<LDS, B8, NT>(a, G, lds_base, pd, lane);
   const int32_t x = a.cp.x, o = a.cp.o, e = a.cp.e;
   const bool fitting = tn > qn + qn / 2 + 10;
   int32_t g = x;
@@ -124,21 +204,24 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* base,
   const int64_t hard_max = (int64_t)(qn + tn) * max(x, 1) + o + e + 4;
   const int64_t explore_to = a.cp.cap >= 0 ? min((int64_t)a.cp.cap, hard_max) : hard_max;
   const int32_t max_step = (int32_t)(explore_to / g);
-  int16_t* Rm = L.ring;
-  int16_t* Ri = L.ring + R * WV_W;
-  int16_t* Rd = L.ring + 2 * R * WV_W;
+  const uint32_t W = S.wmask + 1;
+  typename S_t::ring_t* Rm = S.ring;
+  typename S_t::ring_t* Ri = S.ring + (size_t)R * W;
+  typename S_t::ring_t* Rd = S.ring + (size_t)2 * R * W;
 
-  const int32_t m0 = extend_diag<B8>(qw, tw, qn, tn, 0, 0);
+  const int32_t m0 = extend_diag<B8>(S.qw, S.tw, qn, tn, 0, 0);
   if (lane == 0) {
     // Step 0 holds only m on diagonal 0; its ins/del entries are unset.
-    Rm[0] = (int16_t)m0;
+    Rm[0] = (typename S_t::ring_t)m0;
     Ri[0] = -1;
     Rd[0] = -1;
+    if (NT > 64) sh->key[0] = sh->key[1] = sh->key[2] = INT32_MAX;
   }
-  wave_sync();
+  pair_sync<LDS, NT>();
   int32_t status = ST_OK;
   int32_t step = 0, slot = 0;
-  uint32_t used = 0;
+  uint64_t used = 0;
+  bool hist_on = true;
   int32_t end_k = 0, end_layer = 0;
   bool found = fitting ? (m0 == qn) : (qn == tn && m0 == qn);  // s = 0 endpoint on diagonal 0
   const int32_t kend_g = qn - tn;
@@ -148,36 +231,36 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* base,
     slot = __builtin_amdgcn_readfirstlane(slot + 1 == R ? 0 : slot + 1);
     const int32_t lo = step_lo(step, tn), hi = step_hi(step, qn);
     const int32_t width = hi - lo + 1;
-    if (width > WV_W) { status = ST_OVERFLOW_WIDTH; break; }
-    if ((uint32_t)step >= G.steps_cap || used + (uint32_t)width > G.cells_cap) {
-      status = ST_OVERFLOW_HIST;
-      break;
+    if ((uint32_t)width > W) { status = ST_OVERFLOW_WIDTH; break; }
+    if (hist_on && !S.fits(step, used, width)) {
+      if (LDS) { status = ST_OVERFLOW_HIST; break; }
+      hist_on = false;  // keep going to learn the exact number of steps
     }
-    if (lane == 0) L.toff[step] = (uint16_t)used;
+    if (hist_on && lane == 0) S.toff(step) = (typename S_t::cell_t)used;
+    if (NT > 64 && lane == 0) sh->key[(step + 1) % 3] = INT32_MAX;
     const int32_t sx = step - xn, so = step - oen, se = step - en;
     int32_t rx = slot - xn, ro = slot - oen, re = slot - en;
     rx += rx < 0 ? R : 0;
     ro += ro < 0 ? R : 0;
     re += re < 0 ? R : 0;
-    // Predecessor ranges (empty when the step does not exist).
     const int32_t xlo = sx >= 0 ? step_lo(sx, tn) : 1, xhi = sx >= 0 ? step_hi(sx, qn) : 0;
     const int32_t olo = so >= 0 ? step_lo(so, tn) : 1, ohi = so >= 0 ? step_hi(so, qn) : 0;
     const int32_t elo = se >= 0 ? step_lo(se, tn) : 1, ehi = se >= 0 ? step_hi(se, qn) : 0;
     const int32_t ke_lo = fitting ? max(kend_g, lo) : kend_g;
     const int32_t ke_hi = fitting ? hi : kend_g;
-    const int16_t* Rmx = Rm + rx * WV_W;
-    const int16_t* Rmo = Rm + ro * WV_W;
-    const int16_t* Rie = Ri + re * WV_W;
-    const int16_t* Rde = Rd + re * WV_W;
-    int16_t* Wm = Rm + slot * WV_W;
-    int16_t* Wi = Ri + slot * WV_W;
-    int16_t* Wd = Rd + slot * WV_W;
-    uint16_t* cells = L.cells + used;
-    const int32_t nch = (width + 63) >> 6;
+    const typename S_t::ring_t* Rmx = Rm + (size_t)rx * W;
+    const typename S_t::ring_t* Rmo = Rm + (size_t)ro * W;
+    const typename S_t::ring_t* Rie = Ri + (size_t)re * W;
+    const typename S_t::ring_t* Rde = Rd + (size_t)re * W;
+    typename S_t::ring_t* Wm = Rm + (size_t)slot * W;
+    typename S_t::ring_t* Wi = Ri + (size_t)slot * W;
+    typename S_t::ring_t* Wd = Rd + (size_t)slot * W;
+    typename S_t::cell_t* cells = S.cells + used;
+    const int32_t nch = (width + NT - 1) / NT;
     for (int32_t c = 0; c < nch; ++c) {
-      const int32_t k = lo + c * 64 + lane;
-      const uint32_t ik = (uint32_t)k & (WV_W - 1), il = (uint32_t)(k - 1) & (WV_W - 1),
-                     ir = (uint32_t)(k + 1) & (WV_W - 1);
+      const int32_t k = lo + c * NT + lane;
+      const uint32_t ik = (uint32_t)k & S.wmask, il = (uint32_t)(k - 1) & S.wmask,
+                     ir = (uint32_t)(k + 1) & S.wmask;
       const int32_t rvx = Rmx[ik], rvol = Rmo[il], rvor = Rmo[ir], rvil = Rie[il], rvdr = Rde[ir];
       const int32_t vx = (k >= xlo && k <= xhi) ? rvx : -1;
       const int32_t vol = (k - 1 >= olo && k - 1 <= ohi) ? rvol : -1;
@@ -198,31 +281,46 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* base,
       if (ins > best) { best = ins; mop = 1; }
       if (del > best) { best = del; mop = 2; }
       int32_t mval = best;
-      if (best >= 0) mval = extend_diag<B8>(qw, tw, qn, tn, best, k);
+      if (best >= 0) mval = extend_diag<B8>(S.qw, S.tw, qn, tn, best, k);
       int32_t layer = -1;
       if (k <= hi) {
-        Wm[ik] = (int16_t)mval;
-        Wi[ik] = (int16_t)ins;
-        Wd[ik] = (int16_t)del;
-        cells[k - lo] = (uint16_t)(((uint32_t)(best + 1) << 4) | mop | (iop << 2) | (dop << 3));
+        Wm[ik] = (typename S_t::ring_t)mval;
+        Wi[ik] = (typename S_t::ring_t)ins;
+        Wd[ik] = (typename S_t::ring_t)del;
+        if (hist_on)
+          cells[k - lo] = (typename S_t::cell_t)(((uint32_t)(best + 1) << 4) | mop | (iop << 2) |
+                                                 (dop << 3));
         if (k >= ke_lo && k <= ke_hi) layer = mval == qn ? 0 : ins == qn ? 1 : del == qn ? 2 : -1;
       }
-      const uint64_t mask = __ballot(layer >= 0);
-      if (mask && !found) {
-        const int32_t ln = __builtin_ctzll(mask);
-        found = true;
-        end_k = lo + c * 64 + ln;
-        end_layer = readlane(layer, ln);
+      if (NT == 64) {
+        const uint64_t mask = __ballot(layer >= 0);
+        if (mask && !found) {
+          const int32_t ln = __builtin_ctzll(mask);
+          found = true;
+          end_k = lo + c * 64 + ln;
+          end_layer = readlane(layer, ln);
+        }
+      } else if (layer >= 0) {
+        atomicMin(&sh->key[step % 3], ((k + tn) << 2) | layer);
       }
     }
-    used += (uint32_t)width;
-    wave_sync();
+    used += (uint64_t)width;
+    pair_sync<LDS, NT>();
+    if (NT > 64) {
+      const int32_t key = __builtin_amdgcn_readfirstlane(sh->key[step % 3]);
+      if (key != INT32_MAX) {
+        found = true;
+        end_k = (key >> 2) - tn;
+        end_layer = key & 3;
+      }
+    }
   }
+  if (!hist_on && status == ST_OK) status = ST_OVERFLOW_HIST;
 
   // Traceback (affine_traceback, wfa_simd.rs:785-886), wave-uniform state;
-  // gap runs are resolved 64 cells at a time.
-  PairOut r{status, 0, 0, 0, 0, 0, step, (int32_t)used};
-  if (status == ST_OK) {
+  // gap runs are resolved 64 cells per ballot.
+  PairOut r{status, 0, 0, 0, 0, 0, step, (int32_t)min<uint64_t>(used, INT32_MAX)};
+  if (status == ST_OK && lane < 64) {
     if (step == 0) { end_k = 0; end_layer = 0; }
     const int32_t t_end = qn - end_k;
     RunWriter w;
@@ -236,7 +334,7 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* base,
       if (layer == 0) {
         const int32_t lo = step_lo(s, tn), hi = step_hi(s, qn);
         if (k < lo || k > hi) { r.status = ST_INTERNAL; break; }
-        const uint32_t cell = L.cells[(uint32_t)L.toff[s] + (uint32_t)(k - lo)];
+        const uint32_t cell = S.cells[(uint64_t)S.toff(s) + (uint64_t)(k - lo)];
         const int32_t pre = (int32_t)(cell >> 4) - 1;
         if (pre < 0) { r.status = ST_INTERNAL; break; }
         const uint32_t mop = cell & 3;
@@ -246,12 +344,12 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* base,
         if (mop == 0) { w.emit(OP_X, 1); --qi; --ti; s -= xn; }
         else layer = (int32_t)mop;
       } else {
-        // Gap layer: consecutive cells extend (op bit 1) until the one that
-        // opened the gap (op bit 0) from the m layer o+e earlier.
+        // Gap layer: cells extend (op bit 1) back to the one that opened the
+        // gap (op bit 0) from the m layer o+e earlier.
         const int32_t dir = layer == 1 ? -1 : 1;
         const uint32_t bit = layer == 1 ? 2 : 3;
         bool opened = false;
-        const int32_t n = gap_run(L, s, k, dir, en, bit, qn, tn, lane, &opened);
+        const int32_t n = gap_run<LDS>(S, s, k, dir, en, bit, qn, tn, lane, &opened);
         w.emit(layer == 1 ? OP_D : OP_I, n);
         if (layer == 1) qi -= n; else ti -= n;
         k += dir * n;
@@ -268,34 +366,37 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* base,
     r.nruns = (int32_t)w.n;
     r.run_start = (int32_t)(pd.run_cap - w.n);
   }
-  if (lane == 0) *po = r;
+  publish(a, po, r, lane);
 }
 
 // --- linear --------------------------------------------------------------------
 
-template <bool B8>
-__device__ void wave_linear(const WfaArgs& a, const WaveGeom& G, uint8_t* base,
-                            const PairDesc& pd, PairOut* po, int lane) {
+template <bool LDS, bool B8, int NT>
+__device__ void wave_linear(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_base,
+                            const PairDesc& pd, PairOut* po, int lane, PairShared* sh) {
+  using S_t = WaveStore<LDS>;
   const int32_t qn = pd.qn, tn = pd.tn;
   if (qn == 0) {  // wfa_simd.rs:194-203, :365-367
     if (lane == 0) *po = PairOut{ST_OK, 0, 0, 0, 0, 0, 0, 0};
     return;
   }
-  WaveLds L = wave_lds(base, G);
-  const uint32_t tw_off = stage_seqs<B8>(a, pd, L.seq, lane);
-  const uint64_t* qw = L.seq;
-  const uint64_t* tw = L.seq + tw_off;
+  const S_t S = make_store<LDS, B8, NT>(a, G, lds_base, pd, lane);
   const bool fitting = tn > qn + qn / 2 + 10;
   const int32_t max_s = qn + tn;
   const int32_t explore = (fitting && a.cp.cap >= 0) ? min(a.cp.cap, max_s) : max_s;
   const int32_t kend_global = qn - tn;
-  int16_t* ring = L.ring;  // two slots
+  const uint32_t W = S.wmask + 1;
+  typename S_t::ring_t* ring = S.ring;  // two slots
 
-  const int32_t w0 = extend_diag<B8>(qw, tw, qn, tn, 0, 0);
-  if (lane == 0) ring[0] = (int16_t)w0;
-  wave_sync();
+  const int32_t w0 = extend_diag<B8>(S.qw, S.tw, qn, tn, 0, 0);
+  if (lane == 0) {
+    ring[0] = (typename S_t::ring_t)w0;
+    if (NT > 64) sh->key[0] = sh->key[1] = sh->key[2] = INT32_MAX;
+  }
+  pair_sync<LDS, NT>();
   int32_t step = 0, status = ST_OK, end_k = 0;
-  uint32_t used = 0;
+  uint64_t used = 0;
+  bool hist_on = true;
   bool exhausted = false;
   bool found = fitting ? (w0 >= qn) : (kend_global == 0 && w0 == qn);
   while (!found) {
@@ -307,22 +408,23 @@ __device__ void wave_linear(const WfaArgs& a, const WaveGeom& G, uint8_t* base,
     const int32_t lo = step_lo(step, tn), hi = step_hi(step, qn);
     const int32_t p_lo = step_lo(step - 1, tn), p_hi = step_hi(step - 1, qn);
     const int32_t width = hi - lo + 1;
-    if (width > WV_W) { status = ST_OVERFLOW_WIDTH; break; }
-    if ((uint32_t)step >= G.steps_cap || used + (uint32_t)width > G.cells_cap) {
-      status = ST_OVERFLOW_HIST;
-      break;
+    if ((uint32_t)width > W) { status = ST_OVERFLOW_WIDTH; break; }
+    if (hist_on && !S.fits(step, used, width)) {
+      if (LDS) { status = ST_OVERFLOW_HIST; break; }
+      hist_on = false;
     }
-    if (lane == 0) L.toff[step] = (uint16_t)used;
-    const int16_t* P = ring + ps * WV_W;
-    int16_t* Wn = ring + slot * WV_W;
-    uint16_t* cells = L.cells + used;
+    if (hist_on && lane == 0) S.toff(step) = (typename S_t::cell_t)used;
+    if (NT > 64 && lane == 0) sh->key[(step + 1) % 3] = INT32_MAX;
+    const typename S_t::ring_t* P = ring + (size_t)ps * W;
+    typename S_t::ring_t* Wn = ring + (size_t)slot * W;
+    typename S_t::cell_t* cells = S.cells + used;
     const int32_t ke_lo = fitting ? max(qn - tn, lo) : kend_global;
     const int32_t ke_hi = fitting ? hi : kend_global;
-    const int32_t nch = (width + 63) >> 6;
+    const int32_t nch = (width + NT - 1) / NT;
     for (int32_t c = 0; c < nch; ++c) {
-      const int32_t k = lo + c * 64 + lane;
-      const uint32_t ik = (uint32_t)k & (WV_W - 1), il = (uint32_t)(k - 1) & (WV_W - 1),
-                     ir = (uint32_t)(k + 1) & (WV_W - 1);
+      const int32_t k = lo + c * NT + lane;
+      const uint32_t ik = (uint32_t)k & S.wmask, il = (uint32_t)(k - 1) & S.wmask,
+                     ir = (uint32_t)(k + 1) & S.wmask;
       const int32_t rvm = P[ik], rvl = P[il], rvr = P[ir];
       const int32_t vm = (k >= p_lo && k <= p_hi) ? rvm : -1;
       const int32_t vl = (k > -tn && k - 1 >= p_lo && k - 1 <= p_hi) ? rvl : -1;
@@ -336,24 +438,37 @@ __device__ void wave_linear(const WfaArgs& a, const WaveGeom& G, uint8_t* base,
       if (vr >= 0 && vr >= best) { best = vr; op = 3; }
       const bool ok = op != 0 && best <= qn && best - k >= 0 && best - k <= tn;
       int32_t val = -1;
-      if (ok) val = extend_diag<B8>(qw, tw, qn, tn, best, k);
+      if (ok) val = extend_diag<B8>(S.qw, S.tw, qn, tn, best, k);
       bool hit = false;
       if (k <= hi) {
-        Wn[ik] = (int16_t)val;
-        cells[k - lo] = (uint16_t)(ok ? (((uint32_t)(best + 1) << 4) | op) : 0u);
+        Wn[ik] = (typename S_t::ring_t)val;
+        if (hist_on)
+          cells[k - lo] = (typename S_t::cell_t)(ok ? (((uint32_t)(best + 1) << 4) | op) : 0u);
         hit = ok && k >= ke_lo && k <= ke_hi && val >= qn;
       }
-      const uint64_t mask = __ballot(hit);
-      if (mask && !found) {
-        found = true;
-        end_k = lo + c * 64 + (int32_t)__builtin_ctzll(mask);
+      if (NT == 64) {
+        const uint64_t mask = __ballot(hit);
+        if (mask && !found) {
+          found = true;
+          end_k = lo + c * 64 + (int32_t)__builtin_ctzll(mask);
+        }
+      } else if (hit) {
+        atomicMin(&sh->key[step % 3], k + tn);
       }
     }
-    used += (uint32_t)width;
-    wave_sync();
+    used += (uint64_t)width;
+    pair_sync<LDS, NT>();
+    if (NT > 64) {
+      const int32_t key = __builtin_amdgcn_readfirstlane(sh->key[step % 3]);
+      if (key != INT32_MAX) {
+        found = true;
+        end_k = key - tn;
+      }
+    }
   }
+  if (!hist_on && status == ST_OK && !exhausted) status = ST_OVERFLOW_HIST;
 
-  PairOut r{status, 0, 0, 0, 0, 0, step, (int32_t)used};
+  PairOut r{status, 0, 0, 0, 0, 0, step, (int32_t)min<uint64_t>(used, INT32_MAX)};
   if (status == ST_OK && exhausted) {
     if (fitting) {
       r.status = ST_ALIGNMENT_FAILED;
@@ -361,7 +476,7 @@ __device__ void wave_linear(const WfaArgs& a, const WaveGeom& G, uint8_t* base,
       r.t_end = tn;  // fill_wfa's fallthrough ("", 0) (wfa_simd.rs:343-344)
       r.run_start = (int32_t)pd.run_cap;
     }
-  } else if (status == ST_OK) {
+  } else if (status == ST_OK && lane < 64) {
     const int32_t t_end = qn - end_k;
     const int32_t ed = step;
     if (!fitting && a.cp.cap >= 0 && ed > a.cp.cap) {
@@ -377,7 +492,7 @@ __device__ void wave_linear(const WfaArgs& a, const WaveGeom& G, uint8_t* base,
         }
         const int32_t lo = step_lo(s, tn), hi = step_hi(s, qn);
         if (k < lo || k > hi) { r.status = ST_INTERNAL; break; }
-        const uint32_t cell = L.cells[(uint32_t)L.toff[s] + (uint32_t)(k - lo)];
+        const uint32_t cell = S.cells[(uint64_t)S.toff(s) + (uint64_t)(k - lo)];
         const uint32_t op = cell & 15;
         if (op == 0) { r.status = ST_INTERNAL; break; }
         const int32_t pre = (int32_t)(cell >> 4) - 1;
@@ -398,17 +513,17 @@ __device__ void wave_linear(const WfaArgs& a, const WaveGeom& G, uint8_t* base,
       r.run_start = (int32_t)(pd.run_cap - w.n);
     }
   }
-  if (lane == 0) *po = r;
+  publish(a, po, r, lane);
 }
 
 // --- kernel --------------------------------------------------------------------
 
-template <int MODE>
+template <int MODE, bool LDS>
 __global__ __launch_bounds__(64 * WV_WAVES) void wfa_wave_kernel(WfaArgs a, WaveGeom G) {
   extern __shared__ __align__(16) uint8_t wave_lds_pool[];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
-  uint8_t* base = wave_lds_pool + (size_t)wv * G.per_wave_bytes;
+  uint8_t* base = LDS ? wave_lds_pool + (size_t)wv * G.per_wave_bytes : nullptr;
   for (;;) {
     uint32_t w = 0;
     if (lane == 0) w = atomicAdd(&a.counters[0], 1u);
@@ -418,24 +533,81 @@ __global__ __launch_bounds__(64 * WV_WAVES) void wfa_wave_kernel(WfaArgs a, Wave
     const PairDesc pd = a.pairs[pi];
     PairOut* po = a.out + pi;
     if (MODE == ENG_AFFINE) {
-      if (pd.flags & PF_BYTES) wave_affine<true>(a, G, base, pd, po, lane);
-      else wave_affine<false>(a, G, base, pd, po, lane);
+      if (pd.flags & PF_BYTES) wave_affine<LDS, true, 64>(a, G, base, pd, po, lane, nullptr);
+      else wave_affine<LDS, false, 64>(a, G, base, pd, po, lane, nullptr);
     } else {
-      if (pd.flags & PF_BYTES) wave_linear<true>(a, G, base, pd, po, lane);
-      else wave_linear<false>(a, G, base, pd, po, lane);
+      if (pd.flags & PF_BYTES) wave_linear<LDS, true, 64>(a, G, base, pd, po, lane, nullptr);
+      else wave_linear<LDS, false, 64>(a, G, base, pd, po, lane, nullptr);
     }
-    if (lane == 0 && po->status >= ST_OVERFLOW_HIST) atomicAdd(&a.counters[1], 1u);
-    wave_sync();  // the wave's LDS region is reused by its next pair
+    wave_sync<LDS>();  // the wave's storage is reused by its next pair
   }
 }
 
-hipError_t launch_wfa_wave(int mode, const WfaArgs& a, const WaveGeom& G, uint32_t grid,
-                           hipStream_t st) {
-  const size_t lds = (size_t)G.per_wave_bytes * WV_WAVES;
-  if (mode == ENG_AFFINE)
-    hipLaunchKernelGGL(wfa_wave_kernel<ENG_AFFINE>, dim3(grid), dim3(64 * WV_WAVES), lds, st, a, G);
-  else
-    hipLaunchKernelGGL(wfa_wave_kernel<ENG_LINEAR>, dim3(grid), dim3(64 * WV_WAVES), lds, st, a, G);
+// Workgroup per pair (NT threads) for wide wavefronts; ring and history in
+// the workgroup's HBM slot.
+template <int MODE, int NT>
+__global__ __launch_bounds__(NT) void wfa_block_kernel(WfaArgs a) {
+  __shared__ PairShared sh;
+  const int tid = threadIdx.x;
+  const WaveGeom G{};
+  for (;;) {
+    if (tid == 0) sh.work = atomicAdd(&a.counters[0], 1u);
+    __syncthreads();
+    // The pair index is taken into a scalar register so that the loop exit is
+    // a scalar branch: every wave leaves together.  With a per-lane exit the
+    // compiler may give thread 0 (the only one with work before the next
+    // barrier) a loop of its own, and the other lanes would spin on the old
+    // index.  For the same reason no thread-0-only code follows the barrier at
+    // the end of the body.
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)sh.work);
+    if (w >= a.nwork) break;
+    const uint32_t pi = a.work_list[w];
+    const PairDesc pd = a.pairs[pi];
+    PairOut* po = a.out + pi;
+    if (MODE == ENG_AFFINE) {
+      if (pd.flags & PF_BYTES) wave_affine<false, true, NT>(a, G, nullptr, pd, po, tid, &sh);
+      else wave_affine<false, false, NT>(a, G, nullptr, pd, po, tid, &sh);
+    } else {
+      if (pd.flags & PF_BYTES) wave_linear<false, true, NT>(a, G, nullptr, pd, po, tid, &sh);
+      else wave_linear<false, false, NT>(a, G, nullptr, pd, po, tid, &sh);
+    }
+    __syncthreads();  // the slot is reused by the next pair
+  }
+}
+
+__global__ void zero_u32_kernel(uint32_t* p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t st) {
+  hipLaunchKernelGGL(zero_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_wfa_wave(int mode, bool lds, const WfaArgs& a, const WaveGeom& G,
+                           uint32_t grid, hipStream_t st) {
+  const size_t bytes = lds ? (size_t)G.per_wave_bytes * WV_WAVES : 0;
+  const dim3 block(64 * WV_WAVES);
+  if (mode == ENG_AFFINE) {
+    if (lds) hipLaunchKernelGGL((wfa_wave_kernel<ENG_AFFINE, true>), dim3(grid), block, bytes, st, a, G);
+    else hipLaunchKernelGGL((wfa_wave_kernel<ENG_AFFINE, false>), dim3(grid), block, 0, st, a, G);
+  } else {
+    if (lds) hipLaunchKernelGGL((wfa_wave_kernel<ENG_LINEAR, true>), dim3(grid), block, bytes, st, a, G);
+    else hipLaunchKernelGGL((wfa_wave_kernel<ENG_LINEAR, false>), dim3(grid), block, 0, st, a, G);
+  }
+  return hipGetLastError();
+}
+
+// nt = threads per pair: 256 or 1024.
+hipError_t launch_wfa_block(int mode, int nt, const WfaArgs& a, uint32_t grid, hipStream_t st) {
+  if (mode == ENG_AFFINE) {
+    if (nt == 1024) hipLaunchKernelGGL((wfa_block_kernel<ENG_AFFINE, 1024>), dim3(grid), dim3(1024), 0, st, a);
+    else hipLaunchKernelGGL((wfa_block_kernel<ENG_AFFINE, 256>), dim3(grid), dim3(256), 0, st, a);
+  } else {
+    if (nt == 1024) hipLaunchKernelGGL((wfa_block_kernel<ENG_LINEAR, 1024>), dim3(grid), dim3(1024), 0, st, a);
+    else hipLaunchKernelGGL((wfa_block_kernel<ENG_LINEAR, 256>), dim3(grid), dim3(256), 0, st, a);
+  }
   return hipGetLastError();
 }
 

@@ -193,11 +193,24 @@ def test_long_pairs_escalate_tiers(px):
         compare(px, eng, qs, ts, costs=(4, 6, 2), label="long")
 
 
+def test_block_tiers_for_wide_pairs(px):
+    """Pairs with qn + tn above 8192 run a workgroup per pair (256 or 1024
+    threads); low divergence keeps the oracle's full-width frames small."""
+    rng = random.Random(777)
+    qs, ts = [], []
+    for L, rate in ((5200, 0.03), (6000, 0.01), (9000, 0.01), (9500, 0.005)):
+        q = rand_seq(rng, L)
+        qs.append(q)
+        ts.append(mutate(rng, q, rate) + rand_seq(rng, L // 20))
+    for eng in ("affine", "linear"):
+        compare(px, eng, qs, ts, costs=(4, 6, 2), label="block-tier")
+
+
 def test_tiny_tiers_force_every_escalation(px):
     """A context with deliberately small tiers makes most pairs move through
     several tiers; results must not depend on the route."""
-    ctx = px.Context(0, dict(wave_wcap=16, wave_hist_kib=1, wave_slots=64, block_wcap=64,
-                             block_hist_kib=8, block_slots=16, hbm_budget_mib=2048))
+    ctx = px.Context(0, dict(wave_lds_kib=1, wave_slots=64, hbm_slots=16, hbm_slot_mib=1,
+                             hbm_budget_mib=2048))
     qs, ts, _ = random_batch(4321, 200, 400)
     for eng in ("affine", "linear"):
         compare(px, eng, qs, ts, costs=(4, 6, 2), ctx=ctx, label="tiny-tiers")
