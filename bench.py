@@ -41,30 +41,71 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(w, engine_name, costs, seconds, max_pairs=None):
-    """Time the oracle on all host cores over repeated passes of a bounded sample."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, engine_name, costs, seconds, sample_n=None):
+    """Time the oracle (the C++ restatement of phraya-align's CPU engines) on the
+    host's cores over repeated passes of a bounded, seeded random sample of the
+    workload; also a one-thread figure on a smaller sample."""
     from oracle import oracle as O
-    eng = {"affine": O.AFFINE, "linear": O.LINEAR, "myers_dist": O.MYERS_FWD,
+    # phraya's myers_edit_distance computes the CIGAR too (wfa_simd.rs:3657-3673),
+    # so the Myers baseline is the CIGAR-producing engine; forward-only is extra.
+    eng = {"affine": O.AFFINE, "linear": O.LINEAR, "myers_dist": O.MYERS_GLOBAL,
            "myers": O.MYERS}[engine_name]
     threads = len(os.sched_getaffinity(0))
-    sub = w
-    note = ""
-    if max_pairs is not None and w.n > max_pairs:
-        sub = w.subset(np.arange(max_pairs))
-        note = f"first {max_pairs} pairs"
-    passes, pairs, t0 = 0, 0, time.perf_counter()
-    while True:
-        O.batch(eng, sub.q_seq, sub.q_off, sub.t_seq, sub.t_off, costs=costs, threads=threads,
-                want_cigars=False)
-        passes += 1
-        pairs += sub.n
-        el = time.perf_counter() - t0
-        if el >= seconds:
-            break
-    return {"value": pairs / el, "unit": "pairs/s", "cores": threads, "kind": "port",
-            "sample": f"{passes} passes over {sub.n} pairs of the same workload"
-                      f"{' (' + note + ')' if note else ''}, {el:.1f} s wall, oracle/ C++ "
-                      f"restatement of phraya-align, -O3, SSE2 match-extend"}
+    sub, note = w, "the whole batch"
+    if sample_n is not None and w.n > sample_n:
+        idx = np.sort(np.random.default_rng(12345).choice(w.n, sample_n, replace=False))
+        sub = w.subset(idx)
+        note = f"a seeded uniform random sample of {sample_n} of the {w.n} pairs"
+
+    def timed(e, t, budget, subset):
+        passes, pairs, t0 = 0, 0, time.perf_counter()
+        while True:
+            O.batch(e, subset.q_seq, subset.q_off, subset.t_seq, subset.t_off, costs=costs,
+                    threads=t, want_cigars=True)
+            passes += 1
+            pairs += subset.n
+            el = time.perf_counter() - t0
+            if el >= budget:
+                return pairs / el, passes, el
+
+    rate, passes, el = timed(eng, threads, seconds, sub)
+    one_sub = sub.subset(np.arange(min(sub.n, max(200, sub.n // 16))))
+    rate1, _, _ = timed(eng, 1, max(2.0, seconds / 5), one_sub)
+    out = {"value": rate, "unit": "pairs/s", "cores": threads, "kind": "port",
+           "sample": f"{passes} passes over {note} ({sub.n} pairs), {el:.1f} s wall, "
+                     f"CIGAR strings formatted; oracle/ C++ restatement of phraya-align "
+                     f"(-O3, SSE2 match-extend), one pair per task over {threads} threads",
+           "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+           "one_thread_pairs_per_s": rate1}
+    if engine_name == "myers_dist":
+        fwd, _, _ = timed(O.MYERS_FWD, threads, max(2.0, seconds / 5), sub)
+        out["forward_only_pairs_per_s"] = fwd
+    return out
+
+
+def calibrate_copy(nbytes: int) -> None:
+    """A plain device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE
+    counters of the alignment kernels (run under rocprofv3 --pmc)."""
+    import torch
+    n = nbytes // 4
+    src = torch.ones(n, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}),
+          flush=True)
 
 
 def main():
@@ -77,7 +118,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", type=int, default=2000, help="pairs checked against the oracle")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -150,7 +196,7 @@ def main():
     if not args.no_cpu:
         log(f"cpu baseline ~{args.cpu_seconds:.0f} s")
         cpu = cpu_baseline(w, engine, costs, args.cpu_seconds,
-                           max_pairs=20_000 if n > 20_000 else None)
+                           sample_n={"short150": None, "myers150": 20_000, "mixed": 500}[args.workload])
 
     out = {
         "metric": METRIC,
