@@ -158,6 +158,74 @@ int phx_myers_edit_distance(phx_ctx* ctx, const uint8_t* q, size_t qlen, const u
                             size_t tlen, int64_t* edit_distance, char* cigar, size_t cigar_cap,
                             size_t* cigar_len);
 
+/* ---- Request server (continuous batching) -------------------------------
+ *
+ * Serves a stream of extension requests as they arrive, the per-anchor calls
+ * of extend_anchor (phraya-align/src/executor.rs:379-404) made by many reads
+ * at once.  A native scheduler thread admits each request against an HBM
+ * budget for wavefront state, using a per-request estimate of its history
+ * (O(s^2) cells for an s-step search).  It launches admitted requests in
+ * length-tiered groups on their own streams: short reads, long reads and
+ * the widest pairs each have a lane, so a short read never waits behind a
+ * long one.  Results are delivered as soon as their group completes.
+ * Targets are either explicit or windows of a reference kept resident in HBM.
+ * Every result equals the one phx_align_batch / phx_wfa_extend_affine gives
+ * for the same pair alone.  Requests are independent: one that fails (bad
+ * seed, no memory) reports its own status and never affects another. */
+typedef struct phx_server phx_server;
+
+typedef struct phx_serve_config {
+  int32_t engine;            /* PHX_ENGINE_AFFINE or PHX_ENGINE_LINEAR */
+  phx_costs costs;           /* affine costs */
+  int64_t cap;               /* max search score (upstream max_s_cap); < 0 = none */
+  const uint8_t* reference;  /* copied to HBM at start; NULL = no reference */
+  uint64_t reference_len;
+  uint64_t hbm_budget_mib;   /* HBM for per-request wavefront state; 0 = default */
+  uint32_t max_group;        /* most requests per launch; 0 = default (8192) */
+  uint32_t lanes_per_tier;   /* launch groups in flight per tier; 0 = default (1) */
+} phx_serve_config;
+
+typedef struct phx_request {
+  uint64_t tag;              /* echoed in the response */
+  const uint8_t* query;
+  uint64_t query_len;
+  const uint8_t* target;     /* explicit target, or NULL for a reference window */
+  uint64_t target_len;
+  uint64_t ref_start;        /* window [ref_start, ref_start + ref_len) when target is NULL */
+  uint64_t ref_len;
+  uint64_t seed_q, seed_t;   /* SeedAnchor into the query and the target (window) */
+} phx_request;
+
+typedef struct phx_response {
+  uint64_t tag;
+  phx_alignment aln;         /* positions are relative to the query and target (window) */
+  double latency_ms;         /* from submission to the result being ready on the host */
+} phx_response;
+
+typedef struct phx_serve_stats {
+  uint64_t submitted, completed, waiting, in_flight;
+  uint64_t launches, reruns;
+  uint64_t arena_bytes, arena_used_bytes, arena_peak_bytes;
+} phx_serve_stats;
+
+/* Start a server on ctx's device (own streams; ctx stays usable).  NULL on
+ * failure (phx_ctx_last_error says why). */
+phx_server* phx_serve_start(phx_ctx* ctx, const phx_serve_config* cfg);
+/* Queue n requests (sequences are copied; the call does not block on the
+ * GPU).  PHX_E_ARG if a reference window lies outside the reference. */
+int phx_serve_submit(phx_server* srv, const phx_request* reqs, uint32_t n);
+/* Take up to max_out finished responses, waiting up to timeout_us for the
+ * first one (0 = do not wait).  Response i's CIGAR runs go to
+ * runs[run_off[i] .. run_off[i+1]); responses whose runs do not fit stay
+ * queued (PHX_E_CAPACITY when not even the first one fits; run_off[0] then
+ * holds the words it needs). */
+int phx_serve_poll(phx_server* srv, phx_response* out, uint32_t max_out, uint32_t* runs,
+                   uint64_t runs_cap, uint64_t* run_off, uint32_t* n_out, int32_t timeout_us);
+int phx_serve_stats_get(phx_server* srv, phx_serve_stats* st);
+/* Finish every submitted request, then stop the scheduler and free the
+ * server (responses not yet polled are dropped). */
+int phx_serve_stop(phx_server* srv);
+
 /* Library version string. */
 const char* phx_version(void);
 

@@ -12,7 +12,8 @@ over the C ABI of libphraya_hip.so:
 Each returns what the upstream returns (an Alignment, or raises WfaError's
 InvalidInput / AlignmentFailed where the upstream returns Err), computed on
 the GPU.  align_batch() is the batched form used by a batched executor; Plan
-keeps a batch resident in HBM for repeated launches.
+keeps a batch resident in HBM for repeated launches; Server serves requests
+as they arrive (continuous batching against a resident reference).
 """
 from __future__ import annotations
 
@@ -30,6 +31,7 @@ __all__ = [
     "Context", "wfa_extend", "wfa_extend_naive", "wfa_extend_affine", "myers_extend",
     "myers_edit_distance", "align_batch", "BatchResult", "Plan", "pack_batch", "format_cigar",
     "ENGINE_LINEAR", "ENGINE_AFFINE", "ENGINE_MYERS", "ENGINE_MYERS_GLOBAL", "ENGINE_MYERS_DIST",
+    "Server", "Response",
 ]
 
 ENGINE_LINEAR = L.ENGINE_LINEAR
@@ -308,3 +310,157 @@ def myers_edit_distance(query: bytes, target: bytes) -> tuple[int, str]:
     res = align_batch(ENGINE_MYERS_GLOBAL, [bytes(query)], [bytes(target)])
     a = res.alignment(0)
     return a.edit_distance, a.cigar
+
+
+# ---------------------------------------------------------------------- serving
+
+_REQ_DTYPE = np.dtype([(n, np.uint64) for n in ("tag", "query", "query_len", "target",
+                                                "target_len", "ref_start", "ref_len",
+                                                "seed_q", "seed_t")])
+_RESP_DTYPE = np.dtype([("tag", np.uint64), ("status", np.int32), ("reserved", np.int32),
+                        ("score", np.int64), ("edit_distance", np.int64),
+                        ("query_start", np.int64), ("query_end", np.int64),
+                        ("target_start", np.int64), ("target_end", np.int64),
+                        ("latency_ms", np.float64)])
+assert _REQ_DTYPE.itemsize == ctypes.sizeof(L.CRequest)
+assert _RESP_DTYPE.itemsize == ctypes.sizeof(L.CResponse)
+
+
+@dataclass
+class Response:
+    """One served result: the pair's status and alignment fields, its CIGAR runs
+    and the time from submission to the result being ready."""
+    tag: int
+    status: int
+    score: int
+    edit_distance: int
+    query_start: int
+    query_end: int
+    target_start: int
+    target_end: int
+    runs: np.ndarray
+    latency_ms: float
+
+    @property
+    def cigar(self) -> str:
+        return format_cigar(self.runs)
+
+    def alignment(self) -> Alignment:
+        if self.status == L.PHX_INVALID_INPUT:
+            raise InvalidInput("Seed position beyond sequence length")
+        if self.status == L.PHX_ALIGNMENT_FAILED:
+            raise AlignmentFailed("alignment abandoned: max edit distance exceeded")
+        if self.status != L.PHX_OK:
+            raise RuntimeError(f"request {self.tag}: status {self.status}")
+        return Alignment(self.cigar, self.edit_distance, self.query_start, self.query_end,
+                         self.target_start, self.target_end, self.score)
+
+
+class Server:
+    """Request server (phx_serve_*): requests are admitted against an HBM budget
+    for wavefront state and launched in length tiers as they arrive; results come
+    back through poll() in completion order."""
+
+    def __init__(self, engine: int = ENGINE_AFFINE, costs: AffineCosts = AffineCosts(),
+                 cap: Optional[int] = None, reference=None, hbm_budget_mib: int = 0,
+                 max_group: int = 0, lanes_per_tier: int = 0, ctx: Optional[Context] = None):
+        self._ctx = ctx or default_context()
+        ref = None if reference is None else np.frombuffer(bytes(reference), np.uint8) \
+            if not isinstance(reference, np.ndarray) else np.ascontiguousarray(reference, np.uint8)
+        cfg = L.CServeConfig(engine, L.Costs(costs.mismatch, costs.gap_open, costs.gap_extend),
+                             -1 if cap is None else int(cap),
+                             None if ref is None else ref.ctypes.data,
+                             0 if ref is None else len(ref), hbm_budget_mib, max_group,
+                             lanes_per_tier)
+        self._h = L.lib().phx_serve_start(self._ctx.handle, ctypes.byref(cfg))
+        if not self._h:
+            raise RuntimeError(f"phx_serve_start failed: {self._ctx.last_error()}")
+        self._runs = np.zeros(1 << 16, np.uint32)
+
+    def submit_arrays(self, tags, q_seq: np.ndarray, q_off: np.ndarray, ref_start=None,
+                      ref_len=None, t_seq: Optional[np.ndarray] = None,
+                      t_off: Optional[np.ndarray] = None, seed_q=None, seed_t=None,
+                      idx=None) -> None:
+        """Submit requests whose queries are q_seq[q_off[i]:q_off[i+1]] and whose
+        targets are reference windows (ref_start, ref_len) or t_seq slices; `idx`
+        selects which i to submit."""
+        q_seq = np.ascontiguousarray(q_seq, np.uint8)
+        q_off = np.asarray(q_off, np.uint64)
+        idx = np.arange(len(q_off) - 1) if idx is None else np.asarray(idx)
+        r = np.zeros(len(idx), _REQ_DTYPE)
+        r["tag"] = np.asarray(tags, np.uint64) if np.ndim(tags) else np.uint64(tags)
+        r["query"] = np.uint64(q_seq.ctypes.data) + q_off[idx]
+        r["query_len"] = q_off[idx + 1] - q_off[idx]
+        if t_seq is not None:
+            t_seq = np.ascontiguousarray(t_seq, np.uint8)
+            t_off = np.asarray(t_off, np.uint64)
+            r["target"] = np.uint64(t_seq.ctypes.data) + t_off[idx]
+            r["target_len"] = t_off[idx + 1] - t_off[idx]
+        else:
+            r["ref_start"] = np.asarray(ref_start, np.uint64)[idx]
+            r["ref_len"] = np.asarray(ref_len, np.uint64)[idx]
+        if seed_q is not None:
+            r["seed_q"] = np.asarray(seed_q, np.uint64)[idx]
+        if seed_t is not None:
+            r["seed_t"] = np.asarray(seed_t, np.uint64)[idx]
+        rc = L.lib().phx_serve_submit(self._h, r.ctypes.data, len(r))
+        if rc != L.PHX_OK:
+            raise ValueError(f"phx_serve_submit: status {rc}")
+
+    def submit(self, tag: int, query: bytes, target: Optional[bytes] = None,
+               ref_start: int = 0, ref_len: int = 0, seed: SeedAnchor = SeedAnchor()) -> None:
+        q = np.frombuffer(bytes(query), np.uint8) if len(query) else np.zeros(1, np.uint8)
+        qo = np.array([0, len(query)], np.uint64)
+        if target is not None:
+            t = np.frombuffer(bytes(target), np.uint8) if len(target) else np.zeros(1, np.uint8)
+            self.submit_arrays([tag], q, qo, t_seq=t, t_off=np.array([0, len(target)], np.uint64),
+                               seed_q=[seed.query_pos], seed_t=[seed.target_pos])
+        else:
+            self.submit_arrays([tag], q, qo, ref_start=[ref_start], ref_len=[ref_len],
+                               seed_q=[seed.query_pos], seed_t=[seed.target_pos])
+
+    def poll_raw(self, max_out: int = 4096, timeout_us: int = 0):
+        """Finished responses as (structured array, runs, run_off)."""
+        out = np.zeros(max_out, _RESP_DTYPE)
+        off = np.zeros(max_out + 1, np.uint64)
+        n = ctypes.c_uint32(0)
+        while True:
+            rc = L.lib().phx_serve_poll(self._h, out.ctypes.data, max_out, self._runs.ctypes.data,
+                                        len(self._runs), off.ctypes.data, ctypes.byref(n),
+                                        int(timeout_us))
+            if rc == L.PHX_E_CAPACITY:
+                self._runs = np.zeros(max(2 * len(self._runs), int(off[0]) + 1), np.uint32)
+                continue
+            if rc != L.PHX_OK:
+                raise RuntimeError(f"phx_serve_poll: status {rc}")
+            k = n.value
+            return out[:k].copy(), self._runs[:int(off[k])].copy(), off[:k + 1].copy()
+
+    def poll(self, max_out: int = 4096, timeout_us: int = 0) -> list[Response]:
+        out, runs, off = self.poll_raw(max_out, timeout_us)
+        return [Response(int(o["tag"]), int(o["status"]), int(o["score"]), int(o["edit_distance"]),
+                         int(o["query_start"]), int(o["query_end"]), int(o["target_start"]),
+                         int(o["target_end"]), runs[int(off[i]):int(off[i + 1])],
+                         float(o["latency_ms"])) for i, o in enumerate(out)]
+
+    def stats(self) -> dict:
+        st = L.CServeStats()
+        L.lib().phx_serve_stats_get(self._h, ctypes.byref(st))
+        return {name: getattr(st, name) for name, _ in L.CServeStats._fields_}
+
+    def stop(self) -> None:
+        if getattr(self, "_h", None):
+            L.lib().phx_serve_stop(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.stop()
+
+    def __del__(self):
+        try:
+            self.stop()
+        except Exception:
+            pass

@@ -34,7 +34,8 @@ EXPORTS = (
     "phx_plan_last_ms", "phx_plan_last_kernel_ms", "phx_plan_last_launches",
     "phx_plan_algorithmic_bytes", "phx_plan_destroy", "phx_format_cigar",
     "phx_wfa_extend", "phx_wfa_extend_affine", "phx_myers_extend", "phx_myers_edit_distance",
-    "phx_version",
+    "phx_serve_start", "phx_serve_submit", "phx_serve_poll", "phx_serve_stats_get",
+    "phx_serve_stop", "phx_version",
 )
 
 
@@ -74,6 +75,31 @@ class CTuning(ctypes.Structure):
     _fields_ = [("wave_lds_kib", ctypes.c_uint32), ("wave_slots", ctypes.c_uint32),
                 ("hbm_slots", ctypes.c_uint32), ("hbm_slot_mib", ctypes.c_uint32),
                 ("hbm_budget_mib", ctypes.c_uint64)]
+
+
+class CServeConfig(ctypes.Structure):
+    _fields_ = [("engine", ctypes.c_int32), ("costs", Costs), ("cap", ctypes.c_int64),
+                ("reference", ctypes.c_void_p), ("reference_len", ctypes.c_uint64),
+                ("hbm_budget_mib", ctypes.c_uint64), ("max_group", ctypes.c_uint32),
+                ("lanes_per_tier", ctypes.c_uint32)]
+
+
+class CRequest(ctypes.Structure):
+    _fields_ = [("tag", ctypes.c_uint64), ("query", ctypes.c_void_p),
+                ("query_len", ctypes.c_uint64), ("target", ctypes.c_void_p),
+                ("target_len", ctypes.c_uint64), ("ref_start", ctypes.c_uint64),
+                ("ref_len", ctypes.c_uint64), ("seed_q", ctypes.c_uint64),
+                ("seed_t", ctypes.c_uint64)]
+
+
+class CResponse(ctypes.Structure):
+    _fields_ = [("tag", ctypes.c_uint64), ("aln", CAlignment), ("latency_ms", ctypes.c_double)]
+
+
+class CServeStats(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_uint64) for name in
+                ("submitted", "completed", "waiting", "in_flight", "launches", "reruns",
+                 "arena_bytes", "arena_used_bytes", "arena_peak_bytes")]
 
 
 _lib = None
@@ -118,6 +144,12 @@ def lib() -> ctypes.CDLL:
     L.phx_wfa_extend_affine.argtypes = [P, P, sz, P, sz, u64, u64, Costs, P, P, sz, P]
     L.phx_myers_extend.argtypes = [P, P, sz, P, sz, u64, u64, P, P, sz, P]
     L.phx_myers_edit_distance.argtypes = [P, P, sz, P, sz, P, P, sz, P]
+    L.phx_serve_start.argtypes = [P, P]
+    L.phx_serve_start.restype = P
+    L.phx_serve_submit.argtypes = [P, P, ctypes.c_uint32]
+    L.phx_serve_poll.argtypes = [P, P, ctypes.c_uint32, P, u64, P, P, ctypes.c_int32]
+    L.phx_serve_stats_get.argtypes = [P, P]
+    L.phx_serve_stop.argtypes = [P]
     L.phx_version.restype = ctypes.c_char_p
     _lib = L
     return L

@@ -128,8 +128,8 @@ __global__ __launch_bounds__(256) void myers_kernel(MyersArgs a) {
   if (w >= a.nwork) return;
   const uint32_t pi = a.work_list[w];
   const PairDesc pd = a.pairs[pi];
-  const uint64_t* qw = a.seq + pd.q_word;
-  const uint64_t* tw = a.seq + pd.t_word;
+  const uint64_t* qw = pd.q;
+  const uint64_t* tw = pd.t;
   const uint32_t m = (uint32_t)pd.qn, n = (uint32_t)pd.tn;
   const bool b8 = pd.flags & PF_BYTES;
   const bool with_cigar = a.engine != ENG_MYERS_DIST;

@@ -19,98 +19,19 @@
 #include "phraya_hip.h"
 #include "phx_device.h"
 
-namespace phx {
-hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t st);
-hipError_t launch_myers(int nb_template, const MyersArgs& a, hipStream_t st);
-hipError_t launch_wfa_wave(int mode, bool lds, const WfaArgs& a, const WaveGeom& G,
-                           uint32_t grid, hipStream_t st);
-hipError_t launch_wfa_block(int mode, int nt, const WfaArgs& a, uint32_t grid, hipStream_t st);
-}  // namespace phx
+#include "phx_host.h"
 
 using namespace phx;
+using namespace phx::host;
 
 namespace {
-
-constexpr int kRingSlotsMax = 128;  // WfaShared metadata capacity
-constexpr int32_t kMaxLen = (1 << 27) - 2;  // history cells hold offset + 1 in 28 bits
-constexpr uint64_t kWaveMaxLen = 2048;     // qn + tn limit of the wave-resident tier
-constexpr int kWaveRingMax = 64;           // ring slots the wave tier keeps in lane registers
-constexpr uint32_t kWaveSteps = 128;       // step-table entries per wave
-constexpr int kWaveWaves = 4;              // waves per workgroup of the wave tier
 
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
 };
 
-uint8_t acgt_code[256];
-bool acgt_init = [] {
-  std::memset(acgt_code, 0xFF, sizeof(acgt_code));
-  acgt_code['A'] = 0;
-  acgt_code['C'] = 1;
-  acgt_code['G'] = 2;
-  acgt_code['T'] = 3;
-  return true;
-}();
-
-bool is_acgt(const uint8_t* s, size_t n) {
-  for (size_t i = 0; i < n; ++i)
-    if (acgt_code[s[i]] == 0xFF) return false;
-  return true;
-}
-
-uint64_t words_for(size_t n, bool b8) { return (b8 ? (n + 7) / 8 : (n + 31) / 32) + 1; }
-
-void pack_into(uint64_t* w, const uint8_t* s, size_t n, bool b8) {
-  const uint64_t nw = words_for(n, b8);
-  std::memset(w, 0, nw * 8);
-  if (b8) {
-    std::memcpy(w, s, n);
-    return;
-  }
-  for (size_t i = 0; i < n; ++i) w[i >> 5] |= (uint64_t)acgt_code[s[i]] << ((i & 31) * 2);
-}
-
-uint32_t next_pow2(uint64_t v) {
-  uint32_t p = 1;
-  while (p < v && p < (1u << 30)) p <<= 1;
-  return p;
-}
-
-uint32_t log2u(uint32_t v) {
-  uint32_t l = 0;
-  while ((1u << l) < v) ++l;
-  return l;
-}
-
-int32_t gcd3(int32_t a, int32_t b, int32_t c) {
-  auto g2 = [](int32_t x, int32_t y) {
-    while (y) {
-      int32_t r = x % y;
-      x = y;
-      y = r;
-    }
-    return x;
-  };
-  return g2(g2(a, b), c);
-}
-
 }  // namespace
-
-constexpr int kStreams = 4;  // launch groups of one run overlap on these
-
-struct phx_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;         // = streams[0]
-  hipStream_t streams[kStreams] = {};
-  hipEvent_t join[32] = {};             // cross-stream joins
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  std::vector<hipEvent_t> kev;  // pairs of events around each kernel launch of a run
-  std::mutex mu;
-  phx_tuning tun{};
-  std::string err;
-  int cu_count = 256;
-};
 
 struct phx_plan {
   phx_ctx* ctx = nullptr;
@@ -196,50 +117,13 @@ int kernel_events(phx_ctx* ctx, uint32_t i, hipEvent_t* a, hipEvent_t* b) {
 bool is_wfa(int engine) { return engine == PHX_ENGINE_LINEAR || engine == PHX_ENGINE_AFFINE; }
 bool has_cigar(int engine) { return engine != PHX_ENGINE_MYERS_DIST; }
 
-// WFA tiers.  Tier 0: wave per pair, everything in LDS (short pairs).
-// Tier 1: wave per pair, ring and history in an HBM slot of fixed size.
-// Tier 2: wave per pair, HBM slots sized exactly from the step count the pair
-// reported when it overflowed tier 1.
-
-// sum_{u=1..S} min(u, a)
-uint64_t sum_min(uint64_t S, uint64_t a) {
-  return S <= a ? S * (S + 1) / 2 : a * (a + 1) / 2 + (S - a) * a;
-}
-
-// History words a pair needs for `steps` score steps: cells of every step
-// (width min(u, qn) + min(u, tn) + 1) plus one offset word per step.
-uint64_t exact_hist_words(int32_t steps, int32_t qn, int32_t tn) {
-  const uint64_t S = (uint64_t)std::max(steps, 0);
-  return sum_min(S, (uint64_t)qn) + sum_min(S, (uint64_t)tn) + 2 * S + 16;
-}
-
-// Generous estimate of the score steps of a pair, used to size HBM history
-// slots before the run: an end gap for the length difference plus 9% events
-// per base (40% mismatches, 60% indels of mean length 2), +15%.
-int32_t estimate_steps(const phx_plan* P, int32_t qn, int32_t tn) {
-  const double m = std::max(qn, tn), d = std::abs(tn - qn);
-  double steps;
-  if (P->engine == PHX_ENGINE_AFFINE) {
-    const phx_costs& c = P->costs;
-    const int32_t g = gcd3(c.mismatch, c.gap_open, c.gap_extend);
-    const double score = c.gap_open + c.gap_extend * d +
-                         0.09 * m * (0.4 * c.mismatch + 0.6 * (c.gap_open + 2.0 * c.gap_extend));
-    steps = score / g;
-  } else {
-    steps = d + 0.09 * m;
-  }
-  return (int32_t)std::min(1.15 * steps + 16.0, (double)(qn + tn + 16));
-}
-
 // Launches one group: tier t (0 LDS, 1 HBM) over `work`, scratch buffers of
 // slot `buf`, on stream `st`.
 int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint32_t* d_list,
                     uint32_t* d_counters, uint64_t slot_bytes, int buf, hipStream_t st) {
   phx_ctx* ctx = P->ctx;
-  const int layers = P->engine == PHX_ENGINE_AFFINE ? 3 : 1;
   const int R = P->ring_slots;
   WfaArgs a{};
-  a.seq = (const uint64_t*)P->d_seq.p;
   a.pairs = (const PairDesc*)P->d_pairs.p;
   a.work_list = d_list;
   a.nwork = (uint32_t)work.size();
@@ -250,63 +134,39 @@ int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint3
   a.runs = (uint32_t*)P->d_runs.p;
   const int mode = P->engine == PHX_ENGINE_AFFINE ? ENG_AFFINE : ENG_LINEAR;
   WaveGeom G{};
-  uint32_t waves = 0;
-  if (t == 0) {
+  uint32_t slots = 0;
+  if (t == TIER_LDS) {
     uint32_t seq_words = 2;
-    for (uint32_t i : work) {
-      const PairDesc& pd = P->pairs[i];
-      const bool b8 = pd.flags & PF_BYTES;
-      seq_words = std::max(seq_words, (uint32_t)(words_for((size_t)pd.qn, b8) + words_for((size_t)pd.tn, b8)));
-    }
-    auto al = [](uint32_t v) { return (v + 15u) & ~15u; };
-    G.ring_off = 0;
-    const uint32_t ring_bytes = (uint32_t)(P->engine == PHX_ENGINE_AFFINE ? 3 * R : 2) * 128u * 2u;
-    G.seq_off = al(ring_bytes);
-    G.seq_cap_words = seq_words;
-    G.tab_off = G.seq_off + al(seq_words * 8);
-    G.steps_cap = kWaveSteps;
-    G.cell_off = G.tab_off + al(2 * G.steps_cap);
-    // History cells fill the wave's share of the LDS budget (wave_lds_kib).
-    const uint32_t budget = std::max<uint32_t>(ctx->tun.wave_lds_kib * 1024u,
-                                               std::min<uint32_t>(G.cell_off + 4096u, 15u * 1024u));
-    const uint32_t cell_bytes = budget > G.cell_off + 256 ? (budget - G.cell_off) & ~15u : 256u;
-    G.cells_cap = std::min<uint32_t>(65535, cell_bytes / 2);
-    G.per_wave_bytes = G.cell_off + al(2 * G.cells_cap);
-    waves = std::min<uint32_t>((uint32_t)work.size(), std::max<uint32_t>(kWaveWaves, ctx->tun.wave_slots));
+    for (uint32_t i : work) seq_words = std::max(seq_words, lds_seq_words(P->pairs[i]));
+    G = lds_geometry(P->engine, R, seq_words, ctx->tun.wave_lds_kib);
+    slots = std::min<uint32_t>((uint32_t)work.size(), std::max<uint32_t>(kWaveWaves, ctx->tun.wave_slots));
   } else {
     uint64_t max_sum = 1;
     for (uint32_t i : work) max_sum = std::max<uint64_t>(max_sum, (uint64_t)P->pairs[i].qn + P->pairs[i].tn);
-    const uint32_t wcap = next_pow2(max_sum + 1);
-    const uint64_t ring_words = (uint64_t)(P->engine == PHX_ENGINE_AFFINE ? 3 * R : 2) * wcap;
+    uint32_t wcap = 0;
+    const uint64_t ring_words = ring_words_for(P->engine, R, max_sum, &wcap);
     const uint64_t hist = slot_bytes ? slot_bytes : ((uint64_t)ctx->tun.hbm_slot_mib << 20);
     const uint64_t per = hist + ring_words * 4;
     const uint64_t budget = P->group_budget(buf);
     // Pairs in flight: one per wave (tier 1) or per workgroup (tiers 2-3).
-    const uint64_t max_flight = t == 1 ? ctx->tun.hbm_slots
-                              : t == 2 ? (uint64_t)ctx->cu_count * 4 : (uint64_t)ctx->cu_count;
-    uint64_t slots = std::max<uint64_t>(1, budget / per);
-    slots = std::min<uint64_t>(std::min<uint64_t>(slots, max_flight), work.size());
-    const uint64_t per_wg = t == 1 ? kWaveWaves : 1;
-    const uint64_t wgs = (slots + per_wg - 1) / per_wg;
-    if (dev_alloc(ctx, P->d_hist[buf], (size_t)(wgs * per_wg * hist)) != PHX_OK) return PHX_E_NOMEM;
-    if (dev_alloc(ctx, P->d_ring[buf], (size_t)(wgs * per_wg * ring_words * 4)) != PHX_OK) return PHX_E_NOMEM;
+    const uint64_t max_flight = t == TIER_WAVE ? ctx->tun.hbm_slots
+                              : t == TIER_BLOCK256 ? (uint64_t)ctx->cu_count * 4 : (uint64_t)ctx->cu_count;
+    uint64_t n = std::max<uint64_t>(1, budget / per);
+    n = std::min<uint64_t>(std::min<uint64_t>(n, max_flight), work.size());
+    if (t == TIER_WAVE) n = (n + kWaveWaves - 1) / kWaveWaves * kWaveWaves;  // whole workgroups
+    if (dev_alloc(ctx, P->d_hist[buf], (size_t)(n * hist)) != PHX_OK) return PHX_E_NOMEM;
+    if (dev_alloc(ctx, P->d_ring[buf], (size_t)(n * ring_words * 4)) != PHX_OK) return PHX_E_NOMEM;
     a.hist = (uint8_t*)P->d_hist[buf].p;
     a.hist_stride = hist;
     a.ring_g = (int32_t*)P->d_ring[buf].p;
     a.ring_stride = ring_words;
     a.wcap_log2 = log2u(wcap);
-    waves = (uint32_t)(wgs * per_wg);
+    slots = (uint32_t)n;
   }
-  (void)layers;
   hipEvent_t e0, e1;
   if (kernel_events(ctx, P->last_launches, &e0, &e1) != PHX_OK) return PHX_E_DEVICE;
   HIPCHK(hipEventRecord(e0, st));
-  if (t <= 1) {
-    const uint32_t wgs = std::max<uint32_t>(1, (waves + kWaveWaves - 1) / kWaveWaves);
-    HIPCHK(launch_wfa_wave(mode, t == 0, a, G, wgs, st));
-  } else {
-    HIPCHK(launch_wfa_block(mode, t == 2 ? 256 : 1024, a, std::max<uint32_t>(1, waves), st));
-  }
+  HIPCHK(launch_tier(t, mode, a, G, slots, st));
   HIPCHK(hipEventRecord(e1, st));
   ++P->last_launches;
   return PHX_OK;
@@ -372,7 +232,8 @@ int run_wfa(phx_plan* P) {
     });
     uint64_t words = 0;
     for (uint32_t i : wl) {
-      const int32_t steps = est ? estimate_steps(P, P->pairs[i].qn, P->pairs[i].tn) : outs[i].steps;
+      const int32_t steps = est ? estimate_steps(P->engine, P->costs, P->pairs[i].qn, P->pairs[i].tn)
+                                : outs[i].steps;
       words = std::max(words, exact_hist_words(steps, P->pairs[i].qn, P->pairs[i].tn));
     }
     const uint64_t slot_bytes = (words * 4 + 255) & ~(uint64_t)255;
@@ -381,7 +242,7 @@ int run_wfa(phx_plan* P) {
     HIPCHK(launch_zero(d_ctr, 2, ctx->stream));
     uint64_t maxL = 0;
     for (uint32_t i : wl) maxL = std::max<uint64_t>(maxL, (uint64_t)P->pairs[i].qn + P->pairs[i].tn);
-    const int rt = maxL <= 8192 ? 1 : 3;
+    const int rt = maxL <= 8192 ? TIER_WAVE : TIER_BLOCK1024;
     int rc = launch_wfa_tier(P, rt, wl, (uint32_t*)P->d_work2.p, d_ctr, slot_bytes, 15, ctx->stream);
     if (rc != PHX_OK) return rc;
     HIPCHK(hipMemcpyAsync(ctr_h, d_ctr, 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -421,7 +282,6 @@ int run_myers(phx_plan* P) {
     for (uint64_t c0 = 0; c0 < wl.size(); c0 += chunk) {
       const uint64_t cn = std::min<uint64_t>(chunk, wl.size() - c0);
       MyersArgs a{};
-      a.seq = (const uint64_t*)P->d_seq.p;
       a.pairs = (const PairDesc*)P->d_pairs.p;
       a.work_list = d_list_base + list_off + c0;
       a.nwork = (uint32_t)cn;
@@ -535,8 +395,7 @@ phx_plan* phx_plan_create(phx_ctx* ctx, const phx_batch* b) {
       delete P;
       return nullptr;
     }
-    const int32_t g = gcd3(c.mismatch, c.gap_open, c.gap_extend);
-    P->ring_slots = std::max(c.mismatch, std::max(c.gap_open + c.gap_extend, c.gap_extend)) / g + 1;
+    P->ring_slots = ring_slots(b->engine, c);
     if (P->ring_slots > kRingSlotsMax) {
       ctx->err = "affine costs span more score frames than the device ring holds";
       delete P;
@@ -551,6 +410,7 @@ phx_plan* phx_plan_create(phx_ctx* ctx, const phx_batch* b) {
   P->pairs.assign(n, PairDesc{});
   // Pass 1: seeds, lengths, encodings, word and run offsets.
   std::vector<uint8_t> b8(n, 0);
+  std::vector<uint64_t> q_word(n, 0), t_word(n, 0);
   uint64_t words = 0, runs = 0;
   for (uint32_t i = 0; i < n; ++i) {
     const uint64_t qfull = b->q_off[i + 1] - b->q_off[i], tfull = b->t_off[i + 1] - b->t_off[i];
@@ -580,9 +440,9 @@ phx_plan* phx_plan_create(phx_ctx* ctx, const phx_batch* b) {
     pd.qn = (int32_t)qn;
     pd.tn = (int32_t)tn;
     pd.flags = b8[i] ? PF_BYTES : 0;
-    pd.q_word = words;
+    q_word[i] = words;
     words += words_for(qn, b8[i]);
-    pd.t_word = words;
+    t_word[i] = words;
     words += words_for(tn, b8[i]);
     pd.run_cap = has_cigar(b->engine) ? (uint32_t)(qn + tn + 1) : 0;
     pd.run_off = runs;
@@ -595,9 +455,8 @@ phx_plan* phx_plan_create(phx_ctx* ctx, const phx_batch* b) {
   std::vector<uint64_t> packed(words ? words : 1);
   for (uint32_t i = 0; i < n; ++i) {
     if (P->host_status[i] == ST_INVALID_INPUT) continue;
-    const PairDesc& pd = P->pairs[i];
-    pack_into(packed.data() + pd.q_word, b->q_seq + b->q_off[i] + P->seed_q[i], P->qlen[i], b8[i]);
-    pack_into(packed.data() + pd.t_word, b->t_seq + b->t_off[i] + P->seed_t[i], P->tlen[i], b8[i]);
+    pack_into(packed.data() + q_word[i], b->q_seq + b->q_off[i] + P->seed_q[i], P->qlen[i], b8[i]);
+    pack_into(packed.data() + t_word[i], b->t_seq + b->t_off[i] + P->seed_t[i], P->tlen[i], b8[i]);
   }
   // Work lists.
   std::vector<uint32_t> order;
@@ -624,7 +483,7 @@ phx_plan* phx_plan_create(phx_ctx* ctx, const phx_batch* b) {
       P->work[gi].push_back(i);
       if (gi > 0) {
         const int32_t qn = P->pairs[i].qn, tn = P->pairs[i].tn;
-        est_words[gi] = std::max(est_words[gi], exact_hist_words(estimate_steps(P, qn, tn), qn, tn));
+        est_words[gi] = std::max(est_words[gi], exact_hist_words(estimate_steps(P->engine, P->costs, qn, tn), qn, tn));
       }
     }
     for (int gi = 1; gi < 6; ++gi) P->group_slot[gi] = (est_words[gi] * 4 + 255) & ~(uint64_t)255;
@@ -637,7 +496,7 @@ phx_plan* phx_plan_create(phx_ctx* ctx, const phx_batch* b) {
     for (int gi = 1; gi < 6; ++gi) {
       for (uint32_t i : P->work[gi]) {
         const int32_t qn = P->pairs[i].qn, tn = P->pairs[i].tn;
-        vol[gi] += (double)exact_hist_words(estimate_steps(P, qn, tn), qn, tn);
+        vol[gi] += (double)exact_hist_words(estimate_steps(P->engine, P->costs, qn, tn), qn, tn);
       }
       tot += vol[gi];
     }
@@ -663,6 +522,11 @@ phx_plan* phx_plan_create(phx_ctx* ctx, const phx_batch* b) {
   if (dev_alloc(ctx, P->d_runs, runs * 4 + 4) != PHX_OK) return fail_plan("runs");
   if (dev_alloc(ctx, P->d_work, (size_t)std::max<uint32_t>(n, 1) * 4) != PHX_OK) return fail_plan("work");
   if (dev_alloc(ctx, P->d_ctr, 256) != PHX_OK) return fail_plan("counters");
+  // Descriptors address their sequences by device pointer.
+  for (uint32_t i = 0; i < n; ++i) {
+    P->pairs[i].q = (const uint64_t*)P->d_seq.p + q_word[i];
+    P->pairs[i].t = (const uint64_t*)P->d_seq.p + t_word[i];
+  }
   if (hipMemcpy(P->d_seq.p, packed.data(), words * 8, hipMemcpyHostToDevice) != hipSuccess ||
       (n && hipMemcpy(P->d_pairs.p, P->pairs.data(), sizeof(PairDesc) * n, hipMemcpyHostToDevice) != hipSuccess))
     return fail_plan("upload");

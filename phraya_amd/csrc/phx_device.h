@@ -1,7 +1,7 @@
 // phx_device.h -- device-side data layout shared by the HIP kernels and the
 // host runtime of libphraya_hip.so.
 //
-// Sequences live in one u64 word arena in HBM.  A pair's query and target are
+// Sequences live in u64 word arenas in HBM.  A pair's query and target are
 // each either 2-bit packed (A=0 C=1 G=2 T=3, 32 bases per word, base i in
 // bits [2i, 2i+2) of word i/32) or, when either sequence holds a byte outside
 // uppercase ACGT, 8-bit raw (8 bytes per word, byte i in bits [8i, 8i+8)).
@@ -39,15 +39,27 @@ enum PairFlags : uint32_t {
   PF_BYTES = 1u,  // 8-bit encoding
 };
 
+// A sequence is addressed by the device address of the word holding its
+// first symbol plus that symbol's index inside the word (shift: 0..31 for
+// 2-bit, 0..7 for 8-bit), so windows of a resident reference or contig arena
+// need no copy.  The word after the last symbol's word must be readable.
 struct PairDesc {
-  uint64_t q_word;    // word offset of the query in the sequence arena
-  uint64_t t_word;    // word offset of the target
+  const uint64_t* q;  // query words
+  const uint64_t* t;  // target words
   int32_t qn, tn;     // suffix lengths (the host applies the seed)
   uint32_t flags;
   uint32_t run_cap;   // CIGAR run capacity
   uint64_t run_off;   // offset (in runs) of this pair's run slots
-  uint64_t hist_off;  // byte offset of this pair's history region
-  uint64_t hist_cap;  // bytes in the history region
+  uint32_t q_shift;   // index of the first query symbol in q[0]
+  uint32_t t_shift;   // index of the first target symbol in t[0]
+  // Optional storage of its own for the HBM tiers (request server): history
+  // region of hist_bytes and a ring of (layers x R) << ring_log2 ints.  When
+  // hist is null the pair uses the slot of the wave / workgroup running it.
+  uint8_t* hist;
+  uint64_t hist_bytes;
+  int32_t* ring;
+  uint32_t ring_log2;
+  uint32_t reserved;
 };
 
 struct PairOut {
@@ -71,7 +83,6 @@ enum RunOp : uint32_t { OP_M = 0, OP_X = 1, OP_I = 2, OP_D = 3 };
 
 // Kernel arguments of one WFA tier launch.
 struct WfaArgs {
-  const uint64_t* seq;
   const PairDesc* pairs;
   const uint32_t* work_list;
   uint32_t nwork;
@@ -101,8 +112,7 @@ struct WaveGeom {
 
 // Kernel arguments of one Myers launch.
 struct MyersArgs {
-  const uint64_t* seq;
-  const PairDesc* pairs;
+  const PairDesc* pairs;  // Myers pairs are word aligned (shifts 0)
   const uint32_t* work_list;
   uint32_t nwork;
   PairOut* out;

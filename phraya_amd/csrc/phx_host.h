@@ -1,0 +1,216 @@
+// phx_host.h -- host-side helpers shared by the batch runtime (phx_api.hip)
+// and the request server (phx_serve.hip): sequence packing, the memory model
+// of a WFA pair (score steps and history cells), the LDS layout of the
+// wave-resident tier, and tier launches.
+#ifndef PHX_HOST_H
+#define PHX_HOST_H
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "phraya_hip.h"
+#include "phx_device.h"
+
+namespace phx {
+
+hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t st);
+hipError_t launch_myers(int nb_template, const MyersArgs& a, hipStream_t st);
+hipError_t launch_wfa_wave(int mode, bool lds, const WfaArgs& a, const WaveGeom& G,
+                           uint32_t grid, hipStream_t st);
+hipError_t launch_wfa_block(int mode, int nt, const WfaArgs& a, uint32_t grid, hipStream_t st);
+
+namespace host {
+
+constexpr int kRingSlotsMax = 128;          // score frames a ring may span
+constexpr int32_t kMaxLen = (1 << 27) - 2;  // history cells hold offset + 1 in 28 bits
+constexpr uint64_t kWaveMaxLen = 2048;      // qn + tn limit of the LDS tier
+constexpr int kWaveRingMax = 64;            // ring slots the LDS tier allows
+constexpr uint32_t kWaveSteps = 128;        // LDS tier step-table entries per wave
+constexpr int kWaveWaves = 4;               // waves per workgroup of the wave kernels
+constexpr uint32_t kWaveWidth = 128;        // LDS tier ring width (diagonals)
+
+// Tiers of the WFA kernels.
+enum Tier : int {
+  TIER_LDS = 0,      // wave per pair, everything in LDS
+  TIER_WAVE = 1,     // wave per pair, ring and history in an HBM slot
+  TIER_BLOCK256 = 2, // 256-thread workgroup per pair, HBM slot
+  TIER_BLOCK1024 = 3 // 1024-thread workgroup per pair, HBM slot
+};
+
+inline const uint8_t* acgt_table() {
+  static const std::array<uint8_t, 256> t = [] {
+    std::array<uint8_t, 256> a;
+    a.fill(0xFF);
+    a['A'] = 0;
+    a['C'] = 1;
+    a['G'] = 2;
+    a['T'] = 3;
+    return a;
+  }();
+  return t.data();
+}
+
+inline bool is_acgt(const uint8_t* s, size_t n) {
+  const uint8_t* code = acgt_table();
+  for (size_t i = 0; i < n; ++i)
+    if (code[s[i]] == 0xFF) return false;
+  return true;
+}
+
+// Words of an n-symbol sequence plus the zero pad word.
+inline uint64_t words_for(size_t n, bool b8) { return (b8 ? (n + 7) / 8 : (n + 31) / 32) + 1; }
+
+inline void pack_into(uint64_t* w, const uint8_t* s, size_t n, bool b8) {
+  const uint64_t nw = words_for(n, b8);
+  std::memset(w, 0, nw * 8);
+  if (b8) {
+    std::memcpy(w, s, n);
+    return;
+  }
+  const uint8_t* code = acgt_table();
+  for (size_t i = 0; i < n; ++i) w[i >> 5] |= (uint64_t)code[s[i]] << ((i & 31) * 2);
+}
+
+inline uint32_t next_pow2(uint64_t v) {
+  uint32_t p = 1;
+  while (p < v && p < (1u << 30)) p <<= 1;
+  return p;
+}
+
+inline uint32_t log2u(uint32_t v) {
+  uint32_t l = 0;
+  while ((1u << l) < v) ++l;
+  return l;
+}
+
+inline int32_t gcd3(int32_t a, int32_t b, int32_t c) {
+  auto g2 = [](int32_t x, int32_t y) {
+    while (y) {
+      const int32_t r = x % y;
+      x = y;
+      y = r;
+    }
+    return x;
+  };
+  return g2(g2(a, b), c);
+}
+
+// Score frames the affine ring spans: max(x, o+e, e)/g + 1 (linear: 2).
+inline int32_t ring_slots(int engine, const phx_costs& c) {
+  if (engine != PHX_ENGINE_AFFINE) return 2;
+  const int32_t g = gcd3(c.mismatch, c.gap_open, c.gap_extend);
+  return std::max(c.mismatch, std::max(c.gap_open + c.gap_extend, c.gap_extend)) / g + 1;
+}
+
+// sum_{u=1..S} min(u, a)
+inline uint64_t sum_min(uint64_t S, uint64_t a) {
+  return S <= a ? S * (S + 1) / 2 : a * (a + 1) / 2 + (S - a) * a;
+}
+
+// History words a pair needs for `steps` score steps: the cells of every step
+// (width min(u, qn) + min(u, tn) + 1) plus one offset word per step.
+inline uint64_t exact_hist_words(int32_t steps, int32_t qn, int32_t tn) {
+  const uint64_t S = (uint64_t)std::max(steps, 0);
+  return sum_min(S, (uint64_t)qn) + sum_min(S, (uint64_t)tn) + 2 * S + 16;
+}
+
+// Generous estimate of the score steps of a pair, used to size HBM history
+// slots before a run: an end gap for the length difference plus 9% events per
+// base (40% mismatches, 60% indels of mean length 2), +15%.  A pair that needs
+// more reports its exact step count and is re-run in a slot of that size.
+inline int32_t estimate_steps(int engine, const phx_costs& c, int32_t qn, int32_t tn) {
+  const double m = std::max(qn, tn), d = std::abs(tn - qn);
+  double steps;
+  if (engine == PHX_ENGINE_AFFINE) {
+    const int32_t g = gcd3(c.mismatch, c.gap_open, c.gap_extend);
+    const double score = c.gap_open + c.gap_extend * d +
+                         0.09 * m * (0.4 * c.mismatch + 0.6 * (c.gap_open + 2.0 * c.gap_extend));
+    steps = score / g;
+  } else {
+    steps = d + 0.09 * m;
+  }
+  return (int32_t)std::min(1.15 * steps + 16.0, (double)(qn + tn + 16));
+}
+
+// Ring words of one HBM slot for pairs with qn + tn <= max_sum.
+inline uint64_t ring_words_for(int engine, int32_t R, uint64_t max_sum, uint32_t* wcap_out) {
+  const uint32_t wcap = next_pow2(max_sum + 1);
+  if (wcap_out) *wcap_out = wcap;
+  return (uint64_t)(engine == PHX_ENGINE_AFFINE ? 3 * R : 2) * wcap;
+}
+
+// Bytes of an HBM history slot holding `steps` steps of a qn x tn pair.
+inline uint64_t hist_slot_bytes(int32_t steps, int32_t qn, int32_t tn) {
+  return (exact_hist_words(steps, qn, tn) * 4 + 255) & ~(uint64_t)255;
+}
+
+// LDS layout of one wave of the LDS tier: int16 ring [layers][R][128], the
+// packed query and target (max_seq_words words), the u16 step table and u16
+// history cells filling the wave's share of lds_kib.
+inline WaveGeom lds_geometry(int engine, int32_t R, uint32_t max_seq_words, uint32_t lds_kib) {
+  auto al = [](uint32_t v) { return (v + 15u) & ~15u; };
+  WaveGeom G{};
+  G.ring_off = 0;
+  const uint32_t ring_bytes = (uint32_t)(engine == PHX_ENGINE_AFFINE ? 3 * R : 2) * kWaveWidth * 2u;
+  G.seq_off = al(ring_bytes);
+  G.seq_cap_words = std::max<uint32_t>(max_seq_words, 2);
+  G.tab_off = G.seq_off + al(G.seq_cap_words * 8);
+  G.steps_cap = kWaveSteps;
+  G.cell_off = G.tab_off + al(2 * G.steps_cap);
+  const uint32_t budget = std::max<uint32_t>(lds_kib * 1024u, std::min<uint32_t>(G.cell_off + 4096u, 15u * 1024u));
+  const uint32_t cell_bytes = budget > G.cell_off + 256 ? (budget - G.cell_off) & ~15u : 256u;
+  G.cells_cap = std::min<uint32_t>(65535, cell_bytes / 2);
+  G.per_wave_bytes = G.cell_off + al(2 * G.cells_cap);
+  return G;
+}
+
+// Words the LDS tier stages for one pair (query then target, with shifts).
+inline uint32_t lds_seq_words(const PairDesc& pd) {
+  const bool b8 = pd.flags & PF_BYTES;
+  return (uint32_t)(words_for((size_t)pd.q_shift + (size_t)pd.qn, b8) +
+                    words_for((size_t)pd.t_shift + (size_t)pd.tn, b8));
+}
+
+// Launch one tier over a.work_list: `slots` pairs in flight (waves for the
+// wave tiers, workgroups for the block tiers).
+inline hipError_t launch_tier(int tier, int mode, const WfaArgs& a, const WaveGeom& G,
+                              uint32_t slots, hipStream_t st) {
+  if (tier <= TIER_WAVE) {
+    const uint32_t wgs = std::max<uint32_t>(1, (slots + kWaveWaves - 1) / kWaveWaves);
+    return launch_wfa_wave(mode, tier == TIER_LDS, a, G, wgs, st);
+  }
+  return launch_wfa_block(mode, tier == TIER_BLOCK256 ? 256 : 1024, a, std::max<uint32_t>(1, slots), st);
+}
+
+// Tier of an HBM pair by qn + tn: a wave up to 8192, then workgroups.
+inline int hbm_tier_for(uint64_t L) {
+  return L <= 8192 ? TIER_WAVE : L <= 16384 ? TIER_BLOCK256 : TIER_BLOCK1024;
+}
+
+}  // namespace host
+}  // namespace phx
+
+constexpr int kStreams = 4;  // launch groups of one run overlap on these
+
+struct phx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;         // = streams[0]
+  hipStream_t streams[kStreams] = {};
+  hipEvent_t join[32] = {};             // cross-stream joins
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<hipEvent_t> kev;  // pairs of events around each kernel launch of a run
+  std::mutex mu;
+  phx_tuning tun{};
+  std::string err;
+  int cu_count = 256;
+};
+
+#endif

@@ -33,6 +33,7 @@ struct Gen {
   std::vector<uint8_t> q, t;
   std::vector<uint64_t> qo{0}, to{0};
   std::vector<uint8_t> cls;
+  std::vector<uint64_t> pos;  // window start of each pair in the genome
 };
 
 }  // namespace
@@ -127,6 +128,7 @@ void* synth_pairs(const synth_params* p, const uint8_t* genome) {
     g->t.insert(g->t.end(), genome + pos, genome + pos + W);
     g->to.push_back(g->t.size());
     g->cls.push_back(a ? 0 : 1);
+    g->pos.push_back(pos);
   }
   return g;
 }
@@ -141,6 +143,11 @@ void synth_copy(const void* h, uint8_t* q, uint64_t* qo, uint8_t* t, uint64_t* t
   std::memcpy(t, g->t.data(), g->t.size());
   std::memcpy(to, g->to.data(), g->to.size() * 8);
   if (cls) std::memcpy(cls, g->cls.data(), g->cls.size());
+}
+
+void synth_positions(const void* h, uint64_t* pos) {
+  const Gen* g = (const Gen*)h;
+  std::memcpy(pos, g->pos.data(), g->pos.size() * 8);
 }
 
 void synth_free(void* h) { delete (Gen*)h; }

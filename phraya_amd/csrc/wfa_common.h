@@ -29,10 +29,11 @@ __device__ __forceinline__ uint64_t seq_window(const uint64_t* __restrict__ w, u
 // the matrix, else position + longest common prefix of the two suffixes.
 // 32 bases (or 8 raw bytes) are compared per 64-bit XOR; the first differing
 // symbol is the lowest set bit.
+// qsh / tsh: index of symbol 0 inside word 0 of each sequence.
 template <bool B8>
-__device__ __forceinline__ int32_t extend_diag(const uint64_t* __restrict__ qw,
-                                               const uint64_t* __restrict__ tw, int32_t qn,
-                                               int32_t tn, int32_t p, int32_t k) {
+__device__ __forceinline__ int32_t extend_diag(const uint64_t* __restrict__ qw, uint32_t qsh,
+                                               const uint64_t* __restrict__ tw, uint32_t tsh,
+                                               int32_t qn, int32_t tn, int32_t p, int32_t k) {
   const int32_t j = p - k;
   if (p < 0 || p >= qn || j < 0 || j >= tn) return p;
   const int32_t lim = min(qn - p, tn - j);
@@ -40,7 +41,8 @@ __device__ __forceinline__ int32_t extend_diag(const uint64_t* __restrict__ qw,
   constexpr uint32_t CTZ_SHIFT = B8 ? 3 : 1;
   int32_t n = 0;
   for (;;) {
-    const uint64_t d = seq_window<B8>(qw, (uint32_t)(p + n)) ^ seq_window<B8>(tw, (uint32_t)(j + n));
+    const uint64_t d = seq_window<B8>(qw, qsh + (uint32_t)(p + n)) ^
+                       seq_window<B8>(tw, tsh + (uint32_t)(j + n));
     if (d) {
       n += (int32_t)((uint32_t)__builtin_ctzll(d) >> CTZ_SHIFT);
       break;

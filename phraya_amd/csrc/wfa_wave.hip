@@ -87,6 +87,7 @@ struct WaveStore {
   uint32_t wmask;
   const uint64_t* qw;
   const uint64_t* tw;
+  uint32_t qsh, tsh;  // index of symbol 0 in word 0
 
   __device__ __forceinline__ cell_t& toff(int32_t step) const {
     return LDS ? toff_base[step] : *(toff_base - 1 - step);
@@ -112,9 +113,10 @@ __device__ __forceinline__ WaveStore<LDS> make_store(const WfaArgs& a, const Wav
   if (LDS) {
     S.ring = reinterpret_cast<typename S_t::ring_t*>(lds_base + G.ring_off);
     uint64_t* seq = reinterpret_cast<uint64_t*>(lds_base + G.seq_off);
-    const uint32_t nq = seq_words<B8>(pd.qn), nt = seq_words<B8>(pd.tn);
+    const uint32_t nq = seq_words<B8>((int32_t)pd.q_shift + pd.qn);
+    const uint32_t nt = seq_words<B8>((int32_t)pd.t_shift + pd.tn);
     for (uint32_t i = (uint32_t)lane; i < nq + nt; i += 64)
-      seq[i] = i < nq ? a.seq[pd.q_word + i] : a.seq[pd.t_word + (i - nq)];
+      seq[i] = i < nq ? pd.q[i] : pd.t[i - nq];
     S.qw = seq;
     S.tw = seq + nq;
     S.toff_base = reinterpret_cast<typename S_t::cell_t*>(lds_base + G.tab_off);
@@ -124,20 +126,33 @@ __device__ __forceinline__ WaveStore<LDS> make_store(const WfaArgs& a, const Wav
     S.wmask = WV_W - 1;
     wave_sync<true>();
   } else {
-    // One HBM slot per pair in flight: per wave, or per workgroup when the
-    // whole workgroup shares a pair.
-    const uint64_t gw = NT == 64 ? (uint64_t)blockIdx.x * WV_WAVES + (threadIdx.x >> 6)
-                                 : (uint64_t)blockIdx.x;
-    S.ring = reinterpret_cast<typename S_t::ring_t*>(a.ring_g + gw * a.ring_stride);
-    uint8_t* h = a.hist + gw * a.hist_stride;
+    // Storage of the pair's own (request server), else one HBM slot per pair
+    // in flight: per wave, or per workgroup when the whole workgroup shares a
+    // pair.
+    uint8_t* h;
+    uint64_t hbytes;
+    if (pd.hist != nullptr) {
+      S.ring = reinterpret_cast<typename S_t::ring_t*>(pd.ring);
+      h = pd.hist;
+      hbytes = pd.hist_bytes;
+      S.wmask = (1u << pd.ring_log2) - 1;
+    } else {
+      const uint64_t gw = NT == 64 ? (uint64_t)blockIdx.x * WV_WAVES + (threadIdx.x >> 6)
+                                   : (uint64_t)blockIdx.x;
+      S.ring = reinterpret_cast<typename S_t::ring_t*>(a.ring_g + gw * a.ring_stride);
+      h = a.hist + gw * a.hist_stride;
+      hbytes = a.hist_stride;
+      S.wmask = (1u << a.wcap_log2) - 1;
+    }
     S.cells = reinterpret_cast<typename S_t::cell_t*>(h);
-    S.toff_base = reinterpret_cast<typename S_t::cell_t*>(h + a.hist_stride);
-    S.cap = a.hist_stride / 4;
+    S.toff_base = reinterpret_cast<typename S_t::cell_t*>(h + hbytes);
+    S.cap = hbytes / 4;
     S.steps_cap = 0;
-    S.wmask = (1u << a.wcap_log2) - 1;
-    S.qw = a.seq + pd.q_word;
-    S.tw = a.seq + pd.t_word;
+    S.qw = pd.q;
+    S.tw = pd.t;
   }
+  S.qsh = pd.q_shift;
+  S.tsh = pd.t_shift;
   return S;
 }
 
@@ -209,7 +224,7 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
   typename S_t::ring_t* Ri = S.ring + (size_t)R * W;
   typename S_t::ring_t* Rd = S.ring + (size_t)2 * R * W;
 
-  const int32_t m0 = extend_diag<B8>(S.qw, S.tw, qn, tn, 0, 0);
+  const int32_t m0 = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, 0, 0);
   if (lane == 0) {
     // Step 0 holds only m on diagonal 0; its ins/del entries are unset.
     Rm[0] = (typename S_t::ring_t)m0;
@@ -281,7 +296,7 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
       if (ins > best) { best = ins; mop = 1; }
       if (del > best) { best = del; mop = 2; }
       int32_t mval = best;
-      if (best >= 0) mval = extend_diag<B8>(S.qw, S.tw, qn, tn, best, k);
+      if (best >= 0) mval = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, best, k);
       int32_t layer = -1;
       if (k <= hi) {
         Wm[ik] = (typename S_t::ring_t)mval;
@@ -388,7 +403,7 @@ __device__ void wave_linear(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
   const uint32_t W = S.wmask + 1;
   typename S_t::ring_t* ring = S.ring;  // two slots
 
-  const int32_t w0 = extend_diag<B8>(S.qw, S.tw, qn, tn, 0, 0);
+  const int32_t w0 = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, 0, 0);
   if (lane == 0) {
     ring[0] = (typename S_t::ring_t)w0;
     if (NT > 64) sh->key[0] = sh->key[1] = sh->key[2] = INT32_MAX;
@@ -438,7 +453,7 @@ __device__ void wave_linear(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
       if (vr >= 0 && vr >= best) { best = vr; op = 3; }
       const bool ok = op != 0 && best <= qn && best - k >= 0 && best - k <= tn;
       int32_t val = -1;
-      if (ok) val = extend_diag<B8>(S.qw, S.tw, qn, tn, best, k);
+      if (ok) val = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, best, k);
       bool hit = false;
       if (k <= hi) {
         Wn[ik] = (typename S_t::ring_t)val;

@@ -57,6 +57,7 @@ def _load():
         L.synth_t_bytes.argtypes = [P]
         L.synth_t_bytes.restype = ctypes.c_uint64
         L.synth_copy.argtypes = [P, P, P, P, P, P]
+        L.synth_positions.argtypes = [P, P]
         L.synth_free.argtypes = [P]
         _lib = L
     return _lib
@@ -71,6 +72,7 @@ class Workload:
     t_off: np.ndarray
     cls: np.ndarray
     params: dict
+    pos: np.ndarray | None = None  # window start of each target in the genome
 
     @property
     def n(self) -> int:
@@ -90,7 +92,8 @@ class Workload:
         to[1:] = np.cumsum([len(x) for x in ts])
         return Workload(self.name, np.concatenate(qs) if qs else np.zeros(0, np.uint8), qo,
                         np.concatenate(ts) if ts else np.zeros(0, np.uint8), to, self.cls[idx],
-                        dict(self.params, subset=len(idx)))
+                        dict(self.params, subset=len(idx)),
+                        None if self.pos is None else self.pos[idx])
 
 
 def genome(length: int = 5_000_000, seed: int = 0x5EED0001) -> np.ndarray:
@@ -112,10 +115,12 @@ def generate(p: dict, name: str = "custom") -> Workload:
         qo = np.empty(n + 1, np.uint64)
         to = np.empty(n + 1, np.uint64)
         cls = np.empty(max(n, 1), np.uint8)
+        pos = np.empty(max(n, 1), np.uint64)
         L.synth_copy(h, q.ctypes.data, qo.ctypes.data, t.ctypes.data, to.ctypes.data, cls.ctypes.data)
+        L.synth_positions(h, pos.ctypes.data)
     finally:
         L.synth_free(h)
-    return Workload(name, q, qo, t, to, cls[:n], dict(p))
+    return Workload(name, q, qo, t, to, cls[:n], dict(p), pos[:n])
 
 
 WORKLOADS = {
@@ -142,3 +147,73 @@ def workload(name: str, n: int | None = None, seed: int | None = None) -> Worklo
     if seed is not None:
         p["read_seed"] = seed
     return generate(p, name)
+
+
+# Request classes of the served stream: 150 bp reads (1% events, 10% indels)
+# against 170 bp windows; long reads log-uniform 1-50 kbp (8% events, 60%
+# indels, mean length 2) against 1.1x windows; contig gaps log-uniform 5-20 kbp
+# (2% events, 10% indels of mean length 10, up to 50) against equal windows.
+SERVE_CLASSES = {
+    "short": dict(len_lo=150, len_hi=150, log_uniform=0, window_scale=1.0, window_add=20,
+                  class_a_frac=1.0, a_err_lo=0.011, a_err_hi=0.011, a_indel_frac=0.001 / 0.011,
+                  b_err_lo=0, b_err_hi=0, b_indel_frac=0, indel_mean=2.0, indel_max=3),
+    "long": dict(len_lo=1000, len_hi=50_000, log_uniform=1, window_scale=1.1, window_add=0,
+                 class_a_frac=1.0, a_err_lo=0.08, a_err_hi=0.08, a_indel_frac=0.6,
+                 b_err_lo=0, b_err_hi=0, b_indel_frac=0, indel_mean=2.0, indel_max=20),
+    "gap": dict(len_lo=5000, len_hi=20_000, log_uniform=1, window_scale=1.0, window_add=0,
+                class_a_frac=1.0, a_err_lo=0.02, a_err_hi=0.02, a_indel_frac=0.1,
+                b_err_lo=0, b_err_hi=0, b_indel_frac=0, indel_mean=10.0, indel_max=50),
+}
+SERVE_CLASS_NAMES = ("short", "long", "gap")
+
+
+@dataclass
+class RequestStream:
+    """Requests against one resident genome: query i is q_seq[q_off[i]:q_off[i+1]]
+    and its target is genome[pos[i] : pos[i] + win[i]]."""
+    genome: np.ndarray
+    q_seq: np.ndarray
+    q_off: np.ndarray
+    pos: np.ndarray
+    win: np.ndarray
+    cls: np.ndarray
+
+    @property
+    def n(self) -> int:
+        return len(self.q_off) - 1
+
+    def query(self, i: int) -> bytes:
+        return self.q_seq[self.q_off[i]:self.q_off[i + 1]].tobytes()
+
+    def target(self, i: int) -> bytes:
+        return self.genome[self.pos[i]:self.pos[i] + self.win[i]].tobytes()
+
+
+def serve_stream(n: int, mix=(0.6, 0.3, 0.1), seed: int = 0xC0FFEE05,
+                 genome_seed: int = 0x5EED0001) -> RequestStream:
+    """The served request mix, classes interleaved by a seeded draw."""
+    rng = np.random.default_rng(seed)
+    cls = rng.choice(3, size=n, p=np.asarray(mix, dtype=np.float64) / sum(mix)).astype(np.uint8)
+    g = genome(5_000_000, genome_seed)
+    parts = []
+    for c, name in enumerate(SERVE_CLASS_NAMES):
+        k = int((cls == c).sum())
+        if k == 0:
+            parts.append(None)
+            continue
+        p = dict(SERVE_CLASSES[name], n=k, read_seed=seed + 17 * (c + 1), genome_seed=genome_seed)
+        parts.append(generate(p, name))
+    idx = [0, 0, 0]
+    qs, pos, win = [], np.zeros(n, np.uint64), np.zeros(n, np.uint64)
+    for i in range(n):
+        c = int(cls[i])
+        w = parts[c]
+        j = idx[c]
+        idx[c] += 1
+        qs.append(w.q_seq[w.q_off[j]:w.q_off[j + 1]])
+        pos[i] = w.pos[j]
+        win[i] = w.t_off[j + 1] - w.t_off[j]
+    q_off = np.zeros(n + 1, np.uint64)
+    q_off[1:] = np.cumsum([len(x) for x in qs])
+    q_seq = np.concatenate(qs) if qs else np.zeros(0, np.uint8)
+    return RequestStream(g, q_seq, q_off, pos, win, cls)
