@@ -34,6 +34,10 @@ WORKLOADS = {
                  "1M x (150 bp read vs 180 bp window), Myers edit distance + end column"),
     "mixed": ("mixed", "affine", (4, 6, 2),
               "200k reads log-uniform 100 bp-20 kbp vs 1.1x windows, affine WFA 4/6/2, score+CIGAR"),
+    "serve": (None, "affine", (4, 6, 2),
+              "Poisson request stream: 60% 150 bp reads (1%), 30% long reads 1-50 kbp (8%, "
+              "indel-heavy), 10% 5-20 kbp contig gaps (2%), affine WFA 4/6/2 score+CIGAR against "
+              "a device-resident 5 Mbp reference"),
 }
 
 
@@ -108,6 +112,195 @@ def calibrate_copy(nbytes: int) -> None:
           flush=True)
 
 
+def _pct(x, q):
+    return float(np.percentile(x, q)) if len(x) else None
+
+
+def _lat_summary(lat_ms, cls):
+    from phraya_amd import synth
+    out = {"p50": _pct(lat_ms, 50), "p99": _pct(lat_ms, 99), "max": float(np.max(lat_ms))}
+    for c, name in enumerate(synth.SERVE_CLASS_NAMES):
+        sel = lat_ms[cls == c]
+        if len(sel):
+            out[name] = {"n": int(len(sel)), "p50": _pct(sel, 50), "p99": _pct(sel, 99)}
+    return out
+
+
+def run_serve(args):
+    """Served stream vs the plain batched path on the same Poisson arrivals.
+
+    served: requests go to the request server (phx_serve_*) at their arrival
+    times; targets are windows of the resident reference.
+    plain:  the drop-in batch path (phx_align_batch) used as a serving loop:
+    whatever has arrived is aligned as one batch, then the next batch.
+    Latency = result ready - arrival.  pairs/s = requests / (last result - first
+    arrival)."""
+    import phraya_amd as px
+    from phraya_amd import synth
+    from oracle import oracle as O
+
+    costs = px.AffineCosts(4, 6, 2)
+    rng = np.random.default_rng(20250101)
+    lanes = max(1, args.serve_lanes)
+    # Capacity probe: a closed-loop burst through the server.
+    probe_n = 3000
+    log(f"generating probe stream ({probe_n})")
+    probe = synth.serve_stream(probe_n, seed=0xC0FFEE07)
+    genome = probe.genome
+    srv = px.Server(px.ENGINE_AFFINE, costs, reference=genome, lanes_per_tier=lanes)
+
+    def drain(server, n, limit_s=300.0):
+        got, t_start = 0, time.perf_counter()
+        while got < n:
+            out, _, _ = server.poll_raw(8192, timeout_us=2000)
+            got += len(out)
+            if time.perf_counter() - t_start > limit_s:
+                raise RuntimeError(f"server returned {got} of {n} in {limit_s:.0f} s")
+
+    t0 = time.perf_counter()
+    srv.submit_arrays(np.arange(probe_n), probe.q_seq, probe.q_off, probe.pos, probe.win)
+    drain(srv, probe_n)
+    cap_rate = probe_n / (time.perf_counter() - t0)
+    rate = args.serve_rate or args.serve_load * cap_rate
+    log(f"server capacity on a closed burst: {cap_rate:.0f} req/s; offered load {rate:.0f} req/s")
+
+    n = max(200, int(rate * args.serve_seconds))
+    log(f"generating {n} requests")
+    rs = synth.serve_stream(n, seed=0xC0FFEE05)
+    arrive = np.cumsum(rng.exponential(1.0 / rate, n))
+
+    # --- served
+    log("served run")
+    lat_sv = np.zeros(n)
+    st_sv = np.zeros(n, np.int32)
+    done_sv = np.zeros(n)
+    nxt, got = 0, 0
+    sv_runs = [None] * n  # served (status, score, target_end, runs) of every request
+    qlen = np.diff(rs.q_off)
+    small = np.flatnonzero(qlen <= 5000)  # the oracle keeps full-width frames: check small ones
+    check_idx = rng.choice(small, min(len(small), 200), replace=False).tolist()
+    limit_s = 10.0 * args.serve_seconds + 120.0
+    t0 = time.perf_counter()
+    while got < n:
+        now = time.perf_counter() - t0
+        if now > limit_s:
+            raise RuntimeError(f"served run incomplete after {now:.0f} s: {got} of {n}")
+        if nxt < n and arrive[nxt] <= now:
+            hi = int(np.searchsorted(arrive, now, side="right"))
+            srv.submit_arrays(np.arange(nxt, hi), rs.q_seq, rs.q_off, rs.pos, rs.win,
+                              idx=np.arange(nxt, hi))
+            nxt = hi
+        out, runs, off = srv.poll_raw(8192, timeout_us=0 if nxt < n else 1000)
+        tnow = time.perf_counter() - t0
+        for k, o in enumerate(out):
+            i = int(o["tag"])
+            lat_sv[i] = float(o["latency_ms"])
+            st_sv[i] = int(o["status"])
+            done_sv[i] = tnow
+            sv_runs[i] = (int(o["status"]), int(o["score"]), int(o["target_end"]),
+                          runs[int(off[k]):int(off[k + 1])].copy())
+        got += len(out)
+        if not len(out) and nxt < n:
+            wait = arrive[nxt] - (time.perf_counter() - t0)
+            if wait > 0.0002:
+                time.sleep(min(wait, 0.0005))
+    span_sv = float(done_sv.max())
+    stats = srv.stats()
+    srv.stop()
+
+    # --- plain batched path (drop-in), same arrivals
+    log("plain batched run")
+    tq = [rs.target(i) for i in range(n)]
+    t_off = np.zeros(n + 1, np.uint64)
+    t_off[1:] = np.cumsum([len(x) for x in tq])
+    t_seq = np.frombuffer(b"".join(tq), np.uint8)
+    del tq
+    lat_pl = np.zeros(n)
+    done_pl = np.zeros(n)
+    nxt, batches, served_vs_plain_mism = 0, 0, 0
+    t0 = time.perf_counter()
+    while nxt < n:
+        now = time.perf_counter() - t0
+        if arrive[nxt] > now:
+            time.sleep(min(arrive[nxt] - now, 0.001))
+            continue
+        hi = int(np.searchsorted(arrive, now, side="right"))
+        qa, qb, ta, tb = int(rs.q_off[nxt]), int(rs.q_off[hi]), int(t_off[nxt]), int(t_off[hi])
+        pb = px.PackedBatch(rs.q_seq[qa:qb], rs.q_off[nxt:hi + 1] - rs.q_off[nxt],
+                            t_seq[ta:tb], t_off[nxt:hi + 1] - t_off[nxt],
+                            np.zeros(hi - nxt, np.uint64), np.zeros(hi - nxt, np.uint64))
+        plan = px.Plan(pb, px.ENGINE_AFFINE, costs)
+        plan.run()
+        res = plan.results()
+        plan.close()
+        tdone = time.perf_counter() - t0
+        lat_pl[nxt:hi] = (tdone - arrive[nxt:hi]) * 1000.0
+        done_pl[nxt:hi] = tdone
+        # Served and plain results agree on every request, CIGAR runs included.
+        for i in range(nxt, hi):
+            j = i - nxt
+            a = res.aln[j]
+            stv, scv, tev, runs = sv_runs[i]
+            same = int(a["status"]) == stv
+            if same and stv == 0:
+                same = int(a["score"]) == scv and int(a["target_end"]) == tev and np.array_equal(
+                    res.runs[int(res.run_off[j]):int(res.run_off[j + 1])], runs)
+            if not same:
+                served_vs_plain_mism += 1
+        nxt = hi
+        batches += 1
+    span_pl = float(done_pl.max())
+
+    # --- parity of the served results against the oracle on a sample
+    chk = sorted(check_idx)
+    mism = 0
+    for i in chk:
+        stv, scv, tev, runs = sv_runs[i]
+        r = O.wfa_extend_affine(rs.query(i), rs.target(i), (0, 0), (4, 6, 2))
+        if r.status != stv or (r.status == 0 and (
+                r.cigar != px.format_cigar(runs) or r.score != scv or r.target_end != tev)):
+            mism += 1
+    log(f"parity: {len(chk)} served requests checked against the oracle, {mism} mismatches; "
+        f"served vs plain: {served_vs_plain_mism} mismatches over {n}")
+
+    sv_rate = n / span_sv
+    pl_rate = n / span_pl
+    lat_sv_ms = lat_sv
+    out = {
+        "metric": METRIC,
+        "value": sv_rate,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": span_sv * 1000.0,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": "serve", "description": WORKLOADS["serve"][3], "requests": n,
+                   "lanes_per_tier": lanes,
+                   "gpu_max_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
+                   "offered_req_per_s": rate, "server_capacity_closed_burst": cap_rate,
+                   "engine": "affine", "costs": [4, 6, 2], "parallelism": "single GPU"},
+        "latency_ms": dict(_lat_summary(lat_sv_ms, rs.cls),
+                           what="served: submission to result ready, per request"),
+        "plain_batched": {"pairs_per_s": pl_rate, "batches": batches,
+                          "latency_ms": _lat_summary(lat_pl, rs.cls),
+                          "what": "phx_align_batch over whatever has arrived, one batch at a time"},
+        "served_over_plain": sv_rate / pl_rate,
+        "served_p99_over_plain_p99": _pct(lat_sv_ms, 99) / _pct(lat_pl, 99),
+        "server_stats": stats,
+        "status_counts": {str(k): int(v) for k, v in zip(*np.unique(st_sv, return_counts=True))},
+        "parity": {"checked": len(chk), "mismatches": mism, "oracle_sample": "requests <= 5 kbp",
+                   "served_vs_plain_compared": n, "served_vs_plain_mismatches": served_vs_plain_mism},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,18 +313,37 @@ def main():
     ap.add_argument("--check", type=int, default=2000, help="pairs checked against the oracle")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (counter calibration)")
+    ap.add_argument("--serve-seconds", type=float, default=15.0,
+                    help="serve: length of the arrival stream")
+    ap.add_argument("--serve-load", type=float, default=0.6,
+                    help="serve: offered load as a fraction of the measured server capacity")
+    ap.add_argument("--serve-rate", type=float, default=0.0,
+                    help="serve: offered requests/s (overrides --serve-load)")
+    ap.add_argument("--serve-lanes", type=int, default=3,
+                    help="serve: launch groups in flight per tier")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
         return
 
+    if args.workload == "serve":
+        # One hardware queue per stream (4 tiers x lanes, plus the context's
+        # own 4): streams that share a queue run one after another, which
+        # would put short reads behind long ones.  Read by the HIP runtime at
+        # its first call.
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, 4 * max(1, args.serve_lanes) + 4))
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(pairs shard trivially; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+              "(pairs are independent and shard with no exchange). Run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
 
     import phraya_amd as px
     from phraya_amd import synth
+
+    if args.workload == "serve":
+        run_serve(args)
+        return
 
     wname, engine, costs, desc = WORKLOADS[args.workload]
     log(f"generating {wname}")

@@ -226,6 +226,16 @@ int phx_serve_stats_get(phx_server* srv, phx_serve_stats* st);
  * server (responses not yet polled are dropped). */
 int phx_serve_stop(phx_server* srv);
 
+/* Memory model of one WFA request; host only, no GPU needed.  *steps (may be
+ * NULL) gets the score steps its search is estimated to take: the end gap
+ * for the length difference plus the per-base event rate implied by how many
+ * of the query's 12-mers occur in the target, with margin.  Returns the HBM
+ * bytes (history cells of every step + score-frame ring) that the server
+ * reserves for it in an HBM tier, or -1 for bad arguments.  No counterpart
+ * in phraya-align; it is for callers that budget device memory themselves. */
+int64_t phx_estimate_request(int32_t engine, phx_costs costs, const uint8_t* q, uint64_t qn,
+                             const uint8_t* t, uint64_t tn, int32_t* steps);
+
 /* Library version string. */
 const char* phx_version(void);
 

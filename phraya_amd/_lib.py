@@ -35,7 +35,7 @@ EXPORTS = (
     "phx_plan_algorithmic_bytes", "phx_plan_destroy", "phx_format_cigar",
     "phx_wfa_extend", "phx_wfa_extend_affine", "phx_myers_extend", "phx_myers_edit_distance",
     "phx_serve_start", "phx_serve_submit", "phx_serve_poll", "phx_serve_stats_get",
-    "phx_serve_stop", "phx_version",
+    "phx_serve_stop", "phx_estimate_request", "phx_version",
 )
 
 
@@ -150,6 +150,8 @@ def lib() -> ctypes.CDLL:
     L.phx_serve_poll.argtypes = [P, P, ctypes.c_uint32, P, u64, P, P, ctypes.c_int32]
     L.phx_serve_stats_get.argtypes = [P, P]
     L.phx_serve_stop.argtypes = [P]
+    L.phx_estimate_request.argtypes = [ctypes.c_int32, Costs, P, u64, P, u64, P]
+    L.phx_estimate_request.restype = ctypes.c_int64
     L.phx_version.restype = ctypes.c_char_p
     _lib = L
     return L

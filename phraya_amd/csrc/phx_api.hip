@@ -310,6 +310,20 @@ extern "C" {
 
 const char* phx_version(void) { return "phraya_hip 0.1 gfx950"; }
 
+int64_t phx_estimate_request(int32_t engine, phx_costs costs, const uint8_t* q, uint64_t qn,
+                             const uint8_t* t, uint64_t tn, int32_t* steps) {
+  if ((engine != PHX_ENGINE_AFFINE && engine != PHX_ENGINE_LINEAR) || (qn && !q) || (tn && !t) ||
+      qn > (uint64_t)kMaxLen || tn > (uint64_t)kMaxLen)
+    return -1;
+  if (engine == PHX_ENGINE_AFFINE && (costs.mismatch < 1 || costs.gap_extend < 1 || costs.gap_open < 0))
+    return -1;
+  const int32_t s = estimate_steps_seq(engine, costs, q, qn, t, tn);
+  if (steps) *steps = s;
+  uint32_t wcap = 0;
+  const uint64_t ring = ring_words_for(engine, ring_slots(engine, costs), qn + tn, &wcap) * 4;
+  return (int64_t)(hist_slot_bytes(s, (int32_t)qn, (int32_t)tn) + ((ring + 255) & ~(uint64_t)255));
+}
+
 phx_ctx* phx_ctx_create(int device, uint32_t flags, const phx_tuning* tuning) {
   (void)flags;
   phx_ctx* c = new phx_ctx;
@@ -474,16 +488,30 @@ phx_plan* phx_plan_create(phx_ctx* ctx, const phx_batch* b) {
     P->group_tier.assign(6, 1);
     P->group_tier[0] = 0;
     P->group_slot.assign(6, 0);
+    // Step estimates from the sequences (k-mer event rate) for the pairs that
+    // need HBM slots, on host threads.
+    auto lds_ok = [&](uint32_t i) {
+      return P->qlen[i] + P->tlen[i] <= kWaveMaxLen && P->ring_slots <= kWaveRingMax;
+    };
+    std::vector<int32_t> est(n, 0);
+    std::vector<uint32_t> wide;
+    for (uint32_t i : order)
+      if (!lds_ok(i)) wide.push_back(i);
+    parallel_for(wide.size(), [&](size_t k) {
+      const uint32_t i = wide[k];
+      est[i] = estimate_steps_seq(b->engine, b->costs, b->q_seq + b->q_off[i] + P->seed_q[i],
+                                  P->qlen[i], b->t_seq + b->t_off[i] + P->seed_t[i], P->tlen[i]);
+    });
     std::vector<uint64_t> est_words(6, 0);
     for (uint32_t i : order) {
       const uint64_t L = P->qlen[i] + P->tlen[i];
       int gi;
-      if (L <= kWaveMaxLen && P->ring_slots <= kWaveRingMax) gi = 0;
+      if (lds_ok(i)) gi = 0;
       else gi = L <= 4096 ? 1 : L <= 8192 ? 2 : L <= 16384 ? 3 : L <= 32768 ? 4 : 5;
       P->work[gi].push_back(i);
       if (gi > 0) {
         const int32_t qn = P->pairs[i].qn, tn = P->pairs[i].tn;
-        est_words[gi] = std::max(est_words[gi], exact_hist_words(estimate_steps(P->engine, P->costs, qn, tn), qn, tn));
+        est_words[gi] = std::max(est_words[gi], exact_hist_words(est[i], qn, tn));
       }
     }
     for (int gi = 1; gi < 6; ++gi) P->group_slot[gi] = (est_words[gi] * 4 + 255) & ~(uint64_t)255;
@@ -496,7 +524,7 @@ phx_plan* phx_plan_create(phx_ctx* ctx, const phx_batch* b) {
     for (int gi = 1; gi < 6; ++gi) {
       for (uint32_t i : P->work[gi]) {
         const int32_t qn = P->pairs[i].qn, tn = P->pairs[i].tn;
-        vol[gi] += (double)exact_hist_words(estimate_steps(P->engine, P->costs, qn, tn), qn, tn);
+        vol[gi] += (double)exact_hist_words(est[i], qn, tn);
       }
       tot += vol[gi];
     }

@@ -9,11 +9,13 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "phraya_hip.h"
@@ -138,6 +140,98 @@ inline int32_t estimate_steps(int engine, const phx_costs& c, int32_t qn, int32_
     steps = d + 0.09 * m;
   }
   return (int32_t)std::min(1.15 * steps + 16.0, (double)(qn + tn + 16));
+}
+
+// Per-base event rate of a pair, from how many of the query's 12-mers occur
+// in the target: a k-mer survives p events per base with probability about
+// (1 - p)^k, so p = 1 - f^(1/k) for a surviving fraction f.  Target 12-mers go
+// into an open-addressing table; up to 1024 evenly spaced query 12-mers are
+// looked up.  Returns -1 when either sequence is too short to say.
+inline double kmer_event_rate(const uint8_t* q, size_t qn, const uint8_t* t, size_t tn) {
+  constexpr int K = 12;
+  constexpr uint32_t MASK = (1u << (2 * K)) - 1;
+  if (qn < 4 * K || tn < K) return -1.0;
+  const uint8_t* code = acgt_table();
+  const uint32_t cap = next_pow2(std::max<uint64_t>(64, 2 * (uint64_t)tn));
+  static thread_local std::vector<uint32_t> table;
+  table.assign(cap, 0);
+  auto slot = [&](uint32_t key) { return (key * 0x9E3779B1u) >> (32 - log2u(cap)); };
+  uint32_t h = 0;
+  int valid = 0;
+  for (size_t j = 0; j < tn; ++j) {
+    const uint8_t c = code[t[j]];
+    if (c == 0xFF) { valid = 0; continue; }
+    h = ((h << 2) | c) & MASK;
+    if (++valid < K) continue;
+    const uint32_t key = h + 1;
+    for (uint32_t i = slot(key);; i = (i + 1) & (cap - 1)) {
+      if (table[i] == key) break;
+      if (table[i] == 0) { table[i] = key; break; }
+    }
+  }
+  const size_t positions = qn - K + 1;
+  const size_t samples = std::min<size_t>(1024, positions);
+  size_t hits = 0, tried = 0;
+  for (size_t s = 0; s < samples; ++s) {
+    const size_t at = s * positions / samples;
+    uint32_t key = 0;
+    bool ok = true;
+    for (int k = 0; k < K; ++k) {
+      const uint8_t c = code[q[at + k]];
+      if (c == 0xFF) { ok = false; break; }
+      key = (key << 2) | c;
+    }
+    if (!ok) continue;
+    ++tried;
+    key += 1;
+    for (uint32_t i = slot(key);; i = (i + 1) & (cap - 1)) {
+      if (table[i] == key) { ++hits; break; }
+      if (table[i] == 0) break;
+    }
+  }
+  if (tried < 16) return -1.0;
+  const double f = (double)hits / (double)tried;
+  if (f <= 0.0) return 0.35;
+  return std::min(0.35, std::max(0.0, 1.0 - std::pow(f, 1.0 / K)));
+}
+
+// Score steps of a pair estimated from its sequences: the end gap for the
+// length difference plus the k-mer event rate, each event costed as the mean
+// of a mismatch and a 2-base gap, with 25% margin.  Pairs the k-mer probe
+// cannot judge fall back to estimate_steps.  A pair that needs more steps
+// than estimated still completes: it reports its exact count and is re-run.
+inline int32_t estimate_steps_seq(int engine, const phx_costs& c, const uint8_t* q, size_t qn,
+                                  const uint8_t* t, size_t tn) {
+  const double p = kmer_event_rate(q, qn, t, tn);
+  if (p < 0) return estimate_steps(engine, c, (int32_t)qn, (int32_t)tn);
+  const double m = (double)std::max(qn, tn), d = std::fabs((double)tn - (double)qn);
+  const double rate = std::max(p, 0.002) * 1.25;
+  double steps;
+  if (engine == PHX_ENGINE_AFFINE) {
+    const int32_t g = gcd3(c.mismatch, c.gap_open, c.gap_extend);
+    const double per_event = 0.5 * (c.mismatch + c.gap_open + 2.0 * c.gap_extend);
+    steps = (c.gap_open + c.gap_extend * d + m * rate * per_event) / g;
+  } else {
+    steps = d + m * rate * 2.0;  // a mismatch costs 1, an indel event its length
+  }
+  return (int32_t)std::min(1.1 * steps + 32.0, (double)(qn + tn + 16));
+}
+
+// f(i) for i in [0, n) on up to 16 host threads.
+template <class F>
+void parallel_for(size_t n, F f) {
+  const unsigned T = std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
+  if (n < 32 || T == 1) {
+    for (size_t i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<size_t> next{0};
+  std::vector<std::thread> pool;
+  for (unsigned k = 0; k < std::min<size_t>(T, n); ++k)
+    pool.emplace_back([&] {
+      for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
+    });
+  for (auto& th : pool) th.join();
 }
 
 // Ring words of one HBM slot for pairs with qn + tn <= max_sum.

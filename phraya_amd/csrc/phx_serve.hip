@@ -137,6 +137,7 @@ struct Req {
   bool b8 = false;
   double t_submit = 0;
   int tier = 0;
+  int32_t est = 0;    // estimated score steps (HBM tiers)
   int32_t steps = 0;  // exact score steps once known
   int attempts = 0;
   // Arena reservation while in flight.
@@ -180,6 +181,7 @@ struct phx_server {
   uint32_t wave_lds_kib = 9, wave_slots = 4096;
   // Resident reference: 2-bit and 8-bit copies, positions of non-ACGT bytes.
   DBuf d_ref2, d_ref8;
+  std::vector<uint8_t> ref_host;  // for the memory estimates of reference windows
   uint64_t ref_len = 0;
   std::vector<uint64_t> ref_bad;
   // Wavefront state of HBM-tier requests.
@@ -217,7 +219,7 @@ int initial_tier(const phx_server* S, int32_t qn, int32_t tn) {
 // estimate its history slot is sized for (or the exact count once known).
 int32_t step_bound(const Req& r, int tier, const phx_server* S) {
   if (tier == TIER_LDS) return (int32_t)kWaveSteps;
-  return r.steps > 0 ? r.steps : estimate_steps(S->engine, S->costs, r.qn, r.tn);
+  return r.steps > 0 ? r.steps : r.est > 0 ? r.est : estimate_steps(S->engine, S->costs, r.qn, r.tn);
 }
 
 // CIGAR run slots: every edit run costs at least one score step and M runs
@@ -399,6 +401,8 @@ void finish(phx_server* S, Lane& L, bool device_ok, std::vector<Done>& fin) {
       S->reruns += 1;
       if (L.tier == TIER_LDS) {
         r.tier = hbm_tier_for((uint64_t)r.qn + r.tn);
+        const uint8_t* tp = r.ref ? S->ref_host.data() + r.ref_pos : r.t.data();
+        r.est = estimate_steps_seq(S->engine, S->costs, r.q.data(), r.q.size(), tp, (size_t)r.tn);
       } else {
         // History overflow: the kernel kept going and o.steps is exact; a run
         // overflow also reports the exact count.
@@ -541,6 +545,7 @@ phx_server* phx_serve_start(phx_ctx* ctx, const phx_serve_config* cfg) {
   if (cfg->reference && cfg->reference_len) {
     const uint64_t n = cfg->reference_len;
     S->ref_len = n;
+    S->ref_host.assign(cfg->reference, cfg->reference + n);
     const uint8_t* code = acgt_table();
     for (uint64_t i = 0; i < n; ++i)
       if (code[cfg->reference[i]] == 0xFF) S->ref_bad.push_back(i);
@@ -629,6 +634,10 @@ int phx_serve_submit(phx_server* S, const phx_request* reqs, uint32_t n) {
     }
     r.b8 = !acgt;
     r.tier = initial_tier(S, r.qn, r.tn);
+    if (r.tier != TIER_LDS) {
+      const uint8_t* tp = r.ref ? S->ref_host.data() + r.ref_pos : r.t.data();
+      r.est = estimate_steps_seq(S->engine, S->costs, r.q.data(), r.q.size(), tp, (size_t)r.tn);
+    }
     batch.push_back(std::move(r));
   }
   S->submitted += n;
