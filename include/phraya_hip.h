@@ -226,6 +226,62 @@ int phx_serve_stats_get(phx_server* srv, phx_serve_stats* st);
  * server (responses not yet polled are dropped). */
 int phx_serve_stop(phx_server* srv);
 
+/* ---- Contig mode -----------------------------------------------------------
+ *
+ * Aligns pairs of long contig segments (query i against target i, end to
+ * end) by anchoring them on shared minimizers and aligning every gap between
+ * anchors with the batched engine.  No counterpart in phraya-align's
+ * extension API; its gap alignments are plain fill_wfa / wfa_extend_affine
+ * calls (phraya-align/src/wfa_simd.rs:231-345, :919-935) on the windows.
+ * Anchors: forward-strand (k, w) minimizers (leftmost smallest hash of a
+ * mixing function of the 2-bit k-mer) that occur once among each
+ * sequence's minimizers, paired by k-mer, reduced to the longest chain
+ * increasing in both sequences, then spaced out: walking the chain, the next
+ * anchor kept is the first at least `spacing` query bases past the previous
+ * one, spacing drawn log-uniform in [gap_min, gap_max] from a SplitMix64
+ * stream seeded by (seed, pair index).  The contig CIGAR is gap 0, anchor 1
+ * as k M, gap 1, ..., with equal neighbouring runs merged; a gap alignment
+ * that leaves target bases unconsumed (fitting branch) adds them as I; the
+ * contig score is the sum of its gap scores. */
+typedef struct phx_contig_params {
+  uint32_t k;        /* minimizer k (8..16); 0 = 15 */
+  uint32_t w;        /* minimizer window in k-mers; 0 = 10 */
+  uint32_t gap_min;  /* anchor spacing range in bases; 0 = 1000 */
+  uint32_t gap_max;  /* 0 = 20000 */
+  uint64_t seed;
+} phx_contig_params;
+
+typedef struct phx_contig_plan phx_contig_plan;
+
+/* Anchors of one pair (host only): writes up to cap (query pos, target pos)
+ * pairs to out[2i], out[2i+1] and returns how many there are (-1 on bad
+ * arguments). */
+int64_t phx_contig_anchors(const uint8_t* q, uint64_t qn, const uint8_t* t, uint64_t tn,
+                           const phx_contig_params* params, uint64_t pair_index, uint64_t* out,
+                           uint64_t cap);
+/* Anchor every pair of the batch (seeds ignored), keep both sides resident in
+ * HBM and prepare one batched plan over all gaps. */
+phx_contig_plan* phx_contig_plan_create(phx_ctx* ctx, int32_t engine, phx_costs costs,
+                                        const phx_batch* contigs, const phx_contig_params* params);
+/* Align all gaps on the GPU, then stitch each contig's CIGAR (host threads). */
+int phx_contig_plan_run(phx_contig_plan* plan);
+/* Per contig: aln[i] (score = sum of gap scores, edit distance of the stitched
+ * CIGAR) and the stitched runs, as phx_plan_results. */
+int phx_contig_plan_results(phx_contig_plan* plan, phx_results* out);
+/* Gap alignments per run, and contig i's gaps as (q0, q1, t0, t1) quadruples
+ * (returns the gap count; writes up to cap quadruples). */
+uint64_t phx_contig_plan_gaps(const phx_contig_plan* plan);
+int64_t phx_contig_plan_layout(const phx_contig_plan* plan, uint32_t contig, uint64_t* gaps,
+                               uint64_t cap);
+/* The batched plan of all gaps (to read its kernel times / algorithmic bytes). */
+phx_plan* phx_contig_plan_gap_plan(phx_contig_plan* plan);
+/* Wall time of the last run (GPU alignment + result download + stitching),
+ * the device time of the gap alignment, and the stitching time, in ms. */
+double phx_contig_plan_last_ms(const phx_contig_plan* plan);
+double phx_contig_plan_last_gpu_ms(const phx_contig_plan* plan);
+double phx_contig_plan_last_stitch_ms(const phx_contig_plan* plan);
+void phx_contig_plan_destroy(phx_contig_plan* plan);
+
 /* Memory model of one WFA request; host only, no GPU needed.  *steps (may be
  * NULL) gets the score steps its search is estimated to take: the end gap
  * for the length difference plus the per-base event rate implied by how many

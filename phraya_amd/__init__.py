@@ -31,7 +31,7 @@ __all__ = [
     "Context", "wfa_extend", "wfa_extend_naive", "wfa_extend_affine", "myers_extend",
     "myers_edit_distance", "align_batch", "BatchResult", "Plan", "pack_batch", "format_cigar",
     "ENGINE_LINEAR", "ENGINE_AFFINE", "ENGINE_MYERS", "ENGINE_MYERS_GLOBAL", "ENGINE_MYERS_DIST",
-    "Server", "Response",
+    "Server", "Response", "ContigParams", "ContigPlan", "contig_anchors",
 ]
 
 ENGINE_LINEAR = L.ENGINE_LINEAR
@@ -243,11 +243,14 @@ class Plan:
     def results(self) -> BatchResult:
         n = self.pb.n
         aln = np.zeros(n, dtype=L.ALN_DTYPE)
-        cap = int(L.lib().phx_cigar_bound(ctypes.byref(self._cb)))
-        runs = np.zeros(max(cap, 1), dtype=np.uint32)
         off = np.zeros(n + 1, dtype=np.uint64)
-        res = L.CResults(aln.ctypes.data, runs.ctypes.data, cap, off.ctypes.data)
+        # First call sizes the run arena (run_off[n]), the second fills it.
+        res = L.CResults(aln.ctypes.data, None, 0, off.ctypes.data)
         rc = L.lib().phx_plan_results(self._h, ctypes.byref(res))
+        runs = np.zeros(max(int(off[n]), 1), dtype=np.uint32)
+        if rc == L.PHX_E_CAPACITY:
+            res = L.CResults(aln.ctypes.data, runs.ctypes.data, len(runs), off.ctypes.data)
+            rc = L.lib().phx_plan_results(self._h, ctypes.byref(res))
         if rc != L.PHX_OK:
             raise RuntimeError(f"phx_plan_results failed ({rc}): {self.ctx.last_error()}")
         return BatchResult(aln, runs[: int(off[-1])], off)
@@ -462,5 +465,115 @@ class Server:
     def __del__(self):
         try:
             self.stop()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ contig mode
+
+@dataclass(frozen=True)
+class ContigParams:
+    """Anchoring of contig mode: (k, w) minimizers, anchor spacing drawn
+    log-uniform in [gap_min, gap_max] bases from a stream seeded by `seed`."""
+    k: int = 15
+    w: int = 10
+    gap_min: int = 1000
+    gap_max: int = 20000
+    seed: int = 0
+
+    def c(self) -> L.CContigParams:
+        return L.CContigParams(self.k, self.w, self.gap_min, self.gap_max, self.seed)
+
+
+def contig_anchors(query: bytes, target: bytes, params: ContigParams = ContigParams(),
+                   pair_index: int = 0) -> list[tuple[int, int]]:
+    """Kept anchors (query pos, target pos) of one pair (host only)."""
+    q, t = bytes(query), bytes(target)
+    p = params.c()
+    cap = max(16, len(q) // max(1, params.gap_min) + 16)
+    out = np.zeros(2 * cap, np.uint64)
+    n = L.lib().phx_contig_anchors(q, len(q), t, len(t), ctypes.byref(p), pair_index,
+                                   out.ctypes.data, cap)
+    if n < 0:
+        raise ValueError("bad contig parameters")
+    if n > cap:
+        out = np.zeros(2 * n, np.uint64)
+        L.lib().phx_contig_anchors(q, len(q), t, len(t), ctypes.byref(p), pair_index,
+                                   out.ctypes.data, n)
+    return [(int(out[2 * i]), int(out[2 * i + 1])) for i in range(n)]
+
+
+class ContigPlan:
+    """Contig pairs anchored, resident in HBM, with one batched plan over all
+    inter-anchor gaps (phx_contig_plan_*); run() aligns and stitches."""
+
+    def __init__(self, q_seq: np.ndarray, q_off: np.ndarray, t_seq: np.ndarray, t_off: np.ndarray,
+                 engine: int = ENGINE_AFFINE, costs: AffineCosts = AffineCosts(),
+                 params: ContigParams = ContigParams(), ctx: Optional[Context] = None):
+        self._ctx = ctx or default_context()
+        self.n = len(q_off) - 1
+        self._keep = (np.ascontiguousarray(q_seq, np.uint8), np.asarray(q_off, np.uint64),
+                      np.ascontiguousarray(t_seq, np.uint8), np.asarray(t_off, np.uint64))
+        qs, qo, ts, to = self._keep
+        b = L.CBatch(self.n, engine, L.Costs(costs.mismatch, costs.gap_open, costs.gap_extend), -1,
+                     qs.ctypes.data, qo.ctypes.data, ts.ctypes.data, to.ctypes.data, None, None)
+        p = params.c()
+        self._h = L.lib().phx_contig_plan_create(self._ctx.handle, engine, b.costs,
+                                                 ctypes.byref(b), ctypes.byref(p))
+        if not self._h:
+            raise RuntimeError(f"phx_contig_plan_create failed: {self._ctx.last_error()}")
+        self.params = params
+
+    @property
+    def gaps(self) -> int:
+        return int(L.lib().phx_contig_plan_gaps(self._h))
+
+    def layout(self, i: int) -> np.ndarray:
+        """Contig i's gaps as rows (q0, q1, t0, t1)."""
+        n = L.lib().phx_contig_plan_layout(self._h, i, None, 0)
+        out = np.zeros((max(n, 1), 4), np.uint64)
+        L.lib().phx_contig_plan_layout(self._h, i, out.ctypes.data, n)
+        return out[:n]
+
+    def gap_plan_algorithmic_bytes(self) -> int:
+        return int(L.lib().phx_plan_algorithmic_bytes(L.lib().phx_contig_plan_gap_plan(self._h)))
+
+    def gap_kernel_ms(self) -> float:
+        return float(L.lib().phx_plan_last_kernel_ms(L.lib().phx_contig_plan_gap_plan(self._h)))
+
+    def run(self) -> None:
+        rc = L.lib().phx_contig_plan_run(self._h)
+        if rc != L.PHX_OK:
+            raise RuntimeError(f"phx_contig_plan_run: status {rc}: {self._ctx.last_error()}")
+
+    def last_ms(self) -> float:
+        return float(L.lib().phx_contig_plan_last_ms(self._h))
+
+    def last_gpu_ms(self) -> float:
+        return float(L.lib().phx_contig_plan_last_gpu_ms(self._h))
+
+    def last_stitch_ms(self) -> float:
+        return float(L.lib().phx_contig_plan_last_stitch_ms(self._h))
+
+    def results(self) -> BatchResult:
+        aln = np.zeros(self.n, L.ALN_DTYPE)
+        off = np.zeros(self.n + 1, np.uint64)
+        res = L.CResults(aln.ctypes.data, None, 0, off.ctypes.data)
+        rc = L.lib().phx_contig_plan_results(self._h, ctypes.byref(res))
+        runs = np.zeros(max(int(off[self.n]), 1), np.uint32)
+        res = L.CResults(aln.ctypes.data, runs.ctypes.data, len(runs), off.ctypes.data)
+        rc = L.lib().phx_contig_plan_results(self._h, ctypes.byref(res))
+        if rc != L.PHX_OK:
+            raise RuntimeError(f"phx_contig_plan_results: status {rc}")
+        return BatchResult(aln, runs, off)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().phx_contig_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass

@@ -35,7 +35,11 @@ EXPORTS = (
     "phx_plan_algorithmic_bytes", "phx_plan_destroy", "phx_format_cigar",
     "phx_wfa_extend", "phx_wfa_extend_affine", "phx_myers_extend", "phx_myers_edit_distance",
     "phx_serve_start", "phx_serve_submit", "phx_serve_poll", "phx_serve_stats_get",
-    "phx_serve_stop", "phx_estimate_request", "phx_version",
+    "phx_serve_stop", "phx_estimate_request", "phx_contig_anchors", "phx_contig_plan_create",
+    "phx_contig_plan_run", "phx_contig_plan_results", "phx_contig_plan_gaps",
+    "phx_contig_plan_layout", "phx_contig_plan_gap_plan", "phx_contig_plan_last_ms",
+    "phx_contig_plan_last_gpu_ms", "phx_contig_plan_last_stitch_ms", "phx_contig_plan_destroy",
+    "phx_version",
 )
 
 
@@ -102,6 +106,11 @@ class CServeStats(ctypes.Structure):
                  "arena_bytes", "arena_used_bytes", "arena_peak_bytes")]
 
 
+class CContigParams(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_uint32), ("w", ctypes.c_uint32), ("gap_min", ctypes.c_uint32),
+                ("gap_max", ctypes.c_uint32), ("seed", ctypes.c_uint64)]
+
+
 _lib = None
 
 
@@ -152,6 +161,23 @@ def lib() -> ctypes.CDLL:
     L.phx_serve_stop.argtypes = [P]
     L.phx_estimate_request.argtypes = [ctypes.c_int32, Costs, P, u64, P, u64, P]
     L.phx_estimate_request.restype = ctypes.c_int64
+    L.phx_contig_anchors.argtypes = [P, u64, P, u64, P, u64, P, u64]
+    L.phx_contig_anchors.restype = ctypes.c_int64
+    L.phx_contig_plan_create.argtypes = [P, ctypes.c_int32, Costs, P, P]
+    L.phx_contig_plan_create.restype = P
+    L.phx_contig_plan_run.argtypes = [P]
+    L.phx_contig_plan_results.argtypes = [P, P]
+    L.phx_contig_plan_gaps.argtypes = [P]
+    L.phx_contig_plan_gaps.restype = u64
+    L.phx_contig_plan_layout.argtypes = [P, ctypes.c_uint32, P, u64]
+    L.phx_contig_plan_layout.restype = ctypes.c_int64
+    L.phx_contig_plan_gap_plan.argtypes = [P]
+    L.phx_contig_plan_gap_plan.restype = P
+    for f in ("phx_contig_plan_last_ms", "phx_contig_plan_last_gpu_ms",
+              "phx_contig_plan_last_stitch_ms"):
+        getattr(L, f).argtypes = [P]
+        getattr(L, f).restype = ctypes.c_double
+    L.phx_contig_plan_destroy.argtypes = [P]
     L.phx_version.restype = ctypes.c_char_p
     _lib = L
     return L

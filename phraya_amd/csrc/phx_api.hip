@@ -47,6 +47,7 @@ struct phx_plan {
   uint64_t input_bytes = 0;                            // packed algorithmic input bytes
   int32_t ring_slots = 2;
   DevBuf d_seq, d_pairs, d_out, d_runs, d_work, d_work2, d_ctr, d_scratch;
+  DevBuf d_gather, d_gidx;  // compacted CIGAR runs and their source / target offsets
   DevBuf d_hist[16], d_ring[16];  // per launch group, so groups in flight never share scratch
   uint32_t* ctr_host = nullptr;  // pinned readback of the tier counters
   std::vector<std::vector<uint32_t>> work;             // initial work per launch group / Myers group
@@ -259,6 +260,118 @@ int run_wfa(phx_plan* P) {
   return PHX_OK;
 }
 
+// Work lists (length groups and tiers), memory estimates, device buffers and
+// descriptor upload; shared by packed-batch plans and window plans.  qh / th:
+// host bytes of each pair's query and target (memory estimates).  packed /
+// q_word / t_word: the plan's own sequence arena and each pair's word offsets
+// in it, or null when the descriptors already point into resident memory.
+phx_plan* plan_setup(phx_plan* P, const std::vector<const uint8_t*>& qh,
+                     const std::vector<const uint8_t*>& th, const std::vector<uint64_t>* packed,
+                     const std::vector<uint64_t>* q_word, const std::vector<uint64_t>* t_word) {
+  phx_ctx* ctx = P->ctx;
+  const uint32_t n = P->n;
+  // Work lists.
+  std::vector<uint32_t> order;
+  order.reserve(n);
+  for (uint32_t i = 0; i < n; ++i)
+    if (P->host_status[i] == ST_PENDING) order.push_back(i);
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) {
+    return P->qlen[a] + P->tlen[a] > P->qlen[c] + P->tlen[c];
+  });
+  if (is_wfa(P->engine)) {
+    // Group 0: LDS tier.  Groups 1..5: HBM tier, length buckets with edges
+    // 4096, 8192, 16384, 32768 (qn + tn); each bucket's slot fits the largest
+    // estimate in it.
+    P->work.assign(6, {});
+    P->group_tier.assign(6, 1);
+    P->group_tier[0] = 0;
+    P->group_slot.assign(6, 0);
+    // Step estimates from the sequences (k-mer event rate) for the pairs that
+    // need HBM slots, on host threads.
+    auto lds_ok = [&](uint32_t i) {
+      return P->qlen[i] + P->tlen[i] <= kWaveMaxLen && P->ring_slots <= kWaveRingMax;
+    };
+    std::vector<int32_t> est(n, 0);
+    std::vector<uint32_t> wide;
+    for (uint32_t i : order)
+      if (!lds_ok(i)) wide.push_back(i);
+    parallel_for(wide.size(), [&](size_t k) {
+      const uint32_t i = wide[k];
+      est[i] = estimate_steps_seq(P->engine, P->costs, qh[i], P->qlen[i], th[i], P->tlen[i]);
+    });
+    std::vector<uint64_t> est_words(6, 0);
+    for (uint32_t i : order) {
+      const uint64_t L = P->qlen[i] + P->tlen[i];
+      int gi;
+      if (lds_ok(i)) gi = 0;
+      else gi = L <= 4096 ? 1 : L <= 8192 ? 2 : L <= 16384 ? 3 : L <= 32768 ? 4 : 5;
+      P->work[gi].push_back(i);
+      if (gi > 0) {
+        const int32_t qn = P->pairs[i].qn, tn = P->pairs[i].tn;
+        est_words[gi] = std::max(est_words[gi], exact_hist_words(est[i], qn, tn));
+      }
+    }
+    for (int gi = 1; gi < 6; ++gi) P->group_slot[gi] = (est_words[gi] * 4 + 255) & ~(uint64_t)255;
+    // Tiers: buckets up to 8192 run a wave per pair, 16384 a 256-thread
+    // workgroup per pair, longer a 1024-thread workgroup per pair.
+    P->group_tier = {0, 1, 1, 2, 3, 3};
+    // HBM budget shares follow each group's estimated history volume.
+    std::vector<double> vol(6, 0.0);
+    double tot = 0;
+    for (int gi = 1; gi < 6; ++gi) {
+      for (uint32_t i : P->work[gi]) {
+        const int32_t qn = P->pairs[i].qn, tn = P->pairs[i].tn;
+        vol[gi] += (double)exact_hist_words(est[i], qn, tn);
+      }
+      tot += vol[gi];
+    }
+    P->group_share.assign(6, 0.0);
+    for (int gi = 1; gi < 6; ++gi) P->group_share[gi] = tot > 0 ? std::max(0.02, vol[gi] / tot) : 0.2;
+  } else {
+    P->work.assign(9, {});
+    for (uint32_t i : order) {
+      const uint64_t m = P->qlen[i];
+      const uint64_t nb = m == 0 ? 1 : (m + 63) / 64;
+      P->work[nb <= 8 ? nb : 0].push_back(i);
+    }
+  }
+  // Device buffers.
+  auto fail_plan = [&](const char* what) -> phx_plan* {
+    ctx->err = std::string("device allocation failed: ") + what;
+    phx_plan_destroy(P);
+    return nullptr;
+  };
+  const uint64_t words = packed ? packed->size() : 0;
+  if (dev_alloc(ctx, P->d_seq, words * 8 + 8) != PHX_OK) return fail_plan("seq");
+  if (dev_alloc(ctx, P->d_pairs, sizeof(PairDesc) * (size_t)std::max<uint32_t>(n, 1)) != PHX_OK) return fail_plan("pairs");
+  if (dev_alloc(ctx, P->d_out, sizeof(PairOut) * (size_t)std::max<uint32_t>(n, 1)) != PHX_OK) return fail_plan("out");
+  if (dev_alloc(ctx, P->d_runs, P->runs_total * 4 + 4) != PHX_OK) return fail_plan("runs");
+  if (dev_alloc(ctx, P->d_work, (size_t)std::max<uint32_t>(n, 1) * 4) != PHX_OK) return fail_plan("work");
+  if (dev_alloc(ctx, P->d_ctr, 256) != PHX_OK) return fail_plan("counters");
+  // Descriptors address their sequences by device pointer (packed batches:
+  // into the plan's own arena; window plans come with pointers set).
+  if (packed) {
+    for (uint32_t i = 0; i < n; ++i) {
+      P->pairs[i].q = (const uint64_t*)P->d_seq.p + (*q_word)[i];
+      P->pairs[i].t = (const uint64_t*)P->d_seq.p + (*t_word)[i];
+    }
+  }
+  if ((words && hipMemcpy(P->d_seq.p, packed->data(), words * 8, hipMemcpyHostToDevice) != hipSuccess) ||
+      (n && hipMemcpy(P->d_pairs.p, P->pairs.data(), sizeof(PairDesc) * n, hipMemcpyHostToDevice) != hipSuccess))
+    return fail_plan("upload");
+  if (hipHostMalloc((void**)&P->ctr_host, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
+    return fail_plan("pinned counters");
+  {
+    // Work lists are static: upload once, grouped by tier / block count.
+    std::vector<uint32_t> flat;
+    for (auto& w : P->work) flat.insert(flat.end(), w.begin(), w.end());
+    if (!flat.empty() &&
+        hipMemcpy(P->d_work.p, flat.data(), flat.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+      return fail_plan("work upload");
+  }
+  return P;
+}
+
 int run_myers(phx_plan* P) {
   phx_ctx* ctx = P->ctx;
   uint32_t* d_list_base = (uint32_t*)P->d_work.p;
@@ -305,6 +418,55 @@ int run_myers(phx_plan* P) {
 }
 
 }  // namespace
+
+namespace phx {
+namespace host {
+
+phx_plan* plan_create_windows(phx_ctx* ctx, int engine, phx_costs costs, int64_t cap,
+                              std::vector<PairDesc> pairs, const std::vector<const uint8_t*>& qh,
+                              const std::vector<const uint8_t*>& th) {
+  if (!ctx || (engine != PHX_ENGINE_LINEAR && engine != PHX_ENGINE_AFFINE)) return nullptr;
+  if (engine == PHX_ENGINE_AFFINE &&
+      (costs.mismatch < 1 || costs.gap_extend < 1 || costs.gap_open < 0 ||
+       ring_slots(engine, costs) > kRingSlotsMax)) {
+    ctx->err = "affine costs need mismatch >= 1, gap_extend >= 1, gap_open >= 0";
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (hipSetDevice(ctx->device) != hipSuccess) {
+    ctx->err = "hipSetDevice failed";
+    return nullptr;
+  }
+  phx_plan* P = new phx_plan;
+  P->ctx = ctx;
+  P->engine = engine;
+  P->costs = engine == PHX_ENGINE_AFFINE ? costs : phx_costs{0, 0, 0};
+  P->cap = cap;
+  P->n = (uint32_t)pairs.size();
+  P->ring_slots = ring_slots(engine, P->costs);
+  P->seed_q.assign(P->n, 0);
+  P->seed_t.assign(P->n, 0);
+  P->qlen.assign(P->n, 0);
+  P->tlen.assign(P->n, 0);
+  P->host_status.assign(P->n, ST_PENDING);
+  uint64_t runs = 0;
+  for (uint32_t i = 0; i < P->n; ++i) {
+    PairDesc& pd = pairs[i];
+    P->qlen[i] = (uint64_t)pd.qn;
+    P->tlen[i] = (uint64_t)pd.tn;
+    pd.run_cap = (uint32_t)(pd.qn + pd.tn + 1);
+    pd.run_off = runs;
+    runs += pd.run_cap;
+    const bool b8 = pd.flags & PF_BYTES;
+    P->input_bytes += b8 ? (uint64_t)pd.qn + pd.tn : ((uint64_t)pd.qn + 3) / 4 + ((uint64_t)pd.tn + 3) / 4;
+  }
+  P->runs_total = runs;
+  P->pairs = std::move(pairs);
+  return plan_setup(P, qh, th, nullptr, nullptr, nullptr);
+}
+
+}  // namespace host
+}  // namespace phx
 
 extern "C" {
 
@@ -472,103 +634,13 @@ phx_plan* phx_plan_create(phx_ctx* ctx, const phx_batch* b) {
     pack_into(packed.data() + q_word[i], b->q_seq + b->q_off[i] + P->seed_q[i], P->qlen[i], b8[i]);
     pack_into(packed.data() + t_word[i], b->t_seq + b->t_off[i] + P->seed_t[i], P->tlen[i], b8[i]);
   }
-  // Work lists.
-  std::vector<uint32_t> order;
-  order.reserve(n);
-  for (uint32_t i = 0; i < n; ++i)
-    if (P->host_status[i] == ST_PENDING) order.push_back(i);
-  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) {
-    return P->qlen[a] + P->tlen[a] > P->qlen[c] + P->tlen[c];
-  });
-  if (is_wfa(b->engine)) {
-    // Group 0: LDS tier.  Groups 1..5: HBM tier, length buckets with edges
-    // 4096, 8192, 16384, 32768 (qn + tn); each bucket's slot fits the largest
-    // estimate in it.
-    P->work.assign(6, {});
-    P->group_tier.assign(6, 1);
-    P->group_tier[0] = 0;
-    P->group_slot.assign(6, 0);
-    // Step estimates from the sequences (k-mer event rate) for the pairs that
-    // need HBM slots, on host threads.
-    auto lds_ok = [&](uint32_t i) {
-      return P->qlen[i] + P->tlen[i] <= kWaveMaxLen && P->ring_slots <= kWaveRingMax;
-    };
-    std::vector<int32_t> est(n, 0);
-    std::vector<uint32_t> wide;
-    for (uint32_t i : order)
-      if (!lds_ok(i)) wide.push_back(i);
-    parallel_for(wide.size(), [&](size_t k) {
-      const uint32_t i = wide[k];
-      est[i] = estimate_steps_seq(b->engine, b->costs, b->q_seq + b->q_off[i] + P->seed_q[i],
-                                  P->qlen[i], b->t_seq + b->t_off[i] + P->seed_t[i], P->tlen[i]);
-    });
-    std::vector<uint64_t> est_words(6, 0);
-    for (uint32_t i : order) {
-      const uint64_t L = P->qlen[i] + P->tlen[i];
-      int gi;
-      if (lds_ok(i)) gi = 0;
-      else gi = L <= 4096 ? 1 : L <= 8192 ? 2 : L <= 16384 ? 3 : L <= 32768 ? 4 : 5;
-      P->work[gi].push_back(i);
-      if (gi > 0) {
-        const int32_t qn = P->pairs[i].qn, tn = P->pairs[i].tn;
-        est_words[gi] = std::max(est_words[gi], exact_hist_words(est[i], qn, tn));
-      }
-    }
-    for (int gi = 1; gi < 6; ++gi) P->group_slot[gi] = (est_words[gi] * 4 + 255) & ~(uint64_t)255;
-    // Tiers: buckets up to 8192 run a wave per pair, 16384 a 256-thread
-    // workgroup per pair, longer a 1024-thread workgroup per pair.
-    P->group_tier = {0, 1, 1, 2, 3, 3};
-    // HBM budget shares follow each group's estimated history volume.
-    std::vector<double> vol(6, 0.0);
-    double tot = 0;
-    for (int gi = 1; gi < 6; ++gi) {
-      for (uint32_t i : P->work[gi]) {
-        const int32_t qn = P->pairs[i].qn, tn = P->pairs[i].tn;
-        vol[gi] += (double)exact_hist_words(est[i], qn, tn);
-      }
-      tot += vol[gi];
-    }
-    P->group_share.assign(6, 0.0);
-    for (int gi = 1; gi < 6; ++gi) P->group_share[gi] = tot > 0 ? std::max(0.02, vol[gi] / tot) : 0.2;
-  } else {
-    P->work.assign(9, {});
-    for (uint32_t i : order) {
-      const uint64_t m = P->qlen[i];
-      const uint64_t nb = m == 0 ? 1 : (m + 63) / 64;
-      P->work[nb <= 8 ? nb : 0].push_back(i);
-    }
-  }
-  // Device buffers.
-  auto fail_plan = [&](const char* what) -> phx_plan* {
-    ctx->err = std::string("device allocation failed: ") + what;
-    phx_plan_destroy(P);
-    return nullptr;
-  };
-  if (dev_alloc(ctx, P->d_seq, words * 8 + 8) != PHX_OK) return fail_plan("seq");
-  if (dev_alloc(ctx, P->d_pairs, sizeof(PairDesc) * (size_t)std::max<uint32_t>(n, 1)) != PHX_OK) return fail_plan("pairs");
-  if (dev_alloc(ctx, P->d_out, sizeof(PairOut) * (size_t)std::max<uint32_t>(n, 1)) != PHX_OK) return fail_plan("out");
-  if (dev_alloc(ctx, P->d_runs, runs * 4 + 4) != PHX_OK) return fail_plan("runs");
-  if (dev_alloc(ctx, P->d_work, (size_t)std::max<uint32_t>(n, 1) * 4) != PHX_OK) return fail_plan("work");
-  if (dev_alloc(ctx, P->d_ctr, 256) != PHX_OK) return fail_plan("counters");
-  // Descriptors address their sequences by device pointer.
+  std::vector<const uint8_t*> qh(n, nullptr), th(n, nullptr);
   for (uint32_t i = 0; i < n; ++i) {
-    P->pairs[i].q = (const uint64_t*)P->d_seq.p + q_word[i];
-    P->pairs[i].t = (const uint64_t*)P->d_seq.p + t_word[i];
+    if (P->host_status[i] != ST_PENDING) continue;
+    qh[i] = b->q_seq + b->q_off[i] + P->seed_q[i];
+    th[i] = b->t_seq + b->t_off[i] + P->seed_t[i];
   }
-  if (hipMemcpy(P->d_seq.p, packed.data(), words * 8, hipMemcpyHostToDevice) != hipSuccess ||
-      (n && hipMemcpy(P->d_pairs.p, P->pairs.data(), sizeof(PairDesc) * n, hipMemcpyHostToDevice) != hipSuccess))
-    return fail_plan("upload");
-  if (hipHostMalloc((void**)&P->ctr_host, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
-    return fail_plan("pinned counters");
-  {
-    // Work lists are static: upload once, grouped by tier / block count.
-    std::vector<uint32_t> flat;
-    for (auto& w : P->work) flat.insert(flat.end(), w.begin(), w.end());
-    if (!flat.empty() &&
-        hipMemcpy(P->d_work.p, flat.data(), flat.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
-      return fail_plan("work upload");
-  }
-  return P;
+  return plan_setup(P, qh, th, &packed, &q_word, &t_word);
 }
 
 int phx_plan_run(phx_plan* P) {
@@ -629,11 +701,8 @@ int phx_plan_results(phx_plan* P, phx_results* r) {
   HIPCHK(hipSetDevice(ctx->device));
   int rc = fetch_out(P);
   if (rc != PHX_OK) return rc;
-  std::vector<uint32_t> runs_h;
-  if (has_cigar(P->engine) && P->runs_total) {
-    runs_h.resize(P->runs_total);
-    HIPCHK(hipMemcpy(runs_h.data(), P->d_runs.p, P->runs_total * 4, hipMemcpyDeviceToHost));
-  }
+  // Per-pair fields and the compacted run layout.
+  std::vector<uint64_t> src(P->n, 0);
   uint64_t pos = 0;
   int ret = PHX_OK;
   for (uint32_t i = 0; i < P->n; ++i) {
@@ -665,25 +734,37 @@ int phx_plan_results(phx_plan* P, phx_results* r) {
       a.target_end = (int64_t)P->tlen[i];
     }
     if (has_cigar(P->engine) && o.nruns > 0) {
-      if (r->runs && pos + (uint64_t)o.nruns <= r->runs_cap)
-        std::memcpy(r->runs + pos, runs_h.data() + P->pairs[i].run_off + (uint64_t)o.run_start,
-                    (size_t)o.nruns * 4);
+      src[i] = P->pairs[i].run_off + (uint64_t)o.run_start;
       pos += (uint64_t)o.nruns;
     }
   }
   r->run_off[P->n] = pos;
-  if (ret == PHX_OK && has_cigar(P->engine) && (pos > r->runs_cap || (!r->runs && pos > 0))) {
+  if (ret != PHX_OK) return ret;
+  if (!has_cigar(P->engine) || pos == 0) return PHX_OK;
+  if (pos > r->runs_cap || !r->runs) {
     ctx->err = "CIGAR arena too small";
     return PHX_E_CAPACITY;
   }
-  return ret;
+  // Gather the used runs on the device, then one copy of exactly those words.
+  std::vector<uint64_t> idx(2 * (size_t)P->n + 1);
+  std::copy(src.begin(), src.end(), idx.begin());
+  std::copy(r->run_off, r->run_off + P->n + 1, idx.begin() + P->n);
+  if (dev_alloc(ctx, P->d_gidx, idx.size() * 8) != PHX_OK) return PHX_E_NOMEM;
+  if (dev_alloc(ctx, P->d_gather, pos * 4) != PHX_OK) return PHX_E_NOMEM;
+  HIPCHK(hipMemcpyAsync(P->d_gidx.p, idx.data(), idx.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+  const uint64_t* d_src = (const uint64_t*)P->d_gidx.p;
+  HIPCHK(launch_gather_runs((const uint32_t*)P->d_runs.p, d_src, d_src + P->n,
+                            (uint32_t*)P->d_gather.p, P->n, ctx->stream));
+  HIPCHK(hipMemcpyAsync(r->runs, P->d_gather.p, pos * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return PHX_OK;
 }
 
 void phx_plan_destroy(phx_plan* P) {
   if (!P) return;
   (void)hipStreamSynchronize(P->ctx->stream);
   for (DevBuf* b : {&P->d_seq, &P->d_pairs, &P->d_out, &P->d_runs, &P->d_work, &P->d_work2, &P->d_ctr,
-                    &P->d_scratch})
+                    &P->d_scratch, &P->d_gather, &P->d_gidx})
     dev_free(*b);
   for (int t = 0; t < 16; ++t) {
     dev_free(P->d_hist[t]);

@@ -28,6 +28,8 @@ hipError_t launch_myers(int nb_template, const MyersArgs& a, hipStream_t st);
 hipError_t launch_wfa_wave(int mode, bool lds, const WfaArgs& a, const WaveGeom& G,
                            uint32_t grid, hipStream_t st);
 hipError_t launch_wfa_block(int mode, int nt, const WfaArgs& a, uint32_t grid, hipStream_t st);
+hipError_t launch_gather_runs(const uint32_t* runs, const uint64_t* src, const uint64_t* off,
+                              uint32_t* dst, uint32_t n, hipStream_t st);
 
 namespace host {
 
@@ -283,6 +285,14 @@ inline hipError_t launch_tier(int tier, int mode, const WfaArgs& a, const WaveGe
   }
   return launch_wfa_block(mode, tier == TIER_BLOCK256 ? 256 : 1024, a, std::max<uint32_t>(1, slots), st);
 }
+
+// A batch plan over windows of resident sequences: the descriptors carry
+// device pointers, shifts, lengths and flags; qh / th are host copies of each
+// window (memory estimates).  Run and read it with phx_plan_run /
+// phx_plan_results.  Defined in phx_api.hip.
+phx_plan* plan_create_windows(phx_ctx* ctx, int engine, phx_costs costs, int64_t cap,
+                              std::vector<PairDesc> pairs, const std::vector<const uint8_t*>& qh,
+                              const std::vector<const uint8_t*>& th);
 
 // Tier of an HBM pair by qn + tn: a wave up to 8192, then workgroups.
 inline int hbm_tier_for(uint64_t L) {

@@ -56,6 +56,7 @@ typedef struct synth_params {
   double b_indel_frac;
   double indel_mean;         // geometric indel length mean (p = 1 / mean)
   int32_t indel_max;         // indel length truncation
+  int32_t indel_uniform;     // 1: indel lengths uniform in [1, indel_max] instead
 } synth_params;
 
 void synth_genome(uint64_t len, uint64_t seed, uint8_t* out) {
@@ -69,13 +70,16 @@ void synth_genome(uint64_t len, uint64_t seed, uint8_t* out) {
 
 // Mutate src[0..n) into out; stops once out holds `want` bases.
 static void mutate(const uint8_t* src, uint64_t n, double sub, double indel, double p_len,
-                   int32_t max_len, SplitMix64& r, std::vector<uint8_t>& out, uint64_t want) {
+                   int32_t max_len, bool uniform_len, SplitMix64& r, std::vector<uint8_t>& out,
+                   uint64_t want) {
   uint64_t i = 0;
   while (i < n && out.size() < want) {
     const uint8_t b = src[i];
     if (indel > 0 && r.uniform() < indel) {
       int32_t len = 1;
-      while (len < max_len && r.uniform() >= p_len) ++len;
+      if (uniform_len) len = 1 + (int32_t)r.below((uint64_t)std::max(max_len, 1));
+      else
+        while (len < max_len && r.uniform() >= p_len) ++len;
       if (r.uniform() < 0.5) {
         for (int32_t k = 0; k < len; ++k) out.push_back((uint8_t)kBases[r.below(4)]);
         continue;
@@ -122,7 +126,8 @@ void* synth_pairs(const synth_params* p, const uint8_t* genome) {
     const double ifrac = a ? p->a_indel_frac : p->b_indel_frac;
     const double e = lo + (hi - lo) * r.uniform();
     buf.clear();
-    mutate(genome + pos, src_len, e * (1.0 - ifrac), e * ifrac, p_len, p->indel_max, r, buf, L);
+    mutate(genome + pos, src_len, e * (1.0 - ifrac), e * ifrac, p_len, p->indel_max,
+           p->indel_uniform != 0, r, buf, L);
     g->q.insert(g->q.end(), buf.begin(), buf.end());
     g->qo.push_back(g->q.size());
     g->t.insert(g->t.end(), genome + pos, genome + pos + W);

@@ -590,6 +590,27 @@ __global__ __launch_bounds__(NT) void wfa_block_kernel(WfaArgs a) {
   }
 }
 
+// Compacts CIGAR runs: pair i's count = off[i+1] - off[i] words move from
+// runs[src[i] ..] to dst[off[i] ..]; a wave per pair.
+__global__ __launch_bounds__(256) void gather_runs_kernel(const uint32_t* __restrict__ runs,
+                                                          const uint64_t* __restrict__ src,
+                                                          const uint64_t* __restrict__ off,
+                                                          uint32_t* __restrict__ dst, uint32_t n) {
+  const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63;
+  if (i >= n) return;
+  const uint64_t a = off[i], cnt = off[i + 1] - a;
+  const uint32_t* s = runs + src[i];
+  for (uint64_t j = lane; j < cnt; j += 64) dst[a + j] = s[j];
+}
+
+hipError_t launch_gather_runs(const uint32_t* runs, const uint64_t* src, const uint64_t* off,
+                              uint32_t* dst, uint32_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_runs_kernel, dim3((n + 3) / 4), dim3(256), 0, st, runs, src, off, dst, n);
+  return hipGetLastError();
+}
+
 __global__ void zero_u32_kernel(uint32_t* p, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;

@@ -35,6 +35,7 @@ class SynthParams(ctypes.Structure):
         ("a_indel_frac", ctypes.c_double), ("b_err_lo", ctypes.c_double),
         ("b_err_hi", ctypes.c_double), ("b_indel_frac", ctypes.c_double),
         ("indel_mean", ctypes.c_double), ("indel_max", ctypes.c_int32),
+        ("indel_uniform", ctypes.c_int32),
     ]
 
 
@@ -133,6 +134,13 @@ WORKLOADS = {
                      len_hi=150, log_uniform=0, window_scale=1.0, window_add=30,
                      class_a_frac=1.0, a_err_lo=0.0, a_err_hi=0.05, a_indel_frac=0.2,
                      b_err_lo=0, b_err_hi=0, b_indel_frac=0, indel_mean=2.0, indel_max=3),
+    # Contig segments 100 kbp - 1 Mbp (log-uniform), divergence U[0.5%, 2%], 10% of
+    # events indels of length U[1, 50]; query = mutated segment, target = segment.
+    "contig": dict(genome_seed=0x5EED0001, read_seed=0xC0FFEE04, n=2_000, len_lo=100_000,
+                   len_hi=1_000_000, log_uniform=1, window_scale=1.0, window_add=0,
+                   class_a_frac=1.0, a_err_lo=0.005, a_err_hi=0.02, a_indel_frac=0.1,
+                   b_err_lo=0, b_err_hi=0, b_indel_frac=0, indel_mean=25.0, indel_max=50,
+                   indel_uniform=1),
     "mixed": dict(genome_seed=0x5EED0001, read_seed=0xC0FFEE03, n=200_000, len_lo=100,
                   len_hi=20_000, log_uniform=1, window_scale=1.1, window_add=0,
                   class_a_frac=0.7, a_err_lo=0.01, a_err_hi=0.01, a_indel_frac=0.1,
