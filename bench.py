@@ -178,6 +178,12 @@ def run_contig(args):
         t_create = time.perf_counter() - t0
         G = plan.gaps
         log(f"{engine}: {G} gaps over {n} contigs (anchoring + upload {t_create:.1f} s)")
+        tuned, tune_s = None, 0.0
+        if not args.no_autotune:
+            t0 = time.perf_counter()
+            tuned, tuned_ms = plan.autotune()
+            tune_s = time.perf_counter() - t0
+            log(f"{engine}: autotune {tune_s:.1f} s -> {tuned} ({tuned_ms:.1f} ms)")
         for _ in range(max(1, args.warmup)):
             plan.run()
         steps = max(1, args.steps)
@@ -201,7 +207,7 @@ def run_contig(args):
         ms = float(np.mean(walls)) * 1000.0
         res_by[engine] = dict(res=res, G=G, ms=ms, gpu_ms=float(np.mean(gpu)),
                               stitch_ms=float(np.mean(stitch)), kernel_ms=float(np.mean(kern)),
-                              alg=alg, create_s=t_create)
+                              alg=alg, create_s=t_create, tuning=tuned, autotune_s=tune_s)
         log(f"{engine}: {G / (ms / 1000):.4g} gaps/s, {ms:.1f} ms/step "
             f"(gpu {np.mean(gpu):.1f} ms, stitch {np.mean(stitch):.1f} ms)")
         plan.close()
@@ -275,10 +281,12 @@ def run_contig(args):
                    "parallelism": "single GPU"},
         "affine": {"gaps_per_s": A["G"] / (A["ms"] / 1000.0), "contigs_per_s": n / (A["ms"] / 1000.0),
                    "ms_per_step": A["ms"], "gpu_ms": A["gpu_ms"], "stitch_ms": A["stitch_ms"],
-                   "kernel_ms_sum": A["kernel_ms"], "plan_create_s": A["create_s"]},
+                   "kernel_ms_sum": A["kernel_ms"], "plan_create_s": A["create_s"],
+                   "tuning": A["tuning"], "autotune_s": A["autotune_s"]},
         "linear": {"gaps_per_s": Lr["G"] / (Lr["ms"] / 1000.0), "contigs_per_s": n / (Lr["ms"] / 1000.0),
                    "ms_per_step": Lr["ms"], "gpu_ms": Lr["gpu_ms"], "stitch_ms": Lr["stitch_ms"],
-                   "kernel_ms_sum": Lr["kernel_ms"], "plan_create_s": Lr["create_s"]},
+                   "kernel_ms_sum": Lr["kernel_ms"], "plan_create_s": Lr["create_s"],
+                   "tuning": Lr["tuning"], "autotune_s": Lr["autotune_s"]},
         "gcups_equiv": gcells / (A["ms"] / 1000.0) / 1e9,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
@@ -487,6 +495,10 @@ def main():
                     help="serve: offered requests/s (overrides --serve-load)")
     ap.add_argument("--serve-lanes", type=int, default=3,
                     help="serve: launch groups in flight per tier")
+    ap.add_argument("--tune", default="",
+                    help="phx_tuning overrides, e.g. block256_min_len=16384,heavy_permille=20")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the autotuner (runs before the timed steps)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
@@ -507,6 +519,11 @@ def main():
     import phraya_amd as px
     from phraya_amd import synth
 
+    if args.tune:
+        tuning = {k: int(v) for k, v in (kv.split("=") for kv in args.tune.split(","))}
+        px.default_context(tuning)
+        log(f"tuning {tuning}")
+
     if args.workload == "serve":
         run_serve(args)
         return
@@ -524,6 +541,12 @@ def main():
                         np.zeros(n, np.uint64))
     log(f"uploading {n} pairs")
     plan = px.Plan(pb, eng_id, px.AffineCosts(*costs) if engine == "affine" else px.AffineCosts())
+    tuned, tune_s = None, 0.0
+    if not args.no_autotune and engine in ("affine", "linear"):
+        t0 = time.perf_counter()
+        tuned, tuned_ms = plan.autotune()
+        tune_s = time.perf_counter() - t0
+        log(f"autotune {tune_s:.1f} s -> {tuned} ({tuned_ms:.3f} ms)")
     log(f"warmup {args.warmup}")
     for _ in range(args.warmup):
         plan.run()
@@ -577,7 +600,7 @@ def main():
     if not args.no_cpu:
         log(f"cpu baseline ~{args.cpu_seconds:.0f} s")
         cpu = cpu_baseline(w, engine, costs, args.cpu_seconds,
-                           sample_n={"short150": None, "myers150": 20_000, "mixed": 500}[args.workload])
+                           sample_n={"short150": None, "myers150": 20_000, "mixed": 200}[args.workload])
 
     out = {
         "metric": METRIC,
@@ -594,7 +617,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "description": desc, "pairs_per_step": n,
                    "engine": engine, "costs": list(costs) if engine == "affine" else None,
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU", "tuning": tuned,
+                   "autotune_s": tune_s},
         "latency_ms": {"p50": float(np.percentile(step_ms, 50)),
                        "p99": float(np.percentile(step_ms, 99)),
                        "what": "device time of one whole batch (events around phx_plan_run)"},

@@ -102,9 +102,13 @@ typedef struct phx_plan phx_plan;
 typedef struct phx_tuning {
   uint32_t wave_lds_kib;     /* LDS tier: LDS bytes (KiB) per wave (ring + sequences + history) */
   uint32_t wave_slots;       /* LDS tier: pairs in flight */
-  uint32_t hbm_slots;        /* HBM tiers: pairs in flight */
-  uint32_t hbm_slot_mib;     /* HBM tier 1: history slot per pair (MiB) */
+  uint32_t hbm_slots;        /* HBM wave tier: pairs in flight */
+  uint32_t hbm_slot_mib;     /* HBM wave tier: history slot per pair when no estimate (MiB) */
   uint64_t hbm_budget_mib;   /* device scratch budget of the HBM tiers (MiB) */
+  uint32_t block256_min_len; /* qn + tn above which a pair runs a 256-thread workgroup (0 = 8192) */
+  uint32_t block1024_min_len;/* ... a 1024-thread workgroup (0 = 16384) */
+  uint32_t heavy_permille;   /* per mille of each length bucket (largest estimates) run one tier wider (0 = 50) */
+  uint32_t reserved;
 } phx_tuning;
 
 /* Context: owns the stream, device buffers and the submission lock.
@@ -134,6 +138,17 @@ int phx_plan_results(phx_plan* plan, phx_results* out);
 double phx_plan_last_ms(const phx_plan* plan);
 double phx_plan_last_kernel_ms(const phx_plan* plan);
 uint32_t phx_plan_last_launches(const phx_plan* plan);
+/* Re-tune a plan (non-zero fields of *tuning replace the plan's): the pairs
+ * are regrouped into tiers without re-uploading them.  Results never depend
+ * on the tuning. */
+int phx_plan_set_tuning(phx_plan* plan, const phx_tuning* tuning);
+int phx_plan_get_tuning(const phx_plan* plan, phx_tuning* tuning);
+/* Autotuner: times the plan under candidate tunings (tier length edges, i.e.
+ * threads per pair, and heavy share; HBM wave-tier pairs in flight; LDS per
+ * wave vs waves in flight of the LDS tier), stage by stage, keeps the
+ * fastest and applies it.  max_trials = 0: no limit (about 20 trials).
+ * *best / *best_ms (may be NULL) get the chosen tuning and its run time. */
+int phx_plan_autotune(phx_plan* plan, uint32_t max_trials, phx_tuning* best, double* best_ms);
 /* Algorithmic HBM bytes of one run: packed inputs + results + CIGAR runs. */
 uint64_t phx_plan_algorithmic_bytes(const phx_plan* plan);
 void phx_plan_destroy(phx_plan* plan);

@@ -124,11 +124,13 @@ _default_ctx: Optional[Context] = None
 _ctx_lock = threading.Lock()
 
 
-def default_context() -> Context:
+def default_context(tuning: Optional[dict] = None) -> Context:
+    """The process-wide context; `tuning` (phx_tuning fields) applies when it
+    is first created."""
     global _default_ctx
     with _ctx_lock:
         if _default_ctx is None:
-            _default_ctx = Context(0)
+            _default_ctx = Context(0, tuning)
         return _default_ctx
 
 
@@ -237,6 +239,21 @@ class Plan:
     def last_launches(self) -> int:
         return int(L.lib().phx_plan_last_launches(self._h))
 
+    def tuning(self) -> dict:
+        t = L.CTuning()
+        L.lib().phx_plan_get_tuning(self._h, ctypes.byref(t))
+        return {name: getattr(t, name) for name, _ in L.CTuning._fields_ if name != "reserved"}
+
+    def set_tuning(self, tuning: dict) -> None:
+        t = L.CTuning(**tuning)
+        rc = L.lib().phx_plan_set_tuning(self._h, ctypes.byref(t))
+        if rc != L.PHX_OK:
+            raise RuntimeError(f"phx_plan_set_tuning: status {rc}")
+
+    def autotune(self, max_trials: int = 0) -> tuple[dict, float]:
+        """Time candidate tunings on this plan, keep the fastest (phx_plan_autotune)."""
+        return _autotune(self._h, max_trials)
+
     def algorithmic_bytes(self) -> int:
         return int(L.lib().phx_plan_algorithmic_bytes(self._h))
 
@@ -265,6 +282,15 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def _autotune(handle, max_trials: int) -> tuple[dict, float]:
+    t = L.CTuning()
+    ms = ctypes.c_double(0)
+    rc = L.lib().phx_plan_autotune(handle, max_trials, ctypes.byref(t), ctypes.byref(ms))
+    if rc != L.PHX_OK:
+        raise RuntimeError(f"phx_plan_autotune: status {rc}")
+    return {name: getattr(t, name) for name, _ in L.CTuning._fields_ if name != "reserved"}, ms.value
 
 
 def align_batch(engine: int, queries: Sequence[bytes], targets: Sequence[bytes],
@@ -537,6 +563,10 @@ class ContigPlan:
 
     def gap_plan_algorithmic_bytes(self) -> int:
         return int(L.lib().phx_plan_algorithmic_bytes(L.lib().phx_contig_plan_gap_plan(self._h)))
+
+    def autotune(self, max_trials: int = 0) -> tuple[dict, float]:
+        """Tune the batched gap plan (tier edges, slots) on this contig set."""
+        return _autotune(L.lib().phx_contig_plan_gap_plan(self._h), max_trials)
 
     def gap_kernel_ms(self) -> float:
         return float(L.lib().phx_plan_last_kernel_ms(L.lib().phx_contig_plan_gap_plan(self._h)))

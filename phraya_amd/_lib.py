@@ -33,6 +33,7 @@ EXPORTS = (
     "phx_align_batch", "phx_plan_create", "phx_plan_run", "phx_plan_results",
     "phx_plan_last_ms", "phx_plan_last_kernel_ms", "phx_plan_last_launches",
     "phx_plan_algorithmic_bytes", "phx_plan_destroy", "phx_format_cigar",
+    "phx_plan_set_tuning", "phx_plan_get_tuning", "phx_plan_autotune",
     "phx_wfa_extend", "phx_wfa_extend_affine", "phx_myers_extend", "phx_myers_edit_distance",
     "phx_serve_start", "phx_serve_submit", "phx_serve_poll", "phx_serve_stats_get",
     "phx_serve_stop", "phx_estimate_request", "phx_contig_anchors", "phx_contig_plan_create",
@@ -78,7 +79,9 @@ class CResults(ctypes.Structure):
 class CTuning(ctypes.Structure):
     _fields_ = [("wave_lds_kib", ctypes.c_uint32), ("wave_slots", ctypes.c_uint32),
                 ("hbm_slots", ctypes.c_uint32), ("hbm_slot_mib", ctypes.c_uint32),
-                ("hbm_budget_mib", ctypes.c_uint64)]
+                ("hbm_budget_mib", ctypes.c_uint64), ("block256_min_len", ctypes.c_uint32),
+                ("block1024_min_len", ctypes.c_uint32), ("heavy_permille", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
 
 class CServeConfig(ctypes.Structure):
@@ -146,6 +149,9 @@ def lib() -> ctypes.CDLL:
     L.phx_plan_algorithmic_bytes.argtypes = [P]
     L.phx_plan_algorithmic_bytes.restype = ctypes.c_uint64
     L.phx_plan_destroy.argtypes = [P]
+    L.phx_plan_set_tuning.argtypes = [P, P]
+    L.phx_plan_get_tuning.argtypes = [P, P]
+    L.phx_plan_autotune.argtypes = [P, ctypes.c_uint32, P, P]
     L.phx_format_cigar.argtypes = [P, ctypes.c_uint64, P, ctypes.c_size_t]
     L.phx_format_cigar.restype = ctypes.c_size_t
     sz, u64 = ctypes.c_size_t, ctypes.c_uint64

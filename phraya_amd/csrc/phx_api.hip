@@ -54,9 +54,14 @@ struct phx_plan {
   std::vector<int> group_tier;                         // WFA: 0 = LDS tier, 1 = HBM tier
   std::vector<uint64_t> group_slot;                    // WFA: HBM slot bytes of each group
   std::vector<double> group_share;                     // WFA: share of the HBM budget
+  std::vector<double> group_load;                      // WFA: duration model of each group
+  phx_tuning tun{};                                    // this plan's tuning (from the context)
+  std::vector<uint32_t> order;                         // pending pairs, longest first
+  std::vector<int32_t> est;                            // WFA: estimated score steps per pair
   uint64_t group_budget(int buf) const {
-    const uint64_t total = ctx->tun.hbm_budget_mib << 20;
-    if (buf >= 0 && buf < (int)group_share.size()) return (uint64_t)(total * group_share[buf]);
+    const uint64_t total = tun.hbm_budget_mib << 20;
+    if (buf >= 0 && buf < (int)group_share.size())
+      return std::max<uint64_t>((uint64_t)(total * group_share[buf]), 64ull << 20);
     // Re-runs get what the initial groups leave of the budget.
     uint64_t held = 0;
     for (int i = 0; i < 15; ++i) held += d_hist[i].bytes + d_ring[i].bytes;
@@ -116,6 +121,20 @@ int kernel_events(phx_ctx* ctx, uint32_t i, hipEvent_t* a, hipEvent_t* b) {
 }
 
 bool is_wfa(int engine) { return engine == PHX_ENGINE_LINEAR || engine == PHX_ENGINE_AFFINE; }
+
+// base with every non-zero field of t applied.
+phx_tuning merge_tuning(phx_tuning d, const phx_tuning* t) {
+  if (!t) return d;
+  if (t->wave_lds_kib) d.wave_lds_kib = std::max<uint32_t>(1, std::min<uint32_t>(t->wave_lds_kib, 38));
+  if (t->wave_slots) d.wave_slots = t->wave_slots;
+  if (t->hbm_slots) d.hbm_slots = t->hbm_slots;
+  if (t->hbm_slot_mib) d.hbm_slot_mib = t->hbm_slot_mib;
+  if (t->hbm_budget_mib) d.hbm_budget_mib = t->hbm_budget_mib;
+  if (t->block256_min_len) d.block256_min_len = t->block256_min_len;
+  if (t->block1024_min_len) d.block1024_min_len = t->block1024_min_len;
+  if (t->heavy_permille) d.heavy_permille = std::min<uint32_t>(t->heavy_permille, 1000);
+  return d;
+}
 bool has_cigar(int engine) { return engine != PHX_ENGINE_MYERS_DIST; }
 
 // Launches one group: tier t (0 LDS, 1 HBM) over `work`, scratch buffers of
@@ -139,18 +158,18 @@ int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint3
   if (t == TIER_LDS) {
     uint32_t seq_words = 2;
     for (uint32_t i : work) seq_words = std::max(seq_words, lds_seq_words(P->pairs[i]));
-    G = lds_geometry(P->engine, R, seq_words, ctx->tun.wave_lds_kib);
-    slots = std::min<uint32_t>((uint32_t)work.size(), std::max<uint32_t>(kWaveWaves, ctx->tun.wave_slots));
+    G = lds_geometry(P->engine, R, seq_words, P->tun.wave_lds_kib);
+    slots = std::min<uint32_t>((uint32_t)work.size(), std::max<uint32_t>(kWaveWaves, P->tun.wave_slots));
   } else {
     uint64_t max_sum = 1;
     for (uint32_t i : work) max_sum = std::max<uint64_t>(max_sum, (uint64_t)P->pairs[i].qn + P->pairs[i].tn);
     uint32_t wcap = 0;
     const uint64_t ring_words = ring_words_for(P->engine, R, max_sum, &wcap);
-    const uint64_t hist = slot_bytes ? slot_bytes : ((uint64_t)ctx->tun.hbm_slot_mib << 20);
+    const uint64_t hist = slot_bytes ? slot_bytes : ((uint64_t)P->tun.hbm_slot_mib << 20);
     const uint64_t per = hist + ring_words * 4;
     const uint64_t budget = P->group_budget(buf);
     // Pairs in flight: one per wave (tier 1) or per workgroup (tiers 2-3).
-    const uint64_t max_flight = t == TIER_WAVE ? ctx->tun.hbm_slots
+    const uint64_t max_flight = t == TIER_WAVE ? P->tun.hbm_slots
                               : t == TIER_BLOCK256 ? (uint64_t)ctx->cu_count * 4 : (uint64_t)ctx->cu_count;
     uint64_t n = std::max<uint64_t>(1, budget / per);
     n = std::min<uint64_t>(std::min<uint64_t>(n, max_flight), work.size());
@@ -185,22 +204,32 @@ int run_wfa(phx_plan* P) {
   HIPCHK(launch_zero(d_ctr, 2 * (uint32_t)ngroups, ctx->stream));
   HIPCHK(hipEventRecord(ctx->join[0], ctx->stream));
   std::vector<int> launched;
-  uint64_t off = 0;
+  // Group gi's list sits at offset group_off[gi] of d_work.  Launch the
+  // groups longest first, each on the stream with the least work queued
+  // (groups on one stream run one after another).
+  std::vector<uint64_t> group_off(ngroups + 1, 0);
+  for (int gi = 0; gi < ngroups; ++gi) group_off[gi + 1] = group_off[gi] + P->work[gi].size();
+  std::vector<int> order(ngroups);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return P->group_load[a] > P->group_load[b];
+  });
+  double queued[kStreams] = {0, 0, 0, 0};
   bool used_stream[kStreams] = {true, false, false, false};
-  for (int gi = 0; gi < ngroups; ++gi) {
+  for (int gi : order) {
     const auto& wl = P->work[gi];
     if (wl.empty()) continue;
-    const int si = (int)launched.size() % kStreams;
+    const int si = (int)(std::min_element(queued, queued + kStreams) - queued);
+    queued[si] += std::max(P->group_load[gi], 1.0);
     hipStream_t st = ctx->streams[si];
     if (si != 0 && !used_stream[si]) {
       HIPCHK(hipStreamWaitEvent(st, ctx->join[0], 0));
       used_stream[si] = true;
     }
-    int rc = launch_wfa_tier(P, P->group_tier[gi], wl, (uint32_t*)P->d_work.p + off, d_ctr + 2 * gi,
-                             P->group_slot[gi], gi, st);
+    int rc = launch_wfa_tier(P, P->group_tier[gi], wl, (uint32_t*)P->d_work.p + group_off[gi],
+                             d_ctr + 2 * gi, P->group_slot[gi], gi, st);
     if (rc != PHX_OK) return rc;
     launched.push_back(gi);
-    off += wl.size();
   }
   if (launched.empty()) return PHX_OK;
   for (int si = 1; si < kStreams; ++si) {
@@ -243,7 +272,8 @@ int run_wfa(phx_plan* P) {
     HIPCHK(launch_zero(d_ctr, 2, ctx->stream));
     uint64_t maxL = 0;
     for (uint32_t i : wl) maxL = std::max<uint64_t>(maxL, (uint64_t)P->pairs[i].qn + P->pairs[i].tn);
-    const int rt = maxL <= 8192 ? TIER_WAVE : TIER_BLOCK1024;
+    // Re-runs are few and run after everything else: give them wide tiers.
+    const int rt = maxL <= 8192 ? TIER_BLOCK256 : TIER_BLOCK1024;
     int rc = launch_wfa_tier(P, rt, wl, (uint32_t*)P->d_work2.p, d_ctr, slot_bytes, 15, ctx->stream);
     if (rc != PHX_OK) return rc;
     HIPCHK(hipMemcpyAsync(ctr_h, d_ctr, 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -260,6 +290,125 @@ int run_wfa(phx_plan* P) {
   return PHX_OK;
 }
 
+// WFA launch groups from the plan's tuning: the LDS group, then HBM length
+// buckets, each split into regular and heavy groups, with slot sizes, budget
+// shares and a duration model for the launch order.
+void plan_group(phx_plan* P) {
+  phx_ctx* ctx = P->ctx;
+  // Group 0: LDS tier.  Then HBM length buckets with edges 4096, 8192,
+  // 16384, 32768 (qn + tn), each split by estimated history into a regular
+  // group (slot = the bucket's 95th-percentile estimate) and a heavy group
+  // (the rest, slot = its largest estimate), so a few outliers do not
+  // shrink the slots in flight of the whole bucket.  A pair that outgrows
+  // its slot still completes (exact re-run).
+  auto lds_ok = [&](uint32_t i) {
+    return P->qlen[i] + P->tlen[i] <= kWaveMaxLen && P->ring_slots <= kWaveRingMax;
+  };
+  auto words_of = [&](uint32_t i) {
+    return exact_hist_words(P->est[i], P->pairs[i].qn, P->pairs[i].tn);
+  };
+  std::vector<uint32_t> bucket[6];
+  for (uint32_t i : P->order) {
+    const uint64_t L = P->qlen[i] + P->tlen[i];
+    const int b = lds_ok(i) ? 0 : L <= 4096 ? 1 : L <= 8192 ? 2 : L <= 16384 ? 3 : L <= 32768 ? 4 : 5;
+    bucket[b].push_back(i);
+  }
+  // Tier of each bucket from its upper length edge and the tuning.
+  static const uint64_t kBucketEdge[6] = {2048, 4096, 8192, 16384, 32768, ~0ull};
+  int kBucketTier[6];
+  for (int b = 0; b < 6; ++b)
+    kBucketTier[b] = b == 0 ? TIER_LDS
+                   : kBucketEdge[b] > P->tun.block1024_min_len ? TIER_BLOCK1024
+                   : kBucketEdge[b] > P->tun.block256_min_len ? TIER_BLOCK256 : TIER_WAVE;
+  P->work.clear();
+  P->group_tier.clear();
+  P->group_slot.clear();
+  P->work.push_back(bucket[0]);
+  P->group_tier.push_back(TIER_LDS);
+  P->group_slot.push_back(0);
+  std::vector<double> vol(1, 0.0);  // estimated history words per group
+  for (int b = 1; b < 6; ++b) {
+    if (bucket[b].empty()) continue;
+    std::vector<uint64_t> w;
+    for (uint32_t i : bucket[b]) w.push_back(words_of(i));
+    std::vector<uint64_t> sorted = w;
+    std::sort(sorted.begin(), sorted.end());
+    const uint64_t keep = 1000 - std::min<uint32_t>(P->tun.heavy_permille, 1000);
+    const uint64_t cut = keep >= 1000 ? sorted.back() : sorted[(size_t)((sorted.size() - 1) * keep / 1000)];
+    std::vector<uint32_t> reg, heavy;
+    uint64_t reg_max = 0, heavy_max = 0;
+    double reg_vol = 0, heavy_vol = 0;
+    for (size_t k = 0; k < w.size(); ++k) {
+      if (w[k] <= cut) {
+        reg.push_back(bucket[b][k]);
+        reg_max = std::max(reg_max, w[k]);
+        reg_vol += (double)w[k];
+      } else {
+        heavy.push_back(bucket[b][k]);
+        heavy_max = std::max(heavy_max, w[k]);
+        heavy_vol += (double)w[k];
+      }
+    }
+    for (int h = 0; h < 2; ++h) {
+      std::vector<uint32_t>& list = h ? heavy : reg;
+      if (list.empty()) continue;
+      P->work.push_back(list);
+      // Heavy pairs are few and long-running: one tier wider, so they do
+      // not trail the rest of the launch.
+      P->group_tier.push_back(h ? std::min<int>(kBucketTier[b] + 1, TIER_BLOCK1024) : kBucketTier[b]);
+      P->group_slot.push_back(((h ? heavy_max : reg_max) * 4 + 255) & ~(uint64_t)255);
+      vol.push_back(h ? heavy_vol : reg_vol);
+    }
+  }
+  // HBM budget: each group asks for its pairs in flight x (slot + ring);
+  // when the asks exceed the budget they are scaled down together.
+  const uint64_t total = P->tun.hbm_budget_mib << 20;
+  std::vector<double> ask(P->work.size(), 0.0);
+  double ask_sum = 0;
+  for (size_t g = 1; g < P->work.size(); ++g) {
+    uint64_t max_sum = 1;
+    for (uint32_t i : P->work[g]) max_sum = std::max<uint64_t>(max_sum, P->qlen[i] + P->tlen[i]);
+    const uint64_t ring = ring_words_for(P->engine, P->ring_slots, max_sum, nullptr) * 4;
+    const int t = P->group_tier[g];
+    const uint64_t flight = t == TIER_WAVE ? P->tun.hbm_slots
+                          : t == TIER_BLOCK256 ? (uint64_t)ctx->cu_count * 4 : (uint64_t)ctx->cu_count;
+    ask[g] = (double)std::min<uint64_t>(flight, P->work[g].size()) * (double)(P->group_slot[g] + ring);
+    ask_sum += ask[g];
+  }
+  P->group_share.assign(P->work.size(), 0.0);
+  for (size_t g = 1; g < P->work.size(); ++g)
+    P->group_share[g] = ask_sum <= (double)total ? ask[g] / (double)total
+                                                 : std::max(0.01, ask[g] / ask_sum);
+  // Duration model per group for the launch order: a pair's time grows as
+  // steps^2 / threads (cells) + steps (per-step latency); a group takes at
+  // least its longest pair and at least its total over its pairs in flight.
+  P->group_load.assign(P->work.size(), 0.0);
+  for (size_t g = 0; g < P->work.size(); ++g) {
+    const int t = P->group_tier[g];
+    const double threads = t == TIER_BLOCK1024 ? 1024 : t == TIER_BLOCK256 ? 256 : 64;
+    const double flight = t == TIER_LDS ? (double)P->tun.wave_slots
+                        : t == TIER_WAVE ? (double)P->tun.hbm_slots
+                        : t == TIER_BLOCK256 ? ctx->cu_count * 4.0 : (double)ctx->cu_count;
+    double sum = 0, longest = 0;
+    for (uint32_t i : P->work[g]) {
+      const double st = t == TIER_LDS ? 32.0 : (double)P->est[i];
+      const double d = st * st / threads + 8.0 * st;
+      sum += d;
+      longest = std::max(longest, d);
+    }
+    P->group_load[g] = std::max(longest, sum / std::max(1.0, std::min(flight, (double)P->work[g].size())));
+  }
+}
+
+// Uploads the flattened work lists (group by group).
+int upload_work(phx_plan* P) {
+  phx_ctx* ctx = P->ctx;
+  std::vector<uint32_t> flat;
+  for (auto& w : P->work) flat.insert(flat.end(), w.begin(), w.end());
+  if (!flat.empty()) HIPCHK(hipMemcpy(P->d_work.p, flat.data(), flat.size() * 4, hipMemcpyHostToDevice));
+  return PHX_OK;
+}
+
 // Work lists (length groups and tiers), memory estimates, device buffers and
 // descriptor upload; shared by packed-batch plans and window plans.  qh / th:
 // host bytes of each pair's query and target (memory estimates).  packed /
@@ -271,7 +420,8 @@ phx_plan* plan_setup(phx_plan* P, const std::vector<const uint8_t*>& qh,
   phx_ctx* ctx = P->ctx;
   const uint32_t n = P->n;
   // Work lists.
-  std::vector<uint32_t> order;
+  std::vector<uint32_t>& order = P->order;
+  order.clear();
   order.reserve(n);
   for (uint32_t i = 0; i < n; ++i)
     if (P->host_status[i] == ST_PENDING) order.push_back(i);
@@ -279,54 +429,20 @@ phx_plan* plan_setup(phx_plan* P, const std::vector<const uint8_t*>& qh,
     return P->qlen[a] + P->tlen[a] > P->qlen[c] + P->tlen[c];
   });
   if (is_wfa(P->engine)) {
-    // Group 0: LDS tier.  Groups 1..5: HBM tier, length buckets with edges
-    // 4096, 8192, 16384, 32768 (qn + tn); each bucket's slot fits the largest
-    // estimate in it.
-    P->work.assign(6, {});
-    P->group_tier.assign(6, 1);
-    P->group_tier[0] = 0;
-    P->group_slot.assign(6, 0);
-    // Step estimates from the sequences (k-mer event rate) for the pairs that
-    // need HBM slots, on host threads.
     auto lds_ok = [&](uint32_t i) {
       return P->qlen[i] + P->tlen[i] <= kWaveMaxLen && P->ring_slots <= kWaveRingMax;
     };
-    std::vector<int32_t> est(n, 0);
+    // Step estimates from the sequences (k-mer event rate) for the pairs that
+    // need HBM slots, on host threads.
+    P->est.assign(n, 0);
     std::vector<uint32_t> wide;
-    for (uint32_t i : order)
+    for (uint32_t i : P->order)
       if (!lds_ok(i)) wide.push_back(i);
     parallel_for(wide.size(), [&](size_t k) {
       const uint32_t i = wide[k];
-      est[i] = estimate_steps_seq(P->engine, P->costs, qh[i], P->qlen[i], th[i], P->tlen[i]);
+      P->est[i] = estimate_steps_seq(P->engine, P->costs, qh[i], P->qlen[i], th[i], P->tlen[i]);
     });
-    std::vector<uint64_t> est_words(6, 0);
-    for (uint32_t i : order) {
-      const uint64_t L = P->qlen[i] + P->tlen[i];
-      int gi;
-      if (lds_ok(i)) gi = 0;
-      else gi = L <= 4096 ? 1 : L <= 8192 ? 2 : L <= 16384 ? 3 : L <= 32768 ? 4 : 5;
-      P->work[gi].push_back(i);
-      if (gi > 0) {
-        const int32_t qn = P->pairs[i].qn, tn = P->pairs[i].tn;
-        est_words[gi] = std::max(est_words[gi], exact_hist_words(est[i], qn, tn));
-      }
-    }
-    for (int gi = 1; gi < 6; ++gi) P->group_slot[gi] = (est_words[gi] * 4 + 255) & ~(uint64_t)255;
-    // Tiers: buckets up to 8192 run a wave per pair, 16384 a 256-thread
-    // workgroup per pair, longer a 1024-thread workgroup per pair.
-    P->group_tier = {0, 1, 1, 2, 3, 3};
-    // HBM budget shares follow each group's estimated history volume.
-    std::vector<double> vol(6, 0.0);
-    double tot = 0;
-    for (int gi = 1; gi < 6; ++gi) {
-      for (uint32_t i : P->work[gi]) {
-        const int32_t qn = P->pairs[i].qn, tn = P->pairs[i].tn;
-        vol[gi] += (double)exact_hist_words(est[i], qn, tn);
-      }
-      tot += vol[gi];
-    }
-    P->group_share.assign(6, 0.0);
-    for (int gi = 1; gi < 6; ++gi) P->group_share[gi] = tot > 0 ? std::max(0.02, vol[gi] / tot) : 0.2;
+    plan_group(P);
   } else {
     P->work.assign(9, {});
     for (uint32_t i : order) {
@@ -361,21 +477,15 @@ phx_plan* plan_setup(phx_plan* P, const std::vector<const uint8_t*>& qh,
     return fail_plan("upload");
   if (hipHostMalloc((void**)&P->ctr_host, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
     return fail_plan("pinned counters");
-  {
-    // Work lists are static: upload once, grouped by tier / block count.
-    std::vector<uint32_t> flat;
-    for (auto& w : P->work) flat.insert(flat.end(), w.begin(), w.end());
-    if (!flat.empty() &&
-        hipMemcpy(P->d_work.p, flat.data(), flat.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
-      return fail_plan("work upload");
-  }
+  // Work lists are static between tunings: upload once.
+  if (upload_work(P) != PHX_OK) return fail_plan("work upload");
   return P;
 }
 
 int run_myers(phx_plan* P) {
   phx_ctx* ctx = P->ctx;
   uint32_t* d_list_base = (uint32_t*)P->d_work.p;
-  const uint64_t budget = ctx->tun.hbm_budget_mib << 20;
+  const uint64_t budget = P->tun.hbm_budget_mib << 20;
   uint64_t list_off = 0;
   for (size_t g = 0; g < P->work.size(); ++g) {
     const auto& wl = P->work[g];
@@ -439,6 +549,7 @@ phx_plan* plan_create_windows(phx_ctx* ctx, int engine, phx_costs costs, int64_t
   }
   phx_plan* P = new phx_plan;
   P->ctx = ctx;
+  P->tun = ctx->tun;
   P->engine = engine;
   P->costs = engine == PHX_ENGINE_AFFINE ? costs : phx_costs{0, 0, 0};
   P->cap = cap;
@@ -519,14 +630,10 @@ phx_ctx* phx_ctx_create(int device, uint32_t flags, const phx_tuning* tuning) {
   d.hbm_budget_mib = 65536;
   if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b > 0)
     d.hbm_budget_mib = std::max<uint64_t>(1024, (uint64_t)(0.6 * (double)free_b) >> 20);
-  if (tuning) {
-    if (tuning->wave_lds_kib) d.wave_lds_kib = tuning->wave_lds_kib;
-    if (tuning->wave_slots) d.wave_slots = tuning->wave_slots;
-    if (tuning->hbm_slots) d.hbm_slots = tuning->hbm_slots;
-    if (tuning->hbm_slot_mib) d.hbm_slot_mib = tuning->hbm_slot_mib;
-    if (tuning->hbm_budget_mib) d.hbm_budget_mib = tuning->hbm_budget_mib;
-  }
-  c->tun = d;
+  d.block256_min_len = 8192;
+  d.block1024_min_len = 16384;
+  d.heavy_permille = 50;
+  c->tun = merge_tuning(d, tuning);
   return c;
 }
 
@@ -559,6 +666,7 @@ phx_plan* phx_plan_create(phx_ctx* ctx, const phx_batch* b) {
   if (hipSetDevice(ctx->device) != hipSuccess) { ctx->err = "hipSetDevice failed"; return nullptr; }
   phx_plan* P = new phx_plan;
   P->ctx = ctx;
+  P->tun = ctx->tun;
   P->engine = b->engine;
   P->costs = b->costs;
   P->cap = b->cap;
@@ -671,6 +779,110 @@ int phx_plan_run(phx_plan* P) {
 double phx_plan_last_ms(const phx_plan* P) { return P ? P->last_ms : 0.0; }
 double phx_plan_last_kernel_ms(const phx_plan* P) { return P ? P->last_kernel_ms : 0.0; }
 uint32_t phx_plan_last_launches(const phx_plan* P) { return P ? P->last_launches : 0; }
+
+int phx_plan_set_tuning(phx_plan* P, const phx_tuning* t) {
+  if (!P || !t) return PHX_E_ARG;
+  phx_ctx* ctx = P->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  P->tun = merge_tuning(P->tun, t);
+  if (!is_wfa(P->engine)) return PHX_OK;
+  plan_group(P);
+  return upload_work(P);
+}
+
+int phx_plan_get_tuning(const phx_plan* P, phx_tuning* t) {
+  if (!P || !t) return PHX_E_ARG;
+  *t = P->tun;
+  return PHX_OK;
+}
+
+int phx_plan_autotune(phx_plan* P, uint32_t max_trials, phx_tuning* best_out, double* best_ms_out) {
+  if (!P) return PHX_E_ARG;
+  if (!is_wfa(P->engine)) {  // Myers has no tier choices
+    if (best_out) *best_out = P->tun;
+    if (best_ms_out) *best_ms_out = 0;
+    return PHX_OK;
+  }
+  const uint32_t cu = (uint32_t)P->ctx->cu_count;
+  uint32_t trials = 0;
+  // Time one tuning: a warm run, then the best of a few timed runs.
+  auto measure = [&](const phx_tuning& t, double* ms) -> int {
+    int rc = phx_plan_set_tuning(P, &t);
+    if (rc == PHX_OK) rc = phx_plan_run(P);
+    if (rc != PHX_OK) return rc;
+    double best = 1e300;
+    for (int r = 0; r < 5; ++r) {
+      rc = phx_plan_run(P);
+      if (rc != PHX_OK) return rc;
+      best = std::min(best, P->last_ms);
+      if (best > 20.0) break;  // long runs: one timed run is enough
+    }
+    *ms = best;
+    ++trials;
+    return PHX_OK;
+  };
+  phx_tuning best = P->tun;
+  double best_ms = 0;
+  int rc = measure(best, &best_ms);
+  if (rc != PHX_OK) return rc;
+  auto consider = [&](phx_tuning t) -> int {
+    if (max_trials && trials >= max_trials) return PHX_OK;
+    double ms = 0;
+    const int r = measure(t, &ms);
+    if (r != PHX_OK) return r;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = P->tun;
+    }
+    return PHX_OK;
+  };
+  bool has_lds = false, has_hbm = false, has_wave = false;
+  for (size_t g = 0; g < P->work.size(); ++g) {
+    if (P->work[g].empty()) continue;
+    has_lds |= P->group_tier[g] == TIER_LDS;
+    has_hbm |= P->group_tier[g] != TIER_LDS;
+  }
+  // Stage 1: tier edges (threads per pair by length) and the heavy share.
+  if (has_hbm) {
+    static const uint32_t kEdges[6][2] = {{8192, 16384}, {8192, 32768}, {16384, 32768},
+                                          {16384, 65536}, {32768, 65536}, {65536, 1u << 30}};
+    for (const auto& e : kEdges)
+      for (uint32_t heavy : {20u, 50u}) {
+        phx_tuning t = best;
+        t.block256_min_len = e[0];
+        t.block1024_min_len = e[1];
+        t.heavy_permille = heavy;
+        if ((rc = consider(t)) != PHX_OK) return rc;
+      }
+  }
+  rc = phx_plan_set_tuning(P, &best);
+  if (rc != PHX_OK) return rc;
+  for (size_t g = 0; g < P->work.size(); ++g)
+    has_wave |= !P->work[g].empty() && P->group_tier[g] == TIER_WAVE;
+  // Stage 2: pairs in flight of the HBM wave tier.
+  if (has_wave)
+    for (uint32_t k : {4u, 16u}) {
+      phx_tuning t = best;
+      t.hbm_slots = cu * k;
+      if ((rc = consider(t)) != PHX_OK) return rc;
+    }
+  // Stage 3: LDS per wave vs waves in flight of the LDS tier.
+  if (has_lds) {
+    static const uint32_t kLds[3][2] = {{7, 20}, {6, 24}, {12, 12}};
+    for (const auto& l : kLds) {
+      phx_tuning t = best;
+      t.wave_lds_kib = l[0];
+      t.wave_slots = cu * l[1];
+      if ((rc = consider(t)) != PHX_OK) return rc;
+    }
+  }
+  rc = phx_plan_set_tuning(P, &best);
+  if (rc != PHX_OK) return rc;
+  if (best_out) *best_out = best;
+  if (best_ms_out) *best_ms_out = best_ms;
+  return PHX_OK;
+}
 
 static int fetch_out(phx_plan* P) {
   phx_ctx* ctx = P->ctx;

@@ -254,3 +254,31 @@ def test_deterministic_and_isolated(px):
     for i in (0, 17, 123):
         one = px.wfa_extend_affine(qs[i], ts[i], px.SeedAnchor(), px.AffineCosts(4, 6, 2))
         assert one == a.alignment(i)
+
+
+def test_results_do_not_depend_on_tuning(px):
+    """Regrouping a plan under other tier edges / heavy shares / slot counts,
+    and autotuning it, leaves every result byte unchanged."""
+    from phraya_amd import synth
+    w = synth.workload("mixed", n=300)
+    idx = [i for i in range(w.n) if w.q_off[i + 1] - w.q_off[i] <= 6000][:150]
+    sub = w.subset(idx)
+    pb = px.PackedBatch(sub.q_seq, sub.q_off, sub.t_seq, sub.t_off,
+                        np.zeros(sub.n, np.uint64), np.zeros(sub.n, np.uint64))
+    plan = px.Plan(pb, px.ENGINE_AFFINE, px.AffineCosts(4, 6, 2))
+    plan.run()
+    ref = plan.results()
+    for t in (dict(block256_min_len=2048, block1024_min_len=4096, heavy_permille=300),
+              dict(block256_min_len=1 << 20, block1024_min_len=1 << 21, heavy_permille=1),
+              dict(hbm_slots=64, wave_lds_kib=5, wave_slots=256)):
+        say(f"tuning {t}")
+        plan.set_tuning(t)
+        plan.run()
+        r = plan.results()
+        assert np.array_equal(r.aln, ref.aln) and np.array_equal(r.runs, ref.runs)
+    tuned, ms = plan.autotune(max_trials=6)
+    say(f"autotuned {tuned} {ms:.2f} ms")
+    plan.run()
+    r = plan.results()
+    assert np.array_equal(r.aln, ref.aln) and np.array_equal(r.runs, ref.runs)
+    plan.close()
