@@ -146,7 +146,8 @@ int phx_plan_get_tuning(const phx_plan* plan, phx_tuning* tuning);
 /* Autotuner: times the plan under candidate tunings (tier length edges, i.e.
  * threads per pair, and heavy share; HBM wave-tier pairs in flight; LDS per
  * wave vs waves in flight of the LDS tier), stage by stage, keeps the
- * fastest and applies it.  max_trials = 0: no limit (about 20 trials).
+ * fastest and applies it; every trial runs the whole plan.  max_trials = 0:
+ * no limit (at most 15 trials).
  * *best / *best_ms (may be NULL) get the chosen tuning and its run time. */
 int phx_plan_autotune(phx_plan* plan, uint32_t max_trials, phx_tuning* best, double* best_ms);
 /* Algorithmic HBM bytes of one run: packed inputs + results + CIGAR runs. */

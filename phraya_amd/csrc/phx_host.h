@@ -114,6 +114,14 @@ inline int32_t ring_slots(int engine, const phx_costs& c) {
   return std::max(c.mismatch, std::max(c.gap_open + c.gap_extend, c.gap_extend)) / g + 1;
 }
 
+// Frames of score ring a pair keeps: affine max(x, o+e)/g + 1 for the m layer
+// and e/g + 1 for each gap layer; linear 2.
+inline int32_t ring_frames(int engine, const phx_costs& c) {
+  if (engine != PHX_ENGINE_AFFINE) return 2;
+  const int32_t g = gcd3(c.mismatch, c.gap_open, c.gap_extend);
+  return (std::max(c.mismatch, c.gap_open + c.gap_extend) / g + 1) + 2 * (c.gap_extend / g + 1);
+}
+
 // sum_{u=1..S} min(u, a)
 inline uint64_t sum_min(uint64_t S, uint64_t a) {
   return S <= a ? S * (S + 1) / 2 : a * (a + 1) / 2 + (S - a) * a;
@@ -236,11 +244,12 @@ void parallel_for(size_t n, F f) {
   for (auto& th : pool) th.join();
 }
 
-// Ring words of one HBM slot for pairs with qn + tn <= max_sum.
-inline uint64_t ring_words_for(int engine, int32_t R, uint64_t max_sum, uint32_t* wcap_out) {
+// Ring words of one HBM slot (`frames` from ring_frames) for pairs with
+// qn + tn <= max_sum.
+inline uint64_t ring_words_for(int32_t frames, uint64_t max_sum, uint32_t* wcap_out) {
   const uint32_t wcap = next_pow2(max_sum + 1);
   if (wcap_out) *wcap_out = wcap;
-  return (uint64_t)(engine == PHX_ENGINE_AFFINE ? 3 * R : 2) * wcap;
+  return (uint64_t)frames * wcap;
 }
 
 // Bytes of an HBM history slot holding `steps` steps of a qn x tn pair.
@@ -248,14 +257,14 @@ inline uint64_t hist_slot_bytes(int32_t steps, int32_t qn, int32_t tn) {
   return (exact_hist_words(steps, qn, tn) * 4 + 255) & ~(uint64_t)255;
 }
 
-// LDS layout of one wave of the LDS tier: int16 ring [layers][R][128], the
+// LDS layout of one wave of the LDS tier: int16 ring [frames][128], the
 // packed query and target (max_seq_words words), the u16 step table and u16
 // history cells filling the wave's share of lds_kib.
-inline WaveGeom lds_geometry(int engine, int32_t R, uint32_t max_seq_words, uint32_t lds_kib) {
+inline WaveGeom lds_geometry(int32_t frames, uint32_t max_seq_words, uint32_t lds_kib) {
   auto al = [](uint32_t v) { return (v + 15u) & ~15u; };
   WaveGeom G{};
   G.ring_off = 0;
-  const uint32_t ring_bytes = (uint32_t)(engine == PHX_ENGINE_AFFINE ? 3 * R : 2) * kWaveWidth * 2u;
+  const uint32_t ring_bytes = (uint32_t)frames * kWaveWidth * 2u;
   G.seq_off = al(ring_bytes);
   G.seq_cap_words = std::max<uint32_t>(max_seq_words, 2);
   G.tab_off = G.seq_off + al(G.seq_cap_words * 8);

@@ -261,7 +261,7 @@ bool launch(phx_server* S, Lane& L, std::vector<Done>& fin) {
     if (L.tier != TIER_LDS) {
       const int32_t steps = step_bound(r, L.tier, S);
       uint32_t wcap = 0;
-      const uint64_t ring_w = ring_words_for(S->engine, S->R, (uint64_t)r.qn + r.tn, &wcap);
+      const uint64_t ring_w = ring_words_for(ring_frames(S->engine, S->costs), (uint64_t)r.qn + r.tn, &wcap);
       r.hist_bytes = hist_slot_bytes(steps, r.qn, r.tn);
       r.res_bytes = r.hist_bytes + ((ring_w * 4 + 255) & ~(uint64_t)255);
       r.ring_log2 = log2u(wcap);
@@ -355,7 +355,7 @@ bool launch(phx_server* S, Lane& L, std::vector<Done>& fin) {
   WaveGeom G{};
   uint32_t slots = n;
   if (L.tier == TIER_LDS) {
-    G = lds_geometry(S->engine, S->R, max_seq_words, S->wave_lds_kib);
+    G = lds_geometry(ring_frames(S->engine, S->costs), max_seq_words, S->wave_lds_kib);
     slots = std::min<uint32_t>(n, std::max<uint32_t>(kWaveWaves, S->wave_slots));
   }
   hipError_t e = hipMemcpyAsync(L.d_seq.p, L.h_seq.p, words * 8, hipMemcpyHostToDevice, st);

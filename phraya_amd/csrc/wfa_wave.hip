@@ -43,20 +43,16 @@ namespace phx {
 constexpr int WV_W = 128;    // LDS tier ring width (diagonals)
 constexpr int WV_WAVES = 4;  // waves (pairs) per workgroup
 
-// Orders one wave's memory traffic between steps.  LDS accesses of a wave
-// complete in order, so a wavefront-scope fence (a compiler barrier) is
-// enough; the HBM tier also waits for its global stores (workgroup scope).
+// Orders one wave's memory traffic between steps when the wave owns its pair
+// alone.  A wave's own LDS and global accesses are performed in program
+// order, so a wavefront-scope fence (no waits; it only keeps the compiler
+// from reordering) is all a lane needs to see what another lane of the same
+// wave stored in an earlier step, in LDS or in its HBM slot alike.
 template <bool LDS>
 __device__ __forceinline__ void wave_sync() {
-  if (LDS) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  } else {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Synchronisation between the threads of one pair: a wave-level fence for a
@@ -215,26 +211,28 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
   for (int32_t b = o; b != 0;) { const int32_t r = g % b; g = b; b = r; }
   for (int32_t b = e; b != 0;) { const int32_t r = g % b; g = b; b = r; }
   const int32_t xn = x / g, oen = (o + e) / g, en = e / g;
-  const int32_t R = max(xn, max(oen, en)) + 1;
+  // The m layer reads xn and oen steps back, the gap layers en steps back:
+  // Rm and Rg frames of ring (the host sizes the ring as Rm + 2 Rg frames).
+  const int32_t Rm = max(xn, oen) + 1, Rg = en + 1;
   const int64_t hard_max = (int64_t)(qn + tn) * max(x, 1) + o + e + 4;
   const int64_t explore_to = a.cp.cap >= 0 ? min((int64_t)a.cp.cap, hard_max) : hard_max;
   const int32_t max_step = (int32_t)(explore_to / g);
   const uint32_t W = S.wmask + 1;
-  typename S_t::ring_t* Rm = S.ring;
-  typename S_t::ring_t* Ri = S.ring + (size_t)R * W;
-  typename S_t::ring_t* Rd = S.ring + (size_t)2 * R * W;
+  typename S_t::ring_t* Mr = S.ring;
+  typename S_t::ring_t* Ri = S.ring + (size_t)Rm * W;
+  typename S_t::ring_t* Rd = Ri + (size_t)Rg * W;
 
   const int32_t m0 = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, 0, 0);
   if (lane == 0) {
     // Step 0 holds only m on diagonal 0; its ins/del entries are unset.
-    Rm[0] = (typename S_t::ring_t)m0;
+    Mr[0] = (typename S_t::ring_t)m0;
     Ri[0] = -1;
     Rd[0] = -1;
     if (NT > 64) sh->key[0] = sh->key[1] = sh->key[2] = INT32_MAX;
   }
   pair_sync<LDS, NT>();
   int32_t status = ST_OK;
-  int32_t step = 0, slot = 0;
+  int32_t step = 0, slot = 0, gslot = 0;  // step mod Rm, step mod Rg
   uint64_t used = 0;
   bool hist_on = true;
   int32_t end_k = 0, end_layer = 0;
@@ -243,7 +241,8 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
   while (!found) {
     step = __builtin_amdgcn_readfirstlane(step + 1);
     if (step > max_step) { status = ST_ALIGNMENT_FAILED; break; }
-    slot = __builtin_amdgcn_readfirstlane(slot + 1 == R ? 0 : slot + 1);
+    slot = __builtin_amdgcn_readfirstlane(slot + 1 == Rm ? 0 : slot + 1);
+    gslot = __builtin_amdgcn_readfirstlane(gslot + 1 == Rg ? 0 : gslot + 1);
     const int32_t lo = step_lo(step, tn), hi = step_hi(step, qn);
     const int32_t width = hi - lo + 1;
     if ((uint32_t)width > W) { status = ST_OVERFLOW_WIDTH; break; }
@@ -254,22 +253,22 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
     if (hist_on && lane == 0) S.toff(step) = (typename S_t::cell_t)used;
     if (NT > 64 && lane == 0) sh->key[(step + 1) % 3] = INT32_MAX;
     const int32_t sx = step - xn, so = step - oen, se = step - en;
-    int32_t rx = slot - xn, ro = slot - oen, re = slot - en;
-    rx += rx < 0 ? R : 0;
-    ro += ro < 0 ? R : 0;
-    re += re < 0 ? R : 0;
+    int32_t rx = slot - xn, ro = slot - oen;
+    rx += rx < 0 ? Rm : 0;
+    ro += ro < 0 ? Rm : 0;
+    const int32_t re = gslot + 1 == Rg ? 0 : gslot + 1;  // (step - en) mod Rg, en = Rg - 1
     const int32_t xlo = sx >= 0 ? step_lo(sx, tn) : 1, xhi = sx >= 0 ? step_hi(sx, qn) : 0;
     const int32_t olo = so >= 0 ? step_lo(so, tn) : 1, ohi = so >= 0 ? step_hi(so, qn) : 0;
     const int32_t elo = se >= 0 ? step_lo(se, tn) : 1, ehi = se >= 0 ? step_hi(se, qn) : 0;
     const int32_t ke_lo = fitting ? max(kend_g, lo) : kend_g;
     const int32_t ke_hi = fitting ? hi : kend_g;
-    const typename S_t::ring_t* Rmx = Rm + (size_t)rx * W;
-    const typename S_t::ring_t* Rmo = Rm + (size_t)ro * W;
+    const typename S_t::ring_t* Rmx = Mr + (size_t)rx * W;
+    const typename S_t::ring_t* Rmo = Mr + (size_t)ro * W;
     const typename S_t::ring_t* Rie = Ri + (size_t)re * W;
     const typename S_t::ring_t* Rde = Rd + (size_t)re * W;
-    typename S_t::ring_t* Wm = Rm + (size_t)slot * W;
-    typename S_t::ring_t* Wi = Ri + (size_t)slot * W;
-    typename S_t::ring_t* Wd = Rd + (size_t)slot * W;
+    typename S_t::ring_t* Wm = Mr + (size_t)slot * W;
+    typename S_t::ring_t* Wi = Ri + (size_t)gslot * W;
+    typename S_t::ring_t* Wd = Rd + (size_t)gslot * W;
     typename S_t::cell_t* cells = S.cells + used;
     const int32_t nch = (width + NT - 1) / NT;
     for (int32_t c = 0; c < nch; ++c) {
