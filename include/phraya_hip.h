@@ -108,7 +108,8 @@ typedef struct phx_tuning {
   uint32_t block256_min_len; /* qn + tn above which a pair runs a 256-thread workgroup (0 = 8192) */
   uint32_t block1024_min_len;/* ... a 1024-thread workgroup (0 = 16384) */
   uint32_t heavy_permille;   /* per mille of each length bucket (largest estimates) run one tier wider (0 = 50) */
-  uint32_t reserved;
+  uint32_t block_lds_ring;   /* workgroup tiers: LDS ring width in diagonals for pairs whose wavefront fits
+                                (power of two; 0 = 1024; 1 = ring always in HBM) */
 } phx_tuning;
 
 /* Context: owns the stream, device buffers and the submission lock.

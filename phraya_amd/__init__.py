@@ -242,7 +242,7 @@ class Plan:
     def tuning(self) -> dict:
         t = L.CTuning()
         L.lib().phx_plan_get_tuning(self._h, ctypes.byref(t))
-        return {name: getattr(t, name) for name, _ in L.CTuning._fields_ if name != "reserved"}
+        return {name: getattr(t, name) for name, _ in L.CTuning._fields_}
 
     def set_tuning(self, tuning: dict) -> None:
         t = L.CTuning(**tuning)
@@ -290,7 +290,7 @@ def _autotune(handle, max_trials: int) -> tuple[dict, float]:
     rc = L.lib().phx_plan_autotune(handle, max_trials, ctypes.byref(t), ctypes.byref(ms))
     if rc != L.PHX_OK:
         raise RuntimeError(f"phx_plan_autotune: status {rc}")
-    return {name: getattr(t, name) for name, _ in L.CTuning._fields_ if name != "reserved"}, ms.value
+    return {name: getattr(t, name) for name, _ in L.CTuning._fields_}, ms.value
 
 
 def align_batch(engine: int, queries: Sequence[bytes], targets: Sequence[bytes],

@@ -55,6 +55,7 @@ struct phx_plan {
   std::vector<uint64_t> group_slot;                    // WFA: HBM slot bytes of each group
   std::vector<double> group_share;                     // WFA: share of the HBM budget
   std::vector<double> group_load;                      // WFA: duration model of each group
+  std::vector<uint8_t> group_rl;                       // WFA: block tier with ring + sequences in LDS
   phx_tuning tun{};                                    // this plan's tuning (from the context)
   std::vector<uint32_t> order;                         // pending pairs, longest first
   std::vector<int32_t> est;                            // WFA: estimated score steps per pair
@@ -132,6 +133,7 @@ phx_tuning merge_tuning(phx_tuning d, const phx_tuning* t) {
   if (t->block256_min_len) d.block256_min_len = t->block256_min_len;
   if (t->block1024_min_len) d.block1024_min_len = t->block1024_min_len;
   if (t->heavy_permille) d.heavy_permille = std::min<uint32_t>(t->heavy_permille, 1000);
+  if (t->block_lds_ring) d.block_lds_ring = t->block_lds_ring == 1 ? 1 : next_pow2(t->block_lds_ring);
   return d;
 }
 bool has_cigar(int engine) { return engine != PHX_ENGINE_MYERS_DIST; }
@@ -144,11 +146,15 @@ struct GroupSlots {
 };
 
 GroupSlots group_slots(const phx_plan* P, int t, const std::vector<uint32_t>& work,
-                       uint64_t slot_bytes, uint64_t budget) {
+                       uint64_t slot_bytes, uint64_t budget, bool rl = false) {
   GroupSlots g;
   uint64_t max_sum = 1;
   for (uint32_t i : work) max_sum = std::max<uint64_t>(max_sum, (uint64_t)P->pairs[i].qn + P->pairs[i].tn);
   g.ring_words = ring_words_for(ring_frames(P->engine, P->costs), max_sum, &g.wcap);
+  if (rl) {  // the ring lives in LDS
+    g.ring_words = 0;
+    g.wcap = P->tun.block_lds_ring;
+  }
   g.hist = slot_bytes ? slot_bytes : ((uint64_t)P->tun.hbm_slot_mib << 20);
   g.per = g.hist + ((g.ring_words * 4 + 255) & ~(uint64_t)255);
   // Pairs in flight: one per wave (wave tier) or per workgroup (block tiers).
@@ -163,7 +169,8 @@ GroupSlots group_slots(const phx_plan* P, int t, const std::vector<uint32_t>& wo
 // Launches one group: tier t over `work` on stream `st`; HBM tiers use the
 // scratch slice [slice, slice + gs.n * gs.per).
 int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint32_t* d_list,
-                    uint32_t* d_counters, const GroupSlots& gs, uint8_t* slice, hipStream_t st) {
+                    uint32_t* d_counters, const GroupSlots& gs, uint8_t* slice, hipStream_t st,
+                    bool rl = false) {
   phx_ctx* ctx = P->ctx;
   WfaArgs a{};
   a.pairs = (const PairDesc*)P->d_pairs.p;
@@ -189,11 +196,16 @@ int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint3
     a.ring_stride = (gs.per - gs.hist) / 4;
     a.wcap_log2 = log2u(gs.wcap);
     slots = (uint32_t)gs.n;
+    if (rl) {
+      uint32_t seq_words = 2;
+      for (uint32_t i : work) seq_words = std::max(seq_words, lds_seq_words(P->pairs[i]));
+      G = block_lds_geometry(ring_frames(P->engine, P->costs), gs.wcap, seq_words);
+    }
   }
   hipEvent_t e0, e1;
   if (kernel_events(ctx, P->last_launches, &e0, &e1) != PHX_OK) return PHX_E_DEVICE;
   HIPCHK(hipEventRecord(e0, st));
-  HIPCHK(launch_tier(t, mode, a, G, slots, st));
+  HIPCHK(launch_tier(t, mode, a, G, slots, st, rl));
   HIPCHK(hipEventRecord(e1, st));
   ++P->last_launches;
   return PHX_OK;
@@ -227,7 +239,8 @@ int run_wfa(phx_plan* P) {
   uint64_t arena = 0;
   for (int gi = 0; gi < ngroups; ++gi) {
     if (P->work[gi].empty() || P->group_tier[gi] == TIER_LDS) continue;
-    gs[gi] = group_slots(P, P->group_tier[gi], P->work[gi], P->group_slot[gi], P->group_budget(gi));
+    gs[gi] = group_slots(P, P->group_tier[gi], P->work[gi], P->group_slot[gi], P->group_budget(gi),
+                         P->group_rl[gi] != 0);
     slice_off[gi] = arena;
     arena += gs[gi].n * gs[gi].per;
   }
@@ -245,7 +258,8 @@ int run_wfa(phx_plan* P) {
       used_stream[si] = true;
     }
     int rc = launch_wfa_tier(P, P->group_tier[gi], wl, (uint32_t*)P->d_work.p + group_off[gi],
-                             d_ctr + 2 * gi, gs[gi], (uint8_t*)P->d_arena.p + slice_off[gi], st);
+                             d_ctr + 2 * gi, gs[gi], (uint8_t*)P->d_arena.p + slice_off[gi], st,
+                             P->group_rl[gi] != 0);
     if (rc != PHX_OK) return rc;
     launched.push_back(gi);
   }
@@ -280,8 +294,10 @@ int run_wfa(phx_plan* P) {
     });
     uint64_t words = 0;
     for (uint32_t i : wl) {
+      // A width overflow (LDS ring) stopped early: its step count is partial.
       const int32_t steps = est ? estimate_steps(P->engine, P->costs, P->pairs[i].qn, P->pairs[i].tn)
-                                : outs[i].steps;
+                          : outs[i].status == ST_OVERFLOW_WIDTH ? std::max(outs[i].steps, P->est[i])
+                                                                 : outs[i].steps;
       words = std::max(words, exact_hist_words(steps, P->pairs[i].qn, P->pairs[i].tn));
     }
     const uint64_t slot_bytes = (words * 4 + 255) & ~(uint64_t)255;
@@ -345,8 +361,10 @@ void plan_group(phx_plan* P) {
   P->work.clear();
   P->group_tier.clear();
   P->group_slot.clear();
+  P->group_rl.clear();
   P->work.push_back(bucket[0]);
   P->group_tier.push_back(TIER_LDS);
+  P->group_rl.push_back(0);
   P->group_slot.push_back(0);
   std::vector<double> vol(1, 0.0);  // estimated history words per group
   for (int b = 1; b < 6; ++b) {
@@ -358,28 +376,40 @@ void plan_group(phx_plan* P) {
     const uint64_t keep = 1000 - std::min<uint32_t>(P->tun.heavy_permille, 1000);
     const uint64_t cut = keep >= 1000 ? sorted.back() : sorted[(size_t)((sorted.size() - 1) * keep / 1000)];
     std::vector<uint32_t> reg, heavy;
-    uint64_t reg_max = 0, heavy_max = 0;
-    double reg_vol = 0, heavy_vol = 0;
-    for (size_t k = 0; k < w.size(); ++k) {
-      if (w[k] <= cut) {
-        reg.push_back(bucket[b][k]);
-        reg_max = std::max(reg_max, w[k]);
-        reg_vol += (double)w[k];
-      } else {
-        heavy.push_back(bucket[b][k]);
-        heavy_max = std::max(heavy_max, w[k]);
-        heavy_vol += (double)w[k];
-      }
-    }
+    for (size_t k = 0; k < w.size(); ++k) (w[k] <= cut ? reg : heavy).push_back(bucket[b][k]);
     for (int h = 0; h < 2; ++h) {
       std::vector<uint32_t>& list = h ? heavy : reg;
       if (list.empty()) continue;
-      P->work.push_back(list);
       // Heavy pairs are few and long-running: one tier wider, so they do
       // not trail the rest of the launch.
-      P->group_tier.push_back(h ? std::min<int>(kBucketTier[b] + 1, TIER_BLOCK1024) : kBucketTier[b]);
-      P->group_slot.push_back(((h ? heavy_max : reg_max) * 4 + 255) & ~(uint64_t)255);
-      vol.push_back(h ? heavy_vol : reg_vol);
+      const int tier = h ? std::min<int>(kBucketTier[b] + 1, TIER_BLOCK1024) : kBucketTier[b];
+      // Workgroup tiers: pairs whose estimated wavefront and sequences fit
+      // keep ring and sequences in LDS; the rest use the HBM ring.
+      std::vector<uint32_t> parts[2];  // [0] HBM ring, [1] LDS ring
+      const uint32_t W = P->tun.block_lds_ring;
+      const uint64_t ring_bytes = (uint64_t)ring_frames(P->engine, P->costs) * W * 4;
+      for (uint32_t i : list) {
+        bool fit = false;
+        if (tier >= TIER_BLOCK256 && W > 1 && ring_bytes + 1024 <= kBlockLdsMax) {
+          const uint64_t width = std::min<uint64_t>(2 * (uint64_t)P->est[i] + 1, P->qlen[i] + P->tlen[i] + 1);
+          fit = width <= W && ring_bytes + 16 + 8ull * lds_seq_words(P->pairs[i]) <= kBlockLdsMax;
+        }
+        parts[fit].push_back(i);
+      }
+      for (int r = 0; r < 2; ++r) {
+        if (parts[r].empty()) continue;
+        uint64_t mx = 0;
+        double v = 0;
+        for (uint32_t i : parts[r]) {
+          mx = std::max<uint64_t>(mx, words_of(i));
+          v += (double)words_of(i);
+        }
+        P->work.push_back(parts[r]);
+        P->group_tier.push_back(tier);
+        P->group_rl.push_back((uint8_t)r);
+        P->group_slot.push_back((mx * 4 + 255) & ~(uint64_t)255);
+        vol.push_back(v);
+      }
     }
   }
   // HBM budget: each group asks for its pairs in flight x (slot + ring);
@@ -390,7 +420,7 @@ void plan_group(phx_plan* P) {
   for (size_t g = 1; g < P->work.size(); ++g) {
     uint64_t max_sum = 1;
     for (uint32_t i : P->work[g]) max_sum = std::max<uint64_t>(max_sum, P->qlen[i] + P->tlen[i]);
-    const uint64_t ring = ring_words_for(ring_frames(P->engine, P->costs), max_sum, nullptr) * 4;
+    const uint64_t ring = P->group_rl[g] ? 0 : ring_words_for(ring_frames(P->engine, P->costs), max_sum, nullptr) * 4;
     const int t = P->group_tier[g];
     const uint64_t flight = t == TIER_WAVE ? P->tun.hbm_slots
                           : t == TIER_BLOCK256 ? (uint64_t)ctx->cu_count * 4 : (uint64_t)ctx->cu_count;
@@ -655,6 +685,7 @@ phx_ctx* phx_ctx_create(int device, uint32_t flags, const phx_tuning* tuning) {
   d.block256_min_len = 8192;
   d.block1024_min_len = 16384;
   d.heavy_permille = 50;
+  d.block_lds_ring = 1024;
   c->tun = merge_tuning(d, tuning);
   return c;
 }

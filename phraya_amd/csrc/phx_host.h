@@ -27,7 +27,8 @@ hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t st);
 hipError_t launch_myers(int nb_template, const MyersArgs& a, hipStream_t st);
 hipError_t launch_wfa_wave(int mode, bool lds, const WfaArgs& a, const WaveGeom& G,
                            uint32_t grid, hipStream_t st);
-hipError_t launch_wfa_block(int mode, int nt, const WfaArgs& a, uint32_t grid, hipStream_t st);
+hipError_t launch_wfa_block(int mode, int nt, bool rl, const WfaArgs& a, const WaveGeom& G,
+                            uint32_t grid, hipStream_t st);
 hipError_t launch_gather_runs(const uint32_t* runs, const uint64_t* src, const uint64_t* off,
                               uint32_t* dst, uint32_t n, hipStream_t st);
 
@@ -285,15 +286,29 @@ inline uint32_t lds_seq_words(const PairDesc& pd) {
 }
 
 // Launch one tier over a.work_list: `slots` pairs in flight (waves for the
-// wave tiers, workgroups for the block tiers).
+// wave tiers, workgroups for the block tiers); rl: block tiers keep ring and
+// sequences in LDS (layout in G).
 inline hipError_t launch_tier(int tier, int mode, const WfaArgs& a, const WaveGeom& G,
-                              uint32_t slots, hipStream_t st) {
+                              uint32_t slots, hipStream_t st, bool rl = false) {
   if (tier <= TIER_WAVE) {
     const uint32_t wgs = std::max<uint32_t>(1, (slots + kWaveWaves - 1) / kWaveWaves);
     return launch_wfa_wave(mode, tier == TIER_LDS, a, G, wgs, st);
   }
-  return launch_wfa_block(mode, tier == TIER_BLOCK256 ? 256 : 1024, a, std::max<uint32_t>(1, slots), st);
+  return launch_wfa_block(mode, tier == TIER_BLOCK256 ? 256 : 1024, rl, a, G,
+                          std::max<uint32_t>(1, slots), st);
 }
+
+// LDS layout of a block-tier workgroup with the ring in LDS: int32 ring
+// [frames][width] then the packed sequences (seq_words).
+inline WaveGeom block_lds_geometry(int32_t frames, uint32_t width, uint32_t seq_words) {
+  WaveGeom G{};
+  G.ring_off = 0;
+  G.seq_off = ((uint32_t)frames * width * 4u + 15u) & ~15u;
+  G.seq_cap_words = seq_words;
+  G.per_wave_bytes = G.seq_off + seq_words * 8u;
+  return G;
+}
+constexpr uint32_t kBlockLdsMax = 64u * 1024u;  // dynamic LDS a block-tier workgroup may take
 
 // A batch plan over windows of resident sequences: the descriptors carry
 // device pointers, shifts, lengths and flags; qh / th are host copies of each

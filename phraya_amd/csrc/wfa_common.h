@@ -14,8 +14,10 @@ constexpr int32_t UNSET = INT32_MIN;
 // --- 2-bit / 8-bit match extension -------------------------------------------
 
 // Symbols [pos, pos + per_word) of a packed sequence as one little-endian word.
-template <bool B8>
-__device__ __forceinline__ uint64_t seq_window(const uint64_t* __restrict__ w, uint32_t pos) {
+// WP: a pointer to const u64 words in LDS or global memory (address space
+// carried by the type, so accesses compile to ds_ / global_ instructions).
+template <bool B8, class WP>
+__device__ __forceinline__ uint64_t seq_window(WP w, uint32_t pos) {
   constexpr uint32_t LOG = B8 ? 3 : 5;
   constexpr uint32_t BITS = B8 ? 8 : 2;
   const uint32_t wi = pos >> LOG;
@@ -30,9 +32,8 @@ __device__ __forceinline__ uint64_t seq_window(const uint64_t* __restrict__ w, u
 // 32 bases (or 8 raw bytes) are compared per 64-bit XOR; the first differing
 // symbol is the lowest set bit.
 // qsh / tsh: index of symbol 0 inside word 0 of each sequence.
-template <bool B8>
-__device__ __forceinline__ int32_t extend_diag(const uint64_t* __restrict__ qw, uint32_t qsh,
-                                               const uint64_t* __restrict__ tw, uint32_t tsh,
+template <bool B8, class WP>
+__device__ __forceinline__ int32_t extend_diag(WP qw, uint32_t qsh, WP tw, uint32_t tsh,
                                                int32_t qn, int32_t tn, int32_t p, int32_t k) {
   const int32_t j = p - k;
   if (p < 0 || p >= qn || j < 0 || j >= tn) return p;

@@ -55,11 +55,22 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Synchronisation between the threads of one pair: a wave-level fence for a
-// wave-per-pair launch, a workgroup barrier when NT threads share the pair.
-template <bool LDS, int NT>
+// Workgroup barrier that orders LDS only: global stores issued before it may
+// still be in flight (they are not read by other waves until a full barrier).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Synchronisation between the threads of one pair per score step: a
+// wave-level fence for a wave-per-pair launch; a workgroup barrier when NT
+// threads share the pair, ordering LDS only when the ring lives in LDS (RL),
+// else LDS and global memory.
+template <bool LDS, int NT, bool RL = false>
 __device__ __forceinline__ void pair_sync() {
   if (NT == 64) wave_sync<LDS>();
+  else if (RL) lds_barrier();
   else __syncthreads();
 }
 
@@ -70,19 +81,32 @@ __device__ __forceinline__ int32_t readlane(int32_t v, int32_t lane) {
 __device__ __forceinline__ int32_t step_lo(int32_t u, int32_t tn) { return max(-u, -tn); }
 __device__ __forceinline__ int32_t step_hi(int32_t u, int32_t qn) { return min(u, qn); }
 
-// Storage of one wave's pair: LDS (short) or HBM (wide).
-template <bool LDS>
+// Element types qualified with their address space (1 = global, 3 = LDS), so
+// every access compiles to a global_ / ds_ instruction instead of flat.
+template <class T>
+using gmem_t = __attribute__((address_space(1))) T;
+template <class T>
+using lmem_t = __attribute__((address_space(3))) T;
+template <bool IN_LDS, class T>
+using mem_t = typename std::conditional<IN_LDS, lmem_t<T>, gmem_t<T>>::type;
+
+// Storage of one pair: all in LDS (LDS tier), all in HBM, or (RL) ring and
+// sequences in LDS with history in HBM.
+template <bool LDS, bool RL = false>
 struct WaveStore {
-  using ring_t = typename std::conditional<LDS, int16_t, int32_t>::type;
-  using cell_t = typename std::conditional<LDS, uint16_t, uint32_t>::type;
+  using ring_v = typename std::conditional<LDS, int16_t, int32_t>::type;
+  using cell_v = typename std::conditional<LDS, uint16_t, uint32_t>::type;
+  using ring_t = mem_t<LDS || RL, ring_v>;
+  using cell_t = mem_t<LDS, cell_v>;
+  using seq_t = mem_t<LDS || RL, const uint64_t>;
   ring_t* ring;
   cell_t* cells;
   cell_t* toff_base;  // LDS: toff[step]; HBM: entries grow down from toff_base
   uint64_t cap;       // LDS: cells; HBM: 32-bit words of the slot
   uint32_t steps_cap; // LDS only
   uint32_t wmask;
-  const uint64_t* qw;
-  const uint64_t* tw;
+  seq_t* qw;
+  seq_t* tw;
   uint32_t qsh, tsh;  // index of symbol 0 in word 0
 
   __device__ __forceinline__ cell_t& toff(int32_t step) const {
@@ -100,23 +124,24 @@ __device__ __forceinline__ uint32_t seq_words(int32_t n) {
   return B8 ? ((uint32_t)n + 7) / 8 + 1 : ((uint32_t)n + 31) / 32 + 1;
 }
 
-template <bool LDS, bool B8, int NT>
-__device__ __forceinline__ WaveStore<LDS> make_store(const WfaArgs& a, const WaveGeom& G,
+template <bool LDS, bool B8, int NT, bool RL = false>
+__device__ __forceinline__ WaveStore<LDS, RL> make_store(const WfaArgs& a, const WaveGeom& G,
                                                      uint8_t* lds_base, const PairDesc& pd,
                                                      int lane) {
-  using S_t = WaveStore<LDS>;
+  using S_t = WaveStore<LDS, RL>;
   S_t S;
-  if (LDS) {
-    S.ring = reinterpret_cast<typename S_t::ring_t*>(lds_base + G.ring_off);
-    uint64_t* seq = reinterpret_cast<uint64_t*>(lds_base + G.seq_off);
+  if constexpr (LDS) {
+    S.ring = (typename S_t::ring_t*)(lds_base + G.ring_off);
+    lmem_t<uint64_t>* seq = (lmem_t<uint64_t>*)(lds_base + G.seq_off);
     const uint32_t nq = seq_words<B8>((int32_t)pd.q_shift + pd.qn);
     const uint32_t nt = seq_words<B8>((int32_t)pd.t_shift + pd.tn);
-    for (uint32_t i = (uint32_t)lane; i < nq + nt; i += 64)
-      seq[i] = i < nq ? pd.q[i] : pd.t[i - nq];
+    const gmem_t<const uint64_t>* gq = (const gmem_t<const uint64_t>*)pd.q;
+    const gmem_t<const uint64_t>* gt = (const gmem_t<const uint64_t>*)pd.t;
+    for (uint32_t i = (uint32_t)lane; i < nq + nt; i += 64) seq[i] = i < nq ? gq[i] : gt[i - nq];
     S.qw = seq;
     S.tw = seq + nq;
-    S.toff_base = reinterpret_cast<typename S_t::cell_t*>(lds_base + G.tab_off);
-    S.cells = reinterpret_cast<typename S_t::cell_t*>(lds_base + G.cell_off);
+    S.toff_base = (typename S_t::cell_t*)(lds_base + G.tab_off);
+    S.cells = (typename S_t::cell_t*)(lds_base + G.cell_off);
     S.cap = G.cells_cap;
     S.steps_cap = G.steps_cap;
     S.wmask = WV_W - 1;
@@ -128,24 +153,42 @@ __device__ __forceinline__ WaveStore<LDS> make_store(const WfaArgs& a, const Wav
     uint8_t* h;
     uint64_t hbytes;
     if (pd.hist != nullptr) {
-      S.ring = reinterpret_cast<typename S_t::ring_t*>(pd.ring);
+      S.ring = (typename S_t::ring_t*)pd.ring;
       h = pd.hist;
       hbytes = pd.hist_bytes;
       S.wmask = (1u << pd.ring_log2) - 1;
     } else {
-      const uint64_t gw = NT == 64 ? (uint64_t)blockIdx.x * WV_WAVES + (threadIdx.x >> 6)
-                                   : (uint64_t)blockIdx.x;
-      S.ring = reinterpret_cast<typename S_t::ring_t*>(a.ring_g + gw * a.ring_stride);
+      const uint32_t wv = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(threadIdx.x >> 6));
+      const uint64_t gw = NT == 64 ? (uint64_t)blockIdx.x * WV_WAVES + wv : (uint64_t)blockIdx.x;
+      S.ring = (typename S_t::ring_t*)(a.ring_g + gw * a.ring_stride);
       h = a.hist + gw * a.hist_stride;
       hbytes = a.hist_stride;
       S.wmask = (1u << a.wcap_log2) - 1;
     }
-    S.cells = reinterpret_cast<typename S_t::cell_t*>(h);
-    S.toff_base = reinterpret_cast<typename S_t::cell_t*>(h + hbytes);
+    S.cells = (typename S_t::cell_t*)h;
+    S.toff_base = (typename S_t::cell_t*)(h + hbytes);
     S.cap = hbytes / 4;
     S.steps_cap = 0;
-    S.qw = pd.q;
-    S.tw = pd.t;
+    if constexpr (RL) {
+      // Ring (int32, 1 << wcap_log2 diagonals) and both sequences in the
+      // workgroup's LDS; history stays in the HBM slot.
+      S.ring = (typename S_t::ring_t*)(lds_base + G.ring_off);
+      S.wmask = (1u << a.wcap_log2) - 1;
+      lmem_t<uint64_t>* seq = (lmem_t<uint64_t>*)(lds_base + G.seq_off);
+      const uint32_t nq = seq_words<B8>((int32_t)pd.q_shift + pd.qn);
+      const uint32_t nt = seq_words<B8>((int32_t)pd.t_shift + pd.tn);
+      const gmem_t<const uint64_t>* gq = (const gmem_t<const uint64_t>*)pd.q;
+      const gmem_t<const uint64_t>* gt = (const gmem_t<const uint64_t>*)pd.t;
+      for (uint32_t i = (uint32_t)lane; i < nq + nt; i += NT) seq[i] = i < nq ? gq[i] : gt[i - nq];
+      S.qw = seq;
+      S.tw = seq + nq;
+      S.qsh = pd.q_shift;
+      S.tsh = pd.t_shift;
+      lds_barrier();
+      return S;
+    }
+    S.qw = (typename S_t::seq_t*)pd.q;
+    S.tw = (typename S_t::seq_t*)pd.t;
   }
   S.qsh = pd.q_shift;
   S.tsh = pd.t_shift;
@@ -156,8 +199,8 @@ __device__ __forceinline__ WaveStore<LDS> make_store(const WfaArgs& a, const Wav
 // diagonal k + dir*j and reports whether the gap was opened there (op bit 0).
 // Returns the gap bases up to and including the opening cell (64 and
 // *opened = false when the opening lies beyond this window).
-template <bool LDS>
-__device__ __forceinline__ int32_t gap_run(const WaveStore<LDS>& S, int32_t s, int32_t k,
+template <class S_t>
+__device__ __forceinline__ int32_t gap_run(const S_t& S, int32_t s, int32_t k,
                                            int32_t dir, int32_t en, uint32_t bit, int32_t qn,
                                            int32_t tn, int lane, bool* opened) {
   const int32_t sj = s - lane * en;
@@ -195,16 +238,16 @@ struct PairShared {
 
 // --- affine --------------------------------------------------------------------
 
-template <bool LDS, bool B8, int NT>
+template <bool LDS, bool B8, int NT, bool RL = false>
 __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_base,
                             const PairDesc& pd, PairOut* po, int lane, PairShared* sh) {
-  using S_t = WaveStore<LDS>;
+  using S_t = WaveStore<LDS, RL>;
   const int32_t qn = pd.qn, tn = pd.tn;
   if (qn == 0) {  // wfa_simd.rs:925-927
     if (lane == 0) *po = PairOut{ST_OK, 0, 0, 0, 0, 0, 0, 0};
     return;
   }
-  const S_t S = make_store<LDS, B8, NT>(a, G, lds_base, pd, lane);
+  const S_t S = make_store<LDS, B8, NT, RL>(a, G, lds_base, pd, lane);
   const int32_t x = a.cp.x, o = a.cp.o, e = a.cp.e;
   const bool fitting = tn > qn + qn / 2 + 10;
   int32_t g = x;
@@ -225,12 +268,12 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
   const int32_t m0 = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, 0, 0);
   if (lane == 0) {
     // Step 0 holds only m on diagonal 0; its ins/del entries are unset.
-    Mr[0] = (typename S_t::ring_t)m0;
+    Mr[0] = (typename S_t::ring_v)m0;
     Ri[0] = -1;
     Rd[0] = -1;
     if (NT > 64) sh->key[0] = sh->key[1] = sh->key[2] = INT32_MAX;
   }
-  pair_sync<LDS, NT>();
+  pair_sync<LDS, NT, RL>();
   int32_t status = ST_OK;
   int32_t step = 0, slot = 0, gslot = 0;  // step mod Rm, step mod Rg
   uint64_t used = 0;
@@ -250,7 +293,7 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
       if (LDS) { status = ST_OVERFLOW_HIST; break; }
       hist_on = false;  // keep going to learn the exact number of steps
     }
-    if (hist_on && lane == 0) S.toff(step) = (typename S_t::cell_t)used;
+    if (hist_on && lane == 0) S.toff(step) = (typename S_t::cell_v)used;
     if (NT > 64 && lane == 0) sh->key[(step + 1) % 3] = INT32_MAX;
     const int32_t sx = step - xn, so = step - oen, se = step - en;
     int32_t rx = slot - xn, ro = slot - oen;
@@ -298,11 +341,11 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
       if (best >= 0) mval = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, best, k);
       int32_t layer = -1;
       if (k <= hi) {
-        Wm[ik] = (typename S_t::ring_t)mval;
-        Wi[ik] = (typename S_t::ring_t)ins;
-        Wd[ik] = (typename S_t::ring_t)del;
+        Wm[ik] = (typename S_t::ring_v)mval;
+        Wi[ik] = (typename S_t::ring_v)ins;
+        Wd[ik] = (typename S_t::ring_v)del;
         if (hist_on)
-          cells[k - lo] = (typename S_t::cell_t)(((uint32_t)(best + 1) << 4) | mop | (iop << 2) |
+          cells[k - lo] = (typename S_t::cell_v)(((uint32_t)(best + 1) << 4) | mop | (iop << 2) |
                                                  (dop << 3));
         if (k >= ke_lo && k <= ke_hi) layer = mval == qn ? 0 : ins == qn ? 1 : del == qn ? 2 : -1;
       }
@@ -319,7 +362,7 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
       }
     }
     used += (uint64_t)width;
-    pair_sync<LDS, NT>();
+    pair_sync<LDS, NT, RL>();
     if (NT > 64) {
       const int32_t key = __builtin_amdgcn_readfirstlane(sh->key[step % 3]);
       if (key != INT32_MAX) {
@@ -330,6 +373,7 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
     }
   }
   if (!hist_on && status == ST_OK) status = ST_OVERFLOW_HIST;
+  if (NT > 64 && RL) __syncthreads();  // history stores of every wave, for wave 0
 
   // Traceback (affine_traceback, wfa_simd.rs:785-886), wave-uniform state;
   // gap runs are resolved 64 cells per ballot.
@@ -363,7 +407,7 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
         const int32_t dir = layer == 1 ? -1 : 1;
         const uint32_t bit = layer == 1 ? 2 : 3;
         bool opened = false;
-        const int32_t n = gap_run<LDS>(S, s, k, dir, en, bit, qn, tn, lane, &opened);
+        const int32_t n = gap_run(S, s, k, dir, en, bit, qn, tn, lane, &opened);
         w.emit(layer == 1 ? OP_D : OP_I, n);
         if (layer == 1) qi -= n; else ti -= n;
         k += dir * n;
@@ -385,16 +429,16 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
 
 // --- linear --------------------------------------------------------------------
 
-template <bool LDS, bool B8, int NT>
+template <bool LDS, bool B8, int NT, bool RL = false>
 __device__ void wave_linear(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_base,
                             const PairDesc& pd, PairOut* po, int lane, PairShared* sh) {
-  using S_t = WaveStore<LDS>;
+  using S_t = WaveStore<LDS, RL>;
   const int32_t qn = pd.qn, tn = pd.tn;
   if (qn == 0) {  // wfa_simd.rs:194-203, :365-367
     if (lane == 0) *po = PairOut{ST_OK, 0, 0, 0, 0, 0, 0, 0};
     return;
   }
-  const S_t S = make_store<LDS, B8, NT>(a, G, lds_base, pd, lane);
+  const S_t S = make_store<LDS, B8, NT, RL>(a, G, lds_base, pd, lane);
   const bool fitting = tn > qn + qn / 2 + 10;
   const int32_t max_s = qn + tn;
   const int32_t explore = (fitting && a.cp.cap >= 0) ? min(a.cp.cap, max_s) : max_s;
@@ -404,10 +448,10 @@ __device__ void wave_linear(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
 
   const int32_t w0 = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, 0, 0);
   if (lane == 0) {
-    ring[0] = (typename S_t::ring_t)w0;
+    ring[0] = (typename S_t::ring_v)w0;
     if (NT > 64) sh->key[0] = sh->key[1] = sh->key[2] = INT32_MAX;
   }
-  pair_sync<LDS, NT>();
+  pair_sync<LDS, NT, RL>();
   int32_t step = 0, status = ST_OK, end_k = 0;
   uint64_t used = 0;
   bool hist_on = true;
@@ -427,7 +471,7 @@ __device__ void wave_linear(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
       if (LDS) { status = ST_OVERFLOW_HIST; break; }
       hist_on = false;
     }
-    if (hist_on && lane == 0) S.toff(step) = (typename S_t::cell_t)used;
+    if (hist_on && lane == 0) S.toff(step) = (typename S_t::cell_v)used;
     if (NT > 64 && lane == 0) sh->key[(step + 1) % 3] = INT32_MAX;
     const typename S_t::ring_t* P = ring + (size_t)ps * W;
     typename S_t::ring_t* Wn = ring + (size_t)slot * W;
@@ -455,9 +499,9 @@ __device__ void wave_linear(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
       if (ok) val = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, best, k);
       bool hit = false;
       if (k <= hi) {
-        Wn[ik] = (typename S_t::ring_t)val;
+        Wn[ik] = (typename S_t::ring_v)val;
         if (hist_on)
-          cells[k - lo] = (typename S_t::cell_t)(ok ? (((uint32_t)(best + 1) << 4) | op) : 0u);
+          cells[k - lo] = (typename S_t::cell_v)(ok ? (((uint32_t)(best + 1) << 4) | op) : 0u);
         hit = ok && k >= ke_lo && k <= ke_hi && val >= qn;
       }
       if (NT == 64) {
@@ -471,7 +515,7 @@ __device__ void wave_linear(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
       }
     }
     used += (uint64_t)width;
-    pair_sync<LDS, NT>();
+    pair_sync<LDS, NT, RL>();
     if (NT > 64) {
       const int32_t key = __builtin_amdgcn_readfirstlane(sh->key[step % 3]);
       if (key != INT32_MAX) {
@@ -481,6 +525,7 @@ __device__ void wave_linear(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
     }
   }
   if (!hist_on && status == ST_OK && !exhausted) status = ST_OVERFLOW_HIST;
+  if (NT > 64 && RL) __syncthreads();  // history stores of every wave, for wave 0
 
   PairOut r{status, 0, 0, 0, 0, 0, step, (int32_t)min<uint64_t>(used, INT32_MAX)};
   if (status == ST_OK && exhausted) {
@@ -536,7 +581,7 @@ template <int MODE, bool LDS>
 __global__ __launch_bounds__(64 * WV_WAVES) void wfa_wave_kernel(WfaArgs a, WaveGeom G) {
   extern __shared__ __align__(16) uint8_t wave_lds_pool[];
   const int lane = threadIdx.x & 63;
-  const int wv = threadIdx.x >> 6;
+  const int wv = __builtin_amdgcn_readfirstlane((int32_t)(threadIdx.x >> 6));
   uint8_t* base = LDS ? wave_lds_pool + (size_t)wv * G.per_wave_bytes : nullptr;
   for (;;) {
     uint32_t w = 0;
@@ -559,11 +604,12 @@ __global__ __launch_bounds__(64 * WV_WAVES) void wfa_wave_kernel(WfaArgs a, Wave
 
 // Workgroup per pair (NT threads) for wide wavefronts; ring and history in
 // the workgroup's HBM slot.
-template <int MODE, int NT>
-__global__ __launch_bounds__(NT) void wfa_block_kernel(WfaArgs a) {
+template <int MODE, int NT, bool RL>
+__global__ __launch_bounds__(NT) void wfa_block_kernel(WfaArgs a, WaveGeom G) {
   __shared__ PairShared sh;
+  extern __shared__ __align__(16) uint8_t block_lds[];
+  uint8_t* base = RL ? block_lds : nullptr;
   const int tid = threadIdx.x;
-  const WaveGeom G{};
   for (;;) {
     if (tid == 0) sh.work = atomicAdd(&a.counters[0], 1u);
     __syncthreads();
@@ -579,11 +625,11 @@ __global__ __launch_bounds__(NT) void wfa_block_kernel(WfaArgs a) {
     const PairDesc pd = a.pairs[pi];
     PairOut* po = a.out + pi;
     if (MODE == ENG_AFFINE) {
-      if (pd.flags & PF_BYTES) wave_affine<false, true, NT>(a, G, nullptr, pd, po, tid, &sh);
-      else wave_affine<false, false, NT>(a, G, nullptr, pd, po, tid, &sh);
+      if (pd.flags & PF_BYTES) wave_affine<false, true, NT, RL>(a, G, base, pd, po, tid, &sh);
+      else wave_affine<false, false, NT, RL>(a, G, base, pd, po, tid, &sh);
     } else {
-      if (pd.flags & PF_BYTES) wave_linear<false, true, NT>(a, G, nullptr, pd, po, tid, &sh);
-      else wave_linear<false, false, NT>(a, G, nullptr, pd, po, tid, &sh);
+      if (pd.flags & PF_BYTES) wave_linear<false, true, NT, RL>(a, G, base, pd, po, tid, &sh);
+      else wave_linear<false, false, NT, RL>(a, G, base, pd, po, tid, &sh);
     }
     __syncthreads();  // the slot is reused by the next pair
   }
@@ -634,16 +680,26 @@ hipError_t launch_wfa_wave(int mode, bool lds, const WfaArgs& a, const WaveGeom&
   return hipGetLastError();
 }
 
-// nt = threads per pair: 256 or 1024.
-hipError_t launch_wfa_block(int mode, int nt, const WfaArgs& a, uint32_t grid, hipStream_t st) {
-  if (mode == ENG_AFFINE) {
-    if (nt == 1024) hipLaunchKernelGGL((wfa_block_kernel<ENG_AFFINE, 1024>), dim3(grid), dim3(1024), 0, st, a);
-    else hipLaunchKernelGGL((wfa_block_kernel<ENG_AFFINE, 256>), dim3(grid), dim3(256), 0, st, a);
+// nt = threads per pair: 256 or 1024; rl: ring and sequences in LDS (G
+// gives the layout; dynamic LDS = G.per_wave_bytes).
+template <int MODE>
+hipError_t launch_block_mode(int nt, bool rl, const WfaArgs& a, const WaveGeom& G, uint32_t grid,
+                             hipStream_t st) {
+  const size_t lds = rl ? G.per_wave_bytes : 0;
+  if (nt == 1024) {
+    if (rl) hipLaunchKernelGGL((wfa_block_kernel<MODE, 1024, true>), dim3(grid), dim3(1024), lds, st, a, G);
+    else hipLaunchKernelGGL((wfa_block_kernel<MODE, 1024, false>), dim3(grid), dim3(1024), 0, st, a, G);
   } else {
-    if (nt == 1024) hipLaunchKernelGGL((wfa_block_kernel<ENG_LINEAR, 1024>), dim3(grid), dim3(1024), 0, st, a);
-    else hipLaunchKernelGGL((wfa_block_kernel<ENG_LINEAR, 256>), dim3(grid), dim3(256), 0, st, a);
+    if (rl) hipLaunchKernelGGL((wfa_block_kernel<MODE, 256, true>), dim3(grid), dim3(256), lds, st, a, G);
+    else hipLaunchKernelGGL((wfa_block_kernel<MODE, 256, false>), dim3(grid), dim3(256), 0, st, a, G);
   }
   return hipGetLastError();
+}
+
+hipError_t launch_wfa_block(int mode, int nt, bool rl, const WfaArgs& a, const WaveGeom& G,
+                            uint32_t grid, hipStream_t st) {
+  return mode == ENG_AFFINE ? launch_block_mode<ENG_AFFINE>(nt, rl, a, G, grid, st)
+                            : launch_block_mode<ENG_LINEAR>(nt, rl, a, G, grid, st);
 }
 
 }  // namespace phx

@@ -282,3 +282,21 @@ def test_results_do_not_depend_on_tuning(px):
     r = plan.results()
     assert np.array_equal(r.aln, ref.aln) and np.array_equal(r.runs, ref.runs)
     plan.close()
+
+
+def test_lds_ring_workgroup_tier_and_width_overflow(px):
+    """Workgroup tiers with the score ring in LDS: pairs routed there by their
+    estimate, and (with a deliberately narrow ring) pairs that outgrow it and
+    finish on the HBM ring, all bit-exact."""
+    rng = random.Random(4242)
+    qs, ts = [], []
+    for L, rate in ((3000, 0.005), (4500, 0.01), (5200, 0.02), (6000, 0.03), (7000, 0.01)):
+        for _ in range(3):
+            q = rand_seq(rng, L)
+            qs.append(q)
+            ts.append(mutate(rng, q, rate) + rand_seq(rng, L // 50))
+    for ring in (1024, 128):
+        ctx = px.Context(0, dict(block256_min_len=4096, block1024_min_len=12000, block_lds_ring=ring))
+        for eng in ("affine", "linear"):
+            compare(px, eng, qs, ts, costs=(4, 6, 2), ctx=ctx, label=f"lds-ring-{ring}")
+        ctx.close()
