@@ -499,6 +499,9 @@ def main():
                     help="phx_tuning overrides, e.g. block256_min_len=16384,heavy_permille=20")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the autotuner (runs before the timed steps)")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay each WFA step as a HIP graph (recorded in warmup) instead of "
+                         "launching it eagerly")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
@@ -547,7 +550,11 @@ def main():
         tuned, tuned_ms = plan.autotune()
         tune_s = time.perf_counter() - t0
         log(f"autotune {tune_s:.1f} s -> {tuned} ({tuned_ms:.3f} ms)")
-    log(f"warmup {args.warmup}")
+    graph = engine in ("affine", "linear") and args.graph
+    if graph:  # the first warmup run records the launch sequence
+        plan.use_graph(True)
+        args.warmup = max(args.warmup, 2)
+    log(f"warmup {args.warmup}" + (" (graph mode)" if graph else ""))
     for _ in range(args.warmup):
         plan.run()
     log(f"timing {args.steps} steps")
@@ -618,7 +625,7 @@ def main():
         "config": {"workload": args.workload, "description": desc, "pairs_per_step": n,
                    "engine": engine, "costs": list(costs) if engine == "affine" else None,
                    "parallelism": "single GPU", "tuning": tuned,
-                   "autotune_s": tune_s},
+                   "autotune_s": tune_s, "hip_graph": graph},
         "latency_ms": {"p50": float(np.percentile(step_ms, 50)),
                        "p99": float(np.percentile(step_ms, 99)),
                        "what": "device time of one whole batch (events around phx_plan_run)"},

@@ -151,6 +151,15 @@ int phx_plan_get_tuning(const phx_plan* plan, phx_tuning* tuning);
  * no limit (at most 15 trials).
  * *best / *best_ms (may be NULL) get the chosen tuning and its run time. */
 int phx_plan_autotune(phx_plan* plan, uint32_t max_trials, phx_tuning* best, double* best_ms);
+/* Graph mode (WFA engines; no counterpart in the reference, which has no
+ * device launches): the run after enabling records its launch sequence
+ * (counter reset, every group's kernel on its stream, join, counter
+ * readback) into a HIP graph, and later runs replay it with one
+ * hipGraphLaunch.  A new tuning or a grown scratch arena re-records it; the
+ * rare overflow re-runs stay outside the graph.  Results are the same in
+ * both modes; event nodes around each kernel node keep the per-launch
+ * timings above.  enable: 0 or 1. */
+int phx_plan_use_graph(phx_plan* plan, int enable);
 /* Algorithmic HBM bytes of one run: packed inputs + results + CIGAR runs. */
 uint64_t phx_plan_algorithmic_bytes(const phx_plan* plan);
 void phx_plan_destroy(phx_plan* plan);

@@ -17,8 +17,10 @@ as they arrive (continuous batching against a resident reference).
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import threading
+import weakref
 from dataclasses import dataclass
 from typing import Iterable, Optional, Sequence
 
@@ -87,6 +89,26 @@ def format_cigar(runs: np.ndarray) -> str:
     return "".join(f"{int(r) >> 4}{_OPS[int(r) & 3]}" for r in runs)
 
 
+# Live handles by teardown rank: plans and contig plans, then servers, then
+# contexts.  At interpreter exit they are released in that order while the
+# HIP runtime is fully up, instead of whenever finalization collects them.
+_live = (weakref.WeakSet(), weakref.WeakSet(), weakref.WeakSet())
+
+
+def _track(obj, rank: int) -> None:
+    _live[rank].add(obj)
+
+
+@atexit.register
+def _release_all() -> None:
+    for rank, attr in ((0, "close"), (1, "stop"), (2, "close")):
+        for obj in list(_live[rank]):
+            try:
+                getattr(obj, attr)()
+            except Exception:
+                pass
+
+
 class Context:
     """A device context (phx_ctx): stream, buffers, submission lock."""
 
@@ -99,6 +121,7 @@ class Context:
         self._h = lib.phx_ctx_create(device, 0, tun)
         if not self._h:
             raise RuntimeError(f"phx_ctx_create({device}) failed: no usable GPU")
+        _track(self, 2)
         self.device = device
 
     @property
@@ -224,6 +247,7 @@ class Plan:
         self._h = L.lib().phx_plan_create(self.ctx.handle, ctypes.byref(self._cb))
         if not self._h:
             raise RuntimeError(f"phx_plan_create failed: {self.ctx.last_error()}")
+        _track(self, 0)
 
     def run(self) -> None:
         rc = L.lib().phx_plan_run(self._h)
@@ -253,6 +277,12 @@ class Plan:
     def autotune(self, max_trials: int = 0) -> tuple[dict, float]:
         """Time candidate tunings on this plan, keep the fastest (phx_plan_autotune)."""
         return _autotune(self._h, max_trials)
+
+    def use_graph(self, enable: bool = True) -> None:
+        """Replay the run's launch sequence as one HIP graph (phx_plan_use_graph)."""
+        rc = L.lib().phx_plan_use_graph(self._h, 1 if enable else 0)
+        if rc != L.PHX_OK:
+            raise RuntimeError(f"phx_plan_use_graph: status {rc}")
 
     def algorithmic_bytes(self) -> int:
         return int(L.lib().phx_plan_algorithmic_bytes(self._h))
@@ -404,6 +434,7 @@ class Server:
         self._h = L.lib().phx_serve_start(self._ctx.handle, ctypes.byref(cfg))
         if not self._h:
             raise RuntimeError(f"phx_serve_start failed: {self._ctx.last_error()}")
+        _track(self, 1)
         self._runs = np.zeros(1 << 16, np.uint32)
 
     def submit_arrays(self, tags, q_seq: np.ndarray, q_off: np.ndarray, ref_start=None,
@@ -548,6 +579,7 @@ class ContigPlan:
                                                  ctypes.byref(b), ctypes.byref(p))
         if not self._h:
             raise RuntimeError(f"phx_contig_plan_create failed: {self._ctx.last_error()}")
+        _track(self, 0)
         self.params = params
 
     @property

@@ -33,7 +33,7 @@ EXPORTS = (
     "phx_align_batch", "phx_plan_create", "phx_plan_run", "phx_plan_results",
     "phx_plan_last_ms", "phx_plan_last_kernel_ms", "phx_plan_last_launches",
     "phx_plan_algorithmic_bytes", "phx_plan_destroy", "phx_format_cigar",
-    "phx_plan_set_tuning", "phx_plan_get_tuning", "phx_plan_autotune",
+    "phx_plan_set_tuning", "phx_plan_get_tuning", "phx_plan_autotune", "phx_plan_use_graph",
     "phx_wfa_extend", "phx_wfa_extend_affine", "phx_myers_extend", "phx_myers_edit_distance",
     "phx_serve_start", "phx_serve_submit", "phx_serve_poll", "phx_serve_stats_get",
     "phx_serve_stop", "phx_estimate_request", "phx_contig_anchors", "phx_contig_plan_create",
@@ -152,6 +152,7 @@ def lib() -> ctypes.CDLL:
     L.phx_plan_set_tuning.argtypes = [P, P]
     L.phx_plan_get_tuning.argtypes = [P, P]
     L.phx_plan_autotune.argtypes = [P, ctypes.c_uint32, P, P]
+    L.phx_plan_use_graph.argtypes = [P, ctypes.c_int]
     L.phx_format_cigar.argtypes = [P, ctypes.c_uint64, P, ctypes.c_size_t]
     L.phx_format_cigar.restype = ctypes.c_size_t
     sz, u64 = ctypes.c_size_t, ctypes.c_uint64

@@ -56,6 +56,13 @@ struct phx_plan {
   std::vector<double> group_share;                     // WFA: share of the HBM budget
   std::vector<double> group_load;                      // WFA: duration model of each group
   std::vector<uint8_t> group_rl;                       // WFA: block tier with ring + sequences in LDS
+  // hipGraph of a run's launch sequence (WFA): replayed by later runs.
+  bool use_graph = false;
+  bool graph_wanted = false;                           // build one after the current run
+  hipGraphExec_t gexec = nullptr;
+  void* g_arena = nullptr;                             // arena the graph's kernels point into
+  std::vector<int> g_launched;
+  uint32_t g_launches = 0;
   phx_tuning tun{};                                    // this plan's tuning (from the context)
   std::vector<uint32_t> order;                         // pending pairs, longest first
   std::vector<int32_t> est;                            // WFA: estimated score steps per pair
@@ -170,7 +177,7 @@ GroupSlots group_slots(const phx_plan* P, int t, const std::vector<uint32_t>& wo
 // scratch slice [slice, slice + gs.n * gs.per).
 int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint32_t* d_list,
                     uint32_t* d_counters, const GroupSlots& gs, uint8_t* slice, hipStream_t st,
-                    bool rl = false) {
+                    bool rl = false, bool timing = true) {
   phx_ctx* ctx = P->ctx;
   WfaArgs a{};
   a.pairs = (const PairDesc*)P->d_pairs.p;
@@ -202,6 +209,10 @@ int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint3
       G = block_lds_geometry(ring_frames(P->engine, P->costs), gs.wcap, seq_words);
     }
   }
+  if (!timing) {  // graph capture: timing nodes are added around the kernel node
+    HIPCHK(launch_tier(t, mode, a, G, slots, st, rl));
+    return PHX_OK;
+  }
   hipEvent_t e0, e1;
   if (kernel_events(ctx, P->last_launches, &e0, &e1) != PHX_OK) return PHX_E_DEVICE;
   HIPCHK(hipEventRecord(e0, st));
@@ -215,17 +226,19 @@ int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint3
 // group could not hold are staged per run in d_work2.  Counters: two words per
 // launch (queue head, overflow count).  Groups run concurrently on the
 // context's streams and join on the main stream.
-int run_wfa(phx_plan* P) {
+// Phase 1 of a WFA run on the context's streams: counter reset, every group
+// launched (longest first, each on the stream with the least work queued;
+// groups on one stream run one after another), join, counter readback.
+// capture: the streams are being captured into a graph; no timing events are
+// recorded and each kernel node is returned in *knodes.
+int issue_groups(phx_plan* P, bool capture, std::vector<int>* launched,
+                 std::vector<hipGraphNode_t>* knodes) {
   phx_ctx* ctx = P->ctx;
   const int ngroups = (int)P->work.size();
   uint32_t* d_ctr = (uint32_t*)P->d_ctr.p;
-  uint32_t* ctr_h = P->ctr_host;
   HIPCHK(launch_zero(d_ctr, 2 * (uint32_t)ngroups, ctx->stream));
   HIPCHK(hipEventRecord(ctx->join[0], ctx->stream));
-  std::vector<int> launched;
-  // Group gi's list sits at offset group_off[gi] of d_work.  Launch the
-  // groups longest first, each on the stream with the least work queued
-  // (groups on one stream run one after another).
+  // Group gi's list sits at offset group_off[gi] of d_work.
   std::vector<uint64_t> group_off(ngroups + 1, 0);
   for (int gi = 0; gi < ngroups; ++gi) group_off[gi + 1] = group_off[gi] + P->work[gi].size();
   std::vector<int> order(ngroups);
@@ -244,6 +257,7 @@ int run_wfa(phx_plan* P) {
     slice_off[gi] = arena;
     arena += gs[gi].n * gs[gi].per;
   }
+  if (arena > P->d_arena.bytes && capture) return PHX_E_INTERNAL;  // never allocate while capturing
   if (arena && dev_alloc(ctx, P->d_arena, (size_t)arena) != PHX_OK) return PHX_E_NOMEM;
   double queued[kStreams] = {0, 0, 0, 0};
   bool used_stream[kStreams] = {true, false, false, false};
@@ -259,17 +273,93 @@ int run_wfa(phx_plan* P) {
     }
     int rc = launch_wfa_tier(P, P->group_tier[gi], wl, (uint32_t*)P->d_work.p + group_off[gi],
                              d_ctr + 2 * gi, gs[gi], (uint8_t*)P->d_arena.p + slice_off[gi], st,
-                             P->group_rl[gi] != 0);
+                             P->group_rl[gi] != 0, !capture);
     if (rc != PHX_OK) return rc;
-    launched.push_back(gi);
+    if (capture) {
+      hipStreamCaptureStatus cs;
+      unsigned long long id = 0;
+      hipGraph_t cg = nullptr;
+      const hipGraphNode_t* deps = nullptr;
+      size_t nd = 0;
+      HIPCHK(hipStreamGetCaptureInfo_v2(st, &cs, &id, &cg, &deps, &nd));
+      if (nd != 1) return PHX_E_INTERNAL;
+      knodes->push_back(deps[0]);
+    }
+    launched->push_back(gi);
   }
-  if (launched.empty()) return PHX_OK;
+  if (launched->empty()) return PHX_OK;
   for (int si = 1; si < kStreams; ++si) {
     if (!used_stream[si]) continue;
     HIPCHK(hipEventRecord(ctx->join[si], ctx->streams[si]));
     HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->join[si], 0));
   }
-  HIPCHK(hipMemcpyAsync(ctr_h, d_ctr, 8 * (size_t)ngroups, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(P->ctr_host, d_ctr, 8 * (size_t)ngroups, hipMemcpyDeviceToHost, ctx->stream));
+  return PHX_OK;
+}
+
+// Captures phase 1 into a graph (nothing runs), with event-record nodes
+// around every kernel node for the per-launch timings.  Called right after a
+// normal run, so every buffer it points to already exists.
+int build_graph(phx_plan* P) {
+  phx_ctx* ctx = P->ctx;
+  if (P->gexec) {
+    (void)hipGraphExecDestroy(P->gexec);
+    P->gexec = nullptr;
+  }
+  std::vector<int> launched;
+  std::vector<hipGraphNode_t> knodes;
+  HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+  const int rc = issue_groups(P, true, &launched, &knodes);
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+  if (rc != PHX_OK || ec != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc != PHX_OK ? rc : fail(ctx, PHX_E_DEVICE, "graph capture failed");
+  }
+  int ret = PHX_OK;
+  for (size_t i = 0; i < knodes.size() && ret == PHX_OK; ++i) {
+    hipEvent_t e0, e1;
+    if (kernel_events(ctx, (uint32_t)i, &e0, &e1) != PHX_OK) ret = PHX_E_DEVICE;
+    size_t nd = 0;
+    std::vector<hipGraphNode_t> deps;
+    hipGraphNode_t n0 = nullptr, n1 = nullptr;
+    if (ret == PHX_OK && hipGraphNodeGetDependencies(knodes[i], nullptr, &nd) == hipSuccess) {
+      deps.resize(nd);
+      if (nd) (void)hipGraphNodeGetDependencies(knodes[i], deps.data(), &nd);
+    }
+    if (ret == PHX_OK &&
+        (hipGraphAddEventRecordNode(&n0, g, deps.data(), deps.size(), e0) != hipSuccess ||
+         hipGraphAddDependencies(g, &n0, &knodes[i], 1) != hipSuccess ||
+         hipGraphAddEventRecordNode(&n1, g, &knodes[i], 1, e1) != hipSuccess))
+      ret = fail(ctx, PHX_E_DEVICE, "graph timing nodes");
+  }
+  if (ret == PHX_OK && hipGraphInstantiate(&P->gexec, g, nullptr, nullptr, 0) != hipSuccess) {
+    P->gexec = nullptr;
+    ret = fail(ctx, PHX_E_DEVICE, "graph instantiation failed");
+  }
+  (void)hipGraphDestroy(g);
+  if (ret != PHX_OK) return ret;
+  P->g_launched = launched;
+  P->g_launches = (uint32_t)knodes.size();
+  P->g_arena = P->d_arena.p;
+  return PHX_OK;
+}
+
+int run_wfa(phx_plan* P) {
+  phx_ctx* ctx = P->ctx;
+  uint32_t* d_ctr = (uint32_t*)P->d_ctr.p;
+  uint32_t* ctr_h = P->ctr_host;
+  std::vector<int> launched;
+  if (P->use_graph && P->gexec && P->g_arena == P->d_arena.p) {
+    HIPCHK(hipGraphLaunch(P->gexec, ctx->stream));
+    launched = P->g_launched;
+    P->last_launches = P->g_launches;
+  } else {
+    const int rc = issue_groups(P, false, &launched, nullptr);
+    if (rc != PHX_OK) return rc;
+    P->graph_wanted = P->use_graph && !launched.empty();
+  }
+  if (launched.empty()) return PHX_OK;
   HIPCHK(hipStreamSynchronize(ctx->stream));
   bool any = false;
   for (int gi : launched) any |= ctr_h[2 * gi + 1] != 0;
@@ -826,6 +916,21 @@ int phx_plan_run(phx_plan* P) {
     kms += k;
   }
   P->last_kernel_ms = kms;
+  if (P->graph_wanted) {  // outside the timed region
+    P->graph_wanted = false;
+    return build_graph(P);
+  }
+  return PHX_OK;
+}
+
+int phx_plan_use_graph(phx_plan* P, int enable) {
+  if (!P || enable < 0 || enable > 1) return PHX_E_ARG;
+  std::lock_guard<std::mutex> lk(P->ctx->mu);
+  P->use_graph = enable != 0 && is_wfa(P->engine);
+  if (!P->use_graph && P->gexec) {
+    (void)hipGraphExecDestroy(P->gexec);
+    P->gexec = nullptr;
+  }
   return PHX_OK;
 }
 
@@ -840,6 +945,10 @@ int phx_plan_set_tuning(phx_plan* P, const phx_tuning* t) {
   HIPCHK(hipSetDevice(ctx->device));
   P->tun = merge_tuning(P->tun, t);
   if (!is_wfa(P->engine)) return PHX_OK;
+  if (P->gexec) {  // the launch sequence changes with the grouping
+    (void)hipGraphExecDestroy(P->gexec);
+    P->gexec = nullptr;
+  }
   plan_group(P);
   return upload_work(P);
 }
@@ -1038,6 +1147,7 @@ void phx_plan_destroy(phx_plan* P) {
                     &P->d_scratch, &P->d_gather, &P->d_gidx})
     dev_free(*b);
   dev_free(P->d_arena);
+  if (P->gexec) (void)hipGraphExecDestroy(P->gexec);
   if (P->ctr_host) (void)hipHostFree(P->ctr_host);
   delete P;
 }

@@ -284,6 +284,54 @@ def test_results_do_not_depend_on_tuning(px):
     plan.close()
 
 
+def test_graph_replays_match_eager_runs(px):
+    """Graph mode: the launch sequence recorded once and replayed gives the
+    eager run's results on every replay, with other eager work (another plan
+    on the same context, result downloads) between replays, across a retune
+    (re-recorded graph), and with overflow re-runs after the replay."""
+    from phraya_amd import synth
+    w = synth.workload("mixed", n=300)
+    idx = [i for i in range(w.n) if w.q_off[i + 1] - w.q_off[i] <= 6000][:120]
+    sub = w.subset(idx)
+    pb = px.PackedBatch(sub.q_seq, sub.q_off, sub.t_seq, sub.t_off,
+                        np.zeros(sub.n, np.uint64), np.zeros(sub.n, np.uint64))
+    # A narrow LDS ring makes some workgroup-tier pairs overflow every run.
+    ctx = px.Context(0, dict(block256_min_len=2048, block1024_min_len=6000, block_lds_ring=128))
+    short = synth.workload("short150", n=2000)
+    pb2 = px.PackedBatch(short.q_seq, short.q_off, short.t_seq, short.t_off,
+                         np.zeros(short.n, np.uint64), np.zeros(short.n, np.uint64))
+    other = px.Plan(pb2, px.ENGINE_LINEAR, ctx=ctx)
+    other.run()
+    other_ref = other.results()
+    for eng in (px.ENGINE_AFFINE, px.ENGINE_LINEAR):
+        plan = px.Plan(pb, eng, px.AffineCosts(4, 6, 2), ctx=ctx)
+        plan.run()
+        ref = plan.results()
+        launches = plan.last_launches()
+        plan.use_graph(True)
+        for rep in range(5):
+            if rep == 3:
+                plan.set_tuning(dict(heavy_permille=300))
+            plan.run()
+            say(f"engine {eng} replay {rep}: {plan.last_launches()} launches "
+                f"{plan.last_kernel_ms():.3f} ms kernels, {plan.last_ms():.3f} ms run")
+            assert plan.last_kernel_ms() > 0
+            if rep not in (0, 3):  # recorded runs execute eagerly
+                assert plan.last_launches() >= launches - 1
+            r = plan.results()
+            assert np.array_equal(r.aln, ref.aln) and np.array_equal(r.runs, ref.runs), f"replay {rep}"
+            other.run()
+            o = other.results()
+            assert np.array_equal(o.aln, other_ref.aln) and np.array_equal(o.runs, other_ref.runs)
+        plan.use_graph(False)
+        plan.run()
+        r = plan.results()
+        assert np.array_equal(r.aln, ref.aln) and np.array_equal(r.runs, ref.runs)
+        plan.close()
+    other.close()
+    ctx.close()
+
+
 def test_lds_ring_workgroup_tier_and_width_overflow(px):
     """Workgroup tiers with the score ring in LDS: pairs routed there by their
     estimate, and (with a deliberately narrow ring) pairs that outgrow it and
