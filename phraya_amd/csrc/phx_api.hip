@@ -49,7 +49,8 @@ struct phx_plan {
   DevBuf d_seq, d_pairs, d_out, d_runs, d_work, d_work2, d_ctr, d_scratch;
   DevBuf d_gather, d_gidx;  // compacted CIGAR runs and their source / target offsets
   DevBuf d_arena;  // HBM-tier scratch (history + ring slots); each group of a run gets its own slice
-  uint32_t* ctr_host = nullptr;  // pinned readback of the tier counters
+  uint32_t* ctr_host = nullptr;  // pinned overflow flags, one per group (kernels write them)
+  uint32_t* ctr_dev = nullptr;   // the same flags' device address
   std::vector<std::vector<uint32_t>> work;             // initial work per launch group / Myers group
   std::vector<int> group_tier;                         // WFA: 0 = LDS tier, 1 = HBM tier
   std::vector<uint64_t> group_slot;                    // WFA: HBM slot bytes of each group
@@ -79,6 +80,11 @@ struct phx_plan {
   double last_ms = 0;
   double last_kernel_ms = 0;
   uint32_t last_launches = 0;
+  // Timing of the current run: launch 0 may start at ctx->ev0 (nothing else
+  // between them on the stream), and the run may end at an event recorded
+  // by run_wfa (null: phx_plan_run records ctx->ev1).
+  bool first_start_ev0 = false;
+  hipEvent_t end_ev = nullptr;
 };
 
 namespace {
@@ -175,15 +181,21 @@ GroupSlots group_slots(const phx_plan* P, int t, const std::vector<uint32_t>& wo
 
 // Launches one group: tier t over `work` on stream `st`; HBM tiers use the
 // scratch slice [slice, slice + gs.n * gs.per).
+// Timing events around a launch: both, the end only (the start is the run's
+// ctx->ev0, recorded right before on the same stream), or none (graph
+// capture: event nodes are added around the kernel node).
+enum LaunchTiming { TIME_BOTH, TIME_END, TIME_NONE };
+
 int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint32_t* d_list,
-                    uint32_t* d_counters, const GroupSlots& gs, uint8_t* slice, hipStream_t st,
-                    bool rl = false, bool timing = true) {
+                    uint32_t* d_counters, uint32_t* overflow, const GroupSlots& gs, uint8_t* slice,
+                    hipStream_t st, bool rl = false, LaunchTiming timing = TIME_BOTH) {
   phx_ctx* ctx = P->ctx;
   WfaArgs a{};
   a.pairs = (const PairDesc*)P->d_pairs.p;
   a.work_list = d_list;
   a.nwork = (uint32_t)work.size();
   a.counters = d_counters;
+  a.overflow = overflow;
   a.cp = CostParams{P->costs.mismatch, P->costs.gap_open, P->costs.gap_extend,
                     P->cap < 0 ? -1 : (int32_t)std::min<int64_t>(P->cap, INT32_MAX)};
   a.out = (PairOut*)P->d_out.p;
@@ -195,7 +207,8 @@ int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint3
     uint32_t seq_words = 2;
     for (uint32_t i : work) seq_words = std::max(seq_words, lds_seq_words(P->pairs[i]));
     G = lds_geometry(ring_frames(P->engine, P->costs), seq_words, P->tun.wave_lds_kib);
-    slots = std::min<uint32_t>((uint32_t)work.size(), std::max<uint32_t>(kWaveWaves, P->tun.wave_slots));
+    slots = lds_tier_waves(work.size(), P->tun.wave_slots);
+    a.static_order = 1;  // the list is stride-balanced for exactly these waves
   } else {
     a.hist = slice;
     a.hist_stride = gs.hist;
@@ -209,15 +222,16 @@ int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint3
       G = block_lds_geometry(ring_frames(P->engine, P->costs), gs.wcap, seq_words);
     }
   }
-  if (!timing) {  // graph capture: timing nodes are added around the kernel node
+  if (timing == TIME_NONE) {
     HIPCHK(launch_tier(t, mode, a, G, slots, st, rl));
     return PHX_OK;
   }
   hipEvent_t e0, e1;
   if (kernel_events(ctx, P->last_launches, &e0, &e1) != PHX_OK) return PHX_E_DEVICE;
-  HIPCHK(hipEventRecord(e0, st));
+  if (timing == TIME_BOTH) HIPCHK(hipEventRecord(e0, st));
   HIPCHK(launch_tier(t, mode, a, G, slots, st, rl));
   HIPCHK(hipEventRecord(e1, st));
+  if (P->last_launches == 0) P->first_start_ev0 = timing == TIME_END;
   ++P->last_launches;
   return PHX_OK;
 }
@@ -236,8 +250,18 @@ int issue_groups(phx_plan* P, bool capture, std::vector<int>* launched,
   phx_ctx* ctx = P->ctx;
   const int ngroups = (int)P->work.size();
   uint32_t* d_ctr = (uint32_t*)P->d_ctr.p;
-  HIPCHK(launch_zero(d_ctr, 2 * (uint32_t)ngroups, ctx->stream));
-  HIPCHK(hipEventRecord(ctx->join[0], ctx->stream));
+  // Queue heads of the groups that use one (the LDS group strides over its
+  // list statically); overflow flags live in pinned host memory and are
+  // cleared by the host before each run.
+  bool queued_group = false;
+  for (int gi = 0; gi < ngroups; ++gi)
+    queued_group |= !P->work[gi].empty() && P->group_tier[gi] != TIER_LDS;
+  if (queued_group) HIPCHK(launch_zero(d_ctr, 2 * (uint32_t)ngroups, ctx->stream));
+  int nonempty = 0;
+  for (int gi = 0; gi < ngroups; ++gi) nonempty += !P->work[gi].empty();
+  // One launch and nothing else on the stream: its end event ends the run.
+  const bool single = !capture && !queued_group && nonempty == 1;
+  if (!single) HIPCHK(hipEventRecord(ctx->join[0], ctx->stream));
   // Group gi's list sits at offset group_off[gi] of d_work.
   std::vector<uint64_t> group_off(ngroups + 1, 0);
   for (int gi = 0; gi < ngroups; ++gi) group_off[gi + 1] = group_off[gi] + P->work[gi].size();
@@ -272,8 +296,9 @@ int issue_groups(phx_plan* P, bool capture, std::vector<int>* launched,
       used_stream[si] = true;
     }
     int rc = launch_wfa_tier(P, P->group_tier[gi], wl, (uint32_t*)P->d_work.p + group_off[gi],
-                             d_ctr + 2 * gi, gs[gi], (uint8_t*)P->d_arena.p + slice_off[gi], st,
-                             P->group_rl[gi] != 0, !capture);
+                             d_ctr + 2 * gi, P->ctr_dev + gi, gs[gi],
+                             (uint8_t*)P->d_arena.p + slice_off[gi], st, P->group_rl[gi] != 0,
+                             capture ? TIME_NONE : TIME_BOTH);
     if (rc != PHX_OK) return rc;
     if (capture) {
       hipStreamCaptureStatus cs;
@@ -293,7 +318,15 @@ int issue_groups(phx_plan* P, bool capture, std::vector<int>* launched,
     HIPCHK(hipEventRecord(ctx->join[si], ctx->streams[si]));
     HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->join[si], 0));
   }
-  HIPCHK(hipMemcpyAsync(P->ctr_host, d_ctr, 8 * (size_t)ngroups, hipMemcpyDeviceToHost, ctx->stream));
+  if (!capture) {
+    hipEvent_t e0, e1;
+    if (single && kernel_events(ctx, 0, &e0, &e1) == PHX_OK) {
+      P->end_ev = e1;
+    } else {
+      HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+      P->end_ev = ctx->ev1;
+    }
+  }
   return PHX_OK;
 }
 
@@ -348,10 +381,13 @@ int build_graph(phx_plan* P) {
 int run_wfa(phx_plan* P) {
   phx_ctx* ctx = P->ctx;
   uint32_t* d_ctr = (uint32_t*)P->d_ctr.p;
-  uint32_t* ctr_h = P->ctr_host;
+  volatile uint32_t* ctr_h = P->ctr_host;
+  for (size_t gi = 0; gi < P->work.size(); ++gi) ctr_h[gi] = 0;
   std::vector<int> launched;
   if (P->use_graph && P->gexec && P->g_arena == P->d_arena.p) {
     HIPCHK(hipGraphLaunch(P->gexec, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    P->end_ev = ctx->ev1;
     launched = P->g_launched;
     P->last_launches = P->g_launches;
   } else {
@@ -362,7 +398,7 @@ int run_wfa(phx_plan* P) {
   if (launched.empty()) return PHX_OK;
   HIPCHK(hipStreamSynchronize(ctx->stream));
   bool any = false;
-  for (int gi : launched) any |= ctr_h[2 * gi + 1] != 0;
+  for (int gi : launched) any |= ctr_h[gi] != 0;
   if (!any) return PHX_OK;
   // Re-route pairs that overflowed their group (rare path): LDS-tier pairs go
   // to an HBM slot sized by estimate, HBM pairs to a slot sized exactly from
@@ -394,6 +430,7 @@ int run_wfa(phx_plan* P) {
     if (dev_alloc(ctx, P->d_work2, wl.size() * 4) != PHX_OK) return PHX_E_NOMEM;
     HIPCHK(hipMemcpy(P->d_work2.p, wl.data(), wl.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(launch_zero(d_ctr, 2, ctx->stream));
+    ctr_h[0] = 0;
     uint64_t maxL = 0;
     for (uint32_t i : wl) maxL = std::max<uint64_t>(maxL, (uint64_t)P->pairs[i].qn + P->pairs[i].tn);
     // Re-runs are few and run after everything else: give them wide tiers.
@@ -401,13 +438,14 @@ int run_wfa(phx_plan* P) {
     // Every group has finished: the re-run may use the whole arena.
     const GroupSlots rg = group_slots(P, rt, wl, slot_bytes, P->group_budget(-1));
     if (dev_alloc(ctx, P->d_arena, (size_t)(rg.n * rg.per)) != PHX_OK) return PHX_E_NOMEM;
-    int rc = launch_wfa_tier(P, rt, wl, (uint32_t*)P->d_work2.p, d_ctr, rg, (uint8_t*)P->d_arena.p,
-                             ctx->stream);
+    int rc = launch_wfa_tier(P, rt, wl, (uint32_t*)P->d_work2.p, d_ctr, P->ctr_dev, rg,
+                             (uint8_t*)P->d_arena.p, ctx->stream);
     if (rc != PHX_OK) return rc;
-    HIPCHK(hipMemcpyAsync(ctr_h, d_ctr, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    P->end_ev = ctx->ev1;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     std::vector<uint32_t> again;
-    if (ctr_h[1] != 0) {
+    if (ctr_h[0] != 0) {
       HIPCHK(hipMemcpy(outs.data(), P->d_out.p, sizeof(PairOut) * P->n, hipMemcpyDeviceToHost));
       for (uint32_t i : wl)
         if (overflowed(i)) again.push_back(i);
@@ -452,6 +490,8 @@ void plan_group(phx_plan* P) {
   P->group_tier.clear();
   P->group_slot.clear();
   P->group_rl.clear();
+  // LDS tier: no work queue; the waves stride over a cost-balanced list.
+  stride_balance(bucket[0], P->est, lds_tier_waves(bucket[0].size(), P->tun.wave_slots));
   P->work.push_back(bucket[0]);
   P->group_tier.push_back(TIER_LDS);
   P->group_rl.push_back(0);
@@ -617,7 +657,8 @@ phx_plan* plan_setup(phx_plan* P, const std::vector<const uint8_t*>& qh,
   if ((words && hipMemcpy(P->d_seq.p, packed->data(), words * 8, hipMemcpyHostToDevice) != hipSuccess) ||
       (n && hipMemcpy(P->d_pairs.p, P->pairs.data(), sizeof(PairDesc) * n, hipMemcpyHostToDevice) != hipSuccess))
     return fail_plan("upload");
-  if (hipHostMalloc((void**)&P->ctr_host, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
+  if (hipHostMalloc((void**)&P->ctr_host, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&P->ctr_dev, P->ctr_host, 0) != hipSuccess)
     return fail_plan("pinned counters");
   // Work lists are static between tunings: upload once.
   if (upload_work(P) != PHX_OK) return fail_plan("work upload");
@@ -901,18 +942,24 @@ int phx_plan_run(phx_plan* P) {
   HIPCHK(hipSetDevice(ctx->device));
   P->out_valid = false;
   P->last_launches = 0;
+  P->first_start_ev0 = false;
+  P->end_ev = nullptr;
   HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
   int rc = is_wfa(P->engine) ? run_wfa(P) : run_myers(P);
   if (rc != PHX_OK) return rc;
-  HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
-  HIPCHK(hipEventSynchronize(ctx->ev1));
+  if (!P->end_ev) {
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    P->end_ev = ctx->ev1;
+  }
+  HIPCHK(hipEventSynchronize(P->end_ev));
   float ms = 0;
-  HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, P->end_ev));
   P->last_ms = ms;
   double kms = 0;
   for (uint32_t i = 0; i < P->last_launches; ++i) {
     float k = 0;
-    HIPCHK(hipEventElapsedTime(&k, ctx->kev[2 * i], ctx->kev[2 * i + 1]));
+    const hipEvent_t start = i == 0 && P->first_start_ev0 ? ctx->ev0 : ctx->kev[2 * i];
+    HIPCHK(hipEventElapsedTime(&k, start, ctx->kev[2 * i + 1]));
     kms += k;
   }
   P->last_kernel_ms = kms;

@@ -86,7 +86,8 @@ struct WfaArgs {
   const PairDesc* pairs;
   const uint32_t* work_list;
   uint32_t nwork;
-  uint32_t* counters;    // [0] work queue head, [1] pairs that overflowed this tier
+  uint32_t* counters;    // [0] work queue head (zero at launch unless static_order)
+  uint32_t* overflow;    // host-visible flag set to 1 when a pair overflows; may be null
   CostParams cp;
   PairOut* out;
   uint32_t* runs;
@@ -95,6 +96,7 @@ struct WfaArgs {
   int32_t* ring_g;       // one slot of ring_stride ints per workgroup (HBM ring tiers)
   uint64_t ring_stride;
   uint32_t wcap_log2;    // ring width in diagonals = 1 << wcap_log2
+  uint32_t static_order; // wave kernels: wave g takes items g, g + waves, ... (no queue)
 };
 
 // LDS layout of one wave of the wave-resident WFA tier (byte offsets inside

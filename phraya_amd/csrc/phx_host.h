@@ -278,6 +278,28 @@ inline WaveGeom lds_geometry(int32_t frames, uint32_t max_seq_words, uint32_t ld
   return G;
 }
 
+// Waves of an LDS-tier launch over n pairs with `wave_slots` waves wanted:
+// whole workgroups of kWaveWaves, at most one wave per pair (rounded up).
+inline uint32_t lds_tier_waves(uint64_t n, uint32_t wave_slots) {
+  const uint32_t slots = (uint32_t)std::min<uint64_t>(n, std::max<uint32_t>(kWaveWaves, wave_slots));
+  return std::max<uint32_t>(1, (slots + kWaveWaves - 1) / kWaveWaves) * kWaveWaves;
+}
+
+// Order of an LDS-tier work list for static striding over `waves` waves
+// (wave g takes positions g, g + waves, ...): pairs by estimated cost,
+// largest first, dealt to the waves in alternating direction each round.
+inline void stride_balance(std::vector<uint32_t>& list, const std::vector<int32_t>& est, uint32_t waves) {
+  std::stable_sort(list.begin(), list.end(), [&](uint32_t a, uint32_t b) { return est[a] > est[b]; });
+  std::vector<uint32_t> out(list.size());
+  const size_t S = waves;
+  for (size_t r0 = 0; r0 < list.size(); r0 += S) {
+    const size_t m = std::min(S, list.size() - r0);
+    const bool rev = (r0 / S) & 1;
+    for (size_t g = 0; g < m; ++g) out[r0 + g] = list[r0 + (rev ? m - 1 - g : g)];
+  }
+  list.swap(out);
+}
+
 // Words the LDS tier stages for one pair (query then target, with shifts).
 inline uint32_t lds_seq_words(const PairDesc& pd) {
   const bool b8 = pd.flags & PF_BYTES;

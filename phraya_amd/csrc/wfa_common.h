@@ -11,47 +11,65 @@ namespace phx {
 
 constexpr int32_t UNSET = INT32_MIN;
 
+// Element types qualified with their address space (1 = global, 3 = LDS), so
+// every access compiles to a global_ / ds_ instruction instead of flat.
+template <class T>
+using gmem_t = __attribute__((address_space(1))) T;
+template <class T>
+using lmem_t = __attribute__((address_space(3))) T;
+
 // --- 2-bit / 8-bit match extension -------------------------------------------
 
-// Symbols [pos, pos + per_word) of a packed sequence as one little-endian word.
-// WP: a pointer to const u64 words in LDS or global memory (address space
-// carried by the type, so accesses compile to ds_ / global_ instructions).
-template <bool B8, class WP>
-__device__ __forceinline__ uint64_t seq_window(WP w, uint32_t pos) {
-  constexpr uint32_t LOG = B8 ? 3 : 5;
-  constexpr uint32_t BITS = B8 ? 8 : 2;
-  const uint32_t wi = pos >> LOG;
-  const uint32_t sh = (pos & ((1u << LOG) - 1)) * BITS;
-  const uint64_t lo = w[wi] >> sh;
-  const uint64_t hi = (w[wi + 1] << (63 - sh)) << 1;  // zero when sh == 0
-  return lo | hi;
+// The same packed words seen as 32-bit halves, in the same address space.
+__device__ __forceinline__ const lmem_t<const uint32_t>* as_dwords(const lmem_t<const uint64_t>* w) {
+  return (const lmem_t<const uint32_t>*)w;
+}
+__device__ __forceinline__ const gmem_t<const uint32_t>* as_dwords(const gmem_t<const uint64_t>* w) {
+  return (const gmem_t<const uint32_t>*)w;
+}
+
+// 64 bits of a packed symbol stream starting at bit `bit` (symbol 0 in the
+// lowest bits of dword 0): two funnel shifts over three dwords.  Reads dwords
+// bit / 32 .. bit / 32 + 2, inside the stream's words (each packed sequence
+// carries one word past its last symbol).
+template <class DP>
+__device__ __forceinline__ uint64_t bit_window(DP d, uint32_t bit) {
+  const uint32_t i = bit >> 5;
+  const uint32_t d0 = d[i], d1 = d[i + 1], d2 = d[i + 2];
+  const uint32_t lo = __builtin_amdgcn_alignbit(d1, d0, bit);  // shift = bit & 31
+  const uint32_t hi = __builtin_amdgcn_alignbit(d2, d1, bit);
+  return ((uint64_t)hi << 32) | lo;
 }
 
 // Upstream `extend` closure (wfa_simd.rs:252-262, :608-616): unchanged outside
 // the matrix, else position + longest common prefix of the two suffixes.
 // 32 bases (or 8 raw bytes) are compared per 64-bit XOR; the first differing
-// symbol is the lowest set bit.
+// symbol is the lowest set bit.  The loop is uniform over the wave: a lane
+// that has found its mismatch (or was never inside the matrix) keeps its
+// position and count while the others go on, so no per-lane branches.
 // qsh / tsh: index of symbol 0 inside word 0 of each sequence.
 template <bool B8, class WP>
 __device__ __forceinline__ int32_t extend_diag(WP qw, uint32_t qsh, WP tw, uint32_t tsh,
                                                int32_t qn, int32_t tn, int32_t p, int32_t k) {
-  const int32_t j = p - k;
-  if (p < 0 || p >= qn || j < 0 || j >= tn) return p;
-  const int32_t lim = min(qn - p, tn - j);
-  constexpr int32_t SPW = B8 ? 8 : 32;
+  constexpr uint32_t BITS = B8 ? 8 : 2;
   constexpr uint32_t CTZ_SHIFT = B8 ? 3 : 1;
+  const int32_t j = p - k;
+  const bool inside = p >= 0 && p < qn && j >= 0 && j < tn;
+  bool act = inside;
+  const int32_t lim = inside ? min(qn - p, tn - j) : 0;
+  const uint32_t qb = (qsh + (uint32_t)(inside ? p : 0)) * BITS;
+  const uint32_t tb = (tsh + (uint32_t)(inside ? j : 0)) * BITS;
+  const auto qd = as_dwords(qw);
+  const auto td = as_dwords(tw);
   int32_t n = 0;
-  for (;;) {
-    const uint64_t d = seq_window<B8>(qw, qsh + (uint32_t)(p + n)) ^
-                       seq_window<B8>(tw, tsh + (uint32_t)(j + n));
-    if (d) {
-      n += (int32_t)((uint32_t)__builtin_ctzll(d) >> CTZ_SHIFT);
-      break;
-    }
-    n += SPW;
-    if (n >= lim) break;
+  while (__ballot(act)) {
+    const uint64_t d = bit_window(qd, qb + (uint32_t)n * BITS) ^ bit_window(td, tb + (uint32_t)n * BITS);
+    // Equal symbols before the first difference; a whole window when none.
+    const int32_t z = (int32_t)((uint32_t)__builtin_ctzg(d, 64) >> CTZ_SHIFT);
+    n = act ? n + z : n;
+    act = act && d == 0 && n < lim;
   }
-  return p + min(n, lim);
+  return inside ? p + min(n, lim) : p;
 }
 
 __device__ __forceinline__ bool pos_valid(int32_t pos, int32_t k, int32_t qn, int32_t tn) {

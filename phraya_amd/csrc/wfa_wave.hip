@@ -81,12 +81,6 @@ __device__ __forceinline__ int32_t readlane(int32_t v, int32_t lane) {
 __device__ __forceinline__ int32_t step_lo(int32_t u, int32_t tn) { return max(-u, -tn); }
 __device__ __forceinline__ int32_t step_hi(int32_t u, int32_t qn) { return min(u, qn); }
 
-// Element types qualified with their address space (1 = global, 3 = LDS), so
-// every access compiles to a global_ / ds_ instruction instead of flat.
-template <class T>
-using gmem_t = __attribute__((address_space(1))) T;
-template <class T>
-using lmem_t = __attribute__((address_space(3))) T;
 template <bool IN_LDS, class T>
 using mem_t = typename std::conditional<IN_LDS, lmem_t<T>, gmem_t<T>>::type;
 
@@ -220,12 +214,14 @@ __device__ __forceinline__ int32_t gap_run(const S_t& S, int32_t s, int32_t k,
   return mask ? (int32_t)__builtin_ctzll(mask) + 1 : 64;
 }
 
-// Thread 0 stores the pair's result and counts an overflow for the host (one
-// counter read tells the host whether any pair must be re-run).
+// Thread 0 stores the pair's result and, on an overflow, raises the launch's
+// host-visible flag (a system-scope store to pinned host memory, so the host
+// learns whether any pair must be re-run without a copy).
 __device__ __forceinline__ void publish(const WfaArgs& a, PairOut* po, const PairOut& r, int lane) {
   if (lane == 0) {
     *po = r;
-    if (r.status >= ST_OVERFLOW_HIST) atomicAdd(&a.counters[1], 1u);
+    if (r.status >= ST_OVERFLOW_HIST && a.overflow)
+      __hip_atomic_store(a.overflow, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
@@ -233,8 +229,221 @@ __device__ __forceinline__ void publish(const WfaArgs& a, PairOut* po, const Pai
 // triple-buffered so a step can reset the next buffer before its barrier.
 struct PairShared {
   int32_t key[3];
-  uint32_t work;
+  uint32_t next[2];  // next work item, by parity of the item count
 };
+
+// --- register wavefront (LDS tier) -------------------------------------------
+//
+// While a pair's diagonal range fits 64 diagonals, lane L of the wave owns
+// diagonal kb + L for the whole pair and keeps the last score frames of its
+// diagonal in registers; the k - 1 / k + 1 neighbours come from the adjacent
+// lanes by a wave-wide DPP shift.  A lane outside a step's range holds -1
+// (unset) for that step, which is exactly what the ring path's range checks
+// produce, so no range checks are needed on reads.  History cells go to the
+// same LDS layout as the ring path (cells at toff[step] + k - lo), so the
+// traceback is shared.  When the next step's range would leave the window,
+// the frames are written to the LDS ring and the ring path continues from
+// that step; results are identical either way.
+
+// Value of the lane below (k - 1); lane 0 reads -1.  Call with every lane
+// active: a source lane outside EXEC also reads as -1.
+__device__ __forceinline__ int32_t from_below(int32_t v) {
+  return __builtin_amdgcn_update_dpp(-1, v, 0x138, 0xF, 0xF, false);  // wave_shr:1
+}
+// Value of the lane above (k + 1); lane 63 reads -1.
+__device__ __forceinline__ int32_t from_above(int32_t v) {
+  return __builtin_amdgcn_update_dpp(-1, v, 0x130, 0xF, 0xF, false);  // wave_shl:1
+}
+
+// Loop state shared by the register phase and the ring loop.
+struct StepState {
+  int32_t step;
+  uint64_t used;
+  bool found;
+  int32_t end_k, end_layer;
+  int32_t status;
+};
+
+// First diagonal of the register window: 64 diagonals centred on 0, or
+// starting at -tn when the target is short.
+__device__ __forceinline__ int32_t reg_window_base(int32_t tn) { return max(-32, -tn); }
+
+// Writes the toff entries of the completed steps 1 .. done (<= 63) held by
+// the lanes.
+__device__ __forceinline__ void flush_toff(const WaveStore<true>& S, uint32_t toffr, int32_t done, int lane) {
+  if (lane >= 1 && lane <= done) S.toff(lane) = (uint16_t)toffr;
+}
+
+// Gap-affine steps in registers.  XN, OEN, EN: mismatch, open + extend and
+// extend costs in score steps; the m layer keeps max(XN, OEN) frames, each
+// gap layer EN.  Recurrence and tie rules as in wave_affine's ring loop.
+template <int XN, int OEN, int EN, bool B8>
+__device__ void affine_regs(const WaveStore<true>& S, int32_t qn, int32_t tn, int32_t m0,
+                            bool fitting, int32_t max_step, int32_t Rm, int32_t Rg, int lane,
+                            StepState& st) {
+  constexpr int MD = XN > OEN ? XN : OEN;
+  const int32_t kb = reg_window_base(tn);
+  const int32_t k = kb + lane;
+  int32_t Mf[MD], If[EN], Df[EN];  // [j] = frame of step (step - j)
+#pragma unroll
+  for (int j = 0; j < MD; ++j) Mf[j] = -1;
+#pragma unroll
+  for (int j = 0; j < EN; ++j) If[j] = Df[j] = -1;
+  Mf[0] = k == 0 ? m0 : -1;
+  uint32_t toffr = 0;  // lane s: toff of step s
+  const int32_t kend_g = qn - tn;
+  int32_t step = 0;
+  uint64_t used = 0;
+  bool found = false;
+  int32_t end_k = 0, end_layer = 0, status = ST_OK;
+  for (;;) {
+    const int32_t s = step + 1;
+    const int32_t lo = step_lo(s, tn), hi = step_hi(s, qn);
+    if (lo < kb || hi > kb + 63 || s > 63) break;  // continue on the ring
+    step = s;
+    if (s > max_step) { status = ST_ALIGNMENT_FAILED; break; }
+    const int32_t width = hi - lo + 1;
+    if (!S.fits(s, used, width)) { status = ST_OVERFLOW_HIST; break; }
+    toffr = lane == s ? (uint32_t)used : toffr;
+    const bool inr = k >= lo && k <= hi;
+    const int32_t vx = Mf[XN - 1];
+    const int32_t vol = from_below(Mf[OEN - 1]), vor = from_above(Mf[OEN - 1]);
+    const int32_t vil = from_below(If[EN - 1]), vdr = from_above(Df[EN - 1]);
+    int32_t ins = (vol >= 0 && vol < qn) ? vol + 1 : -1;
+    uint32_t iop = 0;
+    if (vil >= 0 && vil < qn && vil + 1 > ins) { ins = vil + 1; iop = 1; }
+    int32_t del = (vor >= 0 && vor - k <= tn) ? vor : -1;
+    uint32_t dop = 0;
+    if (vdr >= 0 && vdr - k <= tn && vdr > del) { del = vdr; dop = 1; }
+    int32_t best = (vx >= 0 && vx < qn && vx + 1 - k <= tn) ? vx + 1 : -1;
+    uint32_t mop = 0;
+    if (ins > best) { best = ins; mop = 1; }
+    if (del > best) { best = del; mop = 2; }
+    const int32_t mval = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, inr ? best : -1, k);
+    if (inr)
+      S.cells[used + (uint64_t)(k - lo)] =
+          (uint16_t)(((uint32_t)(best + 1) << 4) | mop | (iop << 2) | (dop << 3));
+    const int32_t ke_lo = fitting ? max(kend_g, lo) : kend_g;
+    const int32_t ke_hi = fitting ? hi : kend_g;
+    int32_t layer = -1;
+    if (inr && k >= ke_lo && k <= ke_hi) layer = mval == qn ? 0 : ins == qn ? 1 : del == qn ? 2 : -1;
+#pragma unroll
+    for (int j = MD - 1; j > 0; --j) Mf[j] = Mf[j - 1];
+#pragma unroll
+    for (int j = EN - 1; j > 0; --j) { If[j] = If[j - 1]; Df[j] = Df[j - 1]; }
+    Mf[0] = inr ? mval : -1;
+    If[0] = inr ? ins : -1;
+    Df[0] = inr ? del : -1;
+    used += (uint64_t)width;
+    const uint64_t mask = __ballot(layer >= 0);
+    if (mask) {
+      const int32_t ln = (int32_t)__builtin_ctzll(mask);
+      found = true;
+      end_k = kb + ln;
+      end_layer = readlane(layer, ln);
+      break;
+    }
+  }
+  flush_toff(S, toffr, status == ST_OK ? step : step - 1, lane);
+  if (!found && status == ST_OK && step > 0) {
+    // Hand the frames to the ring: frame of step (step - j) into its slot.
+    const uint32_t W = S.wmask + 1;
+    const uint32_t ik = (uint32_t)k & S.wmask;
+    lmem_t<int16_t>* Mr = S.ring;
+    lmem_t<int16_t>* Ri = Mr + (size_t)Rm * W;
+    lmem_t<int16_t>* Rd = Ri + (size_t)Rg * W;
+#pragma unroll
+    for (int j = 0; j < MD; ++j)
+      if (step - j >= 0) Mr[(size_t)((step - j) % Rm) * W + ik] = (int16_t)Mf[j];
+#pragma unroll
+    for (int j = 0; j < EN; ++j)
+      if (step - j >= 0) {
+        Ri[(size_t)((step - j) % Rg) * W + ik] = (int16_t)If[j];
+        Rd[(size_t)((step - j) % Rg) * W + ik] = (int16_t)Df[j];
+      }
+  }
+  wave_sync<true>();
+  st = StepState{step, used, found, end_k, end_layer, status};
+}
+
+// Cost shapes with a register phase; others run the ring loop from step 1.
+template <bool B8>
+__device__ __forceinline__ bool affine_regs_any(const WaveStore<true>& S, int32_t qn, int32_t tn,
+                                                int32_t m0, bool fitting, int32_t max_step,
+                                                int32_t xn, int32_t oen, int32_t en, int lane,
+                                                StepState& st) {
+  const int32_t Rm = max(xn, oen) + 1, Rg = en + 1;
+  if (xn == 2 && oen == 4 && en == 1)  // 4/6/2
+    affine_regs<2, 4, 1, B8>(S, qn, tn, m0, fitting, max_step, Rm, Rg, lane, st);
+  else if (xn == 2 && oen == 5 && en == 1)  // 2/4/1
+    affine_regs<2, 5, 1, B8>(S, qn, tn, m0, fitting, max_step, Rm, Rg, lane, st);
+  else if (xn == 1 && oen == 1 && en == 1)  // 1/0/1
+    affine_regs<1, 1, 1, B8>(S, qn, tn, m0, fitting, max_step, Rm, Rg, lane, st);
+  else if (xn == 3 && oen == 7 && en == 2)  // 3/5/2
+    affine_regs<3, 7, 2, B8>(S, qn, tn, m0, fitting, max_step, Rm, Rg, lane, st);
+  else
+    return false;
+  return true;
+}
+
+// Gap-linear steps in registers (one frame back); recurrence, bounds and tie
+// rules as in wave_linear's ring loop.
+template <bool B8>
+__device__ void linear_regs(const WaveStore<true>& S, int32_t qn, int32_t tn, int32_t w0,
+                            bool fitting, int32_t explore, int lane, StepState& st, bool& exhausted) {
+  const int32_t kb = reg_window_base(tn);
+  const int32_t k = kb + lane;
+  int32_t Pf = k == 0 ? w0 : -1;
+  uint32_t toffr = 0;
+  const int32_t kend_global = qn - tn;
+  int32_t step = 0;
+  uint64_t used = 0;
+  bool found = false;
+  int32_t end_k = 0, status = ST_OK;
+  for (;;) {
+    const int32_t s = step + 1;
+    const int32_t lo = step_lo(s, tn), hi = step_hi(s, qn);
+    if (lo < kb || hi > kb + 63 || s > 63) break;
+    step = s;
+    if (s > explore) { exhausted = true; break; }
+    const int32_t width = hi - lo + 1;
+    if (!S.fits(s, used, width)) { status = ST_OVERFLOW_HIST; break; }
+    toffr = lane == s ? (uint32_t)used : toffr;
+    const bool inr = k >= lo && k <= hi;
+    // The shifts run on every lane: a lane left out of a DPP move would read
+    // as -1 to its neighbour.
+    const int32_t below = from_below(Pf), above = from_above(Pf);
+    const int32_t vm = Pf;
+    const int32_t vl = k > -tn ? below : -1;
+    const int32_t vr = k < qn ? above : -1;
+    int32_t best = -1;
+    uint32_t op = 0;
+    if (vm >= 0) { best = vm + 1; op = 1; }
+    if (vl >= 0 && vl + 1 >= best) { best = vl + 1; op = 2; }
+    if (vr >= 0 && vr >= best) { best = vr; op = 3; }
+    const bool ok = op != 0 && best <= qn && best - k >= 0 && best - k <= tn;
+    const int32_t val = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, inr && ok ? best : -1, k);
+    if (inr) S.cells[used + (uint64_t)(k - lo)] = (uint16_t)(ok ? (((uint32_t)(best + 1) << 4) | op) : 0u);
+    const int32_t ke_lo = fitting ? max(qn - tn, lo) : kend_global;
+    const int32_t ke_hi = fitting ? hi : kend_global;
+    const bool hit = inr && ok && k >= ke_lo && k <= ke_hi && val >= qn;
+    Pf = inr ? val : -1;
+    used += (uint64_t)width;
+    const uint64_t mask = __ballot(hit);
+    if (mask) {
+      found = true;
+      end_k = kb + (int32_t)__builtin_ctzll(mask);
+      break;
+    }
+  }
+  flush_toff(S, toffr, status == ST_OK && !exhausted ? step : step - 1, lane);
+  if (!found && !exhausted && status == ST_OK && step > 0) {
+    const uint32_t W = S.wmask + 1;
+    S.ring[(size_t)(step & 1) * W + ((uint32_t)k & S.wmask)] = (int16_t)Pf;
+  }
+  wave_sync<true>();
+  st = StepState{step, used, found, end_k, 0, status};
+}
 
 // --- affine --------------------------------------------------------------------
 
@@ -281,7 +490,20 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
   int32_t end_k = 0, end_layer = 0;
   bool found = fitting ? (m0 == qn) : (qn == tn && m0 == qn);  // s = 0 endpoint on diagonal 0
   const int32_t kend_g = qn - tn;
-  while (!found) {
+  if constexpr (LDS && NT == 64) {
+    StepState st;
+    if (!found && affine_regs_any<B8>(S, qn, tn, m0, fitting, max_step, xn, oen, en, lane, st)) {
+      step = __builtin_amdgcn_readfirstlane(st.step);
+      used = st.used;
+      found = st.found;
+      end_k = st.end_k;
+      end_layer = st.end_layer;
+      status = st.status;
+      slot = __builtin_amdgcn_readfirstlane(step % Rm);
+      gslot = __builtin_amdgcn_readfirstlane(step % Rg);
+    }
+  }
+  while (!found && status == ST_OK) {
     step = __builtin_amdgcn_readfirstlane(step + 1);
     if (step > max_step) { status = ST_ALIGNMENT_FAILED; break; }
     slot = __builtin_amdgcn_readfirstlane(slot + 1 == Rm ? 0 : slot + 1);
@@ -337,8 +559,7 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
       uint32_t mop = 0;
       if (ins > best) { best = ins; mop = 1; }
       if (del > best) { best = del; mop = 2; }
-      int32_t mval = best;
-      if (best >= 0) mval = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, best, k);
+      const int32_t mval = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, best, k);
       int32_t layer = -1;
       if (k <= hi) {
         Wm[ik] = (typename S_t::ring_v)mval;
@@ -457,7 +678,18 @@ __device__ void wave_linear(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
   bool hist_on = true;
   bool exhausted = false;
   bool found = fitting ? (w0 >= qn) : (kend_global == 0 && w0 == qn);
-  while (!found) {
+  if constexpr (LDS && NT == 64) {
+    if (!found) {
+      StepState st;
+      linear_regs<B8>(S, qn, tn, w0, fitting, explore, lane, st, exhausted);
+      step = __builtin_amdgcn_readfirstlane(st.step);
+      used = st.used;
+      found = st.found;
+      end_k = st.end_k;
+      status = st.status;
+    }
+  }
+  while (!found && !exhausted && status == ST_OK) {
     step = __builtin_amdgcn_readfirstlane(step + 1);
     if (step > explore) { exhausted = true; break; }
     const int32_t slot = step & 1, ps = slot ^ 1;
@@ -495,8 +727,7 @@ __device__ void wave_linear(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
       if (vl >= 0 && vl + 1 >= best) { best = vl + 1; op = 2; }
       if (vr >= 0 && vr >= best) { best = vr; op = 3; }
       const bool ok = op != 0 && best <= qn && best - k >= 0 && best - k <= tn;
-      int32_t val = -1;
-      if (ok) val = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, best, k);
+      const int32_t val = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, ok ? best : -1, k);
       bool hit = false;
       if (k <= hi) {
         Wn[ik] = (typename S_t::ring_v)val;
@@ -583,11 +814,17 @@ __global__ __launch_bounds__(64 * WV_WAVES) void wfa_wave_kernel(WfaArgs a, Wave
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int32_t)(threadIdx.x >> 6));
   uint8_t* base = LDS ? wave_lds_pool + (size_t)wv * G.per_wave_bytes : nullptr;
-  for (;;) {
-    uint32_t w = 0;
-    if (lane == 0) w = atomicAdd(&a.counters[0], 1u);
-    w = (uint32_t)readlane((int32_t)w, 0);
-    if (w >= a.nwork) break;
+  // Work items: the first one per wave is the wave's own index.  With
+  // static_order the wave then takes every nwaves-th item (the host orders
+  // the list so that these strides balance); else later items come from the
+  // queue head (offset by the number of waves), claimed when a pair starts so
+  // that the atomic's round trip overlaps the pair.  Claims on the one queue
+  // head serialise (about 12 ns each), which bounds a queue of short pairs.
+  const uint32_t nwaves = gridDim.x * WV_WAVES;
+  uint32_t w = blockIdx.x * WV_WAVES + (uint32_t)wv;
+  while (w < a.nwork) {
+    uint32_t nx = w;
+    if (!a.static_order && lane == 0) nx = atomicAdd(&a.counters[0], 1u);
     const uint32_t pi = a.work_list[w];
     const PairDesc pd = a.pairs[pi];
     PairOut* po = a.out + pi;
@@ -599,6 +836,7 @@ __global__ __launch_bounds__(64 * WV_WAVES) void wfa_wave_kernel(WfaArgs a, Wave
       else wave_linear<LDS, false, 64>(a, G, base, pd, po, lane, nullptr);
     }
     wave_sync<LDS>();  // the wave's storage is reused by its next pair
+    w = (a.static_order ? w : (uint32_t)readlane((int32_t)nx, 0)) + nwaves;
   }
 }
 
@@ -610,17 +848,20 @@ __global__ __launch_bounds__(NT) void wfa_block_kernel(WfaArgs a, WaveGeom G) {
   extern __shared__ __align__(16) uint8_t block_lds[];
   uint8_t* base = RL ? block_lds : nullptr;
   const int tid = threadIdx.x;
-  for (;;) {
-    if (tid == 0) sh.work = atomicAdd(&a.counters[0], 1u);
-    __syncthreads();
-    // The pair index is taken into a scalar register so that the loop exit is
-    // a scalar branch: every wave leaves together.  With a per-lane exit the
-    // compiler may give thread 0 (the only one with work before the next
-    // barrier) a loop of its own, and the other lanes would spin on the old
-    // index.  For the same reason no thread-0-only code follows the barrier at
-    // the end of the body.
-    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)sh.work);
-    if (w >= a.nwork) break;
+  // First item: the workgroup's index; later ones from the queue head
+  // (offset by the grid size), claimed when a pair starts and read after the
+  // pair's closing barrier.  The slot alternates so that the claim for item
+  // i + 1 never overwrites the value some wave is still reading for item i.
+  // The index is kept in a scalar register so that the loop exit is a scalar
+  // branch: every wave leaves together.  With a per-lane exit the compiler
+  // may give thread 0 (the only one with work before the next barrier) a loop
+  // of its own, and the other lanes would spin on the old index.  For the
+  // same reason no thread-0-only code follows the barrier at the end of the
+  // body.
+  uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)blockIdx.x);
+  uint32_t par = 0;
+  while (w < a.nwork) {
+    if (tid == 0) sh.next[par] = atomicAdd(&a.counters[0], 1u);
     const uint32_t pi = a.work_list[w];
     const PairDesc pd = a.pairs[pi];
     PairOut* po = a.out + pi;
@@ -631,7 +872,9 @@ __global__ __launch_bounds__(NT) void wfa_block_kernel(WfaArgs a, WaveGeom G) {
       if (pd.flags & PF_BYTES) wave_linear<false, true, NT, RL>(a, G, base, pd, po, tid, &sh);
       else wave_linear<false, false, NT, RL>(a, G, base, pd, po, tid, &sh);
     }
-    __syncthreads();  // the slot is reused by the next pair
+    __syncthreads();  // the slot is reused by the next pair; sh.next[par] is set
+    w = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)sh.next[par]) + gridDim.x;
+    par ^= 1u;
   }
 }
 
