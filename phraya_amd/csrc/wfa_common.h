@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "phx_device.h"
 
 namespace phx {
@@ -20,33 +22,49 @@ using lmem_t = __attribute__((address_space(3))) T;
 
 // --- 2-bit / 8-bit match extension -------------------------------------------
 
-// The same packed words seen as 32-bit halves, in the same address space.
-__device__ __forceinline__ const lmem_t<const uint32_t>* as_dwords(const lmem_t<const uint64_t>* w) {
-  return (const lmem_t<const uint32_t>*)w;
-}
-__device__ __forceinline__ const gmem_t<const uint32_t>* as_dwords(const gmem_t<const uint64_t>* w) {
-  return (const gmem_t<const uint32_t>*)w;
-}
+// Whether a word pointer addresses LDS (address space 3).
+template <class WP>
+struct in_lds : std::false_type {};
+template <>
+struct in_lds<const lmem_t<const uint64_t>*> : std::true_type {};
 
 // 64 bits of a packed symbol stream starting at bit `bit` (symbol 0 in the
-// lowest bits of dword 0): two funnel shifts over three dwords.  Reads dwords
-// bit / 32 .. bit / 32 + 2, inside the stream's words (each packed sequence
-// carries one word past its last symbol).
-template <class DP>
-__device__ __forceinline__ uint64_t bit_window(DP d, uint32_t bit) {
-  const uint32_t i = bit >> 5;
-  const uint32_t d0 = d[i], d1 = d[i + 1], d2 = d[i + 2];
-  const uint32_t lo = __builtin_amdgcn_alignbit(d1, d0, bit);  // shift = bit & 31
-  const uint32_t hi = __builtin_amdgcn_alignbit(d2, d1, bit);
+// lowest bits of word 0), as two 32-bit funnel shifts over three halves.
+// LDS: the three 32-bit halves at bit / 32 .. + 2 (two read instructions,
+// no selects).  Global memory: the two aligned words bit / 64 and + 1 (one
+// paired load each, fewer memory instructions) and a select of the three
+// halves that matter.  Either way the reads stay inside the stream's words
+// (each packed sequence carries one word past its last symbol).
+template <class WP>
+__device__ __forceinline__ uint64_t bit_window(WP w, uint32_t bit) {
+  uint32_t x0, x1, x2;
+  if constexpr (in_lds<WP>::value) {
+    const lmem_t<const uint32_t>* d = (const lmem_t<const uint32_t>*)w;
+    const uint32_t i = bit >> 5;
+    x0 = d[i];
+    x1 = d[i + 1];
+    x2 = d[i + 2];
+  } else {
+    const uint32_t wi = bit >> 6;
+    const uint64_t w0 = w[wi], w1 = w[wi + 1];
+    const bool upper = (bit & 32u) != 0;
+    x0 = upper ? (uint32_t)(w0 >> 32) : (uint32_t)w0;
+    x1 = upper ? (uint32_t)w1 : (uint32_t)(w0 >> 32);
+    x2 = upper ? (uint32_t)(w1 >> 32) : (uint32_t)w1;
+  }
+  const uint32_t lo = __builtin_amdgcn_alignbit(x1, x0, bit);  // shift = bit & 31
+  const uint32_t hi = __builtin_amdgcn_alignbit(x2, x1, bit);
   return ((uint64_t)hi << 32) | lo;
 }
 
 // Upstream `extend` closure (wfa_simd.rs:252-262, :608-616): unchanged outside
 // the matrix, else position + longest common prefix of the two suffixes.
 // 32 bases (or 8 raw bytes) are compared per 64-bit XOR; the first differing
-// symbol is the lowest set bit.  The loop is uniform over the wave: a lane
-// that has found its mismatch (or was never inside the matrix) keeps its
-// position and count while the others go on, so no per-lane branches.
+// symbol is the lowest set bit.  The loop runs while any lane of the wave
+// is still extending.  In LDS every lane executes the body (a lane that has
+// found its mismatch, or was never inside the matrix, keeps its count and
+// re-reads a valid window), which saves the per-lane branch; in global
+// memory finished lanes are masked off so they issue no loads.
 // qsh / tsh: index of symbol 0 inside word 0 of each sequence.
 template <bool B8, class WP>
 __device__ __forceinline__ int32_t extend_diag(WP qw, uint32_t qsh, WP tw, uint32_t tsh,
@@ -59,15 +77,15 @@ __device__ __forceinline__ int32_t extend_diag(WP qw, uint32_t qsh, WP tw, uint3
   const int32_t lim = inside ? min(qn - p, tn - j) : 0;
   const uint32_t qb = (qsh + (uint32_t)(inside ? p : 0)) * BITS;
   const uint32_t tb = (tsh + (uint32_t)(inside ? j : 0)) * BITS;
-  const auto qd = as_dwords(qw);
-  const auto td = as_dwords(tw);
   int32_t n = 0;
   while (__ballot(act)) {
-    const uint64_t d = bit_window(qd, qb + (uint32_t)n * BITS) ^ bit_window(td, tb + (uint32_t)n * BITS);
-    // Equal symbols before the first difference; a whole window when none.
-    const int32_t z = (int32_t)((uint32_t)__builtin_ctzg(d, 64) >> CTZ_SHIFT);
-    n = act ? n + z : n;
-    act = act && d == 0 && n < lim;
+    if (in_lds<WP>::value || act) {
+      const uint64_t d = bit_window(qw, qb + (uint32_t)n * BITS) ^ bit_window(tw, tb + (uint32_t)n * BITS);
+      // Equal symbols before the first difference; a whole window when none.
+      const int32_t z = (int32_t)((uint32_t)__builtin_ctzg(d, 64) >> CTZ_SHIFT);
+      n = act ? n + z : n;
+      act = act && d == 0 && n < lim;
+    }
   }
   return inside ? p + min(n, lim) : p;
 }

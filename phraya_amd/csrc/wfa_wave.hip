@@ -816,15 +816,14 @@ __global__ __launch_bounds__(64 * WV_WAVES) void wfa_wave_kernel(WfaArgs a, Wave
   uint8_t* base = LDS ? wave_lds_pool + (size_t)wv * G.per_wave_bytes : nullptr;
   // Work items: the first one per wave is the wave's own index.  With
   // static_order the wave then takes every nwaves-th item (the host orders
-  // the list so that these strides balance); else later items come from the
-  // queue head (offset by the number of waves), claimed when a pair starts so
-  // that the atomic's round trip overlaps the pair.  Claims on the one queue
-  // head serialise (about 12 ns each), which bounds a queue of short pairs.
+  // the list so that these strides balance); else the next item comes from
+  // the queue head (offset by the number of waves) once the current one is
+  // done, so no wave holds an item while another wave idles.  Claims on the
+  // one queue head serialise (about 12 ns each), which would bound a queue
+  // of short pairs; those use static_order.
   const uint32_t nwaves = gridDim.x * WV_WAVES;
   uint32_t w = blockIdx.x * WV_WAVES + (uint32_t)wv;
   while (w < a.nwork) {
-    uint32_t nx = w;
-    if (!a.static_order && lane == 0) nx = atomicAdd(&a.counters[0], 1u);
     const uint32_t pi = a.work_list[w];
     const PairDesc pd = a.pairs[pi];
     PairOut* po = a.out + pi;
@@ -836,7 +835,13 @@ __global__ __launch_bounds__(64 * WV_WAVES) void wfa_wave_kernel(WfaArgs a, Wave
       else wave_linear<LDS, false, 64>(a, G, base, pd, po, lane, nullptr);
     }
     wave_sync<LDS>();  // the wave's storage is reused by its next pair
-    w = (a.static_order ? w : (uint32_t)readlane((int32_t)nx, 0)) + nwaves;
+    if (a.static_order) {
+      w += nwaves;
+    } else {
+      uint32_t nx = 0;
+      if (lane == 0) nx = atomicAdd(&a.counters[0], 1u);
+      w = (uint32_t)readlane((int32_t)nx, 0) + nwaves;
+    }
   }
 }
 
@@ -849,9 +854,10 @@ __global__ __launch_bounds__(NT) void wfa_block_kernel(WfaArgs a, WaveGeom G) {
   uint8_t* base = RL ? block_lds : nullptr;
   const int tid = threadIdx.x;
   // First item: the workgroup's index; later ones from the queue head
-  // (offset by the grid size), claimed when a pair starts and read after the
-  // pair's closing barrier.  The slot alternates so that the claim for item
-  // i + 1 never overwrites the value some wave is still reading for item i.
+  // (offset by the grid size), claimed by thread 0 when a pair is done and
+  // read by all after the closing barrier.  The slot alternates so that a
+  // claim never overwrites the value some wave is still reading for the
+  // previous item (a pair with qn = 0 returns without a barrier).
   // The index is kept in a scalar register so that the loop exit is a scalar
   // branch: every wave leaves together.  With a per-lane exit the compiler
   // may give thread 0 (the only one with work before the next barrier) a loop
@@ -861,7 +867,6 @@ __global__ __launch_bounds__(NT) void wfa_block_kernel(WfaArgs a, WaveGeom G) {
   uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)blockIdx.x);
   uint32_t par = 0;
   while (w < a.nwork) {
-    if (tid == 0) sh.next[par] = atomicAdd(&a.counters[0], 1u);
     const uint32_t pi = a.work_list[w];
     const PairDesc pd = a.pairs[pi];
     PairOut* po = a.out + pi;
@@ -872,6 +877,7 @@ __global__ __launch_bounds__(NT) void wfa_block_kernel(WfaArgs a, WaveGeom G) {
       if (pd.flags & PF_BYTES) wave_linear<false, true, NT, RL>(a, G, base, pd, po, tid, &sh);
       else wave_linear<false, false, NT, RL>(a, G, base, pd, po, tid, &sh);
     }
+    if (tid == 0) sh.next[par] = atomicAdd(&a.counters[0], 1u);
     __syncthreads();  // the slot is reused by the next pair; sh.next[par] is set
     w = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)sh.next[par]) + gridDim.x;
     par ^= 1u;
