@@ -321,6 +321,63 @@ int64_t phx_estimate_request(int32_t engine, phx_costs costs, const uint8_t* q, 
 /* Library version string. */
 const char* phx_version(void);
 
+/* ---------------------------------------------------------------------------
+ * Batched read extension: phraya-align's extend_chains (executor.rs:528-567)
+ * for many reads at once.  Every anchor of every read is one pair of one
+ * device batch per engine; the host then picks each read's primary and
+ * alternatives with score_alignments' rules (lib.rs:170-199).
+ * ------------------------------------------------------------------------- */
+
+/* Strategy and gap model of phraya's executor (executor.rs:330-336, :379-404). */
+enum { PHX_STRATEGY_FAST = 0, PHX_STRATEGY_BALANCED = 1, PHX_STRATEGY_SENSITIVE = 2 };
+enum { PHX_GAP_AFFINE = 0, PHX_GAP_LINEAR = 1 };
+
+/* One read's extension input: its bases and the target_start of each chain
+ * of chaining's output, in rank order (Chain::target_start). */
+typedef struct phx_read_chains {
+  const uint8_t* query;
+  uint64_t query_len;
+  const uint64_t* chain_target_start;
+  uint32_t n_chains;
+} phx_read_chains;
+
+/* One read's extension output.  Anchors [first_anchor, first_anchor +
+ * n_anchors) of the anchor arrays belong to it, in anchor order. */
+typedef struct phx_read_extension {
+  int32_t status;          /* PHX_OK: some anchor extended; PHX_ALIGNMENT_FAILED: none did
+                              (upstream: extend_chains returns None) */
+  int32_t primary;         /* anchor index (relative to first_anchor) of the primary, or -1 */
+  uint32_t n_anchors;
+  uint32_t n_alternatives; /* anchors with role 3 */
+  uint64_t first_anchor;
+} phx_read_extension;
+
+/* chain_cap (executor.rs:330-336): anchors extended per read. */
+uint32_t phx_chain_cap(int32_t strategy);
+/* Anchors of a batch and an upper bound of their CIGAR runs (host only). */
+int phx_extend_chains_bounds(int32_t strategy, uint64_t target_len, const phx_read_chains* reads,
+                             uint32_t n_reads, uint64_t* anchors, uint64_t* runs);
+/* Extends every read's anchors against target (anchors_from_chains,
+ * executor.rs:347-362; window target[..min(target_pos + 2 qlen, |target|)],
+ * :551-553; engine by extend_anchor, :379-404) and scores them.  Per anchor:
+ * anchor_aln (status per pair as in phx_align_batch; positions from the
+ * window start), anchor_role (0 failed, 1 extended, 2 primary, 3
+ * alternative) and its runs at run_off (anchors + 1 entries).  Capacities
+ * below phx_extend_chains_bounds give PHX_E_CAPACITY before any work. */
+int phx_extend_chains_batch(phx_ctx* ctx, int32_t strategy, int32_t gap_model,
+                            const uint8_t* target, uint64_t target_len,
+                            const phx_read_chains* reads, uint32_t n_reads,
+                            phx_read_extension* out, phx_alignment* anchor_aln,
+                            uint8_t* anchor_role, uint64_t anchors_cap, uint32_t* runs,
+                            uint64_t runs_cap, uint64_t* run_off);
+/* score_alignments (lib.rs:170-199) over n >= 1 successful alignments with
+ * their runs: *primary = the first with the smallest edit distance;
+ * is_alternative[i] = 1 for every alignment not equal to the primary (all
+ * fields and the CIGAR) whose (1 - ed/query_len) / (1 - ed_primary/query_len)
+ * is >= 0.95.  Host only.  n = 0 (upstream panics) is PHX_E_ARG. */
+int phx_score_alignments(const phx_alignment* aln, const uint32_t* runs, const uint64_t* run_off,
+                         uint32_t n, uint64_t query_len, int32_t* primary, uint8_t* is_alternative);
+
 #ifdef __cplusplus
 }
 #endif

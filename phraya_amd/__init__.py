@@ -639,3 +639,115 @@ class ContigPlan:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------- read extension
+
+STRATEGY_FAST, STRATEGY_BALANCED, STRATEGY_SENSITIVE = 0, 1, 2
+GAP_AFFINE, GAP_LINEAR = 0, 1
+
+
+@dataclass
+class ScoredAlignments:
+    """ScoredAlignments (phraya-align/src/lib.rs:53-56)."""
+    primary: Alignment
+    alternatives: list
+
+
+def _parse_runs(cigar: str) -> np.ndarray:
+    import re
+    ops = {"M": 0, "X": 1, "I": 2, "D": 3}
+    return np.array([(int(n) << 4) | ops[op] for n, op in re.findall(r"(\d+)([MXID])", cigar)],
+                    dtype=np.uint32)
+
+
+def score_alignments(alignments: Sequence[Alignment], query_len: int) -> ScoredAlignments:
+    """score_alignments (lib.rs:170-199) through phx_score_alignments (host only)."""
+    if not alignments:
+        raise ValueError("At least one alignment required")
+    n = len(alignments)
+    aln = np.zeros(n, dtype=L.ALN_DTYPE)
+    runs = [_parse_runs(a.cigar) for a in alignments]
+    off = np.zeros(n + 1, np.uint64)
+    off[1:] = np.cumsum([len(r) for r in runs])
+    flat = np.concatenate(runs + [np.zeros(1, np.uint32)]).astype(np.uint32)
+    for i, a in enumerate(alignments):
+        aln[i] = (0, 0, a.score, a.edit_distance, a.query_start, a.query_end, a.target_start,
+                  a.target_end)
+    primary = ctypes.c_int32(-1)
+    alt = np.zeros(n, np.uint8)
+    rc = L.lib().phx_score_alignments(aln.ctypes.data, flat.ctypes.data, off.ctypes.data, n,
+                                      int(query_len), ctypes.byref(primary), alt.ctypes.data)
+    if rc != L.PHX_OK:
+        raise RuntimeError(f"phx_score_alignments: status {rc}")
+    p = primary.value
+    return ScoredAlignments(alignments[p], [alignments[i] for i in range(n) if alt[i]])
+
+
+@dataclass
+class ReadExtension:
+    """One read's batched extend_chains result: `scored` is None when no anchor
+    extended (upstream returns None); `anchors` holds (role, Alignment or
+    status) per anchor in anchor order (roles: 0 failed, 1 extended, 2
+    primary, 3 alternative)."""
+    scored: Optional[ScoredAlignments]
+    anchors: list
+
+
+def chain_cap(strategy: int) -> int:
+    """chain_cap (executor.rs:330-336)."""
+    return int(L.lib().phx_chain_cap(strategy))
+
+
+def extend_chains_batch(strategy: int, gap_model: int, target: bytes,
+                        reads: Sequence[tuple], ctx: Optional[Context] = None) -> list:
+    """extend_chains (executor.rs:528-567) for many reads at once
+    (phx_extend_chains_batch).  reads: (query bytes, chain target_starts in
+    rank order) per read."""
+    ctx = ctx or default_context()
+    n = len(reads)
+    t = np.frombuffer(bytes(target), np.uint8)
+    keep, rc_arr = [], (L.CReadChains * max(n, 1))()
+    for i, (q, starts) in enumerate(reads):
+        qa = np.frombuffer(bytes(q), np.uint8)
+        sa = np.asarray(list(starts), dtype=np.uint64)
+        keep += [qa, sa]
+        rc_arr[i] = L.CReadChains(qa.ctypes.data if len(qa) else None, len(qa),
+                                  sa.ctypes.data if len(sa) else None, len(sa))
+    na, nr = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    rc = L.lib().phx_extend_chains_bounds(strategy, len(t), rc_arr, n, ctypes.byref(na),
+                                          ctypes.byref(nr))
+    if rc != L.PHX_OK:
+        raise RuntimeError(f"phx_extend_chains_bounds: status {rc}")
+    A, R = na.value, max(nr.value, 1)
+    out = (L.CReadExtension * max(n, 1))()
+    aln = np.zeros(max(A, 1), dtype=L.ALN_DTYPE)
+    role = np.zeros(max(A, 1), np.uint8)
+    runs = np.zeros(R, np.uint32)
+    off = np.zeros(A + 1, np.uint64)
+    rc = L.lib().phx_extend_chains_batch(ctx.handle, strategy, gap_model,
+                                         t.ctypes.data if len(t) else None, len(t), rc_arr, n,
+                                         out, aln.ctypes.data, role.ctypes.data, A,
+                                         runs.ctypes.data, R, off.ctypes.data)
+    if rc != L.PHX_OK:
+        raise RuntimeError(f"phx_extend_chains_batch failed ({rc}): {ctx.last_error()}")
+    res = []
+    for i in range(n):
+        o = out[i]
+        anchors = []
+        for a in range(o.first_anchor, o.first_anchor + o.n_anchors):
+            r = aln[a]
+            if int(r["status"]) != L.PHX_OK:
+                anchors.append((0, int(r["status"])))
+                continue
+            anchors.append((int(role[a]), Alignment(
+                format_cigar(runs[int(off[a]):int(off[a + 1])]), int(r["edit_distance"]),
+                int(r["query_start"]), int(r["query_end"]), int(r["target_start"]),
+                int(r["target_end"]), int(r["score"]))))
+        scored = None
+        if o.status == L.PHX_OK:
+            prim = anchors[o.primary][1]
+            alts = [al for rl, al in anchors if rl == 3]
+            scored = ScoredAlignments(prim, alts)
+        res.append(ReadExtension(scored, anchors))
+    return res

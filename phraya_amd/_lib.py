@@ -40,7 +40,8 @@ EXPORTS = (
     "phx_contig_plan_run", "phx_contig_plan_results", "phx_contig_plan_gaps",
     "phx_contig_plan_layout", "phx_contig_plan_gap_plan", "phx_contig_plan_last_ms",
     "phx_contig_plan_last_gpu_ms", "phx_contig_plan_last_stitch_ms", "phx_contig_plan_destroy",
-    "phx_version",
+    "phx_version", "phx_chain_cap", "phx_extend_chains_bounds", "phx_extend_chains_batch",
+    "phx_score_alignments",
 )
 
 
@@ -74,6 +75,17 @@ class CBatch(ctypes.Structure):
 class CResults(ctypes.Structure):
     _fields_ = [("aln", ctypes.c_void_p), ("runs", ctypes.c_void_p),
                 ("runs_cap", ctypes.c_uint64), ("run_off", ctypes.c_void_p)]
+
+
+class CReadChains(ctypes.Structure):
+    _fields_ = [("query", ctypes.c_void_p), ("query_len", ctypes.c_uint64),
+                ("chain_target_start", ctypes.c_void_p), ("n_chains", ctypes.c_uint32)]
+
+
+class CReadExtension(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("primary", ctypes.c_int32),
+                ("n_anchors", ctypes.c_uint32), ("n_alternatives", ctypes.c_uint32),
+                ("first_anchor", ctypes.c_uint64)]
 
 
 class CTuning(ctypes.Structure):
@@ -153,6 +165,13 @@ def lib() -> ctypes.CDLL:
     L.phx_plan_get_tuning.argtypes = [P, P]
     L.phx_plan_autotune.argtypes = [P, ctypes.c_uint32, P, P]
     L.phx_plan_use_graph.argtypes = [P, ctypes.c_int]
+    L.phx_chain_cap.argtypes = [ctypes.c_int32]
+    L.phx_chain_cap.restype = ctypes.c_uint32
+    L.phx_extend_chains_bounds.argtypes = [ctypes.c_int32, ctypes.c_uint64, P, ctypes.c_uint32, P, P]
+    L.phx_extend_chains_batch.argtypes = [P, ctypes.c_int32, ctypes.c_int32, P, ctypes.c_uint64, P,
+                                          ctypes.c_uint32, P, P, P, ctypes.c_uint64, P,
+                                          ctypes.c_uint64, P]
+    L.phx_score_alignments.argtypes = [P, P, P, ctypes.c_uint32, ctypes.c_uint64, P, P]
     L.phx_format_cigar.argtypes = [P, ctypes.c_uint64, P, ctypes.c_size_t]
     L.phx_format_cigar.restype = ctypes.c_size_t
     sz, u64 = ctypes.c_size_t, ctypes.c_uint64
