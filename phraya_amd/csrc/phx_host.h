@@ -248,7 +248,10 @@ void parallel_for(size_t n, F f) {
 // Ring words of one HBM slot (`frames` from ring_frames) for pairs with
 // qn + tn <= max_sum.
 inline uint64_t ring_words_for(int32_t frames, uint64_t max_sum, uint32_t* wcap_out) {
-  const uint32_t wcap = next_pow2(max_sum + 1);
+  // Every diagonal of the pair plus -1 margins of up to `frames` entries on
+  // both sides and one clamped read past the top: the kernels index an HBM
+  // ring without wrap-around.
+  const uint32_t wcap = next_pow2(max_sum + 1 + 2 * (uint64_t)frames + 2);
   if (wcap_out) *wcap_out = wcap;
   return (uint64_t)frames * wcap;
 }

@@ -445,6 +445,29 @@ __device__ void linear_regs(const WaveStore<true>& S, int32_t qn, int32_t tn, in
   st = StepState{step, used, found, end_k, 0, status};
 }
 
+// --- HBM ring without wrap-around ----------------------------------------------
+//
+// An HBM ring (wave and workgroup tiers whose ring is not in LDS) is wider
+// than any pair it holds: W >= qn + tn + 2 * frames + 3 (host sizing), so
+// diagonal k sits at index k + tn + M with no wrap, where M is at least every
+// lookback + 1.  Each frame written at step f also gets -1 in the M entries
+// beyond both ends of its range [lo(f), hi(f)]; a later step reads frame f at
+// most lookback + 1 diagonals beyond that range, so out-of-range reads come
+// out -1 by themselves and the per-cell range checks of the wrapped rings are
+// not needed.  At pair start the neighbourhood of diagonal 0 of every frame
+// is set to -1 (the frames of steps before 0 and step 0's margins).
+template <int NT>
+__device__ __forceinline__ void ring_fill_origin(int32_t* ring, uint32_t W, int32_t nframes,
+                                                 int32_t span, int32_t koff, int32_t qn, int32_t tn,
+                                                 int32_t M, int lane) {
+  const int32_t klo = max(-span, -tn - M), khi = min(span, qn + M);
+  const int32_t n = khi - klo + 1;
+  for (int32_t t = lane; t < n * nframes; t += NT) {
+    const int32_t fr = t / n, k = klo + t % n;
+    ring[(size_t)fr * W + (uint32_t)(k + koff)] = -1;
+  }
+}
+
 // --- affine --------------------------------------------------------------------
 
 template <bool LDS, bool B8, int NT, bool RL = false>
@@ -473,13 +496,23 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
   typename S_t::ring_t* Mr = S.ring;
   typename S_t::ring_t* Ri = S.ring + (size_t)Rm * W;
   typename S_t::ring_t* Rd = Ri + (size_t)Rg * W;
+  // HBM ring: unwrapped with -1 margins of M = Rm entries (ring_fill_origin).
+  constexpr bool UNW = !LDS && !RL;
+  const int32_t M = Rm, koff = tn + Rm;
+  auto rix = [&](int32_t k) -> uint32_t {
+    return UNW ? (uint32_t)(k + koff) : ((uint32_t)k & S.wmask);
+  };
+  if constexpr (UNW) {
+    ring_fill_origin<NT>((int32_t*)S.ring, W, Rm + 2 * Rg, M + Rm + 1, koff, qn, tn, M, lane);
+    pair_sync<LDS, NT, RL>();
+  }
 
   const int32_t m0 = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, 0, 0);
   if (lane == 0) {
     // Step 0 holds only m on diagonal 0; its ins/del entries are unset.
-    Mr[0] = (typename S_t::ring_v)m0;
-    Ri[0] = -1;
-    Rd[0] = -1;
+    Mr[rix(0)] = (typename S_t::ring_v)m0;
+    Ri[rix(0)] = -1;
+    Rd[rix(0)] = -1;
     if (NT > 64) sh->key[0] = sh->key[1] = sh->key[2] = INT32_MAX;
   }
   pair_sync<LDS, NT, RL>();
@@ -535,17 +568,27 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
     typename S_t::ring_t* Wi = Ri + (size_t)gslot * W;
     typename S_t::ring_t* Wd = Rd + (size_t)gslot * W;
     typename S_t::cell_t* cells = S.cells + used;
+    if constexpr (UNW) {  // this step's frames: -1 margins beyond [lo, hi]
+      for (int32_t t = lane; t < 2 * M; t += NT) {
+        const uint32_t ix = rix(t < M ? lo - M + t : hi + 1 + (t - M));
+        Wm[ix] = -1;
+        Wi[ix] = -1;
+        Wd[ix] = -1;
+      }
+    }
     const int32_t nch = (width + NT - 1) / NT;
     for (int32_t c = 0; c < nch; ++c) {
       const int32_t k = lo + c * NT + lane;
-      const uint32_t ik = (uint32_t)k & S.wmask, il = (uint32_t)(k - 1) & S.wmask,
-                     ir = (uint32_t)(k + 1) & S.wmask;
+      // Lanes past hi compute nothing that is kept; their reads stay in the
+      // frame (k clamped to hi + 1).
+      const int32_t kr = UNW ? min(k, hi + 1) : k;
+      const uint32_t ik = rix(kr), il = rix(kr - 1), ir = rix(kr + 1);
       const int32_t rvx = Rmx[ik], rvol = Rmo[il], rvor = Rmo[ir], rvil = Rie[il], rvdr = Rde[ir];
-      const int32_t vx = (k >= xlo && k <= xhi) ? rvx : -1;
-      const int32_t vol = (k - 1 >= olo && k - 1 <= ohi) ? rvol : -1;
-      const int32_t vor = (k + 1 >= olo && k + 1 <= ohi) ? rvor : -1;
-      const int32_t vil = (k - 1 >= elo && k - 1 <= ehi) ? rvil : -1;
-      const int32_t vdr = (k + 1 >= elo && k + 1 <= ehi) ? rvdr : -1;
+      const int32_t vx = (UNW || (k >= xlo && k <= xhi)) ? rvx : -1;
+      const int32_t vol = (UNW || (k - 1 >= olo && k - 1 <= ohi)) ? rvol : -1;
+      const int32_t vor = (UNW || (k + 1 >= olo && k + 1 <= ohi)) ? rvor : -1;
+      const int32_t vil = (UNW || (k - 1 >= elo && k - 1 <= ehi)) ? rvil : -1;
+      const int32_t vdr = (UNW || (k + 1 >= elo && k + 1 <= ehi)) ? rvdr : -1;
       // Stored offsets are valid on their own diagonal, so a candidate from
       // k-1 (+1 query) needs offset <= qn only, one from k+1 needs target <= tn
       // only, and the mismatch candidate needs both (affine_pos_valid).
@@ -666,10 +709,20 @@ __device__ void wave_linear(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
   const int32_t kend_global = qn - tn;
   const uint32_t W = S.wmask + 1;
   typename S_t::ring_t* ring = S.ring;  // two slots
+  // HBM ring: unwrapped with -1 margins of M = 2 entries (ring_fill_origin).
+  constexpr bool UNW = !LDS && !RL;
+  const int32_t M = 2, koff = tn + 2;
+  auto rix = [&](int32_t k) -> uint32_t {
+    return UNW ? (uint32_t)(k + koff) : ((uint32_t)k & S.wmask);
+  };
+  if constexpr (UNW) {
+    ring_fill_origin<NT>((int32_t*)S.ring, W, 2, M + 3, koff, qn, tn, M, lane);
+    pair_sync<LDS, NT, RL>();
+  }
 
   const int32_t w0 = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, 0, 0);
   if (lane == 0) {
-    ring[0] = (typename S_t::ring_v)w0;
+    ring[rix(0)] = (typename S_t::ring_v)w0;
     if (NT > 64) sh->key[0] = sh->key[1] = sh->key[2] = INT32_MAX;
   }
   pair_sync<LDS, NT, RL>();
@@ -710,15 +763,20 @@ __device__ void wave_linear(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
     typename S_t::cell_t* cells = S.cells + used;
     const int32_t ke_lo = fitting ? max(qn - tn, lo) : kend_global;
     const int32_t ke_hi = fitting ? hi : kend_global;
+    if constexpr (UNW) {  // this step's frame: -1 margins beyond [lo, hi]
+      for (int32_t t = lane; t < 2 * M; t += NT) Wn[rix(t < M ? lo - M + t : hi + 1 + (t - M))] = -1;
+    }
     const int32_t nch = (width + NT - 1) / NT;
     for (int32_t c = 0; c < nch; ++c) {
       const int32_t k = lo + c * NT + lane;
-      const uint32_t ik = (uint32_t)k & S.wmask, il = (uint32_t)(k - 1) & S.wmask,
-                     ir = (uint32_t)(k + 1) & S.wmask;
+      const int32_t kr = UNW ? min(k, hi + 1) : k;  // lanes past hi: reads stay in the frame
+      const uint32_t ik = rix(kr), il = rix(kr - 1), ir = rix(kr + 1);
       const int32_t rvm = P[ik], rvl = P[il], rvr = P[ir];
-      const int32_t vm = (k >= p_lo && k <= p_hi) ? rvm : -1;
-      const int32_t vl = (k > -tn && k - 1 >= p_lo && k - 1 <= p_hi) ? rvl : -1;
-      const int32_t vr = (k < qn && k + 1 >= p_lo && k + 1 <= p_hi) ? rvr : -1;
+      // k - 1 < -tn and k + 1 > qn fall in the margins, so with UNW the
+      // upstream bounds (k > -tn, k < qn) come out of the reads as well.
+      const int32_t vm = (UNW || (k >= p_lo && k <= p_hi)) ? rvm : -1;
+      const int32_t vl = (UNW || (k > -tn && k - 1 >= p_lo && k - 1 <= p_hi)) ? rvl : -1;
+      const int32_t vr = (UNW || (k < qn && k + 1 >= p_lo && k + 1 <= p_hi)) ? rvr : -1;
       // Candidates [mismatch, query-only, target-only]; the last maximum wins;
       // then the winner alone is bounds-checked (wfa_simd.rs:316-326).
       int32_t best = -1;
