@@ -110,6 +110,9 @@ typedef struct phx_tuning {
   uint32_t heavy_permille;   /* per mille of each length bucket (largest estimates) run one tier wider (0 = 50) */
   uint32_t block_lds_ring;   /* workgroup tiers: LDS ring width in diagonals for pairs whose wavefront fits
                                 (power of two; 0 = 1024; 1 = ring always in HBM) */
+  uint32_t block_flight;     /* workgroup tiers: pairs in flight per CU, as a multiple of 4 (256-thread)
+                                and 1 (1024-thread); 0 = 1, at most 4 */
+  uint32_t reserved;
 } phx_tuning;
 
 /* Context: owns the stream, device buffers and the submission lock.
@@ -145,10 +148,10 @@ uint32_t phx_plan_last_launches(const phx_plan* plan);
 int phx_plan_set_tuning(phx_plan* plan, const phx_tuning* tuning);
 int phx_plan_get_tuning(const phx_plan* plan, phx_tuning* tuning);
 /* Autotuner: times the plan under candidate tunings (tier length edges, i.e.
- * threads per pair, and heavy share; HBM wave-tier pairs in flight; LDS per
- * wave vs waves in flight of the LDS tier), stage by stage, keeps the
- * fastest and applies it; every trial runs the whole plan.  max_trials = 0:
- * no limit (at most 15 trials).
+ * threads per pair, and heavy share; workgroup-tier and HBM wave-tier pairs
+ * in flight; LDS per wave vs waves in flight of the LDS tier), stage by
+ * stage, keeps the fastest and applies it; every trial runs the whole plan.
+ * max_trials = 0: no limit (at most 16 trials).
  * *best / *best_ms (may be NULL) get the chosen tuning and its run time. */
 int phx_plan_autotune(phx_plan* plan, uint32_t max_trials, phx_tuning* best, double* best_ms);
 /* Graph mode (WFA engines; no counterpart in the reference, which has no

@@ -147,6 +147,7 @@ phx_tuning merge_tuning(phx_tuning d, const phx_tuning* t) {
   if (t->block1024_min_len) d.block1024_min_len = t->block1024_min_len;
   if (t->heavy_permille) d.heavy_permille = std::min<uint32_t>(t->heavy_permille, 1000);
   if (t->block_lds_ring) d.block_lds_ring = t->block_lds_ring == 1 ? 1 : next_pow2(t->block_lds_ring);
+  if (t->block_flight) d.block_flight = std::min<uint32_t>(t->block_flight, 4);
   return d;
 }
 bool has_cigar(int engine) { return engine != PHX_ENGINE_MYERS_DIST; }
@@ -172,7 +173,8 @@ GroupSlots group_slots(const phx_plan* P, int t, const std::vector<uint32_t>& wo
   g.per = g.hist + ((g.ring_words * 4 + 255) & ~(uint64_t)255);
   // Pairs in flight: one per wave (wave tier) or per workgroup (block tiers).
   const uint64_t max_flight = t == TIER_WAVE ? P->tun.hbm_slots
-                            : t == TIER_BLOCK256 ? (uint64_t)P->ctx->cu_count * 4 : (uint64_t)P->ctx->cu_count;
+                            : t == TIER_BLOCK256 ? (uint64_t)P->ctx->cu_count * 4 * P->tun.block_flight
+                                                 : (uint64_t)P->ctx->cu_count * P->tun.block_flight;
   g.n = std::max<uint64_t>(1, budget / g.per);
   g.n = std::min<uint64_t>(std::min<uint64_t>(g.n, max_flight), work.size());
   if (t == TIER_WAVE) g.n = (g.n + kWaveWaves - 1) / kWaveWaves * kWaveWaves;  // whole workgroups
@@ -553,7 +555,8 @@ void plan_group(phx_plan* P) {
     const uint64_t ring = P->group_rl[g] ? 0 : ring_words_for(ring_frames(P->engine, P->costs), max_sum, nullptr) * 4;
     const int t = P->group_tier[g];
     const uint64_t flight = t == TIER_WAVE ? P->tun.hbm_slots
-                          : t == TIER_BLOCK256 ? (uint64_t)ctx->cu_count * 4 : (uint64_t)ctx->cu_count;
+                          : t == TIER_BLOCK256 ? (uint64_t)ctx->cu_count * 4 * P->tun.block_flight
+                                               : (uint64_t)ctx->cu_count * P->tun.block_flight;
     ask[g] = (double)std::min<uint64_t>(flight, P->work[g].size()) * (double)(P->group_slot[g] + ring);
     ask_sum += ask[g];
   }
@@ -570,7 +573,8 @@ void plan_group(phx_plan* P) {
     const double threads = t == TIER_BLOCK1024 ? 1024 : t == TIER_BLOCK256 ? 256 : 64;
     const double flight = t == TIER_LDS ? (double)P->tun.wave_slots
                         : t == TIER_WAVE ? (double)P->tun.hbm_slots
-                        : t == TIER_BLOCK256 ? ctx->cu_count * 4.0 : (double)ctx->cu_count;
+                        : t == TIER_BLOCK256 ? ctx->cu_count * 4.0 * P->tun.block_flight
+                                             : (double)ctx->cu_count * P->tun.block_flight;
     double sum = 0, longest = 0;
     for (uint32_t i : P->work[g]) {
       const double st = t == TIER_LDS ? 32.0 : (double)P->est[i];
@@ -817,6 +821,7 @@ phx_ctx* phx_ctx_create(int device, uint32_t flags, const phx_tuning* tuning) {
   d.block1024_min_len = 16384;
   d.heavy_permille = 50;
   d.block_lds_ring = 1024;
+  d.block_flight = 1;
   c->tun = merge_tuning(d, tuning);
   return c;
 }
@@ -1075,7 +1080,14 @@ int phx_plan_autotune(phx_plan* P, uint32_t max_trials, phx_tuning* best_out, do
   }
   rc = phx_plan_set_tuning(P, &best);
   if (rc != PHX_OK) return rc;
-  // Stage 2: pairs in flight of the HBM wave tier.
+  // Stage 2: pairs in flight of the workgroup tiers (a second workgroup per
+  // CU hides memory latency when a group has pairs to spare), then of the
+  // HBM wave tier.
+  if (pairs_in(TIER_BLOCK256) + pairs_in(TIER_BLOCK1024) > 2 * (size_t)cu) {
+    phx_tuning t = best;
+    t.block_flight = 2;
+    if ((rc = consider(t)) != PHX_OK) return rc;
+  }
   if (pairs_in(TIER_WAVE))
     for (uint32_t k : {4u, 16u}) {
       phx_tuning t = best;

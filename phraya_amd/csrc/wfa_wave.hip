@@ -84,6 +84,36 @@ __device__ __forceinline__ int32_t step_hi(int32_t u, int32_t qn) { return min(u
 template <bool IN_LDS, class T>
 using mem_t = typename std::conditional<IN_LDS, lmem_t<T>, gmem_t<T>>::type;
 
+// Element `idx` of a wave-uniform array through a 32-bit byte offset, so a
+// global access is "uniform base + 32-bit lane offset" (no per-lane 64-bit
+// address arithmetic).  idx * sizeof(T) stays below 4 GiB for every ring
+// frame and history step.
+template <class T>
+__device__ __forceinline__ gmem_t<T>& elem(gmem_t<T>* base, uint32_t idx) {
+  return *(gmem_t<T>*)((gmem_t<char>*)base + idx * (uint32_t)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ const gmem_t<T>& elem(const gmem_t<T>* base, uint32_t idx) {
+  return *(const gmem_t<T>*)((const gmem_t<char>*)base + idx * (uint32_t)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ lmem_t<T>& elem(lmem_t<T>* base, uint32_t idx) {
+  return base[idx];
+}
+template <class T>
+__device__ __forceinline__ const lmem_t<T>& elem(const lmem_t<T>* base, uint32_t idx) {
+  return base[idx];
+}
+
+// A wave-uniform 64-bit pointer held in scalar registers.
+template <class T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+  const uint64_t v = (uint64_t)(uintptr_t)p;
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)(v >> 32));
+  return (T*)(uintptr_t)(((uint64_t)hi << 32) | lo);
+}
+
 // Storage of one pair: all in LDS (LDS tier), all in HBM, or (RL) ring and
 // sequences in LDS with history in HBM.
 template <bool LDS, bool RL = false>
@@ -181,8 +211,8 @@ __device__ __forceinline__ WaveStore<LDS, RL> make_store(const WfaArgs& a, const
       lds_barrier();
       return S;
     }
-    S.qw = (typename S_t::seq_t*)pd.q;
-    S.tw = (typename S_t::seq_t*)pd.t;
+    S.qw = uniform_ptr((typename S_t::seq_t*)pd.q);
+    S.tw = uniform_ptr((typename S_t::seq_t*)pd.t);
   }
   S.qsh = pd.q_shift;
   S.tsh = pd.t_shift;
@@ -571,9 +601,9 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
     if constexpr (UNW) {  // this step's frames: -1 margins beyond [lo, hi]
       for (int32_t t = lane; t < 2 * M; t += NT) {
         const uint32_t ix = rix(t < M ? lo - M + t : hi + 1 + (t - M));
-        Wm[ix] = -1;
-        Wi[ix] = -1;
-        Wd[ix] = -1;
+        elem(Wm, ix) = -1;
+        elem(Wi, ix) = -1;
+        elem(Wd, ix) = -1;
       }
     }
     const int32_t nch = (width + NT - 1) / NT;
@@ -583,7 +613,8 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
       // frame (k clamped to hi + 1).
       const int32_t kr = UNW ? min(k, hi + 1) : k;
       const uint32_t ik = rix(kr), il = rix(kr - 1), ir = rix(kr + 1);
-      const int32_t rvx = Rmx[ik], rvol = Rmo[il], rvor = Rmo[ir], rvil = Rie[il], rvdr = Rde[ir];
+      const int32_t rvx = elem(Rmx, ik), rvol = elem(Rmo, il), rvor = elem(Rmo, ir),
+                    rvil = elem(Rie, il), rvdr = elem(Rde, ir);
       const int32_t vx = (UNW || (k >= xlo && k <= xhi)) ? rvx : -1;
       const int32_t vol = (UNW || (k - 1 >= olo && k - 1 <= ohi)) ? rvol : -1;
       const int32_t vor = (UNW || (k + 1 >= olo && k + 1 <= ohi)) ? rvor : -1;
@@ -605,11 +636,11 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
       const int32_t mval = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, best, k);
       int32_t layer = -1;
       if (k <= hi) {
-        Wm[ik] = (typename S_t::ring_v)mval;
-        Wi[ik] = (typename S_t::ring_v)ins;
-        Wd[ik] = (typename S_t::ring_v)del;
+        elem(Wm, ik) = (typename S_t::ring_v)mval;
+        elem(Wi, ik) = (typename S_t::ring_v)ins;
+        elem(Wd, ik) = (typename S_t::ring_v)del;
         if (hist_on)
-          cells[k - lo] = (typename S_t::cell_v)(((uint32_t)(best + 1) << 4) | mop | (iop << 2) |
+          elem(cells, (uint32_t)(k - lo)) = (typename S_t::cell_v)(((uint32_t)(best + 1) << 4) | mop | (iop << 2) |
                                                  (dop << 3));
         if (k >= ke_lo && k <= ke_hi) layer = mval == qn ? 0 : ins == qn ? 1 : del == qn ? 2 : -1;
       }
@@ -764,14 +795,14 @@ __device__ void wave_linear(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
     const int32_t ke_lo = fitting ? max(qn - tn, lo) : kend_global;
     const int32_t ke_hi = fitting ? hi : kend_global;
     if constexpr (UNW) {  // this step's frame: -1 margins beyond [lo, hi]
-      for (int32_t t = lane; t < 2 * M; t += NT) Wn[rix(t < M ? lo - M + t : hi + 1 + (t - M))] = -1;
+      for (int32_t t = lane; t < 2 * M; t += NT) elem(Wn, rix(t < M ? lo - M + t : hi + 1 + (t - M))) = -1;
     }
     const int32_t nch = (width + NT - 1) / NT;
     for (int32_t c = 0; c < nch; ++c) {
       const int32_t k = lo + c * NT + lane;
       const int32_t kr = UNW ? min(k, hi + 1) : k;  // lanes past hi: reads stay in the frame
       const uint32_t ik = rix(kr), il = rix(kr - 1), ir = rix(kr + 1);
-      const int32_t rvm = P[ik], rvl = P[il], rvr = P[ir];
+      const int32_t rvm = elem(P, ik), rvl = elem(P, il), rvr = elem(P, ir);
       // k - 1 < -tn and k + 1 > qn fall in the margins, so with UNW the
       // upstream bounds (k > -tn, k < qn) come out of the reads as well.
       const int32_t vm = (UNW || (k >= p_lo && k <= p_hi)) ? rvm : -1;
@@ -788,9 +819,9 @@ __device__ void wave_linear(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
       const int32_t val = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, ok ? best : -1, k);
       bool hit = false;
       if (k <= hi) {
-        Wn[ik] = (typename S_t::ring_v)val;
+        elem(Wn, ik) = (typename S_t::ring_v)val;
         if (hist_on)
-          cells[k - lo] = (typename S_t::cell_v)(ok ? (((uint32_t)(best + 1) << 4) | op) : 0u);
+          elem(cells, (uint32_t)(k - lo)) = (typename S_t::cell_v)(ok ? (((uint32_t)(best + 1) << 4) | op) : 0u);
         hit = ok && k >= ke_lo && k <= ke_hi && val >= qn;
       }
       if (NT == 64) {
