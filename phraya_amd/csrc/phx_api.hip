@@ -57,6 +57,8 @@ struct phx_plan {
   std::vector<double> group_share;                     // WFA: share of the HBM budget
   std::vector<double> group_load;                      // WFA: duration model of each group
   std::vector<uint8_t> group_rl;                       // WFA: block tier with ring + sequences in LDS
+  std::vector<uint64_t> group_maxsum;                  // WFA: largest qn + tn of each group
+  std::vector<uint32_t> group_seqw;                    // WFA: largest staged sequence words of each group
   // hipGraph of a run's launch sequence (WFA): replayed by later runs.
   bool use_graph = false;
   bool graph_wanted = false;                           // build one after the current run
@@ -159,11 +161,23 @@ struct GroupSlots {
   uint32_t wcap = 0;
 };
 
+uint64_t max_sum_of(const phx_plan* P, const std::vector<uint32_t>& work) {
+  uint64_t m = 1;
+  for (uint32_t i : work) m = std::max<uint64_t>(m, (uint64_t)P->pairs[i].qn + P->pairs[i].tn);
+  return m;
+}
+
+uint32_t seq_words_of(const phx_plan* P, const std::vector<uint32_t>& work) {
+  uint32_t w = 2;
+  for (uint32_t i : work) w = std::max(w, lds_seq_words(P->pairs[i]));
+  return w;
+}
+
+// max_sum / seq_words: the group's cached maxima (0 = compute from work).
 GroupSlots group_slots(const phx_plan* P, int t, const std::vector<uint32_t>& work,
-                       uint64_t slot_bytes, uint64_t budget, bool rl = false) {
+                       uint64_t slot_bytes, uint64_t budget, bool rl = false, uint64_t max_sum = 0) {
   GroupSlots g;
-  uint64_t max_sum = 1;
-  for (uint32_t i : work) max_sum = std::max<uint64_t>(max_sum, (uint64_t)P->pairs[i].qn + P->pairs[i].tn);
+  if (max_sum == 0) max_sum = max_sum_of(P, work);
   g.ring_words = ring_words_for(ring_frames(P->engine, P->costs), max_sum, &g.wcap);
   if (rl) {  // the ring lives in LDS
     g.ring_words = 0;
@@ -190,7 +204,8 @@ enum LaunchTiming { TIME_BOTH, TIME_END, TIME_NONE };
 
 int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint32_t* d_list,
                     uint32_t* d_counters, uint32_t* overflow, const GroupSlots& gs, uint8_t* slice,
-                    hipStream_t st, bool rl = false, LaunchTiming timing = TIME_BOTH) {
+                    hipStream_t st, bool rl = false, LaunchTiming timing = TIME_BOTH,
+                    uint32_t seq_words = 0) {
   phx_ctx* ctx = P->ctx;
   WfaArgs a{};
   a.pairs = (const PairDesc*)P->d_pairs.p;
@@ -205,9 +220,8 @@ int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint3
   const int mode = P->engine == PHX_ENGINE_AFFINE ? ENG_AFFINE : ENG_LINEAR;
   WaveGeom G{};
   uint32_t slots = 0;
+  if ((t == TIER_LDS || rl) && seq_words == 0) seq_words = seq_words_of(P, work);
   if (t == TIER_LDS) {
-    uint32_t seq_words = 2;
-    for (uint32_t i : work) seq_words = std::max(seq_words, lds_seq_words(P->pairs[i]));
     G = lds_geometry(ring_frames(P->engine, P->costs), seq_words, P->tun.wave_lds_kib);
     slots = lds_tier_waves(work.size(), P->tun.wave_slots);
     a.static_order = 1;  // the list is stride-balanced for exactly these waves
@@ -219,8 +233,6 @@ int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint3
     a.wcap_log2 = log2u(gs.wcap);
     slots = (uint32_t)gs.n;
     if (rl) {
-      uint32_t seq_words = 2;
-      for (uint32_t i : work) seq_words = std::max(seq_words, lds_seq_words(P->pairs[i]));
       G = block_lds_geometry(ring_frames(P->engine, P->costs), gs.wcap, seq_words);
     }
   }
@@ -279,7 +291,7 @@ int issue_groups(phx_plan* P, bool capture, std::vector<int>* launched,
   for (int gi = 0; gi < ngroups; ++gi) {
     if (P->work[gi].empty() || P->group_tier[gi] == TIER_LDS) continue;
     gs[gi] = group_slots(P, P->group_tier[gi], P->work[gi], P->group_slot[gi], P->group_budget(gi),
-                         P->group_rl[gi] != 0);
+                         P->group_rl[gi] != 0, P->group_maxsum[gi]);
     slice_off[gi] = arena;
     arena += gs[gi].n * gs[gi].per;
   }
@@ -300,7 +312,7 @@ int issue_groups(phx_plan* P, bool capture, std::vector<int>* launched,
     int rc = launch_wfa_tier(P, P->group_tier[gi], wl, (uint32_t*)P->d_work.p + group_off[gi],
                              d_ctr + 2 * gi, P->ctr_dev + gi, gs[gi],
                              (uint8_t*)P->d_arena.p + slice_off[gi], st, P->group_rl[gi] != 0,
-                             capture ? TIME_NONE : TIME_BOTH);
+                             capture ? TIME_NONE : TIME_BOTH, P->group_seqw[gi]);
     if (rc != PHX_OK) return rc;
     if (capture) {
       hipStreamCaptureStatus cs;
@@ -583,6 +595,13 @@ void plan_group(phx_plan* P) {
       longest = std::max(longest, d);
     }
     P->group_load[g] = std::max(longest, sum / std::max(1.0, std::min(flight, (double)P->work[g].size())));
+  }
+  // Per-group maxima the launches need, computed once per grouping.
+  P->group_maxsum.assign(P->work.size(), 0);
+  P->group_seqw.assign(P->work.size(), 0);
+  for (size_t g = 0; g < P->work.size(); ++g) {
+    P->group_maxsum[g] = max_sum_of(P, P->work[g]);
+    if (P->group_tier[g] == TIER_LDS || P->group_rl[g]) P->group_seqw[g] = seq_words_of(P, P->work[g]);
   }
 }
 
