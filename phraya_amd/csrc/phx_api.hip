@@ -222,7 +222,8 @@ int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint3
   uint32_t slots = 0;
   if ((t == TIER_LDS || rl) && seq_words == 0) seq_words = seq_words_of(P, work);
   if (t == TIER_LDS) {
-    G = lds_geometry(ring_frames(P->engine, P->costs), seq_words, P->tun.wave_lds_kib);
+    G = lds_geometry(lds_tier_ring(P->engine, P->costs) ? ring_frames(P->engine, P->costs) : 0,
+                     seq_words, P->tun.wave_lds_kib);
     slots = lds_tier_waves(work.size(), P->tun.wave_slots);
     a.static_order = 1;  // the list is stride-balanced for exactly these waves
   } else {

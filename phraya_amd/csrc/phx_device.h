@@ -81,6 +81,14 @@ struct CostParams {
 // CIGAR run encoding: (length << 4) | op.
 enum RunOp : uint32_t { OP_M = 0, OP_X = 1, OP_I = 2, OP_D = 3 };
 
+// Cost shapes (mismatch, gap open + extend and gap extend in score steps of
+// g = gcd(x, o, e)) whose LDS-tier kernel keeps the wavefront in registers;
+// their LDS-tier layout has no ring.  Linear WFA always does.
+__host__ __device__ inline bool reg_wavefront_shape(int32_t xn, int32_t oen, int32_t en) {
+  return (xn == 2 && oen == 4 && en == 1) || (xn == 2 && oen == 5 && en == 1) ||
+         (xn == 1 && oen == 1 && en == 1) || (xn == 3 && oen == 7 && en == 2);
+}
+
 // Kernel arguments of one WFA tier launch.
 struct WfaArgs {
   const PairDesc* pairs;

@@ -123,6 +123,14 @@ inline int32_t ring_frames(int engine, const phx_costs& c) {
   return (std::max(c.mismatch, c.gap_open + c.gap_extend) / g + 1) + 2 * (c.gap_extend / g + 1);
 }
 
+// Whether the LDS tier's layout needs an LDS ring: only for affine cost
+// shapes without a register wavefront (reg_wavefront_shape).
+inline bool lds_tier_ring(int engine, const phx_costs& c) {
+  if (engine != PHX_ENGINE_AFFINE) return false;
+  const int32_t g = gcd3(c.mismatch, c.gap_open, c.gap_extend);
+  return !reg_wavefront_shape(c.mismatch / g, (c.gap_open + c.gap_extend) / g, c.gap_extend / g);
+}
+
 // sum_{u=1..S} min(u, a)
 inline uint64_t sum_min(uint64_t S, uint64_t a) {
   return S <= a ? S * (S + 1) / 2 : a * (a + 1) / 2 + (S - a) * a;

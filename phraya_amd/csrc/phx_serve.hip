@@ -355,7 +355,8 @@ bool launch(phx_server* S, Lane& L, std::vector<Done>& fin) {
   WaveGeom G{};
   uint32_t slots = n;
   if (L.tier == TIER_LDS) {
-    G = lds_geometry(ring_frames(S->engine, S->costs), max_seq_words, S->wave_lds_kib);
+    G = lds_geometry(lds_tier_ring(S->engine, S->costs) ? ring_frames(S->engine, S->costs) : 0,
+                     max_seq_words, S->wave_lds_kib);
     slots = std::min<uint32_t>(n, std::max<uint32_t>(kWaveWaves, S->wave_slots));
   }
   hipError_t e = hipMemcpyAsync(L.d_seq.p, L.h_seq.p, words * 8, hipMemcpyHostToDevice, st);

@@ -264,16 +264,22 @@ struct PairShared {
 
 // --- register wavefront (LDS tier) -------------------------------------------
 //
-// While a pair's diagonal range fits 64 diagonals, lane L of the wave owns
-// diagonal kb + L for the whole pair and keeps the last score frames of its
-// diagonal in registers; the k - 1 / k + 1 neighbours come from the adjacent
-// lanes by a wave-wide DPP shift.  A lane outside a step's range holds -1
-// (unset) for that step, which is exactly what the ring path's range checks
-// produce, so no range checks are needed on reads.  History cells go to the
-// same LDS layout as the ring path (cells at toff[step] + k - lo), so the
-// traceback is shared.  When the next step's range would leave the window,
-// the frames are written to the LDS ring and the ring path continues from
-// that step; results are identical either way.
+// The LDS tier keeps the wavefront in registers.  Phase 1: while the
+// pair's diagonal range fits 64 diagonals, lane L owns diagonal kb + L for
+// the whole pair and keeps the last score frames of its diagonal in
+// registers; the k - 1 / k + 1 neighbours come from the adjacent lanes by a
+// wave-wide DPP shift.  Phase 2: once the range outgrows that window, the
+// frames move (one cross-lane permute per register) to a window of 128
+// diagonals, two per lane (kb2 + L and kb2 + 64 + L); the neighbours across
+// the two halves come from one lane read each.  A lane outside a step's
+// range holds -1 (unset) for that step, which is exactly what the ring
+// path's range checks produce, so no range checks are needed on reads.
+// History cells go to the ring path's LDS layout (cells at toff[step] + k -
+// lo), so the traceback is shared.  A pair whose range outgrows 128
+// diagonals stops with ST_OVERFLOW_WIDTH (the ring path's limit as well) and
+// is re-run on an HBM tier.  With register phases the LDS tier needs no LDS
+// ring at all (the host leaves it out of the layout), so more waves fit a
+// CU.
 
 // Value of the lane below (k - 1); lane 0 reads -1.  Call with every lane
 // active: a source lane outside EXEC also reads as -1.
@@ -285,7 +291,26 @@ __device__ __forceinline__ int32_t from_above(int32_t v) {
   return __builtin_amdgcn_update_dpp(-1, v, 0x130, 0xF, 0xF, false);  // wave_shl:1
 }
 
-// Loop state shared by the register phase and the ring loop.
+// Neighbours over a window of CH chunks of 64 diagonals (chunk c, lane L =
+// diagonal kb + 64 c + L): the lane below / above, across chunk edges too;
+// -1 past either end of the window.
+template <int CH>
+__device__ __forceinline__ void neighbours(const int32_t (&v)[CH], int lane, int32_t (&below)[CH],
+                                           int32_t (&above)[CH]) {
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    below[c] = from_below(v[c]);
+    above[c] = from_above(v[c]);
+  }
+#pragma unroll
+  for (int c = 0; c + 1 < CH; ++c) {
+    const int32_t top = readlane(v[c], 63), bottom = readlane(v[c + 1], 0);
+    above[c] = lane == 63 ? bottom : above[c];
+    below[c + 1] = lane == 0 ? top : below[c + 1];
+  }
+}
+
+// Loop state shared by the register phases and the ring loop.
 struct StepState {
   int32_t step;
   uint64_t used;
@@ -294,185 +319,247 @@ struct StepState {
   int32_t status;
 };
 
-// First diagonal of the register window: 64 diagonals centred on 0, or
-// starting at -tn when the target is short.
-__device__ __forceinline__ int32_t reg_window_base(int32_t tn) { return max(-32, -tn); }
-
-// Writes the toff entries of the completed steps 1 .. done (<= 63) held by
-// the lanes.
-__device__ __forceinline__ void flush_toff(const WaveStore<true>& S, uint32_t toffr, int32_t done, int lane) {
-  if (lane >= 1 && lane <= done) S.toff(lane) = (uint16_t)toffr;
+// First diagonal of a register window of n diagonals: centred on 0, moved
+// down when the query is short (the range never passes qn) and up to -tn
+// when the target is short (it never passes -tn).
+__device__ __forceinline__ int32_t reg_window_base(int32_t n, int32_t qn, int32_t tn) {
+  return max(-tn, min(-(n / 2), qn - (n - 1)));
 }
 
-// Gap-affine steps in registers.  XN, OEN, EN: mismatch, open + extend and
-// extend costs in score steps; the m layer keeps max(XN, OEN) frames, each
-// gap layer EN.  Recurrence and tie rules as in wave_affine's ring loop.
-template <int XN, int OEN, int EN, bool B8>
-__device__ void affine_regs(const WaveStore<true>& S, int32_t qn, int32_t tn, int32_t m0,
-                            bool fitting, int32_t max_step, int32_t Rm, int32_t Rg, int lane,
-                            StepState& st) {
+// Moves a phase-1 register (lane L = diagonal kb + L) into the phase-2
+// window (chunk c, lane L = diagonal kb2 + 64 c + L); d = kb - kb2 >= 0.
+__device__ __forceinline__ void widen(int32_t v, int32_t d, int lane, int32_t (&out)[2]) {
+  const int32_t s0 = lane - d, s1 = lane + 64 - d;
+  const int32_t p0 = __builtin_amdgcn_ds_bpermute((s0 & 63) << 2, v);
+  const int32_t p1 = __builtin_amdgcn_ds_bpermute((s1 & 63) << 2, v);
+  out[0] = s0 >= 0 ? p0 : -1;
+  out[1] = s1 <= 63 ? p1 : -1;
+}
+
+// Writes the toff entries of the completed steps 1 .. done (<= 127) held
+// by the lanes (lane s: step s, and step 64 + s in the second register).
+__device__ __forceinline__ void flush_toff(const WaveStore<true>& S, uint32_t t0, uint32_t t1,
+                                           int32_t done, int lane) {
+  if (lane >= 1 && lane <= done) S.toff(lane) = (uint16_t)t0;
+  if (64 + lane <= done) S.toff(64 + lane) = (uint16_t)t1;
+}
+
+// Gap-affine steps in registers over a window of CH chunks.  XN, OEN, EN:
+// mismatch, open + extend and extend costs in score steps; the m layer keeps
+// max(XN, OEN) frames, each gap layer EN.  Recurrence and tie rules as in
+// wave_affine's ring loop.  Returns false when the next step's range leaves
+// the window (the state is then that of the last completed step).
+template <int CH, int XN, int OEN, int EN, bool B8>
+__device__ bool affine_steps(const WaveStore<true>& S, int32_t qn, int32_t tn, int32_t kb,
+                             bool fitting, int32_t max_step, int lane,
+                             int32_t (&Mf)[CH][XN > OEN ? XN : OEN], int32_t (&If)[CH][EN],
+                             int32_t (&Df)[CH][EN], uint32_t& t0, uint32_t& t1, StepState& st) {
   constexpr int MD = XN > OEN ? XN : OEN;
-  const int32_t kb = reg_window_base(tn);
-  const int32_t k = kb + lane;
-  int32_t Mf[MD], If[EN], Df[EN];  // [j] = frame of step (step - j)
-#pragma unroll
-  for (int j = 0; j < MD; ++j) Mf[j] = -1;
-#pragma unroll
-  for (int j = 0; j < EN; ++j) If[j] = Df[j] = -1;
-  Mf[0] = k == 0 ? m0 : -1;
-  uint32_t toffr = 0;  // lane s: toff of step s
   const int32_t kend_g = qn - tn;
-  int32_t step = 0;
-  uint64_t used = 0;
-  bool found = false;
-  int32_t end_k = 0, end_layer = 0, status = ST_OK;
   for (;;) {
-    const int32_t s = step + 1;
+    const int32_t s = st.step + 1;
     const int32_t lo = step_lo(s, tn), hi = step_hi(s, qn);
-    if (lo < kb || hi > kb + 63 || s > 63) break;  // continue on the ring
-    step = s;
-    if (s > max_step) { status = ST_ALIGNMENT_FAILED; break; }
+    if (lo < kb || hi > kb + 64 * CH - 1 || s > 64 * CH - 1) return false;
+    st.step = s;
+    if (s > max_step) { st.status = ST_ALIGNMENT_FAILED; return true; }
     const int32_t width = hi - lo + 1;
-    if (!S.fits(s, used, width)) { status = ST_OVERFLOW_HIST; break; }
-    toffr = lane == s ? (uint32_t)used : toffr;
-    const bool inr = k >= lo && k <= hi;
-    const int32_t vx = Mf[XN - 1];
-    const int32_t vol = from_below(Mf[OEN - 1]), vor = from_above(Mf[OEN - 1]);
-    const int32_t vil = from_below(If[EN - 1]), vdr = from_above(Df[EN - 1]);
-    int32_t ins = (vol >= 0 && vol < qn) ? vol + 1 : -1;
-    uint32_t iop = 0;
-    if (vil >= 0 && vil < qn && vil + 1 > ins) { ins = vil + 1; iop = 1; }
-    int32_t del = (vor >= 0 && vor - k <= tn) ? vor : -1;
-    uint32_t dop = 0;
-    if (vdr >= 0 && vdr - k <= tn && vdr > del) { del = vdr; dop = 1; }
-    int32_t best = (vx >= 0 && vx < qn && vx + 1 - k <= tn) ? vx + 1 : -1;
-    uint32_t mop = 0;
-    if (ins > best) { best = ins; mop = 1; }
-    if (del > best) { best = del; mop = 2; }
-    const int32_t mval = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, inr ? best : -1, k);
-    if (inr)
-      S.cells[used + (uint64_t)(k - lo)] =
-          (uint16_t)(((uint32_t)(best + 1) << 4) | mop | (iop << 2) | (dop << 3));
+    if (!S.fits(s, st.used, width)) { st.status = ST_OVERFLOW_HIST; return true; }
+    if (s < 64) t0 = lane == s ? (uint32_t)st.used : t0;
+    else t1 = lane == s - 64 ? (uint32_t)st.used : t1;
+    int32_t om[CH], ie[CH], de[CH], vol[CH], vor[CH], vil[CH], vdr[CH], dummy[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      om[c] = Mf[c][OEN - 1];
+      ie[c] = If[c][EN - 1];
+      de[c] = Df[c][EN - 1];
+    }
+    neighbours<CH>(om, lane, vol, vor);
+    neighbours<CH>(ie, lane, vil, dummy);
+    neighbours<CH>(de, lane, dummy, vdr);
     const int32_t ke_lo = fitting ? max(kend_g, lo) : kend_g;
     const int32_t ke_hi = fitting ? hi : kend_g;
-    int32_t layer = -1;
-    if (inr && k >= ke_lo && k <= ke_hi) layer = mval == qn ? 0 : ins == qn ? 1 : del == qn ? 2 : -1;
+    uint64_t fmask = 0;
+    int32_t fc = 0, flayer = -1;
 #pragma unroll
-    for (int j = MD - 1; j > 0; --j) Mf[j] = Mf[j - 1];
+    for (int c = 0; c < CH; ++c) {
+      const int32_t k = kb + 64 * c + lane;
+      const bool inr = k >= lo && k <= hi;
+      const int32_t vx = Mf[c][XN - 1];
+      int32_t ins = (vol[c] >= 0 && vol[c] < qn) ? vol[c] + 1 : -1;
+      uint32_t iop = 0;
+      if (vil[c] >= 0 && vil[c] < qn && vil[c] + 1 > ins) { ins = vil[c] + 1; iop = 1; }
+      int32_t del = (vor[c] >= 0 && vor[c] - k <= tn) ? vor[c] : -1;
+      uint32_t dop = 0;
+      if (vdr[c] >= 0 && vdr[c] - k <= tn && vdr[c] > del) { del = vdr[c]; dop = 1; }
+      int32_t best = (vx >= 0 && vx < qn && vx + 1 - k <= tn) ? vx + 1 : -1;
+      uint32_t mop = 0;
+      if (ins > best) { best = ins; mop = 1; }
+      if (del > best) { best = del; mop = 2; }
+      const int32_t mval = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, inr ? best : -1, k);
+      if (inr)
+        S.cells[st.used + (uint64_t)(k - lo)] =
+            (uint16_t)(((uint32_t)(best + 1) << 4) | mop | (iop << 2) | (dop << 3));
+      int32_t layer = -1;
+      if (inr && k >= ke_lo && k <= ke_hi) layer = mval == qn ? 0 : ins == qn ? 1 : del == qn ? 2 : -1;
 #pragma unroll
-    for (int j = EN - 1; j > 0; --j) { If[j] = If[j - 1]; Df[j] = Df[j - 1]; }
-    Mf[0] = inr ? mval : -1;
-    If[0] = inr ? ins : -1;
-    Df[0] = inr ? del : -1;
-    used += (uint64_t)width;
-    const uint64_t mask = __ballot(layer >= 0);
-    if (mask) {
-      const int32_t ln = (int32_t)__builtin_ctzll(mask);
-      found = true;
-      end_k = kb + ln;
-      end_layer = readlane(layer, ln);
-      break;
+      for (int j = MD - 1; j > 0; --j) Mf[c][j] = Mf[c][j - 1];
+#pragma unroll
+      for (int j = EN - 1; j > 0; --j) { If[c][j] = If[c][j - 1]; Df[c][j] = Df[c][j - 1]; }
+      Mf[c][0] = inr ? mval : -1;
+      If[c][0] = inr ? ins : -1;
+      Df[c][0] = inr ? del : -1;
+      const uint64_t m = __ballot(layer >= 0);
+      if (m && !fmask) {  // the lowest diagonal reaching the end
+        fmask = m;
+        fc = c;
+        flayer = readlane(layer, (int32_t)__builtin_ctzll(m));
+      }
+    }
+    st.used += (uint64_t)width;
+    if (fmask) {
+      st.found = true;
+      st.end_k = kb + 64 * fc + (int32_t)__builtin_ctzll(fmask);
+      st.end_layer = flayer;
+      return true;
     }
   }
-  flush_toff(S, toffr, status == ST_OK ? step : step - 1, lane);
-  if (!found && status == ST_OK && step > 0) {
-    // Hand the frames to the ring: frame of step (step - j) into its slot.
-    const uint32_t W = S.wmask + 1;
-    const uint32_t ik = (uint32_t)k & S.wmask;
-    lmem_t<int16_t>* Mr = S.ring;
-    lmem_t<int16_t>* Ri = Mr + (size_t)Rm * W;
-    lmem_t<int16_t>* Rd = Ri + (size_t)Rg * W;
-#pragma unroll
-    for (int j = 0; j < MD; ++j)
-      if (step - j >= 0) Mr[(size_t)((step - j) % Rm) * W + ik] = (int16_t)Mf[j];
-#pragma unroll
-    for (int j = 0; j < EN; ++j)
-      if (step - j >= 0) {
-        Ri[(size_t)((step - j) % Rg) * W + ik] = (int16_t)If[j];
-        Rd[(size_t)((step - j) % Rg) * W + ik] = (int16_t)Df[j];
-      }
-  }
-  wave_sync<true>();
-  st = StepState{step, used, found, end_k, end_layer, status};
 }
 
-// Cost shapes with a register phase; others run the ring loop from step 1.
+// The whole register wavefront of one gap-affine pair (phase 1, then phase
+// 2 if needed).
+template <int XN, int OEN, int EN, bool B8>
+__device__ void affine_regs(const WaveStore<true>& S, int32_t qn, int32_t tn, int32_t m0,
+                            bool fitting, int32_t max_step, int lane, StepState& st) {
+  constexpr int MD = XN > OEN ? XN : OEN;
+  st = StepState{0, 0, false, 0, 0, ST_OK};
+  const int32_t kb = reg_window_base(64, qn, tn);
+  int32_t Mf[1][MD], If[1][EN], Df[1][EN];  // [c][j] = frame of step (step - j)
+#pragma unroll
+  for (int j = 0; j < MD; ++j) Mf[0][j] = -1;
+#pragma unroll
+  for (int j = 0; j < EN; ++j) If[0][j] = Df[0][j] = -1;
+  Mf[0][0] = kb + lane == 0 ? m0 : -1;
+  uint32_t t0 = 0, t1 = 0;
+  bool done = affine_steps<1, XN, OEN, EN, B8>(S, qn, tn, kb, fitting, max_step, lane, Mf, If, Df,
+                                               t0, t1, st);
+  if (!done) {
+    const int32_t kb2 = reg_window_base(128, qn, tn);
+    int32_t M2[2][MD], I2[2][EN], D2[2][EN], w[2];
+#pragma unroll
+    for (int j = 0; j < MD; ++j) { widen(Mf[0][j], kb - kb2, lane, w); M2[0][j] = w[0]; M2[1][j] = w[1]; }
+#pragma unroll
+    for (int j = 0; j < EN; ++j) {
+      widen(If[0][j], kb - kb2, lane, w); I2[0][j] = w[0]; I2[1][j] = w[1];
+      widen(Df[0][j], kb - kb2, lane, w); D2[0][j] = w[0]; D2[1][j] = w[1];
+    }
+    done = affine_steps<2, XN, OEN, EN, B8>(S, qn, tn, kb2, fitting, max_step, lane, M2, I2, D2,
+                                            t0, t1, st);
+    if (!done) {  // wider than 128 diagonals: the ring limit of this tier
+      st.step += 1;
+      st.status = ST_OVERFLOW_WIDTH;
+    }
+  }
+  flush_toff(S, t0, t1, st.status == ST_OK ? st.step : st.step - 1, lane);
+  wave_sync<true>();
+}
+
+// Cost shapes with a register wavefront (reg_wavefront_shape); others run
+// the LDS ring loop from step 1.
 template <bool B8>
 __device__ __forceinline__ bool affine_regs_any(const WaveStore<true>& S, int32_t qn, int32_t tn,
                                                 int32_t m0, bool fitting, int32_t max_step,
                                                 int32_t xn, int32_t oen, int32_t en, int lane,
                                                 StepState& st) {
-  const int32_t Rm = max(xn, oen) + 1, Rg = en + 1;
   if (xn == 2 && oen == 4 && en == 1)  // 4/6/2
-    affine_regs<2, 4, 1, B8>(S, qn, tn, m0, fitting, max_step, Rm, Rg, lane, st);
+    affine_regs<2, 4, 1, B8>(S, qn, tn, m0, fitting, max_step, lane, st);
   else if (xn == 2 && oen == 5 && en == 1)  // 2/4/1
-    affine_regs<2, 5, 1, B8>(S, qn, tn, m0, fitting, max_step, Rm, Rg, lane, st);
+    affine_regs<2, 5, 1, B8>(S, qn, tn, m0, fitting, max_step, lane, st);
   else if (xn == 1 && oen == 1 && en == 1)  // 1/0/1
-    affine_regs<1, 1, 1, B8>(S, qn, tn, m0, fitting, max_step, Rm, Rg, lane, st);
+    affine_regs<1, 1, 1, B8>(S, qn, tn, m0, fitting, max_step, lane, st);
   else if (xn == 3 && oen == 7 && en == 2)  // 3/5/2
-    affine_regs<3, 7, 2, B8>(S, qn, tn, m0, fitting, max_step, Rm, Rg, lane, st);
+    affine_regs<3, 7, 2, B8>(S, qn, tn, m0, fitting, max_step, lane, st);
   else
     return false;
   return true;
 }
 
-// Gap-linear steps in registers (one frame back); recurrence, bounds and tie
-// rules as in wave_linear's ring loop.
+// Gap-linear steps in registers over a window of CH chunks (one frame
+// back); recurrence, bounds and tie rules as in wave_linear's ring loop.
+template <int CH, bool B8>
+__device__ bool linear_steps(const WaveStore<true>& S, int32_t qn, int32_t tn, int32_t kb,
+                             bool fitting, int32_t explore, int lane, int32_t (&Pf)[CH],
+                             uint32_t& t0, uint32_t& t1, StepState& st, bool& exhausted) {
+  const int32_t kend_global = qn - tn;
+  for (;;) {
+    const int32_t s = st.step + 1;
+    const int32_t lo = step_lo(s, tn), hi = step_hi(s, qn);
+    if (lo < kb || hi > kb + 64 * CH - 1 || s > 64 * CH - 1) return false;
+    st.step = s;
+    if (s > explore) { exhausted = true; return true; }
+    const int32_t width = hi - lo + 1;
+    if (!S.fits(s, st.used, width)) { st.status = ST_OVERFLOW_HIST; return true; }
+    if (s < 64) t0 = lane == s ? (uint32_t)st.used : t0;
+    else t1 = lane == s - 64 ? (uint32_t)st.used : t1;
+    int32_t below[CH], above[CH];
+    neighbours<CH>(Pf, lane, below, above);
+    const int32_t ke_lo = fitting ? max(qn - tn, lo) : kend_global;
+    const int32_t ke_hi = fitting ? hi : kend_global;
+    uint64_t fmask = 0;
+    int32_t fc = 0;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int32_t k = kb + 64 * c + lane;
+      const bool inr = k >= lo && k <= hi;
+      const int32_t vm = Pf[c];
+      const int32_t vl = k > -tn ? below[c] : -1;
+      const int32_t vr = k < qn ? above[c] : -1;
+      int32_t best = -1;
+      uint32_t op = 0;
+      if (vm >= 0) { best = vm + 1; op = 1; }
+      if (vl >= 0 && vl + 1 >= best) { best = vl + 1; op = 2; }
+      if (vr >= 0 && vr >= best) { best = vr; op = 3; }
+      const bool ok = op != 0 && best <= qn && best - k >= 0 && best - k <= tn;
+      const int32_t val = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, inr && ok ? best : -1, k);
+      if (inr)
+        S.cells[st.used + (uint64_t)(k - lo)] = (uint16_t)(ok ? (((uint32_t)(best + 1) << 4) | op) : 0u);
+      const bool hit = inr && ok && k >= ke_lo && k <= ke_hi && val >= qn;
+      Pf[c] = inr ? val : -1;
+      const uint64_t m = __ballot(hit);
+      if (m && !fmask) {
+        fmask = m;
+        fc = c;
+      }
+    }
+    st.used += (uint64_t)width;
+    if (fmask) {
+      st.found = true;
+      st.end_k = kb + 64 * fc + (int32_t)__builtin_ctzll(fmask);
+      return true;
+    }
+  }
+}
+
+// The whole register wavefront of one gap-linear pair.
 template <bool B8>
 __device__ void linear_regs(const WaveStore<true>& S, int32_t qn, int32_t tn, int32_t w0,
                             bool fitting, int32_t explore, int lane, StepState& st, bool& exhausted) {
-  const int32_t kb = reg_window_base(tn);
-  const int32_t k = kb + lane;
-  int32_t Pf = k == 0 ? w0 : -1;
-  uint32_t toffr = 0;
-  const int32_t kend_global = qn - tn;
-  int32_t step = 0;
-  uint64_t used = 0;
-  bool found = false;
-  int32_t end_k = 0, status = ST_OK;
-  for (;;) {
-    const int32_t s = step + 1;
-    const int32_t lo = step_lo(s, tn), hi = step_hi(s, qn);
-    if (lo < kb || hi > kb + 63 || s > 63) break;
-    step = s;
-    if (s > explore) { exhausted = true; break; }
-    const int32_t width = hi - lo + 1;
-    if (!S.fits(s, used, width)) { status = ST_OVERFLOW_HIST; break; }
-    toffr = lane == s ? (uint32_t)used : toffr;
-    const bool inr = k >= lo && k <= hi;
-    // The shifts run on every lane: a lane left out of a DPP move would read
-    // as -1 to its neighbour.
-    const int32_t below = from_below(Pf), above = from_above(Pf);
-    const int32_t vm = Pf;
-    const int32_t vl = k > -tn ? below : -1;
-    const int32_t vr = k < qn ? above : -1;
-    int32_t best = -1;
-    uint32_t op = 0;
-    if (vm >= 0) { best = vm + 1; op = 1; }
-    if (vl >= 0 && vl + 1 >= best) { best = vl + 1; op = 2; }
-    if (vr >= 0 && vr >= best) { best = vr; op = 3; }
-    const bool ok = op != 0 && best <= qn && best - k >= 0 && best - k <= tn;
-    const int32_t val = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, inr && ok ? best : -1, k);
-    if (inr) S.cells[used + (uint64_t)(k - lo)] = (uint16_t)(ok ? (((uint32_t)(best + 1) << 4) | op) : 0u);
-    const int32_t ke_lo = fitting ? max(qn - tn, lo) : kend_global;
-    const int32_t ke_hi = fitting ? hi : kend_global;
-    const bool hit = inr && ok && k >= ke_lo && k <= ke_hi && val >= qn;
-    Pf = inr ? val : -1;
-    used += (uint64_t)width;
-    const uint64_t mask = __ballot(hit);
-    if (mask) {
-      found = true;
-      end_k = kb + (int32_t)__builtin_ctzll(mask);
-      break;
+  st = StepState{0, 0, false, 0, 0, ST_OK};
+  const int32_t kb = reg_window_base(64, qn, tn);
+  int32_t Pf[1] = {kb + lane == 0 ? w0 : -1};
+  uint32_t t0 = 0, t1 = 0;
+  bool done = linear_steps<1, B8>(S, qn, tn, kb, fitting, explore, lane, Pf, t0, t1, st, exhausted);
+  if (!done) {
+    const int32_t kb2 = reg_window_base(128, qn, tn);
+    int32_t P2[2];
+    widen(Pf[0], kb - kb2, lane, P2);
+    done = linear_steps<2, B8>(S, qn, tn, kb2, fitting, explore, lane, P2, t0, t1, st, exhausted);
+    if (!done) {
+      st.step += 1;
+      st.status = ST_OVERFLOW_WIDTH;
     }
   }
-  flush_toff(S, toffr, status == ST_OK && !exhausted ? step : step - 1, lane);
-  if (!found && !exhausted && status == ST_OK && step > 0) {
-    const uint32_t W = S.wmask + 1;
-    S.ring[(size_t)(step & 1) * W + ((uint32_t)k & S.wmask)] = (int16_t)Pf;
-  }
+  flush_toff(S, t0, t1, st.status == ST_OK && !exhausted ? st.step : st.step - 1, lane);
   wave_sync<true>();
-  st = StepState{step, used, found, end_k, 0, status};
 }
 
 // --- HBM ring without wrap-around ----------------------------------------------
@@ -536,9 +623,11 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
     ring_fill_origin<NT>((int32_t*)S.ring, W, Rm + 2 * Rg, M + Rm + 1, koff, qn, tn, M, lane);
     pair_sync<LDS, NT, RL>();
   }
+  // LDS tier with a register wavefront: its layout has no ring.
+  const bool regs = LDS && NT == 64 && reg_wavefront_shape(xn, oen, en);
 
   const int32_t m0 = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, 0, 0);
-  if (lane == 0) {
+  if (lane == 0 && !regs) {
     // Step 0 holds only m on diagonal 0; its ins/del entries are unset.
     Mr[rix(0)] = (typename S_t::ring_v)m0;
     Ri[rix(0)] = -1;
@@ -753,7 +842,7 @@ __device__ void wave_linear(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
 
   const int32_t w0 = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, 0, 0);
   if (lane == 0) {
-    ring[rix(0)] = (typename S_t::ring_v)w0;
+    if (!(LDS && NT == 64)) ring[rix(0)] = (typename S_t::ring_v)w0;  // LDS tier: no ring
     if (NT > 64) sh->key[0] = sh->key[1] = sh->key[2] = INT32_MAX;
   }
   pair_sync<LDS, NT, RL>();
