@@ -986,8 +986,10 @@ __device__ void wave_linear(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
 
 // --- kernel --------------------------------------------------------------------
 
+// Up to eight workgroups per CU (32 waves): registers are held to 64 per lane
+// so that occupancy is set by the LDS the tuning gives each wave.
 template <int MODE, bool LDS>
-__global__ __launch_bounds__(64 * WV_WAVES) void wfa_wave_kernel(WfaArgs a, WaveGeom G) {
+__global__ __launch_bounds__(64 * WV_WAVES, 8) void wfa_wave_kernel(WfaArgs a, WaveGeom G) {
   extern __shared__ __align__(16) uint8_t wave_lds_pool[];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int32_t)(threadIdx.x >> 6));
