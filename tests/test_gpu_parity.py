@@ -348,3 +348,29 @@ def test_lds_ring_workgroup_tier_and_width_overflow(px):
         for eng in ("affine", "linear"):
             compare(px, eng, qs, ts, costs=(4, 6, 2), ctx=ctx, label=f"lds-ring-{ring}")
         ctx.close()
+
+
+@pytest.mark.parametrize("costs", [(4, 6, 2), (2, 4, 1), (1, 0, 1), (3, 5, 2), (1, 30, 1)])
+def test_register_wavefront_phases(px, costs):
+    """LDS-tier pairs whose step counts cross the register windows (64 and
+    128 diagonals) and the 128-diagonal limit (re-run on an HBM tier), with
+    short queries or targets (window placement), fitting targets and byte
+    sequences; the last cost set has no register wavefront (LDS ring)."""
+    rng = random.Random(sum(costs) * 7 + 1)
+    qs, ts = [], []
+    for qn in (5, 20, 33, 63, 64, 65, 100, 150, 300, 600, 900):
+        for rate in (0.0, 0.05, 0.15, 0.3):
+            q = rand_seq(rng, qn)
+            t = mutate(rng, q, rate)
+            qs.append(q)
+            ts.append(t)
+            qs.append(q)
+            ts.append(t + rand_seq(rng, qn // 2 + 20))  # fitting
+            qs.append(q[: max(1, qn // 4)])
+            ts.append(t)  # target much longer than the query
+            qs.append(q)
+            ts.append(t[: max(1, len(t) // 5)])  # target much shorter
+    for eng in ("affine", "linear"):
+        compare(px, eng, qs, ts, costs=costs, label=f"regs {costs}")
+    bq = [q.lower() if i % 3 == 0 else q for i, q in enumerate(qs[:120])]
+    compare(px, "affine", bq, ts[:120], costs=costs, label=f"regs bytes {costs}")
