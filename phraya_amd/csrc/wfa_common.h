@@ -28,62 +28,81 @@ struct in_lds : std::false_type {};
 template <>
 struct in_lds<const lmem_t<const uint64_t>*> : std::true_type {};
 
-// 64 bits of a packed symbol stream starting at bit `bit` (symbol 0 in the
-// lowest bits of word 0), as two 32-bit funnel shifts over three halves.
-// LDS: the three 32-bit halves at bit / 32 .. + 2 (two read instructions,
-// no selects).  Global memory: the two aligned words bit / 64 and + 1 (one
+// Sequential 64-bit windows of a packed symbol stream (symbol 0 in the
+// lowest bits of word 0), starting at bit `bit` and moving 64 bits per
+// advance(): each window is two 32-bit funnel shifts over three 32-bit
+// halves, and the shift stays the same from one window to the next.
+// LDS: the three halves at dword bit / 32 .. + 2 (two read instructions, no
+// selects).  Global memory: the two aligned words bit / 64 and + 1 (one
 // paired load each, fewer memory instructions) and a select of the three
 // halves that matter.  Either way the reads stay inside the stream's words
 // (each packed sequence carries one word past its last symbol).
+template <class WP, bool LDS = in_lds<WP>::value>
+struct BitCursor;
+
 template <class WP>
-__device__ __forceinline__ uint64_t bit_window(WP w, uint32_t bit) {
-  uint32_t x0, x1, x2;
-  if constexpr (in_lds<WP>::value) {
-    const lmem_t<const uint32_t>* d = (const lmem_t<const uint32_t>*)w;
-    const uint32_t i = bit >> 5;
-    x0 = d[i];
-    x1 = d[i + 1];
-    x2 = d[i + 2];
-  } else {
-    const uint32_t wi = bit >> 6;
-    const uint64_t w0 = w[wi], w1 = w[wi + 1];
-    const bool upper = (bit & 32u) != 0;
-    x0 = upper ? (uint32_t)(w0 >> 32) : (uint32_t)w0;
-    x1 = upper ? (uint32_t)w1 : (uint32_t)(w0 >> 32);
-    x2 = upper ? (uint32_t)(w1 >> 32) : (uint32_t)w1;
+struct BitCursor<WP, true> {
+  const lmem_t<const uint32_t>* d;
+  uint32_t sh;
+  __device__ __forceinline__ BitCursor(WP w, uint32_t bit)
+      : d((const lmem_t<const uint32_t>*)w + (bit >> 5)), sh(bit) {}
+  __device__ __forceinline__ uint64_t read() const {
+    const uint32_t x0 = d[0], x1 = d[1], x2 = d[2];
+    const uint32_t lo = __builtin_amdgcn_alignbit(x1, x0, sh);  // shift = sh & 31
+    const uint32_t hi = __builtin_amdgcn_alignbit(x2, x1, sh);
+    return ((uint64_t)hi << 32) | lo;
   }
-  const uint32_t lo = __builtin_amdgcn_alignbit(x1, x0, bit);  // shift = bit & 31
-  const uint32_t hi = __builtin_amdgcn_alignbit(x2, x1, bit);
-  return ((uint64_t)hi << 32) | lo;
+  __device__ __forceinline__ void advance() { d += 2; }
+};
+
+template <class WP>
+struct BitCursor<WP, false> {
+  WP w;
+  uint32_t sh;
+  __device__ __forceinline__ BitCursor(WP p, uint32_t bit) : w(p + (bit >> 6)), sh(bit) {}
+  __device__ __forceinline__ uint64_t read() const {
+    const uint64_t w0 = w[0], w1 = w[1];
+    const bool upper = (sh & 32u) != 0;
+    const uint32_t x0 = upper ? (uint32_t)(w0 >> 32) : (uint32_t)w0;
+    const uint32_t x1 = upper ? (uint32_t)w1 : (uint32_t)(w0 >> 32);
+    const uint32_t x2 = upper ? (uint32_t)(w1 >> 32) : (uint32_t)w1;
+    const uint32_t lo = __builtin_amdgcn_alignbit(x1, x0, sh);
+    const uint32_t hi = __builtin_amdgcn_alignbit(x2, x1, sh);
+    return ((uint64_t)hi << 32) | lo;
+  }
+  __device__ __forceinline__ void advance() { w += 1; }
+};
+
+// Index of the lowest set bit of a 64-bit word, 64 when it is zero.
+__device__ __forceinline__ uint32_t lowest_bit64(uint64_t d) {
+  return (uint32_t)__builtin_ctzg(d, 64);
 }
 
 // Upstream `extend` closure (wfa_simd.rs:252-262, :608-616): unchanged outside
 // the matrix, else position + longest common prefix of the two suffixes.
 // 32 bases (or 8 raw bytes) are compared per 64-bit XOR; the first differing
-// symbol is the lowest set bit.  Each lane leaves at its own first
-// difference, so finished lanes issue no further reads.  (A wave-uniform
-// loop in which finished lanes keep re-reading a valid window measured
-// slower on every tier: 4% on 150 bp pairs in LDS, 2-8% in global memory.)
-// qsh / tsh: index of symbol 0 inside word 0 of each sequence.
+// symbol is the lowest set bit, and a window with no difference counts a
+// full word.  Each lane leaves at its own first difference (or at the end
+// of the shorter suffix), so finished lanes issue no further reads.  (A
+// wave-uniform loop in which finished lanes keep re-reading a valid window
+// measured slower on every tier: 4% on 150 bp pairs in LDS, 2-8% in global
+// memory.)  qsh / tsh: index of symbol 0 inside word 0 of each sequence.
 template <bool B8, class WP>
 __device__ __forceinline__ int32_t extend_diag(WP qw, uint32_t qsh, WP tw, uint32_t tsh,
                                                int32_t qn, int32_t tn, int32_t p, int32_t k) {
   constexpr uint32_t BITS = B8 ? 8 : 2;
   constexpr uint32_t CTZ_SHIFT = B8 ? 3 : 1;
-  constexpr int32_t SPW = B8 ? 8 : 32;
   const int32_t j = p - k;
-  if (p < 0 || p >= qn || j < 0 || j >= tn) return p;
+  if ((uint32_t)p >= (uint32_t)qn || (uint32_t)j >= (uint32_t)tn) return p;
   const int32_t lim = min(qn - p, tn - j);
+  BitCursor<WP> qc(qw, (qsh + (uint32_t)p) * BITS), tc(tw, (tsh + (uint32_t)j) * BITS);
   int32_t n = 0;
   for (;;) {
-    const uint64_t d = bit_window(qw, (qsh + (uint32_t)(p + n)) * BITS) ^
-                       bit_window(tw, (tsh + (uint32_t)(j + n)) * BITS);
-    if (d) {
-      n += (int32_t)((uint32_t)__builtin_ctzll(d) >> CTZ_SHIFT);
-      break;
-    }
-    n += SPW;
-    if (n >= lim) break;
+    const uint32_t z = lowest_bit64(qc.read() ^ tc.read());
+    n += (int32_t)(z >> CTZ_SHIFT);  // no difference: a whole word, 32 bases / 8 bytes
+    if (z < 64u || n >= lim) break;
+    qc.advance();
+    tc.advance();
   }
   return p + min(n, lim);
 }

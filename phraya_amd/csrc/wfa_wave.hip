@@ -272,7 +272,7 @@ struct PairShared {
 // frames move (one cross-lane permute per register) to a window of 128
 // diagonals, two per lane (kb2 + L and kb2 + 64 + L); the neighbours across
 // the two halves come from one lane read each.  A lane outside a step's
-// range holds -1 (unset) for that step, which is exactly what the ring
+// range holds "no cell" for that step, which is exactly what the ring
 // path's range checks produce, so no range checks are needed on reads.
 // History cells go to the ring path's LDS layout (cells at toff[step] + k -
 // lo), so the traceback is shared.  A pair whose range outgrows 128
@@ -281,19 +281,23 @@ struct PairShared {
 // ring at all (the host leaves it out of the layout), so more waves fit a
 // CU.
 
-// Value of the lane below (k - 1); lane 0 reads -1.  Call with every lane
-// active: a source lane outside EXEC also reads as -1.
+// Register frames hold P = offset + 1, with 0 for "no cell", so the lane
+// shifts below can fill the wave's ends with 0 through the DPP bound
+// control instead of first loading a -1 into the destination.
+
+// Value of the lane below (k - 1); lane 0 reads 0.  Call with every lane
+// active: a source lane outside EXEC also reads as 0.
 __device__ __forceinline__ int32_t from_below(int32_t v) {
-  return __builtin_amdgcn_update_dpp(-1, v, 0x138, 0xF, 0xF, false);  // wave_shr:1
+  return __builtin_amdgcn_mov_dpp(v, 0x138, 0xF, 0xF, true);  // wave_shr:1
 }
-// Value of the lane above (k + 1); lane 63 reads -1.
+// Value of the lane above (k + 1); lane 63 reads 0.
 __device__ __forceinline__ int32_t from_above(int32_t v) {
-  return __builtin_amdgcn_update_dpp(-1, v, 0x130, 0xF, 0xF, false);  // wave_shl:1
+  return __builtin_amdgcn_mov_dpp(v, 0x130, 0xF, 0xF, true);  // wave_shl:1
 }
 
 // Neighbours over a window of CH chunks of 64 diagonals (chunk c, lane L =
 // diagonal kb + 64 c + L): the lane below / above, across chunk edges too;
-// -1 past either end of the window.
+// 0 past either end of the window.
 template <int CH>
 __device__ __forceinline__ void neighbours(const int32_t (&v)[CH], int lane, int32_t (&below)[CH],
                                            int32_t (&above)[CH]) {
@@ -332,8 +336,8 @@ __device__ __forceinline__ void widen(int32_t v, int32_t d, int lane, int32_t (&
   const int32_t s0 = lane - d, s1 = lane + 64 - d;
   const int32_t p0 = __builtin_amdgcn_ds_bpermute((s0 & 63) << 2, v);
   const int32_t p1 = __builtin_amdgcn_ds_bpermute((s1 & 63) << 2, v);
-  out[0] = s0 >= 0 ? p0 : -1;
-  out[1] = s1 <= 63 ? p1 : -1;
+  out[0] = s0 >= 0 ? p0 : 0;
+  out[1] = s1 <= 63 ? p1 : 0;
 }
 
 // Writes the toff entries of the completed steps 1 .. done (<= 127) held
@@ -344,84 +348,166 @@ __device__ __forceinline__ void flush_toff(const WaveStore<true>& S, uint32_t t0
   if (64 + lane <= done) S.toff(64 + lane) = (uint16_t)t1;
 }
 
-// Gap-affine steps in registers over a window of CH chunks.  XN, OEN, EN:
-// mismatch, open + extend and extend costs in score steps; the m layer keeps
-// max(XN, OEN) frames, each gap layer EN.  Recurrence and tie rules as in
-// wave_affine's ring loop.  Returns false when the next step's range leaves
-// the window (the state is then that of the last completed step).
+// Frames of the gap-affine register wavefront, per chunk.  The m layer keeps
+// MD frames, each gap layer EN; I[c][EN - d] and D[c][EN - d] hold step
+// s - d while step s is computed.  The m layer rotates when MD is small:
+// step s writes its frame into slot R = (s - s0) mod MD, so step s - d sits
+// in slot (R - d) mod MD and no frame moves between registers (the step
+// loop is unrolled MD times, each copy with its R fixed).  Between phases,
+// and always when MD is large, the layout is that of R = 0: M[c][MD - d]
+// holds step s - d and the frames shift down one slot per step.
+template <int CH, int MD, int EN>
+struct AffineFrames {
+  int32_t M[CH][MD];
+  int32_t I[CH][EN];
+  int32_t D[CH][EN];
+};
+
+constexpr bool rotate_frames(int md) { return md <= 5; }
+
+template <int MD, int R>
+__device__ __forceinline__ constexpr int mslot(int d) {
+  return rotate_frames(MD) ? ((R - d) % MD + MD) % MD : MD - d;
+}
+
+// One gap-affine score step in registers over a window of CH chunks, with
+// the m layer at rotation R.  XN, OEN, EN: mismatch, open + extend and
+// extend costs in score steps.  Recurrence and tie rules as in wave_affine's
+// ring loop.  Returns 0 to go on, 1 when the step's range leaves the window
+// (step not taken; frames left in the R = 0 layout) and 2 when the pair is
+// done (end reached, step limit or history full).
+template <int R, int CH, int XN, int OEN, int EN, bool B8>
+__device__ __forceinline__ int affine_step(const WaveStore<true>& S, int32_t qn, int32_t tn,
+                                           int32_t kb, bool fitting, int32_t max_step, int lane,
+                                           AffineFrames<CH, (XN > OEN ? XN : OEN), EN>& F,
+                                           uint32_t& used, uint32_t& t0, uint32_t& t1,
+                                           StepState& st) {
+  constexpr int MD = XN > OEN ? XN : OEN;
+  const int32_t s = st.step + 1;
+  const int32_t lo = step_lo(s, tn), hi = step_hi(s, qn);
+  if (lo < kb || hi > kb + 64 * CH - 1 || s > 64 * CH - 1) {
+    if constexpr (rotate_frames(MD) && R != 0) {
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        int32_t t[MD];
+#pragma unroll
+        for (int d = 1; d <= MD; ++d) t[MD - d] = F.M[c][mslot<MD, R>(d)];
+#pragma unroll
+        for (int j = 0; j < MD; ++j) F.M[c][j] = t[j];
+      }
+    }
+    return 1;
+  }
+  st.step = s;
+  if (s > max_step) { st.status = ST_ALIGNMENT_FAILED; return 2; }
+  const uint32_t width = (uint32_t)(hi - lo + 1);
+  if ((uint32_t)s >= S.steps_cap || used + width > (uint32_t)S.cap) {
+    st.status = ST_OVERFLOW_HIST;
+    return 2;
+  }
+  if (s < 64) t0 = lane == s ? used : t0;
+  else t1 = lane == s - 64 ? used : t1;
+  int32_t om[CH], ie[CH], de[CH], vol[CH], vor[CH], vil[CH], vdr[CH], dummy[CH];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    om[c] = F.M[c][mslot<MD, R>(OEN)];
+    ie[c] = F.I[c][0];
+    de[c] = F.D[c][0];
+  }
+  neighbours<CH>(om, lane, vol, vor);
+  neighbours<CH>(ie, lane, vil, dummy);
+  neighbours<CH>(de, lane, dummy, vdr);
+  const int32_t kend_g = qn - tn;
+  const int32_t ke_lo = fitting ? max(kend_g, lo) : kend_g;
+  const int32_t ke_hi = fitting ? hi : kend_g;
+  const uint32_t uq = (uint32_t)qn;
+  uint64_t fmask = 0;
+  int32_t fc = 0, flayer = -1;
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const int32_t k = kb + 64 * c + lane;
+    const uint32_t kl = (uint32_t)(k - lo);
+    const bool inr = kl <= width - 1;
+    // Unsigned tests of P - 1 (the offset) against a bound also reject P = 0.
+    // tn + k >= 0 on every diagonal of the range, which is all that is kept.
+    const uint32_t utk = (uint32_t)(tn + k);
+    const int32_t vx = F.M[c][mslot<MD, R>(XN)];
+    int32_t ins = (uint32_t)(vol[c] - 1) < uq ? vol[c] + 1 : 0;
+    uint32_t iop = 0;
+    if ((uint32_t)(vil[c] - 1) < uq && vil[c] + 1 > ins) { ins = vil[c] + 1; iop = 1; }
+    int32_t del = (uint32_t)(vor[c] - 1) <= utk ? vor[c] : 0;
+    uint32_t dop = 0;
+    if ((uint32_t)(vdr[c] - 1) <= utk && vdr[c] > del) { del = vdr[c]; dop = 1; }
+    int32_t best = (uint32_t)(vx - 1) < min(uq, utk) ? vx + 1 : 0;
+    uint32_t mop = 0;
+    if (ins > best) { best = ins; mop = 1; }
+    if (del > best) { best = del; mop = 2; }
+    const int32_t mval = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, inr ? best - 1 : -1, k);
+    if (inr)
+      elem(S.cells, used + kl) = (uint16_t)(((uint32_t)best << 4) | mop | (iop << 2) | (dop << 3));
+    int32_t layer = -1;
+    if (inr && k >= ke_lo && k <= ke_hi)
+      layer = mval == qn ? 0 : ins == qn + 1 ? 1 : del == qn + 1 ? 2 : -1;
+    if constexpr (rotate_frames(MD)) {
+      F.M[c][R] = inr ? mval + 1 : 0;
+    } else {
+#pragma unroll
+      for (int j = 0; j + 1 < MD; ++j) F.M[c][j] = F.M[c][j + 1];
+      F.M[c][MD - 1] = inr ? mval + 1 : 0;
+    }
+#pragma unroll
+    for (int j = 0; j + 1 < EN; ++j) { F.I[c][j] = F.I[c][j + 1]; F.D[c][j] = F.D[c][j + 1]; }
+    F.I[c][EN - 1] = inr ? ins : 0;
+    F.D[c][EN - 1] = inr ? del : 0;
+    const uint64_t m = __ballot(layer >= 0);
+    if (m && !fmask) {  // the lowest diagonal reaching the end
+      fmask = m;
+      fc = c;
+      flayer = readlane(layer, (int32_t)__builtin_ctzll(m));
+    }
+  }
+  used += width;
+  if (fmask) {
+    st.found = true;
+    st.end_k = kb + 64 * fc + (int32_t)__builtin_ctzll(fmask);
+    st.end_layer = flayer;
+    return 2;
+  }
+  return 0;
+}
+
+// Steps R, R + 1, .. U - 1 of one pass of the unrolled step loop.
+template <int R, int U, int CH, int XN, int OEN, int EN, bool B8>
+__device__ __forceinline__ int affine_pass(const WaveStore<true>& S, int32_t qn, int32_t tn,
+                                           int32_t kb, bool fitting, int32_t max_step, int lane,
+                                           AffineFrames<CH, (XN > OEN ? XN : OEN), EN>& F,
+                                           uint32_t& used, uint32_t& t0, uint32_t& t1,
+                                           StepState& st) {
+  const int r = affine_step<R, CH, XN, OEN, EN, B8>(S, qn, tn, kb, fitting, max_step, lane, F, used,
+                                                    t0, t1, st);
+  if constexpr (R + 1 < U) {
+    if (r) return r;
+    return affine_pass<R + 1, U, CH, XN, OEN, EN, B8>(S, qn, tn, kb, fitting, max_step, lane, F,
+                                                      used, t0, t1, st);
+  }
+  return r;
+}
+
+// Gap-affine steps in registers over a window of CH chunks, until the pair
+// is done (true) or the next step's range leaves the window (false; the
+// state is then that of the last completed step, frames in the R = 0
+// layout).
 template <int CH, int XN, int OEN, int EN, bool B8>
 __device__ bool affine_steps(const WaveStore<true>& S, int32_t qn, int32_t tn, int32_t kb,
                              bool fitting, int32_t max_step, int lane,
-                             int32_t (&Mf)[CH][XN > OEN ? XN : OEN], int32_t (&If)[CH][EN],
-                             int32_t (&Df)[CH][EN], uint32_t& t0, uint32_t& t1, StepState& st) {
+                             AffineFrames<CH, (XN > OEN ? XN : OEN), EN>& F, uint32_t& used,
+                             uint32_t& t0, uint32_t& t1, StepState& st) {
   constexpr int MD = XN > OEN ? XN : OEN;
-  const int32_t kend_g = qn - tn;
+  constexpr int U = rotate_frames(MD) ? MD : 1;
   for (;;) {
-    const int32_t s = st.step + 1;
-    const int32_t lo = step_lo(s, tn), hi = step_hi(s, qn);
-    if (lo < kb || hi > kb + 64 * CH - 1 || s > 64 * CH - 1) return false;
-    st.step = s;
-    if (s > max_step) { st.status = ST_ALIGNMENT_FAILED; return true; }
-    const int32_t width = hi - lo + 1;
-    if (!S.fits(s, st.used, width)) { st.status = ST_OVERFLOW_HIST; return true; }
-    if (s < 64) t0 = lane == s ? (uint32_t)st.used : t0;
-    else t1 = lane == s - 64 ? (uint32_t)st.used : t1;
-    int32_t om[CH], ie[CH], de[CH], vol[CH], vor[CH], vil[CH], vdr[CH], dummy[CH];
-#pragma unroll
-    for (int c = 0; c < CH; ++c) {
-      om[c] = Mf[c][OEN - 1];
-      ie[c] = If[c][EN - 1];
-      de[c] = Df[c][EN - 1];
-    }
-    neighbours<CH>(om, lane, vol, vor);
-    neighbours<CH>(ie, lane, vil, dummy);
-    neighbours<CH>(de, lane, dummy, vdr);
-    const int32_t ke_lo = fitting ? max(kend_g, lo) : kend_g;
-    const int32_t ke_hi = fitting ? hi : kend_g;
-    uint64_t fmask = 0;
-    int32_t fc = 0, flayer = -1;
-#pragma unroll
-    for (int c = 0; c < CH; ++c) {
-      const int32_t k = kb + 64 * c + lane;
-      const bool inr = k >= lo && k <= hi;
-      const int32_t vx = Mf[c][XN - 1];
-      int32_t ins = (vol[c] >= 0 && vol[c] < qn) ? vol[c] + 1 : -1;
-      uint32_t iop = 0;
-      if (vil[c] >= 0 && vil[c] < qn && vil[c] + 1 > ins) { ins = vil[c] + 1; iop = 1; }
-      int32_t del = (vor[c] >= 0 && vor[c] - k <= tn) ? vor[c] : -1;
-      uint32_t dop = 0;
-      if (vdr[c] >= 0 && vdr[c] - k <= tn && vdr[c] > del) { del = vdr[c]; dop = 1; }
-      int32_t best = (vx >= 0 && vx < qn && vx + 1 - k <= tn) ? vx + 1 : -1;
-      uint32_t mop = 0;
-      if (ins > best) { best = ins; mop = 1; }
-      if (del > best) { best = del; mop = 2; }
-      const int32_t mval = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, inr ? best : -1, k);
-      if (inr)
-        S.cells[st.used + (uint64_t)(k - lo)] =
-            (uint16_t)(((uint32_t)(best + 1) << 4) | mop | (iop << 2) | (dop << 3));
-      int32_t layer = -1;
-      if (inr && k >= ke_lo && k <= ke_hi) layer = mval == qn ? 0 : ins == qn ? 1 : del == qn ? 2 : -1;
-#pragma unroll
-      for (int j = MD - 1; j > 0; --j) Mf[c][j] = Mf[c][j - 1];
-#pragma unroll
-      for (int j = EN - 1; j > 0; --j) { If[c][j] = If[c][j - 1]; Df[c][j] = Df[c][j - 1]; }
-      Mf[c][0] = inr ? mval : -1;
-      If[c][0] = inr ? ins : -1;
-      Df[c][0] = inr ? del : -1;
-      const uint64_t m = __ballot(layer >= 0);
-      if (m && !fmask) {  // the lowest diagonal reaching the end
-        fmask = m;
-        fc = c;
-        flayer = readlane(layer, (int32_t)__builtin_ctzll(m));
-      }
-    }
-    st.used += (uint64_t)width;
-    if (fmask) {
-      st.found = true;
-      st.end_k = kb + 64 * fc + (int32_t)__builtin_ctzll(fmask);
-      st.end_layer = flayer;
-      return true;
-    }
+    const int r = affine_pass<0, U, CH, XN, OEN, EN, B8>(S, qn, tn, kb, fitting, max_step, lane, F,
+                                                         used, t0, t1, st);
+    if (r) return r == 2;
   }
 }
 
@@ -433,32 +519,34 @@ __device__ void affine_regs(const WaveStore<true>& S, int32_t qn, int32_t tn, in
   constexpr int MD = XN > OEN ? XN : OEN;
   st = StepState{0, 0, false, 0, 0, ST_OK};
   const int32_t kb = reg_window_base(64, qn, tn);
-  int32_t Mf[1][MD], If[1][EN], Df[1][EN];  // [c][j] = frame of step (step - j)
+  AffineFrames<1, MD, EN> F;
 #pragma unroll
-  for (int j = 0; j < MD; ++j) Mf[0][j] = -1;
+  for (int j = 0; j < MD; ++j) F.M[0][j] = 0;
 #pragma unroll
-  for (int j = 0; j < EN; ++j) If[0][j] = Df[0][j] = -1;
-  Mf[0][0] = kb + lane == 0 ? m0 : -1;
-  uint32_t t0 = 0, t1 = 0;
-  bool done = affine_steps<1, XN, OEN, EN, B8>(S, qn, tn, kb, fitting, max_step, lane, Mf, If, Df,
-                                               t0, t1, st);
+  for (int j = 0; j < EN; ++j) F.I[0][j] = F.D[0][j] = 0;
+  F.M[0][MD - 1] = kb + lane == 0 ? m0 + 1 : 0;  // step 0, R = 0 layout
+  uint32_t used = 0, t0 = 0, t1 = 0;
+  bool done = affine_steps<1, XN, OEN, EN, B8>(S, qn, tn, kb, fitting, max_step, lane, F, used, t0,
+                                               t1, st);
   if (!done) {
     const int32_t kb2 = reg_window_base(128, qn, tn);
-    int32_t M2[2][MD], I2[2][EN], D2[2][EN], w[2];
+    AffineFrames<2, MD, EN> F2;
+    int32_t w[2];
 #pragma unroll
-    for (int j = 0; j < MD; ++j) { widen(Mf[0][j], kb - kb2, lane, w); M2[0][j] = w[0]; M2[1][j] = w[1]; }
+    for (int j = 0; j < MD; ++j) { widen(F.M[0][j], kb - kb2, lane, w); F2.M[0][j] = w[0]; F2.M[1][j] = w[1]; }
 #pragma unroll
     for (int j = 0; j < EN; ++j) {
-      widen(If[0][j], kb - kb2, lane, w); I2[0][j] = w[0]; I2[1][j] = w[1];
-      widen(Df[0][j], kb - kb2, lane, w); D2[0][j] = w[0]; D2[1][j] = w[1];
+      widen(F.I[0][j], kb - kb2, lane, w); F2.I[0][j] = w[0]; F2.I[1][j] = w[1];
+      widen(F.D[0][j], kb - kb2, lane, w); F2.D[0][j] = w[0]; F2.D[1][j] = w[1];
     }
-    done = affine_steps<2, XN, OEN, EN, B8>(S, qn, tn, kb2, fitting, max_step, lane, M2, I2, D2,
-                                            t0, t1, st);
+    done = affine_steps<2, XN, OEN, EN, B8>(S, qn, tn, kb2, fitting, max_step, lane, F2, used, t0,
+                                            t1, st);
     if (!done) {  // wider than 128 diagonals: the ring limit of this tier
       st.step += 1;
       st.status = ST_OVERFLOW_WIDTH;
     }
   }
+  st.used = used;
   flush_toff(S, t0, t1, st.status == ST_OK ? st.step : st.step - 1, lane);
   wave_sync<true>();
 }
@@ -484,11 +572,13 @@ __device__ __forceinline__ bool affine_regs_any(const WaveStore<true>& S, int32_
 }
 
 // Gap-linear steps in registers over a window of CH chunks (one frame
-// back); recurrence, bounds and tie rules as in wave_linear's ring loop.
+// back, P = offset + 1 as above); recurrence, bounds and tie rules as in
+// wave_linear's ring loop.
 template <int CH, bool B8>
 __device__ bool linear_steps(const WaveStore<true>& S, int32_t qn, int32_t tn, int32_t kb,
                              bool fitting, int32_t explore, int lane, int32_t (&Pf)[CH],
-                             uint32_t& t0, uint32_t& t1, StepState& st, bool& exhausted) {
+                             uint32_t& used, uint32_t& t0, uint32_t& t1, StepState& st,
+                             bool& exhausted) {
   const int32_t kend_global = qn - tn;
   for (;;) {
     const int32_t s = st.step + 1;
@@ -496,10 +586,13 @@ __device__ bool linear_steps(const WaveStore<true>& S, int32_t qn, int32_t tn, i
     if (lo < kb || hi > kb + 64 * CH - 1 || s > 64 * CH - 1) return false;
     st.step = s;
     if (s > explore) { exhausted = true; return true; }
-    const int32_t width = hi - lo + 1;
-    if (!S.fits(s, st.used, width)) { st.status = ST_OVERFLOW_HIST; return true; }
-    if (s < 64) t0 = lane == s ? (uint32_t)st.used : t0;
-    else t1 = lane == s - 64 ? (uint32_t)st.used : t1;
+    const uint32_t width = (uint32_t)(hi - lo + 1);
+    if ((uint32_t)s >= S.steps_cap || used + width > (uint32_t)S.cap) {
+      st.status = ST_OVERFLOW_HIST;
+      return true;
+    }
+    if (s < 64) t0 = lane == s ? used : t0;
+    else t1 = lane == s - 64 ? used : t1;
     int32_t below[CH], above[CH];
     neighbours<CH>(Pf, lane, below, above);
     const int32_t ke_lo = fitting ? max(qn - tn, lo) : kend_global;
@@ -509,28 +602,29 @@ __device__ bool linear_steps(const WaveStore<true>& S, int32_t qn, int32_t tn, i
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       const int32_t k = kb + 64 * c + lane;
-      const bool inr = k >= lo && k <= hi;
+      const uint32_t kl = (uint32_t)(k - lo);
+      const bool inr = kl <= width - 1;
       const int32_t vm = Pf[c];
-      const int32_t vl = k > -tn ? below[c] : -1;
-      const int32_t vr = k < qn ? above[c] : -1;
-      int32_t best = -1;
+      const int32_t vl = k > -tn ? below[c] : 0;
+      const int32_t vr = k < qn ? above[c] : 0;
+      int32_t best = 0;
       uint32_t op = 0;
-      if (vm >= 0) { best = vm + 1; op = 1; }
-      if (vl >= 0 && vl + 1 >= best) { best = vl + 1; op = 2; }
-      if (vr >= 0 && vr >= best) { best = vr; op = 3; }
-      const bool ok = op != 0 && best <= qn && best - k >= 0 && best - k <= tn;
-      const int32_t val = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, inr && ok ? best : -1, k);
-      if (inr)
-        S.cells[st.used + (uint64_t)(k - lo)] = (uint16_t)(ok ? (((uint32_t)(best + 1) << 4) | op) : 0u);
+      if (vm > 0) { best = vm + 1; op = 1; }
+      if (vl > 0 && vl + 1 >= best) { best = vl + 1; op = 2; }
+      if (vr > 0 && vr >= best) { best = vr; op = 3; }
+      const bool ok = op != 0 && best <= qn + 1 && (uint32_t)(best - 1 - k) <= (uint32_t)tn;
+      const int32_t val =
+          extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, inr && ok ? best - 1 : -1, k);
+      if (inr) elem(S.cells, used + kl) = (uint16_t)(ok ? (((uint32_t)best << 4) | op) : 0u);
       const bool hit = inr && ok && k >= ke_lo && k <= ke_hi && val >= qn;
-      Pf[c] = inr ? val : -1;
+      Pf[c] = inr ? val + 1 : 0;
       const uint64_t m = __ballot(hit);
       if (m && !fmask) {
         fmask = m;
         fc = c;
       }
     }
-    st.used += (uint64_t)width;
+    used += width;
     if (fmask) {
       st.found = true;
       st.end_k = kb + 64 * fc + (int32_t)__builtin_ctzll(fmask);
@@ -545,19 +639,22 @@ __device__ void linear_regs(const WaveStore<true>& S, int32_t qn, int32_t tn, in
                             bool fitting, int32_t explore, int lane, StepState& st, bool& exhausted) {
   st = StepState{0, 0, false, 0, 0, ST_OK};
   const int32_t kb = reg_window_base(64, qn, tn);
-  int32_t Pf[1] = {kb + lane == 0 ? w0 : -1};
-  uint32_t t0 = 0, t1 = 0;
-  bool done = linear_steps<1, B8>(S, qn, tn, kb, fitting, explore, lane, Pf, t0, t1, st, exhausted);
+  int32_t Pf[1] = {kb + lane == 0 ? w0 + 1 : 0};
+  uint32_t used = 0, t0 = 0, t1 = 0;
+  bool done =
+      linear_steps<1, B8>(S, qn, tn, kb, fitting, explore, lane, Pf, used, t0, t1, st, exhausted);
   if (!done) {
     const int32_t kb2 = reg_window_base(128, qn, tn);
     int32_t P2[2];
     widen(Pf[0], kb - kb2, lane, P2);
-    done = linear_steps<2, B8>(S, qn, tn, kb2, fitting, explore, lane, P2, t0, t1, st, exhausted);
+    done = linear_steps<2, B8>(S, qn, tn, kb2, fitting, explore, lane, P2, used, t0, t1, st,
+                               exhausted);
     if (!done) {
       st.step += 1;
       st.status = ST_OVERFLOW_WIDTH;
     }
   }
+  st.used = used;
   flush_toff(S, t0, t1, st.status == ST_OK && !exhausted ? st.step : st.step - 1, lane);
   wave_sync<true>();
 }
@@ -588,7 +685,7 @@ __device__ __forceinline__ void ring_fill_origin(int32_t* ring, uint32_t W, int3
 // --- affine --------------------------------------------------------------------
 
 template <bool LDS, bool B8, int NT, bool RL = false>
-__device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_base,
+__device__ __forceinline__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_base,
                             const PairDesc& pd, PairOut* po, int lane, PairShared* sh) {
   using S_t = WaveStore<LDS, RL>;
   const int32_t qn = pd.qn, tn = pd.tn;
@@ -814,7 +911,7 @@ __device__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_ba
 // --- linear --------------------------------------------------------------------
 
 template <bool LDS, bool B8, int NT, bool RL = false>
-__device__ void wave_linear(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_base,
+__device__ __forceinline__ void wave_linear(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_base,
                             const PairDesc& pd, PairOut* po, int lane, PairShared* sh) {
   using S_t = WaveStore<LDS, RL>;
   const int32_t qn = pd.qn, tn = pd.tn;
