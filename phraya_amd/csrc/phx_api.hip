@@ -213,8 +213,7 @@ int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint3
   a.nwork = (uint32_t)work.size();
   a.counters = d_counters;
   a.overflow = overflow;
-  a.cp = CostParams{P->costs.mismatch, P->costs.gap_open, P->costs.gap_extend,
-                    P->cap < 0 ? -1 : (int32_t)std::min<int64_t>(P->cap, INT32_MAX)};
+  a.cp = cost_params(P->costs, P->cap < 0 ? -1 : (int32_t)std::min<int64_t>(P->cap, INT32_MAX));
   a.out = (PairOut*)P->d_out.p;
   a.runs = (uint32_t*)P->d_runs.p;
   const int mode = P->engine == PHX_ENGINE_AFFINE ? ENG_AFFINE : ENG_LINEAR;

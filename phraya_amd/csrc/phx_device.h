@@ -76,6 +76,9 @@ struct PairOut {
 struct CostParams {
   int32_t x, o, e;  // mismatch, gap open, gap extend (affine); unused for linear
   int32_t cap;      // max score explored; -1 = none
+  // Affine: g = gcd(x, o, e) and the costs in score steps of g (mismatch,
+  // open + extend, extend), computed once by the host (cost_params).
+  int32_t g, xn, oen, en;
 };
 
 // CIGAR run encoding: (length << 4) | op.

@@ -108,6 +108,15 @@ inline int32_t gcd3(int32_t a, int32_t b, int32_t c) {
   return g2(g2(a, b), c);
 }
 
+// Kernel cost arguments: the costs, the score cap and their score-step form
+// (valid affine costs have g >= 1; other costs get g = 1, unused).
+inline phx::CostParams cost_params(const phx_costs& c, int32_t cap) {
+  int32_t g = gcd3(c.mismatch, c.gap_open, c.gap_extend);
+  if (g < 1) g = 1;
+  return phx::CostParams{c.mismatch, c.gap_open, c.gap_extend, cap, g, c.mismatch / g,
+                         (c.gap_open + c.gap_extend) / g, c.gap_extend / g};
+}
+
 // Score frames the affine ring spans: max(x, o+e, e)/g + 1 (linear: 2).
 inline int32_t ring_slots(int engine, const phx_costs& c) {
   if (engine != PHX_ENGINE_AFFINE) return 2;

@@ -349,7 +349,7 @@ bool launch(phx_server* S, Lane& L, std::vector<Done>& fin) {
   a.work_list = (const uint32_t*)L.d_work.p;
   a.nwork = n;
   a.counters = (uint32_t*)L.d_ctr.p;
-  a.cp = CostParams{S->costs.mismatch, S->costs.gap_open, S->costs.gap_extend, S->cap};
+  a.cp = cost_params(S->costs, S->cap);
   a.out = (PairOut*)out_dev;
   a.runs = (uint32_t*)runs_dev;
   WaveGeom G{};

@@ -585,7 +585,7 @@ __device__ bool linear_steps(const WaveStore<true>& S, int32_t qn, int32_t tn, i
     const int32_t lo = step_lo(s, tn), hi = step_hi(s, qn);
     if (lo < kb || hi > kb + 64 * CH - 1 || s > 64 * CH - 1) return false;
     st.step = s;
-    if (s > explore) { exhausted = true; return true; }
+      if (s > explore) { exhausted = true; return true; }
     const uint32_t width = (uint32_t)(hi - lo + 1);
     if ((uint32_t)s >= S.steps_cap || used + width > (uint32_t)S.cap) {
       st.status = ST_OVERFLOW_HIST;
@@ -696,16 +696,16 @@ __device__ __forceinline__ void wave_affine(const WfaArgs& a, const WaveGeom& G,
   const S_t S = make_store<LDS, B8, NT, RL>(a, G, lds_base, pd, lane);
   const int32_t x = a.cp.x, o = a.cp.o, e = a.cp.e;
   const bool fitting = tn > qn + qn / 2 + 10;
-  int32_t g = x;
-  for (int32_t b = o; b != 0;) { const int32_t r = g % b; g = b; b = r; }
-  for (int32_t b = e; b != 0;) { const int32_t r = g % b; g = b; b = r; }
-  const int32_t xn = x / g, oen = (o + e) / g, en = e / g;
+  const int32_t g = a.cp.g, xn = a.cp.xn, oen = a.cp.oen, en = a.cp.en;
   // The m layer reads xn and oen steps back, the gap layers en steps back:
   // Rm and Rg frames of ring (the host sizes the ring as Rm + 2 Rg frames).
   const int32_t Rm = max(xn, oen) + 1, Rg = en + 1;
   const int64_t hard_max = (int64_t)(qn + tn) * max(x, 1) + o + e + 4;
   const int64_t explore_to = a.cp.cap >= 0 ? min((int64_t)a.cp.cap, hard_max) : hard_max;
-  const int32_t max_step = (int32_t)(explore_to / g);
+  // A 32-bit division when the bound fits (it does for every pair an LDS or
+  // HBM slot holds); the quotient is the last score step explored.
+  const int32_t max_step = explore_to <= INT32_MAX ? (int32_t)explore_to / g
+                                                   : (int32_t)min<int64_t>(explore_to / g, INT32_MAX);
   const uint32_t W = S.wmask + 1;
   typename S_t::ring_t* Mr = S.ring;
   typename S_t::ring_t* Ri = S.ring + (size_t)Rm * W;
@@ -748,8 +748,10 @@ __device__ __forceinline__ void wave_affine(const WfaArgs& a, const WaveGeom& G,
       end_k = st.end_k;
       end_layer = st.end_layer;
       status = st.status;
-      slot = __builtin_amdgcn_readfirstlane(step % Rm);
-      gslot = __builtin_amdgcn_readfirstlane(step % Rg);
+      if (!found && status == ST_OK) {  // the ring loop goes on from this step
+        slot = __builtin_amdgcn_readfirstlane(step % Rm);
+        gslot = __builtin_amdgcn_readfirstlane(step % Rg);
+      }
     }
   }
   while (!found && status == ST_OK) {
