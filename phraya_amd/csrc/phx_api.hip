@@ -79,14 +79,17 @@ struct phx_plan {
   }
   std::vector<PairOut> out_h;
   bool out_valid = false;
-  double last_ms = 0;
-  double last_kernel_ms = 0;
   uint32_t last_launches = 0;
-  // Timing of the current run: launch 0 may start at ctx->ev0 (nothing else
-  // between them on the stream), and the run may end at an event recorded
-  // by run_wfa (null: phx_plan_run records ctx->ev1).
-  bool first_start_ev0 = false;
-  hipEvent_t end_ev = nullptr;
+  // Timing of the last run, from the plan's own events (so that runs of other
+  // plans on the context leave it alone): a pair of events around every
+  // kernel launch, the run's start (null: the first launch's start event)
+  // and its end.  Elapsed times are read from the events only when asked
+  // for, not on every run.
+  std::vector<hipEvent_t> kev;
+  hipEvent_t ev_start = nullptr, ev_end = nullptr;
+  hipEvent_t start_ev = nullptr, end_ev = nullptr;
+  mutable bool ms_ready = true, kernel_ms_ready = true;
+  mutable double last_ms = 0, last_kernel_ms = 0;
 };
 
 namespace {
@@ -123,15 +126,23 @@ void dev_free(DevBuf& b) {
   b.bytes = 0;
 }
 
-// Event pair number i of the context (created on first use).
-int kernel_events(phx_ctx* ctx, uint32_t i, hipEvent_t* a, hipEvent_t* b) {
-  while (ctx->kev.size() < 2 * (size_t)(i + 1)) {
+// Event pair number i of the plan (created on first use).
+int kernel_events(phx_plan* P, uint32_t i, hipEvent_t* a, hipEvent_t* b) {
+  phx_ctx* ctx = P->ctx;
+  while (P->kev.size() < 2 * (size_t)(i + 1)) {
     hipEvent_t e;
     HIPCHK(hipEventCreate(&e));
-    ctx->kev.push_back(e);
+    P->kev.push_back(e);
   }
-  *a = ctx->kev[2 * i];
-  *b = ctx->kev[2 * i + 1];
+  *a = P->kev[2 * i];
+  *b = P->kev[2 * i + 1];
+  return PHX_OK;
+}
+
+// The plan's run start / end event (created on first use).
+int plan_event(phx_plan* P, hipEvent_t* e) {
+  phx_ctx* ctx = P->ctx;
+  if (!*e) HIPCHK(hipEventCreate(e));
   return PHX_OK;
 }
 
@@ -197,10 +208,9 @@ GroupSlots group_slots(const phx_plan* P, int t, const std::vector<uint32_t>& wo
 
 // Launches one group: tier t over `work` on stream `st`; HBM tiers use the
 // scratch slice [slice, slice + gs.n * gs.per).
-// Timing events around a launch: both, the end only (the start is the run's
-// ctx->ev0, recorded right before on the same stream), or none (graph
-// capture: event nodes are added around the kernel node).
-enum LaunchTiming { TIME_BOTH, TIME_END, TIME_NONE };
+// Timing events around a launch: both, or none (graph capture: event nodes
+// are added around the kernel node).
+enum LaunchTiming { TIME_BOTH, TIME_NONE };
 
 int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint32_t* d_list,
                     uint32_t* d_counters, uint32_t* overflow, const GroupSlots& gs, uint8_t* slice,
@@ -241,11 +251,10 @@ int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint3
     return PHX_OK;
   }
   hipEvent_t e0, e1;
-  if (kernel_events(ctx, P->last_launches, &e0, &e1) != PHX_OK) return PHX_E_DEVICE;
-  if (timing == TIME_BOTH) HIPCHK(hipEventRecord(e0, st));
+  if (kernel_events(P, P->last_launches, &e0, &e1) != PHX_OK) return PHX_E_DEVICE;
+  HIPCHK(hipEventRecord(e0, st));
   HIPCHK(launch_tier(t, mode, a, G, slots, st, rl));
   HIPCHK(hipEventRecord(e1, st));
-  if (P->last_launches == 0) P->first_start_ev0 = timing == TIME_END;
   ++P->last_launches;
   return PHX_OK;
 }
@@ -273,8 +282,14 @@ int issue_groups(phx_plan* P, bool capture, std::vector<int>* launched,
   if (queued_group) HIPCHK(launch_zero(d_ctr, 2 * (uint32_t)ngroups, ctx->stream));
   int nonempty = 0;
   for (int gi = 0; gi < ngroups; ++gi) nonempty += !P->work[gi].empty();
-  // One launch and nothing else on the stream: its end event ends the run.
+  // One launch and nothing else on the stream: its events start and end the
+  // run.
   const bool single = !capture && !queued_group && nonempty == 1;
+  if (!single && !capture) {
+    if (plan_event(P, &P->ev_start) != PHX_OK) return PHX_E_DEVICE;
+    HIPCHK(hipEventRecord(P->ev_start, ctx->stream));
+    P->start_ev = P->ev_start;
+  }
   if (!single) HIPCHK(hipEventRecord(ctx->join[0], ctx->stream));
   // Group gi's list sits at offset group_off[gi] of d_work.
   std::vector<uint64_t> group_off(ngroups + 1, 0);
@@ -334,11 +349,12 @@ int issue_groups(phx_plan* P, bool capture, std::vector<int>* launched,
   }
   if (!capture) {
     hipEvent_t e0, e1;
-    if (single && kernel_events(ctx, 0, &e0, &e1) == PHX_OK) {
+    if (single && kernel_events(P, 0, &e0, &e1) == PHX_OK) {
       P->end_ev = e1;
     } else {
-      HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
-      P->end_ev = ctx->ev1;
+      if (plan_event(P, &P->ev_end) != PHX_OK) return PHX_E_DEVICE;
+      HIPCHK(hipEventRecord(P->ev_end, ctx->stream));
+      P->end_ev = P->ev_end;
     }
   }
   return PHX_OK;
@@ -366,7 +382,7 @@ int build_graph(phx_plan* P) {
   int ret = PHX_OK;
   for (size_t i = 0; i < knodes.size() && ret == PHX_OK; ++i) {
     hipEvent_t e0, e1;
-    if (kernel_events(ctx, (uint32_t)i, &e0, &e1) != PHX_OK) ret = PHX_E_DEVICE;
+    if (kernel_events(P, (uint32_t)i, &e0, &e1) != PHX_OK) ret = PHX_E_DEVICE;
     size_t nd = 0;
     std::vector<hipGraphNode_t> deps;
     hipGraphNode_t n0 = nullptr, n1 = nullptr;
@@ -400,8 +416,9 @@ int run_wfa(phx_plan* P) {
   std::vector<int> launched;
   if (P->use_graph && P->gexec && P->g_arena == P->d_arena.p) {
     HIPCHK(hipGraphLaunch(P->gexec, ctx->stream));
-    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
-    P->end_ev = ctx->ev1;
+    if (plan_event(P, &P->ev_end) != PHX_OK) return PHX_E_DEVICE;
+    HIPCHK(hipEventRecord(P->ev_end, ctx->stream));
+    P->end_ev = P->ev_end;
     launched = P->g_launched;
     P->last_launches = P->g_launches;
   } else {
@@ -455,8 +472,9 @@ int run_wfa(phx_plan* P) {
     int rc = launch_wfa_tier(P, rt, wl, (uint32_t*)P->d_work2.p, d_ctr, P->ctr_dev, rg,
                              (uint8_t*)P->d_arena.p, ctx->stream);
     if (rc != PHX_OK) return rc;
-    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
-    P->end_ev = ctx->ev1;
+    if (plan_event(P, &P->ev_end) != PHX_OK) return PHX_E_DEVICE;
+    HIPCHK(hipEventRecord(P->ev_end, ctx->stream));
+    P->end_ev = P->ev_end;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     std::vector<uint32_t> again;
     if (ctr_h[0] != 0) {
@@ -721,7 +739,7 @@ int run_myers(phx_plan* P) {
       a.engine = P->engine == PHX_ENGINE_MYERS ? ENG_MYERS
                : P->engine == PHX_ENGINE_MYERS_GLOBAL ? ENG_MYERS_GLOBAL : ENG_MYERS_DIST;
       hipEvent_t e0, e1;
-      if (kernel_events(ctx, P->last_launches, &e0, &e1) != PHX_OK) return PHX_E_DEVICE;
+      if (kernel_events(P, P->last_launches, &e0, &e1) != PHX_OK) return PHX_E_DEVICE;
       HIPCHK(hipEventRecord(e0, ctx->stream));
       HIPCHK(launch_myers(nbt, a, ctx->stream));
       HIPCHK(hipEventRecord(e1, ctx->stream));
@@ -821,12 +839,6 @@ phx_ctx* phx_ctx_create(int device, uint32_t flags, const phx_tuning* tuning) {
   for (hipEvent_t& e : c->join) {
     if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) { delete c; return nullptr; }
   }
-  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
-    for (hipEvent_t e : c->join) (void)hipEventDestroy(e);
-  for (int i = 0; i < kStreams; ++i) (void)hipStreamDestroy(c->streams[i]);
-    delete c;
-    return nullptr;
-  }
   phx_tuning d{};
   d.wave_lds_kib = 9;  // 4 workgroups x 4 waves x 9 KiB fit a CU's 160 KiB
   d.wave_slots = (uint32_t)c->cu_count * 16;
@@ -848,9 +860,6 @@ phx_ctx* phx_ctx_create(int device, uint32_t flags, const phx_tuning* tuning) {
 void phx_ctx_destroy(phx_ctx* c) {
   if (!c) return;
   (void)hipStreamSynchronize(c->stream);
-  (void)hipEventDestroy(c->ev0);
-  (void)hipEventDestroy(c->ev1);
-  for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
   for (hipEvent_t e : c->join) (void)hipEventDestroy(e);
   for (int i = 0; i < kStreams; ++i) (void)hipStreamDestroy(c->streams[i]);
   delete c;
@@ -966,27 +975,18 @@ int phx_plan_run(phx_plan* P) {
   HIPCHK(hipSetDevice(ctx->device));
   P->out_valid = false;
   P->last_launches = 0;
-  P->first_start_ev0 = false;
+  P->start_ev = nullptr;
   P->end_ev = nullptr;
-  HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+  P->ms_ready = P->kernel_ms_ready = false;
   int rc = is_wfa(P->engine) ? run_wfa(P) : run_myers(P);
   if (rc != PHX_OK) return rc;
-  if (!P->end_ev) {
-    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
-    P->end_ev = ctx->ev1;
+  if (!P->end_ev && P->last_launches > 0) {
+    if (plan_event(P, &P->ev_end) != PHX_OK) return PHX_E_DEVICE;
+    HIPCHK(hipEventRecord(P->ev_end, ctx->stream));
+    P->end_ev = P->ev_end;
   }
-  HIPCHK(hipEventSynchronize(P->end_ev));
-  float ms = 0;
-  HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, P->end_ev));
-  P->last_ms = ms;
-  double kms = 0;
-  for (uint32_t i = 0; i < P->last_launches; ++i) {
-    float k = 0;
-    const hipEvent_t start = i == 0 && P->first_start_ev0 ? ctx->ev0 : ctx->kev[2 * i];
-    HIPCHK(hipEventElapsedTime(&k, start, ctx->kev[2 * i + 1]));
-    kms += k;
-  }
-  P->last_kernel_ms = kms;
+  if (P->end_ev) HIPCHK(hipEventSynchronize(P->end_ev));
+  else HIPCHK(hipStreamSynchronize(ctx->stream));
   if (P->graph_wanted) {  // outside the timed region
     P->graph_wanted = false;
     return build_graph(P);
@@ -1005,8 +1005,36 @@ int phx_plan_use_graph(phx_plan* P, int enable) {
   return PHX_OK;
 }
 
-double phx_plan_last_ms(const phx_plan* P) { return P ? P->last_ms : 0.0; }
-double phx_plan_last_kernel_ms(const phx_plan* P) { return P ? P->last_kernel_ms : 0.0; }
+// Device time of the last run: from its start (its first launch's start
+// event when it is one launch) to its end.
+double phx_plan_last_ms(const phx_plan* P) {
+  if (!P) return 0.0;
+  if (!P->ms_ready) {
+    float ms = 0;
+    const hipEvent_t start = P->start_ev ? P->start_ev : P->kev.empty() ? nullptr : P->kev[0];
+    if (P->last_launches > 0 && start && P->end_ev &&
+        hipEventElapsedTime(&ms, start, P->end_ev) != hipSuccess)
+      ms = 0;
+    P->last_ms = ms;
+    P->ms_ready = true;
+  }
+  return P->last_ms;
+}
+
+// Sum of the kernel times of the last run (events around each launch).
+double phx_plan_last_kernel_ms(const phx_plan* P) {
+  if (!P) return 0.0;
+  if (!P->kernel_ms_ready) {
+    double kms = 0;
+    for (uint32_t i = 0; i < P->last_launches && 2 * (size_t)i + 1 < P->kev.size(); ++i) {
+      float k = 0;
+      if (hipEventElapsedTime(&k, P->kev[2 * i], P->kev[2 * i + 1]) == hipSuccess) kms += k;
+    }
+    P->last_kernel_ms = kms;
+    P->kernel_ms_ready = true;
+  }
+  return P->last_kernel_ms;
+}
 uint32_t phx_plan_last_launches(const phx_plan* P) { return P ? P->last_launches : 0; }
 
 int phx_plan_set_tuning(phx_plan* P, const phx_tuning* t) {
@@ -1051,7 +1079,7 @@ int phx_plan_autotune(phx_plan* P, uint32_t max_trials, phx_tuning* best_out, do
       rc = phx_plan_run(P);
       if (rc != PHX_OK) return rc;
       if (P->d_arena.bytes != arena) continue;
-      best = std::min(best, P->last_ms);
+      best = std::min(best, phx_plan_last_ms(P));
       if (best > 20.0 || r >= 3) break;
     }
     *ms = best;
@@ -1226,6 +1254,9 @@ void phx_plan_destroy(phx_plan* P) {
     dev_free(*b);
   dev_free(P->d_arena);
   if (P->gexec) (void)hipGraphExecDestroy(P->gexec);
+  for (hipEvent_t e : P->kev) (void)hipEventDestroy(e);
+  for (hipEvent_t e : {P->ev_start, P->ev_end})
+    if (e) (void)hipEventDestroy(e);
   if (P->ctr_host) (void)hipHostFree(P->ctr_host);
   delete P;
 }

@@ -375,8 +375,6 @@ struct phx_ctx {
   hipStream_t stream = nullptr;         // = streams[0]
   hipStream_t streams[kStreams] = {};
   hipEvent_t join[32] = {};             // cross-stream joins
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  std::vector<hipEvent_t> kev;  // pairs of events around each kernel launch of a run
   std::mutex mu;
   phx_tuning tun{};
   std::string err;

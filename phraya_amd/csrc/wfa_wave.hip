@@ -418,8 +418,10 @@ __device__ __forceinline__ int affine_step(const WaveStore<true>& S, int32_t qn,
   neighbours<CH>(ie, lane, vil, dummy);
   neighbours<CH>(de, lane, dummy, vdr);
   const int32_t kend_g = qn - tn;
-  const int32_t ke_lo = fitting ? max(kend_g, lo) : kend_g;
-  const int32_t ke_hi = fitting ? hi : kend_g;
+  const int32_t ke_lo = max(kend_g, lo);  // fitting: any diagonal from qn - tn up
+  // Global: the end cell is on diagonal qn - tn only; its chunk and lane.
+  const int32_t ke_off = kend_g - kb;
+  const bool ke_in = kend_g >= lo && kend_g <= hi;
   const uint32_t uq = (uint32_t)qn;
   uint64_t fmask = 0;
   int32_t fc = 0, flayer = -1;
@@ -445,9 +447,29 @@ __device__ __forceinline__ int affine_step(const WaveStore<true>& S, int32_t qn,
     const int32_t mval = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, inr ? best - 1 : -1, k);
     if (inr)
       elem(S.cells, used + kl) = (uint16_t)(((uint32_t)best << 4) | mop | (iop << 2) | (dop << 3));
-    int32_t layer = -1;
-    if (inr && k >= ke_lo && k <= ke_hi)
-      layer = mval == qn ? 0 : ins == qn + 1 ? 1 : del == qn + 1 ? 2 : -1;
+    if (fitting) {
+      int32_t layer = -1;
+      if (inr && k >= ke_lo) layer = mval == qn ? 0 : ins == qn + 1 ? 1 : del == qn + 1 ? 2 : -1;
+      const uint64_t m = __ballot(layer >= 0);
+      if (m && !fmask) {  // the lowest diagonal reaching the end
+        fmask = m;
+        fc = c;
+        flayer = readlane(layer, (int32_t)__builtin_ctzll(m));
+      }
+    } else if (ke_in && (ke_off >> 6) == c && !fmask) {
+      // Global: three lane reads of the end diagonal instead of a test on
+      // every lane.
+      const int32_t ln = ke_off & 63;
+      const int32_t layer = readlane(mval, ln) == qn           ? 0
+                            : readlane(ins, ln) == qn + 1 ? 1
+                            : readlane(del, ln) == qn + 1 ? 2
+                                                          : -1;
+      if (layer >= 0) {
+        fmask = 1ull << ln;
+        fc = c;
+        flayer = layer;
+      }
+    }
     if constexpr (rotate_frames(MD)) {
       F.M[c][R] = inr ? mval + 1 : 0;
     } else {
@@ -459,12 +481,6 @@ __device__ __forceinline__ int affine_step(const WaveStore<true>& S, int32_t qn,
     for (int j = 0; j + 1 < EN; ++j) { F.I[c][j] = F.I[c][j + 1]; F.D[c][j] = F.D[c][j + 1]; }
     F.I[c][EN - 1] = inr ? ins : 0;
     F.D[c][EN - 1] = inr ? del : 0;
-    const uint64_t m = __ballot(layer >= 0);
-    if (m && !fmask) {  // the lowest diagonal reaching the end
-      fmask = m;
-      fc = c;
-      flayer = readlane(layer, (int32_t)__builtin_ctzll(m));
-    }
   }
   used += width;
   if (fmask) {
