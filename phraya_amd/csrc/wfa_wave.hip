@@ -1101,10 +1101,14 @@ __device__ __forceinline__ void wave_linear(const WfaArgs& a, const WaveGeom& G,
 
 // --- kernel --------------------------------------------------------------------
 
-// Up to eight workgroups per CU (32 waves): registers are held to 64 per lane
-// so that occupancy is set by the LDS the tuning gives each wave.
+// Up to seven workgroups per CU (28 waves, 72 registers per lane): the
+// gap-affine register wavefront needs about 71 registers (held to 64 it
+// spilled a few values to scratch, about 1.5 KiB of writes per wave), and
+// the tuning that fills a CU with short pairs gives each wave 5 KiB of LDS,
+// which holds 28 waves anyway.  Kernels that need fewer registers (linear)
+// still fit eight workgroups.
 template <int MODE, bool LDS>
-__global__ __launch_bounds__(64 * WV_WAVES, 8) void wfa_wave_kernel(WfaArgs a, WaveGeom G) {
+__global__ __launch_bounds__(64 * WV_WAVES, 7) void wfa_wave_kernel(WfaArgs a, WaveGeom G) {
   extern __shared__ __align__(16) uint8_t wave_lds_pool[];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int32_t)(threadIdx.x >> 6));
