@@ -57,11 +57,16 @@ struct BitCursor<WP, true> {
 
 template <class WP>
 struct BitCursor<WP, false> {
-  WP w;
+  WP base;       // the stream's first word (wave-uniform in the HBM tiers)
+  uint32_t off;  // byte offset of the current word: base + 32-bit offset
+                 // addressing, no 64-bit pointer per lane
   uint32_t sh;
-  __device__ __forceinline__ BitCursor(WP p, uint32_t bit) : w(p + (bit >> 6)), sh(bit) {}
+  __device__ __forceinline__ BitCursor(WP p, uint32_t bit) : base(p), off((bit >> 6) << 3), sh(bit) {}
+  __device__ __forceinline__ uint64_t word(uint32_t o) const {
+    return *reinterpret_cast<WP>(reinterpret_cast<const gmem_t<const uint8_t>*>(base) + o);
+  }
   __device__ __forceinline__ uint64_t read() const {
-    const uint64_t w0 = w[0], w1 = w[1];
+    const uint64_t w0 = word(off), w1 = word(off + 8);
     const bool upper = (sh & 32u) != 0;
     const uint32_t x0 = upper ? (uint32_t)(w0 >> 32) : (uint32_t)w0;
     const uint32_t x1 = upper ? (uint32_t)w1 : (uint32_t)(w0 >> 32);
@@ -70,7 +75,7 @@ struct BitCursor<WP, false> {
     const uint32_t hi = __builtin_amdgcn_alignbit(x2, x1, sh);
     return ((uint64_t)hi << 32) | lo;
   }
-  __device__ __forceinline__ void advance() { w += 1; }
+  __device__ __forceinline__ void advance() { off += 8; }
 };
 
 // Index of the lowest set bit of a 64-bit word, 64 when it is zero.
