@@ -51,52 +51,49 @@ __device__ __forceinline__ uint64_t peq8(const uint64_t* qw, uint32_t b, uint32_
 }
 
 // One Myers column over NB blocks held in arrays (registers when NB is a
-// compile-time constant, HBM otherwise).  Returns the score delta.
+// compile-time constant, HBM otherwise).  Returns the score delta.  Bits at
+// or above the query length m in the last block are not cleared: every
+// operation of the update moves information only upward (the addition's
+// carries, the left shifts), so they never reach the score bit m - 1, and
+// the traceback masks them out of its column sums (col_value).
 template <typename Eq>
 __device__ __forceinline__ int32_t myers_column(uint64_t* vp, uint64_t* vn, uint32_t nb,
-                                                uint64_t last_mask, bool last_partial,
                                                 uint32_t score_bit, Eq eq_of) {
-  uint64_t add_carry = 0, hp_carry = 1, hn_carry = 0, last_hp = 0, last_hn = 0;
+  unsigned long long add_carry = 0;
+  uint64_t hp_in = 1, hn_in = 0, last_hp = 0, last_hn = 0;
   for (uint32_t b = 0; b < nb; ++b) {
     const uint64_t eq = eq_of(b);
     const uint64_t v = vp[b], n = vn[b];
     const uint64_t xh = eq | n;
-    const uint64_t s1 = v + (xh & v);
-    const uint64_t c1 = s1 < v;
-    const uint64_t s2 = s1 + add_carry;
-    const uint64_t c2 = s2 < s1;
-    add_carry = c1 | c2;
-    uint64_t d0 = (s2 ^ v) | xh;
-    uint64_t hn = v & d0;
-    uint64_t hp = n | ~(v | d0);
-    const bool lastb = b + 1 == nb;
-    if (lastb && last_partial) {
-      d0 &= last_mask;
-      hn &= last_mask;
-      hp &= last_mask;
-    }
-    if (lastb) {
-      last_hp = hp;
-      last_hn = hn;
-    }
-    const uint64_t x = (hp << 1) | hp_carry;
-    hp_carry = hp >> 63;
-    const uint64_t hs = (hn << 1) | hn_carry;
-    hn_carry = hn >> 63;
+    // v + (xh & v) + carry in, carry out to the next block.
+    unsigned long long cout;
+    const uint64_t s2 = __builtin_addcll(v, xh & v, add_carry, &cout);
+    add_carry = cout;
+    const uint64_t d0 = (s2 ^ v) | xh;
+    const uint64_t hn = v & d0;
+    const uint64_t hp = n | ~(v | d0);
+    last_hp = hp;
+    last_hn = hn;
+    const uint64_t x = (hp << 1) | hp_in;
+    hp_in = hp >> 63;
+    const uint64_t hs = (hn << 1) | hn_in;
+    hn_in = hn >> 63;
     vn[b] = x & d0;
-    const uint64_t v2 = hs | ~(d0 | x);
-    vp[b] = (lastb && last_partial) ? (v2 & last_mask) : v2;
+    vp[b] = hs | ~(d0 | x);
   }
   return (int32_t)((last_hp >> score_bit) & 1) - (int32_t)((last_hn >> score_bit) & 1);
 }
 
-// C_j[i] for j >= 1 from the stored column (vp, vn, score).
+// C_j[i] for j >= 1 from the stored column (vp, vn, score): the vertical
+// deltas of rows i + 1 .. m (bits i .. m - 1; the last block's bits from m
+// up are left over from the update and masked off with last_mask).
 __device__ __forceinline__ int32_t col_value(const uint64_t* vp, const uint64_t* vn, uint32_t nb,
-                                             int32_t score, uint32_t i) {
+                                             uint64_t last_mask, int32_t score, uint32_t i) {
   int32_t pc = 0, nc = 0;
   const uint32_t b0 = i >> 6;
   for (uint32_t b = b0; b < nb; ++b) {
-    const uint64_t keep = b == b0 ? (~0ull << (i & 63)) : ~0ull;
+    uint64_t keep = b == b0 ? (~0ull << (i & 63)) : ~0ull;
+    if (b + 1 == nb) keep &= last_mask;
     pc += __popcll(vp[b] & keep);
     nc += __popcll(vn[b] & keep);
   }
@@ -124,6 +121,14 @@ struct MyersRunWriter {
 // block count with state in the work item's scratch slot.
 template <int NB>
 __global__ __launch_bounds__(256) void myers_kernel(MyersArgs a) {
+  // 2-bit queries of up to three blocks look their Peq words up in LDS,
+  // [symbol][block][thread]: one read per block and column instead of a
+  // four-way select of 64-bit registers; consecutive threads read
+  // consecutive words, so the reads are free of bank conflicts.  Each
+  // thread reads only its own entries (no barrier).
+  constexpr bool PEQ_LDS = NB > 0 && NB <= 3;
+  __shared__ uint64_t peq_lds[PEQ_LDS ? 4 * NB * 256 : 1];
+  const uint32_t tid = threadIdx.x;
   const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= a.nwork) return;
   const uint32_t pi = a.work_list[w];
@@ -175,7 +180,9 @@ __global__ __launch_bounds__(256) void myers_kernel(MyersArgs a) {
         const uint64_t msk = (b + 1 == nb) ? last_mask : ~0ull;
         for (uint32_t c = 0; c < 4; ++c) {
           const uint64_t pe = peq2(qw, b, c, msk);
-          if (NB > 0) regs_peq[c][b] = pe; else gpeq[c * nb + b] = pe;
+          if (PEQ_LDS) peq_lds[(c * NB + b) * 256 + tid] = pe;
+          else if (NB > 0) regs_peq[c][b] = pe;
+          else gpeq[c * nb + b] = pe;
         }
       }
     }
@@ -194,15 +201,18 @@ __global__ __launch_bounds__(256) void myers_kernel(MyersArgs a) {
       }
       int32_t delta;
       if (b8) {
-        delta = myers_column(vp, vn, nb, last_mask, last_partial, score_bit,
+        delta = myers_column(vp, vn, nb, score_bit,
                              [&](uint32_t b) { return peq8(qw, b, m, tb); });
+      } else if (PEQ_LDS) {
+        const uint64_t* row = peq_lds + tb * (NB * 256) + tid;
+        delta = myers_column(vp, vn, nb, score_bit, [&](uint32_t b) { return row[b * 256]; });
       } else if (NB > 0) {
-        delta = myers_column(vp, vn, nb, last_mask, last_partial, score_bit, [&](uint32_t b) {
+        delta = myers_column(vp, vn, nb, score_bit, [&](uint32_t b) {
           const uint64_t p0 = regs_peq[0][b], p1 = regs_peq[1][b], p2 = regs_peq[2][b], p3 = regs_peq[3][b];
           return tb == 0 ? p0 : tb == 1 ? p1 : tb == 2 ? p2 : p3;
         });
       } else {
-        delta = myers_column(vp, vn, nb, last_mask, last_partial, score_bit,
+        delta = myers_column(vp, vn, nb, score_bit,
                              [&](uint32_t b) { return gpeq[tb * nb + b]; });
       }
       score += delta;
@@ -230,7 +240,7 @@ __global__ __launch_bounds__(256) void myers_kernel(MyersArgs a) {
         if (ti == 0) { rw.emit(OP_D); --qi; continue; }
         const uint64_t* cvp = colstore + (size_t)2 * nb * (ti - 1);
         const uint64_t* cvn = cvp + nb;
-        const int32_t cur = col_value(cvp, cvn, nb, colscore[ti - 1], qi);
+        const int32_t cur = col_value(cvp, cvn, nb, last_mask, colscore[ti - 1], qi);
         const bool vbit = (cvp[(qi - 1) >> 6] >> ((qi - 1) & 63)) & 1;  // C[qi] == C[qi-1] + 1
         if (vbit) { rw.emit(OP_D); --qi; continue; }
         int32_t left;
@@ -238,7 +248,7 @@ __global__ __launch_bounds__(256) void myers_kernel(MyersArgs a) {
           left = (int32_t)qi;
         } else {
           const uint64_t* pvp = colstore + (size_t)2 * nb * (ti - 2);
-          left = col_value(pvp, pvp + nb, nb, colscore[ti - 2], qi);
+          left = col_value(pvp, pvp + nb, nb, last_mask, colscore[ti - 2], qi);
         }
         if (cur == left + 1) { rw.emit(OP_I); --ti; continue; }
         const bool eq = sym_at(qw, qi - 1, b8) == sym_at(tw, ti - 1, b8);
