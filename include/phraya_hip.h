@@ -136,9 +136,12 @@ int phx_align_batch(phx_ctx* ctx, const phx_batch* batch, phx_results* out);
 phx_plan* phx_plan_create(phx_ctx* ctx, const phx_batch* batch);
 int phx_plan_run(phx_plan* plan);
 int phx_plan_results(phx_plan* plan, phx_results* out);
-/* Device time of the last phx_plan_run, in milliseconds (HIP events around
- * the whole run), and the summed time of its alignment kernel launches alone
- * (events recorded on the launch stream right before and after each launch). */
+/* Device time of the last phx_plan_run, in milliseconds: from the run's
+ * start (its first kernel's start event when the run is a single launch) to
+ * its end; and the summed time of its alignment kernel launches alone (events
+ * recorded on the launch stream right before and after each launch).  The
+ * events belong to the plan; the times are read from them on the first call
+ * after a run (not by phx_plan_run itself). */
 double phx_plan_last_ms(const phx_plan* plan);
 double phx_plan_last_kernel_ms(const phx_plan* plan);
 uint32_t phx_plan_last_launches(const phx_plan* plan);
