@@ -374,3 +374,36 @@ def test_register_wavefront_phases(px, costs):
         compare(px, eng, qs, ts, costs=costs, label=f"regs {costs}")
     bq = [q.lower() if i % 3 == 0 else q for i, q in enumerate(qs[:120])]
     compare(px, "affine", bq, ts[:120], costs=costs, label=f"regs bytes {costs}")
+
+
+def test_run_timings_belong_to_their_plan(px):
+    """Each plan times its runs with its own events, read on first query:
+    a run of another plan on the same context in between leaves them as
+    they were; kernel time never exceeds the run time; a one-launch run and
+    a multi-group run both report positive times."""
+    from phraya_amd import synth
+    ctx = px.Context(0)
+    short = synth.workload("short150", n=3000)
+    pb1 = px.PackedBatch(short.q_seq, short.q_off, short.t_seq, short.t_off,
+                         np.zeros(short.n, np.uint64), np.zeros(short.n, np.uint64))
+    w = synth.workload("mixed", n=200)
+    idx = [i for i in range(w.n) if w.q_off[i + 1] - w.q_off[i] <= 5000][:80]
+    sub = w.subset(idx)
+    pb2 = px.PackedBatch(sub.q_seq, sub.q_off, sub.t_seq, sub.t_off,
+                         np.zeros(sub.n, np.uint64), np.zeros(sub.n, np.uint64))
+    a = px.Plan(pb1, px.ENGINE_AFFINE, px.AffineCosts(4, 6, 2), ctx=ctx)
+    b = px.Plan(pb2, px.ENGINE_AFFINE, px.AffineCosts(4, 6, 2), ctx=ctx)
+    for _ in range(3):
+        a.run()
+        b.run()  # runs between a's run and the queries of a's times
+        a_ms, a_k, a_n = a.last_ms(), a.last_kernel_ms(), a.last_launches()
+        b_ms, b_k, b_n = b.last_ms(), b.last_kernel_ms(), b.last_launches()
+        say(f"a: {a_n} launch(es) {a_k:.4f} / {a_ms:.4f} ms; b: {b_n} launches {b_k:.4f} / {b_ms:.4f} ms")
+        assert a_n == 1 and b_n >= 2
+        assert 0 < a_k <= a_ms * 1.0001 and a_ms < 50
+        assert 0 < b_k and 0 < b_ms
+        # Queried again: the same numbers (read once per run).
+        assert a.last_ms() == a_ms and a.last_kernel_ms() == a_k
+    a.close()
+    b.close()
+    ctx.close()
