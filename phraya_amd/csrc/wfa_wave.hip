@@ -156,12 +156,26 @@ __device__ __forceinline__ WaveStore<LDS, RL> make_store(const WfaArgs& a, const
   S_t S;
   if constexpr (LDS) {
     S.ring = (typename S_t::ring_t*)(lds_base + G.ring_off);
+    // Both sequences are re-packed to start at bit 0 of their first word (a
+    // funnel shift per word, the last word the zero pad), so the step loop's
+    // extension needs no start offsets.  Reads stay inside the source
+    // words: data word j takes source words j and j + 1 <= the last one.
     lmem_t<uint64_t>* seq = (lmem_t<uint64_t>*)(lds_base + G.seq_off);
-    const uint32_t nq = seq_words<B8>((int32_t)pd.q_shift + pd.qn);
-    const uint32_t nt = seq_words<B8>((int32_t)pd.t_shift + pd.tn);
+    const uint32_t nq = seq_words<B8>(pd.qn), nt = seq_words<B8>(pd.tn);
+    const uint32_t qb = pd.q_shift * (B8 ? 8u : 2u), tb = pd.t_shift * (B8 ? 8u : 2u);
     const gmem_t<const uint64_t>* gq = (const gmem_t<const uint64_t>*)pd.q;
     const gmem_t<const uint64_t>* gt = (const gmem_t<const uint64_t>*)pd.t;
-    for (uint32_t i = (uint32_t)lane; i < nq + nt; i += 64) seq[i] = i < nq ? gq[i] : gt[i - nq];
+    for (uint32_t i = (uint32_t)lane; i < nq + nt; i += 64) {
+      const bool isq = i < nq;
+      const uint32_t j = isq ? i : i - nq, n = isq ? nq : nt, b = isq ? qb : tb;
+      const gmem_t<const uint64_t>* src = isq ? gq : gt;
+      uint64_t v = 0;
+      if (j + 1 < n) {
+        const uint64_t lo = src[j];
+        v = b == 0 ? lo : (lo >> b) | (src[j + 1] << (64 - b));
+      }
+      seq[i] = v;
+    }
     S.qw = seq;
     S.tw = seq + nq;
     S.toff_base = (typename S_t::cell_t*)(lds_base + G.tab_off);
@@ -169,7 +183,10 @@ __device__ __forceinline__ WaveStore<LDS, RL> make_store(const WfaArgs& a, const
     S.cap = G.cells_cap;
     S.steps_cap = G.steps_cap;
     S.wmask = WV_W - 1;
+    S.qsh = 0;
+    S.tsh = 0;
     wave_sync<true>();
+    return S;
   } else {
     // Storage of the pair's own (request server), else one HBM slot per pair
     // in flight: per wave, or per workgroup when the whole workgroup shares a
