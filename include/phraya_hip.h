@@ -384,6 +384,82 @@ int phx_extend_chains_batch(phx_ctx* ctx, int32_t strategy, int32_t gap_model,
 int phx_score_alignments(const phx_alignment* aln, const uint32_t* runs, const uint64_t* run_off,
                          uint32_t n, uint64_t query_len, int32_t* primary, uint8_t* is_alternative);
 
+/* --- Post-extension: variant sites from CIGAR runs ----------------------------
+ *
+ * extract_variants_from_cigar (phraya-align/src/executor.rs:814-1044) for a
+ * batch of alignments of reads against one target, reading the CIGAR runs
+ * directly (no ASCII): each X base is an SNV, each I run (target-only bases,
+ * upstream's WFA convention) a deletion, each D run (query-only bases) an
+ * insertion, under upstream's bounds checks.  Each variant carries its local
+ * coverage window [p - radius (floored at 0), min(p + radius + 1, |target|))
+ * with, per position, how many of the alignment's coverage windows
+ * (compute_windowed_coverage, :1071-1090: one [start, end) per alignment of
+ * the read, clipped to the target, empty ones dropped) cover it
+ * (get_abs_multi, :218-220), and flags for membership of the sorted,
+ * non-overlapping repeat and hotspot intervals (is_in_repeat_region /
+ * is_in_hotspot, :795-811).  The per-read fields upstream attaches (mapq,
+ * qualities, provenance, mate info, strand, confidence) pass through the
+ * caller unchanged and are not part of this call. */
+enum { PHX_VAR_SNV = 0, PHX_VAR_DELETION = 1, PHX_VAR_INSERTION = 2 };
+enum { PHX_VAR_IN_REPEAT = 1, PHX_VAR_IN_HOTSPOT = 2 };
+
+typedef struct phx_variant_batch {
+  uint32_t n;                   /* alignments */
+  const uint32_t* runs;         /* CIGAR runs (length << 4 | op), alignment i at run_off[i] .. run_off[i + 1] */
+  const uint64_t* run_off;      /* n + 1 */
+  const uint64_t* target_start; /* n: target position of the alignment's first base */
+  const uint8_t* q_seq;         /* query bytes, alignment i at q_off[i] .. q_off[i + 1] */
+  const uint64_t* q_off;        /* n + 1 */
+  const uint8_t* target;        /* the target all positions index */
+  uint64_t target_len;
+  const uint64_t* spans;        /* coverage windows, (start, end) pairs; alignment i's at span_off[i] .. span_off[i + 1] */
+  const uint64_t* span_off;     /* n + 1 (in pairs) */
+  const uint64_t* repeats;      /* sorted, non-overlapping (start, end) pairs */
+  uint64_t n_repeats;
+  const uint32_t* hotspots;     /* sorted, non-overlapping (start, end) pairs */
+  uint64_t n_hotspots;
+  uint32_t radius;              /* coverage_window_radius (Fast 150, Balanced 50, Sensitive 25) */
+} phx_variant_batch;
+
+typedef struct phx_variant {
+  uint32_t alignment;           /* index of its alignment in the batch */
+  uint32_t position;            /* target position */
+  uint8_t type;                 /* PHX_VAR_* */
+  uint8_t ref_base;             /* SNV: target base; deletion: first deleted base; insertion: '.' */
+  uint8_t alt_base;             /* SNV: query base; deletion: '.'; insertion: 0 (see ins_*) */
+  uint8_t flags;                /* PHX_VAR_IN_REPEAT | PHX_VAR_IN_HOTSPOT */
+  uint32_t ins_query_pos;       /* insertion: alleles are query[ins_query_pos .. + ins_len) */
+  uint32_t ins_len;
+  uint32_t window_start;        /* coverage window: values at coverage[coverage_off ..], */
+  uint32_t window_len;          /*   one per target position window_start .. + window_len */
+  uint32_t window_offset;       /* position - window_start */
+  uint64_t coverage_off;
+} phx_variant;
+
+typedef struct phx_variant_results {
+  phx_variant* variants;        /* in alignment order, then CIGAR order */
+  uint64_t variants_cap;
+  uint32_t* coverage;
+  uint64_t coverage_cap;
+  uint64_t* var_off;            /* n + 1: alignment i's variants at var_off[i] .. var_off[i + 1] */
+  uint64_t n_variants;          /* out: totals (set also on PHX_E_CAPACITY) */
+  uint64_t n_coverage;
+} phx_variant_results;
+
+typedef struct phx_variant_plan phx_variant_plan;
+
+/* Resident form: create uploads the batch, run extracts on the device (a
+ * counting pass, a scan and a writing pass), results downloads (checking
+ * the capacities; PHX_E_CAPACITY sets the totals and copies nothing). */
+phx_variant_plan* phx_variant_plan_create(phx_ctx* ctx, const phx_variant_batch* batch);
+int phx_variant_plan_run(phx_variant_plan* plan);
+int phx_variant_plan_results(phx_variant_plan* plan, phx_variant_results* out);
+/* Kernel time of the last run (events around its launches), milliseconds. */
+double phx_variant_plan_last_kernel_ms(const phx_variant_plan* plan);
+void phx_variant_plan_destroy(phx_variant_plan* plan);
+/* One call: create, run, results, destroy. */
+int phx_extract_variants(phx_ctx* ctx, const phx_variant_batch* batch, phx_variant_results* out);
+
 #ifdef __cplusplus
 }
 #endif

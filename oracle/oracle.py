@@ -92,6 +92,10 @@ def lib():
         L.orc_batch_cigar_copy.argtypes = [P, P, P]
         L.orc_batch_free.argtypes = [P]
         L.orc_hardware_threads.restype = ctypes.c_int
+        u64 = ctypes.c_uint64
+        L.orc_extract_variants.argtypes = [u32, P, P, P, P, P, P, u64, P, P, P, u64, P, u64, u32,
+                                           ctypes.c_int, P, u64, P, u64, P, P, P]
+        L.orc_extract_variants.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -182,3 +186,49 @@ def batch(engine: int, q_seq: np.ndarray, q_off: np.ndarray, t_seq: np.ndarray, 
     finally:
         L.orc_batch_free(h)
     return res, cigars
+
+
+# Record layout of orc_variant (the same as phx_variant in the C ABI).
+VARIANT_DTYPE = np.dtype([("alignment", np.uint32), ("position", np.uint32), ("type", np.uint8),
+                          ("ref_base", np.uint8), ("alt_base", np.uint8), ("flags", np.uint8),
+                          ("ins_query_pos", np.uint32), ("ins_len", np.uint32),
+                          ("window_start", np.uint32), ("window_len", np.uint32),
+                          ("window_offset", np.uint32), ("coverage_off", np.uint64)], align=True)
+
+
+def extract_variants(runs, run_off, target_start, q_seq, q_off, target, spans, span_off,
+                     repeats=None, hotspots=None, radius=25, threads=None):
+    """extract_variants_from_cigar over a batch (executor.rs:814-1044); see
+    orc_extract_variants.  Returns (variants, var_off, coverage) with the
+    records as VARIANT_DTYPE."""
+    L = lib()
+    n = len(target_start)
+    runs = np.ascontiguousarray(runs, np.uint32)
+    run_off = np.ascontiguousarray(run_off, np.uint64)
+    target_start = np.ascontiguousarray(target_start, np.uint64)
+    q_seq = np.ascontiguousarray(q_seq, np.uint8)
+    q_off = np.ascontiguousarray(q_off, np.uint64)
+    target = np.frombuffer(bytes(target), np.uint8) if not isinstance(target, np.ndarray) else \
+        np.ascontiguousarray(target, np.uint8)
+    spans = np.ascontiguousarray(spans, np.uint64).reshape(-1)
+    span_off = np.ascontiguousarray(span_off, np.uint64)
+    rep = np.ascontiguousarray(np.zeros(0, np.uint64) if repeats is None else repeats, np.uint64).reshape(-1)
+    hot = np.ascontiguousarray(np.zeros(0, np.uint32) if hotspots is None else hotspots, np.uint32).reshape(-1)
+    threads = threads or hardware_threads()
+    nv = ctypes.c_uint64(0)
+    nc = ctypes.c_uint64(0)
+    var_off = np.zeros(n + 1, np.uint64)
+    out = np.zeros(1, VARIANT_DTYPE)
+    cov = np.zeros(1, np.uint32)
+    for _ in range(2):
+        rc = L.orc_extract_variants(
+            n, runs.ctypes.data, run_off.ctypes.data, target_start.ctypes.data, q_seq.ctypes.data,
+            q_off.ctypes.data, target.ctypes.data, len(target), spans.ctypes.data,
+            span_off.ctypes.data, rep.ctypes.data, len(rep) // 2, hot.ctypes.data, len(hot) // 2,
+            radius, threads, out.ctypes.data, len(out), cov.ctypes.data, len(cov),
+            var_off.ctypes.data, ctypes.byref(nv), ctypes.byref(nc))
+        if rc == 0:
+            return out[:nv.value], var_off, cov[:nc.value]
+        out = np.zeros(max(1, nv.value), VARIANT_DTYPE)
+        cov = np.zeros(max(1, nc.value), np.uint32)
+    raise RuntimeError("orc_extract_variants: capacity")

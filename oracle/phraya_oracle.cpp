@@ -882,3 +882,172 @@ int orc_hardware_threads(void) {
 }
 
 }  // extern "C"
+
+// --- post-extension: variant sites from CIGAR runs ---------------------------
+//
+// extract_variants_from_cigar (phraya-align/src/executor.rs:814-1044) for a
+// batch of alignments against one target, restated step by step: M advances
+// both positions; each X base with qp < |query| and tp < |target| is an SNV
+// (ref target[tp], alt query[qp]); an I run (target-only bases, upstream's
+// WFA convention :912) with t_pos < |target| and q_pos < |query| is a
+// deletion at t_pos (ref = first deleted base, alt '.'); a D run (query-only
+// bases) with q_pos < |query| and t_pos < |target| is an insertion at t_pos
+// (ref '.', alleles = the inserted bases query[q_pos .. min(q_pos + count,
+// |query|)]).  Each variant carries its local coverage window
+// [p - r (floored at 0), min(p + r + 1, |target|)) with, per position, the
+// number of the alignment's coverage windows (compute_windowed_coverage,
+// :1071-1090: [min(target_start, |target|), min(target_end, |target|)),
+// counts 1, empty ones dropped) that cover it (get_abs_multi, :218-220), and
+// the repeat / hotspot flags by binary search over sorted, non-overlapping
+// [start, end) intervals (:795-811).  Runs are (length << 4) | op with op
+// M = 0, X = 1, I = 2, D = 3 (compacted, as parse_cigar(compact_cigar(..))).
+struct orc_variant {
+  uint32_t alignment, position;
+  uint8_t type, ref_base, alt_base, flags;
+  uint32_t ins_query_pos, ins_len;
+  uint32_t window_start, window_len, window_offset;
+  uint64_t coverage_off;
+};
+
+namespace {
+
+template <class T>
+bool in_intervals(uint64_t pos, const T* iv, uint64_t n) {
+  // partition_point(start <= pos), then pos < end of the interval before it
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) / 2;
+    if ((uint64_t)iv[2 * mid] <= pos) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo > 0 && pos < (uint64_t)iv[2 * (lo - 1) + 1];
+}
+
+struct VarIn {
+  const uint32_t* runs;
+  const uint64_t *run_off, *target_start;
+  const uint8_t* q_seq;
+  const uint64_t* q_off;
+  const uint8_t* target;
+  uint64_t target_len;
+  const uint64_t *spans, *span_off, *repeats;
+  uint64_t n_repeats;
+  const uint32_t* hotspots;
+  uint64_t n_hotspots;
+  uint32_t radius;
+};
+
+// Walks alignment i; with out == nullptr only counts variants and coverage
+// words.
+void variants_of(const VarIn& in, uint32_t i, orc_variant* out, uint32_t* cov, uint64_t* nv,
+                 uint64_t* nc) {
+  const uint8_t* q = in.q_seq + in.q_off[i];
+  const uint64_t qn = in.q_off[i + 1] - in.q_off[i], tn = in.target_len;
+  uint64_t q_pos = 0, t_pos = in.target_start[i], v = 0, c = 0;
+  auto emit = [&](uint64_t p, uint8_t type, uint8_t ref, uint8_t alt, uint64_t iq, uint64_t il) {
+    const uint64_t ws = p >= in.radius ? p - in.radius : 0;
+    const uint64_t we = std::min<uint64_t>(p + in.radius + 1, tn);
+    if (out) {
+      orc_variant& o = out[v];
+      o.alignment = i;
+      o.position = (uint32_t)p;
+      o.type = type;
+      o.ref_base = ref;
+      o.alt_base = alt;
+      o.flags = (uint8_t)((in_intervals(p, in.repeats, in.n_repeats) ? 1 : 0) |
+                          (in_intervals(p, in.hotspots, in.n_hotspots) ? 2 : 0));
+      o.ins_query_pos = (uint32_t)iq;
+      o.ins_len = (uint32_t)il;
+      o.window_start = (uint32_t)ws;
+      o.window_len = (uint32_t)(we - ws);
+      o.window_offset = (uint32_t)(p - ws);
+      o.coverage_off = c;
+      for (uint64_t pos = ws; pos < we; ++pos) {
+        uint32_t n = 0;
+        for (uint64_t s = in.span_off[i]; s < in.span_off[i + 1]; ++s) {
+          const uint64_t a = std::min(in.spans[2 * s], tn), b = std::min(in.spans[2 * s + 1], tn);
+          if (a < b && pos >= a && pos < b) ++n;
+        }
+        cov[c + (pos - ws)] = n;
+      }
+    }
+    ++v;
+    c += we - ws;
+  };
+  for (uint64_t r = in.run_off[i]; r < in.run_off[i + 1]; ++r) {
+    const uint64_t count = in.runs[r] >> 4;
+    switch (in.runs[r] & 15u) {
+      case 0:  // M
+        q_pos += count;
+        t_pos += count;
+        break;
+      case 1:  // X
+        for (uint64_t k = 0; k < count; ++k) {
+          const uint64_t qp = q_pos + k, tp = t_pos + k;
+          if (qp < qn && tp < tn) emit(tp, 0, in.target[tp], q[qp], 0, 0);
+        }
+        q_pos += count;
+        t_pos += count;
+        break;
+      case 2:  // I: target-only bases -> deletion
+        if (t_pos < tn && q_pos < qn) emit(t_pos, 1, in.target[t_pos], '.', 0, 0);
+        t_pos += count;
+        break;
+      case 3:  // D: query-only bases -> insertion
+        if (q_pos < qn && t_pos < tn)
+          emit(t_pos, 2, '.', 0, q_pos, std::min(q_pos + count, qn) - q_pos);
+        q_pos += count;
+        break;
+      default:
+        break;
+    }
+  }
+  *nv = v;
+  *nc = c;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Two passes over the alignments (counts, then the records at exclusive
+// offsets), each split over `threads` threads.  var_off gets n + 1 entries;
+// *n_var / *n_cov the totals.  Returns 0, or 1 when a capacity is too small
+// (totals still set).
+int orc_extract_variants(uint32_t n, const uint32_t* runs, const uint64_t* run_off,
+                         const uint64_t* target_start, const uint8_t* q_seq, const uint64_t* q_off,
+                         const uint8_t* target, uint64_t target_len, const uint64_t* spans,
+                         const uint64_t* span_off, const uint64_t* repeats, uint64_t n_repeats,
+                         const uint32_t* hotspots, uint64_t n_hotspots, uint32_t radius, int threads,
+                         orc_variant* out, uint64_t out_cap, uint32_t* cov, uint64_t cov_cap,
+                         uint64_t* var_off, uint64_t* n_var, uint64_t* n_cov) {
+  const VarIn in{runs, run_off, target_start, q_seq, q_off, target, target_len, spans, span_off,
+                 repeats, n_repeats, hotspots, n_hotspots, radius};
+  std::vector<uint64_t> nv(n), nc(n), co(n + 1, 0);
+  const int T = std::max(1, threads);
+  auto par = [&](auto&& fn) {
+    std::vector<std::thread> pool;
+    for (int t = 0; t < T; ++t)
+      pool.emplace_back([&, t] {
+        for (uint32_t i = (uint32_t)t; i < n; i += (uint32_t)T) fn(i);
+      });
+    for (auto& th : pool) th.join();
+  };
+  par([&](uint32_t i) { variants_of(in, i, nullptr, nullptr, &nv[i], &nc[i]); });
+  var_off[0] = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    var_off[i + 1] = var_off[i] + nv[i];
+    co[i + 1] = co[i] + nc[i];
+  }
+  *n_var = var_off[n];
+  *n_cov = co[n];
+  if (var_off[n] > out_cap || co[n] > cov_cap) return 1;
+  par([&](uint32_t i) {
+    uint64_t a, b;
+    variants_of(in, i, out + var_off[i], cov + co[i], &a, &b);
+    for (uint64_t k = var_off[i]; k < var_off[i + 1]; ++k) out[k].coverage_off += co[i];
+  });
+  return 0;
+}
+
+}  // extern "C"

@@ -751,3 +751,116 @@ def extend_chains_batch(strategy: int, gap_model: int, target: bytes,
             scored = ScoredAlignments(prim, alts)
         res.append(ReadExtension(scored, anchors))
     return res
+
+
+# --- post-extension: variant sites from CIGAR runs ---------------------------
+
+VAR_SNV, VAR_DELETION, VAR_INSERTION = 0, 1, 2
+VAR_IN_REPEAT, VAR_IN_HOTSPOT = 1, 2
+
+
+@dataclass
+class Variants:
+    """Variant records (L.VARIANT_DTYPE) in alignment then CIGAR order;
+    alignment i's at var_off[i] .. var_off[i + 1]; coverage values of a
+    record at coverage[coverage_off .. + window_len]."""
+    variants: np.ndarray
+    var_off: np.ndarray
+    coverage: np.ndarray
+
+
+class VariantPlan:
+    """extract_variants_from_cigar (executor.rs:814-1044) for a batch of
+    alignments against one target, resident on the device
+    (phx_variant_plan_*).  runs / run_off: CIGAR runs per alignment;
+    target_start: the target position of each alignment's first base;
+    queries: the query bytes of each alignment; spans: (start, end) coverage
+    windows of each alignment's read (compute_windowed_coverage, :1071-1090),
+    one list per alignment; repeats / hotspots: sorted, non-overlapping
+    (start, end) intervals; radius: coverage_window_radius."""
+
+    def __init__(self, runs, run_off, target_start, queries: Sequence[bytes], target: bytes,
+                 spans: Sequence[Sequence[tuple]], repeats=None, hotspots=None, radius: int = 25,
+                 ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        n = len(target_start)
+        self._keep = []
+
+        def arr(x, dt):
+            a = np.ascontiguousarray(x, dtype=dt)
+            self._keep.append(a)
+            return a
+
+        runs = arr(runs, np.uint32)
+        run_off = arr(run_off, np.uint64)
+        ts = arr(target_start, np.uint64)
+        q_seq = arr(np.frombuffer(b"".join(bytes(q) for q in queries), np.uint8), np.uint8)
+        q_off = arr(np.concatenate([[0], np.cumsum([len(q) for q in queries])]), np.uint64)
+        t = arr(np.frombuffer(bytes(target), np.uint8), np.uint8)
+        flat = [(int(a), int(b)) for sp in spans for (a, b) in sp]
+        sp = arr(np.array(flat, dtype=np.uint64).reshape(-1), np.uint64)
+        sp_off = arr(np.concatenate([[0], np.cumsum([len(s) for s in spans])]), np.uint64)
+        rep = arr(np.zeros(0) if repeats is None else repeats, np.uint64).reshape(-1)
+        hot = arr(np.zeros(0) if hotspots is None else hotspots, np.uint32).reshape(-1)
+
+        def ptr(a):
+            return a.ctypes.data if a.size else None
+
+        self.n = n
+        b = L.CVariantBatch(n, ptr(runs), ptr(run_off), ptr(ts), ptr(q_seq), ptr(q_off), ptr(t), len(t),
+                            ptr(sp), ptr(sp_off), ptr(rep), len(rep) // 2, ptr(hot), len(hot) // 2,
+                            radius)
+        self._h = L.lib().phx_variant_plan_create(self.ctx.handle, ctypes.byref(b))
+        if not self._h:
+            raise RuntimeError(f"phx_variant_plan_create failed: {self.ctx.last_error()}")
+        _track(self, 0)
+
+    def run(self) -> None:
+        rc = L.lib().phx_variant_plan_run(self._h)
+        if rc != L.PHX_OK:
+            raise RuntimeError(f"phx_variant_plan_run failed ({rc}): {self.ctx.last_error()}")
+
+    def last_kernel_ms(self) -> float:
+        return float(L.lib().phx_variant_plan_last_kernel_ms(self._h))
+
+    def results(self) -> Variants:
+        out = L.CVariantResults(None, 0, None, 0, None, 0, 0)
+        var_off = np.zeros(self.n + 1, np.uint64)
+        out.var_off = var_off.ctypes.data
+        rc = L.lib().phx_variant_plan_results(self._h, ctypes.byref(out))
+        if rc == L.PHX_E_CAPACITY:
+            v = np.zeros(max(1, out.n_variants), L.VARIANT_DTYPE)
+            c = np.zeros(max(1, out.n_coverage), np.uint32)
+            out.variants, out.variants_cap = v.ctypes.data, len(v)
+            out.coverage, out.coverage_cap = c.ctypes.data, len(c)
+            rc = L.lib().phx_variant_plan_results(self._h, ctypes.byref(out))
+        else:
+            v, c = np.zeros(0, L.VARIANT_DTYPE), np.zeros(0, np.uint32)
+        if rc != L.PHX_OK:
+            raise RuntimeError(f"phx_variant_plan_results failed ({rc}): {self.ctx.last_error()}")
+        return Variants(v[:out.n_variants], var_off, c[:out.n_coverage])
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            L.lib().phx_variant_plan_destroy(self._h)
+            self._h = None
+
+    def _release(self) -> None:
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def extract_variants(runs, run_off, target_start, queries, target, spans, repeats=None,
+                     hotspots=None, radius: int = 25, ctx: Optional[Context] = None) -> Variants:
+    """One-shot VariantPlan: upload, run, download."""
+    p = VariantPlan(runs, run_off, target_start, queries, target, spans, repeats, hotspots, radius, ctx)
+    try:
+        p.run()
+        return p.results()
+    finally:
+        p.close()

@@ -41,7 +41,9 @@ EXPORTS = (
     "phx_contig_plan_layout", "phx_contig_plan_gap_plan", "phx_contig_plan_last_ms",
     "phx_contig_plan_last_gpu_ms", "phx_contig_plan_last_stitch_ms", "phx_contig_plan_destroy",
     "phx_version", "phx_chain_cap", "phx_extend_chains_bounds", "phx_extend_chains_batch",
-    "phx_score_alignments",
+    "phx_score_alignments", "phx_variant_plan_create", "phx_variant_plan_run",
+    "phx_variant_plan_results", "phx_variant_plan_last_kernel_ms", "phx_variant_plan_destroy",
+    "phx_extract_variants",
 )
 
 
@@ -86,6 +88,31 @@ class CReadExtension(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int32), ("primary", ctypes.c_int32),
                 ("n_anchors", ctypes.c_uint32), ("n_alternatives", ctypes.c_uint32),
                 ("first_anchor", ctypes.c_uint64)]
+
+
+# phx_variant (include/phraya_hip.h) as a numpy record.
+VARIANT_DTYPE = np.dtype([("alignment", np.uint32), ("position", np.uint32), ("type", np.uint8),
+                          ("ref_base", np.uint8), ("alt_base", np.uint8), ("flags", np.uint8),
+                          ("ins_query_pos", np.uint32), ("ins_len", np.uint32),
+                          ("window_start", np.uint32), ("window_len", np.uint32),
+                          ("window_offset", np.uint32), ("coverage_off", np.uint64)], align=True)
+
+
+class CVariantBatch(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint32), ("runs", ctypes.c_void_p), ("run_off", ctypes.c_void_p),
+                ("target_start", ctypes.c_void_p), ("q_seq", ctypes.c_void_p),
+                ("q_off", ctypes.c_void_p), ("target", ctypes.c_void_p),
+                ("target_len", ctypes.c_uint64), ("spans", ctypes.c_void_p),
+                ("span_off", ctypes.c_void_p), ("repeats", ctypes.c_void_p),
+                ("n_repeats", ctypes.c_uint64), ("hotspots", ctypes.c_void_p),
+                ("n_hotspots", ctypes.c_uint64), ("radius", ctypes.c_uint32)]
+
+
+class CVariantResults(ctypes.Structure):
+    _fields_ = [("variants", ctypes.c_void_p), ("variants_cap", ctypes.c_uint64),
+                ("coverage", ctypes.c_void_p), ("coverage_cap", ctypes.c_uint64),
+                ("var_off", ctypes.c_void_p), ("n_variants", ctypes.c_uint64),
+                ("n_coverage", ctypes.c_uint64)]
 
 
 class CTuning(ctypes.Structure):
@@ -173,6 +200,14 @@ def lib() -> ctypes.CDLL:
                                           ctypes.c_uint32, P, P, P, ctypes.c_uint64, P,
                                           ctypes.c_uint64, P]
     L.phx_score_alignments.argtypes = [P, P, P, ctypes.c_uint32, ctypes.c_uint64, P, P]
+    L.phx_variant_plan_create.argtypes = [P, P]
+    L.phx_variant_plan_create.restype = P
+    L.phx_variant_plan_run.argtypes = [P]
+    L.phx_variant_plan_results.argtypes = [P, P]
+    L.phx_variant_plan_last_kernel_ms.argtypes = [P]
+    L.phx_variant_plan_last_kernel_ms.restype = ctypes.c_double
+    L.phx_variant_plan_destroy.argtypes = [P]
+    L.phx_extract_variants.argtypes = [P, P, P]
     L.phx_format_cigar.argtypes = [P, ctypes.c_uint64, P, ctypes.c_size_t]
     L.phx_format_cigar.restype = ctypes.c_size_t
     sz, u64 = ctypes.c_size_t, ctypes.c_uint64
