@@ -6,9 +6,10 @@
 // lookups (is_in_repeat_region / is_in_hotspot, :795-811).  Upstream walks
 // one read's CIGAR string per call on a rayon thread; here one thread walks
 // one alignment's runs, twice: a counting pass, an exclusive scan of the
-// counts (hipCUB), then a writing pass at the scanned offsets, so the records
-// come out in alignment order and, per alignment, in CIGAR order, exactly as
-// upstream's Vec would be concatenated.
+// counts, then a writing pass at the scanned offsets, so the records come out
+// in alignment order and, per alignment, in CIGAR order, exactly as
+// upstream's Vec would be concatenated; a last pass gives each variant a wave
+// for its coverage window and interval lookups.
 //
 // Rules per run (ops M = 0, X = 1, I = 2, D = 3; upstream's WFA convention,
 // :912: I = target-only bases, D = query-only bases):
@@ -23,8 +24,6 @@
 // |target|)), each value the number of the alignment's windows [min(start,
 // |target|), min(end, |target|)) (empty ones dropped) that hold it.
 #include <hip/hip_runtime.h>
-
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cstring>
@@ -70,14 +69,13 @@ __device__ bool in_sorted(uint64_t pos, const T* iv, uint64_t n) {
 }
 
 // One alignment's variants.  WRITE = false: counts only (*nv variants, *nc
-// coverage words); WRITE = true: records at out[0 ..], coverage at cov[0 ..]
-// (whose global offset is cov_base).
+// coverage words); WRITE = true: the records at out[0 ..], their windows at
+// coverage offsets from cov_base (the values come from the coverage pass).
 template <bool WRITE>
-__device__ void walk(const VarDev& a, uint32_t i, phx_variant* out, uint32_t* cov,
-                     uint64_t cov_base, uint64_t* nv, uint64_t* nc) {
+__device__ void walk(const VarDev& a, uint32_t i, phx_variant* out, uint64_t cov_base,
+                     uint64_t* nv, uint64_t* nc) {
   const uint8_t* q = a.q_seq + a.q_off[i];
   const uint64_t qn = a.q_off[i + 1] - a.q_off[i], tn = a.target_len;
-  const uint64_t s0 = a.span_off[i], s1 = a.span_off[i + 1];
   uint64_t q_pos = 0, t_pos = a.target_start[i], v = 0, c = 0;
   auto emit = [&](uint64_t p, uint8_t type, uint8_t ref, uint8_t alt, uint64_t iq, uint64_t il) {
     const uint64_t ws = p >= a.radius ? p - a.radius : 0;
@@ -89,8 +87,7 @@ __device__ void walk(const VarDev& a, uint32_t i, phx_variant* out, uint32_t* co
       r.type = type;
       r.ref_base = ref;
       r.alt_base = alt;
-      r.flags = (uint8_t)((in_sorted(p, a.repeats, a.n_repeats) ? PHX_VAR_IN_REPEAT : 0) |
-                          (in_sorted(p, a.hotspots, a.n_hotspots) ? PHX_VAR_IN_HOTSPOT : 0));
+      r.flags = 0;  // set by the coverage pass (its thread for the window's first word)
       r.ins_query_pos = (uint32_t)iq;
       r.ins_len = (uint32_t)il;
       r.window_start = (uint32_t)ws;
@@ -98,14 +95,6 @@ __device__ void walk(const VarDev& a, uint32_t i, phx_variant* out, uint32_t* co
       r.window_offset = (uint32_t)(p - ws);
       r.coverage_off = cov_base + c;
       out[v] = r;
-      for (uint64_t pos = ws; pos < we; ++pos) {
-        uint32_t k = 0;
-        for (uint64_t s = s0; s < s1; ++s) {
-          const uint64_t lo = min(a.spans[2 * s], tn), hi = min(a.spans[2 * s + 1], tn);
-          k += (lo < hi && pos >= lo && pos < hi) ? 1u : 0u;
-        }
-        cov[c + (pos - ws)] = k;
-      }
     }
     ++v;
     c += we - ws;
@@ -145,18 +134,87 @@ __global__ __launch_bounds__(256) void variant_count_kernel(VarDev a, uint64_t* 
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > a.n) return;
   uint64_t v = 0, c = 0;
-  if (i < a.n) walk<false>(a, i, nullptr, nullptr, 0, &v, &c);
+  if (i < a.n) walk<false>(a, i, nullptr, 0, &v, &c);
   nvar[i] = v;  // entry n stays 0, so the exclusive scan's last entry is the total
   ncov[i] = c;
 }
 
+// Exclusive prefix sums of two arrays of m entries in one workgroup: each
+// thread sums a contiguous chunk, the workgroup scans the chunk sums in LDS,
+// then each thread writes its chunk's prefixes.  (Batches are thousands to
+// hundreds of thousands of alignments: a few loads per thread.)
+__global__ __launch_bounds__(1024) void scan2_kernel(const uint64_t* a, const uint64_t* b, uint64_t* ao,
+                                                     uint64_t* bo, uint32_t m, uint64_t* totals) {
+  __shared__ uint64_t sa[1024], sb[1024];
+  const uint32_t t = threadIdx.x, T = blockDim.x;
+  const uint32_t chunk = (m + T - 1) / T;
+  const uint32_t lo = min(m, t * chunk), hi = min(m, lo + chunk);
+  uint64_t xa = 0, xb = 0;
+  for (uint32_t i = lo; i < hi; ++i) {
+    xa += a[i];
+    xb += b[i];
+  }
+  sa[t] = xa;
+  sb[t] = xb;
+  __syncthreads();
+  for (uint32_t d = 1; d < T; d <<= 1) {  // inclusive scan of the chunk sums
+    const uint64_t va = t >= d ? sa[t - d] : 0, vb = t >= d ? sb[t - d] : 0;
+    __syncthreads();
+    sa[t] += va;
+    sb[t] += vb;
+    __syncthreads();
+  }
+  uint64_t pa = sa[t] - xa, pb = sb[t] - xb;
+  for (uint32_t i = lo; i < hi; ++i) {
+    const uint64_t va = a[i], vb = b[i];
+    ao[i] = pa;
+    bo[i] = pb;
+    pa += va;
+    pb += vb;
+  }
+  // The totals straight to pinned host memory (system-scope stores), so the
+  // host reads them after the stream sync with no copy.
+  if (t == T - 1) {
+    __hip_atomic_store(&totals[0], sa[T - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&totals[1], sb[T - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
 __global__ __launch_bounds__(256) void variant_write_kernel(VarDev a, const uint64_t* var_off,
-                                                            const uint64_t* cov_off, phx_variant* out,
-                                                            uint32_t* cov) {
+                                                            const uint64_t* cov_off, phx_variant* out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   uint64_t v = 0, c = 0;
-  walk<true>(a, i, out + var_off[i], cov + cov_off[i], cov_off[i], &v, &c);
+  walk<true>(a, i, out + var_off[i], cov_off[i], &v, &c);
+}
+
+// One wave per variant: lane 0 looks the variant up in the repeat and
+// hotspot intervals (two binary searches), and the lanes fill its coverage
+// window, consecutive lanes consecutive words: for each target position, how
+// many of the alignment's windows [min(start, |target|), min(end, |target|))
+// hold it.  The grid comes from the host's bound on the variant count; waves
+// past the total (read on the device) return.
+__global__ __launch_bounds__(256) void variant_coverage_kernel(VarDev a, phx_variant* out,
+                                                               const uint64_t* var_off, uint32_t* cov) {
+  const uint64_t v = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63;
+  if (v >= var_off[a.n]) return;
+  phx_variant& r = out[v];
+  const uint32_t ws = r.window_start, wl = r.window_len, al = r.alignment;
+  const uint64_t co = r.coverage_off, tn = a.target_len;
+  if (lane == 0)
+    r.flags = (uint8_t)((in_sorted(r.position, a.repeats, a.n_repeats) ? PHX_VAR_IN_REPEAT : 0) |
+                        (in_sorted(r.position, a.hotspots, a.n_hotspots) ? PHX_VAR_IN_HOTSPOT : 0));
+  const uint64_t s0 = a.span_off[al], s1 = a.span_off[al + 1];
+  for (uint32_t j = lane; j < wl; j += 64) {
+    const uint64_t pos = (uint64_t)ws + j;
+    uint32_t k = 0;
+    for (uint64_t s = s0; s < s1; ++s) {
+      const uint64_t lo = min(a.spans[2 * s], tn), hi = min(a.spans[2 * s + 1], tn);
+      k += (lo < hi && pos >= lo && pos < hi) ? 1u : 0u;
+    }
+    cov[co + j] = k;
+  }
 }
 
 struct Buf {
@@ -170,10 +228,12 @@ struct phx_variant_plan {
   phx_ctx* ctx = nullptr;
   VarDev a{};
   Buf runs, run_off, target_start, q_seq, q_off, target, spans, span_off, repeats, hotspots;
-  Buf nvar, ncov, var_off, cov_off, scan_tmp, out, cov;
+  Buf nvar, ncov, var_off, cov_off, out, cov;
   uint64_t cap_v = 0, cap_c = 0;  // device capacities (upper bounds from the runs)
   uint64_t n_var = 0, n_cov = 0;  // totals of the last run
   bool ran = false;
+  uint64_t* tot_host = nullptr;  // pinned, mapped: the scan kernel's totals
+  uint64_t* tot_dev = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   mutable bool kms_ready = true;
   mutable double kms = 0;
@@ -247,11 +307,9 @@ phx_variant_plan* phx_variant_plan_create(phx_ctx* ctx, const phx_variant_batch*
             alloc_copy(P->cov_off, nullptr, (size_t)(n + 1) * 8) &&
             alloc_copy(P->out, nullptr, P->cap_v * sizeof(phx_variant)) &&
             alloc_copy(P->cov, nullptr, P->cap_c * 4);
-  size_t tmp = 0;
-  ok = ok && hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, (uint64_t*)P->nvar.p, (uint64_t*)P->var_off.p,
-                                              (int)(n + 1), ctx->stream) == hipSuccess &&
-       alloc_copy(P->scan_tmp, nullptr, tmp) && hipEventCreate(&P->ev0) == hipSuccess &&
-       hipEventCreate(&P->ev1) == hipSuccess;
+  ok = ok && hipEventCreate(&P->ev0) == hipSuccess && hipEventCreate(&P->ev1) == hipSuccess &&
+       hipHostMalloc((void**)&P->tot_host, 16, hipHostMallocMapped) == hipSuccess &&
+       hipHostGetDevicePointer((void**)&P->tot_dev, P->tot_host, 0) == hipSuccess;
   if (!ok) {
     ctx->err = "phx_variant_plan_create: device allocation or upload failed";
     phx_variant_plan_destroy(P);
@@ -286,30 +344,27 @@ int phx_variant_plan_run(phx_variant_plan* P) {
   P->ran = false;
   P->kms_ready = false;
   const dim3 block(256), grid((n + 1 + 255) / 256);
-  size_t tmp = P->scan_tmp.bytes;
-  uint64_t tot[2] = {0, 0};
   uint64_t* nv = (uint64_t*)P->nvar.p;
   uint64_t* nc = (uint64_t*)P->ncov.p;
   uint64_t* vo = (uint64_t*)P->var_off.p;
   uint64_t* co = (uint64_t*)P->cov_off.p;
   if (hipEventRecord(P->ev0, st) != hipSuccess) return vfail(ctx, PHX_E_DEVICE, "event");
   hipLaunchKernelGGL(variant_count_kernel, grid, block, 0, st, P->a, nv, nc);
-  if (hipGetLastError() != hipSuccess ||
-      hipcub::DeviceScan::ExclusiveSum(P->scan_tmp.p, tmp, nv, vo, (int)(n + 1), st) != hipSuccess ||
-      hipcub::DeviceScan::ExclusiveSum(P->scan_tmp.p, tmp, nc, co, (int)(n + 1), st) != hipSuccess)
-    return vfail(ctx, PHX_E_DEVICE, "variant count / scan");
+  P->tot_host[0] = P->tot_host[1] = ~0ull;
+  hipLaunchKernelGGL(scan2_kernel, dim3(1), dim3(1024), 0, st, (const uint64_t*)nv, (const uint64_t*)nc,
+                     vo, co, n + 1, P->tot_dev);
+  if (hipGetLastError() != hipSuccess) return vfail(ctx, PHX_E_DEVICE, "variant count / scan");
   if (n) {
-    hipLaunchKernelGGL(variant_write_kernel, dim3((n + 255) / 256), block, 0, st, P->a, vo, co,
-                       (phx_variant*)P->out.p, (uint32_t*)P->cov.p);
+    hipLaunchKernelGGL(variant_write_kernel, dim3((n + 255) / 256), block, 0, st, P->a,
+                       (const uint64_t*)vo, (const uint64_t*)co, (phx_variant*)P->out.p);
+    hipLaunchKernelGGL(variant_coverage_kernel, dim3((unsigned)((P->cap_v + 3) / 4)), block, 0, st,
+                       P->a, (phx_variant*)P->out.p, (const uint64_t*)vo, (uint32_t*)P->cov.p);
     if (hipGetLastError() != hipSuccess) return vfail(ctx, PHX_E_DEVICE, "variant write");
   }
-  if (hipEventRecord(P->ev1, st) != hipSuccess ||
-      hipMemcpyAsync(&tot[0], vo + n, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
-      hipMemcpyAsync(&tot[1], co + n, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
-      hipStreamSynchronize(st) != hipSuccess)
+  if (hipEventRecord(P->ev1, st) != hipSuccess || hipEventSynchronize(P->ev1) != hipSuccess)
     return vfail(ctx, PHX_E_DEVICE, "variant totals");
-  P->n_var = tot[0];
-  P->n_cov = tot[1];
+  P->n_var = ((volatile uint64_t*)P->tot_host)[0];
+  P->n_cov = ((volatile uint64_t*)P->tot_host)[1];
   if (P->n_var > P->cap_v || P->n_cov > P->cap_c)  // the bounds hold by construction
     return vfail(ctx, PHX_E_INTERNAL, "variant totals above their bounds");
   P->ran = true;
@@ -351,10 +406,11 @@ void phx_variant_plan_destroy(phx_variant_plan* P) {
   (void)hipStreamSynchronize(P->ctx->stream);
   for (Buf* b : {&P->runs, &P->run_off, &P->target_start, &P->q_seq, &P->q_off, &P->target, &P->spans,
                  &P->span_off, &P->repeats, &P->hotspots, &P->nvar, &P->ncov, &P->var_off, &P->cov_off,
-                 &P->scan_tmp, &P->out, &P->cov})
+                 &P->out, &P->cov})
     free_buf(*b);
   if (P->ev0) (void)hipEventDestroy(P->ev0);
   if (P->ev1) (void)hipEventDestroy(P->ev1);
+  if (P->tot_host) (void)hipHostFree(P->tot_host);
   delete P;
 }
 
