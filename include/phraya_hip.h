@@ -414,9 +414,9 @@ typedef struct phx_variant_batch {
   uint64_t target_len;
   const uint64_t* spans;        /* coverage windows, (start, end) pairs; alignment i's at span_off[i] .. span_off[i + 1] */
   const uint64_t* span_off;     /* n + 1 (in pairs) */
-  const uint64_t* repeats;      /* sorted, non-overlapping (start, end) pairs */
+  const uint64_t* repeats;      /* (start, end) pairs sorted by start, non-overlapping as upstream's */
   uint64_t n_repeats;
-  const uint32_t* hotspots;     /* sorted, non-overlapping (start, end) pairs */
+  const uint32_t* hotspots;     /* likewise; starts out of order are refused (PHX_E_ARG) */
   uint64_t n_hotspots;
   uint32_t radius;              /* coverage_window_radius (Fast 150, Balanced 50, Sensitive 25) */
 } phx_variant_batch;
@@ -457,7 +457,8 @@ int phx_variant_plan_results(phx_variant_plan* plan, phx_variant_results* out);
 /* Kernel time of the last run (events around its launches), milliseconds. */
 double phx_variant_plan_last_kernel_ms(const phx_variant_plan* plan);
 void phx_variant_plan_destroy(phx_variant_plan* plan);
-/* One call: create, run, results, destroy. */
+/* One call: create, run, results, destroy.  PHX_E_ARG for a batch that
+ * plan creation would refuse (the reason in phx_ctx_last_error). */
 int phx_extract_variants(phx_ctx* ctx, const phx_variant_batch* batch, phx_variant_results* out);
 
 #ifdef __cplusplus

@@ -776,8 +776,9 @@ class VariantPlan:
     target_start: the target position of each alignment's first base;
     queries: the query bytes of each alignment; spans: (start, end) coverage
     windows of each alignment's read (compute_windowed_coverage, :1071-1090),
-    one list per alignment; repeats / hotspots: sorted, non-overlapping
-    (start, end) intervals; radius: coverage_window_radius."""
+    one list per alignment; repeats / hotspots: (start, end) intervals
+    sorted by start (creation fails otherwise), non-overlapping as upstream's
+    are; radius: coverage_window_radius."""
 
     def __init__(self, runs, run_off, target_start, queries: Sequence[bytes], target: bytes,
                  spans: Sequence[Sequence[tuple]], repeats=None, hotspots=None, radius: int = 25,

@@ -158,3 +158,87 @@ def test_real_alignments(px):
     got = compare(px, runs, run_off, starts, qs, target, spans, rep, hot, 25, "real")
     say(f"{len(got.variants)} variants, {len(got.coverage)} coverage words")
     assert len(got.variants) > 1000
+
+
+def test_long_alignments_and_several_tiles(px):
+    """3000 alignments (three scan tiles of 1024), some with up to 200 runs
+    (several 64-run chunks per alignment), long X runs on a 400 bp target
+    with radius 150 (windows clipped at both ends), and intervals the lookup
+    directory has to get right: duplicate and nested starts, empty and
+    touching intervals, starts past the target, one interval over most of
+    it."""
+    rng = random.Random(11)
+    tlen = 400
+    target = bytes(rng.choice(b"ACGT") for _ in range(tlen))
+    runs, run_off, starts, queries, spans = [], [0], [], [], []
+    for k in range(3000):
+        nr = rng.choice([0, 3, 8, 70, 200]) if k % 7 == 0 else rng.randint(0, 10)
+        for _ in range(nr):
+            op = rng.choice([0, 1, 1, 2, 3])
+            ln = rng.randint(1, 300) if op == 1 and rng.random() < 0.1 else rng.randint(1, 6)
+            runs.append((ln << 4) | op)
+        run_off.append(len(runs))
+        starts.append(rng.choice([0, 1, rng.randint(0, tlen - 1), tlen - 2, tlen + 5]))
+        queries.append(bytes(rng.choice(b"ACGT") for _ in range(rng.randint(0, 400))))
+        spans.append([(a, a + rng.randint(0, 200)) for a in (rng.randint(0, tlen) for _ in range(rng.randint(0, 3)))])
+    rep = np.array([[5, 5], [5, 40], [5, 12], [30, 31], [31, 60], [100, 390], [395, 396],
+                    [399, 400], [400, 420], [500, 600]], np.uint64)
+    hot = np.array([[0, 1], [2, 3], [3, 4], [150, 160], [155, 170], [390, 1000]], np.uint32)
+    for radius in (0, 150):
+        compare(px, runs, run_off, starts, queries, target, spans, rep, hot, radius, "long/tiles")
+
+
+def _batch(px, runs, run_off, starts, queries, target, spans, rep, hot, radius):
+    from phraya_amd import _lib as L
+    keep = []
+
+    def arr(x, dt):
+        a = np.ascontiguousarray(x, dtype=dt).reshape(-1)
+        keep.append(a)
+        return a.ctypes.data if a.size else None
+
+    qs = b"".join(queries)
+    b = L.CVariantBatch(len(starts), arr(runs, np.uint32), arr(run_off, np.uint64), arr(starts, np.uint64),
+                        arr(np.frombuffer(qs, np.uint8), np.uint8),
+                        arr(np.concatenate([[0], np.cumsum([len(q) for q in queries])]), np.uint64),
+                        arr(np.frombuffer(target, np.uint8), np.uint8), len(target),
+                        arr([p for sp in spans for ab in sp for p in ab], np.uint64),
+                        arr(np.concatenate([[0], np.cumsum([len(s) for s in spans])]), np.uint64),
+                        arr(rep, np.uint64), len(rep), arr(hot, np.uint32), len(hot), radius)
+    return b, keep
+
+
+def test_one_shot_entry_capacity_and_unsorted_intervals(px):
+    """phx_extract_variants through the C ABI: the same records as the plan;
+    too small an output reports PHX_E_CAPACITY with the totals set; interval
+    starts out of order are refused (PHX_E_ARG), by the one-shot call and by
+    plan creation alike."""
+    import ctypes
+    from phraya_amd import _lib as L
+    runs = runs_of("2M1X4M1I3M2D5M")
+    target = b"ACCACGTTTACGTACGTAAAA"
+    args = (runs, [0, len(runs)], [0], [b"ACTACGTTACGGGTACGTA"], target, [[(0, 20)]],
+            np.array([[2, 4], [10, 12]], np.uint64), np.array([[0, 3]], np.uint32), 5)
+    want = compare(px, *args, label="one-shot")
+    b, keep = _batch(px, *args)
+    lib = L.lib()
+    h = px.default_context().handle
+    v = np.zeros(len(want.variants), L.VARIANT_DTYPE)
+    c = np.zeros(len(want.coverage), np.uint32)
+    off = np.zeros(2, np.uint64)
+    out = L.CVariantResults(v.ctypes.data, len(v), c.ctypes.data, len(c), off.ctypes.data, 0, 0)
+    assert lib.phx_extract_variants(h, ctypes.byref(b), ctypes.byref(out)) == L.PHX_OK
+    assert (out.n_variants, out.n_coverage) == (len(v), len(c))
+    assert np.array_equal(off, want.var_off) and np.array_equal(c, want.coverage)
+    for f in L.VARIANT_DTYPE.names:
+        assert np.array_equal(v[f], want.variants[f]), f
+    small = L.CVariantResults(v.ctypes.data, len(v) - 1, c.ctypes.data, len(c), off.ctypes.data, 0, 0)
+    assert lib.phx_extract_variants(h, ctypes.byref(b), ctypes.byref(small)) == L.PHX_E_CAPACITY
+    assert (small.n_variants, small.n_coverage) == (len(v), len(c))
+    bad = list(args)
+    bad[6] = np.array([[10, 12], [2, 4]], np.uint64)
+    b2, keep2 = _batch(px, *bad)
+    assert lib.phx_extract_variants(h, ctypes.byref(b2), ctypes.byref(out)) == L.PHX_E_ARG
+    assert "not sorted" in px.default_context().last_error()
+    with pytest.raises(RuntimeError, match="not sorted"):
+        px.extract_variants(*bad)
