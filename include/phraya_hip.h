@@ -384,6 +384,58 @@ int phx_extend_chains_batch(phx_ctx* ctx, int32_t strategy, int32_t gap_model,
 int phx_score_alignments(const phx_alignment* aln, const uint32_t* runs, const uint64_t* run_off,
                          uint32_t n, uint64_t query_len, int32_t* primary, uint8_t* is_alternative);
 
+/* --- align_read's strand handling for a batch --------------------------------
+ *
+ * phraya-align/src/executor.rs:573-733 after seeding and chaining: each read
+ * comes with its forward and reverse chains (the caller's seed_and_chain of
+ * the read and of its reverse complement; ignored for protein); the
+ * orientations worth extending (only the one with chains when the other has
+ * none, otherwise both) go through one batched extend_chains; the reverse one
+ * is kept only with a strictly smaller primary edit distance. */
+enum { PHX_STRAND_FORWARD = 0, PHX_STRAND_REVERSE = 1 };
+enum {
+  PHX_OUTCOME_PLACED = 0,           /* primary score 1 - ed / max(qlen, 1) >= 0.95 */
+  PHX_OUTCOME_NO_SEED = 1,          /* below 0.95 and the read had no seeds */
+  PHX_OUTCOME_BELOW_THRESHOLD = 2,  /* below 0.95 with seeds */
+  PHX_OUTCOME_FAST_CUTOFF = 3,      /* Fast: ed / max(qlen, 1) > 0.20 (upstream returns None) */
+  PHX_OUTCOME_NO_ALIGNMENT = 4      /* no anchor extended in either orientation */
+};
+
+typedef struct phx_read_strands {
+  const uint8_t* query;                   /* forward bytes as stored */
+  uint64_t query_len;
+  const uint64_t* fwd_chain_target_start; /* chains of the forward orientation, rank order */
+  uint32_t n_fwd_chains;
+  const uint64_t* rev_chain_target_start; /* chains of the reverse complement */
+  uint32_t n_rev_chains;
+  uint32_t had_seeds;                     /* fwd_seeds > 0 || rev_seeds > 0 */
+} phx_read_strands;
+
+typedef struct phx_read_placement {
+  int32_t outcome;        /* PHX_OUTCOME_* */
+  int32_t strand;         /* the kept orientation (PHX_STRAND_*), -1 when none */
+  int32_t entry;          /* its extension entry in ext[], -1 when none */
+  int32_t entry_fwd;      /* entries of the orientations extended, -1 when skipped */
+  int32_t entry_rev;
+  double primary_score;   /* 1 - ed / max(qlen, 1) of the kept primary */
+} phx_read_placement;
+
+/* reverse_complement (phraya-core/src/types.rs:91-107): A<->T, C<->G in both
+ * cases, other bytes unchanged, order reversed.  Host only. */
+void phx_reverse_complement(const uint8_t* seq, uint64_t n, uint8_t* out);
+/* Sizes for phx_align_reads_batch: extension entries, anchors, runs. */
+int phx_align_reads_bounds(int32_t strategy, int32_t is_dna, uint64_t target_len,
+                           const phx_read_strands* reads, uint32_t n_reads, uint64_t* entries,
+                           uint64_t* anchors, uint64_t* runs);
+/* ext[]: one phx_read_extension per extended orientation (the layout of
+ * phx_extend_chains_batch, anchors and runs likewise); placement[] per read.
+ * PHX_E_CAPACITY when ext_cap, anchors_cap or runs_cap is short. */
+int phx_align_reads_batch(phx_ctx* ctx, int32_t strategy, int32_t gap_model, int32_t is_dna,
+                          const uint8_t* target, uint64_t target_len, const phx_read_strands* reads,
+                          uint32_t n_reads, phx_read_placement* placement, phx_read_extension* ext,
+                          uint64_t ext_cap, phx_alignment* anchor_aln, uint8_t* anchor_role,
+                          uint64_t anchors_cap, uint32_t* runs, uint64_t runs_cap, uint64_t* run_off);
+
 /* --- Post-extension: variant sites from CIGAR runs ----------------------------
  *
  * extract_variants_from_cigar (phraya-align/src/executor.rs:814-1044) for a

@@ -731,6 +731,10 @@ def extend_chains_batch(strategy: int, gap_model: int, target: bytes,
                                          runs.ctypes.data, R, off.ctypes.data)
     if rc != L.PHX_OK:
         raise RuntimeError(f"phx_extend_chains_batch failed ({rc}): {ctx.last_error()}")
+    return _read_extensions(out, n, aln, role, runs, off)
+
+
+def _read_extensions(out, n, aln, role, runs, off) -> list:
     res = []
     for i in range(n):
         o = out[i]
@@ -750,6 +754,91 @@ def extend_chains_batch(strategy: int, gap_model: int, target: bytes,
             alts = [al for rl, al in anchors if rl == 3]
             scored = ScoredAlignments(prim, alts)
         res.append(ReadExtension(scored, anchors))
+    return res
+
+
+# --- align_read's strand handling ---------------------------------------------
+
+STRAND_FORWARD, STRAND_REVERSE = 0, 1
+OUTCOME_PLACED, OUTCOME_NO_SEED, OUTCOME_BELOW_THRESHOLD, OUTCOME_FAST_CUTOFF, \
+    OUTCOME_NO_ALIGNMENT = 0, 1, 2, 3, 4
+
+
+def reverse_complement(seq: bytes) -> bytes:
+    """reverse_complement (phraya-core/src/types.rs:91-107), host code of the
+    library."""
+    a = np.frombuffer(bytes(seq), np.uint8)
+    out = np.zeros(max(len(a), 1), np.uint8)
+    L.lib().phx_reverse_complement(a.ctypes.data if len(a) else None, len(a), out.ctypes.data)
+    return out[:len(a)].tobytes()
+
+
+@dataclass
+class ReadPlacement:
+    """align_read's strand handling for one read (executor.rs:573-733):
+    outcome (OUTCOME_*), the kept strand (STRAND_*, -1 when none), the kept
+    orientation's query bytes and ScoredAlignments (None when no alignment;
+    set also for FAST_CUTOFF, where upstream returns None), its primary score,
+    and each extended orientation's ReadExtension (None when skipped)."""
+    outcome: int
+    strand: int
+    query: bytes
+    scored: Optional[ScoredAlignments]
+    primary_score: float
+    forward: Optional[ReadExtension]
+    reverse: Optional[ReadExtension]
+
+
+def align_reads_batch(strategy: int, gap_model: int, target: bytes, reads: Sequence[tuple],
+                      is_dna: bool = True, ctx: Optional[Context] = None) -> list:
+    """phx_align_reads_batch: reads are (query bytes, forward chain
+    target_starts, reverse chain target_starts, had_seeds); every orientation
+    worth extending goes through one batched extend_chains."""
+    ctx = ctx or default_context()
+    n = len(reads)
+    t = np.frombuffer(bytes(target), np.uint8)
+    keep, arr = [], (L.CReadStrands * max(n, 1))()
+
+    def u64(x):
+        a = np.asarray(list(x), dtype=np.uint64)
+        keep.append(a)
+        return (a.ctypes.data if len(a) else None), len(a)
+
+    for i, (q, fwd, rev, seeds) in enumerate(reads):
+        qa = np.frombuffer(bytes(q), np.uint8)
+        keep.append(qa)
+        fp, fn = u64(fwd)
+        rp, rn = u64(rev)
+        arr[i] = L.CReadStrands(qa.ctypes.data if len(qa) else None, len(qa), fp, fn, rp, rn,
+                                1 if seeds else 0)
+    ne, na, nr = ctypes.c_uint64(0), ctypes.c_uint64(0), ctypes.c_uint64(0)
+    rc = L.lib().phx_align_reads_bounds(strategy, int(is_dna), len(t), arr, n, ctypes.byref(ne),
+                                        ctypes.byref(na), ctypes.byref(nr))
+    if rc != L.PHX_OK:
+        raise RuntimeError(f"phx_align_reads_bounds: status {rc}")
+    E, A, R = ne.value, na.value, max(nr.value, 1)
+    place = (L.CReadPlacement * max(n, 1))()
+    ext = (L.CReadExtension * max(E, 1))()
+    aln = np.zeros(max(A, 1), dtype=L.ALN_DTYPE)
+    role = np.zeros(max(A, 1), np.uint8)
+    runs = np.zeros(R, np.uint32)
+    off = np.zeros(A + 1, np.uint64)
+    rc = L.lib().phx_align_reads_batch(ctx.handle, strategy, gap_model, int(is_dna),
+                                       t.ctypes.data if len(t) else None, len(t), arr, n, place,
+                                       ext, E, aln.ctypes.data, role.ctypes.data, A,
+                                       runs.ctypes.data, R, off.ctypes.data)
+    if rc != L.PHX_OK:
+        raise RuntimeError(f"phx_align_reads_batch failed ({rc}): {ctx.last_error()}")
+    exts = _read_extensions(ext, E, aln, role, runs, off)
+    res = []
+    for i, (q, _, _, _) in enumerate(reads):
+        p = place[i]
+        fwd = exts[p.entry_fwd] if p.entry_fwd >= 0 else None
+        rev = exts[p.entry_rev] if p.entry_rev >= 0 else None
+        kept = None if p.strand < 0 else (fwd if p.strand == STRAND_FORWARD else rev)
+        qb = bytes(q) if p.strand != STRAND_REVERSE else (reverse_complement(q) if is_dna else b"")
+        res.append(ReadPlacement(int(p.outcome), int(p.strand), qb,
+                                 kept.scored if kept else None, float(p.primary_score), fwd, rev))
     return res
 
 

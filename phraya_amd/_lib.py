@@ -41,7 +41,8 @@ EXPORTS = (
     "phx_contig_plan_layout", "phx_contig_plan_gap_plan", "phx_contig_plan_last_ms",
     "phx_contig_plan_last_gpu_ms", "phx_contig_plan_last_stitch_ms", "phx_contig_plan_destroy",
     "phx_version", "phx_chain_cap", "phx_extend_chains_bounds", "phx_extend_chains_batch",
-    "phx_score_alignments", "phx_variant_plan_create", "phx_variant_plan_run",
+    "phx_score_alignments", "phx_reverse_complement", "phx_align_reads_bounds",
+    "phx_align_reads_batch", "phx_variant_plan_create", "phx_variant_plan_run",
     "phx_variant_plan_results", "phx_variant_plan_last_kernel_ms", "phx_variant_plan_destroy",
     "phx_extract_variants",
 )
@@ -88,6 +89,19 @@ class CReadExtension(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int32), ("primary", ctypes.c_int32),
                 ("n_anchors", ctypes.c_uint32), ("n_alternatives", ctypes.c_uint32),
                 ("first_anchor", ctypes.c_uint64)]
+
+
+class CReadStrands(ctypes.Structure):
+    _fields_ = [("query", ctypes.c_void_p), ("query_len", ctypes.c_uint64),
+                ("fwd_chain_target_start", ctypes.c_void_p), ("n_fwd_chains", ctypes.c_uint32),
+                ("rev_chain_target_start", ctypes.c_void_p), ("n_rev_chains", ctypes.c_uint32),
+                ("had_seeds", ctypes.c_uint32)]
+
+
+class CReadPlacement(ctypes.Structure):
+    _fields_ = [("outcome", ctypes.c_int32), ("strand", ctypes.c_int32), ("entry", ctypes.c_int32),
+                ("entry_fwd", ctypes.c_int32), ("entry_rev", ctypes.c_int32),
+                ("primary_score", ctypes.c_double)]
 
 
 # phx_variant (include/phraya_hip.h) as a numpy record.
@@ -200,6 +214,13 @@ def lib() -> ctypes.CDLL:
                                           ctypes.c_uint32, P, P, P, ctypes.c_uint64, P,
                                           ctypes.c_uint64, P]
     L.phx_score_alignments.argtypes = [P, P, P, ctypes.c_uint32, ctypes.c_uint64, P, P]
+    L.phx_reverse_complement.argtypes = [P, ctypes.c_uint64, P]
+    L.phx_reverse_complement.restype = None
+    L.phx_align_reads_bounds.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, P,
+                                         ctypes.c_uint32, P, P, P]
+    L.phx_align_reads_batch.argtypes = [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, P,
+                                        ctypes.c_uint64, P, ctypes.c_uint32, P, P, ctypes.c_uint64,
+                                        P, P, ctypes.c_uint64, P, ctypes.c_uint64, P]
     L.phx_variant_plan_create.argtypes = [P, P]
     L.phx_variant_plan_create.restype = P
     L.phx_variant_plan_run.argtypes = [P]

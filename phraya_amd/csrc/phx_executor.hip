@@ -22,6 +22,16 @@
 //    distance; alternatives = every other alignment not equal to the primary
 //    (all fields, CIGAR included) whose normalised score ratio
 //    (1 - ed / qlen) / (1 - ed_primary / qlen) is >= 0.95, in anchor order.
+// And align_read's strand handling (executor.rs:573-733) for a batch:
+//  * reverse_complement (phraya-core/src/types.rs:91-107) of DNA reads;
+//  * extend only the orientation with chains when the other has none,
+//    otherwise both (:676-698; protein: no reverse chains, empty reverse
+//    query, :632-646), all orientations of all reads in one batch;
+//  * keep the reverse orientation only when its primary edit distance is
+//    strictly smaller (:700-711); then the Fast cutoff (divergence
+//    ed / max(qlen, 1) > 0.20, :253, :716-723) and the outcome: placed when
+//    1 - ed / max(qlen, 1) >= 0.95 (:263), else below-threshold when the
+//    read had seeds, else no-seed (:727-733).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -50,9 +60,125 @@ bool same_alignment(const phx_alignment& a, const uint32_t* ra, uint64_t na, con
          (na == 0 || std::memcmp(ra, rb, na * sizeof(uint32_t)) == 0);
 }
 
+uint8_t complement(uint8_t b) {
+  switch (b) {
+    case 'A': return 'T';
+    case 'T': return 'A';
+    case 'C': return 'G';
+    case 'G': return 'C';
+    case 'a': return 't';
+    case 't': return 'a';
+    case 'c': return 'g';
+    case 'g': return 'c';
+    default: return b;
+  }
+}
+
+// The orientations align_read extends (executor.rs:632-698): reverse chains
+// count only for DNA (protein has no reverse strand and its reverse query is
+// empty); one orientation alone when only it has chains, otherwise both.
+struct Orientations {
+  std::vector<phx_read_chains> entry;     // forward and / or reverse extension inputs
+  std::vector<std::vector<uint8_t>> rc;   // reverse complements, one per read
+  std::vector<int32_t> fwd, rev;          // entry of each orientation, -1 when not extended
+};
+
+bool orientations(const phx_read_strands* reads, uint32_t n, int32_t is_dna, Orientations* o) {
+  o->rc.assign(n, {});
+  o->fwd.assign(n, -1);
+  o->rev.assign(n, -1);
+  o->entry.clear();
+  for (uint32_t r = 0; r < n; ++r) {
+    const phx_read_strands& rd = reads[r];
+    if ((!rd.query && rd.query_len) || (rd.n_fwd_chains && !rd.fwd_chain_target_start) ||
+        (is_dna && rd.n_rev_chains && !rd.rev_chain_target_start))
+      return false;
+    const uint32_t nf = rd.n_fwd_chains, nr = is_dna ? rd.n_rev_chains : 0;
+    const bool do_fwd = !(nf == 0 && nr != 0), do_rev = !(nf != 0 && nr == 0);
+    if (do_fwd) {
+      o->fwd[r] = (int32_t)o->entry.size();
+      o->entry.push_back(phx_read_chains{rd.query, rd.query_len, rd.fwd_chain_target_start, nf});
+    }
+    if (do_rev) {
+      if (is_dna) {
+        o->rc[r].resize(rd.query_len);
+        phx_reverse_complement(rd.query, rd.query_len, o->rc[r].data());
+      }
+      o->rev[r] = (int32_t)o->entry.size();
+      o->entry.push_back(phx_read_chains{o->rc[r].empty() ? nullptr : o->rc[r].data(),
+                                         (uint64_t)o->rc[r].size(),
+                                         nr ? rd.rev_chain_target_start : nullptr, nr});
+    }
+  }
+  return true;
+}
+
 }  // namespace
 
 extern "C" {
+
+void phx_reverse_complement(const uint8_t* seq, uint64_t n, uint8_t* out) {
+  for (uint64_t i = 0; i < n; ++i) out[i] = complement(seq[n - 1 - i]);
+}
+
+int phx_align_reads_bounds(int32_t strategy, int32_t is_dna, uint64_t target_len,
+                           const phx_read_strands* reads, uint32_t n_reads, uint64_t* entries,
+                           uint64_t* anchors, uint64_t* runs) {
+  if (!reads || !entries || !anchors || !runs) return PHX_E_ARG;
+  Orientations o;
+  if (!orientations(reads, n_reads, is_dna, &o)) return PHX_E_ARG;
+  *entries = o.entry.size();
+  return phx_extend_chains_bounds(strategy, target_len, o.entry.data(), (uint32_t)o.entry.size(),
+                                  anchors, runs);
+}
+
+int phx_align_reads_batch(phx_ctx* ctx, int32_t strategy, int32_t gap_model, int32_t is_dna,
+                          const uint8_t* target, uint64_t target_len, const phx_read_strands* reads,
+                          uint32_t n_reads, phx_read_placement* placement, phx_read_extension* ext,
+                          uint64_t ext_cap, phx_alignment* anchor_aln, uint8_t* anchor_role,
+                          uint64_t anchors_cap, uint32_t* runs, uint64_t runs_cap, uint64_t* run_off) {
+  if (!ctx || !reads || !placement || !ext) return PHX_E_ARG;
+  Orientations o;
+  if (!orientations(reads, n_reads, is_dna, &o)) return PHX_E_ARG;
+  if (ext_cap < o.entry.size()) return PHX_E_CAPACITY;
+  // Every orientation of every read in one batched extend_chains.
+  const int rc = phx_extend_chains_batch(ctx, strategy, gap_model, target, target_len, o.entry.data(),
+                                         (uint32_t)o.entry.size(), ext, anchor_aln, anchor_role,
+                                         anchors_cap, runs, runs_cap, run_off);
+  if (rc != PHX_OK) return rc;
+  // The better orientation, ties to forward (executor.rs:700-711); then the
+  // Fast divergence cutoff (:716-723) and the outcome (:727-733).
+  for (uint32_t r = 0; r < n_reads; ++r) {
+    phx_read_placement& pl = placement[r];
+    pl.entry_fwd = o.fwd[r];
+    pl.entry_rev = o.rev[r];
+    auto primary_ed = [&](int32_t e) {
+      const phx_read_extension& x = ext[e];
+      return anchor_aln[x.first_anchor + (uint64_t)x.primary].edit_distance;
+    };
+    const bool f = o.fwd[r] >= 0 && ext[o.fwd[r]].status == PHX_OK;
+    const bool v = o.rev[r] >= 0 && ext[o.rev[r]].status == PHX_OK;
+    if (f && v && primary_ed(o.rev[r]) < primary_ed(o.fwd[r])) pl.strand = PHX_STRAND_REVERSE;
+    else if (f) pl.strand = PHX_STRAND_FORWARD;
+    else if (v) pl.strand = PHX_STRAND_REVERSE;
+    else pl.strand = -1;
+    if (pl.strand < 0) {
+      pl.entry = -1;
+      pl.outcome = PHX_OUTCOME_NO_ALIGNMENT;
+      pl.primary_score = 0.0;
+      continue;
+    }
+    pl.entry = pl.strand == PHX_STRAND_FORWARD ? o.fwd[r] : o.rev[r];
+    const double ql = (double)std::max<uint64_t>(reads[r].query_len, 1);
+    const double ed = (double)primary_ed(pl.entry);
+    pl.primary_score = 1.0 - ed / ql;
+    if (strategy == PHX_STRATEGY_FAST && ed / ql > 0.20) pl.outcome = PHX_OUTCOME_FAST_CUTOFF;
+    else if (pl.primary_score >= 0.95) pl.outcome = PHX_OUTCOME_PLACED;
+    else if (reads[r].had_seeds) pl.outcome = PHX_OUTCOME_BELOW_THRESHOLD;
+    else pl.outcome = PHX_OUTCOME_NO_SEED;
+  }
+  return PHX_OK;
+}
 
 uint32_t phx_chain_cap(int32_t strategy) { return cap_of(strategy); }
 
