@@ -384,6 +384,76 @@ int phx_extend_chains_batch(phx_ctx* ctx, int32_t strategy, int32_t gap_model,
 int phx_score_alignments(const phx_alignment* aln, const uint32_t* runs, const uint64_t* run_off,
                          uint32_t n, uint64_t query_len, int32_t* primary, uint8_t* is_alternative);
 
+/* --- Pre-extension: seeding and chaining -------------------------------------
+ *
+ * seed_and_chain (phraya-align/src/executor.rs:511-519) for a batch of query
+ * minimizer sketches against one target sketch: find_seeds_indexed_capped
+ * (seeding.rs:56-74; values occurring more than max_occ times in the target
+ * are masked) and chain_seeds (chaining.rs:131-253), with upstream's orders
+ * throughout.  Sketches are (value, position) lists as phraya_core's
+ * MinimizerSketch holds them; sketching itself (simd-minimizers) is the
+ * caller's. */
+typedef struct phx_seed {
+  uint32_t query_pos;
+  uint32_t target_pos;
+  uint64_t minimizer;
+} phx_seed;
+
+typedef struct phx_chain_params {  /* ChainParams (chaining.rs:41-85); defaults 21, 100, 100, 5000, 25, 0.21, 25 */
+  int64_t seed_len;
+  int64_t max_band;                /* at most 2^20 */
+  int64_t max_dist_x;
+  uint64_t max_iter;
+  uint64_t max_skip;
+  double gap_penalty;
+  int64_t min_chain_score;
+} phx_chain_params;
+
+typedef struct phx_chain {
+  int64_t score;
+  uint64_t target_start;  /* Chain::target_start (chaining.rs:31-35) */
+  uint64_t seed_off;      /* its seeds at seeds[seed_off .. + n_seeds], by query position */
+  uint32_t n_seeds;
+  uint32_t reserved;
+} phx_chain;
+
+typedef struct phx_chain_results {
+  phx_chain* chains;      /* query i's at chain_off[i] .. chain_off[i + 1], score descending */
+  uint64_t chains_cap;
+  phx_seed* seeds;
+  uint64_t seeds_cap;
+  uint64_t* chain_off;    /* nq + 1 */
+  uint64_t* n_seeds;      /* nq: seeds found per query (seed_and_chain's count) */
+  uint64_t n_chains;      /* out: totals (set also on PHX_E_CAPACITY) */
+  uint64_t n_chain_seeds;
+} phx_chain_results;
+
+typedef struct phx_seed_index phx_seed_index;
+typedef struct phx_chain_plan phx_chain_plan;
+
+/* build_minimizer_index (seeding.rs:29-35) of a target sketch, on the device. */
+phx_seed_index* phx_seed_index_create(phx_ctx* ctx, const uint64_t* values, const uint32_t* positions,
+                                      uint64_t n);
+/* seed_occurrence_cap (seeding.rs:90-97): max(floor, 8 x median occurrence). */
+uint64_t phx_seed_occurrence_cap(const phx_seed_index* index, uint64_t floor);
+void phx_seed_index_destroy(phx_seed_index* index);
+/* Resident form: create uploads the query sketches and sizes the working
+ * arrays (a seed-count pass); run seeds and chains every query (counts,
+ * offsets, then a thread per query); results packs them per query. */
+phx_chain_plan* phx_chain_plan_create(phx_ctx* ctx, const phx_seed_index* index, uint64_t max_occ,
+                                      const phx_chain_params* params, const uint64_t* q_values,
+                                      const uint32_t* q_positions, const uint64_t* q_off, uint32_t nq);
+int phx_chain_plan_run(phx_chain_plan* plan);
+int phx_chain_plan_results(phx_chain_plan* plan, phx_chain_results* out);
+double phx_chain_plan_last_kernel_ms(const phx_chain_plan* plan);
+uint64_t phx_chain_plan_seeds(const phx_chain_plan* plan);
+void phx_chain_plan_destroy(phx_chain_plan* plan);
+/* One call: create, run, results, destroy (PHX_E_ARG when creation fails). */
+int phx_seed_and_chain_batch(phx_ctx* ctx, const phx_seed_index* index, uint64_t max_occ,
+                             const phx_chain_params* params, const uint64_t* q_values,
+                             const uint32_t* q_positions, const uint64_t* q_off, uint32_t nq,
+                             phx_chain_results* out);
+
 /* --- align_read's strand handling for a batch --------------------------------
  *
  * phraya-align/src/executor.rs:573-733 after seeding and chaining: each read

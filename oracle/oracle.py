@@ -96,6 +96,11 @@ def lib():
         L.orc_extract_variants.argtypes = [u32, P, P, P, P, P, P, u64, P, P, P, u64, P, u64, u32,
                                            ctypes.c_int, P, u64, P, u64, P, P, P]
         L.orc_extract_variants.restype = ctypes.c_int
+        L.orc_seed_occurrence_cap.argtypes = [P, P, u64, u64]
+        L.orc_seed_occurrence_cap.restype = ctypes.c_uint64
+        L.orc_seed_and_chain.argtypes = [P, P, u64, u64, P, P, P, u32, P, ctypes.c_int, P, P, u64,
+                                         P, P, u64, P, P]
+        L.orc_seed_and_chain.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -232,3 +237,68 @@ def extract_variants(runs, run_off, target_start, q_seq, q_off, target, spans, s
         out = np.zeros(max(1, nv.value), VARIANT_DTYPE)
         cov = np.zeros(max(1, nc.value), np.uint32)
     raise RuntimeError("orc_extract_variants: capacity")
+
+
+# --- pre-extension: seeding and chaining (seeding.rs, chaining.rs) ----------
+
+SEED_DTYPE = np.dtype([("query_pos", np.uint32), ("target_pos", np.uint32), ("minimizer", np.uint64)])
+CHAIN_DTYPE = np.dtype([("score", np.int64), ("target_start", np.uint64), ("seed_off", np.uint64),
+                        ("n_seeds", np.uint32), ("reserved", np.uint32)])
+
+
+class OrcChainParams(ctypes.Structure):
+    _fields_ = [("seed_len", ctypes.c_int64), ("max_band", ctypes.c_int64),
+                ("max_dist_x", ctypes.c_int64), ("max_iter", ctypes.c_uint64),
+                ("max_skip", ctypes.c_uint64), ("gap_penalty", ctypes.c_double),
+                ("min_chain_score", ctypes.c_int64)]
+
+
+def chain_params(**kw) -> OrcChainParams:
+    """ChainParams::default() (chaining.rs:69-85) with overrides."""
+    d = dict(seed_len=21, max_band=100, max_dist_x=100, max_iter=5000, max_skip=25,
+             gap_penalty=0.21, min_chain_score=25)
+    d.update(kw)
+    return OrcChainParams(**d)
+
+
+def _sketch(mins):
+    a = np.asarray(mins, dtype=np.uint64).reshape(-1, 2) if len(mins) else np.zeros((0, 2), np.uint64)
+    return np.ascontiguousarray(a[:, 0]), np.ascontiguousarray(a[:, 1].astype(np.uint32))
+
+
+def seed_occurrence_cap(target_mins, floor: int) -> int:
+    """seed_occurrence_cap (seeding.rs:90-97) of the index built from a
+    target sketch [(value, position), ...]."""
+    v, p = _sketch(target_mins)
+    return int(lib().orc_seed_occurrence_cap(v.ctypes.data, p.ctypes.data, len(v), floor))
+
+
+def seed_and_chain(target_mins, queries, max_occ=2**64 - 1, params=None, threads=None):
+    """seed_and_chain (executor.rs:511-519) per query sketch: returns
+    (n_seeds, chains, chain_off, seeds) with chains as CHAIN_DTYPE (query i's
+    at chain_off[i] .. chain_off[i + 1], score descending) and their seeds
+    as SEED_DTYPE."""
+    L = lib()
+    tv, tp = _sketch(target_mins)
+    qv = np.concatenate([_sketch(q)[0] for q in queries] + [np.zeros(0, np.uint64)])
+    qp = np.concatenate([_sketch(q)[1] for q in queries] + [np.zeros(0, np.uint32)])
+    q_off = np.concatenate([[0], np.cumsum([len(q) for q in queries])]).astype(np.uint64)
+    nq = len(queries)
+    params = params or chain_params()
+    threads = threads or hardware_threads()
+    n_seeds = np.zeros(max(nq, 1), np.uint64)
+    chain_off = np.zeros(nq + 1, np.uint64)
+    tc, ts = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    chains = np.zeros(1, CHAIN_DTYPE)
+    seeds = np.zeros(1, SEED_DTYPE)
+    for _ in range(2):
+        rc = L.orc_seed_and_chain(tv.ctypes.data, tp.ctypes.data, len(tv), max_occ, qv.ctypes.data,
+                                  qp.ctypes.data, q_off.ctypes.data, nq, ctypes.byref(params), threads,
+                                  n_seeds.ctypes.data, chains.ctypes.data, len(chains),
+                                  chain_off.ctypes.data, seeds.ctypes.data, len(seeds),
+                                  ctypes.byref(tc), ctypes.byref(ts))
+        if rc == 0:
+            return n_seeds[:nq], chains[:tc.value], chain_off, seeds[:ts.value]
+        chains = np.zeros(max(1, tc.value), CHAIN_DTYPE)
+        seeds = np.zeros(max(1, ts.value), SEED_DTYPE)
+    raise RuntimeError("orc_seed_and_chain: capacity")

@@ -15,9 +15,11 @@
 #include <algorithm>
 #include <atomic>
 #include <climits>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 namespace {
@@ -1047,6 +1049,218 @@ int orc_extract_variants(uint32_t n, const uint32_t* runs, const uint64_t* run_o
     variants_of(in, i, out + var_off[i], cov + co[i], &a, &b);
     for (uint64_t k = var_off[i]; k < var_off[i + 1]; ++k) out[k].coverage_off += co[i];
   });
+  return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Pre-extension step: seeding and chaining (phraya-align/src/seeding.rs,
+// phraya-align/src/chaining.rs), restated with upstream's data structures and
+// orders: a hash map value -> target positions in sketch order
+// (build_minimizer_index, seeding.rs:29-35), seeds in query-sketch order then
+// stable-sorted by query position (find_seeds_indexed_capped, :50-74), and
+// minibwa-style chaining DP (chain_seeds, chaining.rs:131-253) with its
+// stable sorts, backward scan, skip bookkeeping and backtrack.
+// ---------------------------------------------------------------------------
+extern "C" {
+typedef struct orc_seed {
+  uint32_t query_pos;
+  uint32_t target_pos;
+  uint64_t minimizer;
+} orc_seed;
+
+typedef struct orc_chain_params {
+  int64_t seed_len, max_band, max_dist_x;
+  uint64_t max_iter, max_skip;
+  double gap_penalty;
+  int64_t min_chain_score;
+} orc_chain_params;
+
+typedef struct orc_chain {
+  int64_t score;
+  uint64_t target_start;  // Chain::target_start (chaining.rs:31-35)
+  uint64_t seed_off;      // its seeds at [seed_off, seed_off + n_seeds) of the seed output, by query position
+  uint32_t n_seeds;
+  uint32_t reserved;
+} orc_chain;
+}
+
+namespace {
+
+using SeedIndex = std::unordered_map<uint64_t, std::vector<uint32_t>>;
+
+SeedIndex seed_index(const uint64_t* vals, const uint32_t* pos, uint64_t n) {
+  SeedIndex ix;
+  for (uint64_t i = 0; i < n; ++i) ix[vals[i]].push_back(pos[i]);  // seeding.rs:31-34
+  return ix;
+}
+
+// find_seeds_indexed_capped (seeding.rs:56-74).
+std::vector<orc_seed> find_seeds(const SeedIndex& ix, const uint64_t* qv, const uint32_t* qp, uint64_t n,
+                                 uint64_t max_occ) {
+  std::vector<orc_seed> seeds;
+  for (uint64_t i = 0; i < n; ++i) {
+    auto it = ix.find(qv[i]);
+    if (it == ix.end()) continue;
+    if (it->second.size() > max_occ) continue;  // :63-65
+    for (uint32_t tp : it->second) seeds.push_back(orc_seed{qp[i], tp, qv[i]});
+  }
+  std::stable_sort(seeds.begin(), seeds.end(),
+                   [](const orc_seed& a, const orc_seed& b) { return a.query_pos < b.query_pos; });
+  return seeds;
+}
+
+// chain_score (chaining.rs:96-117).
+bool chain_score(const orc_seed& si, const orc_seed& sj, const orc_chain_params& p, int64_t* sc) {
+  const int64_t dq = (int64_t)si.query_pos - (int64_t)sj.query_pos;
+  if (dq <= 0) return false;
+  const int64_t dr = (int64_t)si.target_pos - (int64_t)sj.target_pos;
+  if (dr <= 0 || dr > p.max_dist_x + p.seed_len) return false;
+  const int64_t dd = dr > dq ? dr - dq : dq - dr;
+  if (dd > p.max_band) return false;
+  const int64_t dg = std::min(dr, dq);
+  int64_t s = std::min(p.seed_len, dg);
+  if (dd > 0) {
+    const double lin = p.gap_penalty * (double)dd;
+    const double lg = 0.5 * std::log2((double)dd + 1.0);
+    s -= (int64_t)std::round(lin + lg);  // f64::round: half away from zero
+  }
+  *sc = s;
+  return true;
+}
+
+struct OrcChain {
+  std::vector<orc_seed> seeds;
+  int64_t score;
+};
+
+// chain_seeds (chaining.rs:131-253).
+std::vector<OrcChain> chain_seeds(const std::vector<orc_seed>& seeds, const orc_chain_params& p) {
+  std::vector<OrcChain> chains;
+  if (seeds.empty()) return chains;
+  std::vector<orc_seed> s = seeds;
+  std::stable_sort(s.begin(), s.end(), [](const orc_seed& a, const orc_seed& b) {
+    return a.target_pos != b.target_pos ? a.target_pos < b.target_pos : a.query_pos < b.query_pos;
+  });
+  const size_t n = s.size();
+  std::vector<int64_t> f(n, 0);
+  std::vector<int64_t> pred(n, -1);
+  std::vector<uint64_t> dominated(n, 0);
+  for (size_t i = 0; i < n; ++i) {
+    int64_t best = p.seed_len;
+    int64_t best_pred = -1;
+    uint64_t n_skip = 0;
+    const size_t floor = i > p.max_iter ? i - p.max_iter : 0;  // saturating_sub
+    for (size_t j = i; j-- > floor;) {
+      if ((int64_t)s[i].target_pos - (int64_t)s[j].target_pos >= p.max_dist_x + p.seed_len) break;
+      int64_t sc;
+      if (!chain_score(s[i], s[j], p, &sc)) continue;
+      const int64_t total = sc + f[j];
+      if (total > best) {
+        best = total;
+        best_pred = (int64_t)j;
+        if (n_skip > 0) --n_skip;
+      } else {
+        dominated[j] += 1;
+        if (dominated[j] > p.max_skip) {
+          n_skip += 1;
+          if (n_skip > p.max_skip) break;
+        }
+      }
+    }
+    f[i] = best;
+    pred[i] = best_pred;
+  }
+  std::vector<size_t> order(n);
+  for (size_t i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return f[a] > f[b]; });
+  std::vector<bool> used(n, false);
+  for (size_t end : order) {
+    if (used[end] || f[end] < p.min_chain_score) continue;
+    std::vector<orc_seed> rev;
+    int64_t cur = (int64_t)end;
+    while (cur >= 0) {
+      if (used[cur]) break;
+      used[cur] = true;
+      rev.push_back(s[cur]);
+      cur = pred[cur];
+    }
+    if (rev.empty()) continue;
+    std::reverse(rev.begin(), rev.end());
+    std::stable_sort(rev.begin(), rev.end(),
+                     [](const orc_seed& a, const orc_seed& b) { return a.query_pos < b.query_pos; });
+    chains.push_back(OrcChain{std::move(rev), f[end]});
+  }
+  std::stable_sort(chains.begin(), chains.end(),
+                   [](const OrcChain& a, const OrcChain& b) { return a.score > b.score; });
+  return chains;
+}
+
+uint64_t chain_target_start(const OrcChain& c) {
+  const int64_t v = (int64_t)c.seeds[0].target_pos - (int64_t)c.seeds[0].query_pos;
+  return v > 0 ? (uint64_t)v : 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// seed_occurrence_cap (seeding.rs:90-97): max(floor, 8 x the median occurrence).
+uint64_t orc_seed_occurrence_cap(const uint64_t* vals, const uint32_t* pos, uint64_t n, uint64_t floor) {
+  const SeedIndex ix = seed_index(vals, pos, n);
+  if (ix.empty()) return floor;
+  std::vector<uint64_t> counts;
+  for (const auto& kv : ix) counts.push_back(kv.second.size());
+  std::sort(counts.begin(), counts.end());
+  const uint64_t median = counts[counts.size() / 2];
+  const uint64_t m8 = median > UINT64_MAX / 8 ? UINT64_MAX : median * 8;
+  return std::max(floor, m8);
+}
+
+// seed_and_chain (executor.rs:511-519) for a batch of query sketches against
+// one target sketch.  Query i's minimizers at q_off[i] .. q_off[i + 1].  Two
+// calls: with chains_cap / seeds_cap too small it returns 1 and sets the
+// totals; outputs are per query in order: n_seeds[i], chains at
+// chain_off[i] .. chain_off[i + 1], each chain's seeds in out_seeds.
+int orc_seed_and_chain(const uint64_t* t_vals, const uint32_t* t_pos, uint64_t tn, uint64_t max_occ,
+                       const uint64_t* q_vals, const uint32_t* q_pos, const uint64_t* q_off, uint32_t nq,
+                       const orc_chain_params* p, int threads, uint64_t* n_seeds, orc_chain* chains,
+                       uint64_t chains_cap, uint64_t* chain_off, orc_seed* out_seeds, uint64_t seeds_cap,
+                       uint64_t* total_chains, uint64_t* total_seeds) {
+  const SeedIndex ix = seed_index(t_vals, t_pos, tn);
+  std::vector<std::vector<OrcChain>> per(nq);
+  std::vector<uint64_t> ns(nq, 0);
+  const int T = std::max(1, threads);
+  std::vector<std::thread> pool;
+  for (int w = 0; w < T; ++w)
+    pool.emplace_back([&, w]() {
+      for (uint32_t i = (uint32_t)w; i < nq; i += (uint32_t)T) {
+        const auto seeds = find_seeds(ix, q_vals + q_off[i], q_pos + q_off[i], q_off[i + 1] - q_off[i], max_occ);
+        ns[i] = seeds.size();
+        per[i] = chain_seeds(seeds, *p);
+      }
+    });
+  for (auto& t : pool) t.join();
+  uint64_t C = 0, S = 0;
+  for (uint32_t i = 0; i < nq; ++i) {
+    C += per[i].size();
+    for (const auto& c : per[i]) S += c.seeds.size();
+  }
+  *total_chains = C;
+  *total_seeds = S;
+  for (uint32_t i = 0; i < nq; ++i) n_seeds[i] = ns[i];
+  if (C > chains_cap || S > seeds_cap) return 1;
+  uint64_t c = 0, s = 0;
+  for (uint32_t i = 0; i < nq; ++i) {
+    chain_off[i] = c;
+    for (const auto& ch : per[i]) {
+      chains[c] = orc_chain{ch.score, chain_target_start(ch), s, (uint32_t)ch.seeds.size(), 0};
+      for (const auto& sd : ch.seeds) out_seeds[s++] = sd;
+      ++c;
+    }
+  }
+  chain_off[nq] = c;
   return 0;
 }
 

@@ -757,6 +757,140 @@ def _read_extensions(out, n, aln, role, runs, off) -> list:
     return res
 
 
+# --- pre-extension: seeding and chaining --------------------------------------
+
+def default_chain_params(**kw) -> "L.CChainParams":
+    """ChainParams::default() (chaining.rs:69-85) with overrides."""
+    d = dict(seed_len=21, max_band=100, max_dist_x=100, max_iter=5000, max_skip=25,
+             gap_penalty=0.21, min_chain_score=25)
+    d.update(kw)
+    return L.CChainParams(**d)
+
+
+def _sketch_arrays(mins):
+    a = np.asarray(mins, dtype=np.uint64).reshape(-1, 2) if len(mins) else np.zeros((0, 2), np.uint64)
+    return np.ascontiguousarray(a[:, 0]), np.ascontiguousarray(a[:, 1].astype(np.uint32))
+
+
+class SeedIndex:
+    """build_minimizer_index (seeding.rs:29-35) of a target sketch
+    [(value, position), ...], resident on the device."""
+
+    def __init__(self, target_mins, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        v, p = _sketch_arrays(target_mins)
+        self._h = L.lib().phx_seed_index_create(self.ctx.handle, v.ctypes.data if len(v) else None,
+                                                p.ctypes.data if len(p) else None, len(v))
+        if not self._h:
+            raise RuntimeError(f"phx_seed_index_create failed: {self.ctx.last_error()}")
+        _track(self, 0)
+
+    def occurrence_cap(self, floor: int) -> int:
+        """seed_occurrence_cap (seeding.rs:90-97)."""
+        return int(L.lib().phx_seed_occurrence_cap(self._h, floor))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            L.lib().phx_seed_index_destroy(self._h)
+            self._h = None
+
+    def _release(self) -> None:
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class Chains:
+    n_seeds: np.ndarray    # per query
+    chains: np.ndarray     # CHAIN_DTYPE, query i's at chain_off[i] .. chain_off[i + 1]
+    chain_off: np.ndarray
+    seeds: np.ndarray      # SEED_DTYPE, each chain's at seed_off .. + n_seeds
+
+
+class ChainPlan:
+    """seed_and_chain (executor.rs:511-519) for a batch of query sketches
+    against a SeedIndex, resident on the device (phx_chain_plan_*)."""
+
+    def __init__(self, index: SeedIndex, queries, max_occ: int = 2 ** 64 - 1, params=None):
+        self.ctx = index.ctx
+        self.index = index
+        qs = [_sketch_arrays(q) for q in queries]
+        self._qv = np.concatenate([a for a, _ in qs] + [np.zeros(0, np.uint64)])
+        self._qp = np.concatenate([b for _, b in qs] + [np.zeros(0, np.uint32)])
+        self._off = np.concatenate([[0], np.cumsum([len(q) for q in queries])]).astype(np.uint64)
+        self.n = len(queries)
+        self._params = params or default_chain_params()
+        self._h = L.lib().phx_chain_plan_create(self.ctx.handle, index._h, max_occ, ctypes.byref(self._params),
+                                                self._qv.ctypes.data if len(self._qv) else None,
+                                                self._qp.ctypes.data if len(self._qp) else None,
+                                                self._off.ctypes.data, self.n)
+        if not self._h:
+            raise RuntimeError(f"phx_chain_plan_create failed: {self.ctx.last_error()}")
+        _track(self, 0)
+
+    @property
+    def seeds(self) -> int:
+        return int(L.lib().phx_chain_plan_seeds(self._h))
+
+    def run(self) -> None:
+        rc = L.lib().phx_chain_plan_run(self._h)
+        if rc != L.PHX_OK:
+            raise RuntimeError(f"phx_chain_plan_run failed ({rc}): {self.ctx.last_error()}")
+
+    def last_kernel_ms(self) -> float:
+        return float(L.lib().phx_chain_plan_last_kernel_ms(self._h))
+
+    def results(self) -> Chains:
+        n_seeds = np.zeros(max(self.n, 1), np.uint64)
+        off = np.zeros(self.n + 1, np.uint64)
+        out = L.CChainResults(None, 0, None, 0, off.ctypes.data, n_seeds.ctypes.data, 0, 0)
+        rc = L.lib().phx_chain_plan_results(self._h, ctypes.byref(out))
+        ch, sd = np.zeros(0, L.CHAIN_DTYPE), np.zeros(0, L.SEED_DTYPE)
+        if rc == L.PHX_E_CAPACITY:
+            ch = np.zeros(max(1, out.n_chains), L.CHAIN_DTYPE)
+            sd = np.zeros(max(1, out.n_chain_seeds), L.SEED_DTYPE)
+            out.chains, out.chains_cap = ch.ctypes.data, len(ch)
+            out.seeds, out.seeds_cap = sd.ctypes.data, len(sd)
+            rc = L.lib().phx_chain_plan_results(self._h, ctypes.byref(out))
+        if rc != L.PHX_OK:
+            raise RuntimeError(f"phx_chain_plan_results failed ({rc}): {self.ctx.last_error()}")
+        return Chains(n_seeds[:self.n], ch[:out.n_chains], off, sd[:out.n_chain_seeds])
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            L.lib().phx_chain_plan_destroy(self._h)
+            self._h = None
+
+    def _release(self) -> None:
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def seed_and_chain(target_mins, queries, max_occ: int = 2 ** 64 - 1, params=None,
+                   ctx: Optional[Context] = None) -> Chains:
+    """One-shot: index the target sketch, seed and chain every query sketch."""
+    ix = SeedIndex(target_mins, ctx)
+    try:
+        p = ChainPlan(ix, queries, max_occ, params)
+        try:
+            p.run()
+            return p.results()
+        finally:
+            p.close()
+    finally:
+        ix.close()
+
+
 # --- align_read's strand handling ---------------------------------------------
 
 STRAND_FORWARD, STRAND_REVERSE = 0, 1

@@ -41,6 +41,9 @@ EXPORTS = (
     "phx_contig_plan_layout", "phx_contig_plan_gap_plan", "phx_contig_plan_last_ms",
     "phx_contig_plan_last_gpu_ms", "phx_contig_plan_last_stitch_ms", "phx_contig_plan_destroy",
     "phx_version", "phx_chain_cap", "phx_extend_chains_bounds", "phx_extend_chains_batch",
+    "phx_seed_index_create", "phx_seed_occurrence_cap", "phx_seed_index_destroy", "phx_chain_plan_create",
+    "phx_chain_plan_run", "phx_chain_plan_results", "phx_chain_plan_last_kernel_ms", "phx_chain_plan_seeds",
+    "phx_chain_plan_destroy", "phx_seed_and_chain_batch",
     "phx_score_alignments", "phx_reverse_complement", "phx_align_reads_bounds",
     "phx_align_reads_batch", "phx_variant_plan_create", "phx_variant_plan_run",
     "phx_variant_plan_results", "phx_variant_plan_last_kernel_ms", "phx_variant_plan_destroy",
@@ -102,6 +105,26 @@ class CReadPlacement(ctypes.Structure):
     _fields_ = [("outcome", ctypes.c_int32), ("strand", ctypes.c_int32), ("entry", ctypes.c_int32),
                 ("entry_fwd", ctypes.c_int32), ("entry_rev", ctypes.c_int32),
                 ("primary_score", ctypes.c_double)]
+
+
+class CChainParams(ctypes.Structure):
+    _fields_ = [("seed_len", ctypes.c_int64), ("max_band", ctypes.c_int64),
+                ("max_dist_x", ctypes.c_int64), ("max_iter", ctypes.c_uint64),
+                ("max_skip", ctypes.c_uint64), ("gap_penalty", ctypes.c_double),
+                ("min_chain_score", ctypes.c_int64)]
+
+
+class CChainResults(ctypes.Structure):
+    _fields_ = [("chains", ctypes.c_void_p), ("chains_cap", ctypes.c_uint64),
+                ("seeds", ctypes.c_void_p), ("seeds_cap", ctypes.c_uint64),
+                ("chain_off", ctypes.c_void_p), ("n_seeds", ctypes.c_void_p),
+                ("n_chains", ctypes.c_uint64), ("n_chain_seeds", ctypes.c_uint64)]
+
+
+# phx_seed / phx_chain (include/phraya_hip.h) as numpy records.
+SEED_DTYPE = np.dtype([("query_pos", np.uint32), ("target_pos", np.uint32), ("minimizer", np.uint64)])
+CHAIN_DTYPE = np.dtype([("score", np.int64), ("target_start", np.uint64), ("seed_off", np.uint64),
+                        ("n_seeds", np.uint32), ("reserved", np.uint32)])
 
 
 # phx_variant (include/phraya_hip.h) as a numpy record.
@@ -214,6 +237,23 @@ def lib() -> ctypes.CDLL:
                                           ctypes.c_uint32, P, P, P, ctypes.c_uint64, P,
                                           ctypes.c_uint64, P]
     L.phx_score_alignments.argtypes = [P, P, P, ctypes.c_uint32, ctypes.c_uint64, P, P]
+    L.phx_seed_index_create.argtypes = [P, P, P, ctypes.c_uint64]
+    L.phx_seed_index_create.restype = P
+    L.phx_seed_occurrence_cap.argtypes = [P, ctypes.c_uint64]
+    L.phx_seed_occurrence_cap.restype = ctypes.c_uint64
+    L.phx_seed_index_destroy.argtypes = [P]
+    L.phx_seed_index_destroy.restype = None
+    L.phx_chain_plan_create.argtypes = [P, P, ctypes.c_uint64, P, P, P, P, ctypes.c_uint32]
+    L.phx_chain_plan_create.restype = P
+    L.phx_chain_plan_run.argtypes = [P]
+    L.phx_chain_plan_results.argtypes = [P, P]
+    L.phx_chain_plan_last_kernel_ms.argtypes = [P]
+    L.phx_chain_plan_last_kernel_ms.restype = ctypes.c_double
+    L.phx_chain_plan_seeds.argtypes = [P]
+    L.phx_chain_plan_seeds.restype = ctypes.c_uint64
+    L.phx_chain_plan_destroy.argtypes = [P]
+    L.phx_chain_plan_destroy.restype = None
+    L.phx_seed_and_chain_batch.argtypes = [P, P, ctypes.c_uint64, P, P, P, P, ctypes.c_uint32, P]
     L.phx_reverse_complement.argtypes = [P, ctypes.c_uint64, P]
     L.phx_reverse_complement.restype = None
     L.phx_align_reads_bounds.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, P,

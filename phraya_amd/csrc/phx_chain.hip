@@ -1,0 +1,552 @@
+// phx_chain.hip -- pre-extension step: seeding and chaining for a batch.
+//
+// seed_and_chain (phraya-align/src/executor.rs:511-519) for many query
+// sketches against one target sketch:
+//  * build_minimizer_index (seeding.rs:29-35): value -> target positions in
+//    sketch order.  Here a sorted array of the distinct values with, per
+//    value, its positions in sketch order (built on the host once per
+//    target, like upstream's TargetContext);
+//  * find_seeds_indexed_capped (seeding.rs:56-74): for each query minimizer
+//    in sketch order, every target position of its value unless the value
+//    occurs more than max_occ times; then a stable sort by query position;
+//  * chain_seeds (chaining.rs:131-253): stable sort by (target, query)
+//    position, the backward-scan DP with chain_score (:96-117), the
+//    dominated / skip bookkeeping, endpoints by score (stable), backtrack
+//    without reuse, each chain's seeds stable-sorted by query position,
+//    chains by score (stable).
+// Every sort here sorts by a key made unique by the element's earlier
+// position, so the heap sorts give exactly the stable sorts' orders.  The
+// gap penalty round(gap_penalty * dd + 0.5 * log2(dd + 1)) takes only
+// max_band values; the host computes them once with the same double
+// arithmetic as upstream and the kernel reads the table.
+//
+// Three launches per run: seed counts (a thread per query), offsets (a scan
+// per 1024-query tile), then a thread per query for its seeds, sorts, DP and
+// chains, with its working arrays at its seed offset.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "phraya_hip.h"
+#include "phx_host.h"
+
+namespace {
+
+constexpr uint32_t TILE = 1024;
+
+struct ChainDev {
+  const uint64_t* ix_val;  // distinct target minimizer values, ascending
+  uint64_t ix_n;
+  const uint64_t* ix_off;  // ix_n + 1: positions of value k at ix_pos[ix_off[k] .. ix_off[k + 1])
+  const uint32_t* ix_pos;
+  uint64_t max_occ;
+  const uint64_t* q_val;
+  const uint32_t* q_pos;
+  const uint64_t* q_off;
+  uint32_t nq;
+  int64_t seed_len, max_band, max_dist_x, min_chain_score;
+  uint64_t max_iter, max_skip;
+  const int64_t* pen;  // pen[dd], dd in [1, max_band]
+};
+
+struct Work {  // per seed, at the query's seed offset
+  uint32_t *gq, *gt;   // seeds in generation order
+  uint64_t* gv;
+  uint32_t* ord;       // permutations
+  uint32_t *sq, *st;   // seeds by (target, query) position
+  uint64_t* sv;
+  int64_t* f;
+  int32_t* pred;
+  uint32_t* dom;
+  uint8_t* used;
+  phx_chain* tmp;      // chains in backtrack order
+};
+
+__device__ inline bool lookup(const ChainDev& c, uint64_t v, uint64_t* lo, uint64_t* hi) {
+  uint64_t a = 0, b = c.ix_n;
+  while (a < b) {
+    const uint64_t m = (a + b) >> 1;
+    if (c.ix_val[m] < v) a = m + 1;
+    else b = m;
+  }
+  if (a == c.ix_n || c.ix_val[a] != v) return false;
+  *lo = c.ix_off[a];
+  *hi = c.ix_off[a + 1];
+  return true;
+}
+
+template <class Less>
+__device__ void sift_down(uint32_t* a, uint32_t root, uint32_t n, const Less& less) {
+  while (2 * root + 1 < n) {
+    uint32_t ch = 2 * root + 1;
+    if (ch + 1 < n && less(a[ch], a[ch + 1])) ++ch;
+    if (!less(a[root], a[ch])) return;
+    const uint32_t x = a[root];
+    a[root] = a[ch];
+    a[ch] = x;
+    root = ch;
+  }
+}
+
+// Ascending by a strict total order (keys made unique by the caller).
+template <class Less>
+__device__ void heap_sort(uint32_t* a, uint32_t n, const Less& less) {
+  for (uint32_t s = n / 2; s-- > 0;) sift_down(a, s, n, less);
+  for (uint32_t e = n; e-- > 1;) {
+    const uint32_t x = a[0];
+    a[0] = a[e];
+    a[e] = x;
+    sift_down(a, 0, e, less);
+  }
+}
+
+__global__ __launch_bounds__(256) void seed_count_kernel(ChainDev c, uint64_t* n_seeds) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= c.nq) return;
+  uint64_t n = 0;
+  for (uint64_t m = c.q_off[q]; m < c.q_off[q + 1]; ++m) {
+    uint64_t lo, hi;
+    if (lookup(c, c.q_val[m], &lo, &hi) && hi - lo <= c.max_occ) n += hi - lo;
+  }
+  n_seeds[q] = n;
+}
+
+// Exclusive prefix sums of the seed counts within each TILE-query tile;
+// the tile's total to tile_sum.
+__global__ __launch_bounds__(256) void seed_tile_scan_kernel(const uint64_t* n_seeds, uint64_t* pre,
+                                                             uint64_t* tile_sum, uint32_t n) {
+  __shared__ uint64_t ws[4];
+  const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const uint64_t base = (uint64_t)blockIdx.x * TILE + 4ull * t;
+  uint64_t x[4], s = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    x[k] = base + k < n ? n_seeds[base + k] : 0;
+    s += x[k];
+  }
+  uint64_t incl = s;
+#pragma unroll
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const uint64_t y = (uint64_t)__shfl_up((unsigned long long)incl, d, 64);
+    if (lane >= d) incl += y;
+  }
+  if (lane == 63) ws[wave] = incl;
+  __syncthreads();
+  uint64_t e = incl - s;
+  for (uint32_t j = 0; j < wave; ++j) e += ws[j];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < n) pre[base + k] = e;
+    e += x[k];
+  }
+  if (t == 255) tile_sum[blockIdx.x] = e;
+}
+
+__global__ __launch_bounds__(64) void seed_chain_kernel(ChainDev c, const uint64_t* n_seeds, const uint64_t* pre,
+                                                        const uint64_t* tile_sum, Work w, uint64_t* seed_base,
+                                                        uint32_t* n_chains, phx_chain* out, phx_seed* cs) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= c.nq) return;
+  uint64_t base = pre[q];
+  for (uint32_t j = 0; j < q / TILE; ++j) base += tile_sum[j];
+  seed_base[q] = base;
+  const uint32_t n = (uint32_t)n_seeds[q];
+  uint32_t* gq = w.gq + base;
+  uint32_t* gt = w.gt + base;
+  uint64_t* gv = w.gv + base;
+  uint32_t* ord = w.ord + base;
+  // Seeds in find_seeds' generation order.
+  uint32_t g = 0;
+  for (uint64_t m = c.q_off[q]; m < c.q_off[q + 1]; ++m) {
+    uint64_t lo, hi;
+    if (!lookup(c, c.q_val[m], &lo, &hi) || hi - lo > c.max_occ) continue;
+    for (uint64_t k = lo; k < hi; ++k) {
+      gq[g] = c.q_pos[m];
+      gt[g] = c.ix_pos[k];
+      gv[g] = c.q_val[m];
+      ord[g] = g;
+      ++g;
+    }
+  }
+  // find_seeds' stable sort by query position followed by chain_seeds'
+  // stable sort by (target, query): one sort by (target, query, generation).
+  heap_sort(ord, n, [&](uint32_t a, uint32_t b) {
+    if (gt[a] != gt[b]) return gt[a] < gt[b];
+    if (gq[a] != gq[b]) return gq[a] < gq[b];
+    return a < b;
+  });
+  uint32_t* sq = w.sq + base;
+  uint32_t* st = w.st + base;
+  uint64_t* sv = w.sv + base;
+  int64_t* f = w.f + base;
+  int32_t* pred = w.pred + base;
+  uint32_t* dom = w.dom + base;
+  uint8_t* used = w.used + base;
+  for (uint32_t k = 0; k < n; ++k) {
+    sq[k] = gq[ord[k]];
+    st[k] = gt[ord[k]];
+    sv[k] = gv[ord[k]];
+    dom[k] = 0;
+    used[k] = 0;
+  }
+  const int64_t reach = c.max_dist_x + c.seed_len;
+  for (uint32_t i = 0; i < n; ++i) {
+    int64_t best = c.seed_len;
+    int32_t bp = -1;
+    uint64_t skip = 0;
+    const uint32_t first = (uint64_t)i > c.max_iter ? (uint32_t)(i - c.max_iter) : 0u;  // saturating
+    const int64_t qi = sq[i], ti = st[i];
+    for (uint32_t j = i; j-- > first;) {
+      const int64_t dr = ti - (int64_t)st[j];
+      if (dr >= reach) break;
+      const int64_t dq = qi - (int64_t)sq[j];
+      if (dq <= 0 || dr <= 0) continue;
+      const int64_t dd = dr > dq ? dr - dq : dq - dr;
+      if (dd > c.max_band) continue;
+      const int64_t sc = min(c.seed_len, min(dr, dq)) - (dd > 0 ? c.pen[dd] : 0);
+      const int64_t total = sc + f[j];
+      if (total > best) {
+        best = total;
+        bp = (int32_t)j;
+        if (skip > 0) --skip;
+      } else {
+        dom[j] += 1;
+        if (dom[j] > c.max_skip && ++skip > c.max_skip) break;
+      }
+    }
+    f[i] = best;
+    pred[i] = bp;
+  }
+  // Endpoints by score descending (stable): key (-f, index).
+  for (uint32_t k = 0; k < n; ++k) ord[k] = k;
+  heap_sort(ord, n, [&](uint32_t a, uint32_t b) { return f[a] != f[b] ? f[a] > f[b] : a < b; });
+  phx_chain* tmp = w.tmp + base;
+  phx_seed* out_seeds = cs + base;
+  uint32_t nc = 0, ns = 0;
+  for (uint32_t k = 0; k < n; ++k) {
+    const uint32_t e = ord[k];
+    if (used[e] || f[e] < c.min_chain_score) continue;
+    const uint32_t start = ns;
+    for (int32_t cur = (int32_t)e; cur >= 0 && !used[cur]; cur = pred[cur]) {
+      used[cur] = 1;
+      out_seeds[ns++] = phx_seed{sq[cur], st[cur], sv[cur]};
+    }
+    for (uint32_t a = start, b = ns - 1; a < b; ++a, --b) {  // back to target order
+      const phx_seed x = out_seeds[a];
+      out_seeds[a] = out_seeds[b];
+      out_seeds[b] = x;
+    }
+    for (uint32_t a = start + 1; a < ns; ++a) {  // stable insertion sort by query position
+      const phx_seed x = out_seeds[a];
+      uint32_t b = a;
+      while (b > start && out_seeds[b - 1].query_pos > x.query_pos) {
+        out_seeds[b] = out_seeds[b - 1];
+        --b;
+      }
+      out_seeds[b] = x;
+    }
+    const int64_t ts = (int64_t)out_seeds[start].target_pos - (int64_t)out_seeds[start].query_pos;
+    tmp[nc++] = phx_chain{f[e], (uint64_t)(ts > 0 ? ts : 0), base + start, ns - start, 0};
+  }
+  // Chains by score descending (stable): key (-score, backtrack order).
+  for (uint32_t k = 0; k < nc; ++k) ord[k] = k;
+  heap_sort(ord, nc, [&](uint32_t a, uint32_t b) {
+    return tmp[a].score != tmp[b].score ? tmp[a].score > tmp[b].score : a < b;
+  });
+  for (uint32_t k = 0; k < nc; ++k) out[base + k] = tmp[ord[k]];
+  n_chains[q] = nc;
+}
+
+struct Buf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+void free_buf(Buf& b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+bool alloc_copy(Buf& b, const void* src, size_t bytes) {
+  b.bytes = std::max<size_t>(bytes, 8);
+  if (hipMalloc(&b.p, b.bytes) != hipSuccess) {
+    b.p = nullptr;
+    return false;
+  }
+  return !src || bytes == 0 || hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+}
+
+int cfail(phx_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+}  // namespace
+
+struct phx_seed_index {
+  phx_ctx* ctx = nullptr;
+  Buf val, off, pos;
+  uint64_t n_distinct = 0, n = 0;
+  std::vector<uint64_t> counts;  // occurrences per distinct value (seed_occurrence_cap)
+};
+
+struct phx_chain_plan {
+  phx_ctx* ctx = nullptr;
+  const phx_seed_index* ix = nullptr;
+  ChainDev c{};
+  Buf q_val, q_pos, q_off, pen;
+  Buf n_seeds, pre, tile_sum, seed_base, n_chains;
+  Buf gq, gt, gv, ord, sq, st, sv, f, pred, dom, used, tmp, out, cs;
+  uint32_t n_tiles = 0;
+  uint64_t total = 0;  // seeds of the batch (fixed by the inputs)
+  bool ran = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+extern "C" {
+
+phx_seed_index* phx_seed_index_create(phx_ctx* ctx, const uint64_t* values, const uint32_t* positions,
+                                      uint64_t n) {
+  if (!ctx || (n && (!values || !positions))) return nullptr;
+  // Distinct values ascending; within a value, positions in sketch order.
+  std::vector<uint64_t> idx(n);
+  for (uint64_t i = 0; i < n; ++i) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](uint64_t a, uint64_t b) { return values[a] < values[b]; });
+  std::vector<uint64_t> val, off;
+  std::vector<uint32_t> pos(n);
+  for (uint64_t k = 0; k < n; ++k) {
+    if (k == 0 || values[idx[k]] != values[idx[k - 1]]) {
+      val.push_back(values[idx[k]]);
+      off.push_back(k);
+    }
+    pos[k] = positions[idx[k]];
+  }
+  off.push_back(n);
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (hipSetDevice(ctx->device) != hipSuccess) return nullptr;
+  phx_seed_index* X = new phx_seed_index();
+  X->ctx = ctx;
+  X->n = n;
+  X->n_distinct = val.size();
+  X->counts.resize(val.size());
+  for (size_t k = 0; k < val.size(); ++k) X->counts[k] = off[k + 1] - off[k];
+  if (!alloc_copy(X->val, val.data(), val.size() * 8) || !alloc_copy(X->off, off.data(), off.size() * 8) ||
+      !alloc_copy(X->pos, pos.data(), n * 4)) {
+    ctx->err = "phx_seed_index_create: device allocation or upload failed";
+    phx_seed_index_destroy(X);
+    return nullptr;
+  }
+  return X;
+}
+
+uint64_t phx_seed_occurrence_cap(const phx_seed_index* X, uint64_t floor) {
+  if (!X || X->counts.empty()) return floor;
+  std::vector<uint64_t> c = X->counts;
+  std::sort(c.begin(), c.end());
+  const uint64_t median = c[c.size() / 2];
+  return std::max(floor, median > UINT64_MAX / 8 ? UINT64_MAX : median * 8);
+}
+
+void phx_seed_index_destroy(phx_seed_index* X) {
+  if (!X) return;
+  free_buf(X->val);
+  free_buf(X->off);
+  free_buf(X->pos);
+  delete X;
+}
+
+phx_chain_plan* phx_chain_plan_create(phx_ctx* ctx, const phx_seed_index* X, uint64_t max_occ,
+                                      const phx_chain_params* p, const uint64_t* q_values,
+                                      const uint32_t* q_positions, const uint64_t* q_off, uint32_t nq) {
+  if (!ctx || !X || !p || !q_off) return nullptr;
+  const uint64_t nm = q_off[nq];
+  if (nm && (!q_values || !q_positions)) {
+    ctx->err = "phx_chain_plan_create: null query sketch arrays";
+    return nullptr;
+  }
+  if (p->max_band < 0 || p->max_band > (1 << 20) || p->seed_len < 0 || p->max_dist_x < 0) {
+    ctx->err = "phx_chain_plan_create: max_band must lie in [0, 2^20], seed_len and max_dist_x >= 0";
+    return nullptr;
+  }
+  // chain_score's gap penalty per diagonal drift (chaining.rs:111-115).
+  std::vector<int64_t> pen((size_t)p->max_band + 1, 0);
+  for (int64_t dd = 1; dd <= p->max_band; ++dd)
+    pen[dd] = (int64_t)std::round(p->gap_penalty * (double)dd + 0.5 * std::log2((double)dd + 1.0));
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (hipSetDevice(ctx->device) != hipSuccess) return nullptr;
+  phx_chain_plan* P = new phx_chain_plan();
+  P->ctx = ctx;
+  P->ix = X;
+  P->n_tiles = (nq + TILE - 1) / TILE;
+  bool ok = alloc_copy(P->q_val, q_values, nm * 8) && alloc_copy(P->q_pos, q_positions, nm * 4) &&
+            alloc_copy(P->q_off, q_off, ((size_t)nq + 1) * 8) && alloc_copy(P->pen, pen.data(), pen.size() * 8) &&
+            alloc_copy(P->n_seeds, nullptr, (size_t)nq * 8) && alloc_copy(P->pre, nullptr, (size_t)nq * 8) &&
+            alloc_copy(P->tile_sum, nullptr, (size_t)P->n_tiles * 8) &&
+            alloc_copy(P->seed_base, nullptr, (size_t)nq * 8) && alloc_copy(P->n_chains, nullptr, (size_t)nq * 4) &&
+            hipEventCreate(&P->ev0) == hipSuccess && hipEventCreate(&P->ev1) == hipSuccess;
+  ChainDev& c = P->c;
+  c.ix_val = (const uint64_t*)X->val.p;
+  c.ix_n = X->n_distinct;
+  c.ix_off = (const uint64_t*)X->off.p;
+  c.ix_pos = (const uint32_t*)X->pos.p;
+  c.max_occ = max_occ;
+  c.q_val = (const uint64_t*)P->q_val.p;
+  c.q_pos = (const uint32_t*)P->q_pos.p;
+  c.q_off = (const uint64_t*)P->q_off.p;
+  c.nq = nq;
+  c.seed_len = p->seed_len;
+  c.max_band = p->max_band;
+  c.max_dist_x = p->max_dist_x;
+  c.min_chain_score = p->min_chain_score;
+  c.max_iter = p->max_iter;
+  c.max_skip = p->max_skip;
+  c.pen = (const int64_t*)P->pen.p;
+  // Size the working arrays: the seed count is fixed by the inputs.
+  if (ok && nq) {
+    hipStream_t s = ctx->stream;
+    hipLaunchKernelGGL(seed_count_kernel, dim3((nq + 255) / 256), dim3(256), 0, s, c, (uint64_t*)P->n_seeds.p);
+    std::vector<uint64_t> ns(nq);
+    ok = hipGetLastError() == hipSuccess &&
+         hipMemcpyAsync(ns.data(), P->n_seeds.p, (size_t)nq * 8, hipMemcpyDeviceToHost, s) == hipSuccess &&
+         hipStreamSynchronize(s) == hipSuccess;
+    for (uint32_t i = 0; ok && i < nq; ++i) {
+      if (ns[i] >= (1ull << 31)) ok = false;  // per-query seed indices are 31-bit
+      P->total += ns[i];
+    }
+  }
+  const size_t S = (size_t)P->total;
+  ok = ok && alloc_copy(P->gq, nullptr, S * 4) && alloc_copy(P->gt, nullptr, S * 4) &&
+       alloc_copy(P->gv, nullptr, S * 8) && alloc_copy(P->ord, nullptr, S * 4) && alloc_copy(P->sq, nullptr, S * 4) &&
+       alloc_copy(P->st, nullptr, S * 4) && alloc_copy(P->sv, nullptr, S * 8) && alloc_copy(P->f, nullptr, S * 8) &&
+       alloc_copy(P->pred, nullptr, S * 4) && alloc_copy(P->dom, nullptr, S * 4) &&
+       alloc_copy(P->used, nullptr, S) && alloc_copy(P->tmp, nullptr, S * sizeof(phx_chain)) &&
+       alloc_copy(P->out, nullptr, S * sizeof(phx_chain)) && alloc_copy(P->cs, nullptr, S * sizeof(phx_seed));
+  if (!ok) {
+    ctx->err = "phx_chain_plan_create: device allocation, upload or seed count failed";
+    phx_chain_plan_destroy(P);
+    return nullptr;
+  }
+  return P;
+}
+
+int phx_chain_plan_run(phx_chain_plan* P) {
+  if (!P) return PHX_E_ARG;
+  phx_ctx* ctx = P->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (hipSetDevice(ctx->device) != hipSuccess) return cfail(ctx, PHX_E_DEVICE, "hipSetDevice");
+  hipStream_t s = ctx->stream;
+  const uint32_t nq = P->c.nq;
+  P->ran = false;
+  if (hipEventRecord(P->ev0, s) != hipSuccess) return cfail(ctx, PHX_E_DEVICE, "event");
+  if (nq) {
+    Work w;
+    w.gq = (uint32_t*)P->gq.p;
+    w.gt = (uint32_t*)P->gt.p;
+    w.gv = (uint64_t*)P->gv.p;
+    w.ord = (uint32_t*)P->ord.p;
+    w.sq = (uint32_t*)P->sq.p;
+    w.st = (uint32_t*)P->st.p;
+    w.sv = (uint64_t*)P->sv.p;
+    w.f = (int64_t*)P->f.p;
+    w.pred = (int32_t*)P->pred.p;
+    w.dom = (uint32_t*)P->dom.p;
+    w.used = (uint8_t*)P->used.p;
+    w.tmp = (phx_chain*)P->tmp.p;
+    hipLaunchKernelGGL(seed_count_kernel, dim3((nq + 255) / 256), dim3(256), 0, s, P->c, (uint64_t*)P->n_seeds.p);
+    hipLaunchKernelGGL(seed_tile_scan_kernel, dim3(P->n_tiles), dim3(256), 0, s, (const uint64_t*)P->n_seeds.p,
+                       (uint64_t*)P->pre.p, (uint64_t*)P->tile_sum.p, nq);
+    hipLaunchKernelGGL(seed_chain_kernel, dim3((nq + 63) / 64), dim3(64), 0, s, P->c,
+                       (const uint64_t*)P->n_seeds.p, (const uint64_t*)P->pre.p, (const uint64_t*)P->tile_sum.p, w,
+                       (uint64_t*)P->seed_base.p, (uint32_t*)P->n_chains.p, (phx_chain*)P->out.p,
+                       (phx_seed*)P->cs.p);
+    if (hipGetLastError() != hipSuccess) return cfail(ctx, PHX_E_DEVICE, "seed / chain launch");
+  }
+  if (hipEventRecord(P->ev1, s) != hipSuccess || hipEventSynchronize(P->ev1) != hipSuccess)
+    return cfail(ctx, PHX_E_DEVICE, "seed / chain run");
+  P->ran = true;
+  return PHX_OK;
+}
+
+int phx_chain_plan_results(phx_chain_plan* P, phx_chain_results* out) {
+  if (!P || !out) return PHX_E_ARG;
+  phx_ctx* ctx = P->ctx;
+  if (!P->ran) return cfail(ctx, PHX_E_ARG, "phx_chain_plan_results before a run");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  const uint32_t nq = P->c.nq;
+  const size_t S = (size_t)P->total;
+  std::vector<uint64_t> ns(nq), base(nq);
+  std::vector<uint32_t> nc(nq);
+  std::vector<phx_chain> ch(S);
+  std::vector<phx_seed> cs(S);
+  if (hipSetDevice(ctx->device) != hipSuccess ||
+      (nq && (hipMemcpy(ns.data(), P->n_seeds.p, (size_t)nq * 8, hipMemcpyDeviceToHost) != hipSuccess ||
+              hipMemcpy(base.data(), P->seed_base.p, (size_t)nq * 8, hipMemcpyDeviceToHost) != hipSuccess ||
+              hipMemcpy(nc.data(), P->n_chains.p, (size_t)nq * 4, hipMemcpyDeviceToHost) != hipSuccess)) ||
+      (S && (hipMemcpy(ch.data(), P->out.p, S * sizeof(phx_chain), hipMemcpyDeviceToHost) != hipSuccess ||
+             hipMemcpy(cs.data(), P->cs.p, S * sizeof(phx_seed), hipMemcpyDeviceToHost) != hipSuccess)))
+    return cfail(ctx, PHX_E_DEVICE, "seed / chain download");
+  uint64_t C = 0, CS = 0;
+  for (uint32_t i = 0; i < nq; ++i) {
+    C += nc[i];
+    for (uint32_t k = 0; k < nc[i]; ++k) CS += ch[base[i] + k].n_seeds;
+  }
+  out->n_chains = C;
+  out->n_chain_seeds = CS;
+  if (!out->chain_off || (nq && !out->n_seeds) || out->chains_cap < C || out->seeds_cap < CS ||
+      (C && !out->chains) || (CS && !out->seeds))
+    return PHX_E_CAPACITY;
+  uint64_t c = 0, s = 0;
+  for (uint32_t i = 0; i < nq; ++i) {
+    out->n_seeds[i] = ns[i];
+    out->chain_off[i] = c;
+    for (uint32_t k = 0; k < nc[i]; ++k) {
+      phx_chain x = ch[base[i] + k];
+      std::copy(cs.begin() + x.seed_off, cs.begin() + x.seed_off + x.n_seeds, out->seeds + s);
+      x.seed_off = s;
+      s += x.n_seeds;
+      out->chains[c++] = x;
+    }
+  }
+  out->chain_off[nq] = c;
+  return PHX_OK;
+}
+
+double phx_chain_plan_last_kernel_ms(const phx_chain_plan* P) {
+  if (!P || !P->ran) return 0.0;
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, P->ev0, P->ev1) != hipSuccess) return 0.0;
+  return ms;
+}
+
+uint64_t phx_chain_plan_seeds(const phx_chain_plan* P) { return P ? P->total : 0; }
+
+void phx_chain_plan_destroy(phx_chain_plan* P) {
+  if (!P) return;
+  (void)hipStreamSynchronize(P->ctx->stream);
+  for (Buf* b : {&P->q_val, &P->q_pos, &P->q_off, &P->pen, &P->n_seeds, &P->pre, &P->tile_sum, &P->seed_base,
+                 &P->n_chains, &P->gq, &P->gt, &P->gv, &P->ord, &P->sq, &P->st, &P->sv, &P->f, &P->pred, &P->dom,
+                 &P->used, &P->tmp, &P->out, &P->cs})
+    free_buf(*b);
+  if (P->ev0) (void)hipEventDestroy(P->ev0);
+  if (P->ev1) (void)hipEventDestroy(P->ev1);
+  delete P;
+}
+
+int phx_seed_and_chain_batch(phx_ctx* ctx, const phx_seed_index* X, uint64_t max_occ, const phx_chain_params* p,
+                             const uint64_t* q_values, const uint32_t* q_positions, const uint64_t* q_off,
+                             uint32_t nq, phx_chain_results* out) {
+  if (!ctx || !X || !p || !q_off || !out) return PHX_E_ARG;
+  phx_chain_plan* P = phx_chain_plan_create(ctx, X, max_occ, p, q_values, q_positions, q_off, nq);
+  if (!P) return PHX_E_ARG;
+  int rc = phx_chain_plan_run(P);
+  if (rc == PHX_OK) rc = phx_chain_plan_results(P, out);
+  phx_chain_plan_destroy(P);
+  return rc;
+}
+
+}  // extern "C"
