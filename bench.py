@@ -42,6 +42,10 @@ WORKLOADS = {
                  "post-extension step: variant sites (SNVs, deletions, insertions with +-25 bp "
                  "coverage windows and repeat / hotspot flags) from the CIGAR runs of the 10k "
                  "short150 affine alignments, placed on the 5 Mbp genome"),
+    "chain": ("short150", "chain", (0, 0, 0),
+              "pre-extension step: seeds of the 10k short150 reads' (21, 11)-minimizer sketches "
+              "against the 5 Mbp genome's sketch (occurrence cap max(256, 8 x median)), "
+              "chained with ChainParams::default()"),
     "serve": (None, "affine", (4, 6, 2),
               "Poisson request stream: 60% 150 bp reads (1%), 30% long reads 1-50 kbp (8%, "
               "indel-heavy), 10% 5-20 kbp contig gaps (2%), affine WFA 4/6/2 score+CIGAR against "
@@ -434,6 +438,110 @@ def run_variants(args):
     print(json.dumps(out), flush=True)
 
 
+def run_chain(args):
+    """seed_and_chain over a batch: the 10k short150 reads' minimizer sketches
+    against the 5 Mbp genome's, the index resident on the device (built once
+    per target, as upstream's TargetContext), occurrence cap from
+    seed_occurrence_cap with upstream's floor of 256.  One step = one run of
+    the resident chain plan (seed counts, offsets, seeds + sorts + DP +
+    backtrack for every read)."""
+    import phraya_amd as px
+    from phraya_amd import synth
+    from oracle import oracle as O
+
+    w = synth.workload("short150")
+    g = synth.genome(5_000_000, w.params["genome_seed"])
+    n = w.n
+    log("sketching the genome and the reads (setup)")
+    gs = synth.minimizer_sketch(g)
+    qs = [synth.minimizer_sketch(w.q_seq[w.q_off[i]:w.q_off[i + 1]]) for i in range(n)]
+    ix = px.SeedIndex(gs)
+    cap = ix.occurrence_cap(256)
+    plan = px.ChainPlan(ix, qs, cap)
+    for _ in range(args.warmup):
+        plan.run()
+    log(f"timing {args.steps} steps")
+    kern_ms, step_ms = [], []
+    t0 = time.perf_counter()
+    tp = t0
+    for _ in range(args.steps):
+        plan.run()
+        tn = time.perf_counter()
+        step_ms.append((tn - tp) * 1000.0)
+        kern_ms.append(plan.last_kernel_ms())
+        tp = time.perf_counter()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1000.0 / args.steps
+    value = n * args.steps / wall
+    got = plan.results()
+    n_seeds = int(plan.seeds)
+    plan.close()
+    log(f"{value:.4g} reads/s, {ms_per_step:.3f} ms/step; {n_seeds} seeds, {len(got.chains)} chains")
+
+    threads = len(os.sched_getaffinity(0))
+    ns, ch, off, sd = O.seed_and_chain(gs, qs, cap, threads=threads)
+    mism = int(not np.array_equal(got.n_seeds, ns)) + int(not np.array_equal(got.chain_off, off))
+    if len(ch) == len(got.chains) and len(sd) == len(got.seeds):
+        for f in ("score", "target_start", "seed_off", "n_seeds"):
+            mism += int(np.count_nonzero(got.chains[f] != ch[f]))
+        for f in O.SEED_DTYPE.names:
+            mism += int(np.count_nonzero(got.seeds[f] != sd[f]))
+    else:
+        mism += 1 + abs(len(ch) - len(got.chains))
+    log(f"parity: {n} reads, {len(ch)} chains, {len(sd)} chained seeds against the oracle, {mism} mismatches")
+
+    # Algorithmic bytes: the query sketches (12 B per minimizer), one target
+    # position read per seed (4 B), the chains and chained seeds written
+    # (32 B, 16 B) and the per-read counts (8 B).
+    n_min = sum(len(q) for q in qs)
+    alg = 12 * n_min + 4 * n_seeds + 32 * len(ch) + 16 * len(sd) + 8 * n
+    kmean = float(np.mean(kern_ms))
+    achieved = alg / (kmean / 1000.0) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"cpu baseline ~{args.cpu_seconds:.0f} s")
+        one = O.seed_and_chain_timed(gs, qs, cap, threads=threads, passes=1)
+        passes = max(1, int(args.cpu_seconds / max(one, 1e-3)))
+        el = O.seed_and_chain_timed(gs, qs, cap, threads=threads, passes=passes)
+        el1 = O.seed_and_chain_timed(gs, qs, cap, threads=1, passes=1)
+        cpu = {"value": passes * n / el, "unit": "reads/s", "cores": threads, "kind": "port",
+               "sample": f"{passes} passes over the whole batch ({n} reads), {el:.1f} s wall, index "
+                         f"built once beforehand; oracle/ C++ restatement of find_seeds_indexed_capped "
+                         f"+ chain_seeds (-O3, std::unordered_map index), reads split over {threads} threads",
+               "cpu_model": cpu_model(), "nproc": os.cpu_count(), "one_thread_per_s": n / el1}
+        log(f"cpu: {cpu['value']:.4g} reads/s ({threads} threads), {n / el1:.4g} on 1")
+
+    out = {
+        "metric": "seed-and-chain reads/s (pre-extension step), bit-exact vs CPU",
+        "value": value,
+        "unit": "reads/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": "chain", "description": WORKLOADS["chain"][3], "reads": n,
+                   "query_minimizers": n_min, "target_minimizers": int(len(gs)),
+                   "seeds": n_seeds, "chains": int(len(ch)), "max_occ": int(cap),
+                   "parallelism": "single GPU"},
+        "latency_ms": {"p50": float(np.percentile(step_ms, 50)),
+                       "p99": float(np.percentile(step_ms, 99)),
+                       "what": "one whole batch as the caller sees it (host clock around the run)"},
+        "kernel_ms_mean": kmean,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": alg},
+        "parity": {"checked_reads": n, "checked_chains": int(len(ch)), "mismatches": mism},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_serve(args):
     """Served stream vs the plain batched path on the same Poisson arrivals.
 
@@ -696,6 +804,8 @@ def main():
     if args.workload == "contig":
         run_contig(args)
         return
+    if args.workload == "chain":
+        return run_chain(args)
     if args.workload == "variants":
         run_variants(args)
         return

@@ -101,6 +101,9 @@ def lib():
         L.orc_seed_and_chain.argtypes = [P, P, u64, u64, P, P, P, u32, P, ctypes.c_int, P, P, u64,
                                          P, P, u64, P, P]
         L.orc_seed_and_chain.restype = ctypes.c_int
+        L.orc_seed_and_chain_timed.argtypes = [P, P, u64, u64, P, P, P, u32, P, ctypes.c_int,
+                                               ctypes.c_int, P]
+        L.orc_seed_and_chain_timed.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -302,3 +305,18 @@ def seed_and_chain(target_mins, queries, max_occ=2**64 - 1, params=None, threads
         chains = np.zeros(max(1, tc.value), CHAIN_DTYPE)
         seeds = np.zeros(max(1, ts.value), SEED_DTYPE)
     raise RuntimeError("orc_seed_and_chain: capacity")
+
+
+def seed_and_chain_timed(target_mins, queries, max_occ=2**64 - 1, params=None, threads=None, passes=1):
+    """Seconds for `passes` rounds of seed_and_chain over the queries, the
+    index built once beforehand (CPU baseline)."""
+    tv, tp = _sketch(target_mins)
+    qv = np.concatenate([_sketch(q)[0] for q in queries] + [np.zeros(0, np.uint64)])
+    qp = np.concatenate([_sketch(q)[1] for q in queries] + [np.zeros(0, np.uint32)])
+    q_off = np.concatenate([[0], np.cumsum([len(q) for q in queries])]).astype(np.uint64)
+    params = params or chain_params()
+    chk = ctypes.c_uint64(0)
+    return float(lib().orc_seed_and_chain_timed(tv.ctypes.data, tp.ctypes.data, len(tv), max_occ,
+                                                qv.ctypes.data, qp.ctypes.data, q_off.ctypes.data,
+                                                len(queries), ctypes.byref(params),
+                                                threads or hardware_threads(), passes, ctypes.byref(chk)))

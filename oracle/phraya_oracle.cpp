@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -1262,6 +1263,35 @@ int orc_seed_and_chain(const uint64_t* t_vals, const uint32_t* t_pos, uint64_t t
   }
   chain_off[nq] = c;
   return 0;
+}
+
+// CPU baseline: the index built once (as upstream's TargetContext holds it),
+// then `passes` rounds of seed_and_chain over every query; returns the
+// seconds of the passes alone.
+double orc_seed_and_chain_timed(const uint64_t* t_vals, const uint32_t* t_pos, uint64_t tn, uint64_t max_occ,
+                                const uint64_t* q_vals, const uint32_t* q_pos, const uint64_t* q_off, uint32_t nq,
+                                const orc_chain_params* p, int threads, int passes, uint64_t* checksum) {
+  const SeedIndex ix = seed_index(t_vals, t_pos, tn);
+  const int T = std::max(1, threads);
+  std::atomic<uint64_t> sum{0};
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int r = 0; r < passes; ++r) {
+    std::vector<std::thread> pool;
+    for (int w = 0; w < T; ++w)
+      pool.emplace_back([&, w]() {
+        uint64_t s = 0;
+        for (uint32_t i = (uint32_t)w; i < nq; i += (uint32_t)T) {
+          const auto seeds = find_seeds(ix, q_vals + q_off[i], q_pos + q_off[i], q_off[i + 1] - q_off[i], max_occ);
+          const auto ch = chain_seeds(seeds, *p);
+          s += seeds.size() + ch.size() + (ch.empty() ? 0 : chain_target_start(ch[0]));
+        }
+        sum += s;
+      });
+    for (auto& t : pool) t.join();
+  }
+  const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  *checksum = sum.load();
+  return el;
 }
 
 }  // extern "C"

@@ -3,9 +3,10 @@
 // seed_and_chain (phraya-align/src/executor.rs:511-519) for many query
 // sketches against one target sketch:
 //  * build_minimizer_index (seeding.rs:29-35): value -> target positions in
-//    sketch order.  Here a sorted array of the distinct values with, per
-//    value, its positions in sketch order (built on the host once per
-//    target, like upstream's TargetContext);
+//    sketch order.  Here an open-addressing table of the distinct values
+//    (16-byte slots: value, first position, count) over the positions
+//    grouped by value in sketch order, built on the host once per target
+//    like upstream's TargetContext, so a lookup is usually one load;
 //  * find_seeds_indexed_capped (seeding.rs:56-74): for each query minimizer
 //    in sketch order, every target position of its value unless the value
 //    occurs more than max_occ times; then a stable sort by query position;
@@ -20,9 +21,9 @@
 // max_band values; the host computes them once with the same double
 // arithmetic as upstream and the kernel reads the table.
 //
-// Three launches per run: seed counts (a thread per query), offsets (a scan
-// per 1024-query tile), then a thread per query for its seeds, sorts, DP and
-// chains, with its working arrays at its seed offset.
+// Three launches per run: seed counts (a wave per query), offsets (a scan
+// per 1024-query tile), then a wave per query for its seeds, sorts, DP and
+// chains, in LDS up to 256 seeds, else in global memory at its seed offset.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,12 +37,26 @@
 
 namespace {
 
-constexpr uint32_t TILE = 1024;
+constexpr uint32_t TILE = 1024;       // queries per block of the offset scan
+constexpr uint32_t LDS_SEEDS = 256;   // seeds a query may have for the LDS path
+constexpr uint32_t CW = 2;            // waves (queries) per workgroup of the chain kernel
+
+// Open-addressing slot of the index: a distinct value with its positions at
+// ix_pos[start .. start + count); count == 0 marks an empty slot.
+struct IxSlot {
+  uint64_t value;
+  uint32_t start, count;
+};
+
+__host__ __device__ inline uint64_t mix64(uint64_t z) {  // SplitMix64 finalizer
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
 
 struct ChainDev {
-  const uint64_t* ix_val;  // distinct target minimizer values, ascending
-  uint64_t ix_n;
-  const uint64_t* ix_off;  // ix_n + 1: positions of value k at ix_pos[ix_off[k] .. ix_off[k + 1])
+  const IxSlot* slots;  // linear probing, at most half full
+  uint64_t mask;
   const uint32_t* ix_pos;
   uint64_t max_occ;
   const uint64_t* q_val;
@@ -53,7 +68,7 @@ struct ChainDev {
   const int64_t* pen;  // pen[dd], dd in [1, max_band]
 };
 
-struct Work {  // per seed, at the query's seed offset
+struct Work {  // per seed, at the query's seed offset (queries above LDS_SEEDS seeds)
   uint32_t *gq, *gt;   // seeds in generation order
   uint64_t* gv;
   uint32_t* ord;       // permutations
@@ -66,17 +81,57 @@ struct Work {  // per seed, at the query's seed offset
   phx_chain* tmp;      // chains in backtrack order
 };
 
-__device__ inline bool lookup(const ChainDev& c, uint64_t v, uint64_t* lo, uint64_t* hi) {
-  uint64_t a = 0, b = c.ix_n;
-  while (a < b) {
-    const uint64_t m = (a + b) >> 1;
-    if (c.ix_val[m] < v) a = m + 1;
-    else b = m;
+// The value's seeds (start, count) if the target holds it at most max_occ
+// times (find_seeds_indexed_capped's mask); count 0 otherwise.
+__device__ inline void lookup(const ChainDev& c, uint64_t v, uint32_t* start, uint32_t* count) {
+  uint64_t h = mix64(v) & c.mask;
+  *start = 0;
+  *count = 0;
+  while (true) {
+    const IxSlot s = c.slots[h];
+    if (s.count == 0) return;
+    if (s.value == v) {
+      if (s.count <= c.max_occ) {
+        *start = s.start;
+        *count = s.count;
+      }
+      return;
+    }
+    h = (h + 1) & c.mask;
   }
-  if (a == c.ix_n || c.ix_val[a] != v) return false;
-  *lo = c.ix_off[a];
-  *hi = c.ix_off[a + 1];
-  return true;
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline uint64_t wave_sum(uint64_t x) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) x += (uint64_t)__shfl_xor((unsigned long long)x, d, 64);
+  return x;
+}
+
+__device__ inline uint32_t wave_excl_scan32(uint32_t x, uint32_t lane, uint32_t* total) {
+  uint32_t s = x;
+#pragma unroll
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(s, d, 64);
+    if (lane >= d) s += y;
+  }
+  *total = __shfl(s, 63, 64);
+  return s - x;
+}
+
+// chain_score (chaining.rs:96-117) with the gap penalty from the table;
+// INT64_MIN when j cannot precede i.
+__device__ inline int64_t link_score(const ChainDev& c, int64_t qi, int64_t ti, int64_t qj, int64_t tj) {
+  const int64_t dq = qi - qj, dr = ti - tj;
+  if (dq <= 0 || dr <= 0 || dr > c.max_dist_x + c.seed_len) return INT64_MIN;
+  const int64_t dd = dr > dq ? dr - dq : dq - dr;
+  if (dd > c.max_band) return INT64_MIN;
+  return min(c.seed_len, min(dr, dq)) - (dd > 0 ? c.pen[dd] : 0);
 }
 
 template <class Less>
@@ -104,19 +159,22 @@ __device__ void heap_sort(uint32_t* a, uint32_t n, const Less& less) {
   }
 }
 
+// Pass 1, a wave per query: its seed count (lanes over its minimizers).
 __global__ __launch_bounds__(256) void seed_count_kernel(ChainDev c, uint64_t* n_seeds) {
-  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= c.nq) return;
+  const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (q >= c.nq) return;  // whole waves
   uint64_t n = 0;
-  for (uint64_t m = c.q_off[q]; m < c.q_off[q + 1]; ++m) {
-    uint64_t lo, hi;
-    if (lookup(c, c.q_val[m], &lo, &hi) && hi - lo <= c.max_occ) n += hi - lo;
+  for (uint64_t m = c.q_off[q] + lane; m < c.q_off[q + 1]; m += 64) {
+    uint32_t st, cnt;
+    lookup(c, c.q_val[m], &st, &cnt);
+    n += cnt;
   }
-  n_seeds[q] = n;
+  n = wave_sum(n);
+  if (lane == 0) n_seeds[q] = n;
 }
 
-// Exclusive prefix sums of the seed counts within each TILE-query tile;
-// the tile's total to tile_sum.
+// Pass 2: exclusive prefix sums of the seed counts within each TILE-query
+// tile; the tile's total to tile_sum.
 __global__ __launch_bounds__(256) void seed_tile_scan_kernel(const uint64_t* n_seeds, uint64_t* pre,
                                                              uint64_t* tile_sum, uint32_t n) {
   __shared__ uint64_t ws[4];
@@ -146,27 +204,21 @@ __global__ __launch_bounds__(256) void seed_tile_scan_kernel(const uint64_t* n_s
   if (t == 255) tile_sum[blockIdx.x] = e;
 }
 
-__global__ __launch_bounds__(64) void seed_chain_kernel(ChainDev c, const uint64_t* n_seeds, const uint64_t* pre,
-                                                        const uint64_t* tile_sum, Work w, uint64_t* seed_base,
-                                                        uint32_t* n_chains, phx_chain* out, phx_seed* cs) {
-  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= c.nq) return;
-  uint64_t base = pre[q];
-  for (uint32_t j = 0; j < q / TILE; ++j) base += tile_sum[j];
-  seed_base[q] = base;
-  const uint32_t n = (uint32_t)n_seeds[q];
+// One query on one thread with its working arrays in global memory: the
+// path for queries with more than LDS_SEEDS seeds.
+__device__ void chain_serial(const ChainDev& c, uint32_t q, uint64_t base, uint32_t n, const Work& w,
+                             phx_chain* out, phx_seed* cs, uint32_t* n_chains) {
   uint32_t* gq = w.gq + base;
   uint32_t* gt = w.gt + base;
   uint64_t* gv = w.gv + base;
   uint32_t* ord = w.ord + base;
-  // Seeds in find_seeds' generation order.
-  uint32_t g = 0;
+  uint32_t g = 0;  // seeds in find_seeds' generation order
   for (uint64_t m = c.q_off[q]; m < c.q_off[q + 1]; ++m) {
-    uint64_t lo, hi;
-    if (!lookup(c, c.q_val[m], &lo, &hi) || hi - lo > c.max_occ) continue;
-    for (uint64_t k = lo; k < hi; ++k) {
+    uint32_t lo, cnt;
+    lookup(c, c.q_val[m], &lo, &cnt);
+    for (uint32_t k = 0; k < cnt; ++k) {
       gq[g] = c.q_pos[m];
-      gt[g] = c.ix_pos[k];
+      gt[g] = c.ix_pos[lo + k];
       gv[g] = c.q_val[m];
       ord[g] = g;
       ++g;
@@ -201,13 +253,9 @@ __global__ __launch_bounds__(64) void seed_chain_kernel(ChainDev c, const uint64
     const uint32_t first = (uint64_t)i > c.max_iter ? (uint32_t)(i - c.max_iter) : 0u;  // saturating
     const int64_t qi = sq[i], ti = st[i];
     for (uint32_t j = i; j-- > first;) {
-      const int64_t dr = ti - (int64_t)st[j];
-      if (dr >= reach) break;
-      const int64_t dq = qi - (int64_t)sq[j];
-      if (dq <= 0 || dr <= 0) continue;
-      const int64_t dd = dr > dq ? dr - dq : dq - dr;
-      if (dd > c.max_band) continue;
-      const int64_t sc = min(c.seed_len, min(dr, dq)) - (dd > 0 ? c.pen[dd] : 0);
+      if (ti - (int64_t)st[j] >= reach) break;
+      const int64_t sc = link_score(c, qi, ti, sq[j], st[j]);
+      if (sc == INT64_MIN) continue;
       const int64_t total = sc + f[j];
       if (total > best) {
         best = total;
@@ -221,8 +269,7 @@ __global__ __launch_bounds__(64) void seed_chain_kernel(ChainDev c, const uint64
     f[i] = best;
     pred[i] = bp;
   }
-  // Endpoints by score descending (stable): key (-f, index).
-  for (uint32_t k = 0; k < n; ++k) ord[k] = k;
+  for (uint32_t k = 0; k < n; ++k) ord[k] = k;  // endpoints: key (-f, index)
   heap_sort(ord, n, [&](uint32_t a, uint32_t b) { return f[a] != f[b] ? f[a] > f[b] : a < b; });
   phx_chain* tmp = w.tmp + base;
   phx_seed* out_seeds = cs + base;
@@ -252,13 +299,211 @@ __global__ __launch_bounds__(64) void seed_chain_kernel(ChainDev c, const uint64
     const int64_t ts = (int64_t)out_seeds[start].target_pos - (int64_t)out_seeds[start].query_pos;
     tmp[nc++] = phx_chain{f[e], (uint64_t)(ts > 0 ? ts : 0), base + start, ns - start, 0};
   }
-  // Chains by score descending (stable): key (-score, backtrack order).
-  for (uint32_t k = 0; k < nc; ++k) ord[k] = k;
+  for (uint32_t k = 0; k < nc; ++k) ord[k] = k;  // chains: key (-score, backtrack order)
   heap_sort(ord, nc, [&](uint32_t a, uint32_t b) {
     return tmp[a].score != tmp[b].score ? tmp[a].score > tmp[b].score : a < b;
   });
   for (uint32_t k = 0; k < nc; ++k) out[base + k] = tmp[ord[k]];
   n_chains[q] = nc;
+}
+
+// A query's working arrays in LDS (queries of up to LDS_SEEDS seeds).
+struct WaveLds {
+  uint32_t gq[LDS_SEEDS], gt[LDS_SEEDS], gm[LDS_SEEDS];  // generation order; later the chained seeds
+  uint32_t sq[LDS_SEEDS], st[LDS_SEEDS], sm[LDS_SEEDS];  // by (target, query, generation)
+  int64_t f[LDS_SEEDS];
+  int64_t cscore[LDS_SEEDS];
+  uint64_t cts[LDS_SEEDS];
+  int16_t pred[LDS_SEEDS];
+  uint16_t dom[LDS_SEEDS], ord[LDS_SEEDS], cstart[LDS_SEEDS], ccount[LDS_SEEDS];
+  uint8_t used[LDS_SEEDS];
+  uint32_t nc;
+};
+
+// Pass 3, a wave per query.  Up to LDS_SEEDS seeds: the lanes generate the
+// seeds (prefix sums of the per-minimizer counts give each lane its slots),
+// rank-sort them (each lane counts the keys below its own: keys are unique,
+// so the ranks are the stable sort's positions), lane 0 runs the DP and the
+// backtrack in LDS, the lanes rank the endpoints and the chains and write
+// the results.  Larger queries take chain_serial on lane 0.
+__global__ __launch_bounds__(64 * CW) void seed_chain_kernel(ChainDev c, const uint64_t* n_seeds,
+                                                             const uint64_t* pre, const uint64_t* tile_sum,
+                                                             Work w, uint64_t* seed_base, uint32_t* n_chains,
+                                                             phx_chain* out, phx_seed* cs) {
+  __shared__ WaveLds lds[CW];
+  const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t q = blockIdx.x * CW + wv;
+  if (q >= c.nq) return;  // whole waves; no workgroup barrier below
+  WaveLds& S = lds[wv];
+  uint64_t tp = 0;
+  for (uint32_t j = lane; j < q / TILE; j += 64) tp += tile_sum[j];
+  const uint64_t base = pre[q] + wave_sum(tp);
+  const uint32_t n = (uint32_t)n_seeds[q];
+  if (lane == 0) seed_base[q] = base;
+  if (n > LDS_SEEDS) {
+    if (lane == 0) chain_serial(c, q, base, n, w, out, cs, n_chains);
+    return;
+  }
+  const uint64_t m0 = c.q_off[q], m1 = c.q_off[q + 1];
+  uint32_t g = 0;
+  for (uint64_t mb = m0; mb < m1; mb += 64) {
+    const uint64_t m = mb + lane;
+    uint32_t lo = 0, cnt = 0;
+    if (m < m1) lookup(c, c.q_val[m], &lo, &cnt);
+    uint32_t tot;
+    const uint32_t at = g + wave_excl_scan32(cnt, lane, &tot);
+    for (uint32_t k = 0; k < cnt; ++k) {
+      S.gq[at + k] = c.q_pos[m];
+      S.gt[at + k] = c.ix_pos[lo + k];
+      S.gm[at + k] = (uint32_t)(m - m0);
+    }
+    g += tot;
+  }
+  wave_sync();
+  for (uint32_t i = lane; i < n; i += 64) {  // rank by (target, query, generation)
+    const uint32_t ti = S.gt[i], qi = S.gq[i];
+    uint32_t r = 0;
+    for (uint32_t j = 0; j < n; ++j) {
+      const uint32_t tj = S.gt[j], qj = S.gq[j];
+      r += tj < ti || (tj == ti && (qj < qi || (qj == qi && j < i)));
+    }
+    S.sq[r] = qi;
+    S.st[r] = ti;
+    S.sm[r] = S.gm[i];
+    S.dom[r] = 0;
+    S.used[r] = 0;
+  }
+  wave_sync();
+  // The DP (chaining.rs:158-196), one seed i at a time, its candidate
+  // predecessors j = i-1, i-2, ... 64 per round on the lanes in scan order.
+  // Upstream's sequential scan is reproduced exactly: the positional break
+  // cuts the candidates at the first lane too far back; j improves the
+  // running best when its total beats every earlier total (an exclusive
+  // prefix max); the skip counter (saturating -1 on an improvement, +1 on a
+  // dominated j whose count passes max_skip) is a scan of the maps
+  // x -> max(x + a, b), and the scan stops after the first lane where it
+  // passes max_skip.  Only the lanes up to that stop count as scanned.
+  const int64_t reach = c.max_dist_x + c.seed_len;
+  constexpr int32_t NEG = INT32_MIN / 4;
+  for (uint32_t i = 0; i < n; ++i) {
+    const int64_t qi = S.sq[i], ti = S.st[i];
+    int64_t best = c.seed_len;
+    int32_t bp = -1;
+    int32_t skip = 0;
+    const int64_t first = (uint64_t)i > c.max_iter ? (int64_t)(i - c.max_iter) : 0;
+    for (int64_t top = (int64_t)i - 1; top >= first; top -= 64) {
+      const int64_t j = top - (int64_t)lane;
+      const bool in = j >= first;
+      const int64_t tj = in ? (int64_t)S.st[j] : 0;
+      const uint64_t brk = __ballot(in && ti - tj >= reach);
+      const uint32_t cut = brk ? (uint32_t)__builtin_ctzll(brk) : 64u;
+      const bool act = in && lane < cut;
+      int64_t sc = INT64_MIN, total = INT64_MIN;
+      if (act) sc = link_score(c, qi, ti, (int64_t)S.sq[j], tj);
+      const bool valid = sc != INT64_MIN;
+      if (valid) total = sc + S.f[j];
+      int64_t pm = valid ? total : INT64_MIN;  // exclusive prefix max of the totals
+      int64_t incl = pm;
+#pragma unroll
+      for (uint32_t d = 1; d < 64; d <<= 1) {
+        const int64_t y = (int64_t)__shfl_up((long long)incl, d, 64);
+        if (lane >= d) incl = max(incl, y);
+      }
+      pm = (int64_t)__shfl_up((long long)incl, 1, 64);
+      if (lane == 0) pm = INT64_MIN;
+      const bool imp = valid && total > max(best, pm);
+      const bool dominated = valid && !imp;
+      const uint32_t dnew = dominated ? S.dom[j] + 1u : 0u;
+      const bool inc = dominated && dnew > c.max_skip;
+      int32_t A = imp ? -1 : inc ? 1 : 0, B = imp ? 0 : NEG;  // this lane's map x -> max(x + A, B)
+#pragma unroll
+      for (uint32_t d = 1; d < 64; d <<= 1) {  // inclusive composition, earlier lanes first
+        const int32_t pa = __shfl_up(A, d, 64), pb = __shfl_up(B, d, 64);
+        if (lane >= d) {
+          B = max(pb + A, B);
+          A = pa + A;
+        }
+      }
+      const int32_t after = max(skip + A, B);
+      const uint64_t stopm = __ballot(inc && (uint64_t)after > c.max_skip);
+      const uint32_t stop = stopm ? (uint32_t)__builtin_ctzll(stopm) : 63u;
+      const bool scanned = act && lane <= stop;
+      if (scanned && dominated) S.dom[j] = (uint16_t)dnew;
+      const uint64_t impm = __ballot(scanned && imp);
+      if (impm) {
+        const int src = 63 - __builtin_clzll(impm);
+        best = (int64_t)__shfl((long long)total, src, 64);
+        bp = (int32_t)__shfl((int)j, src, 64);
+      }
+      skip = __shfl(after, (int)stop, 64);
+      if (stopm || brk) break;
+    }
+    if (lane == 0) {
+      S.f[i] = best;
+      S.pred[i] = (int16_t)bp;
+    }
+    wave_sync();
+  }
+  for (uint32_t i = lane; i < n; i += 64) {  // endpoints by (-f, index)
+    const int64_t fi = S.f[i];
+    uint32_t r = 0;
+    for (uint32_t j = 0; j < n; ++j) r += S.f[j] > fi || (S.f[j] == fi && j < i);
+    S.ord[r] = (uint16_t)i;
+  }
+  wave_sync();
+  if (lane == 0) {  // backtrack (chaining.rs:198-241); chained seeds staged in gq / gt / gm
+    uint32_t nc = 0, ns = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+      const uint32_t e = S.ord[k];
+      if (S.used[e] || S.f[e] < c.min_chain_score) continue;
+      const uint32_t start = ns;
+      for (int32_t cur = (int32_t)e; cur >= 0 && !S.used[cur]; cur = S.pred[cur]) {
+        S.used[cur] = 1;
+        S.gq[ns] = S.sq[cur];
+        S.gt[ns] = S.st[cur];
+        S.gm[ns] = S.sm[cur];
+        ++ns;
+      }
+      for (uint32_t a = start, b = ns - 1; a < b; ++a, --b) {  // back to target order
+        uint32_t x = S.gq[a]; S.gq[a] = S.gq[b]; S.gq[b] = x;
+        x = S.gt[a]; S.gt[a] = S.gt[b]; S.gt[b] = x;
+        x = S.gm[a]; S.gm[a] = S.gm[b]; S.gm[b] = x;
+      }
+      for (uint32_t a = start + 1; a < ns; ++a) {  // stable insertion sort by query position
+        const uint32_t xq = S.gq[a], xt = S.gt[a], xm = S.gm[a];
+        uint32_t b = a;
+        while (b > start && S.gq[b - 1] > xq) {
+          S.gq[b] = S.gq[b - 1];
+          S.gt[b] = S.gt[b - 1];
+          S.gm[b] = S.gm[b - 1];
+          --b;
+        }
+        S.gq[b] = xq;
+        S.gt[b] = xt;
+        S.gm[b] = xm;
+      }
+      const int64_t ts = (int64_t)S.gt[start] - (int64_t)S.gq[start];
+      S.cscore[nc] = S.f[e];
+      S.cts[nc] = (uint64_t)(ts > 0 ? ts : 0);
+      S.cstart[nc] = (uint16_t)start;
+      S.ccount[nc] = (uint16_t)(ns - start);
+      ++nc;
+    }
+    S.nc = nc;
+  }
+  wave_sync();
+  const uint32_t nc = S.nc;
+  uint32_t ns = 0;
+  for (uint32_t k = 0; k < nc; ++k) ns += S.ccount[k];
+  for (uint32_t k = lane; k < ns; k += 64)
+    cs[base + k] = phx_seed{S.gq[k], S.gt[k], c.q_val[m0 + S.gm[k]]};
+  for (uint32_t i = lane; i < nc; i += 64) {  // chains by (-score, backtrack order)
+    const int64_t si = S.cscore[i];
+    uint32_t r = 0;
+    for (uint32_t j = 0; j < nc; ++j) r += S.cscore[j] > si || (S.cscore[j] == si && j < i);
+    out[base + r] = phx_chain{si, S.cts[i], base + S.cstart[i], S.ccount[i], 0};
+  }
+  if (lane == 0) n_chains[q] = nc;
 }
 
 struct Buf {
@@ -290,8 +535,8 @@ int cfail(phx_ctx* ctx, int code, const std::string& msg) {
 
 struct phx_seed_index {
   phx_ctx* ctx = nullptr;
-  Buf val, off, pos;
-  uint64_t n_distinct = 0, n = 0;
+  Buf slots, pos;
+  uint64_t mask = 0, n = 0;
   std::vector<uint64_t> counts;  // occurrences per distinct value (seed_occurrence_cap)
 };
 
@@ -313,30 +558,42 @@ extern "C" {
 phx_seed_index* phx_seed_index_create(phx_ctx* ctx, const uint64_t* values, const uint32_t* positions,
                                       uint64_t n) {
   if (!ctx || (n && (!values || !positions))) return nullptr;
-  // Distinct values ascending; within a value, positions in sketch order.
+  if (n >= (1ull << 32)) {
+    ctx->err = "phx_seed_index_create: more than 2^32 - 1 minimizers";
+    return nullptr;
+  }
+  // Distinct values; within a value, positions in sketch order
+  // (build_minimizer_index pushes them in that order).
   std::vector<uint64_t> idx(n);
   for (uint64_t i = 0; i < n; ++i) idx[i] = i;
   std::stable_sort(idx.begin(), idx.end(), [&](uint64_t a, uint64_t b) { return values[a] < values[b]; });
-  std::vector<uint64_t> val, off;
   std::vector<uint32_t> pos(n);
-  for (uint64_t k = 0; k < n; ++k) {
-    if (k == 0 || values[idx[k]] != values[idx[k - 1]]) {
-      val.push_back(values[idx[k]]);
-      off.push_back(k);
+  std::vector<uint64_t> counts;
+  uint64_t slots_n = 16;
+  while (slots_n < 2 * n) slots_n <<= 1;  // at most half full
+  std::vector<IxSlot> slots(slots_n, IxSlot{0, 0, 0});
+  const uint64_t mask = slots_n - 1;
+  for (uint64_t k = 0; k < n;) {
+    uint64_t e = k;
+    while (e < n && values[idx[e]] == values[idx[k]]) {
+      pos[e] = positions[idx[e]];
+      ++e;
     }
-    pos[k] = positions[idx[k]];
+    const uint64_t v = values[idx[k]];
+    uint64_t h = mix64(v) & mask;
+    while (slots[h].count) h = (h + 1) & mask;
+    slots[h] = IxSlot{v, (uint32_t)k, (uint32_t)(e - k)};
+    counts.push_back(e - k);
+    k = e;
   }
-  off.push_back(n);
   std::lock_guard<std::mutex> lk(ctx->mu);
   if (hipSetDevice(ctx->device) != hipSuccess) return nullptr;
   phx_seed_index* X = new phx_seed_index();
   X->ctx = ctx;
   X->n = n;
-  X->n_distinct = val.size();
-  X->counts.resize(val.size());
-  for (size_t k = 0; k < val.size(); ++k) X->counts[k] = off[k + 1] - off[k];
-  if (!alloc_copy(X->val, val.data(), val.size() * 8) || !alloc_copy(X->off, off.data(), off.size() * 8) ||
-      !alloc_copy(X->pos, pos.data(), n * 4)) {
+  X->mask = mask;
+  X->counts = std::move(counts);
+  if (!alloc_copy(X->slots, slots.data(), slots.size() * sizeof(IxSlot)) || !alloc_copy(X->pos, pos.data(), n * 4)) {
     ctx->err = "phx_seed_index_create: device allocation or upload failed";
     phx_seed_index_destroy(X);
     return nullptr;
@@ -354,8 +611,7 @@ uint64_t phx_seed_occurrence_cap(const phx_seed_index* X, uint64_t floor) {
 
 void phx_seed_index_destroy(phx_seed_index* X) {
   if (!X) return;
-  free_buf(X->val);
-  free_buf(X->off);
+  free_buf(X->slots);
   free_buf(X->pos);
   delete X;
 }
@@ -390,9 +646,8 @@ phx_chain_plan* phx_chain_plan_create(phx_ctx* ctx, const phx_seed_index* X, uin
             alloc_copy(P->seed_base, nullptr, (size_t)nq * 8) && alloc_copy(P->n_chains, nullptr, (size_t)nq * 4) &&
             hipEventCreate(&P->ev0) == hipSuccess && hipEventCreate(&P->ev1) == hipSuccess;
   ChainDev& c = P->c;
-  c.ix_val = (const uint64_t*)X->val.p;
-  c.ix_n = X->n_distinct;
-  c.ix_off = (const uint64_t*)X->off.p;
+  c.slots = (const IxSlot*)X->slots.p;
+  c.mask = X->mask;
   c.ix_pos = (const uint32_t*)X->pos.p;
   c.max_occ = max_occ;
   c.q_val = (const uint64_t*)P->q_val.p;
@@ -409,7 +664,7 @@ phx_chain_plan* phx_chain_plan_create(phx_ctx* ctx, const phx_seed_index* X, uin
   // Size the working arrays: the seed count is fixed by the inputs.
   if (ok && nq) {
     hipStream_t s = ctx->stream;
-    hipLaunchKernelGGL(seed_count_kernel, dim3((nq + 255) / 256), dim3(256), 0, s, c, (uint64_t*)P->n_seeds.p);
+    hipLaunchKernelGGL(seed_count_kernel, dim3((nq + 3) / 4), dim3(256), 0, s, c, (uint64_t*)P->n_seeds.p);
     std::vector<uint64_t> ns(nq);
     ok = hipGetLastError() == hipSuccess &&
          hipMemcpyAsync(ns.data(), P->n_seeds.p, (size_t)nq * 8, hipMemcpyDeviceToHost, s) == hipSuccess &&
@@ -457,10 +712,10 @@ int phx_chain_plan_run(phx_chain_plan* P) {
     w.dom = (uint32_t*)P->dom.p;
     w.used = (uint8_t*)P->used.p;
     w.tmp = (phx_chain*)P->tmp.p;
-    hipLaunchKernelGGL(seed_count_kernel, dim3((nq + 255) / 256), dim3(256), 0, s, P->c, (uint64_t*)P->n_seeds.p);
+    hipLaunchKernelGGL(seed_count_kernel, dim3((nq + 3) / 4), dim3(256), 0, s, P->c, (uint64_t*)P->n_seeds.p);
     hipLaunchKernelGGL(seed_tile_scan_kernel, dim3(P->n_tiles), dim3(256), 0, s, (const uint64_t*)P->n_seeds.p,
                        (uint64_t*)P->pre.p, (uint64_t*)P->tile_sum.p, nq);
-    hipLaunchKernelGGL(seed_chain_kernel, dim3((nq + 63) / 64), dim3(64), 0, s, P->c,
+    hipLaunchKernelGGL(seed_chain_kernel, dim3((nq + CW - 1) / CW), dim3(64 * CW), 0, s, P->c,
                        (const uint64_t*)P->n_seeds.p, (const uint64_t*)P->pre.p, (const uint64_t*)P->tile_sum.p, w,
                        (uint64_t*)P->seed_base.p, (uint32_t*)P->n_chains.p, (phx_chain*)P->out.p,
                        (phx_seed*)P->cs.p);

@@ -103,6 +103,35 @@ def genome(length: int = 5_000_000, seed: int = 0x5EED0001) -> np.ndarray:
     return g
 
 
+def minimizer_sketch(seq, k: int = 21, w: int = 11) -> np.ndarray:
+    """Forward (k, w) minimizers of an ACGT byte sequence, for synthetic
+    seeding / chaining workloads: per window of w consecutive k-mers the
+    leftmost one of smallest 64-bit hash (the SplitMix64 finalizer of its
+    2-bit packed value), one entry per selected position.  Rows are
+    (hash, position), the (value, position) layout of a MinimizerSketch.
+    phraya's own sketches come from simd-minimizers (canonical, k = 21,
+    w = 11); seeding and chaining depend only on equal values at shared
+    k-mers, which this keeps."""
+    s = np.frombuffer(bytes(seq), np.uint8) if not isinstance(seq, np.ndarray) else seq.astype(np.uint8)
+    n = len(s) - k + 1
+    if n < w:
+        return np.zeros((0, 2), np.uint64)
+    code = ((s >> 1) & 3).astype(np.uint64)  # A 0, C 1, T 2, G 3
+    km = np.zeros(n, np.uint64)
+    for j in range(k):
+        km = (km << np.uint64(2)) | code[j:j + n]
+    z = km
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    h = z ^ (z >> np.uint64(31))
+    win = np.lib.stride_tricks.sliding_window_view(h, w)
+    am = win.argmin(axis=1) + np.arange(n - w + 1)
+    keep = np.ones(len(am), bool)
+    keep[1:] = am[1:] != am[:-1]
+    pos = am[keep]
+    return np.stack([h[pos], pos.astype(np.uint64)], axis=1)
+
+
 def generate(p: dict, name: str = "custom") -> Workload:
     L = _load()
     g = genome(p.get("genome_len", 5_000_000), p.get("genome_seed", 0x5EED0001))
