@@ -38,7 +38,7 @@
 namespace {
 
 constexpr uint32_t TILE = 1024;       // queries per block of the offset scan
-constexpr uint32_t LDS_SEEDS = 256;   // seeds a query may have for the LDS path
+constexpr uint32_t LDS_SEEDS = 128;   // seeds a query may have for the LDS path
 constexpr uint32_t CW = 2;            // waves (queries) per workgroup of the chain kernel
 
 // Open-addressing slot of the index: a distinct value with its positions at
@@ -397,6 +397,8 @@ __global__ __launch_bounds__(64 * CW) void seed_chain_kernel(ChainDev c, const u
       const int64_t tj = in ? (int64_t)S.st[j] : 0;
       const uint64_t brk = __ballot(in && ti - tj >= reach);
       const uint32_t cut = brk ? (uint32_t)__builtin_ctzll(brk) : 64u;
+      // Lanes [0, span) take part; the scans need only log2(span) steps.
+      const uint32_t span = (uint32_t)min((int64_t)cut, top - first + 1);
       const bool act = in && lane < cut;
       int64_t sc = INT64_MIN, total = INT64_MIN;
       if (act) sc = link_score(c, qi, ti, (int64_t)S.sq[j], tj);
@@ -404,8 +406,7 @@ __global__ __launch_bounds__(64 * CW) void seed_chain_kernel(ChainDev c, const u
       if (valid) total = sc + S.f[j];
       int64_t pm = valid ? total : INT64_MIN;  // exclusive prefix max of the totals
       int64_t incl = pm;
-#pragma unroll
-      for (uint32_t d = 1; d < 64; d <<= 1) {
+      for (uint32_t d = 1; d < span; d <<= 1) {
         const int64_t y = (int64_t)__shfl_up((long long)incl, d, 64);
         if (lane >= d) incl = max(incl, y);
       }
@@ -416,8 +417,7 @@ __global__ __launch_bounds__(64 * CW) void seed_chain_kernel(ChainDev c, const u
       const uint32_t dnew = dominated ? S.dom[j] + 1u : 0u;
       const bool inc = dominated && dnew > c.max_skip;
       int32_t A = imp ? -1 : inc ? 1 : 0, B = imp ? 0 : NEG;  // this lane's map x -> max(x + A, B)
-#pragma unroll
-      for (uint32_t d = 1; d < 64; d <<= 1) {  // inclusive composition, earlier lanes first
+      for (uint32_t d = 1; d < span; d <<= 1) {  // inclusive composition, earlier lanes first
         const int32_t pa = __shfl_up(A, d, 64), pb = __shfl_up(B, d, 64);
         if (lane >= d) {
           B = max(pb + A, B);
@@ -426,7 +426,7 @@ __global__ __launch_bounds__(64 * CW) void seed_chain_kernel(ChainDev c, const u
       }
       const int32_t after = max(skip + A, B);
       const uint64_t stopm = __ballot(inc && (uint64_t)after > c.max_skip);
-      const uint32_t stop = stopm ? (uint32_t)__builtin_ctzll(stopm) : 63u;
+      const uint32_t stop = stopm ? (uint32_t)__builtin_ctzll(stopm) : (span ? span - 1 : 0u);
       const bool scanned = act && lane <= stop;
       if (scanned && dominated) S.dom[j] = (uint16_t)dnew;
       const uint64_t impm = __ballot(scanned && imp);
