@@ -124,6 +124,55 @@ __device__ inline uint32_t wave_excl_scan32(uint32_t x, uint32_t lane, uint32_t*
   return s - x;
 }
 
+// Lane shifts within each row of 16 lanes (DPP row_shr:D): a lane whose
+// source lies outside its row gets `old`.
+template <int D>
+__device__ __forceinline__ int32_t row_shr(int32_t old, int32_t x) {
+  return __builtin_amdgcn_update_dpp(old, x, 0x110 + D, 0xf, 0xf, false);
+}
+
+template <int D>
+__device__ __forceinline__ int64_t row_shr64(int64_t old, int64_t x) {
+  const int32_t lo = row_shr<D>((int32_t)(uint32_t)old, (int32_t)(uint32_t)x);
+  const int32_t hi = row_shr<D>((int32_t)(old >> 32), (int32_t)(x >> 32));
+  return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+__device__ __forceinline__ int64_t readlane64(int64_t x, int src) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, src);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(x >> 32), src);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// The DP's two scans over the lanes [0, span) of row 0, span <= 16, in DPP
+// steps: an inclusive max, and the inclusive composition of the maps
+// x -> max(x + A, B) (earlier lanes applied first; identity (0, neg)).
+template <int D>
+__device__ __forceinline__ void max_step(int64_t& v) {
+  v = max(v, row_shr64<D>(INT64_MIN, v));
+}
+
+template <int D>
+__device__ __forceinline__ void map_step(int32_t& A, int32_t& B, int32_t neg) {
+  const int32_t pa = row_shr<D>(0, A), pb = row_shr<D>(neg, B);
+  B = max(pb + A, B);
+  A = pa + A;
+}
+
+__device__ __forceinline__ void max_scan_row(int64_t& v, uint32_t span) {
+  if (span > 1) max_step<1>(v);
+  if (span > 2) max_step<2>(v);
+  if (span > 4) max_step<4>(v);
+  if (span > 8) max_step<8>(v);
+}
+
+__device__ __forceinline__ void map_scan_row(int32_t& A, int32_t& B, uint32_t span, int32_t neg) {
+  if (span > 1) map_step<1>(A, B, neg);
+  if (span > 2) map_step<2>(A, B, neg);
+  if (span > 4) map_step<4>(A, B, neg);
+  if (span > 8) map_step<8>(A, B, neg);
+}
+
 // chain_score (chaining.rs:96-117) with the gap penalty from the table;
 // INT64_MIN when j cannot precede i.
 __device__ inline int64_t link_score(const ChainDev& c, int64_t qi, int64_t ti, int64_t qj, int64_t tj) {
@@ -404,24 +453,33 @@ __global__ __launch_bounds__(64 * CW) void seed_chain_kernel(ChainDev c, const u
       if (act) sc = link_score(c, qi, ti, (int64_t)S.sq[j], tj);
       const bool valid = sc != INT64_MIN;
       if (valid) total = sc + S.f[j];
-      int64_t pm = valid ? total : INT64_MIN;  // exclusive prefix max of the totals
-      int64_t incl = pm;
-      for (uint32_t d = 1; d < span; d <<= 1) {
-        const int64_t y = (int64_t)__shfl_up((long long)incl, d, 64);
-        if (lane >= d) incl = max(incl, y);
+      int64_t incl = valid ? total : INT64_MIN;  // inclusive prefix max of the totals
+      int64_t pm;                                // exclusive
+      if (span <= 16) {  // one row: DPP steps
+        max_scan_row(incl, span);
+        pm = row_shr64<1>(INT64_MIN, incl);
+      } else {
+        for (uint32_t d = 1; d < span; d <<= 1) {
+          const int64_t y = (int64_t)__shfl_up((long long)incl, d, 64);
+          if (lane >= d) incl = max(incl, y);
+        }
+        pm = (int64_t)__shfl_up((long long)incl, 1, 64);
+        if (lane == 0) pm = INT64_MIN;
       }
-      pm = (int64_t)__shfl_up((long long)incl, 1, 64);
-      if (lane == 0) pm = INT64_MIN;
       const bool imp = valid && total > max(best, pm);
       const bool dominated = valid && !imp;
       const uint32_t dnew = dominated ? S.dom[j] + 1u : 0u;
       const bool inc = dominated && dnew > c.max_skip;
       int32_t A = imp ? -1 : inc ? 1 : 0, B = imp ? 0 : NEG;  // this lane's map x -> max(x + A, B)
-      for (uint32_t d = 1; d < span; d <<= 1) {  // inclusive composition, earlier lanes first
-        const int32_t pa = __shfl_up(A, d, 64), pb = __shfl_up(B, d, 64);
-        if (lane >= d) {
-          B = max(pb + A, B);
-          A = pa + A;
+      if (span <= 16) {
+        map_scan_row(A, B, span, NEG);
+      } else {
+        for (uint32_t d = 1; d < span; d <<= 1) {  // inclusive composition, earlier lanes first
+          const int32_t pa = __shfl_up(A, d, 64), pb = __shfl_up(B, d, 64);
+          if (lane >= d) {
+            B = max(pb + A, B);
+            A = pa + A;
+          }
         }
       }
       const int32_t after = max(skip + A, B);
@@ -430,12 +488,12 @@ __global__ __launch_bounds__(64 * CW) void seed_chain_kernel(ChainDev c, const u
       const bool scanned = act && lane <= stop;
       if (scanned && dominated) S.dom[j] = (uint16_t)dnew;
       const uint64_t impm = __ballot(scanned && imp);
-      if (impm) {
+      if (impm) {  // uniform source lane: a register read, no LDS round trip
         const int src = 63 - __builtin_clzll(impm);
-        best = (int64_t)__shfl((long long)total, src, 64);
-        bp = (int32_t)__shfl((int)j, src, 64);
+        best = readlane64(total, src);
+        bp = __builtin_amdgcn_readlane((int)j, src);
       }
-      skip = __shfl(after, (int)stop, 64);
+      skip = __builtin_amdgcn_readlane(after, (int)stop);
       if (stopm || brk) break;
     }
     if (lane == 0) {
