@@ -65,7 +65,9 @@ struct ChainDev {
   uint32_t nq;
   int64_t seed_len, max_band, max_dist_x, min_chain_score;
   uint64_t max_iter, max_skip;
-  const int64_t* pen;  // pen[dd], dd in [1, max_band]
+  const int64_t* pen;    // pen[dd], dd in [1, max_band]
+  const int32_t* pen32;  // the same, for the 32-bit LDS path
+  int32_t small;         // parameters and query positions fit the 32-bit LDS path
 };
 
 struct Work {  // per seed, at the query's seed offset (queries above LDS_SEEDS seeds)
@@ -131,25 +133,12 @@ __device__ __forceinline__ int32_t row_shr(int32_t old, int32_t x) {
   return __builtin_amdgcn_update_dpp(old, x, 0x110 + D, 0xf, 0xf, false);
 }
 
-template <int D>
-__device__ __forceinline__ int64_t row_shr64(int64_t old, int64_t x) {
-  const int32_t lo = row_shr<D>((int32_t)(uint32_t)old, (int32_t)(uint32_t)x);
-  const int32_t hi = row_shr<D>((int32_t)(old >> 32), (int32_t)(x >> 32));
-  return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
-}
-
-__device__ __forceinline__ int64_t readlane64(int64_t x, int src) {
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, src);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(x >> 32), src);
-  return (int64_t)(((uint64_t)hi << 32) | lo);
-}
-
 // The DP's two scans over the lanes [0, span) of row 0, span <= 16, in DPP
 // steps: an inclusive max, and the inclusive composition of the maps
 // x -> max(x + A, B) (earlier lanes applied first; identity (0, neg)).
 template <int D>
-__device__ __forceinline__ void max_step(int64_t& v) {
-  v = max(v, row_shr64<D>(INT64_MIN, v));
+__device__ __forceinline__ void max_step(int32_t& v) {
+  v = max(v, row_shr<D>(INT32_MIN, v));
 }
 
 template <int D>
@@ -159,7 +148,7 @@ __device__ __forceinline__ void map_step(int32_t& A, int32_t& B, int32_t neg) {
   A = pa + A;
 }
 
-__device__ __forceinline__ void max_scan_row(int64_t& v, uint32_t span) {
+__device__ __forceinline__ void max_scan_row(int32_t& v, uint32_t span) {
   if (span > 1) max_step<1>(v);
   if (span > 2) max_step<2>(v);
   if (span > 4) max_step<4>(v);
@@ -360,8 +349,8 @@ __device__ void chain_serial(const ChainDev& c, uint32_t q, uint64_t base, uint3
 struct WaveLds {
   uint32_t gq[LDS_SEEDS], gt[LDS_SEEDS], gm[LDS_SEEDS];  // generation order; later the chained seeds
   uint32_t sq[LDS_SEEDS], st[LDS_SEEDS], sm[LDS_SEEDS];  // by (target, query, generation)
-  int64_t f[LDS_SEEDS];
-  int64_t cscore[LDS_SEEDS];
+  int32_t f[LDS_SEEDS];
+  int32_t cscore[LDS_SEEDS];
   uint64_t cts[LDS_SEEDS];
   int16_t pred[LDS_SEEDS];
   uint16_t dom[LDS_SEEDS], ord[LDS_SEEDS], cstart[LDS_SEEDS], ccount[LDS_SEEDS];
@@ -389,7 +378,7 @@ __global__ __launch_bounds__(64 * CW) void seed_chain_kernel(ChainDev c, const u
   const uint64_t base = pre[q] + wave_sum(tp);
   const uint32_t n = (uint32_t)n_seeds[q];
   if (lane == 0) seed_base[q] = base;
-  if (n > LDS_SEEDS) {
+  if (n > LDS_SEEDS || !c.small) {
     if (lane == 0) chain_serial(c, q, base, n, w, out, cs, n_chains);
     return;
   }
@@ -432,44 +421,55 @@ __global__ __launch_bounds__(64 * CW) void seed_chain_kernel(ChainDev c, const u
   // dominated j whose count passes max_skip) is a scan of the maps
   // x -> max(x + a, b), and the scan stops after the first lane where it
   // passes max_skip.  Only the lanes up to that stop count as scanned.
-  const int64_t reach = c.max_dist_x + c.seed_len;
+  // In 32 bits: plan creation checked seed_len, max_dist_x, max_band and
+  // every gap penalty within 2^20 and query positions below 2^31, so the
+  // target gap of a candidate (below reach after the break test), the query
+  // gap, the scores and f (at most LDS_SEEDS x (seed_len + penalty)) fit.
+  // Seeds are sorted by target position, so ti - tj needs no sign.
+  const uint32_t reach = (uint32_t)(c.max_dist_x + c.seed_len);
+  const int32_t seed_len = (int32_t)c.seed_len, band = (int32_t)c.max_band;
+  const uint32_t mskip = c.max_skip > 0xffffffffull ? 0xffffffffu : (uint32_t)c.max_skip;
   constexpr int32_t NEG = INT32_MIN / 4;
   for (uint32_t i = 0; i < n; ++i) {
-    const int64_t qi = S.sq[i], ti = S.st[i];
-    int64_t best = c.seed_len;
-    int32_t bp = -1;
-    int32_t skip = 0;
-    const int64_t first = (uint64_t)i > c.max_iter ? (int64_t)(i - c.max_iter) : 0;
-    for (int64_t top = (int64_t)i - 1; top >= first; top -= 64) {
-      const int64_t j = top - (int64_t)lane;
+    const int32_t qi = (int32_t)S.sq[i];
+    const uint32_t ti = S.st[i];
+    int32_t best = seed_len, bp = -1, skip = 0;
+    const int32_t first = (uint64_t)i > c.max_iter ? (int32_t)(i - c.max_iter) : 0;
+    for (int32_t top = (int32_t)i - 1; top >= first; top -= 64) {
+      const int32_t j = top - (int32_t)lane;
       const bool in = j >= first;
-      const int64_t tj = in ? (int64_t)S.st[j] : 0;
+      const uint32_t tj = in ? S.st[j] : ti;
       const uint64_t brk = __ballot(in && ti - tj >= reach);
       const uint32_t cut = brk ? (uint32_t)__builtin_ctzll(brk) : 64u;
       // Lanes [0, span) take part; the scans need only log2(span) steps.
-      const uint32_t span = (uint32_t)min((int64_t)cut, top - first + 1);
+      const uint32_t span = min(cut, (uint32_t)(top - first + 1));
       const bool act = in && lane < cut;
-      int64_t sc = INT64_MIN, total = INT64_MIN;
-      if (act) sc = link_score(c, qi, ti, (int64_t)S.sq[j], tj);
-      const bool valid = sc != INT64_MIN;
-      if (valid) total = sc + S.f[j];
-      int64_t incl = valid ? total : INT64_MIN;  // inclusive prefix max of the totals
-      int64_t pm;                                // exclusive
-      if (span <= 16) {  // one row: DPP steps
+      bool valid = false;
+      int32_t total = INT32_MIN;
+      if (act) {  // chain_score (chaining.rs:96-117)
+        const int32_t dq = qi - (int32_t)S.sq[j], dr = (int32_t)(ti - tj);
+        const int32_t dd = dr > dq ? dr - dq : dq - dr;
+        if (dq > 0 && dr > 0 && dd <= band) {
+          valid = true;
+          total = min(seed_len, min(dr, dq)) - (dd > 0 ? c.pen32[dd] : 0) + S.f[j];
+        }
+      }
+      int32_t incl = total, pm;  // inclusive / exclusive prefix max of the totals
+      if (span <= 16) {          // one row: DPP steps
         max_scan_row(incl, span);
-        pm = row_shr64<1>(INT64_MIN, incl);
+        pm = row_shr<1>(INT32_MIN, incl);
       } else {
         for (uint32_t d = 1; d < span; d <<= 1) {
-          const int64_t y = (int64_t)__shfl_up((long long)incl, d, 64);
+          const int32_t y = __shfl_up(incl, d, 64);
           if (lane >= d) incl = max(incl, y);
         }
-        pm = (int64_t)__shfl_up((long long)incl, 1, 64);
-        if (lane == 0) pm = INT64_MIN;
+        pm = __shfl_up(incl, 1, 64);
+        if (lane == 0) pm = INT32_MIN;
       }
       const bool imp = valid && total > max(best, pm);
       const bool dominated = valid && !imp;
       const uint32_t dnew = dominated ? S.dom[j] + 1u : 0u;
-      const bool inc = dominated && dnew > c.max_skip;
+      const bool inc = dominated && dnew > mskip;
       int32_t A = imp ? -1 : inc ? 1 : 0, B = imp ? 0 : NEG;  // this lane's map x -> max(x + A, B)
       if (span <= 16) {
         map_scan_row(A, B, span, NEG);
@@ -483,15 +483,15 @@ __global__ __launch_bounds__(64 * CW) void seed_chain_kernel(ChainDev c, const u
         }
       }
       const int32_t after = max(skip + A, B);
-      const uint64_t stopm = __ballot(inc && (uint64_t)after > c.max_skip);
+      const uint64_t stopm = __ballot(inc && (uint32_t)after > mskip);
       const uint32_t stop = stopm ? (uint32_t)__builtin_ctzll(stopm) : (span ? span - 1 : 0u);
       const bool scanned = act && lane <= stop;
       if (scanned && dominated) S.dom[j] = (uint16_t)dnew;
       const uint64_t impm = __ballot(scanned && imp);
       if (impm) {  // uniform source lane: a register read, no LDS round trip
         const int src = 63 - __builtin_clzll(impm);
-        best = readlane64(total, src);
-        bp = __builtin_amdgcn_readlane((int)j, src);
+        best = __builtin_amdgcn_readlane(total, src);
+        bp = __builtin_amdgcn_readlane(j, src);
       }
       skip = __builtin_amdgcn_readlane(after, (int)stop);
       if (stopm || brk) break;
@@ -503,7 +503,7 @@ __global__ __launch_bounds__(64 * CW) void seed_chain_kernel(ChainDev c, const u
     wave_sync();
   }
   for (uint32_t i = lane; i < n; i += 64) {  // endpoints by (-f, index)
-    const int64_t fi = S.f[i];
+    const int32_t fi = S.f[i];
     uint32_t r = 0;
     for (uint32_t j = 0; j < n; ++j) r += S.f[j] > fi || (S.f[j] == fi && j < i);
     S.ord[r] = (uint16_t)i;
@@ -556,7 +556,7 @@ __global__ __launch_bounds__(64 * CW) void seed_chain_kernel(ChainDev c, const u
   for (uint32_t k = lane; k < ns; k += 64)
     cs[base + k] = phx_seed{S.gq[k], S.gt[k], c.q_val[m0 + S.gm[k]]};
   for (uint32_t i = lane; i < nc; i += 64) {  // chains by (-score, backtrack order)
-    const int64_t si = S.cscore[i];
+    const int32_t si = S.cscore[i];
     uint32_t r = 0;
     for (uint32_t j = 0; j < nc; ++j) r += S.cscore[j] > si || (S.cscore[j] == si && j < i);
     out[base + r] = phx_chain{si, S.cts[i], base + S.cstart[i], S.ccount[i], 0};
@@ -602,7 +602,7 @@ struct phx_chain_plan {
   phx_ctx* ctx = nullptr;
   const phx_seed_index* ix = nullptr;
   ChainDev c{};
-  Buf q_val, q_pos, q_off, pen;
+  Buf q_val, q_pos, q_off, pen, pen32;
   Buf n_seeds, pre, tile_sum, seed_base, n_chains;
   Buf gq, gt, gv, ord, sq, st, sv, f, pred, dom, used, tmp, out, cs;
   uint32_t n_tiles = 0;
@@ -689,8 +689,15 @@ phx_chain_plan* phx_chain_plan_create(phx_ctx* ctx, const phx_seed_index* X, uin
   }
   // chain_score's gap penalty per diagonal drift (chaining.rs:111-115).
   std::vector<int64_t> pen((size_t)p->max_band + 1, 0);
-  for (int64_t dd = 1; dd <= p->max_band; ++dd)
+  std::vector<int32_t> pen32(pen.size(), 0);
+  constexpr int64_t SMALL = 1 << 20;
+  bool small = p->seed_len <= SMALL && p->max_dist_x <= SMALL;
+  for (int64_t dd = 1; dd <= p->max_band; ++dd) {
     pen[dd] = (int64_t)std::round(p->gap_penalty * (double)dd + 0.5 * std::log2((double)dd + 1.0));
+    small = small && pen[dd] >= -SMALL && pen[dd] <= SMALL;
+    pen32[dd] = small ? (int32_t)pen[dd] : 0;
+  }
+  for (uint64_t m = 0; small && m < nm; ++m) small = q_positions[m] < (1u << 31);
   std::lock_guard<std::mutex> lk(ctx->mu);
   if (hipSetDevice(ctx->device) != hipSuccess) return nullptr;
   phx_chain_plan* P = new phx_chain_plan();
@@ -699,6 +706,7 @@ phx_chain_plan* phx_chain_plan_create(phx_ctx* ctx, const phx_seed_index* X, uin
   P->n_tiles = (nq + TILE - 1) / TILE;
   bool ok = alloc_copy(P->q_val, q_values, nm * 8) && alloc_copy(P->q_pos, q_positions, nm * 4) &&
             alloc_copy(P->q_off, q_off, ((size_t)nq + 1) * 8) && alloc_copy(P->pen, pen.data(), pen.size() * 8) &&
+            alloc_copy(P->pen32, pen32.data(), pen32.size() * 4) &&
             alloc_copy(P->n_seeds, nullptr, (size_t)nq * 8) && alloc_copy(P->pre, nullptr, (size_t)nq * 8) &&
             alloc_copy(P->tile_sum, nullptr, (size_t)P->n_tiles * 8) &&
             alloc_copy(P->seed_base, nullptr, (size_t)nq * 8) && alloc_copy(P->n_chains, nullptr, (size_t)nq * 4) &&
@@ -719,6 +727,8 @@ phx_chain_plan* phx_chain_plan_create(phx_ctx* ctx, const phx_seed_index* X, uin
   c.max_iter = p->max_iter;
   c.max_skip = p->max_skip;
   c.pen = (const int64_t*)P->pen.p;
+  c.pen32 = (const int32_t*)P->pen32.p;
+  c.small = small ? 1 : 0;
   // Size the working arrays: the seed count is fixed by the inputs.
   if (ok && nq) {
     hipStream_t s = ctx->stream;
@@ -841,7 +851,7 @@ uint64_t phx_chain_plan_seeds(const phx_chain_plan* P) { return P ? P->total : 0
 void phx_chain_plan_destroy(phx_chain_plan* P) {
   if (!P) return;
   (void)hipStreamSynchronize(P->ctx->stream);
-  for (Buf* b : {&P->q_val, &P->q_pos, &P->q_off, &P->pen, &P->n_seeds, &P->pre, &P->tile_sum, &P->seed_base,
+  for (Buf* b : {&P->q_val, &P->q_pos, &P->q_off, &P->pen, &P->pen32, &P->n_seeds, &P->pre, &P->tile_sum, &P->seed_base,
                  &P->n_chains, &P->gq, &P->gt, &P->gv, &P->ord, &P->sq, &P->st, &P->sv, &P->f, &P->pred, &P->dom,
                  &P->used, &P->tmp, &P->out, &P->cs})
     free_buf(*b);

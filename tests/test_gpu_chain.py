@@ -107,6 +107,16 @@ def test_random_batches(px, seed):
             max_dist_x=30, max_iter=3, max_skip=1, min_chain_score=0)
 
 
+def test_parameters_outside_the_32_bit_path(px):
+    # A seed length or gap penalties beyond 2^20 take the 64-bit path for
+    # every read; the results must not change character.
+    rng = random.Random(4)
+    target, queries = random_case(rng, 300)
+    compare(px, target, queries, label="seed_len 3e6", seed_len=3_000_000, min_chain_score=0)
+    compare(px, target, queries, label="gap penalty 4e6", gap_penalty=4.0e6, min_chain_score=0)
+    compare(px, target, queries, label="negative gap penalty", gap_penalty=-0.5)
+
+
 def test_dense_repeats_and_occurrence_cap(px):
     # Every query minimizer hits many target copies: long seed lists, ties
     # in target position, the skip heuristic and the iteration cap at work.
