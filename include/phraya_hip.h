@@ -437,9 +437,10 @@ phx_seed_index* phx_seed_index_create(phx_ctx* ctx, const uint64_t* values, cons
 /* seed_occurrence_cap (seeding.rs:90-97): max(floor, 8 x median occurrence). */
 uint64_t phx_seed_occurrence_cap(const phx_seed_index* index, uint64_t floor);
 void phx_seed_index_destroy(phx_seed_index* index);
-/* Resident form: create uploads the query sketches and sizes the working
- * arrays (a seed-count pass); run seeds and chains every query (counts,
- * offsets, then a thread per query); results packs them per query. */
+/* Resident form: create uploads the query sketches and lays out the working
+ * arrays (a seed-count pass fixes each query's offset); run is one launch,
+ * a wave per query (lookups, seed count, seeds, sorts, DP, chains); results
+ * packs them per query. */
 phx_chain_plan* phx_chain_plan_create(phx_ctx* ctx, const phx_seed_index* index, uint64_t max_occ,
                                       const phx_chain_params* params, const uint64_t* q_values,
                                       const uint32_t* q_positions, const uint64_t* q_off, uint32_t nq);

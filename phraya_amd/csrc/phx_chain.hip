@@ -21,9 +21,10 @@
 // max_band values; the host computes them once with the same double
 // arithmetic as upstream and the kernel reads the table.
 //
-// Three launches per run: seed counts (a wave per query), offsets (a scan
-// per 1024-query tile), then a wave per query for its seeds, sorts, DP and
-// chains, in LDS up to 256 seeds, else in global memory at its seed offset.
+// Plan creation counts each query's seeds once (a wave per query) to lay
+// out the working arrays; a run is one launch, a wave per query: lookups,
+// seed count, seeds, sorts, DP and chains, in LDS up to 128 seeds, else in
+// global memory at the query's offset.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,7 +38,6 @@
 
 namespace {
 
-constexpr uint32_t TILE = 1024;       // queries per block of the offset scan
 constexpr uint32_t LDS_SEEDS = 128;   // seeds a query may have for the LDS path
 constexpr uint32_t CW = 2;            // waves (queries) per workgroup of the chain kernel
 
@@ -197,7 +197,8 @@ __device__ void heap_sort(uint32_t* a, uint32_t n, const Less& less) {
   }
 }
 
-// Pass 1, a wave per query: its seed count (lanes over its minimizers).
+// A wave per query: its seed count (lanes over its minimizers); run once at
+// plan creation to lay out the working arrays.
 __global__ __launch_bounds__(256) void seed_count_kernel(ChainDev c, uint64_t* n_seeds) {
   const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (q >= c.nq) return;  // whole waves
@@ -209,37 +210,6 @@ __global__ __launch_bounds__(256) void seed_count_kernel(ChainDev c, uint64_t* n
   }
   n = wave_sum(n);
   if (lane == 0) n_seeds[q] = n;
-}
-
-// Pass 2: exclusive prefix sums of the seed counts within each TILE-query
-// tile; the tile's total to tile_sum.
-__global__ __launch_bounds__(256) void seed_tile_scan_kernel(const uint64_t* n_seeds, uint64_t* pre,
-                                                             uint64_t* tile_sum, uint32_t n) {
-  __shared__ uint64_t ws[4];
-  const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const uint64_t base = (uint64_t)blockIdx.x * TILE + 4ull * t;
-  uint64_t x[4], s = 0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    x[k] = base + k < n ? n_seeds[base + k] : 0;
-    s += x[k];
-  }
-  uint64_t incl = s;
-#pragma unroll
-  for (uint32_t d = 1; d < 64; d <<= 1) {
-    const uint64_t y = (uint64_t)__shfl_up((unsigned long long)incl, d, 64);
-    if (lane >= d) incl += y;
-  }
-  if (lane == 63) ws[wave] = incl;
-  __syncthreads();
-  uint64_t e = incl - s;
-  for (uint32_t j = 0; j < wave; ++j) e += ws[j];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    if (base + k < n) pre[base + k] = e;
-    e += x[k];
-  }
-  if (t == 255) tile_sum[blockIdx.x] = e;
 }
 
 // One query on one thread with its working arrays in global memory: the
@@ -364,30 +334,39 @@ struct WaveLds {
 // so the ranks are the stable sort's positions), lane 0 runs the DP and the
 // backtrack in LDS, the lanes rank the endpoints and the chains and write
 // the results.  Larger queries take chain_serial on lane 0.
-__global__ __launch_bounds__(64 * CW) void seed_chain_kernel(ChainDev c, const uint64_t* n_seeds,
-                                                             const uint64_t* pre, const uint64_t* tile_sum,
-                                                             Work w, uint64_t* seed_base, uint32_t* n_chains,
+__global__ __launch_bounds__(64 * CW) void seed_chain_kernel(ChainDev c, const uint64_t* seed_base, Work w,
+                                                             uint64_t* n_seeds, uint32_t* n_chains,
                                                              phx_chain* out, phx_seed* cs) {
   __shared__ WaveLds lds[CW];
   const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t q = blockIdx.x * CW + wv;
   if (q >= c.nq) return;  // whole waves; no workgroup barrier below
   WaveLds& S = lds[wv];
-  uint64_t tp = 0;
-  for (uint32_t j = lane; j < q / TILE; j += 64) tp += tile_sum[j];
-  const uint64_t base = pre[q] + wave_sum(tp);
-  const uint32_t n = (uint32_t)n_seeds[q];
-  if (lane == 0) seed_base[q] = base;
+  const uint64_t base = seed_base[q];
+  const uint64_t m0 = c.q_off[q], m1 = c.q_off[q + 1];
+  // Lookups: the first 64 minimizers' stay in registers for the generation.
+  uint32_t lo0 = 0, cnt0 = 0;
+  if (m0 + lane < m1) lookup(c, c.q_val[m0 + lane], &lo0, &cnt0);
+  uint64_t nn = cnt0;
+  for (uint64_t m = m0 + 64 + lane; m < m1; m += 64) {
+    uint32_t lo, cnt;
+    lookup(c, c.q_val[m], &lo, &cnt);
+    nn += cnt;
+  }
+  const uint32_t n = (uint32_t)wave_sum(nn);  // below 2^31 (checked when the plan was laid out)
+  if (lane == 0) n_seeds[q] = n;
   if (n > LDS_SEEDS || !c.small) {
     if (lane == 0) chain_serial(c, q, base, n, w, out, cs, n_chains);
     return;
   }
-  const uint64_t m0 = c.q_off[q], m1 = c.q_off[q + 1];
   uint32_t g = 0;
   for (uint64_t mb = m0; mb < m1; mb += 64) {
     const uint64_t m = mb + lane;
-    uint32_t lo = 0, cnt = 0;
-    if (m < m1) lookup(c, c.q_val[m], &lo, &cnt);
+    uint32_t lo = lo0, cnt = cnt0;
+    if (mb != m0) {
+      lo = cnt = 0;
+      if (m < m1) lookup(c, c.q_val[m], &lo, &cnt);
+    }
     uint32_t tot;
     const uint32_t at = g + wave_excl_scan32(cnt, lane, &tot);
     for (uint32_t k = 0; k < cnt; ++k) {
@@ -603,9 +582,8 @@ struct phx_chain_plan {
   const phx_seed_index* ix = nullptr;
   ChainDev c{};
   Buf q_val, q_pos, q_off, pen, pen32;
-  Buf n_seeds, pre, tile_sum, seed_base, n_chains;
+  Buf n_seeds, seed_base, n_chains;
   Buf gq, gt, gv, ord, sq, st, sv, f, pred, dom, used, tmp, out, cs;
-  uint32_t n_tiles = 0;
   uint64_t total = 0;  // seeds of the batch (fixed by the inputs)
   bool ran = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -703,13 +681,10 @@ phx_chain_plan* phx_chain_plan_create(phx_ctx* ctx, const phx_seed_index* X, uin
   phx_chain_plan* P = new phx_chain_plan();
   P->ctx = ctx;
   P->ix = X;
-  P->n_tiles = (nq + TILE - 1) / TILE;
   bool ok = alloc_copy(P->q_val, q_values, nm * 8) && alloc_copy(P->q_pos, q_positions, nm * 4) &&
             alloc_copy(P->q_off, q_off, ((size_t)nq + 1) * 8) && alloc_copy(P->pen, pen.data(), pen.size() * 8) &&
             alloc_copy(P->pen32, pen32.data(), pen32.size() * 4) &&
-            alloc_copy(P->n_seeds, nullptr, (size_t)nq * 8) && alloc_copy(P->pre, nullptr, (size_t)nq * 8) &&
-            alloc_copy(P->tile_sum, nullptr, (size_t)P->n_tiles * 8) &&
-            alloc_copy(P->seed_base, nullptr, (size_t)nq * 8) && alloc_copy(P->n_chains, nullptr, (size_t)nq * 4) &&
+            alloc_copy(P->n_seeds, nullptr, (size_t)nq * 8) && alloc_copy(P->n_chains, nullptr, (size_t)nq * 4) &&
             hipEventCreate(&P->ev0) == hipSuccess && hipEventCreate(&P->ev1) == hipSuccess;
   ChainDev& c = P->c;
   c.slots = (const IxSlot*)X->slots.p;
@@ -737,11 +712,15 @@ phx_chain_plan* phx_chain_plan_create(phx_ctx* ctx, const phx_seed_index* X, uin
     ok = hipGetLastError() == hipSuccess &&
          hipMemcpyAsync(ns.data(), P->n_seeds.p, (size_t)nq * 8, hipMemcpyDeviceToHost, s) == hipSuccess &&
          hipStreamSynchronize(s) == hipSuccess;
+    std::vector<uint64_t> base(nq);
     for (uint32_t i = 0; ok && i < nq; ++i) {
       if (ns[i] >= (1ull << 31)) ok = false;  // per-query seed indices are 31-bit
+      base[i] = P->total;
       P->total += ns[i];
     }
+    ok = ok && alloc_copy(P->seed_base, base.data(), (size_t)nq * 8);
   }
+  ok = ok && (P->seed_base.p || alloc_copy(P->seed_base, nullptr, 8));
   const size_t S = (size_t)P->total;
   ok = ok && alloc_copy(P->gq, nullptr, S * 4) && alloc_copy(P->gt, nullptr, S * 4) &&
        alloc_copy(P->gv, nullptr, S * 8) && alloc_copy(P->ord, nullptr, S * 4) && alloc_copy(P->sq, nullptr, S * 4) &&
@@ -780,13 +759,9 @@ int phx_chain_plan_run(phx_chain_plan* P) {
     w.dom = (uint32_t*)P->dom.p;
     w.used = (uint8_t*)P->used.p;
     w.tmp = (phx_chain*)P->tmp.p;
-    hipLaunchKernelGGL(seed_count_kernel, dim3((nq + 3) / 4), dim3(256), 0, s, P->c, (uint64_t*)P->n_seeds.p);
-    hipLaunchKernelGGL(seed_tile_scan_kernel, dim3(P->n_tiles), dim3(256), 0, s, (const uint64_t*)P->n_seeds.p,
-                       (uint64_t*)P->pre.p, (uint64_t*)P->tile_sum.p, nq);
     hipLaunchKernelGGL(seed_chain_kernel, dim3((nq + CW - 1) / CW), dim3(64 * CW), 0, s, P->c,
-                       (const uint64_t*)P->n_seeds.p, (const uint64_t*)P->pre.p, (const uint64_t*)P->tile_sum.p, w,
-                       (uint64_t*)P->seed_base.p, (uint32_t*)P->n_chains.p, (phx_chain*)P->out.p,
-                       (phx_seed*)P->cs.p);
+                       (const uint64_t*)P->seed_base.p, w, (uint64_t*)P->n_seeds.p, (uint32_t*)P->n_chains.p,
+                       (phx_chain*)P->out.p, (phx_seed*)P->cs.p);
     if (hipGetLastError() != hipSuccess) return cfail(ctx, PHX_E_DEVICE, "seed / chain launch");
   }
   if (hipEventRecord(P->ev1, s) != hipSuccess || hipEventSynchronize(P->ev1) != hipSuccess)
@@ -851,7 +826,7 @@ uint64_t phx_chain_plan_seeds(const phx_chain_plan* P) { return P ? P->total : 0
 void phx_chain_plan_destroy(phx_chain_plan* P) {
   if (!P) return;
   (void)hipStreamSynchronize(P->ctx->stream);
-  for (Buf* b : {&P->q_val, &P->q_pos, &P->q_off, &P->pen, &P->pen32, &P->n_seeds, &P->pre, &P->tile_sum, &P->seed_base,
+  for (Buf* b : {&P->q_val, &P->q_pos, &P->q_off, &P->pen, &P->pen32, &P->n_seeds, &P->seed_base,
                  &P->n_chains, &P->gq, &P->gt, &P->gv, &P->ord, &P->sq, &P->st, &P->sv, &P->f, &P->pred, &P->dom,
                  &P->used, &P->tmp, &P->out, &P->cs})
     free_buf(*b);
