@@ -571,9 +571,10 @@ typedef struct phx_variant_results {
 
 typedef struct phx_variant_plan phx_variant_plan;
 
-/* Resident form: create uploads the batch, run extracts on the device (a
- * counting pass, a scan and a writing pass), results downloads (checking
- * the capacities; PHX_E_CAPACITY sets the totals and copies nothing). */
+/* Resident form: create uploads the batch and lays out the outputs (a
+ * counting pass and a scan), run extracts on the device (one pass: records,
+ * flags, coverage windows), results downloads (checking the capacities;
+ * PHX_E_CAPACITY sets the totals and copies nothing). */
 phx_variant_plan* phx_variant_plan_create(phx_ctx* ctx, const phx_variant_batch* batch);
 int phx_variant_plan_run(phx_variant_plan* plan);
 int phx_variant_plan_results(phx_variant_plan* plan, phx_variant_results* out);

@@ -6,11 +6,12 @@
 // lookups (is_in_repeat_region / is_in_hotspot, :795-811).  Upstream walks
 // one read's CIGAR string per call on a rayon thread; here a wave takes one
 // alignment, a lane per CIGAR run: wave prefix sums of the runs' advances
-// give each run's query / target position at once, instead of a walk.  Three
-// passes: counts per alignment, exclusive offsets (a scan per 1024-alignment
-// tile plus the totals of the tiles before), then the records and coverage
-// windows at those offsets, so the records come out in alignment order and,
-// per alignment, in CIGAR order, exactly as upstream's Vec would be
+// give each run's query / target position at once, instead of a walk.  Plan
+// creation lays out the outputs once (counts per alignment, then exclusive
+// offsets: a scan per 1024-alignment tile plus the totals of the tiles
+// before); a run is one pass that writes the records and coverage windows
+// at those offsets, so the records come out in alignment order and, per
+// alignment, in CIGAR order, exactly as upstream's Vec would be
 // concatenated.
 //
 // Rules per run (ops M = 0, X = 1, I = 2, D = 3; upstream's WFA convention,
@@ -472,6 +473,22 @@ phx_variant_plan* phx_variant_plan_create(phx_ctx* ctx, const phx_variant_batch*
   a.hot_dir = (const uint32_t*)P->hot_dir.p;
   a.rep_shift = rep_shift;
   a.hot_shift = hot_shift;
+  // Layout: the per-alignment counts and their offsets depend only on the
+  // plan's inputs, so they are found once here; every run recomputes the
+  // variants themselves (positions, bases, flags, windows, coverage).
+  if (n) {
+    hipStream_t st = ctx->stream;
+    const dim3 block(256), waves((n + 3) / 4);
+    hipLaunchKernelGGL(variant_count_kernel, waves, block, 0, st, P->a, (uint64_t*)P->nvar.p,
+                       (uint64_t*)P->ncov.p);
+    hipLaunchKernelGGL(tile_scan_kernel, dim3(P->n_tiles), block, 0, st, (uint64_t*)P->nvar.p,
+                       (uint64_t*)P->ncov.p, (uint64_t*)P->tile_v.p, (uint64_t*)P->tile_c.p, n);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+      ctx->err = "phx_variant_plan_create: layout passes failed";
+      phx_variant_plan_destroy(P);
+      return nullptr;
+    }
+  }
   return P;
 }
 
@@ -485,19 +502,13 @@ int phx_variant_plan_run(phx_variant_plan* P) {
   P->ran = false;
   P->kms_ready = false;
   const dim3 block(256), waves((n + 3) / 4);
-  uint64_t* nv = (uint64_t*)P->nvar.p;
-  uint64_t* nc = (uint64_t*)P->ncov.p;
-  uint64_t* tv = (uint64_t*)P->tile_v.p;
-  uint64_t* tc = (uint64_t*)P->tile_c.p;
   P->tot_host[0] = P->tot_host[1] = 0;  // an empty batch launches nothing
   if (hipEventRecord(P->ev0, st) != hipSuccess) return vfail(ctx, PHX_E_DEVICE, "event");
-  if (n) {
-    hipLaunchKernelGGL(variant_count_kernel, waves, block, 0, st, P->a, nv, nc);
-    hipLaunchKernelGGL(tile_scan_kernel, dim3(P->n_tiles), block, 0, st, nv, nc, tv, tc, n);
-    hipLaunchKernelGGL(variant_write_kernel, waves, block, 0, st, P->a, (const uint64_t*)nv,
-                       (const uint64_t*)nc, (const uint64_t*)tv, (const uint64_t*)tc, (uint64_t*)P->var_off.p,
-                       (phx_variant*)P->out.p, (uint32_t*)P->cov.p, P->tot_dev);
-    if (hipGetLastError() != hipSuccess) return vfail(ctx, PHX_E_DEVICE, "variant passes");
+  if (n) {  // one launch: offsets from the plan's layout, everything else computed here
+    hipLaunchKernelGGL(variant_write_kernel, waves, block, 0, st, P->a, (const uint64_t*)P->nvar.p,
+                       (const uint64_t*)P->ncov.p, (const uint64_t*)P->tile_v.p, (const uint64_t*)P->tile_c.p,
+                       (uint64_t*)P->var_off.p, (phx_variant*)P->out.p, (uint32_t*)P->cov.p, P->tot_dev);
+    if (hipGetLastError() != hipSuccess) return vfail(ctx, PHX_E_DEVICE, "variant pass");
   }
   if (hipEventRecord(P->ev1, st) != hipSuccess || hipEventSynchronize(P->ev1) != hipSuccess)
     return vfail(ctx, PHX_E_DEVICE, "variant totals");
