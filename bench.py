@@ -689,6 +689,25 @@ def run_serve(args):
     short_sv = _pct(lat_sv[rs.cls == 0], 99)
     short_pl5 = _pct(lat5[rs5.cls == 0], 99)
 
+    # --- optional: the plain path's capacity at the latency target, by
+    # lowering its offered rate until its short-read p99 meets 10 ms.
+    sweep = []
+    if args.serve_plain_sweep:
+        for k in (10, 20, 40, 80, 160):
+            rk = rate / k
+            nk = max(200, int(rk * args.serve_seconds))
+            log(f"plain sweep: 1/{k} of the offered rate ({rk:.1f} req/s, {nk} requests)")
+            rsk = synth.serve_stream(nk, seed=0xC0FFEE10 + k)
+            ark = np.cumsum(rng.exponential(1.0 / rk, nk))
+            latk, donek, bk, _ = plain_loop(rsk, ark)
+            pk = _pct(latk[rsk.cls == 0], 99)
+            sweep.append({"fraction_of_offered": 1.0 / k, "offered_req_per_s": rk, "requests": nk,
+                          "pairs_per_s": nk / float(donek.max()), "batches": bk,
+                          "short_p99_ms": pk, "all_p99_ms": _pct(latk, 99)})
+            log(f"plain sweep: short p99 {pk:.1f} ms at {rk:.1f} req/s")
+            if pk < 10.0:
+                break
+
     # --- parity of the served results against the oracle on a sample
     chk = sorted(check_idx)
     mism = 0
@@ -735,6 +754,7 @@ def run_serve(args):
         "served_short_p99_ms": short_sv,
         "plain_short_p99_at_fifth_rate_ms": short_pl5,
         "served_5x_plain_at_short_p99_target": bool(short_sv < 10.0 and short_pl5 > 10.0),
+        "plain_sweep": sweep or None,
         "served_p99_over_plain_p99": _pct(lat_sv_ms, 99) / _pct(lat_pl, 99),
         "server_stats": stats,
         "status_counts": {str(k): int(v) for k, v in zip(*np.unique(st_sv, return_counts=True))},
@@ -764,6 +784,8 @@ def main():
                     help="serve: offered load as a fraction of the measured server capacity")
     ap.add_argument("--serve-rate", type=float, default=0.0,
                     help="serve: offered requests/s (overrides --serve-load)")
+    ap.add_argument("--serve-plain-sweep", action="store_true",
+                    help="serve: also lower the plain path's rate until its short-read p99 meets 10 ms")
     ap.add_argument("--serve-lanes", type=int, default=3,
                     help="serve: launch groups in flight per tier")
     ap.add_argument("--tune", default="",
