@@ -56,7 +56,7 @@ struct VarDev {
   const uint32_t* hotspots;
   uint64_t n_hotspots;
   uint32_t radius;
-  const uint32_t* rep_dir;  // interval directories (in_intervals)
+  const uint32_t* rep_dir;  // interval directories (interval_flags)
   const uint32_t* hot_dir;
   uint32_t rep_shift, hot_shift;
 };
@@ -70,17 +70,31 @@ constexpr uint32_t TILE = 1024;  // alignments per block of the offset scan
 // b << shift) narrows the search to the intervals starting in pos's bucket,
 // usually none or one, so a lookup is two or three dependent loads instead
 // of a binary search over every interval.
-template <class T>
-__device__ bool in_intervals(uint64_t pos, const T* iv, uint64_t n, const uint32_t* dir, uint32_t shift) {
-  if (n == 0) return false;
-  const uint64_t b = pos >> shift;
-  uint64_t lo = dir[b], hi = dir[b + 1];
-  while (lo < hi) {
-    const uint64_t mid = (lo + hi) >> 1;
-    if ((uint64_t)iv[2 * mid] <= pos) lo = mid + 1;
-    else hi = mid;
+// Both interval sets are looked up at once: the four directory entries are
+// loaded before either search, so the two lookups overlap.
+__device__ inline uint8_t interval_flags(const VarDev& a, uint64_t pos) {
+  uint64_t rlo = 0, rhi = 0, hlo = 0, hhi = 0;
+  if (a.n_repeats) {
+    rlo = a.rep_dir[pos >> a.rep_shift];
+    rhi = a.rep_dir[(pos >> a.rep_shift) + 1];
   }
-  return lo > 0 && pos < (uint64_t)iv[2 * lo - 1];
+  if (a.n_hotspots) {
+    hlo = a.hot_dir[pos >> a.hot_shift];
+    hhi = a.hot_dir[(pos >> a.hot_shift) + 1];
+  }
+  while (rlo < rhi) {
+    const uint64_t mid = (rlo + rhi) >> 1;
+    if (a.repeats[2 * mid] <= pos) rlo = mid + 1;
+    else rhi = mid;
+  }
+  while (hlo < hhi) {
+    const uint64_t mid = (hlo + hhi) >> 1;
+    if ((uint64_t)a.hotspots[2 * mid] <= pos) hlo = mid + 1;
+    else hhi = mid;
+  }
+  const bool rep = rlo > 0 && pos < a.repeats[2 * rlo - 1];
+  const bool hot = hlo > 0 && pos < (uint64_t)a.hotspots[2 * hlo - 1];
+  return (uint8_t)((rep ? PHX_VAR_IN_REPEAT : 0) | (hot ? PHX_VAR_IN_HOTSPOT : 0));
 }
 
 __device__ inline uint64_t shfl_up64(uint64_t x, uint32_t d) {
@@ -241,6 +255,10 @@ __global__ __launch_bounds__(256) void variant_write_kernel(VarDev a, const uint
   if (lane == 0) var_off[i] = vb;
   const uint8_t* q = a.q_seq + a.q_off[i];
   const uint64_t s0 = a.span_off[i], s1 = a.span_off[i + 1], r = a.radius;
+  // The first two coverage windows, loaded now (they do not depend on the
+  // runs); further windows are read where they are used.
+  const uint64_t w0a = s1 > s0 ? a.spans[2 * s0] : 0, w0b = s1 > s0 ? a.spans[2 * s0 + 1] : 0;
+  const uint64_t w1a = s1 > s0 + 1 ? a.spans[2 * s0 + 2] : 0, w1b = s1 > s0 + 1 ? a.spans[2 * s0 + 3] : 0;
   Walk w(a, i);
   uint64_t v = vb, c = cb;  // the next chunk's first record / coverage word
   for (uint64_t rr = w.r0; rr < w.r1; rr += 64) {
@@ -258,8 +276,7 @@ __global__ __launch_bounds__(256) void variant_write_kernel(VarDev a, const uint
       rec.type = L.op == phx::OP_X ? PHX_VAR_SNV : L.op == phx::OP_I ? PHX_VAR_DELETION : PHX_VAR_INSERTION;
       rec.ref_base = L.op == phx::OP_D ? (uint8_t)'.' : a.target[p];
       rec.alt_base = L.op == phx::OP_X ? q[L.q_pos + k] : L.op == phx::OP_I ? (uint8_t)'.' : 0;
-      rec.flags = (uint8_t)((in_intervals(p, a.repeats, a.n_repeats, a.rep_dir, a.rep_shift) ? PHX_VAR_IN_REPEAT : 0) |
-                            (in_intervals(p, a.hotspots, a.n_hotspots, a.hot_dir, a.hot_shift) ? PHX_VAR_IN_HOTSPOT : 0));
+      rec.flags = interval_flags(a, p);
       rec.ins_query_pos = L.op == phx::OP_D ? (uint32_t)L.q_pos : 0;
       rec.ins_len = L.op == phx::OP_D ? (uint32_t)(min(L.q_pos + L.len, w.qn) - L.q_pos) : 0;
       rec.window_start = (uint32_t)ws;
@@ -282,7 +299,8 @@ __global__ __launch_bounds__(256) void variant_write_kernel(VarDev a, const uint
         for (uint64_t j = lane; j < wl; j += 64) {
           const uint64_t pos = ws + j;
           uint32_t hold = 0;
-          for (uint64_t s = s0; s < s1; ++s) hold += a.spans[2 * s] <= pos && pos < a.spans[2 * s + 1];
+          hold += (w0a <= pos && pos < w0b) + (w1a <= pos && pos < w1b);  // empty when absent
+          for (uint64_t s = s0 + 2; s < s1; ++s) hold += a.spans[2 * s] <= pos && pos < a.spans[2 * s + 1];
           cov[cw + j] = hold;
         }
         cw += wl;
