@@ -424,6 +424,9 @@ def run_variants(args):
         "config": {"workload": "variants", "description": WORKLOADS["variants"][3],
                    "alignments": n, "variants": nv, "coverage_words": nc, "radius": radius,
                    "repeat_intervals": len(rep), "hotspot_intervals": len(hot),
+                   "layout": "per-alignment output offsets found once at plan creation (a counting "
+                             "pass and a scan, not timed); each step computes every record, flag "
+                             "and coverage word",
                    "parallelism": "single GPU"},
         "latency_ms": {"p50": float(np.percentile(step_ms, 50)),
                        "p99": float(np.percentile(step_ms, 99)),
@@ -528,6 +531,8 @@ def run_chain(args):
         "config": {"workload": "chain", "description": WORKLOADS["chain"][3], "reads": n,
                    "query_minimizers": n_min, "target_minimizers": int(len(gs)),
                    "seeds": n_seeds, "chains": int(len(ch)), "max_occ": int(cap),
+                   "layout": "per-read working-array offsets found once at plan creation (a seed "
+                             "count, not timed); each step looks up, counts, sorts and chains every read",
                    "parallelism": "single GPU"},
         "latency_ms": {"p50": float(np.percentile(step_ms, 50)),
                        "p99": float(np.percentile(step_ms, 99)),
