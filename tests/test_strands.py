@@ -68,3 +68,58 @@ def test_orientations_extended():
     assert bounds([(100, 3, 4)], is_dna=False) == (1, 3)
     assert bounds([(100, 0, 4)], is_dna=False) == (2, 2)
     assert bounds([(10, 1, 0), (20, 0, 1), (30, 1, 1)]) == (4, 4)
+
+
+def test_bounds_reject_malformed_reads():
+    # A null query with a length, or null chains with a count, is PHX_E_ARG.
+    for qlen, nf, nr, null_q, null_f, null_r in [(5, 0, 0, True, False, False),
+                                                 (5, 2, 0, False, True, False),
+                                                 (5, 0, 2, False, False, True)]:
+        q = np.zeros(5, np.uint8)
+        f = np.zeros(2, np.uint64)
+        arr = (L.CReadStrands * 1)()
+        arr[0] = L.CReadStrands(None if null_q else q.ctypes.data, qlen,
+                                None if null_f else f.ctypes.data, nf,
+                                None if null_r else f.ctypes.data, nr, 0)
+        e, a, n = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        rc = L.lib().phx_align_reads_bounds(2, 1, 100, arr, 1, ctypes.byref(e), ctypes.byref(a),
+                                            ctypes.byref(n))
+        assert rc == L.PHX_E_ARG
+    # Null reverse chains are fine for protein: they are never read.
+    arr = (L.CReadStrands * 1)()
+    q = np.zeros(5, np.uint8)
+    arr[0] = L.CReadStrands(q.ctypes.data, 5, None, 0, None, 3, 0)
+    e, a, n = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    assert L.lib().phx_align_reads_bounds(2, 0, 100, arr, 1, ctypes.byref(e), ctypes.byref(a),
+                                          ctypes.byref(n)) == L.PHX_OK
+    assert (e.value, a.value) == (2, 2)
+
+
+def test_bench_sketcher_selects_window_minima():
+    # synth.minimizer_sketch (the synthetic workloads' sketcher): every
+    # window of w k-mers has its leftmost smallest hash selected, and every
+    # selected position is that minimum of some window, in order, once.
+    from phraya_amd import synth
+    rng = random.Random(8)
+    for k, w in [(5, 3), (21, 11), (7, 1)]:
+        seq = bytes(rng.choice(b"ACGT") for _ in range(300))
+        sk = synth.minimizer_sketch(seq, k, w)
+        n = len(seq) - k + 1
+        code = {65: 0, 67: 1, 84: 2, 71: 3}
+        mask = (1 << 64) - 1
+
+        def h(i):
+            z = 0
+            for c in seq[i:i + k]:
+                z = (z << 2) | code[c]
+            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask
+            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & mask
+            return z ^ (z >> 31)
+        hs = [h(i) for i in range(n)]
+        want = []
+        for s in range(n - w + 1):
+            m = min(range(s, s + w), key=lambda i: (hs[i], i))
+            if not want or want[-1] != m:
+                want.append(m)
+        assert [int(p) for p in sk[:, 1]] == want
+        assert [int(v) for v in sk[:, 0]] == [hs[p] for p in want]
