@@ -58,6 +58,8 @@ def test_upstream_cases(px):
     queries = [q1, q2a + q2b, q3a + q3b, q4 + qn, qb, [], [(7, 0), (9, 5)], [(0xdead, 3)]]
     compare(px, target, queries, label="upstream")
     compare(px, target, queries, max_occ=4, label="upstream masked", min_chain_score=10)
+    got = compare(px, target, queries, max_occ=0, label="everything masked", min_chain_score=0)
+    assert int(got.n_seeds.sum()) == 0 and len(got.chains) == 0
 
 
 def test_empty_index_and_queries(px):
