@@ -119,7 +119,13 @@ typedef struct phx_tuning {
  * device < 0 selects the current device. Returns NULL on failure. */
 phx_ctx* phx_ctx_create(int device, uint32_t flags, const phx_tuning* tuning);
 void phx_ctx_destroy(phx_ctx* ctx);
-/* Last error text of this context (static storage owned by the context). */
+/* Text of the last failure of a call the calling thread made on ctx ("" when
+ * its last failure was on another context or there was none).  Errors are
+ * kept per thread, so threads sharing a context never overwrite each other's;
+ * the text stays valid until that thread's next failing call.  One-shot entry
+ * points (phx_align_batch, phx_seed_and_chain_batch, phx_extract_variants)
+ * return the failing step's code: PHX_E_NOMEM for device memory,
+ * PHX_E_DEVICE for a HIP runtime error, PHX_E_ARG for the input. */
 const char* phx_ctx_last_error(const phx_ctx* ctx);
 
 /* Upper bound of CIGAR run words a batch can need (sum of qn + tn + 1). */
@@ -263,8 +269,10 @@ int phx_serve_stop(phx_server* srv);
  * Aligns pairs of long contig segments (query i against target i, end to
  * end) by anchoring them on shared minimizers and aligning every gap between
  * anchors with the batched engine.  No counterpart in phraya-align's
- * extension API; its gap alignments are plain fill_wfa / wfa_extend_affine
- * calls (phraya-align/src/wfa_simd.rs:231-345, :919-935) on the windows.
+ * extension API; its gap alignments are the global branches fill_wfa and
+ * fill_wfa_affine_generic(fitting = false) (phraya-align/src/wfa_simd.rs:231-345,
+ * :581-779) on the windows: a gap is pinned at both ends, so the wrappers'
+ * fitting switch never applies.
  * Anchors: forward-strand (k, w) minimizers (leftmost smallest hash of a
  * mixing function of the 2-bit k-mer) that occur once among each
  * sequence's minimizers, paired by k-mer, reduced to the longest chain
@@ -272,9 +280,9 @@ int phx_serve_stop(phx_server* srv);
  * anchor kept is the first at least `spacing` query bases past the previous
  * one, spacing drawn log-uniform in [gap_min, gap_max] from a SplitMix64
  * stream seeded by (seed, pair index).  The contig CIGAR is gap 0, anchor 1
- * as k M, gap 1, ..., with equal neighbouring runs merged; a gap alignment
- * that leaves target bases unconsumed (fitting branch) adds them as I; the
- * contig score is the sum of its gap scores. */
+ * as k M, gap 1, ..., with equal neighbouring runs merged; the contig score
+ * is the sum of its gap scores, which equals the cost of the stitched CIGAR
+ * (anchors are exact matches and cost nothing). */
 typedef struct phx_contig_params {
   uint32_t k;        /* minimizer k (8..16); 0 = 15 */
   uint32_t w;        /* minimizer window in k-mers; 0 = 10 */

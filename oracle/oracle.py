@@ -21,7 +21,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 OK, INVALID_INPUT, ALIGNMENT_FAILED, BAD_COSTS = 0, 1, 2, 3
-LINEAR, AFFINE, MYERS, MYERS_GLOBAL, MYERS_FWD = 0, 1, 2, 3, 4
+LINEAR, AFFINE, MYERS, MYERS_GLOBAL, MYERS_FWD, LINEAR_GLOBAL, AFFINE_GLOBAL = 0, 1, 2, 3, 4, 5, 6
 
 
 class OrcResult(ctypes.Structure):
@@ -82,6 +82,8 @@ def lib():
         L.orc_wfa_extend.argtypes = [P, sz, P, sz, sz, sz, i64, R, P, sz]
         L.orc_wfa_extend_affine.argtypes = [P, sz, P, sz, sz, sz, i32, i32, i32, i64, R, P, sz]
         L.orc_myers_extend.argtypes = [P, sz, P, sz, sz, sz, R, P, sz]
+        L.orc_wfa_global.argtypes = [P, sz, P, sz, R, P, sz]
+        L.orc_wfa_affine_global.argtypes = [P, sz, P, sz, i32, i32, i32, R, P, sz]
         L.orc_myers_edit_distance.argtypes = [P, sz, P, sz, R, P, sz]
         L.orc_myers_extend_distance.argtypes = [P, sz, P, sz, sz, sz, R]
         L.orc_batch_run.argtypes = [ctypes.c_int, u32, P, P, P, P, P, P, i32, i32, i32, i64,
@@ -144,6 +146,17 @@ def wfa_extend_affine(q: bytes, t: bytes, seed=(0, 0), costs=(2, 4, 1), cap: int
     x, o, e = costs
     return _call(lib().orc_wfa_extend_affine, _buf(q), len(q), _buf(t), len(t), seed[0], seed[1],
                  x, o, e, cap)
+
+
+def wfa_global(q: bytes, t: bytes) -> Result:
+    """fill_wfa (wfa_simd.rs:231-345): the global branch whatever the lengths."""
+    return _call(lib().orc_wfa_global, _buf(q), len(q), _buf(t), len(t))
+
+
+def wfa_affine_global(q: bytes, t: bytes, costs=(2, 4, 1)) -> Result:
+    """fill_wfa_affine_generic with fitting = false (wfa_simd.rs:581-779)."""
+    x, o, e = costs
+    return _call(lib().orc_wfa_affine_global, _buf(q), len(q), _buf(t), len(t), x, o, e)
 
 
 def myers_extend(q: bytes, t: bytes, seed=(0, 0)) -> Result:

@@ -779,6 +779,23 @@ int run_engine(int engine, const uint8_t* q, size_t qn, const uint8_t* t, size_t
       }
       return finish(fill_wfa_affine_fitting_impl(S, c, cap), sq, st, qsn, r, cigar, ccap, keep);
     }
+    case ORC_ENGINE_LINEAR_GLOBAL: {
+      // fill_wfa (wfa_simd.rs:231-345) whatever the length ratio; no cap.
+      auto g = fill_wfa(S);
+      return finish({true, g.first, g.second, g.second, static_cast<int64_t>(tsn)}, sq, st, qsn, r,
+                    cigar, ccap, keep);
+    }
+    case ORC_ENGINE_AFFINE_GLOBAL: {
+      if (c.x < 1 || c.e < 1 || c.o < 0) {
+        r->status = ORC_BAD_COSTS;
+        return ORC_BAD_COSTS;
+      }
+      // fill_wfa_affine_generic(fitting = false) (wfa_simd.rs:581-779), the
+      // reported distance from the CIGAR as the wrapper does (:932-934).
+      EngineOut e = fill_wfa_affine_generic(S, c, false, cap);
+      if (e.some) e.edit_distance = static_cast<int64_t>(levenshtein_from_cigar(e.cigar));
+      return finish(e, sq, st, qsn, r, cigar, ccap, keep);
+    }
     case ORC_ENGINE_MYERS:
       return finish(myers_fitting(qs, qsn, ts, tsn, true), sq, st, qsn, r, cigar, ccap, keep);
     case ORC_ENGINE_MYERS_FWD:
@@ -807,6 +824,19 @@ int orc_wfa_extend_affine(const uint8_t* q, size_t qn, const uint8_t* t, size_t 
                           int64_t cap, orc_result* r, char* cigar, size_t cigar_cap) {
   return run_engine(ORC_ENGINE_AFFINE, q, qn, t, tn, seed_q, seed_t,
                     {mismatch, gap_open, gap_extend}, cap, r, cigar, cigar_cap, nullptr);
+}
+
+int orc_wfa_global(const uint8_t* q, size_t qn, const uint8_t* t, size_t tn, orc_result* r,
+                   char* cigar, size_t cigar_cap) {
+  return run_engine(ORC_ENGINE_LINEAR_GLOBAL, q, qn, t, tn, 0, 0, {0, 0, 0}, -1, r, cigar, cigar_cap,
+                    nullptr);
+}
+
+int orc_wfa_affine_global(const uint8_t* q, size_t qn, const uint8_t* t, size_t tn,
+                          int32_t mismatch, int32_t gap_open, int32_t gap_extend, orc_result* r,
+                          char* cigar, size_t cigar_cap) {
+  return run_engine(ORC_ENGINE_AFFINE_GLOBAL, q, qn, t, tn, 0, 0, {mismatch, gap_open, gap_extend},
+                    -1, r, cigar, cigar_cap, nullptr);
 }
 
 int orc_myers_extend(const uint8_t* q, size_t qn, const uint8_t* t, size_t tn, size_t seed_q,

@@ -37,7 +37,9 @@ enum {
   ORC_ENGINE_AFFINE = 1,     /* wfa_extend_affine     lib.rs:139-167 */
   ORC_ENGINE_MYERS = 2,      /* myers_extend          lib.rs:106-127 */
   ORC_ENGINE_MYERS_GLOBAL = 3,  /* myers_edit_distance lib.rs:94-96  */
-  ORC_ENGINE_MYERS_FWD = 4   /* myers_extend distance + end column only (no CIGAR) */
+  ORC_ENGINE_MYERS_FWD = 4,  /* myers_extend distance + end column only (no CIGAR) */
+  ORC_ENGINE_LINEAR_GLOBAL = 5, /* fill_wfa, global whatever the lengths  wfa_simd.rs:231-345 */
+  ORC_ENGINE_AFFINE_GLOBAL = 6  /* fill_wfa_affine_generic(fitting=false) wfa_simd.rs:581-779 */
 };
 
 typedef struct orc_result {
@@ -69,6 +71,14 @@ int orc_myers_extend(const uint8_t* q, size_t qn, const uint8_t* t, size_t tn,
                      orc_result* r, char* cigar, size_t cigar_cap);
 int orc_myers_edit_distance(const uint8_t* q, size_t qn, const uint8_t* t, size_t tn,
                             orc_result* r, char* cigar, size_t cigar_cap);
+/* The global branches alone (no fitting switch), for gaps pinned at both
+ * ends: fill_wfa (wfa_simd.rs:231-345) and fill_wfa_affine_generic with
+ * fitting = false (:581-779; distance from the CIGAR, :932-934). */
+int orc_wfa_global(const uint8_t* q, size_t qn, const uint8_t* t, size_t tn, orc_result* r,
+                   char* cigar, size_t cigar_cap);
+int orc_wfa_affine_global(const uint8_t* q, size_t qn, const uint8_t* t, size_t tn,
+                          int32_t mismatch, int32_t gap_open, int32_t gap_extend, orc_result* r,
+                          char* cigar, size_t cigar_cap);
 /* myers_extend without the backtrace: distance + target_end only. */
 int orc_myers_extend_distance(const uint8_t* q, size_t qn, const uint8_t* t, size_t tn,
                               size_t seed_q, size_t seed_t, orc_result* r);

@@ -95,7 +95,7 @@ struct phx_plan {
 namespace {
 
 int fail(phx_ctx* c, int code, const std::string& msg) {
-  if (c) c->err = msg;
+  set_error(c, msg, code);
   return code;
 }
 
@@ -675,8 +675,8 @@ phx_plan* plan_setup(phx_plan* P, const std::vector<const uint8_t*>& qh,
     }
   }
   // Device buffers.
-  auto fail_plan = [&](const char* what) -> phx_plan* {
-    ctx->err = std::string("device allocation failed: ") + what;
+  auto fail_plan = [&](const char* what, int code = PHX_E_NOMEM) -> phx_plan* {
+    set_error(ctx, std::string("device allocation failed: ") + what, code);
     phx_plan_destroy(P);
     return nullptr;
   };
@@ -697,12 +697,12 @@ phx_plan* plan_setup(phx_plan* P, const std::vector<const uint8_t*>& qh,
   }
   if ((words && hipMemcpy(P->d_seq.p, packed->data(), words * 8, hipMemcpyHostToDevice) != hipSuccess) ||
       (n && hipMemcpy(P->d_pairs.p, P->pairs.data(), sizeof(PairDesc) * n, hipMemcpyHostToDevice) != hipSuccess))
-    return fail_plan("upload");
+    return fail_plan("upload", PHX_E_DEVICE);
   if (hipHostMalloc((void**)&P->ctr_host, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
       hipHostGetDevicePointer((void**)&P->ctr_dev, P->ctr_host, 0) != hipSuccess)
     return fail_plan("pinned counters");
   // Work lists are static between tunings: upload once.
-  if (upload_work(P) != PHX_OK) return fail_plan("work upload");
+  if (upload_work(P) != PHX_OK) return fail_plan("work upload", PHX_E_DEVICE);
   return P;
 }
 
@@ -763,12 +763,17 @@ phx_plan* plan_create_windows(phx_ctx* ctx, int engine, phx_costs costs, int64_t
   if (engine == PHX_ENGINE_AFFINE &&
       (costs.mismatch < 1 || costs.gap_extend < 1 || costs.gap_open < 0 ||
        ring_slots(engine, costs) > kRingSlotsMax)) {
-    ctx->err = "affine costs need mismatch >= 1, gap_extend >= 1, gap_open >= 0";
+    set_error(ctx, "affine costs need mismatch >= 1, gap_extend >= 1, gap_open >= 0");
     return nullptr;
   }
+  for (size_t i = 0; i < pairs.size(); ++i)
+    if (pairs[i].qn < 0 || pairs[i].tn < 0 || pairs[i].qn > kMaxLen || pairs[i].tn > kMaxLen) {
+      set_error(ctx, "window " + std::to_string(i) + " is longer than the device format supports (2^27 - 2)");
+      return nullptr;
+    }
   std::lock_guard<std::mutex> lk(ctx->mu);
   if (hipSetDevice(ctx->device) != hipSuccess) {
-    ctx->err = "hipSetDevice failed";
+    set_error(ctx, "hipSetDevice failed", PHX_E_DEVICE);
     return nullptr;
   }
   phx_plan* P = new phx_plan;
@@ -865,7 +870,11 @@ void phx_ctx_destroy(phx_ctx* c) {
   delete c;
 }
 
-const char* phx_ctx_last_error(const phx_ctx* c) { return c ? c->err.c_str() : "no context"; }
+const char* phx_ctx_last_error(const phx_ctx* c) {
+  if (!c) return "no context";
+  const LastError& e = last_error();
+  return e.ctx == c ? e.msg.c_str() : "";
+}
 
 uint64_t phx_cigar_bound(const phx_batch* b) {
   if (!b || b->engine == PHX_ENGINE_MYERS_DIST) return 0;
@@ -876,11 +885,11 @@ uint64_t phx_cigar_bound(const phx_batch* b) {
 
 phx_plan* phx_plan_create(phx_ctx* ctx, const phx_batch* b) {
   if (!ctx || !b || b->engine < 0 || b->engine > PHX_ENGINE_MYERS_DIST) {
-    if (ctx) ctx->err = "bad batch";
+    set_error(ctx, "bad batch");
     return nullptr;
   }
   std::lock_guard<std::mutex> lk(ctx->mu);
-  if (hipSetDevice(ctx->device) != hipSuccess) { ctx->err = "hipSetDevice failed"; return nullptr; }
+  if (hipSetDevice(ctx->device) != hipSuccess) { set_error(ctx, "hipSetDevice failed", PHX_E_DEVICE); return nullptr; }
   phx_plan* P = new phx_plan;
   P->ctx = ctx;
   P->tun = ctx->tun;
@@ -892,13 +901,13 @@ phx_plan* phx_plan_create(phx_ctx* ctx, const phx_batch* b) {
   if (b->engine == PHX_ENGINE_AFFINE) {
     const phx_costs& c = b->costs;
     if (c.mismatch < 1 || c.gap_extend < 1 || c.gap_open < 0) {
-      ctx->err = "affine costs need mismatch >= 1, gap_extend >= 1, gap_open >= 0";
+      set_error(ctx, "affine costs need mismatch >= 1, gap_extend >= 1, gap_open >= 0");
       delete P;
       return nullptr;
     }
     P->ring_slots = ring_slots(b->engine, c);
     if (P->ring_slots > kRingSlotsMax) {
-      ctx->err = "affine costs span more score frames than the device ring holds";
+      set_error(ctx, "affine costs span more score frames than the device ring holds");
       delete P;
       return nullptr;
     }
@@ -928,7 +937,7 @@ phx_plan* phx_plan_create(phx_ctx* ctx, const phx_batch* b) {
     }
     const uint64_t qn = qfull - sq, tn = tfull - st;
     if (qn > (uint64_t)kMaxLen || tn > (uint64_t)kMaxLen) {
-      ctx->err = "sequence longer than the device format supports (2^27 - 2)";
+      set_error(ctx, "sequence longer than the device format supports (2^27 - 2)");
       delete P;
       return nullptr;
     }
@@ -1203,7 +1212,7 @@ int phx_plan_results(phx_plan* P, phx_results* r) {
     if (o.status != ST_OK && o.status != ST_ALIGNMENT_FAILED) {
       a.status = PHX_E_INTERNAL;
       ret = PHX_E_INTERNAL;
-      ctx->err = "pair " + std::to_string(i) + " ended with device status " + std::to_string(o.status);
+      set_error(ctx, "pair " + std::to_string(i) + " ended with device status " + std::to_string(o.status));
       continue;
     }
     a.status = o.status;
@@ -1228,7 +1237,7 @@ int phx_plan_results(phx_plan* P, phx_results* r) {
   if (ret != PHX_OK) return ret;
   if (!has_cigar(P->engine) || pos == 0) return PHX_OK;
   if (pos > r->runs_cap || !r->runs) {
-    ctx->err = "CIGAR arena too small";
+    set_error(ctx, "CIGAR arena too small");
     return PHX_E_CAPACITY;
   }
   // Gather the used runs on the device, then one copy of exactly those words.
@@ -1264,7 +1273,7 @@ void phx_plan_destroy(phx_plan* P) {
 int phx_align_batch(phx_ctx* ctx, const phx_batch* b, phx_results* out) {
   if (!ctx || !b || !out) return PHX_E_ARG;
   phx_plan* P = phx_plan_create(ctx, b);
-  if (!P) return PHX_E_ARG;
+  if (!P) return last_error_code(ctx);
   int rc = phx_plan_run(P);
   if (rc == PHX_OK) rc = phx_plan_results(P, out);
   phx_plan_destroy(P);

@@ -564,7 +564,7 @@ bool alloc_copy(Buf& b, const void* src, size_t bytes) {
 }
 
 int cfail(phx_ctx* ctx, int code, const std::string& msg) {
-  if (ctx) ctx->err = msg;
+  phx::host::set_error(ctx, msg, code);
   return code;
 }
 
@@ -595,7 +595,7 @@ phx_seed_index* phx_seed_index_create(phx_ctx* ctx, const uint64_t* values, cons
                                       uint64_t n) {
   if (!ctx || (n && (!values || !positions))) return nullptr;
   if (n >= (1ull << 32)) {
-    ctx->err = "phx_seed_index_create: more than 2^32 - 1 minimizers";
+    phx::host::set_error(ctx, "phx_seed_index_create: more than 2^32 - 1 minimizers");
     return nullptr;
   }
   // Distinct values; within a value, positions in sketch order
@@ -630,7 +630,7 @@ phx_seed_index* phx_seed_index_create(phx_ctx* ctx, const uint64_t* values, cons
   X->mask = mask;
   X->counts = std::move(counts);
   if (!alloc_copy(X->slots, slots.data(), slots.size() * sizeof(IxSlot)) || !alloc_copy(X->pos, pos.data(), n * 4)) {
-    ctx->err = "phx_seed_index_create: device allocation or upload failed";
+    phx::host::set_error(ctx, "phx_seed_index_create: device allocation or upload failed", PHX_E_NOMEM);
     phx_seed_index_destroy(X);
     return nullptr;
   }
@@ -658,11 +658,11 @@ phx_chain_plan* phx_chain_plan_create(phx_ctx* ctx, const phx_seed_index* X, uin
   if (!ctx || !X || !p || !q_off) return nullptr;
   const uint64_t nm = q_off[nq];
   if (nm && (!q_values || !q_positions)) {
-    ctx->err = "phx_chain_plan_create: null query sketch arrays";
+    phx::host::set_error(ctx, "phx_chain_plan_create: null query sketch arrays");
     return nullptr;
   }
   if (p->max_band < 0 || p->max_band > (1 << 20) || p->seed_len < 0 || p->max_dist_x < 0) {
-    ctx->err = "phx_chain_plan_create: max_band must lie in [0, 2^20], seed_len and max_dist_x >= 0";
+    phx::host::set_error(ctx, "phx_chain_plan_create: max_band must lie in [0, 2^20], seed_len and max_dist_x >= 0");
     return nullptr;
   }
   // chain_score's gap penalty per diagonal drift (chaining.rs:111-115).
@@ -729,7 +729,7 @@ phx_chain_plan* phx_chain_plan_create(phx_ctx* ctx, const phx_seed_index* X, uin
        alloc_copy(P->used, nullptr, S) && alloc_copy(P->tmp, nullptr, S * sizeof(phx_chain)) &&
        alloc_copy(P->out, nullptr, S * sizeof(phx_chain)) && alloc_copy(P->cs, nullptr, S * sizeof(phx_seed));
   if (!ok) {
-    ctx->err = "phx_chain_plan_create: device allocation, upload or seed count failed";
+    phx::host::set_error(ctx, "phx_chain_plan_create: device allocation, upload or seed count failed", PHX_E_NOMEM);
     phx_chain_plan_destroy(P);
     return nullptr;
   }
@@ -840,7 +840,7 @@ int phx_seed_and_chain_batch(phx_ctx* ctx, const phx_seed_index* X, uint64_t max
                              uint32_t nq, phx_chain_results* out) {
   if (!ctx || !X || !p || !q_off || !out) return PHX_E_ARG;
   phx_chain_plan* P = phx_chain_plan_create(ctx, X, max_occ, p, q_values, q_positions, q_off, nq);
-  if (!P) return PHX_E_ARG;
+  if (!P) return phx::host::last_error_code(ctx);
   int rc = phx_chain_plan_run(P);
   if (rc == PHX_OK) rc = phx_chain_plan_results(P, out);
   phx_chain_plan_destroy(P);

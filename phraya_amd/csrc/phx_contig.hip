@@ -16,8 +16,10 @@
 // window pair of the resident query and target arenas; the plan aligns all of
 // them at once with the chosen engine.  Stitching concatenates, per contig,
 // gap 0's runs, the first anchor as a k-base M run, gap 1's runs, ..., and
-// merges equal neighbouring runs at the joins.  A gap whose alignment leaves
-// target bases unconsumed (fitting branch) contributes them as an I run.
+// merges equal neighbouring runs at the joins.  Every gap is pinned at both
+// ends, so it always takes the global branch (never the fitting switch of the
+// per-pair wrappers) and consumes its whole window; the contig score (the sum
+// of the gap scores) is then the cost of the stitched CIGAR.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -257,11 +259,11 @@ phx_contig_plan* phx_contig_plan_create(phx_ctx* ctx, int32_t engine, phx_costs 
   if (!ctx || !b) return nullptr;
   const phx_contig_params p = fill_defaults(params);
   if (!params_ok(p) || (engine != PHX_ENGINE_AFFINE && engine != PHX_ENGINE_LINEAR)) {
-    ctx->err = "contig mode needs the affine or linear engine and 8 <= k <= 16";
+    set_error(ctx, "contig mode needs the affine or linear engine and 8 <= k <= 16");
     return nullptr;
   }
   if (hipSetDevice(ctx->device) != hipSuccess) {
-    ctx->err = "hipSetDevice failed";
+    set_error(ctx, "hipSetDevice failed", PHX_E_DEVICE);
     return nullptr;
   }
   phx_contig_plan* C = new phx_contig_plan;
@@ -295,12 +297,25 @@ phx_contig_plan* phx_contig_plan_create(phx_ctx* ctx, int32_t engine, phx_costs 
     C->anchors.emplace_back(0, 0);  // no anchor after a contig's last gap
     C->gap_first[i + 1] = C->gaps.size();
   }
+  // Every gap must fit the device format (history cells hold offset + 1 in 28
+  // bits): a pair with no anchors, or a long anchor-free stretch, makes one
+  // gap as long as the stretch.  Refused here, before anything is uploaded.
+  for (size_t g = 0; g < C->gaps.size(); ++g) {
+    const ContigGap& G = C->gaps[g];
+    if (G.q1 - G.q0 > (uint64_t)kMaxLen || G.t1 - G.t0 > (uint64_t)kMaxLen) {
+      set_error(ctx, "contig gap " + std::to_string(g) + " (" + std::to_string(G.q1 - G.q0) + " x " +
+                         std::to_string(G.t1 - G.t0) +
+                         " bases) is longer than the device format supports (2^27 - 2)");
+      delete C;
+      return nullptr;
+    }
+  }
   // Resident arenas and one window descriptor per gap.
   const bool qb = !is_acgt(b->q_seq + b->q_off[0], (size_t)(b->q_off[n] - b->q_off[0]));
   const bool tb = !is_acgt(b->t_seq + b->t_off[0], (size_t)(b->t_off[n] - b->t_off[0]));
   if (!upload_arena(b->q_seq, b->q_off, n, qb || tb, C->d_q2, C->d_q8) ||
       !upload_arena(b->t_seq, b->t_off, n, qb || tb, C->d_t2, C->d_t8)) {
-    ctx->err = "contig arena upload failed";
+    set_error(ctx, "contig arena upload failed", PHX_E_NOMEM);
     delete C;
     return nullptr;
   }
@@ -322,7 +337,9 @@ phx_contig_plan* phx_contig_plan_create(phx_ctx* ctx, int32_t engine, phx_costs 
       pd.t_shift = (uint32_t)(ta % per);
       pd.qn = (int32_t)(G.q1 - G.q0);
       pd.tn = (int32_t)(G.t1 - G.t0);
-      pd.flags = b8 ? PF_BYTES : 0;
+      // A gap is pinned by anchors (or a contig end) on both sides: always
+      // the global branch, so the contig score is the cost of its CIGAR.
+      pd.flags = (b8 ? PF_BYTES : 0) | PF_GLOBAL;
       qh[g] = qs + G.q0;
       th[g] = ts + G.t0;
     }
@@ -396,9 +413,13 @@ int phx_contig_plan_run(phx_contig_plan* C) {
         out.clear();
         return;
       }
+      // Global gap alignments consume their whole target window.
+      if ((uint64_t)x.target_end != C->gaps[g].t1 - C->gaps[g].t0) {
+        a.status = PHX_E_INTERNAL;
+        out.clear();
+        return;
+      }
       for (uint64_t i = goff[g]; i < goff[g + 1]; ++i) put(gr[i] & 15u, gr[i] >> 4);
-      const uint64_t tl = C->gaps[g].t1 - C->gaps[g].t0;
-      if ((uint64_t)x.target_end < tl) put(OP_I, tl - (uint64_t)x.target_end);
       score += x.score;
       if (g + 1 < C->gap_first[c + 1]) put(OP_M, k);
     }

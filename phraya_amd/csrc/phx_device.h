@@ -36,7 +36,9 @@ enum Status : int32_t {
 };
 
 enum PairFlags : uint32_t {
-  PF_BYTES = 1u,  // 8-bit encoding
+  PF_BYTES = 1u,   // 8-bit encoding
+  PF_GLOBAL = 2u,  // always the global branch (both ends pinned, e.g. a gap between two anchors):
+                   // fill_wfa / fill_wfa_affine_generic(fitting = false), wfa_simd.rs:231-345, :581-779
 };
 
 // A sequence is addressed by the device address of the word holding its

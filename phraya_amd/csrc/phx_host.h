@@ -377,8 +377,39 @@ struct phx_ctx {
   hipEvent_t join[32] = {};             // cross-stream joins
   std::mutex mu;
   phx_tuning tun{};
-  std::string err;
   int cu_count = 256;
+  uint8_t* noalloc_scratch = nullptr;   // device address of this device's allocation-free scratch
 };
+
+namespace phx {
+namespace host {
+
+// Error text and code of the last failed call the calling thread made, and
+// the context it was made on.  Per thread, so threads sharing a context never
+// race on it and the text stays valid until that thread's next failure.
+struct LastError {
+  const phx_ctx* ctx = nullptr;
+  std::string msg;
+  int code = PHX_OK;
+};
+inline LastError& last_error() {
+  static thread_local LastError e;
+  return e;
+}
+inline void set_error(const phx_ctx* ctx, const std::string& msg, int code = PHX_E_ARG) {
+  LastError& e = last_error();
+  e.ctx = ctx;
+  e.msg = msg;
+  e.code = code;
+}
+// Code of the calling thread's last failure on ctx (PHX_E_ARG when it made
+// none there): what a one-shot entry point returns when plan creation fails.
+inline int last_error_code(const phx_ctx* ctx) {
+  const LastError& e = last_error();
+  return e.ctx == ctx && e.code != PHX_OK ? e.code : PHX_E_ARG;
+}
+
+}  // namespace host
+}  // namespace phx
 
 #endif

@@ -507,22 +507,22 @@ extern "C" {
 phx_server* phx_serve_start(phx_ctx* ctx, const phx_serve_config* cfg) {
   if (!ctx || !cfg) return nullptr;
   if (cfg->engine != PHX_ENGINE_AFFINE && cfg->engine != PHX_ENGINE_LINEAR) {
-    ctx->err = "the server runs PHX_ENGINE_AFFINE or PHX_ENGINE_LINEAR";
+    set_error(ctx, "the server runs PHX_ENGINE_AFFINE or PHX_ENGINE_LINEAR");
     return nullptr;
   }
   if (cfg->engine == PHX_ENGINE_AFFINE) {
     const phx_costs& c = cfg->costs;
     if (c.mismatch < 1 || c.gap_extend < 1 || c.gap_open < 0) {
-      ctx->err = "affine costs need mismatch >= 1, gap_extend >= 1, gap_open >= 0";
+      set_error(ctx, "affine costs need mismatch >= 1, gap_extend >= 1, gap_open >= 0");
       return nullptr;
     }
     if (ring_slots(cfg->engine, c) > kRingSlotsMax) {
-      ctx->err = "affine costs span more score frames than the device ring holds";
+      set_error(ctx, "affine costs span more score frames than the device ring holds");
       return nullptr;
     }
   }
   if (hipSetDevice(ctx->device) != hipSuccess) {
-    ctx->err = "hipSetDevice failed";
+    set_error(ctx, "hipSetDevice failed", PHX_E_DEVICE);
     return nullptr;
   }
   phx_server* S = new phx_server;
@@ -537,7 +537,7 @@ phx_server* phx_serve_start(phx_ctx* ctx, const phx_serve_config* cfg) {
   S->wave_lds_kib = ctx->tun.wave_lds_kib;
   S->wave_slots = ctx->tun.wave_slots;
   auto fail = [&](const std::string& m) -> phx_server* {
-    ctx->err = m;
+    set_error(ctx, m);
     free_server(S);
     return nullptr;
   };

@@ -360,7 +360,7 @@ bool alloc_copy(Buf& b, const void* src, size_t bytes) {
 }
 
 int vfail(phx_ctx* ctx, int code, const std::string& msg) {
-  if (ctx) ctx->err = msg;
+  phx::host::set_error(ctx, msg, code);
   return code;
 }
 
@@ -422,7 +422,7 @@ phx_variant_plan* phx_variant_plan_create(phx_ctx* ctx, const phx_variant_batch*
   if (!ctx || !b) return nullptr;
   std::string err;
   if (check_batch(b, &err) != PHX_OK) {
-    ctx->err = "phx_variant_plan_create: " + err;
+    phx::host::set_error(ctx, "phx_variant_plan_create: " + err);
     return nullptr;
   }
   const uint32_t n = b->n;
@@ -467,7 +467,7 @@ phx_variant_plan* phx_variant_plan_create(phx_ctx* ctx, const phx_variant_batch*
        hipHostMalloc((void**)&P->tot_host, 16, hipHostMallocMapped) == hipSuccess &&
        hipHostGetDevicePointer((void**)&P->tot_dev, P->tot_host, 0) == hipSuccess;
   if (!ok) {
-    ctx->err = "phx_variant_plan_create: device allocation or upload failed";
+    phx::host::set_error(ctx, "phx_variant_plan_create: device allocation or upload failed", PHX_E_NOMEM);
     phx_variant_plan_destroy(P);
     return nullptr;
   }
@@ -502,7 +502,7 @@ phx_variant_plan* phx_variant_plan_create(phx_ctx* ctx, const phx_variant_batch*
     hipLaunchKernelGGL(tile_scan_kernel, dim3(P->n_tiles), block, 0, st, (uint64_t*)P->nvar.p,
                        (uint64_t*)P->ncov.p, (uint64_t*)P->tile_v.p, (uint64_t*)P->tile_c.p, n);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
-      ctx->err = "phx_variant_plan_create: layout passes failed";
+      phx::host::set_error(ctx, "phx_variant_plan_create: layout passes failed", PHX_E_DEVICE);
       phx_variant_plan_destroy(P);
       return nullptr;
     }
@@ -588,7 +588,7 @@ int phx_extract_variants(phx_ctx* ctx, const phx_variant_batch* b, phx_variant_r
   std::string err;
   if (check_batch(b, &err) != PHX_OK) return vfail(ctx, PHX_E_ARG, "phx_extract_variants: " + err);
   phx_variant_plan* P = phx_variant_plan_create(ctx, b);
-  if (!P) return PHX_E_NOMEM;
+  if (!P) return phx::host::last_error_code(ctx);
   int rc = phx_variant_plan_run(P);
   if (rc == PHX_OK) rc = phx_variant_plan_results(P, out);
   phx_variant_plan_destroy(P);

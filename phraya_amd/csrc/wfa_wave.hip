@@ -722,13 +722,24 @@ __device__ __forceinline__ void wave_affine(const WfaArgs& a, const WaveGeom& G,
                             const PairDesc& pd, PairOut* po, int lane, PairShared* sh) {
   using S_t = WaveStore<LDS, RL>;
   const int32_t qn = pd.qn, tn = pd.tn;
-  if (qn == 0) {  // wfa_simd.rs:925-927
-    if (lane == 0) *po = PairOut{ST_OK, 0, 0, 0, 0, 0, 0, 0};
+  const bool forced_global = (pd.flags & PF_GLOBAL) != 0;
+  if (qn == 0) {
+    // Wrapper (wfa_simd.rs:925-927): empty query -> empty alignment.  Forced
+    // global (fill_wfa_affine_generic with fitting = false, :594-601): every
+    // target base is an I run costing gap_open + tn * gap_extend.
+    if (lane == 0) {
+      if (forced_global && tn > 0) {
+        a.runs[pd.run_off + pd.run_cap - 1] = ((uint32_t)tn << 4) | OP_I;
+        *po = PairOut{ST_OK, a.cp.o + tn * a.cp.e, tn, tn, 1, (int32_t)pd.run_cap - 1, 0, 0};
+      } else {
+        *po = PairOut{ST_OK, 0, 0, 0, 0, 0, 0, 0};
+      }
+    }
     return;
   }
   const S_t S = make_store<LDS, B8, NT, RL>(a, G, lds_base, pd, lane);
   const int32_t x = a.cp.x, o = a.cp.o, e = a.cp.e;
-  const bool fitting = tn > qn + qn / 2 + 10;
+  const bool fitting = !forced_global && tn > qn + qn / 2 + 10;
   const int32_t g = a.cp.g, xn = a.cp.xn, oen = a.cp.oen, en = a.cp.en;
   // The m layer reads xn and oen steps back, the gap layers en steps back:
   // Rm and Rg frames of ring (the host sizes the ring as Rm + 2 Rg frames).
@@ -950,12 +961,23 @@ __device__ __forceinline__ void wave_linear(const WfaArgs& a, const WaveGeom& G,
                             const PairDesc& pd, PairOut* po, int lane, PairShared* sh) {
   using S_t = WaveStore<LDS, RL>;
   const int32_t qn = pd.qn, tn = pd.tn;
-  if (qn == 0) {  // wfa_simd.rs:194-203, :365-367
-    if (lane == 0) *po = PairOut{ST_OK, 0, 0, 0, 0, 0, 0, 0};
+  const bool forced_global = (pd.flags & PF_GLOBAL) != 0;
+  if (qn == 0) {
+    // Wrapper (wfa_simd.rs:194-203, :365-367): empty query -> empty
+    // alignment.  Forced global (fill_wfa, :231-345): the only path is tn
+    // target-only steps, distance tn.
+    if (lane == 0) {
+      if (forced_global && tn > 0) {
+        a.runs[pd.run_off + pd.run_cap - 1] = ((uint32_t)tn << 4) | OP_I;
+        *po = PairOut{ST_OK, tn, tn, tn, 1, (int32_t)pd.run_cap - 1, 0, 0};
+      } else {
+        *po = PairOut{ST_OK, 0, 0, 0, 0, 0, 0, 0};
+      }
+    }
     return;
   }
   const S_t S = make_store<LDS, B8, NT, RL>(a, G, lds_base, pd, lane);
-  const bool fitting = tn > qn + qn / 2 + 10;
+  const bool fitting = !forced_global && tn > qn + qn / 2 + 10;
   const int32_t max_s = qn + tn;
   const int32_t explore = (fitting && a.cp.cap >= 0) ? min(a.cp.cap, max_s) : max_s;
   const int32_t kend_global = qn - tn;

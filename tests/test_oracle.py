@@ -152,6 +152,39 @@ def test_affine_matches_gotoh(costs):
         assert dp.compacted(r.cigar)
 
 
+@pytest.mark.parametrize("costs", [None, (4, 6, 2), (2, 4, 1)])
+def test_global_engines_match_dp(costs):
+    """The global branches alone (fill_wfa, fill_wfa_affine_generic with
+    fitting = false), used for gaps pinned at both ends: the global DP optimum
+    on every shape, including the long targets for which the wrappers switch
+    to fitting and empty queries; identical to the wrappers wherever those
+    take the global branch themselves."""
+    rng = random.Random(77 if costs is None else sum(costs))
+    for it in range(200):
+        q, t = rand_pair(rng, qmax=40, fitting_frac=0.5)
+        if it % 25 == 0:
+            q = b""
+        if costs is None:
+            r = O.wfa_global(q, t)
+            best = int(dp.levenshtein(q, t)[len(q), len(t)])
+            assert dp.unit_cost(r.cigar) == best
+            wrapped = O.wfa_extend(q, t)
+        else:
+            x, o, e = costs
+            r = O.wfa_affine_global(q, t, costs)
+            best = int(dp.gotoh(q, t, x, o, e)[len(q), len(t)]) if (q or t) else 0
+            assert dp.affine_cost(r.cigar, x, o, e) == best
+            assert r.edit_distance == dp.unit_cost(r.cigar)
+            wrapped = O.wfa_extend_affine(q, t, costs=costs)
+        assert r.status == 0 and r.score == best
+        assert r.target_end == len(t)
+        assert dp.replay(r.cigar, q, t) == (len(q), len(t))
+        assert dp.compacted(r.cigar)
+        if q and not _fitting(len(q), len(t)):
+            assert (r.cigar, r.score, r.edit_distance) == (wrapped.cigar, wrapped.score,
+                                                           wrapped.edit_distance)
+
+
 def test_myers_matches_levenshtein():
     rng = random.Random(5)
     for _ in range(400):
