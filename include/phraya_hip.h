@@ -136,6 +136,23 @@ uint64_t phx_cigar_bound(const phx_batch* batch);
  * PHX_ALIGNMENT_FAILED in aln[i].status), else a PHX_E_* code. */
 int phx_align_batch(phx_ctx* ctx, const phx_batch* batch, phx_results* out);
 
+/* Allocation-free form of phx_align_batch (no counterpart in the reference;
+ * for callers that cannot allocate device memory: tight serving loops and
+ * harnesses that forbid hipMalloc).  Same arguments, results and status
+ * codes as phx_align_batch, bit for bit.  It calls no hipMalloc, hipFree,
+ * hipHostMalloc or other allocating routine and creates no events: every
+ * device buffer (packed sequences, descriptors, results, CIGAR run slots,
+ * work lists, overflow flags, wavefront history, Myers scratch) is carved
+ * from a fixed per-GPU scratch array of PHX_NOALLOC_SCRATCH_BYTES that the
+ * library holds from its load until the process ends.  A batch whose
+ * buffers do not fit returns PHX_E_NOMEM (the reason in
+ * phx_ctx_last_error) before anything runs; pairs whose wavefront history
+ * outgrows what is left fail the same way.  Calls on one GPU share its
+ * array and are serialised by a process-wide lock (they would otherwise race
+ * on it); each call clears what it needs zeroed and leaves the rest. */
+#define PHX_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int phx_align_batch_noalloc(phx_ctx* ctx, const phx_batch* batch, phx_results* out);
+
 /* Split form of phx_align_batch for callers that keep inputs resident in HBM
  * and launch repeatedly: create packs and uploads, run launches the kernels
  * on the resident inputs (results stay in HBM), results downloads. */

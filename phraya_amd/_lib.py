@@ -30,7 +30,7 @@ ENGINE_MYERS_DIST = 4
 # Every symbol include/phraya_hip.h declares.
 EXPORTS = (
     "phx_ctx_create", "phx_ctx_destroy", "phx_ctx_last_error", "phx_cigar_bound",
-    "phx_align_batch", "phx_plan_create", "phx_plan_run", "phx_plan_results",
+    "phx_align_batch", "phx_align_batch_noalloc", "phx_plan_create", "phx_plan_run", "phx_plan_results",
     "phx_plan_last_ms", "phx_plan_last_kernel_ms", "phx_plan_last_launches",
     "phx_plan_algorithmic_bytes", "phx_plan_destroy", "phx_format_cigar",
     "phx_plan_set_tuning", "phx_plan_get_tuning", "phx_plan_autotune", "phx_plan_use_graph",
@@ -213,6 +213,7 @@ def lib() -> ctypes.CDLL:
     L.phx_cigar_bound.argtypes = [P]
     L.phx_cigar_bound.restype = ctypes.c_uint64
     L.phx_align_batch.argtypes = [P, P, P]
+    L.phx_align_batch_noalloc.argtypes = [P, P, P]
     L.phx_plan_create.argtypes = [P, P]
     L.phx_plan_create.restype = P
     L.phx_plan_run.argtypes = [P]

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -69,12 +70,23 @@ struct phx_plan {
   phx_tuning tun{};                                    // this plan's tuning (from the context)
   std::vector<uint32_t> order;                         // pending pairs, longest first
   std::vector<int32_t> est;                            // WFA: estimated score steps per pair
+  // Allocation-free plans: every device buffer is a carve of a fixed region
+  // (the device's scratch array) instead of an allocation.  Buffers that live
+  // for the whole plan are carved from its start, up to fixed_transient; the
+  // rest is one region that the run's HBM-tier slots (or Myers scratch) use
+  // and, once the run has finished, the result gather.
+  uint8_t* fixed_base = nullptr;
+  uint64_t fixed_cap = 0, fixed_used = 0, fixed_transient = 0;
+  std::vector<uint32_t> flag_shadow;                   // host copy of the overflow flags (fixed plans)
+  bool fixed() const { return fixed_base != nullptr; }
+  double share_norm = 1.0;                             // sum of group shares, when above 1
   // HBM scratch a group may use (its share of the budget); re-runs (-1) run
   // alone after everything else and may use the whole budget.
   uint64_t group_budget(int g) const {
-    const uint64_t total = tun.hbm_budget_mib << 20;
+    const uint64_t total = fixed() ? fixed_cap - fixed_transient : tun.hbm_budget_mib << 20;
     if (g >= 0 && g < (int)group_share.size())
-      return std::max<uint64_t>((uint64_t)(total * group_share[g]), 64ull << 20);
+      return std::min<uint64_t>(total, std::max<uint64_t>((uint64_t)(total * group_share[g] / share_norm),
+                                                          fixed() ? 0 : 64ull << 20));
     return total;
   }
   std::vector<PairOut> out_h;
@@ -124,6 +136,57 @@ void dev_free(DevBuf& b) {
   if (b.p) (void)hipFree(b.p);
   b.p = nullptr;
   b.bytes = 0;
+}
+
+// A buffer the plan keeps for its whole life: allocated, or on fixed plans
+// carved (256-byte aligned) from the fixed region; a carve never grows.
+int plan_alloc(phx_plan* P, DevBuf& b, size_t bytes) {
+  if (!P->fixed()) return dev_alloc(P->ctx, b, bytes);
+  if (bytes == 0) bytes = 8;
+  if (b.bytes >= bytes) return PHX_OK;
+  const uint64_t at = (P->fixed_used + 255) & ~(uint64_t)255;
+  if (b.p || at + bytes > P->fixed_cap)
+    return fail(P->ctx, PHX_E_NOMEM,
+                "batch needs more device memory than the allocation-free scratch holds (" +
+                    std::to_string(P->fixed_cap >> 20) + " MiB)");
+  b.p = P->fixed_base + at;
+  b.bytes = bytes;
+  P->fixed_used = at + bytes;
+  return PHX_OK;
+}
+
+// A buffer of one phase (a run's HBM slots, Myers scratch, the result
+// gather): allocated (grow-only), or on fixed plans the transient region at
+// `offset`, which every phase reuses.
+int plan_transient(phx_plan* P, DevBuf& b, size_t bytes, uint64_t offset = 0) {
+  if (!P->fixed()) return dev_alloc(P->ctx, b, bytes);
+  if (bytes == 0) bytes = 8;
+  const uint64_t at = P->fixed_transient + ((offset + 255) & ~(uint64_t)255);
+  if (at + bytes > P->fixed_cap)
+    return fail(P->ctx, PHX_E_NOMEM,
+                "pairs need more wavefront / result memory than the allocation-free scratch holds (" +
+                    std::to_string(P->fixed_cap >> 20) + " MiB)");
+  b.p = P->fixed_base + at;
+  b.bytes = bytes;
+  return PHX_OK;
+}
+
+// Overflow flags (one per launch group): pinned host memory the kernels
+// store to, or on fixed plans words of the fixed region cleared by a fill
+// kernel and copied back after the run.
+int clear_flags(phx_plan* P, uint32_t n) {
+  phx_ctx* ctx = P->ctx;
+  if (P->fixed()) {
+    HIPCHK(launch_zero(P->ctr_dev, n, ctx->stream));
+    return PHX_OK;
+  }
+  for (uint32_t i = 0; i < n; ++i) ((volatile uint32_t*)P->ctr_host)[i] = 0;
+  return PHX_OK;
+}
+int read_flags(phx_plan* P, uint32_t n) {
+  phx_ctx* ctx = P->ctx;
+  if (P->fixed() && n) HIPCHK(hipMemcpy(P->ctr_host, P->ctr_dev, n * 4, hipMemcpyDeviceToHost));
+  return PHX_OK;
 }
 
 // Event pair number i of the plan (created on first use).
@@ -202,7 +265,11 @@ GroupSlots group_slots(const phx_plan* P, int t, const std::vector<uint32_t>& wo
                                                  : (uint64_t)P->ctx->cu_count * P->tun.block_flight;
   g.n = std::max<uint64_t>(1, budget / g.per);
   g.n = std::min<uint64_t>(std::min<uint64_t>(g.n, max_flight), work.size());
-  if (t == TIER_WAVE) g.n = (g.n + kWaveWaves - 1) / kWaveWaves * kWaveWaves;  // whole workgroups
+  // Whole workgroups of waves; fixed plans round down where that keeps one
+  // (their region is exactly the budget).
+  if (t == TIER_WAVE)
+    g.n = P->fixed() && g.n >= (uint64_t)kWaveWaves ? g.n / kWaveWaves * kWaveWaves
+                                                     : (g.n + kWaveWaves - 1) / kWaveWaves * kWaveWaves;
   return g;
 }
 
@@ -285,7 +352,9 @@ int issue_groups(phx_plan* P, bool capture, std::vector<int>* launched,
   // One launch and nothing else on the stream: its events start and end the
   // run.
   const bool single = !capture && !queued_group && nonempty == 1;
-  if (!single && !capture) {
+  // Fixed plans record no events (no timing): they create nothing.
+  const bool timed = !capture && !P->fixed();
+  if (!single && timed) {
     if (plan_event(P, &P->ev_start) != PHX_OK) return PHX_E_DEVICE;
     HIPCHK(hipEventRecord(P->ev_start, ctx->stream));
     P->start_ev = P->ev_start;
@@ -302,16 +371,29 @@ int issue_groups(phx_plan* P, bool capture, std::vector<int>* launched,
   // Scratch: one arena (grow-only), a disjoint slice per HBM group.
   std::vector<GroupSlots> gs(ngroups);
   std::vector<uint64_t> slice_off(ngroups, 0);
+  std::vector<int> tier(P->group_tier.begin(), P->group_tier.end());
   uint64_t arena = 0;
   for (int gi = 0; gi < ngroups; ++gi) {
-    if (P->work[gi].empty() || P->group_tier[gi] == TIER_LDS) continue;
-    gs[gi] = group_slots(P, P->group_tier[gi], P->work[gi], P->group_slot[gi], P->group_budget(gi),
-                         P->group_rl[gi] != 0, P->group_maxsum[gi]);
+    if (P->work[gi].empty() || tier[gi] == TIER_LDS) continue;
+    const uint64_t budget = P->group_budget(gi);
+    gs[gi] = group_slots(P, tier[gi], P->work[gi], P->group_slot[gi], budget, P->group_rl[gi] != 0,
+                         P->group_maxsum[gi]);
+    // A fixed region holds exactly the budgets: a wave-tier group whose
+    // budget is short of one workgroup's four slots runs on the 256-thread
+    // workgroup tier instead (one slot per pair; results never depend on
+    // the tier).
+    if (P->fixed() && tier[gi] == TIER_WAVE && gs[gi].n * gs[gi].per > budget) {
+      tier[gi] = TIER_BLOCK256;
+      gs[gi] = group_slots(P, tier[gi], P->work[gi], P->group_slot[gi], budget, false, P->group_maxsum[gi]);
+    }
     slice_off[gi] = arena;
     arena += gs[gi].n * gs[gi].per;
   }
   if (arena > P->d_arena.bytes && capture) return PHX_E_INTERNAL;  // never allocate while capturing
-  if (arena && dev_alloc(ctx, P->d_arena, (size_t)arena) != PHX_OK) return PHX_E_NOMEM;
+  if (arena) {
+    const int rc = plan_transient(P, P->d_arena, (size_t)arena);
+    if (rc != PHX_OK) return rc;
+  }
   double queued[kStreams] = {0, 0, 0, 0};
   bool used_stream[kStreams] = {true, false, false, false};
   for (int gi : order) {
@@ -324,10 +406,10 @@ int issue_groups(phx_plan* P, bool capture, std::vector<int>* launched,
       HIPCHK(hipStreamWaitEvent(st, ctx->join[0], 0));
       used_stream[si] = true;
     }
-    int rc = launch_wfa_tier(P, P->group_tier[gi], wl, (uint32_t*)P->d_work.p + group_off[gi],
+    int rc = launch_wfa_tier(P, tier[gi], wl, (uint32_t*)P->d_work.p + group_off[gi],
                              d_ctr + 2 * gi, P->ctr_dev + gi, gs[gi],
                              (uint8_t*)P->d_arena.p + slice_off[gi], st, P->group_rl[gi] != 0,
-                             capture ? TIME_NONE : TIME_BOTH, P->group_seqw[gi]);
+                             timed ? TIME_BOTH : TIME_NONE, P->group_seqw[gi]);
     if (rc != PHX_OK) return rc;
     if (capture) {
       hipStreamCaptureStatus cs;
@@ -347,7 +429,7 @@ int issue_groups(phx_plan* P, bool capture, std::vector<int>* launched,
     HIPCHK(hipEventRecord(ctx->join[si], ctx->streams[si]));
     HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->join[si], 0));
   }
-  if (!capture) {
+  if (timed) {
     hipEvent_t e0, e1;
     if (single && kernel_events(P, 0, &e0, &e1) == PHX_OK) {
       P->end_ev = e1;
@@ -412,7 +494,8 @@ int run_wfa(phx_plan* P) {
   phx_ctx* ctx = P->ctx;
   uint32_t* d_ctr = (uint32_t*)P->d_ctr.p;
   volatile uint32_t* ctr_h = P->ctr_host;
-  for (size_t gi = 0; gi < P->work.size(); ++gi) ctr_h[gi] = 0;
+  const uint32_t ngroups = (uint32_t)P->work.size();
+  if (clear_flags(P, ngroups) != PHX_OK) return PHX_E_DEVICE;
   std::vector<int> launched;
   if (P->use_graph && P->gexec && P->g_arena == P->d_arena.p) {
     HIPCHK(hipGraphLaunch(P->gexec, ctx->stream));
@@ -428,6 +511,7 @@ int run_wfa(phx_plan* P) {
   }
   if (launched.empty()) return PHX_OK;
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (read_flags(P, ngroups) != PHX_OK) return PHX_E_DEVICE;
   bool any = false;
   for (int gi : launched) any |= ctr_h[gi] != 0;
   if (!any) return PHX_OK;
@@ -458,24 +542,28 @@ int run_wfa(phx_plan* P) {
       words = std::max(words, exact_hist_words(steps, P->pairs[i].qn, P->pairs[i].tn));
     }
     const uint64_t slot_bytes = (words * 4 + 255) & ~(uint64_t)255;
-    if (dev_alloc(ctx, P->d_work2, wl.size() * 4) != PHX_OK) return PHX_E_NOMEM;
+    if (plan_alloc(P, P->d_work2, (size_t)P->n * 4) != PHX_OK) return PHX_E_NOMEM;
     HIPCHK(hipMemcpy(P->d_work2.p, wl.data(), wl.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(launch_zero(d_ctr, 2, ctx->stream));
-    ctr_h[0] = 0;
+    if (clear_flags(P, 1) != PHX_OK) return PHX_E_DEVICE;
     uint64_t maxL = 0;
     for (uint32_t i : wl) maxL = std::max<uint64_t>(maxL, (uint64_t)P->pairs[i].qn + P->pairs[i].tn);
     // Re-runs are few and run after everything else: give them wide tiers.
     const int rt = maxL <= 8192 ? TIER_BLOCK256 : TIER_BLOCK1024;
     // Every group has finished: the re-run may use the whole arena.
     const GroupSlots rg = group_slots(P, rt, wl, slot_bytes, P->group_budget(-1));
-    if (dev_alloc(ctx, P->d_arena, (size_t)(rg.n * rg.per)) != PHX_OK) return PHX_E_NOMEM;
-    int rc = launch_wfa_tier(P, rt, wl, (uint32_t*)P->d_work2.p, d_ctr, P->ctr_dev, rg,
-                             (uint8_t*)P->d_arena.p, ctx->stream);
+    int rc = plan_transient(P, P->d_arena, (size_t)(rg.n * rg.per));
     if (rc != PHX_OK) return rc;
-    if (plan_event(P, &P->ev_end) != PHX_OK) return PHX_E_DEVICE;
-    HIPCHK(hipEventRecord(P->ev_end, ctx->stream));
-    P->end_ev = P->ev_end;
+    rc = launch_wfa_tier(P, rt, wl, (uint32_t*)P->d_work2.p, d_ctr, P->ctr_dev, rg,
+                         (uint8_t*)P->d_arena.p, ctx->stream, false, P->fixed() ? TIME_NONE : TIME_BOTH);
+    if (rc != PHX_OK) return rc;
+    if (!P->fixed()) {
+      if (plan_event(P, &P->ev_end) != PHX_OK) return PHX_E_DEVICE;
+      HIPCHK(hipEventRecord(P->ev_end, ctx->stream));
+      P->end_ev = P->ev_end;
+    }
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (read_flags(P, 1) != PHX_OK) return PHX_E_DEVICE;
     std::vector<uint32_t> again;
     if (ctr_h[0] != 0) {
       HIPCHK(hipMemcpy(outs.data(), P->d_out.p, sizeof(PairOut) * P->n, hipMemcpyDeviceToHost));
@@ -591,9 +679,17 @@ void plan_group(phx_plan* P) {
     ask_sum += ask[g];
   }
   P->group_share.assign(P->work.size(), 0.0);
-  for (size_t g = 1; g < P->work.size(); ++g)
-    P->group_share[g] = ask_sum <= (double)total ? ask[g] / (double)total
+  double share_sum = 0;
+  for (size_t g = 1; g < P->work.size(); ++g) {
+    // Fixed plans split their region (sized only after the persistent
+    // buffers are carved) in proportion to the asks.
+    P->group_share[g] = P->fixed() ? (ask_sum > 0 ? ask[g] / ask_sum : 0.0)
+                      : ask_sum <= (double)total ? ask[g] / (double)total
                                                  : std::max(0.01, ask[g] / ask_sum);
+    share_sum += P->group_share[g];
+  }
+  // The groups' slices together never exceed the budget.
+  P->share_norm = std::max(1.0, share_sum);
   // Duration model per group for the launch order: a pair's time grows as
   // steps^2 / threads (cells) + steps (per-step latency); a group takes at
   // least its longest pair and at least its total over its pairs in flight.
@@ -676,17 +772,32 @@ phx_plan* plan_setup(phx_plan* P, const std::vector<const uint8_t*>& qh,
   }
   // Device buffers.
   auto fail_plan = [&](const char* what, int code = PHX_E_NOMEM) -> phx_plan* {
-    set_error(ctx, std::string("device allocation failed: ") + what, code);
+    // A fixed plan's carve failure has already said what did not fit.
+    if (!(P->fixed() && code == PHX_E_NOMEM))
+      set_error(ctx, std::string("device allocation failed: ") + what, code);
     phx_plan_destroy(P);
     return nullptr;
   };
   const uint64_t words = packed ? packed->size() : 0;
-  if (dev_alloc(ctx, P->d_seq, words * 8 + 8) != PHX_OK) return fail_plan("seq");
-  if (dev_alloc(ctx, P->d_pairs, sizeof(PairDesc) * (size_t)std::max<uint32_t>(n, 1)) != PHX_OK) return fail_plan("pairs");
-  if (dev_alloc(ctx, P->d_out, sizeof(PairOut) * (size_t)std::max<uint32_t>(n, 1)) != PHX_OK) return fail_plan("out");
-  if (dev_alloc(ctx, P->d_runs, P->runs_total * 4 + 4) != PHX_OK) return fail_plan("runs");
-  if (dev_alloc(ctx, P->d_work, (size_t)std::max<uint32_t>(n, 1) * 4) != PHX_OK) return fail_plan("work");
-  if (dev_alloc(ctx, P->d_ctr, 256) != PHX_OK) return fail_plan("counters");
+  if (plan_alloc(P, P->d_seq, words * 8 + 8) != PHX_OK) return fail_plan("seq");
+  if (plan_alloc(P, P->d_pairs, sizeof(PairDesc) * (size_t)std::max<uint32_t>(n, 1)) != PHX_OK) return fail_plan("pairs");
+  if (plan_alloc(P, P->d_out, sizeof(PairOut) * (size_t)std::max<uint32_t>(n, 1)) != PHX_OK) return fail_plan("out");
+  if (plan_alloc(P, P->d_runs, P->runs_total * 4 + 4) != PHX_OK) return fail_plan("runs");
+  if (plan_alloc(P, P->d_work, (size_t)std::max<uint32_t>(n, 1) * 4) != PHX_OK) return fail_plan("work");
+  if (plan_alloc(P, P->d_ctr, 256) != PHX_OK) return fail_plan("counters");
+  if (P->fixed()) {
+    // Re-run lists and the overflow flags (words of the region, read back
+    // after each run) are carved up front: a run allocates nothing.
+    DevBuf flags;
+    if (plan_alloc(P, P->d_work2, (size_t)std::max<uint32_t>(n, 1) * 4) != PHX_OK ||
+        plan_alloc(P, flags, 64 * sizeof(uint32_t)) != PHX_OK)
+      return fail_plan("flags");
+    P->ctr_dev = (uint32_t*)flags.p;
+    P->flag_shadow.assign(64, 0);
+    P->ctr_host = P->flag_shadow.data();
+    P->fixed_transient = (P->fixed_used + 255) & ~(uint64_t)255;
+    if (P->fixed_transient >= P->fixed_cap) return fail_plan("transient region");
+  }
   // Descriptors address their sequences by device pointer (packed batches:
   // into the plan's own arena; window plans come with pointers set).
   if (packed) {
@@ -698,8 +809,9 @@ phx_plan* plan_setup(phx_plan* P, const std::vector<const uint8_t*>& qh,
   if ((words && hipMemcpy(P->d_seq.p, packed->data(), words * 8, hipMemcpyHostToDevice) != hipSuccess) ||
       (n && hipMemcpy(P->d_pairs.p, P->pairs.data(), sizeof(PairDesc) * n, hipMemcpyHostToDevice) != hipSuccess))
     return fail_plan("upload", PHX_E_DEVICE);
-  if (hipHostMalloc((void**)&P->ctr_host, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
-      hipHostGetDevicePointer((void**)&P->ctr_dev, P->ctr_host, 0) != hipSuccess)
+  if (!P->fixed() &&
+      (hipHostMalloc((void**)&P->ctr_host, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
+       hipHostGetDevicePointer((void**)&P->ctr_dev, P->ctr_host, 0) != hipSuccess))
     return fail_plan("pinned counters");
   // Work lists are static between tunings: upload once.
   if (upload_work(P) != PHX_OK) return fail_plan("work upload", PHX_E_DEVICE);
@@ -709,7 +821,7 @@ phx_plan* plan_setup(phx_plan* P, const std::vector<const uint8_t*>& qh,
 int run_myers(phx_plan* P) {
   phx_ctx* ctx = P->ctx;
   uint32_t* d_list_base = (uint32_t*)P->d_work.p;
-  const uint64_t budget = P->tun.hbm_budget_mib << 20;
+  const uint64_t budget = P->fixed() ? P->fixed_cap - P->fixed_transient : P->tun.hbm_budget_mib << 20;
   uint64_t list_off = 0;
   for (size_t g = 0; g < P->work.size(); ++g) {
     const auto& wl = P->work[g];
@@ -725,7 +837,10 @@ int run_myers(phx_plan* P) {
     }
     uint64_t chunk = stride ? std::max<uint64_t>(1, budget / stride) : wl.size();
     chunk = std::min<uint64_t>(chunk, wl.size());
-    if (stride && dev_alloc(ctx, P->d_scratch, (size_t)(chunk * stride)) != PHX_OK) return PHX_E_NOMEM;
+    if (stride) {
+      const int rc = plan_transient(P, P->d_scratch, (size_t)(chunk * stride));
+      if (rc != PHX_OK) return rc;
+    }
     for (uint64_t c0 = 0; c0 < wl.size(); c0 += chunk) {
       const uint64_t cn = std::min<uint64_t>(chunk, wl.size() - c0);
       MyersArgs a{};
@@ -738,11 +853,15 @@ int run_myers(phx_plan* P) {
       a.scratch_stride = stride;
       a.engine = P->engine == PHX_ENGINE_MYERS ? ENG_MYERS
                : P->engine == PHX_ENGINE_MYERS_GLOBAL ? ENG_MYERS_GLOBAL : ENG_MYERS_DIST;
-      hipEvent_t e0, e1;
-      if (kernel_events(P, P->last_launches, &e0, &e1) != PHX_OK) return PHX_E_DEVICE;
-      HIPCHK(hipEventRecord(e0, ctx->stream));
-      HIPCHK(launch_myers(nbt, a, ctx->stream));
-      HIPCHK(hipEventRecord(e1, ctx->stream));
+      if (P->fixed()) {
+        HIPCHK(launch_myers(nbt, a, ctx->stream));
+      } else {
+        hipEvent_t e0, e1;
+        if (kernel_events(P, P->last_launches, &e0, &e1) != PHX_OK) return PHX_E_DEVICE;
+        HIPCHK(hipEventRecord(e0, ctx->stream));
+        HIPCHK(launch_myers(nbt, a, ctx->stream));
+        HIPCHK(hipEventRecord(e1, ctx->stream));
+      }
       ++P->last_launches;
       if (c0 + cn < wl.size()) HIPCHK(hipStreamSynchronize(ctx->stream));  // scratch reuse
     }
@@ -808,6 +927,47 @@ phx_plan* plan_create_windows(phx_ctx* ctx, int engine, phx_costs costs, int64_t
 }  // namespace host
 }  // namespace phx
 
+// Device scratch of phx_align_batch_noalloc: one array per GPU, held from the
+// library's load until the process ends.  In host code this name is only a
+// stand-in for the symbol; the device address of a GPU's copy comes from
+// hipGetSymbolAddress with that device current (noalloc_scratch_of), and only
+// that address is ever passed to kernels or copies.
+static __device__ __align__(256) uint8_t scratch[PHX_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+struct NoAllocDevice {
+  bool looked_up = false;
+  uint8_t* ptr = nullptr;   // this device's copy of `scratch`
+  std::mutex busy;          // one allocation-free call per device at a time
+};
+
+NoAllocDevice& noalloc_device(int device) {
+  static std::mutex mu;
+  static std::vector<std::unique_ptr<NoAllocDevice>> devs;
+  std::lock_guard<std::mutex> lk(mu);
+  if (device >= (int)devs.size()) devs.resize((size_t)device + 1);
+  if (!devs[(size_t)device]) devs[(size_t)device].reset(new NoAllocDevice);
+  return *devs[(size_t)device];
+}
+
+// Address of `scratch` on `device`, looked up once per device (the device
+// must be current); null when the lookup fails.
+uint8_t* noalloc_scratch_of(int device) {
+  static std::mutex mu;
+  NoAllocDevice& d = noalloc_device(device);
+  std::lock_guard<std::mutex> lk(mu);
+  if (!d.looked_up) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    d.ptr = e == hipSuccess ? (uint8_t*)p : nullptr;
+    d.looked_up = true;
+  }
+  return d.ptr;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* phx_version(void) { return "phraya_hip 0.1 gfx950"; }
@@ -837,6 +997,9 @@ phx_ctx* phx_ctx_create(int device, uint32_t flags, const phx_tuning* tuning) {
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
+  // The allocation-free entry point's scratch on this device (its address is
+  // looked up once per device, here, with the device current).
+  c->noalloc_scratch = noalloc_scratch_of(c->device);
   for (int i = 0; i < kStreams; ++i) {
     if (hipStreamCreateWithFlags(&c->streams[i], hipStreamNonBlocking) != hipSuccess) { delete c; return nullptr; }
   }
@@ -883,7 +1046,13 @@ uint64_t phx_cigar_bound(const phx_batch* b) {
   return s;
 }
 
-phx_plan* phx_plan_create(phx_ctx* ctx, const phx_batch* b) {
+}  // extern "C"
+
+namespace {
+
+// phx_plan_create, optionally on a fixed device region (fixed_base /
+// fixed_cap bytes) from which every device buffer is carved.
+phx_plan* plan_create_batch(phx_ctx* ctx, const phx_batch* b, uint8_t* fixed_base, uint64_t fixed_cap) {
   if (!ctx || !b || b->engine < 0 || b->engine > PHX_ENGINE_MYERS_DIST) {
     set_error(ctx, "bad batch");
     return nullptr;
@@ -892,6 +1061,8 @@ phx_plan* phx_plan_create(phx_ctx* ctx, const phx_batch* b) {
   if (hipSetDevice(ctx->device) != hipSuccess) { set_error(ctx, "hipSetDevice failed", PHX_E_DEVICE); return nullptr; }
   phx_plan* P = new phx_plan;
   P->ctx = ctx;
+  P->fixed_base = fixed_base;
+  P->fixed_cap = fixed_base ? fixed_cap : 0;
   P->tun = ctx->tun;
   P->engine = b->engine;
   P->costs = b->costs;
@@ -977,6 +1148,12 @@ phx_plan* phx_plan_create(phx_ctx* ctx, const phx_batch* b) {
   return plan_setup(P, qh, th, &packed, &q_word, &t_word);
 }
 
+}  // namespace
+
+extern "C" {
+
+phx_plan* phx_plan_create(phx_ctx* ctx, const phx_batch* b) { return plan_create_batch(ctx, b, nullptr, 0); }
+
 int phx_plan_run(phx_plan* P) {
   if (!P) return PHX_E_ARG;
   phx_ctx* ctx = P->ctx;
@@ -989,7 +1166,7 @@ int phx_plan_run(phx_plan* P) {
   P->ms_ready = P->kernel_ms_ready = false;
   int rc = is_wfa(P->engine) ? run_wfa(P) : run_myers(P);
   if (rc != PHX_OK) return rc;
-  if (!P->end_ev && P->last_launches > 0) {
+  if (!P->end_ev && P->last_launches > 0 && !P->fixed()) {
     if (plan_event(P, &P->ev_end) != PHX_OK) return PHX_E_DEVICE;
     HIPCHK(hipEventRecord(P->ev_end, ctx->stream));
     P->end_ev = P->ev_end;
@@ -1244,8 +1421,10 @@ int phx_plan_results(phx_plan* P, phx_results* r) {
   std::vector<uint64_t> idx(2 * (size_t)P->n + 1);
   std::copy(src.begin(), src.end(), idx.begin());
   std::copy(r->run_off, r->run_off + P->n + 1, idx.begin() + P->n);
-  if (dev_alloc(ctx, P->d_gidx, idx.size() * 8) != PHX_OK) return PHX_E_NOMEM;
-  if (dev_alloc(ctx, P->d_gather, pos * 4) != PHX_OK) return PHX_E_NOMEM;
+  // (Fixed plans: the run has finished, so its transient region holds these.)
+  rc = plan_transient(P, P->d_gidx, idx.size() * 8);
+  if (rc == PHX_OK) rc = plan_transient(P, P->d_gather, pos * 4, idx.size() * 8);
+  if (rc != PHX_OK) return rc;
   HIPCHK(hipMemcpyAsync(P->d_gidx.p, idx.data(), idx.size() * 8, hipMemcpyHostToDevice, ctx->stream));
   const uint64_t* d_src = (const uint64_t*)P->d_gidx.p;
   HIPCHK(launch_gather_runs((const uint32_t*)P->d_runs.p, d_src, d_src + P->n,
@@ -1258,21 +1437,38 @@ int phx_plan_results(phx_plan* P, phx_results* r) {
 void phx_plan_destroy(phx_plan* P) {
   if (!P) return;
   (void)hipStreamSynchronize(P->ctx->stream);
-  for (DevBuf* b : {&P->d_seq, &P->d_pairs, &P->d_out, &P->d_runs, &P->d_work, &P->d_work2, &P->d_ctr,
-                    &P->d_scratch, &P->d_gather, &P->d_gidx})
-    dev_free(*b);
-  dev_free(P->d_arena);
+  if (!P->fixed()) {  // a fixed plan's buffers are carves of the scratch array
+    for (DevBuf* b : {&P->d_seq, &P->d_pairs, &P->d_out, &P->d_runs, &P->d_work, &P->d_work2, &P->d_ctr,
+                      &P->d_scratch, &P->d_gather, &P->d_gidx})
+      dev_free(*b);
+    dev_free(P->d_arena);
+  }
   if (P->gexec) (void)hipGraphExecDestroy(P->gexec);
   for (hipEvent_t e : P->kev) (void)hipEventDestroy(e);
   for (hipEvent_t e : {P->ev_start, P->ev_end})
     if (e) (void)hipEventDestroy(e);
-  if (P->ctr_host) (void)hipHostFree(P->ctr_host);
+  if (P->ctr_host && !P->fixed()) (void)hipHostFree(P->ctr_host);
   delete P;
 }
 
 int phx_align_batch(phx_ctx* ctx, const phx_batch* b, phx_results* out) {
   if (!ctx || !b || !out) return PHX_E_ARG;
   phx_plan* P = phx_plan_create(ctx, b);
+  if (!P) return last_error_code(ctx);
+  int rc = phx_plan_run(P);
+  if (rc == PHX_OK) rc = phx_plan_results(P, out);
+  phx_plan_destroy(P);
+  return rc;
+}
+
+int phx_align_batch_noalloc(phx_ctx* ctx, const phx_batch* b, phx_results* out) {
+  if (!ctx || !b || !out) return PHX_E_ARG;
+  if (!ctx->noalloc_scratch)
+    return fail(ctx, PHX_E_DEVICE, "the allocation-free scratch has no address on this device");
+  // Every call on this GPU shares its scratch array: calls are serialised.
+  NoAllocDevice& d = noalloc_device(ctx->device);
+  std::lock_guard<std::mutex> busy(d.busy);
+  phx_plan* P = plan_create_batch(ctx, b, ctx->noalloc_scratch, PHX_NOALLOC_SCRATCH_BYTES);
   if (!P) return last_error_code(ctx);
   int rc = phx_plan_run(P);
   if (rc == PHX_OK) rc = phx_plan_results(P, out);
