@@ -110,6 +110,188 @@ def cpu_baseline(w, engine_name, costs, seconds, sample_n=None):
     return out
 
 
+TIER_NAMES = {0: "phx::wfa_wave_kernel<1, true> (LDS-resident wave per pair)",
+              1: "phx::wfa_wave_kernel<1, false> (wave per pair, HBM slots)",
+              2: "phx::wfa_block_kernel<1, 256, *> (256-thread workgroup per pair)",
+              3: "phx::wfa_block_kernel<1, 1024, *> (1024-thread workgroup per pair)",
+              8: "phx::myers_kernel<*> (lane per pair)"}
+
+
+def dominant_kernel(launches):
+    """The tier with the most device time among one run's launches, with its
+    algorithmic bytes per launch over its mean launch duration."""
+    by = {}
+    for l in launches:
+        d = by.setdefault(l["tier"], {"ms": 0.0, "bytes": 0, "launches": 0, "pairs": 0, "cells": 0})
+        d["ms"] += l["ms"]
+        d["bytes"] += l["algorithmic_bytes"]
+        d["launches"] += 1
+        d["pairs"] += l["pairs"]
+        d["cells"] += l["cells"]
+    tier = max(by, key=lambda t: by[t]["ms"])
+    d = by[tier]
+    mean_ms = d["ms"] / d["launches"]
+    per_launch = d["bytes"] / d["launches"]
+    return {"kernel": TIER_NAMES.get(tier, str(tier)), "tier": tier, "launches": d["launches"],
+            "pairs": d["pairs"], "mean_launch_ms": mean_ms, "bytes_per_launch": per_launch,
+            "achieved_gbs": per_launch / (mean_ms / 1000.0) / 1e9 if mean_ms > 0 else 0.0,
+            "cells": d["cells"], "share_of_kernel_time": d["ms"] / sum(v["ms"] for v in by.values())}
+
+
+def mixed_block(args):
+    """The mixed-length workload (200k pairs, 100 bp - 20 kbp) measured beside
+    the default one: a few timed steps on default tuning, the dominant
+    kernel's roofline from the plan's per-launch events, an oracle parity
+    sample of the pairs the oracle holds (qn + tn <= 4000; the full-size
+    checks are in tests/test_gpu_fullsize.py) and a bounded CPU sample."""
+    import phraya_amd as px
+    from phraya_amd import synth
+    from oracle import oracle as O
+    w = synth.workload("mixed")
+    pb = px.PackedBatch(w.q_seq, w.q_off, w.t_seq, w.t_off, np.zeros(w.n, np.uint64),
+                        np.zeros(w.n, np.uint64))
+    plan = px.Plan(pb, px.ENGINE_AFFINE, px.AffineCosts(4, 6, 2))
+    plan.run()  # warmup
+    step_ms, launches = [], None
+    for _ in range(args.mixed_steps):
+        t0 = time.perf_counter()
+        plan.run()
+        step_ms.append((time.perf_counter() - t0) * 1000.0)
+        launches = plan.last_launch_info()
+    res = plan.results()
+    dom = dominant_kernel(launches)
+    qn = np.diff(w.q_off).astype(np.int64)
+    tn = np.diff(w.t_off).astype(np.int64)
+    small = np.nonzero(qn + tn <= 4000)[0]
+    idx = np.random.default_rng(7).choice(small, min(300, len(small)), replace=False)
+    sub = w.subset(idx)
+    ores, ocig = O.batch(O.AFFINE, sub.q_seq, sub.q_off, sub.t_seq, sub.t_off, costs=(4, 6, 2),
+                         threads=len(os.sched_getaffinity(0)))
+    mism = sum(1 for j, i in enumerate(idx)
+               if int(res.aln["score"][i]) != int(ores["score"][j]) or res.cigar(int(i)) != ocig[j]
+               or int(res.aln["edit_distance"][i]) != int(ores["edit_distance"][j]))
+    ms = float(np.mean(step_ms))
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_mixed.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    out = {"workload": "mixed", "description": WORKLOADS["mixed"][3], "pairs_per_step": w.n,
+           "steps": args.mixed_steps, "warmup": 1, "value": w.n / (ms / 1000.0), "unit": "pairs/s",
+           "ms_per_step": ms,
+           "latency_ms": {"p50": float(np.percentile(step_ms, 50)),
+                          "p99": float(np.percentile(step_ms, 99)),
+                          "what": "one whole 200k batch (host clock around phx_plan_run)"},
+           "gcups_equiv": float((qn * tn).sum()) / (ms / 1000.0) / 1e9,
+           "wavefront_cells_per_s": sum(l["cells"] for l in launches) / (ms / 1000.0),
+           "launches_per_step": len(launches),
+           "dominant_kernel": dom,
+           "roofline": {"bound": "hbm", "achieved": dom["achieved_gbs"], "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": dom["achieved_gbs"] / HBM_PEAK_GBS,
+                        "traffic": traffic,
+                        "algorithmic_bytes_per_launch": dom["bytes_per_launch"]},
+           "parity": {"checked": len(idx), "mismatches": mism,
+                      "what": "seeded sample of pairs with qn + tn <= 4000 vs the oracle"},
+           "tuning": "default (no autotuner)"}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, "affine", (4, 6, 2), args.cpu_seconds, sample_n=200)
+        out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    plan.close()
+    return out
+
+
+def drop_in_block(w, reps: int = 10):
+    """The drop-in as a caller of the C ABI sees it, on the default workload
+    (host buffers in, host results out; only the C calls are timed):
+    phx_align_batch (pack, upload, kernel, download) plus the CIGAR text of
+    every pair (phx_format_cigars); per-pair phx_wfa_extend_affine calls; and
+    the executor step phx_align_reads_batch over the same reads in the shape
+    `phraya align` runs (one forward chain per read at its source position,
+    window target[pos .. pos + 2 qlen], so the fitting branch), Sensitive
+    (gap-affine 2/4/1) and Balanced (Myers fitting)."""
+    import ctypes
+    import phraya_amd as px
+    from phraya_amd import _lib as L
+    from phraya_amd import synth
+    lib = L.lib()
+    ctx = px.default_context()
+    n = w.n
+    pb = px.PackedBatch(w.q_seq, w.q_off, w.t_seq, w.t_off, np.zeros(n, np.uint64), np.zeros(n, np.uint64))
+    cb = px._cbatch(pb, px.ENGINE_AFFINE, px.AffineCosts(4, 6, 2), None)
+    cap = int(lib.phx_cigar_bound(ctypes.byref(cb)))
+    aln = np.zeros(n, L.ALN_DTYPE)
+    runs = np.zeros(cap, np.uint32)
+    off = np.zeros(n + 1, np.uint64)
+    text_off = np.zeros(n + 1, np.uint64)
+    text = np.zeros(cap * 8, np.uint8)
+    res = L.CResults(aln.ctypes.data, runs.ctypes.data, cap, off.ctypes.data)
+    out = {}
+    times = []
+    for r in range(reps + 1):
+        t0 = time.perf_counter()
+        rc = lib.phx_align_batch(ctx.handle, ctypes.byref(cb), ctypes.byref(res))
+        tl = lib.phx_format_cigars(runs.ctypes.data, off.ctypes.data, n, text.ctypes.data, len(text),
+                                   text_off.ctypes.data)
+        if r:
+            times.append((time.perf_counter() - t0) * 1000.0)
+        assert rc == 0 and tl <= len(text)
+    ms = float(np.median(times))
+    out["align_batch"] = {"entry": "phx_align_batch + phx_format_cigars", "pairs": n,
+                          "ms_per_call_p50": ms, "ms_per_call_p99": float(np.percentile(times, 99)),
+                          "pairs_per_s": n / (ms / 1000.0), "calls": reps,
+                          "what": "host sequences in, host scores and CIGAR text out (pack, upload, "
+                                  "kernel, download, text), buffers reused across calls"}
+    # Per-pair calls: what an unchanged extend_anchor would see.
+    o = L.CAlignment()
+    cig = ctypes.create_string_buffer(4096)
+    clen = ctypes.c_size_t(0)
+    lat = []
+    for i in range(min(300, n)):
+        q, t = w.pair(i)
+        t0 = time.perf_counter()
+        rc = lib.phx_wfa_extend_affine(ctx.handle, q, len(q), t, len(t), 0, 0, L.Costs(4, 6, 2),
+                                       ctypes.byref(o), cig, 4096, ctypes.byref(clen))
+        lat.append((time.perf_counter() - t0) * 1e6)
+        assert rc == 0
+    lat = lat[20:]
+    out["per_pair"] = {"entry": "phx_wfa_extend_affine", "calls": len(lat),
+                       "us_p50": float(np.median(lat)), "us_p99": float(np.percentile(lat, 99))}
+    # The executor step over the same reads against the genome.
+    g = synth.genome(w.params.get("genome_len", 5_000_000), w.params.get("genome_seed", 0x5EED0001))
+    strands = (L.CReadStrands * n)()
+    starts = np.ascontiguousarray(w.pos, np.uint64)
+    for i in range(n):
+        strands[i] = L.CReadStrands(w.q_seq.ctypes.data + int(w.q_off[i]), int(w.q_off[i + 1] - w.q_off[i]),
+                                    starts.ctypes.data + 8 * i, 1, None, 0, 1)
+    for name, strategy in (("sensitive_affine", px.STRATEGY_SENSITIVE), ("balanced_myers", px.STRATEGY_BALANCED)):
+        ne, na, nr = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        assert lib.phx_align_reads_bounds(strategy, 1, len(g), strands, n, ctypes.byref(ne),
+                                          ctypes.byref(na), ctypes.byref(nr)) == 0
+        E, A, R = ne.value, na.value, nr.value
+        place = (L.CReadPlacement * n)()
+        ext = (L.CReadExtension * E)()
+        a_aln = np.zeros(A, L.ALN_DTYPE)
+        role = np.zeros(A, np.uint8)
+        a_runs = np.zeros(R, np.uint32)
+        a_off = np.zeros(A + 1, np.uint64)
+        t_ms = []
+        for r in range(reps + 1):
+            t0 = time.perf_counter()
+            rc = lib.phx_align_reads_batch(ctx.handle, strategy, 0, 1, g.ctypes.data, len(g), strands, n,
+                                           place, ext, E, a_aln.ctypes.data, role.ctypes.data, A,
+                                           a_runs.ctypes.data, R, a_off.ctypes.data)
+            if r:
+                t_ms.append((time.perf_counter() - t0) * 1000.0)
+            assert rc == 0
+        placed = sum(1 for i in range(n) if place[i].outcome == px.OUTCOME_PLACED)
+        m = float(np.median(t_ms))
+        out[f"align_reads_{name}"] = {"entry": "phx_align_reads_batch", "reads": n, "ms_per_call_p50": m,
+                                      "ms_per_call_p99": float(np.percentile(t_ms, 99)),
+                                      "reads_per_s": n / (m / 1000.0), "placed": placed,
+                                      "what": "one forward chain per read, window target[pos .. pos + 300]"}
+    return out
+
+
 def calibrate_copy(nbytes: int) -> None:
     """A plain device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE
     counters of the alignment kernels (run under rocprofv3 --pmc)."""
@@ -797,6 +979,10 @@ def main():
                     help="phx_tuning overrides, e.g. block256_min_len=16384,heavy_permille=20")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the autotuner (runs before the timed steps)")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="default workload: skip the mixed-length block measured beside it")
+    ap.add_argument("--mixed-steps", type=int, default=3,
+                    help="timed steps of the mixed-length block")
     ap.add_argument("--graph", action="store_true",
                     help="replay each WFA step as a HIP graph (recorded in warmup) instead of "
                          "launching it eagerly")
@@ -947,8 +1133,15 @@ def main():
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]
-    print(json.dumps(out), flush=True)
     plan.close()
+    if args.workload == "short150" and args.n is None:
+        log("drop-in calls")
+        out["drop_in"] = drop_in_block(w)
+    if args.workload == "short150" and args.n is None and not args.no_mixed:
+        log("mixed-length block")
+        out["mixed"] = mixed_block(args)
+        log(f"mixed: {out['mixed']['value']:.4g} pairs/s")
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

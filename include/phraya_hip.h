@@ -168,6 +168,25 @@ int phx_plan_results(phx_plan* plan, phx_results* out);
 double phx_plan_last_ms(const phx_plan* plan);
 double phx_plan_last_kernel_ms(const phx_plan* plan);
 uint32_t phx_plan_last_launches(const phx_plan* plan);
+/* Launch i (0 <= i < launches) of the last run: which kernel tier ran it
+ * (0 LDS-resident wave per pair, 1 wave per pair with HBM slots, 2 a
+ * 256-thread workgroup per pair, 3 a 1024-thread workgroup per pair, 8 Myers
+ * lane per pair), its workgroups and pairs, its device time (events on its
+ * stream around it), and what its pairs account for: algorithmic HBM bytes
+ * (packed inputs + 12 B of results + 4 B per CIGAR run; Myers distance: 16 B),
+ * wavefront cells written and score steps.  Instrumentation; no counterpart
+ * in the reference. */
+typedef struct phx_launch_info {
+  int32_t tier;
+  uint32_t pairs;
+  uint32_t grid;
+  uint32_t reserved;
+  double ms;
+  uint64_t algorithmic_bytes;
+  uint64_t cells;
+  uint64_t steps;
+} phx_launch_info;
+int phx_plan_last_launch(const phx_plan* plan, uint32_t i, phx_launch_info* out);
 /* Re-tune a plan (non-zero fields of *tuning replace the plan's): the pairs
  * are regrouped into tiers without re-uploading them.  Results never depend
  * on the tuning. */
@@ -196,6 +215,13 @@ void phx_plan_destroy(phx_plan* plan);
 /* Render runs as phraya CIGAR text. Writes at most cap bytes including the
  * NUL; returns the full text length (excluding the NUL). */
 size_t phx_format_cigar(const uint32_t* runs, uint64_t nruns, char* buf, size_t cap);
+/* Renders the CIGARs of n pairs in one call: pair i's runs[run_off[i] ..
+ * run_off[i + 1]) become buf[text_off[i] .. text_off[i + 1]) (no NULs).
+ * Returns the total text length; writes no run whose text would pass cap
+ * (text_off is complete either way, so a caller can size buf and retry).
+ * Host only; text_off may be NULL. */
+uint64_t phx_format_cigars(const uint32_t* runs, const uint64_t* run_off, uint64_t n, char* buf,
+                           uint64_t cap, uint64_t* text_off);
 
 /* Per-pair forms of the upstream API.  The CIGAR text is written to cigar
  * (NUL-terminated, truncated to cigar_cap); the return value is the pair

@@ -31,8 +31,8 @@ ENGINE_MYERS_DIST = 4
 EXPORTS = (
     "phx_ctx_create", "phx_ctx_destroy", "phx_ctx_last_error", "phx_cigar_bound",
     "phx_align_batch", "phx_align_batch_noalloc", "phx_plan_create", "phx_plan_run", "phx_plan_results",
-    "phx_plan_last_ms", "phx_plan_last_kernel_ms", "phx_plan_last_launches",
-    "phx_plan_algorithmic_bytes", "phx_plan_destroy", "phx_format_cigar",
+    "phx_plan_last_ms", "phx_plan_last_kernel_ms", "phx_plan_last_launches", "phx_plan_last_launch",
+    "phx_plan_algorithmic_bytes", "phx_plan_destroy", "phx_format_cigar", "phx_format_cigars",
     "phx_plan_set_tuning", "phx_plan_get_tuning", "phx_plan_autotune", "phx_plan_use_graph",
     "phx_wfa_extend", "phx_wfa_extend_affine", "phx_myers_extend", "phx_myers_edit_distance",
     "phx_serve_start", "phx_serve_submit", "phx_serve_poll", "phx_serve_stats_get",
@@ -68,6 +68,13 @@ ALN_DTYPE = np.dtype([("status", np.int32), ("reserved", np.int32), ("score", np
                       ("query_end", np.int64), ("target_start", np.int64),
                       ("target_end", np.int64)])
 assert ALN_DTYPE.itemsize == ctypes.sizeof(CAlignment)
+
+
+class CLaunchInfo(ctypes.Structure):
+    _fields_ = [("tier", ctypes.c_int32), ("pairs", ctypes.c_uint32), ("grid", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("ms", ctypes.c_double),
+                ("algorithmic_bytes", ctypes.c_uint64), ("cells", ctypes.c_uint64),
+                ("steps", ctypes.c_uint64)]
 
 
 class CBatch(ctypes.Structure):
@@ -224,6 +231,7 @@ def lib() -> ctypes.CDLL:
     L.phx_plan_last_kernel_ms.restype = ctypes.c_double
     L.phx_plan_last_launches.argtypes = [P]
     L.phx_plan_last_launches.restype = ctypes.c_uint32
+    L.phx_plan_last_launch.argtypes = [P, ctypes.c_uint32, P]
     L.phx_plan_algorithmic_bytes.argtypes = [P]
     L.phx_plan_algorithmic_bytes.restype = ctypes.c_uint64
     L.phx_plan_destroy.argtypes = [P]
@@ -272,6 +280,8 @@ def lib() -> ctypes.CDLL:
     L.phx_extract_variants.argtypes = [P, P, P]
     L.phx_format_cigar.argtypes = [P, ctypes.c_uint64, P, ctypes.c_size_t]
     L.phx_format_cigar.restype = ctypes.c_size_t
+    L.phx_format_cigars.argtypes = [P, P, ctypes.c_uint64, P, ctypes.c_uint64, P]
+    L.phx_format_cigars.restype = ctypes.c_uint64
     sz, u64 = ctypes.c_size_t, ctypes.c_uint64
     L.phx_wfa_extend.argtypes = [P, P, sz, P, sz, u64, u64, P, P, sz, P]
     L.phx_wfa_extend_affine.argtypes = [P, P, sz, P, sz, u64, u64, Costs, P, P, sz, P]

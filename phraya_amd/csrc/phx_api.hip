@@ -77,6 +77,10 @@ struct phx_plan {
   // and, once the run has finished, the result gather.
   uint8_t* fixed_base = nullptr;
   uint64_t fixed_cap = 0, fixed_used = 0, fixed_transient = 0;
+  // One-shot plans (phx_align_batch, the executor's) borrow their device
+  // buffers, pinned flags and events from the context's cache and give them
+  // back when destroyed, so back-to-back calls allocate nothing.
+  bool cached = false;
   std::vector<uint32_t> flag_shadow;                   // host copy of the overflow flags (fixed plans)
   bool fixed() const { return fixed_base != nullptr; }
   double share_norm = 1.0;                             // sum of group shares, when above 1
@@ -92,6 +96,16 @@ struct phx_plan {
   std::vector<PairOut> out_h;
   bool out_valid = false;
   uint32_t last_launches = 0;
+  // What each kernel launch of the last run processed (for per-launch
+  // timings): its tier, workgroups, and pairs list[off, off + count).
+  struct LaunchRec {
+    int32_t tier = 0;
+    uint32_t grid = 0;
+    const std::vector<uint32_t>* list = nullptr;
+    uint64_t off = 0, count = 0;
+  };
+  std::vector<LaunchRec> launch_rec, g_launch_rec;  // eager runs / the recorded graph
+  std::vector<std::vector<uint32_t>> rerun_lists;   // re-run lists of the last run
   // Timing of the last run, from the plan's own events (so that runs of other
   // plans on the context leave it alone): a pair of events around every
   // kernel launch, the run's start (null: the first launch's start event)
@@ -104,7 +118,66 @@ struct phx_plan {
   mutable double last_ms = 0, last_kernel_ms = 0;
 };
 
+struct PlanCache {
+  bool busy = false;
+  DevBuf seq, pairs, out, runs, work, work2, ctr, scratch, gather, gidx, arena;
+  uint32_t* ctr_host = nullptr;
+  uint32_t* ctr_dev = nullptr;
+  std::vector<hipEvent_t> kev;
+  hipEvent_t ev_start = nullptr, ev_end = nullptr;
+};
+
 namespace {
+
+// Buffers above this size are freed when a one-shot plan returns them, so a
+// context does not hold on to one unusually large batch's memory.
+constexpr size_t kCacheKeepBytes = 256ull << 20;
+
+void swap_cache(PlanCache& C, phx_plan* P) {
+  std::swap(C.seq, P->d_seq);
+  std::swap(C.pairs, P->d_pairs);
+  std::swap(C.out, P->d_out);
+  std::swap(C.runs, P->d_runs);
+  std::swap(C.work, P->d_work);
+  std::swap(C.work2, P->d_work2);
+  std::swap(C.ctr, P->d_ctr);
+  std::swap(C.scratch, P->d_scratch);
+  std::swap(C.gather, P->d_gather);
+  std::swap(C.gidx, P->d_gidx);
+  std::swap(C.arena, P->d_arena);
+  std::swap(C.ctr_host, P->ctr_host);
+  std::swap(C.ctr_dev, P->ctr_dev);
+  std::swap(C.kev, P->kev);
+  std::swap(C.ev_start, P->ev_start);
+  std::swap(C.ev_end, P->ev_end);
+}
+
+// Borrows the context's cached buffers for a one-shot plan (false when
+// another one-shot plan of the context holds them).
+bool cache_take(phx_plan* P) {
+  phx_ctx* ctx = P->ctx;
+  std::lock_guard<std::mutex> lk(ctx->cache_mu);
+  if (!ctx->plan_cache) ctx->plan_cache = new PlanCache;
+  if (ctx->plan_cache->busy) return false;
+  ctx->plan_cache->busy = true;
+  swap_cache(*ctx->plan_cache, P);
+  P->cached = true;
+  return true;
+}
+
+void dev_free(DevBuf& b);
+
+// Gives a one-shot plan's buffers back to the context's cache.
+void cache_return(phx_plan* P) {
+  phx_ctx* ctx = P->ctx;
+  std::lock_guard<std::mutex> lk(ctx->cache_mu);
+  for (DevBuf* b : {&P->d_seq, &P->d_pairs, &P->d_out, &P->d_runs, &P->d_work, &P->d_work2, &P->d_ctr,
+                    &P->d_scratch, &P->d_gather, &P->d_gidx, &P->d_arena})
+    if (b->bytes > kCacheKeepBytes) dev_free(*b);
+  swap_cache(*ctx->plan_cache, P);
+  ctx->plan_cache->busy = false;
+  P->cached = false;
+}
 
 int fail(phx_ctx* c, int code, const std::string& msg) {
   set_error(c, msg, code);
@@ -313,6 +386,13 @@ int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint3
       G = block_lds_geometry(ring_frames(P->engine, P->costs), gs.wcap, seq_words);
     }
   }
+  phx_plan::LaunchRec rec;
+  rec.tier = t;
+  rec.grid = t <= TIER_WAVE ? std::max<uint32_t>(1, (slots + kWaveWaves - 1) / kWaveWaves)
+                            : std::max<uint32_t>(1, slots);
+  rec.list = &work;
+  rec.count = work.size();
+  P->launch_rec.push_back(rec);
   if (timing == TIME_NONE) {
     HIPCHK(launch_tier(t, mode, a, G, slots, st, rl));
     return PHX_OK;
@@ -453,6 +533,8 @@ int build_graph(phx_plan* P) {
   }
   std::vector<int> launched;
   std::vector<hipGraphNode_t> knodes;
+  const std::vector<phx_plan::LaunchRec> eager = P->launch_rec;
+  P->launch_rec.clear();
   HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
   const int rc = issue_groups(P, true, &launched, &knodes);
   hipGraph_t g = nullptr;
@@ -483,9 +565,12 @@ int build_graph(phx_plan* P) {
     ret = fail(ctx, PHX_E_DEVICE, "graph instantiation failed");
   }
   (void)hipGraphDestroy(g);
+  std::swap(P->launch_rec, P->g_launch_rec);
+  P->launch_rec = eager;  // the run that was just timed
   if (ret != PHX_OK) return ret;
   P->g_launched = launched;
   P->g_launches = (uint32_t)knodes.size();
+  P->g_launch_rec = P->launch_rec;
   P->g_arena = P->d_arena.p;
   return PHX_OK;
 }
@@ -504,6 +589,7 @@ int run_wfa(phx_plan* P) {
     P->end_ev = P->ev_end;
     launched = P->g_launched;
     P->last_launches = P->g_launches;
+    P->launch_rec = P->g_launch_rec;
   } else {
     const int rc = issue_groups(P, false, &launched, nullptr);
     if (rc != PHX_OK) return rc;
@@ -554,7 +640,8 @@ int run_wfa(phx_plan* P) {
     const GroupSlots rg = group_slots(P, rt, wl, slot_bytes, P->group_budget(-1));
     int rc = plan_transient(P, P->d_arena, (size_t)(rg.n * rg.per));
     if (rc != PHX_OK) return rc;
-    rc = launch_wfa_tier(P, rt, wl, (uint32_t*)P->d_work2.p, d_ctr, P->ctr_dev, rg,
+    P->rerun_lists.push_back(wl);
+    rc = launch_wfa_tier(P, rt, P->rerun_lists.back(), (uint32_t*)P->d_work2.p, d_ctr, P->ctr_dev, rg,
                          (uint8_t*)P->d_arena.p, ctx->stream, false, P->fixed() ? TIME_NONE : TIME_BOTH);
     if (rc != PHX_OK) return rc;
     if (!P->fixed()) {
@@ -809,7 +896,7 @@ phx_plan* plan_setup(phx_plan* P, const std::vector<const uint8_t*>& qh,
   if ((words && hipMemcpy(P->d_seq.p, packed->data(), words * 8, hipMemcpyHostToDevice) != hipSuccess) ||
       (n && hipMemcpy(P->d_pairs.p, P->pairs.data(), sizeof(PairDesc) * n, hipMemcpyHostToDevice) != hipSuccess))
     return fail_plan("upload", PHX_E_DEVICE);
-  if (!P->fixed() &&
+  if (!P->fixed() && !P->ctr_host &&
       (hipHostMalloc((void**)&P->ctr_host, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
        hipHostGetDevicePointer((void**)&P->ctr_dev, P->ctr_host, 0) != hipSuccess))
     return fail_plan("pinned counters");
@@ -853,6 +940,13 @@ int run_myers(phx_plan* P) {
       a.scratch_stride = stride;
       a.engine = P->engine == PHX_ENGINE_MYERS ? ENG_MYERS
                : P->engine == PHX_ENGINE_MYERS_GLOBAL ? ENG_MYERS_GLOBAL : ENG_MYERS_DIST;
+      phx_plan::LaunchRec rec;
+      rec.tier = 8;
+      rec.grid = (uint32_t)((cn + 255) / 256);
+      rec.list = &wl;
+      rec.off = c0;
+      rec.count = cn;
+      P->launch_rec.push_back(rec);
       if (P->fixed()) {
         HIPCHK(launch_myers(nbt, a, ctx->stream));
       } else {
@@ -877,8 +971,8 @@ namespace host {
 
 phx_plan* plan_create_windows(phx_ctx* ctx, int engine, phx_costs costs, int64_t cap,
                               std::vector<PairDesc> pairs, const std::vector<const uint8_t*>& qh,
-                              const std::vector<const uint8_t*>& th) {
-  if (!ctx || (engine != PHX_ENGINE_LINEAR && engine != PHX_ENGINE_AFFINE)) return nullptr;
+                              const std::vector<const uint8_t*>& th, bool oneshot) {
+  if (!ctx || engine < PHX_ENGINE_LINEAR || engine > PHX_ENGINE_MYERS_DIST) return nullptr;
   if (engine == PHX_ENGINE_AFFINE &&
       (costs.mismatch < 1 || costs.gap_extend < 1 || costs.gap_open < 0 ||
        ring_slots(engine, costs) > kRingSlotsMax)) {
@@ -897,6 +991,7 @@ phx_plan* plan_create_windows(phx_ctx* ctx, int engine, phx_costs costs, int64_t
   }
   phx_plan* P = new phx_plan;
   P->ctx = ctx;
+  if (oneshot) cache_take(P);
   P->tun = ctx->tun;
   P->engine = engine;
   P->costs = engine == PHX_ENGINE_AFFINE ? costs : phx_costs{0, 0, 0};
@@ -913,7 +1008,7 @@ phx_plan* plan_create_windows(phx_ctx* ctx, int engine, phx_costs costs, int64_t
     PairDesc& pd = pairs[i];
     P->qlen[i] = (uint64_t)pd.qn;
     P->tlen[i] = (uint64_t)pd.tn;
-    pd.run_cap = (uint32_t)(pd.qn + pd.tn + 1);
+    pd.run_cap = has_cigar(engine) ? (uint32_t)(pd.qn + pd.tn + 1) : 0;
     pd.run_off = runs;
     runs += pd.run_cap;
     const bool b8 = pd.flags & PF_BYTES;
@@ -1028,6 +1123,16 @@ phx_ctx* phx_ctx_create(int device, uint32_t flags, const phx_tuning* tuning) {
 void phx_ctx_destroy(phx_ctx* c) {
   if (!c) return;
   (void)hipStreamSynchronize(c->stream);
+  if (PlanCache* C = c->plan_cache) {
+    for (DevBuf* b : {&C->seq, &C->pairs, &C->out, &C->runs, &C->work, &C->work2, &C->ctr, &C->scratch,
+                      &C->gather, &C->gidx, &C->arena})
+      dev_free(*b);
+    if (C->ctr_host) (void)hipHostFree(C->ctr_host);
+    for (hipEvent_t e : C->kev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {C->ev_start, C->ev_end})
+      if (e) (void)hipEventDestroy(e);
+    delete C;
+  }
   for (hipEvent_t e : c->join) (void)hipEventDestroy(e);
   for (int i = 0; i < kStreams; ++i) (void)hipStreamDestroy(c->streams[i]);
   delete c;
@@ -1052,7 +1157,8 @@ namespace {
 
 // phx_plan_create, optionally on a fixed device region (fixed_base /
 // fixed_cap bytes) from which every device buffer is carved.
-phx_plan* plan_create_batch(phx_ctx* ctx, const phx_batch* b, uint8_t* fixed_base, uint64_t fixed_cap) {
+phx_plan* plan_create_batch(phx_ctx* ctx, const phx_batch* b, uint8_t* fixed_base, uint64_t fixed_cap,
+                            bool oneshot = false) {
   if (!ctx || !b || b->engine < 0 || b->engine > PHX_ENGINE_MYERS_DIST) {
     set_error(ctx, "bad batch");
     return nullptr;
@@ -1063,6 +1169,7 @@ phx_plan* plan_create_batch(phx_ctx* ctx, const phx_batch* b, uint8_t* fixed_bas
   P->ctx = ctx;
   P->fixed_base = fixed_base;
   P->fixed_cap = fixed_base ? fixed_cap : 0;
+  if (oneshot && !fixed_base) cache_take(P);
   P->tun = ctx->tun;
   P->engine = b->engine;
   P->costs = b->costs;
@@ -1089,11 +1196,11 @@ phx_plan* plan_create_batch(phx_ctx* ctx, const phx_batch* b, uint8_t* fixed_bas
   P->tlen.assign(n, 0);
   P->host_status.assign(n, ST_PENDING);
   P->pairs.assign(n, PairDesc{});
-  // Pass 1: seeds, lengths, encodings, word and run offsets.
-  std::vector<uint8_t> b8(n, 0);
+  // Pass 1 (host threads): seeds, lengths, encodings; then word and run
+  // offsets in one serial prefix pass; pass 2 (host threads): pack.
+  std::vector<uint8_t> b8(n, 0), too_long(n, 0);
   std::vector<uint64_t> q_word(n, 0), t_word(n, 0);
-  uint64_t words = 0, runs = 0;
-  for (uint32_t i = 0; i < n; ++i) {
+  parallel_for(n, [&](size_t i) {
     const uint64_t qfull = b->q_off[i + 1] - b->q_off[i], tfull = b->t_off[i + 1] - b->t_off[i];
     uint64_t sq = 0, st = 0;
     if (b->engine != PHX_ENGINE_MYERS_GLOBAL) {
@@ -1104,19 +1211,26 @@ phx_plan* plan_create_batch(phx_ctx* ctx, const phx_batch* b, uint8_t* fixed_bas
     P->seed_t[i] = st;
     if (sq > qfull || st > tfull) {
       P->host_status[i] = ST_INVALID_INPUT;  // lib.rs:107-111
-      continue;
+      return;
     }
     const uint64_t qn = qfull - sq, tn = tfull - st;
     if (qn > (uint64_t)kMaxLen || tn > (uint64_t)kMaxLen) {
+      too_long[i] = 1;
+      return;
+    }
+    P->qlen[i] = qn;
+    P->tlen[i] = tn;
+    b8[i] = !(is_acgt(b->q_seq + b->q_off[i] + sq, qn) && is_acgt(b->t_seq + b->t_off[i] + st, tn));
+  });
+  uint64_t words = 0, runs = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (too_long[i]) {
       set_error(ctx, "sequence longer than the device format supports (2^27 - 2)");
       delete P;
       return nullptr;
     }
-    P->qlen[i] = qn;
-    P->tlen[i] = tn;
-    const uint8_t* qs = b->q_seq + b->q_off[i] + sq;
-    const uint8_t* ts = b->t_seq + b->t_off[i] + st;
-    b8[i] = !(is_acgt(qs, qn) && is_acgt(ts, tn));
+    if (P->host_status[i] == ST_INVALID_INPUT) continue;
+    const uint64_t qn = P->qlen[i], tn = P->tlen[i];
     PairDesc& pd = P->pairs[i];
     pd.qn = (int32_t)qn;
     pd.tn = (int32_t)tn;
@@ -1132,13 +1246,12 @@ phx_plan* plan_create_batch(phx_ctx* ctx, const phx_batch* b, uint8_t* fixed_bas
   }
   P->seq_words = words;
   P->runs_total = runs;
-  // Pass 2: pack.
   std::vector<uint64_t> packed(words ? words : 1);
-  for (uint32_t i = 0; i < n; ++i) {
-    if (P->host_status[i] == ST_INVALID_INPUT) continue;
+  parallel_for(n, [&](size_t i) {
+    if (P->host_status[i] == ST_INVALID_INPUT) return;
     pack_into(packed.data() + q_word[i], b->q_seq + b->q_off[i] + P->seed_q[i], P->qlen[i], b8[i]);
     pack_into(packed.data() + t_word[i], b->t_seq + b->t_off[i] + P->seed_t[i], P->tlen[i], b8[i]);
-  }
+  });
   std::vector<const uint8_t*> qh(n, nullptr), th(n, nullptr);
   for (uint32_t i = 0; i < n; ++i) {
     if (P->host_status[i] != ST_PENDING) continue;
@@ -1161,6 +1274,9 @@ int phx_plan_run(phx_plan* P) {
   HIPCHK(hipSetDevice(ctx->device));
   P->out_valid = false;
   P->last_launches = 0;
+  P->launch_rec.clear();
+  P->rerun_lists.clear();
+  P->rerun_lists.reserve(8);  // records point into it: no reallocation within a run
   P->start_ev = nullptr;
   P->end_ev = nullptr;
   P->ms_ready = P->kernel_ms_ready = false;
@@ -1222,6 +1338,38 @@ double phx_plan_last_kernel_ms(const phx_plan* P) {
   return P->last_kernel_ms;
 }
 uint32_t phx_plan_last_launches(const phx_plan* P) { return P ? P->last_launches : 0; }
+
+static int fetch_out(phx_plan* P);
+
+int phx_plan_last_launch(const phx_plan* Pc, uint32_t i, phx_launch_info* out) {
+  phx_plan* P = const_cast<phx_plan*>(Pc);
+  if (!P || !out || i >= P->launch_rec.size()) return PHX_E_ARG;
+  std::lock_guard<std::mutex> lk(P->ctx->mu);
+  if (fetch_out(P) != PHX_OK) return PHX_E_DEVICE;
+  const phx_plan::LaunchRec& r = P->launch_rec[i];
+  std::memset(out, 0, sizeof(*out));
+  out->tier = r.tier;
+  out->pairs = (uint32_t)r.count;
+  out->grid = r.grid;
+  if (i < P->last_launches && 2 * (size_t)i + 1 < P->kev.size()) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, P->kev[2 * i], P->kev[2 * i + 1]) == hipSuccess) out->ms = ms;
+  }
+  uint64_t bytes = 0, cells = 0, steps = 0;
+  for (uint64_t j = r.off; j < r.off + r.count; ++j) {
+    const uint32_t pi = (*r.list)[j];
+    const PairDesc& pd = P->pairs[pi];
+    const bool b8 = pd.flags & PF_BYTES;
+    bytes += b8 ? (uint64_t)pd.qn + pd.tn : ((uint64_t)pd.qn + 3) / 4 + ((uint64_t)pd.tn + 3) / 4;
+    bytes += P->engine == PHX_ENGINE_MYERS_DIST ? 16 : 12 + 4ull * (uint64_t)std::max(P->out_h[pi].nruns, 0);
+    cells += (uint64_t)std::max(P->out_h[pi].cells, 0);
+    steps += (uint64_t)std::max(P->out_h[pi].steps, 0);
+  }
+  out->algorithmic_bytes = bytes;
+  out->cells = cells;
+  out->steps = steps;
+  return PHX_OK;
+}
 
 int phx_plan_set_tuning(phx_plan* P, const phx_tuning* t) {
   if (!P || !t) return PHX_E_ARG;
@@ -1437,6 +1585,11 @@ int phx_plan_results(phx_plan* P, phx_results* r) {
 void phx_plan_destroy(phx_plan* P) {
   if (!P) return;
   (void)hipStreamSynchronize(P->ctx->stream);
+  if (P->gexec) {
+    (void)hipGraphExecDestroy(P->gexec);
+    P->gexec = nullptr;
+  }
+  if (P->cached) cache_return(P);  // leaves the plan with nothing to free
   if (!P->fixed()) {  // a fixed plan's buffers are carves of the scratch array
     for (DevBuf* b : {&P->d_seq, &P->d_pairs, &P->d_out, &P->d_runs, &P->d_work, &P->d_work2, &P->d_ctr,
                       &P->d_scratch, &P->d_gather, &P->d_gidx})
@@ -1453,7 +1606,7 @@ void phx_plan_destroy(phx_plan* P) {
 
 int phx_align_batch(phx_ctx* ctx, const phx_batch* b, phx_results* out) {
   if (!ctx || !b || !out) return PHX_E_ARG;
-  phx_plan* P = phx_plan_create(ctx, b);
+  phx_plan* P = plan_create_batch(ctx, b, nullptr, 0, true);
   if (!P) return last_error_code(ctx);
   int rc = phx_plan_run(P);
   if (rc == PHX_OK) rc = phx_plan_results(P, out);
@@ -1488,6 +1641,33 @@ size_t phx_format_cigar(const uint32_t* runs, uint64_t nruns, char* buf, size_t 
     }
   }
   if (buf && cap) buf[std::min(len, cap - 1)] = 0;
+  return len;
+}
+
+uint64_t phx_format_cigars(const uint32_t* runs, const uint64_t* run_off, uint64_t n, char* buf,
+                           uint64_t cap, uint64_t* text_off) {
+  static const char kOps[4] = {'M', 'X', 'I', 'D'};
+  if (!run_off || (!runs && n && run_off[n] > run_off[0])) return 0;
+  uint64_t len = 0;
+  char digits[12];
+  for (uint64_t i = 0; i < n; ++i) {
+    if (text_off) text_off[i] = len;
+    for (uint64_t r = run_off[i]; r < run_off[i + 1]; ++r) {
+      uint32_t v = runs[r] >> 4;
+      int nd = 0;
+      do {
+        digits[nd++] = (char)('0' + v % 10);
+        v /= 10;
+      } while (v);
+      if (buf && len + (uint64_t)nd + 1 <= cap) {
+        for (int d = nd - 1; d >= 0; --d) buf[len++] = digits[d];
+        buf[len++] = kOps[runs[r] & 3];
+      } else {
+        len += (uint64_t)nd + 1;
+      }
+    }
+  }
+  if (text_off) text_off[n] = len;
   return len;
 }
 

@@ -12,7 +12,9 @@
 #include <atomic>
 #include <cmath>
 #include <cstdlib>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -245,21 +247,78 @@ inline int32_t estimate_steps_seq(int engine, const phx_costs& c, const uint8_t*
   return (int32_t)std::min(1.1 * steps + 32.0, (double)(qn + tn + 16));
 }
 
-// f(i) for i in [0, n) on up to 16 host threads.
+// Persistent host worker threads (up to 15, plus the calling thread) for
+// parallel_for: spawning threads per call would cost as much as packing a
+// 10,000-pair batch.  One job at a time; a call that finds the pool busy
+// (another host thread's job, or a nested call) runs on its own thread.
+class HostPool {
+ public:
+  static HostPool& get() {
+    static HostPool* p = new HostPool();  // lives until the process ends
+    return *p;
+  }
+  unsigned workers() const { return (unsigned)threads_.size(); }
+  // Runs f(i) for i in [0, n) on the workers and the caller; false (nothing
+  // run) when the pool is busy.
+  bool run(size_t n, const std::function<void(size_t)>& f) {
+    std::unique_lock<std::mutex> busy(busy_, std::try_to_lock);
+    if (!busy.owns_lock()) return false;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      job_ = &f;
+      n_ = n;
+      next_.store(0);
+      pending_ = (unsigned)threads_.size();
+      ++gen_;
+    }
+    cv_.notify_all();
+    drain();
+    std::unique_lock<std::mutex> lk(mu_);
+    done_.wait(lk, [&] { return pending_ == 0; });
+    job_ = nullptr;
+    return true;
+  }
+
+ private:
+  HostPool() {
+    const unsigned T = std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
+    for (unsigned k = 1; k < T; ++k) threads_.emplace_back([this] { loop(); });
+    for (auto& t : threads_) t.detach();
+  }
+  void drain() {
+    for (size_t i; (i = next_.fetch_add(1)) < n_;) (*job_)(i);
+  }
+  void loop() {
+    uint64_t seen = 0;
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return gen_ != seen; });
+        seen = gen_;
+      }
+      drain();
+      std::lock_guard<std::mutex> lk(mu_);
+      if (--pending_ == 0) done_.notify_all();
+    }
+  }
+  std::mutex busy_, mu_;
+  std::condition_variable cv_, done_;
+  std::vector<std::thread> threads_;
+  const std::function<void(size_t)>* job_ = nullptr;
+  size_t n_ = 0;
+  std::atomic<size_t> next_{0};
+  unsigned pending_ = 0;
+  uint64_t gen_ = 0;
+};
+
+// f(i) for i in [0, n) on the host pool (up to 16 threads).
 template <class F>
 void parallel_for(size_t n, F f) {
-  const unsigned T = std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
-  if (n < 32 || T == 1) {
-    for (size_t i = 0; i < n; ++i) f(i);
-    return;
+  if (n >= 32) {
+    HostPool& pool = HostPool::get();
+    if (pool.workers() > 0 && pool.run(n, std::function<void(size_t)>(f))) return;
   }
-  std::atomic<size_t> next{0};
-  std::vector<std::thread> pool;
-  for (unsigned k = 0; k < std::min<size_t>(T, n); ++k)
-    pool.emplace_back([&] {
-      for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
-    });
-  for (auto& th : pool) th.join();
+  for (size_t i = 0; i < n; ++i) f(i);
 }
 
 // Ring words of one HBM slot (`frames` from ring_frames) for pairs with
@@ -354,11 +413,14 @@ constexpr uint32_t kBlockLdsMax = 64u * 1024u;  // dynamic LDS a block-tier work
 
 // A batch plan over windows of resident sequences: the descriptors carry
 // device pointers, shifts, lengths and flags; qh / th are host copies of each
-// window (memory estimates).  Run and read it with phx_plan_run /
-// phx_plan_results.  Defined in phx_api.hip.
+// window (memory estimates of WFA pairs beyond the LDS tier).  Any engine;
+// Myers pairs need word-aligned queries (q_shift 0).  oneshot: the plan
+// borrows the context's cached buffers (destroy it before the next one).
+// Run and read it with phx_plan_run / phx_plan_results.  Defined in
+// phx_api.hip.
 phx_plan* plan_create_windows(phx_ctx* ctx, int engine, phx_costs costs, int64_t cap,
                               std::vector<PairDesc> pairs, const std::vector<const uint8_t*>& qh,
-                              const std::vector<const uint8_t*>& th);
+                              const std::vector<const uint8_t*>& th, bool oneshot = false);
 
 // Tier of an HBM pair by qn + tn: a wave up to 8192, then workgroups.
 inline int hbm_tier_for(uint64_t L) {
@@ -370,6 +432,8 @@ inline int hbm_tier_for(uint64_t L) {
 
 constexpr int kStreams = 4;  // launch groups of one run overlap on these
 
+struct PlanCache;  // device buffers kept between one-shot calls (phx_api.hip)
+
 struct phx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;         // = streams[0]
@@ -379,6 +443,8 @@ struct phx_ctx {
   phx_tuning tun{};
   int cu_count = 256;
   uint8_t* noalloc_scratch = nullptr;   // device address of this device's allocation-free scratch
+  PlanCache* plan_cache = nullptr;      // buffers of one-shot plans, reused by the next one
+  std::mutex cache_mu;
 };
 
 namespace phx {

@@ -52,3 +52,25 @@ def test_cigar_bound_counts_worst_case_runs():
     assert lib.phx_cigar_bound(ctypes.byref(cb)) == (4 + 3 + 1) + (0 + 2 + 1) + (2 + 0 + 1)
     cb = _cbatch(pb, L.ENGINE_MYERS_DIST, AffineCosts(), None)
     assert lib.phx_cigar_bound(ctypes.byref(cb)) == 0
+
+
+def test_format_cigars_batch_matches_per_pair():
+    lib = L.lib()
+    rng = np.random.default_rng(3)
+    runs_l, off = [], [0]
+    for _ in range(50):
+        k = int(rng.integers(0, 6))
+        for _ in range(k):
+            runs_l.append((int(rng.integers(1, 100000)) << 4) | int(rng.integers(0, 4)))
+        off.append(len(runs_l))
+    runs = np.array(runs_l or [0], np.uint32)
+    run_off = np.array(off, np.uint64)
+    text_off = np.zeros(51, np.uint64)
+    total = lib.phx_format_cigars(runs.ctypes.data, run_off.ctypes.data, 50, None, 0, text_off.ctypes.data)
+    buf = ctypes.create_string_buffer(int(total))
+    assert lib.phx_format_cigars(runs.ctypes.data, run_off.ctypes.data, 50, buf, total,
+                                 text_off.ctypes.data) == total
+    text = buf.raw.decode()
+    for i in range(50):
+        want = format_cigar(runs[off[i]:off[i + 1]])
+        assert text[int(text_off[i]):int(text_off[i + 1])] == want
