@@ -263,7 +263,11 @@ def drop_in_block(w, reps: int = 10):
     for i in range(n):
         strands[i] = L.CReadStrands(w.q_seq.ctypes.data + int(w.q_off[i]), int(w.q_off[i + 1] - w.q_off[i]),
                                     starts.ctypes.data + 8 * i, 1, None, 0, 1)
-    for name, strategy in (("sensitive_affine", px.STRATEGY_SENSITIVE), ("balanced_myers", px.STRATEGY_BALANCED)):
+    ref = px.Reference(g)  # created once per target, like the caller's TargetContext
+    for name, strategy, resident in (("sensitive_affine", px.STRATEGY_SENSITIVE, False),
+                                     ("balanced_myers", px.STRATEGY_BALANCED, False),
+                                     ("sensitive_affine_resident", px.STRATEGY_SENSITIVE, True),
+                                     ("balanced_myers_resident", px.STRATEGY_BALANCED, True)):
         ne, na, nr = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
         assert lib.phx_align_reads_bounds(strategy, 1, len(g), strands, n, ctypes.byref(ne),
                                           ctypes.byref(na), ctypes.byref(nr)) == 0
@@ -277,18 +281,27 @@ def drop_in_block(w, reps: int = 10):
         t_ms = []
         for r in range(reps + 1):
             t0 = time.perf_counter()
-            rc = lib.phx_align_reads_batch(ctx.handle, strategy, 0, 1, g.ctypes.data, len(g), strands, n,
-                                           place, ext, E, a_aln.ctypes.data, role.ctypes.data, A,
-                                           a_runs.ctypes.data, R, a_off.ctypes.data)
+            if resident:
+                rc = lib.phx_align_reads_batch_ref(ctx.handle, strategy, 0, 1, ref._h, strands, n, place, ext,
+                                                   E, a_aln.ctypes.data, role.ctypes.data, A,
+                                                   a_runs.ctypes.data, R, a_off.ctypes.data)
+            else:
+                rc = lib.phx_align_reads_batch(ctx.handle, strategy, 0, 1, g.ctypes.data, len(g), strands,
+                                               n, place, ext, E, a_aln.ctypes.data, role.ctypes.data, A,
+                                               a_runs.ctypes.data, R, a_off.ctypes.data)
             if r:
                 t_ms.append((time.perf_counter() - t0) * 1000.0)
             assert rc == 0
         placed = sum(1 for i in range(n) if place[i].outcome == px.OUTCOME_PLACED)
         m = float(np.median(t_ms))
-        out[f"align_reads_{name}"] = {"entry": "phx_align_reads_batch", "reads": n, "ms_per_call_p50": m,
+        out[f"align_reads_{name}"] = {"entry": "phx_align_reads_batch" + ("_ref" if resident else ""),
+                                      "reads": n, "ms_per_call_p50": m,
                                       "ms_per_call_p99": float(np.percentile(t_ms, 99)),
                                       "reads_per_s": n / (m / 1000.0), "placed": placed,
-                                      "what": "one forward chain per read, window target[pos .. pos + 300]"}
+                                      "what": "one forward chain per read, window target[pos .. pos + 300]" +
+                                              ("; target resident in HBM (phx_reference)" if resident else
+                                               "; target passed as host bytes")}
+    ref.close()
     return out
 
 

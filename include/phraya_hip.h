@@ -409,6 +409,16 @@ typedef struct phx_read_extension {
   uint64_t first_anchor;
 } phx_read_extension;
 
+/* A target kept resident in HBM for the executor entry points below (the
+ * `_ref` forms): packed once, so a call uploads only its reads and each
+ * anchor's window target[pos .. min(pos + 2 qlen, |target|)] is a descriptor
+ * into it.  phraya's TargetContext (executor.rs) would hold one per target;
+ * the reference API has no counterpart (it takes the target slice per call).
+ * The bytes are copied; NULL on failure (phx_ctx_last_error says why). */
+typedef struct phx_reference phx_reference;
+phx_reference* phx_reference_create(phx_ctx* ctx, const uint8_t* target, uint64_t len);
+void phx_reference_destroy(phx_reference* ref);
+
 /* chain_cap (executor.rs:330-336): anchors extended per read. */
 uint32_t phx_chain_cap(int32_t strategy);
 /* Anchors of a batch and an upper bound of their CIGAR runs (host only). */
@@ -427,6 +437,13 @@ int phx_extend_chains_batch(phx_ctx* ctx, int32_t strategy, int32_t gap_model,
                             phx_read_extension* out, phx_alignment* anchor_aln,
                             uint8_t* anchor_role, uint64_t anchors_cap, uint32_t* runs,
                             uint64_t runs_cap, uint64_t* run_off);
+/* phx_extend_chains_batch against a resident reference (same results). */
+int phx_extend_chains_batch_ref(phx_ctx* ctx, int32_t strategy, int32_t gap_model,
+                                const phx_reference* ref, const phx_read_chains* reads,
+                                uint32_t n_reads, phx_read_extension* out,
+                                phx_alignment* anchor_aln, uint8_t* anchor_role,
+                                uint64_t anchors_cap, uint32_t* runs, uint64_t runs_cap,
+                                uint64_t* run_off);
 /* score_alignments (lib.rs:170-199) over n >= 1 successful alignments with
  * their runs: *primary = the first with the smallest edit distance;
  * is_alternative[i] = 1 for every alignment not equal to the primary (all
@@ -557,6 +574,14 @@ int phx_align_reads_batch(phx_ctx* ctx, int32_t strategy, int32_t gap_model, int
                           uint32_t n_reads, phx_read_placement* placement, phx_read_extension* ext,
                           uint64_t ext_cap, phx_alignment* anchor_aln, uint8_t* anchor_role,
                           uint64_t anchors_cap, uint32_t* runs, uint64_t runs_cap, uint64_t* run_off);
+
+/* phx_align_reads_batch against a resident reference (same results). */
+int phx_align_reads_batch_ref(phx_ctx* ctx, int32_t strategy, int32_t gap_model, int32_t is_dna,
+                              const phx_reference* ref, const phx_read_strands* reads,
+                              uint32_t n_reads, phx_read_placement* placement,
+                              phx_read_extension* ext, uint64_t ext_cap, phx_alignment* anchor_aln,
+                              uint8_t* anchor_role, uint64_t anchors_cap, uint32_t* runs,
+                              uint64_t runs_cap, uint64_t* run_off);
 
 /* --- Post-extension: variant sites from CIGAR runs ----------------------------
  *

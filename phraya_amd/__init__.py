@@ -711,14 +711,43 @@ def chain_cap(strategy: int) -> int:
     return int(L.lib().phx_chain_cap(strategy))
 
 
-def extend_chains_batch(strategy: int, gap_model: int, target: bytes,
+class Reference:
+    """A target resident in HBM (phx_reference) for the executor entry points:
+    pass it where they take the target bytes."""
+
+    def __init__(self, target, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        t = np.frombuffer(bytes(target), np.uint8) if not isinstance(target, np.ndarray) \
+            else np.ascontiguousarray(target, np.uint8)
+        self.len = len(t)
+        self._h = L.lib().phx_reference_create(self.ctx.handle, t.ctypes.data if len(t) else None, len(t))
+        if not self._h:
+            raise RuntimeError(f"phx_reference_create failed: {self.ctx.last_error()}")
+        _track(self, 0)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            L.lib().phx_reference_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def extend_chains_batch(strategy: int, gap_model: int, target,
                         reads: Sequence[tuple], ctx: Optional[Context] = None) -> list:
     """extend_chains (executor.rs:528-567) for many reads at once
-    (phx_extend_chains_batch).  reads: (query bytes, chain target_starts in
-    rank order) per read."""
-    ctx = ctx or default_context()
+    (phx_extend_chains_batch, or phx_extend_chains_batch_ref when target is a
+    Reference).  reads: (query bytes, chain target_starts in rank order) per
+    read."""
+    ref = target if isinstance(target, Reference) else None
+    ctx = ref.ctx if ref else (ctx or default_context())
     n = len(reads)
-    t = np.frombuffer(bytes(target), np.uint8)
+    t = np.zeros(0, np.uint8) if ref else np.frombuffer(bytes(target), np.uint8)
+    t_len = ref.len if ref else len(t)
     keep, rc_arr = [], (L.CReadChains * max(n, 1))()
     for i, (q, starts) in enumerate(reads):
         qa = np.frombuffer(bytes(q), np.uint8)
@@ -727,7 +756,7 @@ def extend_chains_batch(strategy: int, gap_model: int, target: bytes,
         rc_arr[i] = L.CReadChains(qa.ctypes.data if len(qa) else None, len(qa),
                                   sa.ctypes.data if len(sa) else None, len(sa))
     na, nr = ctypes.c_uint64(0), ctypes.c_uint64(0)
-    rc = L.lib().phx_extend_chains_bounds(strategy, len(t), rc_arr, n, ctypes.byref(na),
+    rc = L.lib().phx_extend_chains_bounds(strategy, t_len, rc_arr, n, ctypes.byref(na),
                                           ctypes.byref(nr))
     if rc != L.PHX_OK:
         raise RuntimeError(f"phx_extend_chains_bounds: status {rc}")
@@ -737,10 +766,15 @@ def extend_chains_batch(strategy: int, gap_model: int, target: bytes,
     role = np.zeros(max(A, 1), np.uint8)
     runs = np.zeros(R, np.uint32)
     off = np.zeros(A + 1, np.uint64)
-    rc = L.lib().phx_extend_chains_batch(ctx.handle, strategy, gap_model,
-                                         t.ctypes.data if len(t) else None, len(t), rc_arr, n,
-                                         out, aln.ctypes.data, role.ctypes.data, A,
-                                         runs.ctypes.data, R, off.ctypes.data)
+    if ref:
+        rc = L.lib().phx_extend_chains_batch_ref(ctx.handle, strategy, gap_model, ref._h, rc_arr, n,
+                                                 out, aln.ctypes.data, role.ctypes.data, A,
+                                                 runs.ctypes.data, R, off.ctypes.data)
+    else:
+        rc = L.lib().phx_extend_chains_batch(ctx.handle, strategy, gap_model,
+                                             t.ctypes.data if len(t) else None, len(t), rc_arr, n,
+                                             out, aln.ctypes.data, role.ctypes.data, A,
+                                             runs.ctypes.data, R, off.ctypes.data)
     if rc != L.PHX_OK:
         raise RuntimeError(f"phx_extend_chains_batch failed ({rc}): {ctx.last_error()}")
     return _read_extensions(out, n, aln, role, runs, off)
@@ -935,14 +969,17 @@ class ReadPlacement:
     reverse: Optional[ReadExtension]
 
 
-def align_reads_batch(strategy: int, gap_model: int, target: bytes, reads: Sequence[tuple],
+def align_reads_batch(strategy: int, gap_model: int, target, reads: Sequence[tuple],
                       is_dna: bool = True, ctx: Optional[Context] = None) -> list:
-    """phx_align_reads_batch: reads are (query bytes, forward chain
-    target_starts, reverse chain target_starts, had_seeds); every orientation
-    worth extending goes through one batched extend_chains."""
-    ctx = ctx or default_context()
+    """phx_align_reads_batch (phx_align_reads_batch_ref when target is a
+    Reference): reads are (query bytes, forward chain target_starts, reverse
+    chain target_starts, had_seeds); every orientation worth extending goes
+    through one batched extend_chains."""
+    ref = target if isinstance(target, Reference) else None
+    ctx = ref.ctx if ref else (ctx or default_context())
     n = len(reads)
-    t = np.frombuffer(bytes(target), np.uint8)
+    t = np.zeros(0, np.uint8) if ref else np.frombuffer(bytes(target), np.uint8)
+    t_len = ref.len if ref else len(t)
     keep, arr = [], (L.CReadStrands * max(n, 1))()
 
     def u64(x):
@@ -958,7 +995,7 @@ def align_reads_batch(strategy: int, gap_model: int, target: bytes, reads: Seque
         arr[i] = L.CReadStrands(qa.ctypes.data if len(qa) else None, len(qa), fp, fn, rp, rn,
                                 1 if seeds else 0)
     ne, na, nr = ctypes.c_uint64(0), ctypes.c_uint64(0), ctypes.c_uint64(0)
-    rc = L.lib().phx_align_reads_bounds(strategy, int(is_dna), len(t), arr, n, ctypes.byref(ne),
+    rc = L.lib().phx_align_reads_bounds(strategy, int(is_dna), t_len, arr, n, ctypes.byref(ne),
                                         ctypes.byref(na), ctypes.byref(nr))
     if rc != L.PHX_OK:
         raise RuntimeError(f"phx_align_reads_bounds: status {rc}")
@@ -969,10 +1006,15 @@ def align_reads_batch(strategy: int, gap_model: int, target: bytes, reads: Seque
     role = np.zeros(max(A, 1), np.uint8)
     runs = np.zeros(R, np.uint32)
     off = np.zeros(A + 1, np.uint64)
-    rc = L.lib().phx_align_reads_batch(ctx.handle, strategy, gap_model, int(is_dna),
-                                       t.ctypes.data if len(t) else None, len(t), arr, n, place,
-                                       ext, E, aln.ctypes.data, role.ctypes.data, A,
-                                       runs.ctypes.data, R, off.ctypes.data)
+    if ref:
+        rc = L.lib().phx_align_reads_batch_ref(ctx.handle, strategy, gap_model, int(is_dna), ref._h, arr,
+                                               n, place, ext, E, aln.ctypes.data, role.ctypes.data, A,
+                                               runs.ctypes.data, R, off.ctypes.data)
+    else:
+        rc = L.lib().phx_align_reads_batch(ctx.handle, strategy, gap_model, int(is_dna),
+                                           t.ctypes.data if len(t) else None, len(t), arr, n, place,
+                                           ext, E, aln.ctypes.data, role.ctypes.data, A,
+                                           runs.ctypes.data, R, off.ctypes.data)
     if rc != L.PHX_OK:
         raise RuntimeError(f"phx_align_reads_batch failed ({rc}): {ctx.last_error()}")
     exts = _read_extensions(ext, E, aln, role, runs, off)

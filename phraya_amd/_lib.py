@@ -41,6 +41,8 @@ EXPORTS = (
     "phx_contig_plan_layout", "phx_contig_plan_gap_plan", "phx_contig_plan_last_ms",
     "phx_contig_plan_last_gpu_ms", "phx_contig_plan_last_stitch_ms", "phx_contig_plan_destroy",
     "phx_version", "phx_chain_cap", "phx_extend_chains_bounds", "phx_extend_chains_batch",
+    "phx_extend_chains_batch_ref", "phx_reference_create", "phx_reference_destroy",
+    "phx_align_reads_batch_ref",
     "phx_seed_index_create", "phx_seed_occurrence_cap", "phx_seed_index_destroy", "phx_chain_plan_create",
     "phx_chain_plan_run", "phx_chain_plan_results", "phx_chain_plan_last_kernel_ms", "phx_chain_plan_seeds",
     "phx_chain_plan_destroy", "phx_seed_and_chain_batch",
@@ -245,6 +247,9 @@ def lib() -> ctypes.CDLL:
     L.phx_extend_chains_batch.argtypes = [P, ctypes.c_int32, ctypes.c_int32, P, ctypes.c_uint64, P,
                                           ctypes.c_uint32, P, P, P, ctypes.c_uint64, P,
                                           ctypes.c_uint64, P]
+    L.phx_extend_chains_batch_ref.argtypes = [P, ctypes.c_int32, ctypes.c_int32, P, P,
+                                              ctypes.c_uint32, P, P, P, ctypes.c_uint64, P,
+                                              ctypes.c_uint64, P]
     L.phx_score_alignments.argtypes = [P, P, P, ctypes.c_uint32, ctypes.c_uint64, P, P]
     L.phx_seed_index_create.argtypes = [P, P, P, ctypes.c_uint64]
     L.phx_seed_index_create.restype = P
@@ -270,6 +275,12 @@ def lib() -> ctypes.CDLL:
     L.phx_align_reads_batch.argtypes = [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, P,
                                         ctypes.c_uint64, P, ctypes.c_uint32, P, P, ctypes.c_uint64,
                                         P, P, ctypes.c_uint64, P, ctypes.c_uint64, P]
+    L.phx_align_reads_batch_ref.argtypes = [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, P, P,
+                                            ctypes.c_uint32, P, P, ctypes.c_uint64, P, P,
+                                            ctypes.c_uint64, P, ctypes.c_uint64, P]
+    L.phx_reference_create.argtypes = [P, P, ctypes.c_uint64]
+    L.phx_reference_create.restype = P
+    L.phx_reference_destroy.argtypes = [P]
     L.phx_variant_plan_create.argtypes = [P, P]
     L.phx_variant_plan_create.restype = P
     L.phx_variant_plan_run.argtypes = [P]

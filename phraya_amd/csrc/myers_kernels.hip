@@ -135,6 +135,7 @@ __global__ __launch_bounds__(256) void myers_kernel(MyersArgs a) {
   const PairDesc pd = a.pairs[pi];
   const uint64_t* qw = pd.q;
   const uint64_t* tw = pd.t;
+  const uint32_t tsh = pd.t_shift;  // the target may be a window of a resident reference
   const uint32_t m = (uint32_t)pd.qn, n = (uint32_t)pd.tn;
   const bool b8 = pd.flags & PF_BYTES;
   const bool with_cigar = a.engine != ENG_MYERS_DIST;
@@ -193,11 +194,12 @@ __global__ __launch_bounds__(256) void myers_kernel(MyersArgs a) {
     uint64_t tword = 0;
     for (uint32_t j = 0; j < n; ++j) {
       uint32_t tb;
+      const uint32_t jt = j + tsh;
       if (b8) {
-        tb = sym_at(tw, j, true);
+        tb = sym_at(tw, jt, true);
       } else {
-        if ((j & 31) == 0) tword = tw[j >> 5];
-        tb = (uint32_t)(tword >> ((j & 31) * 2)) & 3;
+        if (j == 0 || (jt & 31) == 0) tword = tw[jt >> 5];
+        tb = (uint32_t)(tword >> ((jt & 31) * 2)) & 3;
       }
       int32_t delta;
       if (b8) {
@@ -251,7 +253,7 @@ __global__ __launch_bounds__(256) void myers_kernel(MyersArgs a) {
           left = col_value(pvp, pvp + nb, nb, last_mask, colscore[ti - 2], qi);
         }
         if (cur == left + 1) { rw.emit(OP_I); --ti; continue; }
-        const bool eq = sym_at(qw, qi - 1, b8) == sym_at(tw, ti - 1, b8);
+        const bool eq = sym_at(qw, qi - 1, b8) == sym_at(tw, ti - 1 + tsh, b8);
         rw.emit(eq ? OP_M : OP_X);
         --qi; --ti;
       }

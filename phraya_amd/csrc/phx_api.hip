@@ -81,6 +81,8 @@ struct phx_plan {
   // buffers, pinned flags and events from the context's cache and give them
   // back when destroyed, so back-to-back calls allocate nothing.
   bool cached = false;
+  uint8_t* pin = nullptr;  // the cache's pinned staging while borrowed
+  size_t pin_bytes = 0;
   std::vector<uint32_t> flag_shadow;                   // host copy of the overflow flags (fixed plans)
   bool fixed() const { return fixed_base != nullptr; }
   double share_norm = 1.0;                             // sum of group shares, when above 1
@@ -120,6 +122,8 @@ struct phx_plan {
 
 struct PlanCache {
   bool busy = false;
+  uint8_t* pin = nullptr;  // pinned staging for the plan's uploads and downloads
+  size_t pin_bytes = 0;
   DevBuf seq, pairs, out, runs, work, work2, ctr, scratch, gather, gidx, arena;
   uint32_t* ctr_host = nullptr;
   uint32_t* ctr_dev = nullptr;
@@ -150,6 +154,8 @@ void swap_cache(PlanCache& C, phx_plan* P) {
   std::swap(C.kev, P->kev);
   std::swap(C.ev_start, P->ev_start);
   std::swap(C.ev_end, P->ev_end);
+  std::swap(C.pin, P->pin);
+  std::swap(C.pin_bytes, P->pin_bytes);
 }
 
 // Borrows the context's cached buffers for a one-shot plan (false when
@@ -256,6 +262,38 @@ int clear_flags(phx_plan* P, uint32_t n) {
   for (uint32_t i = 0; i < n; ++i) ((volatile uint32_t*)P->ctr_host)[i] = 0;
   return PHX_OK;
 }
+// Pinned staging of at least `bytes` for a cached plan's copies (grow-only,
+// kept by the context's cache), or null: copies then go through pageable
+// memory.  Staging stays below 256 MiB.
+uint8_t* staging(phx_plan* P, size_t bytes) {
+  if (!P->cached || bytes > kCacheKeepBytes) return nullptr;
+  if (P->pin_bytes < bytes) {
+    if (P->pin) (void)hipHostFree(P->pin);
+    P->pin = nullptr;
+    P->pin_bytes = 0;
+    const size_t want = std::max<size_t>(bytes, 1u << 20);
+    if (hipHostMalloc((void**)&P->pin, want, hipHostMallocDefault) != hipSuccess) {
+      P->pin = nullptr;
+      return nullptr;
+    }
+    P->pin_bytes = want;
+  }
+  return P->pin;
+}
+
+// Device-to-host copy on the context's stream (ordered after the work
+// queued there; its streams do not synchronise with the null stream),
+// through the plan's staging when it has one.
+int d2h(phx_plan* P, void* dst, const void* src, size_t bytes) {
+  phx_ctx* ctx = P->ctx;
+  if (!bytes) return PHX_OK;
+  uint8_t* pin = staging(P, bytes);
+  HIPCHK(hipMemcpyAsync(pin ? pin : dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (pin) std::memcpy(dst, pin, bytes);
+  return PHX_OK;
+}
+
 int read_flags(phx_plan* P, uint32_t n) {
   phx_ctx* ctx = P->ctx;
   if (P->fixed() && n) HIPCHK(hipMemcpy(P->ctr_host, P->ctr_dev, n * 4, hipMemcpyDeviceToHost));
@@ -831,9 +869,8 @@ phx_plan* plan_setup(phx_plan* P, const std::vector<const uint8_t*>& qh,
   order.reserve(n);
   for (uint32_t i = 0; i < n; ++i)
     if (P->host_status[i] == ST_PENDING) order.push_back(i);
-  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) {
-    return P->qlen[a] + P->tlen[a] > P->qlen[c] + P->tlen[c];
-  });
+  // Longest first (qn + tn < 2^28 for every pair that reaches a kernel).
+  radix_sort_desc(order, [&](uint32_t i) { return (uint32_t)(P->qlen[i] + P->tlen[i]); });
   if (is_wfa(P->engine)) {
     auto lds_ok = [&](uint32_t i) {
       return P->qlen[i] + P->tlen[i] <= kWaveMaxLen && P->ring_slots <= kWaveRingMax;
@@ -885,13 +922,37 @@ phx_plan* plan_setup(phx_plan* P, const std::vector<const uint8_t*>& qh,
     P->fixed_transient = (P->fixed_used + 255) & ~(uint64_t)255;
     if (P->fixed_transient >= P->fixed_cap) return fail_plan("transient region");
   }
-  // Descriptors address their sequences by device pointer (packed batches:
-  // into the plan's own arena; window plans come with pointers set).
+  // Descriptors address their sequences by device pointer: into the plan's
+  // own arena at the word given (packed batches; window plans with an arena
+  // for some sequences), else the pointer the descriptor came with.
   if (packed) {
     for (uint32_t i = 0; i < n; ++i) {
-      P->pairs[i].q = (const uint64_t*)P->d_seq.p + (*q_word)[i];
-      P->pairs[i].t = (const uint64_t*)P->d_seq.p + (*t_word)[i];
+      if (q_word && (*q_word)[i] != kNoWord) P->pairs[i].q = (const uint64_t*)P->d_seq.p + (*q_word)[i];
+      if (t_word && (*t_word)[i] != kNoWord) P->pairs[i].t = (const uint64_t*)P->d_seq.p + (*t_word)[i];
     }
+  }
+  // Sequences, descriptors and work lists: through one pinned staging block
+  // (cached plans; async copies on the context's stream, which every run
+  // orders after), else straight from pageable memory.
+  size_t wl_words = 0;
+  for (auto& w : P->work) wl_words += w.size();
+  const size_t seq_b = words * 8, pd_b = sizeof(PairDesc) * (size_t)n, wl_b = wl_words * 4;
+  if (uint8_t* pin = staging(P, seq_b + pd_b + wl_b)) {
+    if (seq_b) std::memcpy(pin, packed->data(), seq_b);
+    if (pd_b) std::memcpy(pin + seq_b, P->pairs.data(), pd_b);
+    uint32_t* wl = (uint32_t*)(pin + seq_b + pd_b);
+    for (auto& w : P->work) wl = std::copy(w.begin(), w.end(), wl);
+    if ((seq_b && hipMemcpyAsync(P->d_seq.p, pin, seq_b, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) ||
+        (pd_b && hipMemcpyAsync(P->d_pairs.p, pin + seq_b, pd_b, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) ||
+        (wl_b && hipMemcpyAsync(P->d_work.p, pin + seq_b + pd_b, wl_b, hipMemcpyHostToDevice, ctx->stream) != hipSuccess))
+      return fail_plan("upload", PHX_E_DEVICE);
+    if (!P->fixed() && !P->ctr_host &&
+        (hipHostMalloc((void**)&P->ctr_host, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
+         hipHostGetDevicePointer((void**)&P->ctr_dev, P->ctr_host, 0) != hipSuccess))
+      return fail_plan("pinned counters");
+    // The staging block is reused by the results: let the uploads finish.
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail_plan("upload", PHX_E_DEVICE);
+    return P;
   }
   if ((words && hipMemcpy(P->d_seq.p, packed->data(), words * 8, hipMemcpyHostToDevice) != hipSuccess) ||
       (n && hipMemcpy(P->d_pairs.p, P->pairs.data(), sizeof(PairDesc) * n, hipMemcpyHostToDevice) != hipSuccess))
@@ -971,7 +1032,9 @@ namespace host {
 
 phx_plan* plan_create_windows(phx_ctx* ctx, int engine, phx_costs costs, int64_t cap,
                               std::vector<PairDesc> pairs, const std::vector<const uint8_t*>& qh,
-                              const std::vector<const uint8_t*>& th, bool oneshot) {
+                              const std::vector<const uint8_t*>& th, bool oneshot,
+                              const std::vector<uint64_t>* arena, const std::vector<uint64_t>* q_word,
+                              const std::vector<uint64_t>* t_word) {
   if (!ctx || engine < PHX_ENGINE_LINEAR || engine > PHX_ENGINE_MYERS_DIST) return nullptr;
   if (engine == PHX_ENGINE_AFFINE &&
       (costs.mismatch < 1 || costs.gap_extend < 1 || costs.gap_open < 0 ||
@@ -1016,7 +1079,7 @@ phx_plan* plan_create_windows(phx_ctx* ctx, int engine, phx_costs costs, int64_t
   }
   P->runs_total = runs;
   P->pairs = std::move(pairs);
-  return plan_setup(P, qh, th, nullptr, nullptr, nullptr);
+  return plan_setup(P, qh, th, arena, q_word, t_word);
 }
 
 }  // namespace host
@@ -1128,6 +1191,7 @@ void phx_ctx_destroy(phx_ctx* c) {
                       &C->gather, &C->gidx, &C->arena})
       dev_free(*b);
     if (C->ctr_host) (void)hipHostFree(C->ctr_host);
+    if (C->pin) (void)hipHostFree(C->pin);
     for (hipEvent_t e : C->kev) (void)hipEventDestroy(e);
     for (hipEvent_t e : {C->ev_start, C->ev_end})
       if (e) (void)hipEventDestroy(e);
@@ -1496,7 +1560,8 @@ static int fetch_out(phx_plan* P) {
   phx_ctx* ctx = P->ctx;
   if (P->out_valid) return PHX_OK;
   P->out_h.resize(P->n);
-  if (P->n) HIPCHK(hipMemcpy(P->out_h.data(), P->d_out.p, sizeof(PairOut) * P->n, hipMemcpyDeviceToHost));
+  const int rc = d2h(P, P->out_h.data(), P->d_out.p, sizeof(PairOut) * P->n);
+  if (rc != PHX_OK) return rc;
   P->out_valid = true;
   return PHX_OK;
 }
@@ -1573,13 +1638,18 @@ int phx_plan_results(phx_plan* P, phx_results* r) {
   rc = plan_transient(P, P->d_gidx, idx.size() * 8);
   if (rc == PHX_OK) rc = plan_transient(P, P->d_gather, pos * 4, idx.size() * 8);
   if (rc != PHX_OK) return rc;
-  HIPCHK(hipMemcpyAsync(P->d_gidx.p, idx.data(), idx.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+  // Staging sized for both directions first, so the download below cannot
+  // reallocate it under the pending upload.
+  if (uint8_t* pin = staging(P, std::max(idx.size() * 8, (size_t)pos * 4))) {
+    std::memcpy(pin, idx.data(), idx.size() * 8);
+    HIPCHK(hipMemcpyAsync(P->d_gidx.p, pin, idx.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+  } else {
+    HIPCHK(hipMemcpyAsync(P->d_gidx.p, idx.data(), idx.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+  }
   const uint64_t* d_src = (const uint64_t*)P->d_gidx.p;
   HIPCHK(launch_gather_runs((const uint32_t*)P->d_runs.p, d_src, d_src + P->n,
                             (uint32_t*)P->d_gather.p, P->n, ctx->stream));
-  HIPCHK(hipMemcpyAsync(r->runs, P->d_gather.p, pos * 4, hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(hipStreamSynchronize(ctx->stream));
-  return PHX_OK;
+  return d2h(P, r->runs, P->d_gather.p, pos * 4);
 }
 
 void phx_plan_destroy(phx_plan* P) {

@@ -127,7 +127,7 @@ struct WaveGeom {
 
 // Kernel arguments of one Myers launch.
 struct MyersArgs {
-  const PairDesc* pairs;  // Myers pairs are word aligned (shifts 0)
+  const PairDesc* pairs;  // Myers queries are word aligned (q_shift 0); targets may be shifted
   const uint32_t* work_list;
   uint32_t nwork;
   PairOut* out;

@@ -65,11 +65,47 @@ inline const uint8_t* acgt_table() {
   return t.data();
 }
 
+// Whether every byte of the 8-byte word x is one of A, C, G, T: a byte is
+// zero after XOR with the letter exactly when it equals it; the classic
+// has-zero-byte test per letter, ORed, must flag all eight bytes.
+inline bool word_acgt(uint64_t x) {
+  constexpr uint64_t L = 0x0101010101010101ull, H = 0x8080808080808080ull;
+  auto zero_bytes = [&](uint64_t v) {  // 0x80 in each byte of v that is zero, exactly
+    return ~(((v & ~H) + ~H) | v) & H;
+  };
+  const uint64_t hit = zero_bytes(x ^ (L * 'A')) | zero_bytes(x ^ (L * 'C')) | zero_bytes(x ^ (L * 'G')) |
+                       zero_bytes(x ^ (L * 'T'));
+  return hit == H;
+}
+
 inline bool is_acgt(const uint8_t* s, size_t n) {
+  size_t i = 0;
+  for (; i + 8 <= n; i += 8) {
+    uint64_t x;
+    std::memcpy(&x, s + i, 8);
+    if (!word_acgt(x)) return false;
+  }
   const uint8_t* code = acgt_table();
-  for (size_t i = 0; i < n; ++i)
+  for (; i < n; ++i)
     if (code[s[i]] == 0xFF) return false;
   return true;
+}
+
+// 2-bit codes of 32 ACGT bytes at s as one word (symbol i in bits 2i..2i+1):
+// for A, C, G, T (0x41, 0x43, 0x47, 0x54), ((c >> 1) ^ (c >> 2)) & 3 is 0, 1,
+// 2, 3; eight codes per 64-bit lane are then gathered to 16 bits.
+inline uint64_t pack32_acgt(const uint8_t* s) {
+  uint64_t out = 0;
+  for (int k = 0; k < 4; ++k) {
+    uint64_t x;
+    std::memcpy(&x, s + 8 * k, 8);
+    uint64_t y = ((x >> 1) ^ (x >> 2)) & 0x0303030303030303ull;  // a code per byte
+    y = (y | (y >> 6)) & 0x000F000F000F000Full;                    // 4 bits per 16
+    y = (y | (y >> 12)) & 0x000000FF000000FFull;                   // 8 bits per 32
+    y = (y | (y >> 24)) & 0xFFFFull;                               // 16 bits
+    out |= y << (16 * k);
+  }
+  return out;
 }
 
 // Words of an n-symbol sequence plus the zero pad word.
@@ -82,8 +118,10 @@ inline void pack_into(uint64_t* w, const uint8_t* s, size_t n, bool b8) {
     std::memcpy(w, s, n);
     return;
   }
+  size_t i = 0;
+  for (; i + 32 <= n; i += 32) w[i >> 5] = pack32_acgt(s + i);
   const uint8_t* code = acgt_table();
-  for (size_t i = 0; i < n; ++i) w[i >> 5] |= (uint64_t)code[s[i]] << ((i & 31) * 2);
+  for (; i < n; ++i) w[i >> 5] |= (uint64_t)code[s[i]] << ((i & 31) * 2);
 }
 
 inline uint32_t next_pow2(uint64_t v) {
@@ -285,8 +323,12 @@ class HostPool {
     for (unsigned k = 1; k < T; ++k) threads_.emplace_back([this] { loop(); });
     for (auto& t : threads_) t.detach();
   }
+  // Indices are claimed in blocks (about eight per thread), so the threads
+  // do not contend on the counter once per index.
   void drain() {
-    for (size_t i; (i = next_.fetch_add(1)) < n_;) (*job_)(i);
+    const size_t block = std::max<size_t>(1, n_ / (8 * (threads_.size() + 1)));
+    for (size_t i; (i = next_.fetch_add(block)) < n_;)
+      for (size_t j = i; j < std::min(n_, i + block); ++j) (*job_)(j);
   }
   void loop() {
     uint64_t seen = 0;
@@ -364,11 +406,40 @@ inline uint32_t lds_tier_waves(uint64_t n, uint32_t wave_slots) {
   return std::max<uint32_t>(1, (slots + kWaveWaves - 1) / kWaveWaves) * kWaveWaves;
 }
 
+// Stable sort of idx by key(i) descending (a 32-bit key): LSD radix passes of
+// 8 bits, skipping the digits every key shares (lengths and step estimates
+// usually need two).
+template <class K>
+void radix_sort_desc(std::vector<uint32_t>& idx, K key) {
+  const size_t n = idx.size();
+  if (n < 2) return;
+  std::vector<uint32_t> keys(n), tmp(n), tk(n);
+  uint32_t all_or = 0, all_and = ~0u;
+  for (size_t i = 0; i < n; ++i) {
+    keys[i] = ~(uint32_t)key(idx[i]);  // ascending on the complement = descending
+    all_or |= keys[i];
+    all_and &= keys[i];
+  }
+  for (int shift = 0; shift < 32; shift += 8) {
+    if ((((all_or ^ all_and) >> shift) & 0xFF) == 0) continue;  // one digit value only
+    size_t count[257] = {0};
+    for (size_t i = 0; i < n; ++i) ++count[((keys[i] >> shift) & 0xFF) + 1];
+    for (int d = 0; d < 256; ++d) count[d + 1] += count[d];
+    for (size_t i = 0; i < n; ++i) {
+      const size_t at = count[(keys[i] >> shift) & 0xFF]++;
+      tmp[at] = idx[i];
+      tk[at] = keys[i];
+    }
+    idx.swap(tmp);
+    keys.swap(tk);
+  }
+}
+
 // Order of an LDS-tier work list for static striding over `waves` waves
 // (wave g takes positions g, g + waves, ...): pairs by estimated cost,
 // largest first, dealt to the waves in alternating direction each round.
 inline void stride_balance(std::vector<uint32_t>& list, const std::vector<int32_t>& est, uint32_t waves) {
-  std::stable_sort(list.begin(), list.end(), [&](uint32_t a, uint32_t b) { return est[a] > est[b]; });
+  radix_sort_desc(list, [&](uint32_t i) { return (uint32_t)std::max(est[i], 0); });
   std::vector<uint32_t> out(list.size());
   const size_t S = waves;
   for (size_t r0 = 0; r0 < list.size(); r0 += S) {
@@ -416,11 +487,18 @@ constexpr uint32_t kBlockLdsMax = 64u * 1024u;  // dynamic LDS a block-tier work
 // window (memory estimates of WFA pairs beyond the LDS tier).  Any engine;
 // Myers pairs need word-aligned queries (q_shift 0).  oneshot: the plan
 // borrows the context's cached buffers (destroy it before the next one).
+// arena: host words the plan uploads as its own sequence buffer; pair i's
+// query / target then starts at word q_word[i] / t_word[i] of it (kNoWord:
+// the descriptor's own pointer).
 // Run and read it with phx_plan_run / phx_plan_results.  Defined in
 // phx_api.hip.
+constexpr uint64_t kNoWord = ~0ull;  // a descriptor that keeps its own device pointer
 phx_plan* plan_create_windows(phx_ctx* ctx, int engine, phx_costs costs, int64_t cap,
                               std::vector<PairDesc> pairs, const std::vector<const uint8_t*>& qh,
-                              const std::vector<const uint8_t*>& th, bool oneshot = false);
+                              const std::vector<const uint8_t*>& th, bool oneshot = false,
+                              const std::vector<uint64_t>* arena = nullptr,
+                              const std::vector<uint64_t>* q_word = nullptr,
+                              const std::vector<uint64_t>* t_word = nullptr);
 
 // Tier of an HBM pair by qn + tn: a wave up to 8192, then workgroups.
 inline int hbm_tier_for(uint64_t L) {

@@ -74,3 +74,13 @@ def test_format_cigars_batch_matches_per_pair():
     for i in range(50):
         want = format_cigar(runs[off[i]:off[i + 1]])
         assert text[int(text_off[i]):int(text_off[i + 1])] == want
+
+
+def test_host_sequence_helpers():
+    """The library's word-at-a-time ACGT test and 2-bit packer against
+    per-byte references (tests/native/host_helpers_test.cpp)."""
+    import subprocess
+    here = os.path.join(ROOT, "tests", "native")
+    subprocess.run(["make", "-s", "-C", here, "host_helpers_test"], check=True)
+    out = subprocess.run([os.path.join(here, "host_helpers_test")], capture_output=True, text=True)
+    assert out.returncode == 0, out.stdout + out.stderr

@@ -79,12 +79,37 @@ def genome():
     return bytes(rng.choice(b"ACGT") for _ in range(60_000))
 
 
+@pytest.fixture(scope="module")
+def genome_bytes():
+    """A target with N runs and lowercase stretches: windows holding them run
+    in the 8-bit encoding (resident references use their 8-bit copy)."""
+    rng = random.Random(78)
+    g = bytearray(rng.choice(b"ACGT") for _ in range(60_000))
+    for _ in range(40):
+        p = rng.randrange(0, len(g) - 30)
+        k = rng.randint(1, 25)
+        g[p:p + k] = b"N" * k
+    for _ in range(20):
+        p = rng.randrange(0, len(g) - 200)
+        g[p:p + 100] = bytes(g[p:p + 100]).lower()
+    return bytes(g)
+
+
+@pytest.mark.parametrize("resident", [False, True], ids=["host_target", "resident_reference"])
 @pytest.mark.parametrize("strategy,gap", [(2, 0), (2, 1), (1, 0), (0, 0)])
-def test_extend_chains_matches_the_executor(genome, strategy, gap):
+def test_extend_chains_matches_the_executor(genome, genome_bytes, strategy, gap, resident):
     import phraya_amd as px
     reads = make_reads(100 + 10 * strategy + gap, genome, 210)
-    say(f"strategy {strategy} gap {gap}: {len(reads)} reads")
-    got = px.extend_chains_batch(strategy, gap, genome, reads)
+    # A third of the batch against the target with N and lowercase bytes.
+    extra = make_reads(900 + 10 * strategy + gap, genome_bytes, 70)
+    say(f"strategy {strategy} gap {gap} resident {resident}: {len(reads)} + {len(extra)} reads")
+    check_batch(px, strategy, gap, genome, reads, resident)
+    check_batch(px, strategy, gap, genome_bytes, extra, resident)
+
+
+def check_batch(px, strategy, gap, genome, reads, resident):
+    target = px.Reference(genome) if resident else genome
+    got = px.extend_chains_batch(strategy, gap, target, reads)
     n_alt = n_none = 0
     for i, ((q, starts), g) in enumerate(zip(reads, got)):
         want = reference(strategy, gap, genome, q, starts)

@@ -160,9 +160,10 @@ def expected(strategy, gap, target, q, fwd, rev, seeds, is_dna):
     return (strand, outcome, kept, todo)
 
 
+@pytest.mark.parametrize("resident", [False, True], ids=["host_target", "resident_reference"])
 @pytest.mark.parametrize("strategy,gap,is_dna", [(SENSITIVE, 0, True), (BALANCED, 0, True),
                                                  (FAST, 0, True), (SENSITIVE, 1, False)])
-def test_random_reads_match_the_executor(px, strategy, gap, is_dna):
+def test_random_reads_match_the_executor(px, strategy, gap, is_dna, resident):
     rng = random.Random(500 + 10 * strategy + gap + int(is_dna))
     G = 20_000
     genome = dna(G, 41)
@@ -181,7 +182,8 @@ def test_random_reads_match_the_executor(px, strategy, gap, is_dna):
         seeds = kind != 4 or i % 14 == 4
         reads.append((q, [max(0, s) for s in fwd], [max(0, s) for s in rev], seeds))
     say(f"strategy {strategy} gap {gap} dna {is_dna}: {len(reads)} reads")
-    got = px.align_reads_batch(strategy, gap, genome, reads, is_dna=is_dna)
+    got = px.align_reads_batch(strategy, gap, px.Reference(genome) if resident else genome, reads,
+                               is_dna=is_dna)
     counts = [0] * 5
     for i, ((q, fwd, rev, seeds), g) in enumerate(zip(reads, got)):
         strand, outcome, kept, todo = expected(strategy, gap, genome, q, fwd, rev, seeds, is_dna)
