@@ -112,7 +112,10 @@ typedef struct phx_tuning {
                                 (power of two; 0 = 1024; 1 = ring always in HBM) */
   uint32_t block_flight;     /* workgroup tiers: pairs in flight per CU, as a multiple of 4 (256-thread)
                                 and 1 (1024-thread); 0 = 1, at most 4 */
-  uint32_t reserved;
+  uint32_t block_regs;       /* workgroup tiers, affine 4/6/2 and 2/4/1 in score steps: keep the
+                                wavefront in registers (0 = pairs whose estimate fits the window,
+                                1 = every pair whose sequences fit LDS, a pair that outgrows the
+                                window being re-run on the ring path, 2 = never) */
 } phx_tuning;
 
 /* Context: owns the stream, device buffers and the submission lock.
@@ -180,7 +183,7 @@ typedef struct phx_launch_info {
   int32_t tier;
   uint32_t pairs;
   uint32_t grid;
-  uint32_t reserved;
+  uint32_t storage; /* workgroup tiers: 0 ring in HBM, 1 ring in LDS, 2 register wavefront */
   double ms;
   uint64_t algorithmic_bytes;
   uint64_t cells;

@@ -265,14 +265,15 @@ class Plan:
 
     def last_launch_info(self) -> list[dict]:
         """Per launch of the last run: tier (0 LDS wave, 1 HBM wave, 2 256-thread
-        and 3 1024-thread workgroup, 8 Myers), grid, pairs, device ms, and the
+        and 3 1024-thread workgroup, 8 Myers), storage of the workgroup tiers
+        (0 HBM ring, 1 LDS ring, 2 register wavefront), grid, pairs, device ms, and the
         algorithmic bytes, cells and steps of its pairs."""
         out = []
         for i in range(self.last_launches()):
             info = L.CLaunchInfo()
             if L.lib().phx_plan_last_launch(self._h, i, ctypes.byref(info)) != L.PHX_OK:
                 break
-            out.append({f: getattr(info, f) for f, _ in L.CLaunchInfo._fields_ if f != "reserved"})
+            out.append({f: getattr(info, f) for f, _ in L.CLaunchInfo._fields_})
         return out
 
     def tuning(self) -> dict:

@@ -74,7 +74,7 @@ assert ALN_DTYPE.itemsize == ctypes.sizeof(CAlignment)
 
 class CLaunchInfo(ctypes.Structure):
     _fields_ = [("tier", ctypes.c_int32), ("pairs", ctypes.c_uint32), ("grid", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32), ("ms", ctypes.c_double),
+                ("storage", ctypes.c_uint32), ("ms", ctypes.c_double),
                 ("algorithmic_bytes", ctypes.c_uint64), ("cells", ctypes.c_uint64),
                 ("steps", ctypes.c_uint64)]
 
@@ -167,7 +167,7 @@ class CTuning(ctypes.Structure):
                 ("hbm_budget_mib", ctypes.c_uint64), ("block256_min_len", ctypes.c_uint32),
                 ("block1024_min_len", ctypes.c_uint32), ("heavy_permille", ctypes.c_uint32),
                 ("block_lds_ring", ctypes.c_uint32), ("block_flight", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32)]
+                ("block_regs", ctypes.c_uint32)]
 
 
 class CServeConfig(ctypes.Structure):

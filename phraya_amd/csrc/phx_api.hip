@@ -56,8 +56,10 @@ struct phx_plan {
   std::vector<int> group_tier;                         // WFA: 0 = LDS tier, 1 = HBM tier
   std::vector<uint64_t> group_slot;                    // WFA: HBM slot bytes of each group
   std::vector<double> group_share;                     // WFA: share of the HBM budget
+  int wide_group = -1;                                 // WFA: the merged 1024-thread register group, if any
   std::vector<double> group_load;                      // WFA: duration model of each group
-  std::vector<uint8_t> group_rl;                       // WFA: block tier with ring + sequences in LDS
+  std::vector<uint8_t> group_rl;                       // WFA block tiers: 0 HBM ring, 1 ring + sequences
+                                                       // in LDS, 2 register wavefront
   std::vector<uint64_t> group_maxsum;                  // WFA: largest qn + tn of each group
   std::vector<uint32_t> group_seqw;                    // WFA: largest staged sequence words of each group
   // hipGraph of a run's launch sequence (WFA): replayed by later runs.
@@ -103,6 +105,7 @@ struct phx_plan {
   struct LaunchRec {
     int32_t tier = 0;
     uint32_t grid = 0;
+    uint32_t storage = 0;  // workgroup tiers: 0 HBM ring, 1 LDS ring, 2 register wavefront
     const std::vector<uint32_t>* list = nullptr;
     uint64_t off = 0, count = 0;
   };
@@ -335,6 +338,7 @@ phx_tuning merge_tuning(phx_tuning d, const phx_tuning* t) {
   if (t->heavy_permille) d.heavy_permille = std::min<uint32_t>(t->heavy_permille, 1000);
   if (t->block_lds_ring) d.block_lds_ring = t->block_lds_ring == 1 ? 1 : next_pow2(t->block_lds_ring);
   if (t->block_flight) d.block_flight = std::min<uint32_t>(t->block_flight, 4);
+  if (t->block_regs) d.block_regs = std::min<uint32_t>(t->block_regs, 2);
   return d;
 }
 bool has_cigar(int engine) { return engine != PHX_ENGINE_MYERS_DIST; }
@@ -392,8 +396,9 @@ enum LaunchTiming { TIME_BOTH, TIME_NONE };
 
 int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint32_t* d_list,
                     uint32_t* d_counters, uint32_t* overflow, const GroupSlots& gs, uint8_t* slice,
-                    hipStream_t st, bool rl = false, LaunchTiming timing = TIME_BOTH,
+                    hipStream_t st, int storage = 0, LaunchTiming timing = TIME_BOTH,
                     uint32_t seq_words = 0) {
+  const bool rl = storage != 0;
   phx_ctx* ctx = P->ctx;
   WfaArgs a{};
   a.pairs = (const PairDesc*)P->d_pairs.p;
@@ -420,25 +425,25 @@ int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint3
     a.ring_stride = (gs.per - gs.hist) / 4;
     a.wcap_log2 = log2u(gs.wcap);
     slots = (uint32_t)gs.n;
-    if (rl) {
-      G = block_lds_geometry(ring_frames(P->engine, P->costs), gs.wcap, seq_words);
-    }
+    if (storage == 1) G = block_lds_geometry(ring_frames(P->engine, P->costs), gs.wcap, seq_words);
+    if (storage == 2) G = block_regs_geometry(t == TIER_BLOCK256 ? 256 : 1024, seq_words);
   }
   phx_plan::LaunchRec rec;
   rec.tier = t;
+  rec.storage = (uint32_t)storage;
   rec.grid = t <= TIER_WAVE ? std::max<uint32_t>(1, (slots + kWaveWaves - 1) / kWaveWaves)
                             : std::max<uint32_t>(1, slots);
   rec.list = &work;
   rec.count = work.size();
   P->launch_rec.push_back(rec);
   if (timing == TIME_NONE) {
-    HIPCHK(launch_tier(t, mode, a, G, slots, st, rl));
+    HIPCHK(launch_tier(t, mode, a, G, slots, st, storage));
     return PHX_OK;
   }
   hipEvent_t e0, e1;
   if (kernel_events(P, P->last_launches, &e0, &e1) != PHX_OK) return PHX_E_DEVICE;
   HIPCHK(hipEventRecord(e0, st));
-  HIPCHK(launch_tier(t, mode, a, G, slots, st, rl));
+  HIPCHK(launch_tier(t, mode, a, G, slots, st, storage));
   HIPCHK(hipEventRecord(e1, st));
   ++P->last_launches;
   return PHX_OK;
@@ -486,6 +491,15 @@ int issue_groups(phx_plan* P, bool capture, std::vector<int>* launched,
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     return P->group_load[a] > P->group_load[b];
   });
+  // The wide register group runs alone on stream 0 and is launched last:
+  // its workgroups (one per CU) take the CUs the other groups leave, and
+  // the rest of them start as the other groups' workgroups finish.
+  const int wide = P->wide_group >= 0 && P->wide_group < ngroups && !P->work[P->wide_group].empty()
+                       ? P->wide_group : -1;
+  if (wide >= 0) {
+    order.erase(std::find(order.begin(), order.end(), wide));
+    order.push_back(wide);
+  }
   // Scratch: one arena (grow-only), a disjoint slice per HBM group.
   std::vector<GroupSlots> gs(ngroups);
   std::vector<uint64_t> slice_off(ngroups, 0);
@@ -517,7 +531,9 @@ int issue_groups(phx_plan* P, bool capture, std::vector<int>* launched,
   for (int gi : order) {
     const auto& wl = P->work[gi];
     if (wl.empty()) continue;
-    const int si = (int)(std::min_element(queued, queued + kStreams) - queued);
+    const int si = gi == wide ? 0
+                 : wide >= 0 ? 1 + (int)(std::min_element(queued + 1, queued + kStreams) - (queued + 1))
+                             : (int)(std::min_element(queued, queued + kStreams) - queued);
     queued[si] += std::max(P->group_load[gi], 1.0);
     hipStream_t st = ctx->streams[si];
     if (si != 0 && !used_stream[si]) {
@@ -526,7 +542,7 @@ int issue_groups(phx_plan* P, bool capture, std::vector<int>* launched,
     }
     int rc = launch_wfa_tier(P, tier[gi], wl, (uint32_t*)P->d_work.p + group_off[gi],
                              d_ctr + 2 * gi, P->ctr_dev + gi, gs[gi],
-                             (uint8_t*)P->d_arena.p + slice_off[gi], st, P->group_rl[gi] != 0,
+                             (uint8_t*)P->d_arena.p + slice_off[gi], st, P->group_rl[gi],
                              timed ? TIME_BOTH : TIME_NONE, P->group_seqw[gi]);
     if (rc != PHX_OK) return rc;
     if (capture) {
@@ -759,19 +775,29 @@ void plan_group(phx_plan* P) {
       // not trail the rest of the launch.
       const int tier = h ? std::min<int>(kBucketTier[b] + 1, TIER_BLOCK1024) : kBucketTier[b];
       // Workgroup tiers: pairs whose estimated wavefront and sequences fit
-      // keep ring and sequences in LDS; the rest use the HBM ring.
-      std::vector<uint32_t> parts[2];  // [0] HBM ring, [1] LDS ring
+      // keep ring and sequences in LDS; cost shapes with a register
+      // wavefront keep it in registers while the range fits its window
+      // (the step estimate runs about 1.25x high; a pair that outgrows the
+      // window is re-run on the ring path); the rest use the HBM ring.
+      std::vector<uint32_t> parts[3];  // [0] HBM ring, [1] LDS ring, [2] registers
       const uint32_t W = P->tun.block_lds_ring;
       const uint64_t ring_bytes = (uint64_t)ring_frames(P->engine, P->costs) * W * 4;
+      const int nt = tier == TIER_BLOCK256 ? 256 : 1024;
+      const bool regs_ok = tier >= TIER_BLOCK256 && P->tun.block_regs != 2 && block_regs_shape(P->engine, P->costs);
+      const WaveGeom RG = block_regs_geometry(nt, 0);
       for (uint32_t i : list) {
-        bool fit = false;
-        if (tier >= TIER_BLOCK256 && W > 1 && ring_bytes + 1024 <= kBlockLdsMax) {
-          const uint64_t width = std::min<uint64_t>(2 * (uint64_t)P->est[i] + 1, P->qlen[i] + P->tlen[i] + 1);
-          fit = width <= W && ring_bytes + 16 + 8ull * lds_seq_words(P->pairs[i]) <= kBlockLdsMax;
-        }
-        parts[fit].push_back(i);
+        int where = 0;
+        const uint64_t width = std::min<uint64_t>(2 * (uint64_t)P->est[i] + 1, P->qlen[i] + P->tlen[i] + 1);
+        const uint64_t seq_b = 16 + 8ull * lds_seq_words(P->pairs[i]);
+        if (regs_ok && (P->tun.block_regs == 1 || (uint64_t)P->est[i] * 4 / 5 < reg_window(nt) / 2) &&
+            RG.per_wave_bytes + seq_b <= kBlockLdsMax)
+          where = 2;
+        else if (tier >= TIER_BLOCK256 && W > 1 && ring_bytes + 1024 <= kBlockLdsMax &&
+                 width <= W && ring_bytes + seq_b <= kBlockLdsMax)
+          where = 1;
+        parts[where].push_back(i);
       }
-      for (int r = 0; r < 2; ++r) {
+      for (int r = 0; r < 3; ++r) {
         if (parts[r].empty()) continue;
         uint64_t mx = 0;
         double v = 0;
@@ -787,9 +813,64 @@ void plan_group(phx_plan* P) {
       }
     }
   }
+  // The register-wavefront groups of the 1024-thread tier become one group,
+  // longest estimate first: its workgroups each fill a CU and take pairs
+  // from one queue, so the longest pairs start first and a CU freed by any
+  // of its pairs goes to the next one (concurrent groups of fixed sizes
+  // leave CUs idle when the smaller ones finish first).  Its history slots
+  // are sized for its largest pair, so it is given its pairs in flight (one
+  // per CU) from the budget before the other groups share the rest.
+  int wide = -1;
+  P->wide_group = -1;
+  {
+    std::vector<uint32_t> merged;
+    uint64_t slot = 0;
+    double v = 0;
+    std::vector<size_t> keep;
+    for (size_t g = 0; g < P->work.size(); ++g) {
+      if (P->group_tier[g] == TIER_BLOCK1024 && P->group_rl[g] == 2) {
+        merged.insert(merged.end(), P->work[g].begin(), P->work[g].end());
+        slot = std::max(slot, P->group_slot[g]);
+        v += vol[g];
+      } else {
+        keep.push_back(g);
+      }
+    }
+    if (!merged.empty()) {
+      std::stable_sort(merged.begin(), merged.end(), [&](uint32_t a, uint32_t b) { return P->est[a] > P->est[b]; });
+      std::vector<std::vector<uint32_t>> work;
+      std::vector<int> tier;
+      std::vector<uint8_t> rl;
+      std::vector<uint64_t> gslot;
+      std::vector<double> gvol;
+      for (size_t g : keep) {
+        work.push_back(std::move(P->work[g]));
+        tier.push_back(P->group_tier[g]);
+        rl.push_back(P->group_rl[g]);
+        gslot.push_back(P->group_slot[g]);
+        gvol.push_back(vol[g]);
+      }
+      wide = P->wide_group = (int)work.size();
+      work.push_back(std::move(merged));
+      tier.push_back(TIER_BLOCK1024);
+      rl.push_back(2);
+      gslot.push_back(slot);
+      gvol.push_back(v);
+      P->work.swap(work);
+      P->group_tier.swap(tier);
+      P->group_rl.swap(rl);
+      P->group_slot.swap(gslot);
+      vol.swap(gvol);
+    }
+  }
   // HBM budget: each group asks for its pairs in flight x (slot + ring);
   // when the asks exceed the budget they are scaled down together.
   const uint64_t total = P->tun.hbm_budget_mib << 20;
+  auto tier_flight = [&](int t) -> uint64_t {
+    return t == TIER_WAVE ? P->tun.hbm_slots
+         : t == TIER_BLOCK256 ? (uint64_t)ctx->cu_count * 4 * P->tun.block_flight
+                              : (uint64_t)ctx->cu_count * P->tun.block_flight;
+  };
   std::vector<double> ask(P->work.size(), 0.0);
   double ask_sum = 0;
   for (size_t g = 1; g < P->work.size(); ++g) {
@@ -797,20 +878,24 @@ void plan_group(phx_plan* P) {
     for (uint32_t i : P->work[g]) max_sum = std::max<uint64_t>(max_sum, P->qlen[i] + P->tlen[i]);
     const uint64_t ring = P->group_rl[g] ? 0 : ring_words_for(ring_frames(P->engine, P->costs), max_sum, nullptr) * 4;
     const int t = P->group_tier[g];
-    const uint64_t flight = t == TIER_WAVE ? P->tun.hbm_slots
-                          : t == TIER_BLOCK256 ? (uint64_t)ctx->cu_count * 4 * P->tun.block_flight
-                                               : (uint64_t)ctx->cu_count * P->tun.block_flight;
+    const uint64_t flight = tier_flight(t);
     ask[g] = (double)std::min<uint64_t>(flight, P->work[g].size()) * (double)(P->group_slot[g] + ring);
     ask_sum += ask[g];
   }
   P->group_share.assign(P->work.size(), 0.0);
   double share_sum = 0;
+  // Over budget: the wide register group first (up to three quarters of
+  // the budget), the others in proportion to their asks.
+  const bool over = !P->fixed() && ask_sum > (double)total;
+  const double wide_share = over && wide >= 0 ? std::min(ask[wide] / (double)total, 0.75) : 0.0;
+  const double rest_sum = over && wide >= 0 ? ask_sum - ask[wide] : ask_sum;
   for (size_t g = 1; g < P->work.size(); ++g) {
     // Fixed plans split their region (sized only after the persistent
     // buffers are carved) in proportion to the asks.
     P->group_share[g] = P->fixed() ? (ask_sum > 0 ? ask[g] / ask_sum : 0.0)
-                      : ask_sum <= (double)total ? ask[g] / (double)total
-                                                 : std::max(0.01, ask[g] / ask_sum);
+                      : !over ? ask[g] / (double)total
+                      : (int)g == wide ? wide_share
+                                       : std::max(0.01, (1.0 - wide_share) * ask[g] / std::max(rest_sum, 1.0));
     share_sum += P->group_share[g];
   }
   // The groups' slices together never exceed the budget.
@@ -823,9 +908,7 @@ void plan_group(phx_plan* P) {
     const int t = P->group_tier[g];
     const double threads = t == TIER_BLOCK1024 ? 1024 : t == TIER_BLOCK256 ? 256 : 64;
     const double flight = t == TIER_LDS ? (double)P->tun.wave_slots
-                        : t == TIER_WAVE ? (double)P->tun.hbm_slots
-                        : t == TIER_BLOCK256 ? ctx->cu_count * 4.0 * P->tun.block_flight
-                                             : (double)ctx->cu_count * P->tun.block_flight;
+                        : (double)tier_flight(t);
     double sum = 0, longest = 0;
     for (uint32_t i : P->work[g]) {
       const double st = t == TIER_LDS ? 32.0 : (double)P->est[i];
@@ -840,7 +923,7 @@ void plan_group(phx_plan* P) {
   P->group_seqw.assign(P->work.size(), 0);
   for (size_t g = 0; g < P->work.size(); ++g) {
     P->group_maxsum[g] = max_sum_of(P, P->work[g]);
-    if (P->group_tier[g] == TIER_LDS || P->group_rl[g]) P->group_seqw[g] = seq_words_of(P, P->work[g]);
+    if (P->group_tier[g] == TIER_LDS || P->group_rl[g] != 0) P->group_seqw[g] = seq_words_of(P, P->work[g]);
   }
 }
 
@@ -1415,6 +1498,7 @@ int phx_plan_last_launch(const phx_plan* Pc, uint32_t i, phx_launch_info* out) {
   out->tier = r.tier;
   out->pairs = (uint32_t)r.count;
   out->grid = r.grid;
+  out->storage = r.storage;
   if (i < P->last_launches && 2 * (size_t)i + 1 < P->kev.size()) {
     float ms = 0;
     if (hipEventElapsedTime(&ms, P->kev[2 * i], P->kev[2 * i + 1]) == hipSuccess) out->ms = ms;

@@ -29,7 +29,7 @@ hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t st);
 hipError_t launch_myers(int nb_template, const MyersArgs& a, hipStream_t st);
 hipError_t launch_wfa_wave(int mode, bool lds, const WfaArgs& a, const WaveGeom& G,
                            uint32_t grid, hipStream_t st);
-hipError_t launch_wfa_block(int mode, int nt, bool rl, const WfaArgs& a, const WaveGeom& G,
+hipError_t launch_wfa_block(int mode, int nt, int storage, const WfaArgs& a, const WaveGeom& G,
                             uint32_t grid, hipStream_t st);
 hipError_t launch_gather_runs(const uint32_t* runs, const uint64_t* src, const uint64_t* off,
                               uint32_t* dst, uint32_t n, hipStream_t st);
@@ -458,15 +458,16 @@ inline uint32_t lds_seq_words(const PairDesc& pd) {
 }
 
 // Launch one tier over a.work_list: `slots` pairs in flight (waves for the
-// wave tiers, workgroups for the block tiers); rl: block tiers keep ring and
-// sequences in LDS (layout in G).
+// wave tiers, workgroups for the block tiers); storage of the block tiers:
+// 0 ring in HBM, 1 ring and sequences in LDS, 2 register wavefront (layout
+// in G).
 inline hipError_t launch_tier(int tier, int mode, const WfaArgs& a, const WaveGeom& G,
-                              uint32_t slots, hipStream_t st, bool rl = false) {
+                              uint32_t slots, hipStream_t st, int storage = 0) {
   if (tier <= TIER_WAVE) {
     const uint32_t wgs = std::max<uint32_t>(1, (slots + kWaveWaves - 1) / kWaveWaves);
     return launch_wfa_wave(mode, tier == TIER_LDS, a, G, wgs, st);
   }
-  return launch_wfa_block(mode, tier == TIER_BLOCK256 ? 256 : 1024, rl, a, G,
+  return launch_wfa_block(mode, tier == TIER_BLOCK256 ? 256 : 1024, storage, a, G,
                           std::max<uint32_t>(1, slots), st);
 }
 
@@ -481,6 +482,32 @@ inline WaveGeom block_lds_geometry(int32_t frames, uint32_t width, uint32_t seq_
   return G;
 }
 constexpr uint32_t kBlockLdsMax = 64u * 1024u;  // dynamic LDS a block-tier workgroup may take
+
+// Register wavefront of the workgroup tiers (storage 2): 16 chunks of 64
+// diagonals per wave, so the 1024-thread tier covers 16,384 diagonals and the
+// 256-thread tier 4,096; chunk-end tables of kRegEdgeSlots steps after the
+// staged sequences.
+constexpr uint32_t kRegEdgeSlots = 6;
+inline uint32_t reg_chunks(int tier_threads) { return tier_threads == 256 ? 4u * 16u : 16u * 16u; }
+inline uint32_t reg_window(int tier_threads) { return 64u * reg_chunks(tier_threads); }
+inline WaveGeom block_regs_geometry(int tier_threads, uint32_t seq_words) {
+  WaveGeom G{};
+  G.ring_off = 0;
+  G.seq_off = 0;
+  G.seq_cap_words = seq_words;
+  G.tab_off = (seq_words * 8u + 15u) & ~15u;
+  // One 16-byte entry per (slot, chunk), plus a zero entry at each end.
+  G.per_wave_bytes = G.tab_off + kRegEdgeSlots * (reg_chunks(tier_threads) + 2u) * 16u;
+  return G;
+}
+// Whether a register-wavefront cost shape (affine) is one the workgroup
+// tiers have (4/6/2 and 2/4/1 in score steps).
+inline bool block_regs_shape(int engine, const phx_costs& c) {
+  if (engine != PHX_ENGINE_AFFINE) return false;
+  const int32_t g = gcd3(c.mismatch, c.gap_open, c.gap_extend);
+  const int32_t xn = c.mismatch / g, oen = (c.gap_open + c.gap_extend) / g, en = c.gap_extend / g;
+  return xn == 2 && en == 1 && (oen == 4 || oen == 5);
+}
 
 // A batch plan over windows of resident sequences: the descriptors carry
 // device pointers, shifts, lengths and flags; qh / th are host copies of each

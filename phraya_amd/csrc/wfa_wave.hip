@@ -604,6 +604,254 @@ __device__ __forceinline__ bool affine_regs_any(const WaveStore<true>& S, int32_
   return true;
 }
 
+// --- register wavefront for workgroup tiers -------------------------------------
+//
+// A workgroup of NT threads (W = NT / 64 waves) keeps a pair's wavefront in
+// registers over a window of C = W * J chunks of 64 diagonals: chunk c is
+// diagonals kb + 64 c .. kb + 64 c + 63, lane L holding kb + 64 c + L, and
+// chunk c belongs to wave c % W as its chunk j = c / W.  Frames as in the
+// LDS tier (P = offset + 1, 0 = no cell); neighbours k - 1 / k + 1 come from
+// the adjacent lanes by DPP, and at a chunk's ends from the neighbouring
+// chunk's end lane, which the other wave published in LDS after the step
+// that computed it (every value a step reads is from an earlier step, and a
+// barrier ends each step).  The step range only grows, so a chunk not yet
+// reached holds no cells and is skipped.  History cells go to the ring
+// path's HBM layout, so the traceback is shared.  A pair whose range would
+// leave the window stops with ST_OVERFLOW_WIDTH and is re-run on the ring
+// path.
+//
+// The step body is written to keep scalar work low (the scalar unit, shared
+// by the CU's waves, issues one instruction per cycle like the vector
+// units): no lane-divergent branch on the common path, table reads without
+// bounds tests, history stores whose disabled lanes the buffer unit drops.
+
+// Published chunk-end values of the last MD + 1 steps, one entry of four
+// ints {m at lane 0, d at lane 0, m at lane 63, i at lane 63} per (slot,
+// chunk) with slot = step % (MD + 1); entries are indexed c + 1 and the
+// entries of c = -1 and c = C stay zero, so the end chunks read "no cell"
+// without a test.  Laid out after the staged sequences.
+constexpr int kBlockEdgeSlots = 6;  // MD + 1 for every shape with a register wavefront
+
+struct BlockEdges {
+  lmem_t<int32_t>* tab;  // [slot][C + 2][4]
+  int32_t C;
+};
+
+// Frames of J chunks, two 16-bit P values per register (P = offset + 1 <=
+// qn + 1 < 65536: the host sends only queries of at most 65,534 bases):
+// the m layer's MD frames in MR registers per chunk (frame d back, d = 1 ..
+// MD, is half (d - 1) % 2 of register (d - 1) / 2, so a step shifts every
+// register by 16 bits), then the gap layers (one frame back, EN = 1) in one
+// register, i low, d high.  Register p of every chunk forms one vector of J
+// elements, element j for chunk j.  The chunk loop is not unrolled (one copy
+// of the step body keeps the kernel inside the instruction cache); its
+// chunk index is wave-uniform, so an element is read and written by a
+// relative register move, with no scratch.
+template <int MD>
+struct BlockShape {
+  static constexpr int MR = (MD + 1) / 2;  // m-layer registers per chunk
+};
+
+template <int MD>
+__device__ __forceinline__ int32_t frame_m(const uint32_t (&m)[(MD + 1) / 2], int d) {
+  const uint32_t r = m[(d - 1) >> 1];
+  return (int32_t)(((d - 1) & 1) ? r >> 16 : r & 0xFFFFu);
+}
+
+template <int MD>
+__device__ __forceinline__ void push_m(uint32_t (&m)[(MD + 1) / 2], uint32_t v) {
+#pragma unroll
+  for (int p = (MD + 1) / 2 - 1; p >= 1; --p) m[p] = __builtin_amdgcn_alignbit(m[p], m[p - 1], 16);
+  m[0] = (m[0] << 16) | v;
+}
+
+// extend_diag over sequences staged in LDS with the first 64-bit window
+// compared by every lane without a branch (a lane with nothing to extend
+// reads word 0 and discards it); only when some lane's match runs past that
+// window does the wave enter extend_diag's loop for the rest.
+template <bool B8>
+__device__ __forceinline__ int32_t extend_first(const lmem_t<const uint64_t>* qw, uint32_t qsh,
+                                                const lmem_t<const uint64_t>* tw, uint32_t tsh, int32_t qn,
+                                                int32_t tn, int32_t p, int32_t k) {
+  constexpr uint32_t BITS = B8 ? 8 : 2;
+  constexpr uint32_t CTZ_SHIFT = B8 ? 3 : 1;
+  constexpr int32_t PER = B8 ? 8 : 32;
+  const int32_t j = p - k;
+  const bool live = (uint32_t)p < (uint32_t)qn && (uint32_t)j < (uint32_t)tn;
+  const int32_t lim = min(qn - p, tn - j);
+  const uint32_t qb = live ? (qsh + (uint32_t)p) * BITS : 0u, tb = live ? (tsh + (uint32_t)j) * BITS : 0u;
+  BitCursor<const lmem_t<const uint64_t>*> qc(qw, qb), tc(tw, tb);
+  const uint32_t z = lowest_bit64(qc.read() ^ tc.read());
+  const int32_t n = (int32_t)(z >> CTZ_SHIFT);
+  const bool more = live && z == 64u && lim > PER;
+  if (__builtin_amdgcn_ballot_w64(more) == 0) return live ? p + min(n, lim) : p;
+  if (more) return extend_diag<B8>(qw, qsh, tw, tsh, qn, tn, p + PER, k);
+  return live ? p + min(n, lim) : p;
+}
+
+template <int NT, int J, int XN, int OEN, int EN, bool B8>
+__device__ void affine_block_regs(const WaveStore<false, true>& S, const BlockEdges& E, int32_t qn,
+                                  int32_t tn, int32_t m0, bool fitting, int32_t max_step, int tid,
+                                  PairShared* sh, bool hist_cap_ok, StepState& st) {
+  constexpr int W = NT / 64, MD = XN > OEN ? XN : OEN, SL = MD + 1;
+  constexpr int MR = BlockShape<MD>::MR;
+  static_assert(EN == 1, "the gap layers keep one frame");
+  using frames_t = uint32_t __attribute__((ext_vector_type(J)));
+  constexpr int32_t NDIAG = 64 * W * J;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int32_t kb = reg_window_base(NDIAG, qn, tn);
+  const int32_t C = E.C, CE = C + 2;  // table entries per slot
+  lmem_t<int32_t>* const tab = E.tab;
+  frames_t FM[MR], FID;
+#pragma unroll
+  for (int p = 0; p < MR; ++p) FM[p] = (frames_t)0u;
+  FID = (frames_t)0u;
+  const int32_t d0 = 0 - kb, c0 = d0 >> 6, l0 = d0 & 63;
+  if (c0 % W == w) {  // step 0 (m0 on diagonal 0), one frame back from step 1
+    const int32_t j0 = __builtin_amdgcn_readfirstlane(c0 / W);
+    FM[0][j0] = lane == l0 ? (uint32_t)(m0 + 1) : 0u;
+  }
+  // The table starts empty, then holds step 0's chunk ends.
+  for (int32_t i = tid; i < SL * CE * 4; i += NT) tab[i] = 0;
+  __syncthreads();
+  if (tid == 0) {
+    if (l0 == 0) tab[(c0 + 1) * 4 + 0] = m0 + 1;
+    if (l0 == 63) tab[(c0 + 1) * 4 + 2] = m0 + 1;
+    sh->key[0] = sh->key[1] = sh->key[2] = INT32_MAX;
+  }
+  __syncthreads();
+  st = StepState{0, 0, false, 0, 0, ST_OK};
+  uint64_t used = 0;
+  bool hist_on = hist_cap_ok;
+  const int32_t kend_g = qn - tn;
+  const uint32_t uq = (uint32_t)qn;
+  const lmem_t<const uint64_t>* qw = S.qw;
+  const lmem_t<const uint64_t>* tw = S.tw;
+  // History stores through a buffer resource over the rest of the slot: a
+  // lane with no cell to store gets an offset past its end, which the
+  // buffer unit drops.
+  constexpr uint32_t kDrop = 0x80000000u;
+  for (;;) {
+    const int32_t s = __builtin_amdgcn_readfirstlane(st.step + 1);
+    const int32_t lo = step_lo(s, tn), hi = step_hi(s, qn);
+    if (lo < kb || hi > kb + NDIAG - 1) {  // the window is too narrow from here on
+      st.status = ST_OVERFLOW_WIDTH;
+      st.step = s;
+      break;
+    }
+    st.step = s;
+    if (s > max_step) { st.status = ST_ALIGNMENT_FAILED; break; }
+    const int32_t width = hi - lo + 1;
+    if (hist_on && !S.fits(s, used, width)) hist_on = false;  // go on to learn the step count
+    if (hist_on && tid == 0) S.toff(s) = (uint32_t)used;
+    if (tid == 0) sh->key[(s + 1) % 3] = INT32_MAX;
+    const uint64_t hb = (uint64_t)(uintptr_t)(S.cells + used);
+    const uint32_t hb_lo = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)hb);
+    const uint32_t hb_hi = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)(hb >> 32));
+    const __amdgpu_buffer_rsrc_t hrs = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(uintptr_t)(((uint64_t)hb_hi << 32) | hb_lo), (short)0, width * 4, 0x00020000);
+    // Table rows of steps s - OEN and s - EN (s >= 1; a step before 0 maps
+    // to a row no step has written yet, which reads as no cell) and of s.
+    const int32_t ro = ((s + SL - OEN) % SL) * CE, re = ((s + SL - EN) % SL) * CE, rw = (s % SL) * CE;
+    // This wave's chunks c = w + W j that the step range reaches.  A wave
+    // with none runs one chunk outside the range (no cell changes: every
+    // lane is out of range), so the loop always runs: a loop that may run
+    // zero times makes the compiler copy every frame vector each step.
+    const int32_t c_lo = (lo - kb) >> 6, c_hi = (hi - kb) >> 6;
+    int32_t j = c_lo <= w ? 0 : (c_lo - w + W - 1) / W;
+    int32_t j_end = c_hi < w ? -1 : (c_hi - w) / W;
+    if (j > j_end) {
+      j = min(j, J - 1);
+      j_end = j;
+    }
+    const int32_t ke_lo = fitting ? max(kend_g, lo) : kend_g;
+    const int32_t ke_hi = fitting ? hi : kend_g;
+    do {
+      const int32_t c = w + W * j;
+      uint32_t m[MR];
+#pragma unroll
+      for (int p = 0; p < MR; ++p) m[p] = FM[p][j];
+      const uint32_t idr = FID[j];
+      const int32_t om = frame_m<MD>(m, OEN);
+      const int32_t ie = (int32_t)(idr & 0xFFFFu), de = (int32_t)(idr >> 16);
+      int32_t vol = from_below(om), vor = from_above(om), vil = from_below(ie), vdr = from_above(de);
+      // Chunk ends: the neighbouring chunks' end lanes, published earlier
+      // (entry c + 1 of the row; c - 1 and c + 1 are entries c and c + 2).
+      const int32_t eb_m = tab[(ro + c) * 4 + 2], ea_m = tab[(ro + c + 2) * 4 + 0];
+      const int32_t eb_i = tab[(re + c) * 4 + 3], ea_d = tab[(re + c + 2) * 4 + 1];
+      vol = lane == 0 ? eb_m : vol;
+      vil = lane == 0 ? eb_i : vil;
+      vor = lane == 63 ? ea_m : vor;
+      vdr = lane == 63 ? ea_d : vdr;
+      const int32_t k = kb + 64 * c + lane;
+      const uint32_t kl = (uint32_t)(k - lo);
+      const bool inr = kl <= (uint32_t)(width - 1);
+      const uint32_t utk = (uint32_t)(tn + k);
+      const int32_t vx = frame_m<MD>(m, XN);
+      int32_t ins = (uint32_t)(vol - 1) < uq ? vol + 1 : 0;
+      uint32_t iop = 0;
+      if ((uint32_t)(vil - 1) < uq && vil + 1 > ins) { ins = vil + 1; iop = 1; }
+      int32_t del = (uint32_t)(vor - 1) <= utk ? vor : 0;
+      uint32_t dop = 0;
+      if ((uint32_t)(vdr - 1) <= utk && vdr > del) { del = vdr; dop = 1; }
+      int32_t best = (uint32_t)(vx - 1) < min(uq, utk) ? vx + 1 : 0;
+      uint32_t mop = 0;
+      if (ins > best) { best = ins; mop = 1; }
+      if (del > best) { best = del; mop = 2; }
+      const int32_t mval = extend_first<B8>(qw, S.qsh, tw, S.tsh, qn, tn, inr ? best - 1 : -1, k);
+      __builtin_amdgcn_raw_buffer_store_b32(((uint32_t)best << 4) | mop | (iop << 2) | (dop << 3), hrs,
+                                            inr && hist_on ? kl * 4u : kDrop, 0, 0);
+      // End test: the chunk holds a diagonal of [ke_lo, ke_hi] (wave-uniform).
+      const int32_t kc = kb + 64 * c;
+      if (kc <= ke_hi && kc + 63 >= ke_lo) {
+        if (inr && k >= ke_lo && k <= ke_hi) {
+          const int32_t layer = mval == qn ? 0 : ins == qn + 1 ? 1 : del == qn + 1 ? 2 : -1;
+          if (layer >= 0) atomicMin(&sh->key[s % 3], ((k + tn) << 2) | layer);
+        }
+      }
+      const int32_t nm = inr ? mval + 1 : 0, ni = inr ? ins : 0, nd = inr ? del : 0;
+      push_m<MD>(m, (uint32_t)nm);
+#pragma unroll
+      for (int p = 0; p < MR; ++p) FM[p][j] = m[p];
+      FID[j] = (uint32_t)ni | ((uint32_t)nd << 16);
+      if (lane == 0 || lane == 63) {
+        const bool top = lane == 63;
+        lmem_t<uint64_t>* dst = (lmem_t<uint64_t>*)(tab + (rw + c + 1) * 4 + (top ? 2 : 0));
+        *dst = (uint64_t)(uint32_t)nm | ((uint64_t)(uint32_t)(top ? ni : nd) << 32);
+      }
+    } while (++j <= j_end);
+    used += (uint64_t)width;
+    lds_barrier();  // this step's chunk ends and end key, for every wave
+    const int32_t key = __builtin_amdgcn_readfirstlane(sh->key[s % 3]);
+    if (key != INT32_MAX) {
+      st.found = true;
+      st.end_k = (key >> 2) - tn;
+      st.end_layer = key & 3;
+      break;
+    }
+  }
+  if (!hist_on && st.status == ST_OK) st.status = ST_OVERFLOW_HIST;
+  st.used = used;
+  __syncthreads();  // every wave's history stores, for the traceback
+}
+
+// Workgroup register wavefront for the cost shapes that have one (shared
+// with the LDS tier); false for others (the ring path runs).
+template <int NT, int J, bool B8>
+__device__ __forceinline__ bool affine_block_regs_any(const WaveStore<false, true>& S, const BlockEdges& E,
+                                                      int32_t qn, int32_t tn, int32_t m0, bool fitting,
+                                                      int32_t max_step, int32_t xn, int32_t oen, int32_t en,
+                                                      int tid, PairShared* sh, StepState& st) {
+  if (xn == 2 && oen == 4 && en == 1)  // 4/6/2
+    affine_block_regs<NT, J, 2, 4, 1, B8>(S, E, qn, tn, m0, fitting, max_step, tid, sh, true, st);
+  else if (xn == 2 && oen == 5 && en == 1)  // 2/4/1
+    affine_block_regs<NT, J, 2, 5, 1, B8>(S, E, qn, tn, m0, fitting, max_step, tid, sh, true, st);
+  else
+    return false;
+  return true;
+}
+
 // Gap-linear steps in registers over a window of CH chunks (one frame
 // back, P = offset + 1 as above); recurrence, bounds and tie rules as in
 // wave_linear's ring loop.
@@ -717,7 +965,10 @@ __device__ __forceinline__ void ring_fill_origin(int32_t* ring, uint32_t W, int3
 
 // --- affine --------------------------------------------------------------------
 
-template <bool LDS, bool B8, int NT, bool RL = false>
+// J > 0 (workgroup tiers, RL layout without a ring): the register
+// wavefront of J chunks per wave (affine_block_regs) instead of the ring loop;
+// the host launches it only for cost shapes that have one.
+template <bool LDS, bool B8, int NT, bool RL = false, int J = 0>
 __device__ __forceinline__ void wave_affine(const WfaArgs& a, const WaveGeom& G, uint8_t* lds_base,
                             const PairDesc& pd, PairOut* po, int lane, PairShared* sh) {
   using S_t = WaveStore<LDS, RL>;
@@ -764,8 +1015,8 @@ __device__ __forceinline__ void wave_affine(const WfaArgs& a, const WaveGeom& G,
     ring_fill_origin<NT>((int32_t*)S.ring, W, Rm + 2 * Rg, M + Rm + 1, koff, qn, tn, M, lane);
     pair_sync<LDS, NT, RL>();
   }
-  // LDS tier with a register wavefront: its layout has no ring.
-  const bool regs = LDS && NT == 64 && reg_wavefront_shape(xn, oen, en);
+  // LDS tier or workgroup tier with a register wavefront: no ring.
+  const bool regs = (LDS && NT == 64 && reg_wavefront_shape(xn, oen, en)) || J > 0;
 
   const int32_t m0 = extend_diag<B8>(S.qw, S.qsh, S.tw, S.tsh, qn, tn, 0, 0);
   if (lane == 0 && !regs) {
@@ -798,7 +1049,25 @@ __device__ __forceinline__ void wave_affine(const WfaArgs& a, const WaveGeom& G,
       }
     }
   }
-  while (!found && status == ST_OK) {
+  if constexpr (J > 0) {
+    if (!found) {
+      constexpr int C = NT / 64 * J;
+      lmem_t<int32_t>* eb = (lmem_t<int32_t>*)(lds_base + G.tab_off);
+      const BlockEdges E{eb, C};
+      StepState st;
+      if (affine_block_regs_any<NT, J, B8>(S, E, qn, tn, m0, fitting, max_step, xn, oen, en, lane, sh, st)) {
+        step = __builtin_amdgcn_readfirstlane(st.step);
+        used = st.used;
+        found = st.found;
+        end_k = st.end_k;
+        end_layer = st.end_layer;
+        status = st.status;
+      } else {
+        status = ST_INTERNAL;
+      }
+    }
+  }
+  while (J == 0 && !found && status == ST_OK) {
     step = __builtin_amdgcn_readfirstlane(step + 1);
     if (step > max_step) { status = ST_ALIGNMENT_FAILED; break; }
     slot = __builtin_amdgcn_readfirstlane(slot + 1 == Rm ? 0 : slot + 1);
@@ -1185,7 +1454,7 @@ __global__ __launch_bounds__(64 * WV_WAVES, 7) void wfa_wave_kernel(WfaArgs a, W
 
 // Workgroup per pair (NT threads) for wide wavefronts; ring and history in
 // the workgroup's HBM slot.
-template <int MODE, int NT, bool RL>
+template <int MODE, int NT, bool RL, int J = 0>
 __global__ __launch_bounds__(NT) void wfa_block_kernel(WfaArgs a, WaveGeom G) {
   __shared__ PairShared sh;
   extern __shared__ __align__(16) uint8_t block_lds[];
@@ -1209,8 +1478,8 @@ __global__ __launch_bounds__(NT) void wfa_block_kernel(WfaArgs a, WaveGeom G) {
     const PairDesc pd = a.pairs[pi];
     PairOut* po = a.out + pi;
     if (MODE == ENG_AFFINE) {
-      if (pd.flags & PF_BYTES) wave_affine<false, true, NT, RL>(a, G, base, pd, po, tid, &sh);
-      else wave_affine<false, false, NT, RL>(a, G, base, pd, po, tid, &sh);
+      if (pd.flags & PF_BYTES) wave_affine<false, true, NT, RL, J>(a, G, base, pd, po, tid, &sh);
+      else wave_affine<false, false, NT, RL, J>(a, G, base, pd, po, tid, &sh);
     } else {
       if (pd.flags & PF_BYTES) wave_linear<false, true, NT, RL>(a, G, base, pd, po, tid, &sh);
       else wave_linear<false, false, NT, RL>(a, G, base, pd, po, tid, &sh);
@@ -1267,12 +1536,21 @@ hipError_t launch_wfa_wave(int mode, bool lds, const WfaArgs& a, const WaveGeom&
   return hipGetLastError();
 }
 
-// nt = threads per pair: 256 or 1024; rl: ring and sequences in LDS (G
-// gives the layout; dynamic LDS = G.per_wave_bytes).
+// nt = threads per pair: 256 or 1024; storage 0: ring in HBM, 1: ring and
+// sequences in LDS, 2: register wavefront, sequences and chunk ends in LDS
+// (G gives the layout; dynamic LDS = G.per_wave_bytes).
 template <int MODE>
-hipError_t launch_block_mode(int nt, bool rl, const WfaArgs& a, const WaveGeom& G, uint32_t grid,
+hipError_t launch_block_mode(int nt, int storage, const WfaArgs& a, const WaveGeom& G, uint32_t grid,
                              hipStream_t st) {
+  const bool rl = storage != 0;
   const size_t lds = rl ? G.per_wave_bytes : 0;
+  if (storage == 2 && MODE == ENG_AFFINE) {
+    // 16 chunks per wave: 16,384 diagonals for the wide tier, 4,096 for the
+    // narrow one.
+    if (nt == 1024) hipLaunchKernelGGL((wfa_block_kernel<MODE, 1024, true, 16>), dim3(grid), dim3(1024), lds, st, a, G);
+    else hipLaunchKernelGGL((wfa_block_kernel<MODE, 256, true, 16>), dim3(grid), dim3(256), lds, st, a, G);
+    return hipGetLastError();
+  }
   if (nt == 1024) {
     if (rl) hipLaunchKernelGGL((wfa_block_kernel<MODE, 1024, true>), dim3(grid), dim3(1024), lds, st, a, G);
     else hipLaunchKernelGGL((wfa_block_kernel<MODE, 1024, false>), dim3(grid), dim3(1024), 0, st, a, G);
@@ -1283,10 +1561,10 @@ hipError_t launch_block_mode(int nt, bool rl, const WfaArgs& a, const WaveGeom& 
   return hipGetLastError();
 }
 
-hipError_t launch_wfa_block(int mode, int nt, bool rl, const WfaArgs& a, const WaveGeom& G,
+hipError_t launch_wfa_block(int mode, int nt, int storage, const WfaArgs& a, const WaveGeom& G,
                             uint32_t grid, hipStream_t st) {
-  return mode == ENG_AFFINE ? launch_block_mode<ENG_AFFINE>(nt, rl, a, G, grid, st)
-                            : launch_block_mode<ENG_LINEAR>(nt, rl, a, G, grid, st);
+  return mode == ENG_AFFINE ? launch_block_mode<ENG_AFFINE>(nt, storage, a, G, grid, st)
+                            : launch_block_mode<ENG_LINEAR>(nt, storage, a, G, grid, st);
 }
 
 }  // namespace phx

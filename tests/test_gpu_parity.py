@@ -81,10 +81,12 @@ def test_gpu_kat(px, k):
 
 # ------------------------------------------------------------------- batch compare
 
-def compare(px, engine, qs, ts, seeds=None, costs=(2, 4, 1), cap=None, ctx=None, label=""):
+def compare(px, engine, qs, ts, seeds=None, costs=(2, 4, 1), cap=None, ctx=None, label="", res=None):
+    """GPU results (res, else one align_batch call) against the oracle."""
     eng_gpu, eng_orc = ENG[engine]
     say(f"compare {engine} {label} n={len(qs)} costs={costs} cap={cap}")
-    res = px.align_batch(eng_gpu, qs, ts, seeds, px.AffineCosts(*costs), cap, ctx)
+    if res is None:
+        res = px.align_batch(eng_gpu, qs, ts, seeds, px.AffineCosts(*costs), cap, ctx)
     pb = px.pack_batch(qs, ts, seeds)
     ores, ocig = O.batch(eng_orc, pb.q_seq, pb.q_off, pb.t_seq, pb.t_off, pb.seed_q, pb.seed_t,
                          costs=costs, cap=-1 if cap is None else cap, threads=8)
@@ -348,6 +350,64 @@ def test_lds_ring_workgroup_tier_and_width_overflow(px):
         for eng in ("affine", "linear"):
             compare(px, eng, qs, ts, costs=(4, 6, 2), ctx=ctx, label=f"lds-ring-{ring}")
         ctx.close()
+
+
+@pytest.mark.parametrize("costs", [(4, 6, 2), (2, 4, 1)])
+def test_workgroup_register_wavefront(px, costs):
+    """Workgroup tiers with the wavefront in registers (16 chunks of 64
+    diagonals per wave, chunk ends exchanged through LDS): pairs of 1-8 kbp
+    on both workgroup tiers, global and fitting, targets much shorter than
+    the query (window placement), byte sequences, 0.5-25% divergence; then
+    with every pair forced onto the register path, pairs whose range
+    outgrows the 4,096-diagonal window of the 256-thread tier stop and are
+    re-run on the ring path.  Every result equals the oracle's, and the
+    launches show the register storage was used (and the re-runs)."""
+    rng = random.Random(costs[0] * 100 + costs[1])
+    qs, ts = [], []
+    for L, rate in ((1100, 0.01), (1500, 0.05), (2000, 0.15), (2600, 0.08), (3300, 0.02),
+                    (4200, 0.01), (5000, 0.08), (6500, 0.03), (8000, 0.005)):
+        for v in range(3):
+            q = rand_seq(rng, L)
+            t = mutate(rng, q, rate)
+            if v == 1:
+                t = t + rand_seq(rng, L // 2 + 40)  # fitting
+            if v == 2:
+                t = t[: len(t) * 3 // 4]  # target shorter than the query
+            qs.append(q)
+            ts.append(t)
+    bq = [q.lower() if i % 4 == 0 else q for i, q in enumerate(qs)]
+    # Wide divergence: the range passes 4,096 diagonals, so the register pass
+    # of the 256-thread tier overflows when every pair is forced onto it.
+    # The first pair has the largest estimate of its length bucket, so it is
+    # the one "heavy" pair moved up to the 1024-thread tier (window 16,384).
+    wq, wt = [], []
+    for L, rate in ((7000, 0.3), (4600, 0.25), (5200, 0.3)):
+        q = rand_seq(rng, L)
+        wq.append(q)
+        wt.append(mutate(rng, q, rate))
+    for tun, pairs, label in (
+            (dict(block256_min_len=1024, block1024_min_len=6000), (qs, ts), "tiers"),
+            (dict(block256_min_len=1024, block1024_min_len=6000), (bq, ts), "bytes"),
+            (dict(block256_min_len=1024, block1024_min_len=20000, block_regs=1, heavy_permille=1),
+             (qs[:6] + wq, ts[:6] + wt), "forced")):
+        ctx = px.Context(0, tun)
+        pb = px.pack_batch(*pairs)
+        plan = px.Plan(pb, px.ENGINE_AFFINE, px.AffineCosts(*costs), ctx=ctx)
+        plan.run()
+        res = plan.results()
+        info = plan.last_launch_info()
+        plan.close()
+        ctx.close()
+        regs = [r for r in info if r["storage"] == 2]
+        say(f"{label}: launches {[(r['tier'], r['storage'], r['pairs']) for r in info]}")
+        assert regs, "no launch used the register wavefront"
+        if label == "tiers":
+            assert {r["tier"] for r in regs} == {2, 3}
+        if label == "forced":
+            # the two wide pairs overflow the register window: one ring re-run
+            # launch takes both (other ring launches here hold one pair)
+            assert max(r["pairs"] for r in info if r["storage"] == 0 and r["tier"] >= 2) >= 2
+        compare(px, "affine", *pairs, costs=costs, label=f"block regs {label}", res=res)
 
 
 @pytest.mark.parametrize("costs", [(4, 6, 2), (2, 4, 1), (1, 0, 1), (3, 5, 2), (1, 30, 1)])
