@@ -72,6 +72,7 @@ struct phx_plan {
   phx_tuning tun{};                                    // this plan's tuning (from the context)
   std::vector<uint32_t> order;                         // pending pairs, longest first
   std::vector<int32_t> est;                            // WFA: estimated score steps per pair
+  std::vector<uint64_t> band_words;                    // WFA: history words within the pair's bound (0 = none)
   // Allocation-free plans: every device buffer is a carve of a fixed region
   // (the device's scratch array) instead of an allocation.  Buffers that live
   // for the whole plan are carved from its start, up to fixed_transient; the
@@ -426,7 +427,10 @@ int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint3
     a.wcap_log2 = log2u(gs.wcap);
     slots = (uint32_t)gs.n;
     if (storage == 1) G = block_lds_geometry(ring_frames(P->engine, P->costs), gs.wcap, seq_words);
-    if (storage == 2) G = block_regs_geometry(t == TIER_BLOCK256 ? 256 : 1024, seq_words);
+    if (storage == 2) {
+      G = block_regs_geometry(t == TIER_BLOCK256 ? 256 : 1024, seq_words);
+      a.band = 1;
+    }
   }
   phx_plan::LaunchRec rec;
   rec.tier = t;
@@ -789,8 +793,11 @@ void plan_group(phx_plan* P) {
         int where = 0;
         const uint64_t width = std::min<uint64_t>(2 * (uint64_t)P->est[i] + 1, P->qlen[i] + P->tlen[i] + 1);
         const uint64_t seq_b = 16 + 8ull * lds_seq_words(P->pairs[i]);
-        if (regs_ok && (P->tun.block_regs == 1 || (uint64_t)P->est[i] * 4 / 5 < reg_window(nt) / 2) &&
-            RG.per_wave_bytes + seq_b <= kBlockLdsMax)
+        // A bounded pair's band spans at most bound + 1 diagonals (en = 1);
+        // an unbounded one spans its whole range, about 2 steps + 1.
+        const bool fits_window = P->pairs[i].bound ? (uint64_t)P->est[i] + 4 <= reg_window(nt)
+                                                   : (uint64_t)P->est[i] * 4 / 5 < reg_window(nt) / 2;
+        if (regs_ok && (P->tun.block_regs == 1 || fits_window) && RG.per_wave_bytes + seq_b <= kBlockLdsMax)
           where = 2;
         else if (tier >= TIER_BLOCK256 && W > 1 && ring_bytes + 1024 <= kBlockLdsMax &&
                  width <= W && ring_bytes + seq_b <= kBlockLdsMax)
@@ -802,8 +809,10 @@ void plan_group(phx_plan* P) {
         uint64_t mx = 0;
         double v = 0;
         for (uint32_t i : parts[r]) {
-          mx = std::max<uint64_t>(mx, words_of(i));
-          v += (double)words_of(i);
+          // Register pairs store only their band's cells.
+          const uint64_t wi = r == 2 && P->pairs[i].bound ? P->band_words[i] : words_of(i);
+          mx = std::max<uint64_t>(mx, wi);
+          v += (double)wi;
         }
         P->work.push_back(parts[r]);
         P->group_tier.push_back(tier);
@@ -961,12 +970,25 @@ phx_plan* plan_setup(phx_plan* P, const std::vector<const uint8_t*>& qh,
     // Step estimates from the sequences (k-mer event rate) for the pairs that
     // need HBM slots, on host threads.
     P->est.assign(n, 0);
+    P->band_words.assign(n, 0);
     std::vector<uint32_t> wide;
     for (uint32_t i : P->order)
       if (!lds_ok(i)) wide.push_back(i);
+    // Cost shapes with a workgroup register wavefront: the estimate is also
+    // the pair's bound for band pruning there (a pair whose optimum exceeds
+    // it is re-run without one).
+    const bool banded = block_regs_shape(P->engine, P->costs);
+    const int32_t en_steps = banded ? P->costs.gap_extend / gcd3(P->costs.mismatch, P->costs.gap_open,
+                                                                 P->costs.gap_extend) : 1;
     parallel_for(wide.size(), [&](size_t k) {
       const uint32_t i = wide[k];
       P->est[i] = estimate_steps_seq(P->engine, P->costs, qh[i], P->qlen[i], th[i], P->tlen[i]);
+      if (banded) {
+        PairDesc& pd = P->pairs[i];
+        const bool fitting = !(pd.flags & PF_GLOBAL) && pd.tn > pd.qn + pd.qn / 2 + 10;
+        pd.bound = (uint32_t)P->est[i];
+        P->band_words[i] = band_hist_words(P->est[i], pd.qn, pd.tn, en_steps, fitting);
+      }
     });
     plan_group(P);
   } else {

@@ -61,7 +61,11 @@ struct PairDesc {
   uint64_t hist_bytes;
   int32_t* ring;
   uint32_t ring_log2;
-  uint32_t reserved;
+  // Upper bound on the pair's optimum in score steps (0 = none).  Launches
+  // with WfaArgs::band set compute only the cells from which the end is
+  // still reachable within it; a pair that needs more stops and is re-run
+  // without it.
+  uint32_t bound;
 };
 
 struct PairOut {
@@ -110,6 +114,7 @@ struct WfaArgs {
   uint64_t ring_stride;
   uint32_t wcap_log2;    // ring width in diagonals = 1 << wcap_log2
   uint32_t static_order; // wave kernels: wave g takes items g, g + waves, ... (no queue)
+  uint32_t band;         // register-wavefront workgroup launches: prune to PairDesc::bound
 };
 
 // LDS layout of one wave of the wave-resident WFA tier (byte offsets inside

@@ -192,6 +192,22 @@ inline uint64_t exact_hist_words(int32_t steps, int32_t qn, int32_t tn) {
   return sum_min(S, (uint64_t)qn) + sum_min(S, (uint64_t)tn) + 2 * S + 16;
 }
 
+// History words of a pair computed within a bound of B score steps (the
+// band of the register-wavefront workgroup tiers): at step s only diagonals
+// within (B - s) / en of kend = qn - tn (fitting: not below kend - (B - s) /
+// en) are stored, plus one offset word per step.
+inline uint64_t band_hist_words(int32_t bound, int32_t qn, int32_t tn, int32_t en, bool fitting) {
+  const int64_t kend = (int64_t)qn - tn;
+  uint64_t sum = 0;
+  for (int64_t s = 1; s <= bound; ++s) {
+    const int64_t r = (bound - s) / en;
+    const int64_t lo = std::max(std::max(-s, -(int64_t)tn), kend - r);
+    const int64_t hi = fitting ? std::min<int64_t>(s, qn) : std::min(std::min<int64_t>(s, qn), kend + r);
+    sum += (uint64_t)std::max<int64_t>(hi - lo + 1, 0) + 1;
+  }
+  return sum + 16;
+}
+
 // Generous estimate of the score steps of a pair, used to size HBM history
 // slots before a run: an end gap for the length difference plus 9% events per
 // base (40% mismatches, 60% indels of mean length 2), +15%.  A pair that needs

@@ -81,6 +81,42 @@ __device__ __forceinline__ int32_t readlane(int32_t v, int32_t lane) {
 __device__ __forceinline__ int32_t step_lo(int32_t u, int32_t tn) { return max(-u, -tn); }
 __device__ __forceinline__ int32_t step_hi(int32_t u, int32_t qn) { return min(u, qn); }
 
+// Diagonals whose cells of step s are computed and stored.  Without a bound
+// (bound = 0): the step's whole range.  With an upper bound B on the
+// optimum (score steps), a cell of step s on diagonal k lies on an
+// alignment of score <= B g only if the gap bases still needed to reach an
+// end diagonal fit: e |k - kend| <= (B - s) g, i.e. |k - kend| <= (B - s) /
+// en (global; fitting ends lie on any diagonal >= kend, so only k < kend
+// is limited).  A cell outside this band is on no optimal alignment, and
+// neither is any cell whose value it would decide: a candidate that attains
+// a cell's maximum, ties included, extends that cell's optimal alignments
+// backwards.  So the cells of every optimal alignment, their values and
+// their tie-breaks are unchanged, and so are the score, the end and the
+// traceback.  If B is below the optimum the band empties before the end is
+// reached (lo > hi), and the pair is re-run without a bound.
+struct Band {
+  int32_t bound;  // 0 = none
+  int32_t kend;   // qn - tn
+  int32_t en;     // gap extend in score steps
+  bool fitting;
+};
+
+__device__ __forceinline__ void band_range(const Band& b, int32_t s, int32_t qn, int32_t tn, int32_t& lo,
+                                           int32_t& hi) {
+  lo = step_lo(s, tn);
+  hi = step_hi(s, qn);
+  if (b.bound > 0) {
+    if (s > b.bound) {
+      lo = 1;
+      hi = 0;
+      return;
+    }
+    const int32_t r = (b.bound - s) / b.en;
+    lo = max(lo, b.kend - r);
+    if (!b.fitting) hi = min(hi, b.kend + r);
+  }
+}
+
 template <bool IN_LDS, class T>
 using mem_t = typename std::conditional<IN_LDS, lmem_t<T>, gmem_t<T>>::type;
 
@@ -241,14 +277,15 @@ __device__ __forceinline__ WaveStore<LDS, RL> make_store(const WfaArgs& a, const
 // Returns the gap bases up to and including the opening cell (64 and
 // *opened = false when the opening lies beyond this window).
 template <class S_t>
-__device__ __forceinline__ int32_t gap_run(const S_t& S, int32_t s, int32_t k,
+__device__ __forceinline__ int32_t gap_run(const S_t& S, const Band& band, int32_t s, int32_t k,
                                            int32_t dir, int32_t en, uint32_t bit, int32_t qn,
                                            int32_t tn, int lane, bool* opened) {
   const int32_t sj = s - lane * en;
   const int32_t kj = k + dir * lane;
   bool open = false;
   if (sj >= 1) {
-    const int32_t lo = step_lo(sj, tn), hi = step_hi(sj, qn);
+    int32_t lo, hi;
+    band_range(band, sj, qn, tn, lo, hi);
     if (kj >= lo && kj <= hi) {
       const uint32_t cell = S.cells[(uint64_t)S.toff(sj) + (uint64_t)(kj - lo)];
       open = ((cell >> bit) & 1u) == 0;
@@ -665,33 +702,37 @@ __device__ __forceinline__ void push_m(uint32_t (&m)[(MD + 1) / 2], uint32_t v) 
   m[0] = (m[0] << 16) | v;
 }
 
-// extend_diag over sequences staged in LDS with the first 64-bit window
-// compared by every lane without a branch (a lane with nothing to extend
-// reads word 0 and discards it); only when some lane's match runs past that
-// window does the wave enter extend_diag's loop for the rest.
+// extend_diag over sequences staged in LDS with a first 32-bit window (16
+// bases, or 4 bytes) compared by every lane without a branch (a lane with
+// nothing to extend reads word 0 and discards it): most cells stop within
+// it, and only when some lane's match runs past it does the wave enter
+// extend_diag's loop for the rest.
 template <bool B8>
 __device__ __forceinline__ int32_t extend_first(const lmem_t<const uint64_t>* qw, uint32_t qsh,
                                                 const lmem_t<const uint64_t>* tw, uint32_t tsh, int32_t qn,
                                                 int32_t tn, int32_t p, int32_t k) {
   constexpr uint32_t BITS = B8 ? 8 : 2;
   constexpr uint32_t CTZ_SHIFT = B8 ? 3 : 1;
-  constexpr int32_t PER = B8 ? 8 : 32;
+  constexpr int32_t PER = B8 ? 4 : 16;
   const int32_t j = p - k;
   const bool live = (uint32_t)p < (uint32_t)qn && (uint32_t)j < (uint32_t)tn;
   const int32_t lim = min(qn - p, tn - j);
   const uint32_t qb = live ? (qsh + (uint32_t)p) * BITS : 0u, tb = live ? (tsh + (uint32_t)j) * BITS : 0u;
-  BitCursor<const lmem_t<const uint64_t>*> qc(qw, qb), tc(tw, tb);
-  const uint32_t z = lowest_bit64(qc.read() ^ tc.read());
+  const lmem_t<const uint32_t>* q32 = (const lmem_t<const uint32_t>*)qw + (qb >> 5);
+  const lmem_t<const uint32_t>* t32 = (const lmem_t<const uint32_t>*)tw + (tb >> 5);
+  const uint32_t qx = __builtin_amdgcn_alignbit(q32[1], q32[0], qb);  // shift = qb & 31
+  const uint32_t tx = __builtin_amdgcn_alignbit(t32[1], t32[0], tb);
+  const uint32_t z = (uint32_t)__builtin_ctzg(qx ^ tx, 32);  // 32 when all match
   const int32_t n = (int32_t)(z >> CTZ_SHIFT);
-  const bool more = live && z == 64u && lim > PER;
+  const bool more = live && z == 32u && lim > PER;
   if (__builtin_amdgcn_ballot_w64(more) == 0) return live ? p + min(n, lim) : p;
   if (more) return extend_diag<B8>(qw, qsh, tw, tsh, qn, tn, p + PER, k);
   return live ? p + min(n, lim) : p;
 }
 
 template <int NT, int J, int XN, int OEN, int EN, bool B8>
-__device__ void affine_block_regs(const WaveStore<false, true>& S, const BlockEdges& E, int32_t qn,
-                                  int32_t tn, int32_t m0, bool fitting, int32_t max_step, int tid,
+__device__ void affine_block_regs(const WaveStore<false, true>& S, const BlockEdges& E, const Band& band,
+                                  int32_t qn, int32_t tn, int32_t m0, bool fitting, int32_t max_step, int tid,
                                   PairShared* sh, bool hist_cap_ok, StepState& st) {
   constexpr int W = NT / 64, MD = XN > OEN ? XN : OEN, SL = MD + 1;
   constexpr int MR = BlockShape<MD>::MR;
@@ -700,7 +741,13 @@ __device__ void affine_block_regs(const WaveStore<false, true>& S, const BlockEd
   constexpr int32_t NDIAG = 64 * W * J;
   const int lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int32_t kb = reg_window_base(NDIAG, qn, tn);
+  // With a bound the computed cells of every step lie within (bound + 1) /
+  // en diagonals centred on kend / 2 (between the start diagonal 0 and the
+  // end diagonal kend), so the window is centred there.
+  const int32_t kend_g = qn - tn;
+  const int32_t kb = band.bound > 0
+                         ? min(0, max(-(NDIAG - 1), max(-tn, min(kend_g / 2 - NDIAG / 2, qn - (NDIAG - 1)))))
+                         : reg_window_base(NDIAG, qn, tn);
   const int32_t C = E.C, CE = C + 2;  // table entries per slot
   lmem_t<int32_t>* const tab = E.tab;
   frames_t FM[MR], FID;
@@ -724,7 +771,6 @@ __device__ void affine_block_regs(const WaveStore<false, true>& S, const BlockEd
   st = StepState{0, 0, false, 0, 0, ST_OK};
   uint64_t used = 0;
   bool hist_on = hist_cap_ok;
-  const int32_t kend_g = qn - tn;
   const uint32_t uq = (uint32_t)qn;
   const lmem_t<const uint64_t>* qw = S.qw;
   const lmem_t<const uint64_t>* tw = S.tw;
@@ -734,8 +780,11 @@ __device__ void affine_block_regs(const WaveStore<false, true>& S, const BlockEd
   constexpr uint32_t kDrop = 0x80000000u;
   for (;;) {
     const int32_t s = __builtin_amdgcn_readfirstlane(st.step + 1);
-    const int32_t lo = step_lo(s, tn), hi = step_hi(s, qn);
-    if (lo < kb || hi > kb + NDIAG - 1) {  // the window is too narrow from here on
+    int32_t lo, hi;  // the step's computed diagonals (its band)
+    band_range(band, s, qn, tn, lo, hi);
+    // The window is too narrow from here on, or the band is empty (the bound
+    // is below the optimum): the pair is re-run on the ring path.
+    if (lo > hi || lo < kb || hi > kb + NDIAG - 1) {
       st.status = ST_OVERFLOW_WIDTH;
       st.step = s;
       break;
@@ -758,7 +807,11 @@ __device__ void affine_block_regs(const WaveStore<false, true>& S, const BlockEd
     // with none runs one chunk outside the range (no cell changes: every
     // lane is out of range), so the loop always runs: a loop that may run
     // zero times makes the compiler copy every frame vector each step.
-    const int32_t c_lo = (lo - kb) >> 6, c_hi = (hi - kb) >> 6;
+    // With a bound the chunks next to the band are run too, with no cell in
+    // range: a chunk the band has left publishes "no cell" ends for the
+    // chunk beside it (its earlier ends would otherwise stay in the table).
+    const int32_t c_lo = max(((lo - kb) >> 6) - (band.bound > 0), 0);
+    const int32_t c_hi = min(((hi - kb) >> 6) + (band.bound > 0), C - 1);
     int32_t j = c_lo <= w ? 0 : (c_lo - w + W - 1) / W;
     int32_t j_end = c_hi < w ? -1 : (c_hi - w) / W;
     if (j > j_end) {
@@ -840,13 +893,13 @@ __device__ void affine_block_regs(const WaveStore<false, true>& S, const BlockEd
 // with the LDS tier); false for others (the ring path runs).
 template <int NT, int J, bool B8>
 __device__ __forceinline__ bool affine_block_regs_any(const WaveStore<false, true>& S, const BlockEdges& E,
-                                                      int32_t qn, int32_t tn, int32_t m0, bool fitting,
-                                                      int32_t max_step, int32_t xn, int32_t oen, int32_t en,
-                                                      int tid, PairShared* sh, StepState& st) {
+                                                      const Band& band, int32_t qn, int32_t tn, int32_t m0,
+                                                      bool fitting, int32_t max_step, int32_t xn, int32_t oen,
+                                                      int32_t en, int tid, PairShared* sh, StepState& st) {
   if (xn == 2 && oen == 4 && en == 1)  // 4/6/2
-    affine_block_regs<NT, J, 2, 4, 1, B8>(S, E, qn, tn, m0, fitting, max_step, tid, sh, true, st);
+    affine_block_regs<NT, J, 2, 4, 1, B8>(S, E, band, qn, tn, m0, fitting, max_step, tid, sh, true, st);
   else if (xn == 2 && oen == 5 && en == 1)  // 2/4/1
-    affine_block_regs<NT, J, 2, 5, 1, B8>(S, E, qn, tn, m0, fitting, max_step, tid, sh, true, st);
+    affine_block_regs<NT, J, 2, 5, 1, B8>(S, E, band, qn, tn, m0, fitting, max_step, tid, sh, true, st);
   else
     return false;
   return true;
@@ -992,6 +1045,9 @@ __device__ __forceinline__ void wave_affine(const WfaArgs& a, const WaveGeom& G,
   const int32_t x = a.cp.x, o = a.cp.o, e = a.cp.e;
   const bool fitting = !forced_global && tn > qn + qn / 2 + 10;
   const int32_t g = a.cp.g, xn = a.cp.xn, oen = a.cp.oen, en = a.cp.en;
+  // Band pruning: register-wavefront workgroup launches that ask for it
+  // (the forward pass and the traceback use the same band).
+  const Band band{J > 0 && a.band ? (int32_t)pd.bound : 0, qn - tn, en, fitting};
   // The m layer reads xn and oen steps back, the gap layers en steps back:
   // Rm and Rg frames of ring (the host sizes the ring as Rm + 2 Rg frames).
   const int32_t Rm = max(xn, oen) + 1, Rg = en + 1;
@@ -1055,7 +1111,8 @@ __device__ __forceinline__ void wave_affine(const WfaArgs& a, const WaveGeom& G,
       lmem_t<int32_t>* eb = (lmem_t<int32_t>*)(lds_base + G.tab_off);
       const BlockEdges E{eb, C};
       StepState st;
-      if (affine_block_regs_any<NT, J, B8>(S, E, qn, tn, m0, fitting, max_step, xn, oen, en, lane, sh, st)) {
+      if (affine_block_regs_any<NT, J, B8>(S, E, band, qn, tn, m0, fitting, max_step, xn, oen, en, lane, sh,
+                                           st)) {
         step = __builtin_amdgcn_readfirstlane(st.step);
         used = st.used;
         found = st.found;
@@ -1186,7 +1243,8 @@ __device__ __forceinline__ void wave_affine(const WfaArgs& a, const WaveGeom& G,
         break;
       }
       if (layer == 0) {
-        const int32_t lo = step_lo(s, tn), hi = step_hi(s, qn);
+        int32_t lo, hi;
+        band_range(band, s, qn, tn, lo, hi);
         if (k < lo || k > hi) { r.status = ST_INTERNAL; break; }
         const uint32_t cell = S.cells[(uint64_t)S.toff(s) + (uint64_t)(k - lo)];
         const int32_t pre = (int32_t)(cell >> 4) - 1;
@@ -1203,7 +1261,7 @@ __device__ __forceinline__ void wave_affine(const WfaArgs& a, const WaveGeom& G,
         const int32_t dir = layer == 1 ? -1 : 1;
         const uint32_t bit = layer == 1 ? 2 : 3;
         bool opened = false;
-        const int32_t n = gap_run(S, s, k, dir, en, bit, qn, tn, lane, &opened);
+        const int32_t n = gap_run(S, band, s, k, dir, en, bit, qn, tn, lane, &opened);
         w.emit(layer == 1 ? OP_D : OP_I, n);
         if (layer == 1) qi -= n; else ti -= n;
         k += dir * n;

@@ -357,11 +357,13 @@ def test_workgroup_register_wavefront(px, costs):
     """Workgroup tiers with the wavefront in registers (16 chunks of 64
     diagonals per wave, chunk ends exchanged through LDS): pairs of 1-8 kbp
     on both workgroup tiers, global and fitting, targets much shorter than
-    the query (window placement), byte sequences, 0.5-25% divergence; then
-    with every pair forced onto the register path, pairs whose range
-    outgrows the 4,096-diagonal window of the 256-thread tier stop and are
-    re-run on the ring path.  Every result equals the oracle's, and the
-    launches show the register storage was used (and the re-runs)."""
+    the query (window placement), byte sequences, 0.5-25% divergence, with
+    band pruning to each pair's step estimate; a pair whose optimum is far
+    above its estimate (the band empties) and, with every pair forced onto
+    the register path, pairs whose range outgrows the 4,096-diagonal window
+    of the 256-thread tier stop and are re-run on the ring path.  Every
+    result equals the oracle's, and the launches show the register storage
+    was used (and the re-runs)."""
     rng = random.Random(costs[0] * 100 + costs[1])
     qs, ts = [], []
     for L, rate in ((1100, 0.01), (1500, 0.05), (2000, 0.15), (2600, 0.08), (3300, 0.02),
@@ -385,7 +387,15 @@ def test_workgroup_register_wavefront(px, costs):
         q = rand_seq(rng, L)
         wq.append(q)
         wt.append(mutate(rng, q, rate))
+    # Band bound below the optimum: the target repeats a block the query has
+    # once and the query repeats a block the target lacks, so nearly every
+    # query k-mer occurs in the target and the step estimate (the band's
+    # bound) is far below the optimum (two gaps of 800 bases); the band
+    # empties before the end and the pair is re-run without a bound.
+    P, D, Q, R = (rand_seq(rng, n) for n in (1500, 800, 1500, 400))
+    uq, ut = [P + D + Q + R + R, qs[20]], [P + D + D + Q, ts[20]]
     for tun, pairs, label in (
+            (dict(block256_min_len=1024, block1024_min_len=6000), (uq, ut), "underestimated"),
             (dict(block256_min_len=1024, block1024_min_len=6000), (qs, ts), "tiers"),
             (dict(block256_min_len=1024, block1024_min_len=6000), (bq, ts), "bytes"),
             (dict(block256_min_len=1024, block1024_min_len=20000, block_regs=1, heavy_permille=1),
@@ -403,6 +413,8 @@ def test_workgroup_register_wavefront(px, costs):
         assert regs, "no launch used the register wavefront"
         if label == "tiers":
             assert {r["tier"] for r in regs} == {2, 3}
+        if label == "underestimated":
+            assert any(r["storage"] == 0 and r["tier"] >= 2 for r in info), info
         if label == "forced":
             # the two wide pairs overflow the register window: one ring re-run
             # launch takes both (other ring launches here hold one pair)
