@@ -116,6 +116,11 @@ typedef struct phx_tuning {
                                 wavefront in registers (0 = pairs whose estimate fits the window,
                                 1 = every pair whose sequences fit LDS, a pair that outgrows the
                                 window being re-run on the ring path, 2 = never) */
+  uint32_t lds_rows;         /* LDS tier, global-branch affine pairs with mismatch 2 and extend 1 in score
+                                steps (4/6/2, 2/4/1) or 1/0/1 and |qn - tn| <= 48: banded rows, four pairs
+                                per wave in 16-lane rows when an alignment within the row's band exists,
+                                else two per wave in 32-lane rows (0 or 1 = on, 2 = off: one pair per
+                                wave, 3 = every such pair in 32-lane rows) */
 } phx_tuning;
 
 /* Context: owns the stream, device buffers and the submission lock.
@@ -183,7 +188,9 @@ typedef struct phx_launch_info {
   int32_t tier;
   uint32_t pairs;
   uint32_t grid;
-  uint32_t storage; /* workgroup tiers: 0 ring in HBM, 1 ring in LDS, 2 register wavefront */
+  uint32_t storage; /* workgroup tiers: 0 ring in HBM, 1 ring in LDS, 2 register wavefront;
+                       LDS tier: 0 a pair per wave, 3 banded rows (2-bit pairs), 4 banded rows
+                       (byte pairs) */
   double ms;
   uint64_t algorithmic_bytes;
   uint64_t cells;

@@ -62,6 +62,8 @@ struct phx_plan {
                                                        // in LDS, 2 register wavefront
   std::vector<uint64_t> group_maxsum;                  // WFA: largest qn + tn of each group
   std::vector<uint32_t> group_seqw;                    // WFA: largest staged sequence words of each group
+  std::vector<uint32_t> group_rows;                    // WFA banded-row groups: history steps per row
+  std::vector<std::array<uint32_t, 4>> group_rowseg;   // WFA banded-row groups: segment sizes (WfaArgs::rows_seg)
   // hipGraph of a run's launch sequence (WFA): replayed by later runs.
   bool use_graph = false;
   bool graph_wanted = false;                           // build one after the current run
@@ -73,6 +75,8 @@ struct phx_plan {
   std::vector<uint32_t> order;                         // pending pairs, longest first
   std::vector<int32_t> est;                            // WFA: estimated score steps per pair
   std::vector<uint64_t> band_words;                    // WFA: history words within the pair's bound (0 = none)
+  std::vector<uint8_t> rows_ok;                        // WFA: LDS-tier pair a banded row can take
+  std::vector<uint8_t> rows_cert;                      // ... whose optimum is bounded by its 16-lane band
   // Allocation-free plans: every device buffer is a carve of a fixed region
   // (the device's scratch array) instead of an allocation.  Buffers that live
   // for the whole plan are carved from its start, up to fixed_transient; the
@@ -340,6 +344,7 @@ phx_tuning merge_tuning(phx_tuning d, const phx_tuning* t) {
   if (t->block_lds_ring) d.block_lds_ring = t->block_lds_ring == 1 ? 1 : next_pow2(t->block_lds_ring);
   if (t->block_flight) d.block_flight = std::min<uint32_t>(t->block_flight, 4);
   if (t->block_regs) d.block_regs = std::min<uint32_t>(t->block_regs, 2);
+  if (t->lds_rows) d.lds_rows = std::min<uint32_t>(t->lds_rows, 3);
   return d;
 }
 bool has_cigar(int engine) { return engine != PHX_ENGINE_MYERS_DIST; }
@@ -361,6 +366,14 @@ uint32_t seq_words_of(const phx_plan* P, const std::vector<uint32_t>& work) {
   uint32_t w = 2;
   for (uint32_t i : work) w = std::max(w, lds_seq_words(P->pairs[i]));
   return w;
+}
+
+// History steps per row of a banded-row group: B + 1 = |qn - tn| + 16.
+uint32_t rows_steps_of(const phx_plan* P, const std::vector<uint32_t>& work) {
+  uint32_t m = 16;
+  for (uint32_t i : work)
+    m = std::max<uint32_t>(m, (uint32_t)std::abs(P->pairs[i].qn - P->pairs[i].tn) + 16u);
+  return m;
 }
 
 // max_sum / seq_words: the group's cached maxima (0 = compute from work).
@@ -398,7 +411,7 @@ enum LaunchTiming { TIME_BOTH, TIME_NONE };
 int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint32_t* d_list,
                     uint32_t* d_counters, uint32_t* overflow, const GroupSlots& gs, uint8_t* slice,
                     hipStream_t st, int storage = 0, LaunchTiming timing = TIME_BOTH,
-                    uint32_t seq_words = 0) {
+                    uint32_t seq_words = 0, uint32_t row_steps = 0, const uint32_t* rows_seg = nullptr) {
   const bool rl = storage != 0;
   phx_ctx* ctx = P->ctx;
   WfaArgs a{};
@@ -417,7 +430,21 @@ int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint3
   if (t == TIER_LDS) {
     G = lds_geometry(lds_tier_ring(P->engine, P->costs) ? ring_frames(P->engine, P->costs) : 0,
                      seq_words, P->tun.wave_lds_kib);
-    slots = lds_tier_waves(work.size(), P->tun.wave_slots);
+    const bool rows = storage >= kRowsStorage;
+    uint64_t sets = work.size();
+    if (rows) {
+      // The group's segment sizes (32-lane down / up, 16-lane down / up):
+      // one wave per two or four of their pairs.
+      if (!rows_seg) return fail(ctx, PHX_E_INTERNAL, "banded-row launch without its segments");
+      const uint32_t* seg = rows_seg;
+      G = rows_geometry(G, seq_words, row_steps ? row_steps : rows_steps_of(P, work), seg[0] + seg[1] > 0);
+      sets = 0;
+      for (int j = 0; j < 4; ++j) {
+        a.rows_seg[j] = seg[j];
+        sets += (seg[j] + (j < 2 ? 1u : 3u)) / (j < 2 ? 2u : 4u);
+      }
+    }
+    slots = lds_tier_waves(sets, P->tun.wave_slots);
     a.static_order = 1;  // the list is stride-balanced for exactly these waves
   } else {
     a.hist = slice;
@@ -466,6 +493,7 @@ int issue_groups(phx_plan* P, bool capture, std::vector<int>* launched,
                  std::vector<hipGraphNode_t>* knodes) {
   phx_ctx* ctx = P->ctx;
   const int ngroups = (int)P->work.size();
+  if (ngroups > (int)kMaxGroups) return fail(ctx, PHX_E_INTERNAL, "too many launch groups");
   uint32_t* d_ctr = (uint32_t*)P->d_ctr.p;
   // Queue heads of the groups that use one (the LDS group strides over its
   // list statically); overflow flags live in pinned host memory and are
@@ -547,7 +575,8 @@ int issue_groups(phx_plan* P, bool capture, std::vector<int>* launched,
     int rc = launch_wfa_tier(P, tier[gi], wl, (uint32_t*)P->d_work.p + group_off[gi],
                              d_ctr + 2 * gi, P->ctr_dev + gi, gs[gi],
                              (uint8_t*)P->d_arena.p + slice_off[gi], st, P->group_rl[gi],
-                             timed ? TIME_BOTH : TIME_NONE, P->group_seqw[gi]);
+                             timed ? TIME_BOTH : TIME_NONE, P->group_seqw[gi], P->group_rows[gi],
+                             P->group_rowseg[gi].data());
     if (rc != PHX_OK) return rc;
     if (capture) {
       hipStreamCaptureStatus cs;
@@ -602,19 +631,24 @@ int build_graph(phx_plan* P) {
     return rc != PHX_OK ? rc : fail(ctx, PHX_E_DEVICE, "graph capture failed");
   }
   int ret = PHX_OK;
+  auto deps_of = [](hipGraphNode_t n) {
+    size_t nd = 0;
+    std::vector<hipGraphNode_t> d;
+    if (hipGraphNodeGetDependencies(n, nullptr, &nd) == hipSuccess && nd) {
+      d.resize(nd);
+      (void)hipGraphNodeGetDependencies(n, d.data(), &nd);
+    }
+    return d;
+  };
   for (size_t i = 0; i < knodes.size() && ret == PHX_OK; ++i) {
     hipEvent_t e0, e1;
     if (kernel_events(P, (uint32_t)i, &e0, &e1) != PHX_OK) ret = PHX_E_DEVICE;
-    size_t nd = 0;
-    std::vector<hipGraphNode_t> deps;
+    hipGraphNode_t first = knodes[i];
+    const std::vector<hipGraphNode_t> deps = ret == PHX_OK ? deps_of(first) : std::vector<hipGraphNode_t>{};
     hipGraphNode_t n0 = nullptr, n1 = nullptr;
-    if (ret == PHX_OK && hipGraphNodeGetDependencies(knodes[i], nullptr, &nd) == hipSuccess) {
-      deps.resize(nd);
-      if (nd) (void)hipGraphNodeGetDependencies(knodes[i], deps.data(), &nd);
-    }
     if (ret == PHX_OK &&
         (hipGraphAddEventRecordNode(&n0, g, deps.data(), deps.size(), e0) != hipSuccess ||
-         hipGraphAddDependencies(g, &n0, &knodes[i], 1) != hipSuccess ||
+         hipGraphAddDependencies(g, &n0, &first, 1) != hipSuccess ||
          hipGraphAddEventRecordNode(&n1, g, &knodes[i], 1, e1) != hipSuccess))
       ret = fail(ctx, PHX_E_DEVICE, "graph timing nodes");
   }
@@ -756,12 +790,50 @@ void plan_group(phx_plan* P) {
   P->group_slot.clear();
   P->group_rl.clear();
   // LDS tier: no work queue; the waves stride over a cost-balanced list.
+  // Global-branch affine pairs of a banded-row cost shape run four to a
+  // wave (one launch group per encoding and window direction); the rest,
+  // and every pair when rows are tuned off, one pair per wave.
+  // A pair whose optimum is bounded by its 16-lane band (rows_cert, from
+  // plan creation) runs in a 16-lane row, four to a wave; the others in
+  // 32-lane rows, two to a wave (or all of them: tuning lds_rows = 3).
+  std::vector<uint32_t> rows[8];  // [bytes * 4 + segment (32 down, 32 up, 16 down, 16 up)]
+  {
+    std::vector<uint32_t> single;
+    for (uint32_t i : bucket[0]) {
+      const PairDesc& pd = P->pairs[i];
+      if (P->tun.lds_rows != 2 && P->rows_ok[i])
+        rows[((pd.flags & PF_BYTES) ? 4 : 0) + (P->rows_cert[i] && P->tun.lds_rows != 3 ? 2 : 0) +
+             (pd.qn > pd.tn ? 1 : 0)].push_back(i);
+      else
+        single.push_back(i);
+    }
+    bucket[0].swap(single);
+  }
   stride_balance(bucket[0], P->est, lds_tier_waves(bucket[0].size(), P->tun.wave_slots));
   P->work.push_back(bucket[0]);
   P->group_tier.push_back(TIER_LDS);
   P->group_rl.push_back(0);
   P->group_slot.push_back(0);
   std::vector<double> vol(1, 0.0);  // estimated history words per group
+  std::vector<std::array<uint32_t, 4>> rowseg(1, {0, 0, 0, 0});
+  for (int b8 = 0; b8 < 2; ++b8) {
+    std::vector<uint32_t> list;
+    std::array<uint32_t, 4> seg{0, 0, 0, 0};
+    for (int j = 0; j < 4; ++j) {
+      std::vector<uint32_t>& r = rows[b8 * 4 + j];
+      const uint32_t per = j < 2 ? 2u : 4u;
+      quad_balance(r, P->est, lds_tier_waves((r.size() + per - 1) / per, P->tun.wave_slots), per);
+      seg[j] = (uint32_t)r.size();
+      list.insert(list.end(), r.begin(), r.end());
+    }
+    if (list.empty()) continue;
+    P->work.push_back(std::move(list));
+    P->group_tier.push_back(TIER_LDS);
+    P->group_rl.push_back((uint8_t)(kRowsStorage + b8));
+    P->group_slot.push_back(0);
+    vol.push_back(0.0);
+    rowseg.push_back(seg);
+  }
   for (int b = 1; b < 6; ++b) {
     if (bucket[b].empty()) continue;
     std::vector<uint64_t> w;
@@ -883,6 +955,7 @@ void plan_group(phx_plan* P) {
   std::vector<double> ask(P->work.size(), 0.0);
   double ask_sum = 0;
   for (size_t g = 1; g < P->work.size(); ++g) {
+    if (P->group_tier[g] == TIER_LDS) continue;
     uint64_t max_sum = 1;
     for (uint32_t i : P->work[g]) max_sum = std::max<uint64_t>(max_sum, P->qlen[i] + P->tlen[i]);
     const uint64_t ring = P->group_rl[g] ? 0 : ring_words_for(ring_frames(P->engine, P->costs), max_sum, nullptr) * 4;
@@ -899,6 +972,7 @@ void plan_group(phx_plan* P) {
   const double wide_share = over && wide >= 0 ? std::min(ask[wide] / (double)total, 0.75) : 0.0;
   const double rest_sum = over && wide >= 0 ? ask_sum - ask[wide] : ask_sum;
   for (size_t g = 1; g < P->work.size(); ++g) {
+    if (P->group_tier[g] == TIER_LDS) continue;
     // Fixed plans split their region (sized only after the persistent
     // buffers are carved) in proportion to the asks.
     P->group_share[g] = P->fixed() ? (ask_sum > 0 ? ask[g] / ask_sum : 0.0)
@@ -927,12 +1001,19 @@ void plan_group(phx_plan* P) {
     }
     P->group_load[g] = std::max(longest, sum / std::max(1.0, std::min(flight, (double)P->work[g].size())));
   }
+  // Segment sizes of the banded-row groups (they keep their places: only
+  // workgroup-tier groups are merged above).
+  P->group_rowseg.assign(P->work.size(), {0, 0, 0, 0});
+  for (size_t g = 0; g < rowseg.size() && g < P->work.size(); ++g) P->group_rowseg[g] = rowseg[g];
   // Per-group maxima the launches need, computed once per grouping.
   P->group_maxsum.assign(P->work.size(), 0);
   P->group_seqw.assign(P->work.size(), 0);
+  P->group_rows.assign(P->work.size(), 0);
   for (size_t g = 0; g < P->work.size(); ++g) {
     P->group_maxsum[g] = max_sum_of(P, P->work[g]);
     if (P->group_tier[g] == TIER_LDS || P->group_rl[g] != 0) P->group_seqw[g] = seq_words_of(P, P->work[g]);
+    if (P->group_tier[g] == TIER_LDS && P->group_rl[g] >= kRowsStorage)
+      P->group_rows[g] = rows_steps_of(P, P->work[g]);
   }
 }
 
@@ -980,6 +1061,27 @@ phx_plan* plan_setup(phx_plan* P, const std::vector<const uint8_t*>& qh,
     const bool banded = block_regs_shape(P->engine, P->costs);
     const int32_t en_steps = banded ? P->costs.gap_extend / gcd3(P->costs.mismatch, P->costs.gap_open,
                                                                  P->costs.gap_extend) : 1;
+    // Banded rows: global-branch affine pairs of a rows cost shape with
+    // |qn - tn| <= kRowsMaxKend, and whether an alignment within the 16-lane
+    // band exists (rows_upper_bound).
+    P->rows_ok.assign(n, 0);
+    P->rows_cert.assign(n, 0);
+    const CostParams rcp = cost_params(P->costs, -1);
+    if (P->engine == PHX_ENGINE_AFFINE && rows_shape(rcp.xn, rcp.oen, rcp.en)) {
+      std::vector<uint32_t> narrow;
+      for (uint32_t i : P->order) {
+        const PairDesc& pd = P->pairs[i];
+        const bool global = (pd.flags & PF_GLOBAL) || pd.tn <= pd.qn + pd.qn / 2 + 10;
+        if (lds_ok(i) && global && pd.qn >= 1 && pd.tn >= 1 && std::abs(pd.qn - pd.tn) <= kRowsMaxKend)
+          narrow.push_back(i);
+      }
+      parallel_for(narrow.size(), [&](size_t k) {
+        const uint32_t i = narrow[k];
+        P->rows_ok[i] = 1;
+        P->rows_cert[i] = rows_upper_bound(qh[i], P->qlen[i], th[i], P->tlen[i], rcp.xn, rcp.oen) <=
+                          (int64_t)std::abs((int64_t)P->qlen[i] - (int64_t)P->tlen[i]) + 15;
+      });
+    }
     parallel_for(wide.size(), [&](size_t k) {
       const uint32_t i = wide[k];
       P->est[i] = estimate_steps_seq(P->engine, P->costs, qh[i], P->qlen[i], th[i], P->tlen[i]);
@@ -1013,16 +1115,16 @@ phx_plan* plan_setup(phx_plan* P, const std::vector<const uint8_t*>& qh,
   if (plan_alloc(P, P->d_out, sizeof(PairOut) * (size_t)std::max<uint32_t>(n, 1)) != PHX_OK) return fail_plan("out");
   if (plan_alloc(P, P->d_runs, P->runs_total * 4 + 4) != PHX_OK) return fail_plan("runs");
   if (plan_alloc(P, P->d_work, (size_t)std::max<uint32_t>(n, 1) * 4) != PHX_OK) return fail_plan("work");
-  if (plan_alloc(P, P->d_ctr, 256) != PHX_OK) return fail_plan("counters");
+  if (plan_alloc(P, P->d_ctr, 2 * kMaxGroups * sizeof(uint32_t)) != PHX_OK) return fail_plan("counters");
   if (P->fixed()) {
     // Re-run lists and the overflow flags (words of the region, read back
     // after each run) are carved up front: a run allocates nothing.
     DevBuf flags;
     if (plan_alloc(P, P->d_work2, (size_t)std::max<uint32_t>(n, 1) * 4) != PHX_OK ||
-        plan_alloc(P, flags, 64 * sizeof(uint32_t)) != PHX_OK)
+        plan_alloc(P, flags, kMaxGroups * sizeof(uint32_t)) != PHX_OK)
       return fail_plan("flags");
     P->ctr_dev = (uint32_t*)flags.p;
-    P->flag_shadow.assign(64, 0);
+    P->flag_shadow.assign(kMaxGroups, 0);
     P->ctr_host = P->flag_shadow.data();
     P->fixed_transient = (P->fixed_used + 255) & ~(uint64_t)255;
     if (P->fixed_transient >= P->fixed_cap) return fail_plan("transient region");
@@ -1052,7 +1154,7 @@ phx_plan* plan_setup(phx_plan* P, const std::vector<const uint8_t*>& qh,
         (wl_b && hipMemcpyAsync(P->d_work.p, pin + seq_b + pd_b, wl_b, hipMemcpyHostToDevice, ctx->stream) != hipSuccess))
       return fail_plan("upload", PHX_E_DEVICE);
     if (!P->fixed() && !P->ctr_host &&
-        (hipHostMalloc((void**)&P->ctr_host, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
+        (hipHostMalloc((void**)&P->ctr_host, kMaxGroups * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
          hipHostGetDevicePointer((void**)&P->ctr_dev, P->ctr_host, 0) != hipSuccess))
       return fail_plan("pinned counters");
     // The staging block is reused by the results: let the uploads finish.
@@ -1063,7 +1165,7 @@ phx_plan* plan_setup(phx_plan* P, const std::vector<const uint8_t*>& qh,
       (n && hipMemcpy(P->d_pairs.p, P->pairs.data(), sizeof(PairDesc) * n, hipMemcpyHostToDevice) != hipSuccess))
     return fail_plan("upload", PHX_E_DEVICE);
   if (!P->fixed() && !P->ctr_host &&
-      (hipHostMalloc((void**)&P->ctr_host, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
+      (hipHostMalloc((void**)&P->ctr_host, kMaxGroups * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
        hipHostGetDevicePointer((void**)&P->ctr_dev, P->ctr_host, 0) != hipSuccess))
     return fail_plan("pinned counters");
   // Work lists are static between tunings: upload once.

@@ -98,6 +98,13 @@ __host__ __device__ inline bool reg_wavefront_shape(int32_t xn, int32_t oen, int
          (xn == 1 && oen == 1 && en == 1) || (xn == 3 && oen == 7 && en == 2);
 }
 
+// Cost shapes of the banded-row kernel (a register shape with en = 1: one
+// diagonal per gap-extend step, so a row's window slides by one diagonal per
+// score step).
+__host__ __device__ inline bool rows_shape(int32_t xn, int32_t oen, int32_t en) {
+  return en == 1 && ((xn == 2 && oen == 4) || (xn == 2 && oen == 5) || (xn == 1 && oen == 1));
+}
+
 // Kernel arguments of one WFA tier launch.
 struct WfaArgs {
   const PairDesc* pairs;
@@ -115,6 +122,10 @@ struct WfaArgs {
   uint32_t wcap_log2;    // ring width in diagonals = 1 << wcap_log2
   uint32_t static_order; // wave kernels: wave g takes items g, g + waves, ... (no queue)
   uint32_t band;         // register-wavefront workgroup launches: prune to PairDesc::bound
+  // Banded-row launches: the list is four segments of these many entries,
+  // in this order: 32-lane rows with qn <= tn, 32-lane rows with qn > tn,
+  // 16-lane rows with qn <= tn, 16-lane rows with qn > tn.
+  uint32_t rows_seg[4];
 };
 
 // LDS layout of one wave of the wave-resident WFA tier (byte offsets inside
@@ -128,7 +139,21 @@ struct WaveGeom {
   uint32_t seq_cap_words;
   uint32_t steps_cap;
   uint32_t cells_cap;
+  // Banded-row launches (four pairs per wave in 16-lane rows, or two in
+  // 32-lane rows): offset of a row's history inside its region (after the
+  // packed query and target) and history steps of a 16-lane row
+  // (rows_region_bytes gives the rest).  A wave's region holds the largest
+  // of four 16-lane row regions, two 32-lane ones and the single-pair layout
+  // above, which serves the pairs a 32-lane row cannot finish.
+  uint32_t rows_hist_off;
+  uint32_t rows_steps;
 };
+
+// Bytes of one row region of `rw` lanes (16 or 32): sequences, then
+// (rows_steps + rw - 16) steps of rw u16 history cells.
+__host__ __device__ inline uint32_t rows_region_bytes(uint32_t hist_off, uint32_t rows_steps, uint32_t rw) {
+  return hist_off + (((rows_steps + rw - 16u) * rw * 2u + 15u) & ~15u);
+}
 
 // Kernel arguments of one Myers launch.
 struct MyersArgs {

@@ -31,6 +31,7 @@ hipError_t launch_wfa_wave(int mode, bool lds, const WfaArgs& a, const WaveGeom&
                            uint32_t grid, hipStream_t st);
 hipError_t launch_wfa_block(int mode, int nt, int storage, const WfaArgs& a, const WaveGeom& G,
                             uint32_t grid, hipStream_t st);
+hipError_t launch_wfa_rows(bool b8, const WfaArgs& a, const WaveGeom& G, uint32_t grid, hipStream_t st);
 hipError_t launch_gather_runs(const uint32_t* runs, const uint64_t* src, const uint64_t* off,
                               uint32_t* dst, uint32_t n, hipStream_t st);
 
@@ -43,6 +44,11 @@ constexpr int kWaveRingMax = 64;            // ring slots the LDS tier allows
 constexpr uint32_t kWaveSteps = 128;        // LDS tier step-table entries per wave
 constexpr int kWaveWaves = 4;               // waves per workgroup of the wave kernels
 constexpr uint32_t kWaveWidth = 128;        // LDS tier ring width (diagonals)
+constexpr uint32_t kMaxGroups = 64;         // launch groups of a WFA plan (counters and flags per group)
+constexpr int32_t kRowsMaxKend = 48;        // banded rows: |qn - tn| limit (history steps per row)
+// LDS-tier launch groups run as banded rows when their storage code is
+// kRowsStorage (2-bit pairs) or kRowsStorage + 1 (bytes).
+constexpr int kRowsStorage = 3;
 
 // Tiers of the WFA kernels.
 enum Tier : int {
@@ -466,6 +472,75 @@ inline void stride_balance(std::vector<uint32_t>& list, const std::vector<int32_
   list.swap(out);
 }
 
+// Layout of a banded-row launch: four 16-lane or two 32-lane row regions
+// of `row_seq_words` packed words and row_steps (+ 16) history steps each
+// (rows_region_bytes), or the single-pair LDS layout G in the same region
+// (for the pairs a 32-lane row cannot finish).
+inline WaveGeom rows_geometry(WaveGeom G, uint32_t row_seq_words, uint32_t row_steps, bool wide_rows) {
+  G.rows_hist_off = (row_seq_words * 8 + 15u) & ~15u;
+  G.rows_steps = row_steps;
+  const uint32_t quads = 4u * rows_region_bytes(G.rows_hist_off, row_steps, 16);
+  G.per_wave_bytes = wide_rows ? std::max(std::max(G.per_wave_bytes, quads),
+                                          2u * rows_region_bytes(G.rows_hist_off, row_steps, 32))
+                               : quads;
+  return G;
+}
+
+// Mismatching bytes of a[0, n) against b[0, n), eight at a time.
+inline uint32_t mismatched_bytes(const uint8_t* a, const uint8_t* b, size_t n) {
+  uint32_t d = 0;
+  size_t i = 0;
+  for (; i + 8 <= n; i += 8) {
+    uint64_t x, y;
+    std::memcpy(&x, a + i, 8);
+    std::memcpy(&y, b + i, 8);
+    uint64_t z = x ^ y;
+    z |= z >> 4;
+    z |= z >> 2;
+    z |= z >> 1;
+    d += (uint32_t)__builtin_popcountll(z & 0x0101010101010101ull);
+  }
+  for (; i < n; ++i) d += a[i] != b[i];
+  return d;
+}
+
+// Upper bound (score steps) on the global gap-affine optimum of q against t
+// for a cost shape with extend = 1 step: the cost of the better of two
+// alignments that both exist, one diagonal of min(qn, tn) bases (xn steps a
+// mismatch) with the |qn - tn| gap (oen + |qn - tn| - 1 steps) at the end,
+// or with the gap at the start.  A pair whose bound is at most |qn - tn| +
+// 15 has its optimum inside a 16-lane row's band.
+inline int64_t rows_upper_bound(const uint8_t* q, size_t qn, const uint8_t* t, size_t tn, int32_t xn,
+                                int32_t oen) {
+  const size_t m = std::min(qn, tn), d = std::max(qn, tn) - m;
+  const int64_t gap = d ? (int64_t)oen + (int64_t)d - 1 : 0;
+  const int64_t end = (int64_t)mismatched_bytes(q, t, m) * xn + gap;
+  if (d == 0) return end;
+  const int64_t start = (int64_t)mismatched_bytes(qn > tn ? q + d : q, qn > tn ? t : t + d, m) * xn + gap;
+  return std::min(end, start);
+}
+
+// Order of a banded-row work list for `waves` waves (wave g takes sets g,
+// g + waves, ...; a set is `per` consecutive entries, one per row): pairs by
+// estimated cost, largest first, so each set holds pairs of similar cost;
+// sets dealt to the waves in alternating direction each round.
+inline void quad_balance(std::vector<uint32_t>& list, const std::vector<int32_t>& est, uint32_t waves,
+                         uint32_t per = 4) {
+  radix_sort_desc(list, [&](uint32_t i) { return (uint32_t)std::max(est[i], 0); });
+  const size_t nq = (list.size() + per - 1) / per, S = waves;
+  std::vector<uint32_t> out;
+  out.reserve(list.size());
+  for (size_t r0 = 0; r0 < nq; r0 += S) {
+    const size_t m = std::min(S, nq - r0);
+    const bool rev = (r0 / S) & 1;
+    for (size_t g = 0; g < m; ++g) {
+      const size_t q = r0 + (rev ? m - 1 - g : g);
+      for (size_t j = per * q; j < std::min(list.size(), per * q + per); ++j) out.push_back(list[j]);
+    }
+  }
+  list.swap(out);
+}
+
 // Words the LDS tier stages for one pair (query then target, with shifts).
 inline uint32_t lds_seq_words(const PairDesc& pd) {
   const bool b8 = pd.flags & PF_BYTES;
@@ -476,9 +551,13 @@ inline uint32_t lds_seq_words(const PairDesc& pd) {
 // Launch one tier over a.work_list: `slots` pairs in flight (waves for the
 // wave tiers, workgroups for the block tiers); storage of the block tiers:
 // 0 ring in HBM, 1 ring and sequences in LDS, 2 register wavefront (layout
-// in G).
+// in G); of the LDS tier: 0 a pair per wave, kRowsStorage.. banded rows.
 inline hipError_t launch_tier(int tier, int mode, const WfaArgs& a, const WaveGeom& G,
                               uint32_t slots, hipStream_t st, int storage = 0) {
+  if (tier == TIER_LDS && storage >= kRowsStorage) {
+    const uint32_t wgs = std::max<uint32_t>(1, (slots + kWaveWaves - 1) / kWaveWaves);
+    return launch_wfa_rows(storage != kRowsStorage, a, G, wgs, st);
+  }
   if (tier <= TIER_WAVE) {
     const uint32_t wgs = std::max<uint32_t>(1, (slots + kWaveWaves - 1) / kWaveWaves);
     return launch_wfa_wave(mode, tier == TIER_LDS, a, G, wgs, st);

@@ -1465,6 +1465,294 @@ __device__ __forceinline__ void wave_linear(const WfaArgs& a, const WaveGeom& G,
   publish(a, po, r, lane);
 }
 
+// --- banded rows: four short pairs per wave ---------------------------------------
+//
+// Global-branch gap-affine pairs whose cost shape has en = 1 (rows_shape):
+// each row of RW lanes aligns a pair of its own.  At score step s, lane r of
+// a row holds diagonal k = B0 + dir s + r: dir = -1, B0 = 0 when qn <= tn
+// (the window [-s, RW - 1 - s] follows the lower end of the reachable range
+// down) and dir = +1, B0 = 1 - RW when qn > tn (window [s - RW + 1, s]).
+//
+// Band: a cell (s, k) lies on an alignment of score <= B steps only if
+// s + |k - kend| <= B, kend = qn - tn (each diagonal between k and the end
+// costs at least one more gap-extend step).  With B = |kend| + RW - 1 every
+// such cell is inside the row's window (going down: k <= kend + B - s =
+// RW - 1 - s).  A candidate (s', k') of a cell (s, k) has s' + |k' - kend| <=
+// s + |k - kend|, so the cells of the band, their values and their
+// tie-breaks only depend on cells of the band: a pair whose optimum is at
+// most B steps gets exactly the unbanded result (end, score, CIGAR), and the
+// window cells outside the band compute values that nothing reads.  The end
+// cell is on diagonal kend, inside the band at every step <= B.  Cells out
+// of reach (|k| > s) need no test: their candidates are out of reach too and
+// come out unset, as on the single-pair path.
+//
+// A wave first aligns four pairs in 16-lane rows; the pairs whose optimum
+// is above their band (end not found by step B) are aligned again two at a
+// time in 32-lane rows (B = |kend| + 31), and the rare ones beyond that by
+// the whole wave on the single-pair path, all in the same launch.
+//
+// Frames keep each step's own lane layout (P = offset + 1, 0 = no cell); a
+// read of step s - d on diagonal k + j is lane r + dir d + j of the same row:
+// one DPP row shift with zero fill past the row's ends for 16-lane rows, a
+// lane permute with the ends masked for 32-lane rows, so rows never mix.
+// History: one u16 cell per (step, lane) of the row (the LDS tier's cell
+// format), so the traceback finds diagonal k of step s at lane k - B0 - dir s.
+
+// Value of lane r + N of the same row of RW lanes; 0 past either end of the
+// row.  Call with every lane active.
+template <int N, int RW>
+__device__ __forceinline__ int32_t row_read(int32_t v, int lane) {
+  static_assert(N > -RW && N < RW, "a row shift stays inside the row");
+  if constexpr (N == 0) {
+    return v;
+  } else if constexpr (RW == 16) {
+    if constexpr (N > 0) return __builtin_amdgcn_mov_dpp(v, 0x100 + N, 0xF, 0xF, true);  // row_shl:N
+    else return __builtin_amdgcn_mov_dpp(v, 0x110 - N, 0xF, 0xF, true);                  // row_shr:-N
+  } else {
+    const int32_t x = __builtin_amdgcn_ds_bpermute(((lane + N) & 63) << 2, v);
+    return (uint32_t)((lane & (RW - 1)) + N) < (uint32_t)RW ? x : 0;
+  }
+}
+
+// The row's pair re-packed at bit 0 into its LDS region (query, then
+// target, each followed by a zero word), the row's RW lanes striding.
+template <bool B8, int RW>
+__device__ __forceinline__ void stage_row(const PairDesc& pd, lmem_t<uint64_t>* seq, int r) {
+  const uint32_t nq = seq_words<B8>(pd.qn), nt = seq_words<B8>(pd.tn);
+  const uint32_t qb = pd.q_shift * (B8 ? 8u : 2u), tb = pd.t_shift * (B8 ? 8u : 2u);
+  const gmem_t<const uint64_t>* gq = (const gmem_t<const uint64_t>*)pd.q;
+  const gmem_t<const uint64_t>* gt = (const gmem_t<const uint64_t>*)pd.t;
+  for (uint32_t i = (uint32_t)r; i < nq + nt; i += RW) {
+    const bool isq = i < nq;
+    const uint32_t j = isq ? i : i - nq, n = isq ? nq : nt, b = isq ? qb : tb;
+    const gmem_t<const uint64_t>* src = isq ? gq : gt;
+    uint64_t v = 0;
+    if (j + 1 < n) {
+      const uint64_t lo = src[j];
+      v = b == 0 ? lo : (lo >> b) | (src[j + 1] << (64 - b));
+    }
+    seq[i] = v;
+  }
+}
+
+// Match extension over sequences staged at bit 0 in LDS, for the banded
+// rows: a first 32-bit window (16 bases, or 4 bytes) on every lane without a
+// branch, then, while any lane's match runs on, 128-bit windows (64 bases,
+// or 16 bytes): the ten dwords of a window are read together and compared
+// with one wait, so a long match (the leading diagonal of a row runs to the
+// next difference, about a hundred bases at 1% divergence) costs half the
+// LDS round trips of 64-bit windows.  Reads run at most four dwords past a
+// sequence's staged words, still inside the row's region (the target is
+// followed by the history); bases past the suffix ends are cut by `lim`.
+template <bool B8>
+__device__ __forceinline__ int32_t extend_rows(const lmem_t<const uint64_t>* qw, const lmem_t<const uint64_t>* tw,
+                                               int32_t qn, int32_t tn, int32_t p, int32_t k) {
+  constexpr uint32_t BITS = B8 ? 8 : 2;
+  constexpr uint32_t CTZ_SHIFT = B8 ? 3 : 1;
+  constexpr int32_t PER = B8 ? 4 : 16;
+  const int32_t j = p - k;
+  const bool live = (uint32_t)p < (uint32_t)qn && (uint32_t)j < (uint32_t)tn;
+  const int32_t lim = min(qn - p, tn - j);
+  const uint32_t qb = live ? (uint32_t)p * BITS : 0u, tb = live ? (uint32_t)j * BITS : 0u;
+  const lmem_t<const uint32_t>* q32 = (const lmem_t<const uint32_t>*)qw + (qb >> 5);
+  const lmem_t<const uint32_t>* t32 = (const lmem_t<const uint32_t>*)tw + (tb >> 5);
+  const uint32_t qx = __builtin_amdgcn_alignbit(q32[1], q32[0], qb);  // shift = qb & 31
+  const uint32_t tx = __builtin_amdgcn_alignbit(t32[1], t32[0], tb);
+  const uint32_t z0 = (uint32_t)__builtin_ctzg(qx ^ tx, 32);  // 32 when all match
+  const bool more = live && z0 == 32u && lim > PER;
+  if (__builtin_amdgcn_ballot_w64(more) == 0) return live ? p + min((int32_t)(z0 >> CTZ_SHIFT), lim) : p;
+  int32_t n = (int32_t)(z0 >> CTZ_SHIFT);
+  if (more) {
+    q32 += 1;
+    t32 += 1;
+    for (;;) {
+      uint32_t qa[5], ta[5];
+#pragma unroll
+      for (int i = 0; i < 5; ++i) {
+        qa[i] = q32[i];
+        ta[i] = t32[i];
+      }
+      uint32_t z = 128u;
+#pragma unroll
+      for (int i = 3; i >= 0; --i) {
+        const uint32_t x = __builtin_amdgcn_alignbit(qa[i + 1], qa[i], qb) ^ __builtin_amdgcn_alignbit(ta[i + 1], ta[i], tb);
+        z = x ? 32u * (uint32_t)i + (uint32_t)__builtin_ctz(x) : z;
+      }
+      n += (int32_t)(z >> CTZ_SHIFT);
+      if (z < 128u || n >= lim) break;
+      q32 += 4;
+      t32 += 4;
+    }
+  }
+  return live ? p + min(n, lim) : p;
+}
+
+// One pass of the wave's 64 / RW rows (pi = ~0u: empty row).  XN, OEN:
+// mismatch and open + extend in score steps (extend = 1); DIR: window
+// direction of every pair.  Returns true on the lanes of a row whose pair's
+// optimum lies beyond its band (nothing stored for it).
+template <int XN, int OEN, int DIR, bool B8, int RW>
+__device__ __forceinline__ bool rows_pass(const WfaArgs& a, const WaveGeom& G, uint8_t* base, int lane,
+                                          uint32_t pi) {
+  constexpr int MD = XN > OEN ? XN : OEN;
+  constexpr int32_t B0 = DIR < 0 ? 0 : 1 - RW;
+  constexpr uint64_t RMASK = RW == 64 ? ~0ull : (1ull << RW) - 1;
+  const int r = lane & (RW - 1);
+  const uint32_t rsh = (uint32_t)lane & (uint32_t)(64 - RW);  // the row's bits in a ballot
+  const bool has = pi != 0xFFFFFFFFu;
+  PairDesc pd{};
+  if (has) pd = a.pairs[pi];
+  const int32_t qn = pd.qn, tn = pd.tn;
+  uint8_t* rb = base + (uint32_t)(lane / RW) * rows_region_bytes(G.rows_hist_off, G.rows_steps, RW);
+  lmem_t<uint64_t>* seq = (lmem_t<uint64_t>*)rb;
+  lmem_t<uint16_t>* hist = (lmem_t<uint16_t>*)(rb + G.rows_hist_off);
+  if (has) stage_row<B8, RW>(pd, seq, r);
+  wave_sync<true>();
+  const lmem_t<const uint64_t>* qw = seq;
+  const lmem_t<const uint64_t>* tw = seq + seq_words<B8>(qn);
+  const int32_t kend = qn - tn;
+  const int32_t B = (DIR < 0 ? -kend : kend) + (RW - 1);
+  const int32_t g = a.cp.g;
+  // The cap and the last score step explored, as on the single-pair path.
+  const int64_t hard_max = (int64_t)(qn + tn) * max(a.cp.x, 1) + a.cp.o + a.cp.e + 4;
+  const int64_t explore_to = a.cp.cap >= 0 ? min((int64_t)a.cp.cap, hard_max) : hard_max;
+  const int32_t max_step = explore_to <= INT32_MAX ? (int32_t)explore_to / g
+                                                   : (int32_t)min<int64_t>(explore_to / g, INT32_MAX);
+  // Every live row's step stays below its region's step count: lim <= B <=
+  // rows_steps + RW - 17 (the host sizes rows_steps for the largest |kend|).
+  const int32_t lim = min(B, max_step);
+  const uint32_t span = (uint32_t)(qn + tn);  // k + tn in [0, qn + tn]: inside the matrix
+
+  const int32_t m0 = has ? extend_diag<B8>(qw, 0u, tw, 0u, qn, tn, 0, 0) : 0;
+  int32_t M[MD];  // M[MD - d]: step s - d
+#pragma unroll
+  for (int j = 0; j < MD; ++j) M[j] = 0;
+  M[MD - 1] = has && B0 + r == 0 ? m0 + 1 : 0;
+  int32_t I1 = 0, D1 = 0;  // step s - 1
+  bool live = has;
+  int32_t end_s = -1, end_layer = 0;
+  if (has && kend == 0 && m0 == qn) {  // s = 0 endpoint on diagonal 0
+    end_s = 0;
+    live = false;
+  }
+  for (int32_t s = 1;; ++s) {
+    live = live && s <= lim;
+    if (__builtin_amdgcn_ballot_w64(live) == 0) break;
+    const int32_t k = B0 + DIR * s + r;
+    const bool inr = live && (uint32_t)(k + tn) <= span;
+    const int32_t vx = row_read<DIR * XN, RW>(M[MD - XN], lane);
+    const int32_t vol = row_read<DIR * OEN - 1, RW>(M[MD - OEN], lane);
+    const int32_t vor = row_read<DIR * OEN + 1, RW>(M[MD - OEN], lane);
+    const int32_t vil = row_read<DIR - 1, RW>(I1, lane);
+    const int32_t vdr = row_read<DIR + 1, RW>(D1, lane);
+    // Candidate tests on P - 1 (the offset), as in affine_step.
+    const uint32_t uq = (uint32_t)qn, utk = (uint32_t)(tn + k);
+    int32_t ins = (uint32_t)(vol - 1) < uq ? vol + 1 : 0;
+    uint32_t iop = 0;
+    if ((uint32_t)(vil - 1) < uq && vil + 1 > ins) { ins = vil + 1; iop = 1; }
+    int32_t del = (uint32_t)(vor - 1) <= utk ? vor : 0;
+    uint32_t dop = 0;
+    if ((uint32_t)(vdr - 1) <= utk && vdr > del) { del = vdr; dop = 1; }
+    int32_t best = (uint32_t)(vx - 1) < min(uq, utk) ? vx + 1 : 0;
+    uint32_t mop = 0;
+    if (ins > best) { best = ins; mop = 1; }
+    if (del > best) { best = del; mop = 2; }
+    const int32_t mval = extend_rows<B8>(qw, tw, qn, tn, inr ? best - 1 : -1, k);
+    // Every row stores (s stays inside every region of the launch while
+    // any row is live); steps past a row's end are never read.
+    hist[(uint32_t)s * RW + (uint32_t)r] =
+        inr ? (uint16_t)(((uint32_t)best << 4) | mop | (iop << 2) | (dop << 3)) : (uint16_t)0;
+    const bool hit = inr && k == kend && (mval == qn || ins == qn + 1 || del == qn + 1);
+    const uint64_t hm = __builtin_amdgcn_ballot_w64(hit);
+    if (hm) {
+      const int32_t layer = mval == qn ? 0 : ins == qn + 1 ? 1 : 2;
+      const uint64_t h1 = __builtin_amdgcn_ballot_w64(hit && layer == 1);
+      const uint64_t h2 = __builtin_amdgcn_ballot_w64(hit && layer == 2);
+      if ((hm >> rsh) & RMASK) {
+        end_s = s;
+        end_layer = ((h1 >> rsh) & RMASK) ? 1 : ((h2 >> rsh) & RMASK) ? 2 : 0;
+        live = false;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j + 1 < MD; ++j) M[j] = M[j + 1];
+    M[MD - 1] = inr ? mval + 1 : 0;
+    I1 = inr ? ins : 0;
+    D1 = inr ? del : 0;
+  }
+  // A row that did not reach its end: beyond the cap it fails as upstream's
+  // search does, beyond the band alone it needs a wider pass.
+  const bool failed = has && end_s < 0 && max_step <= B;
+  const bool wider = has && end_s < 0 && max_step > B;
+
+  // Traceback of each row that found its end (affine_traceback,
+  // wfa_simd.rs:785-886), row-uniform state, gap runs resolved RW cells per
+  // ballot.
+  RunWriter w;
+  w.init(a.runs + pd.run_off, pd.run_cap, r == 0);
+  int32_t status = ST_OK;
+  bool tb = end_s >= 0;
+  int32_t qi = qn, ti = tn, k = kend, s = end_s, layer = end_layer;
+  while (__builtin_amdgcn_ballot_w64(tb)) {
+    if (tb) {
+      if (s == 0) {
+        w.emit(OP_M, min(qi, ti));
+        tb = false;
+      } else if (layer == 0) {
+        const int32_t rr = k - (B0 + DIR * s);
+        const uint32_t cell = (uint32_t)rr < (uint32_t)RW ? (uint32_t)hist[(uint32_t)s * RW + (uint32_t)rr] : 0u;
+        const int32_t pre = (int32_t)(cell >> 4) - 1;
+        if (pre < 0) {
+          status = ST_INTERNAL;
+          tb = false;
+        } else {
+          const uint32_t mop = cell & 3;
+          const int32_t run = max(qi - pre, 0);
+          w.emit(OP_M, run);
+          qi -= run;
+          ti -= run;
+          if (mop == 0) { w.emit(OP_X, 1); --qi; --ti; s -= XN; }
+          else layer = (int32_t)mop;
+        }
+      } else {
+        // Lane j of the row looks at step s - j on diagonal k + dir j and
+        // reports whether the gap was opened there (op bit 0).
+        const int32_t dir = layer == 1 ? -1 : 1;
+        const uint32_t bit = layer == 1 ? 2u : 3u;
+        const int32_t sj = s - r, kj = k + dir * r;
+        bool open = false;
+        if (sj >= 1) {
+          const int32_t rj = kj - (B0 + DIR * sj);
+          open = (uint32_t)rj < (uint32_t)RW ? ((hist[(uint32_t)sj * RW + (uint32_t)rj] >> bit) & 1u) == 0 : true;
+        }
+        const uint64_t mr = (__builtin_amdgcn_ballot_w64(open) >> rsh) & RMASK;
+        const int32_t n = mr ? (int32_t)__builtin_ctzll(mr) + 1 : RW;
+        w.emit(layer == 1 ? OP_D : OP_I, n);
+        if (layer == 1) qi -= n;
+        else ti -= n;
+        k += dir * n;
+        if (mr) { s -= (n - 1) + OEN; layer = 0; }
+        else s -= n;
+      }
+      if (tb && s < 0) { status = ST_INTERNAL; tb = false; }
+      if (tb && qi <= 0 && ti <= 0) tb = false;
+    }
+  }
+  w.flush();
+  if (has && !wider && r == 0) {
+    PairOut res{ST_ALIGNMENT_FAILED, 0, 0, 0, 0, 0, lim + 1, 0};
+    if (!failed) {
+      if (w.overflow && status == ST_OK) status = ST_OVERFLOW_RUNS;
+      res = PairOut{status, end_s * g, w.non_m, tn, (int32_t)w.n, (int32_t)(pd.run_cap - w.n), end_s,
+                    end_s * RW};
+    }
+    a.out[pi] = res;
+    if (res.status >= ST_OVERFLOW_HIST && a.overflow)
+      __hip_atomic_store(a.overflow, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+  return wider;
+}
+
 // --- kernel --------------------------------------------------------------------
 
 // Up to seven workgroups per CU (28 waves, 72 registers per lane): the
@@ -1507,6 +1795,77 @@ __global__ __launch_bounds__(64 * WV_WAVES, 7) void wfa_wave_kernel(WfaArgs a, W
       if (lane == 0) nx = atomicAdd(&a.counters[0], 1u);
       w = (uint32_t)readlane((int32_t)nx, 0) + nwaves;
     }
+  }
+}
+
+// Banded rows: one launch over a list of four segments (rows_seg): pairs of
+// entries in 32-lane rows, then quads in 16-lane rows, each kind with the
+// window going down (qn <= tn) and then up.  Wave v of the launch (v, v +
+// waves, ... when the launch has fewer waves than sets) takes set v of the
+// concatenated sets; the host orders each segment so that these strides
+// balance, and the 32-lane sets (the pairs with the most steps) start first.
+// A 16-lane row holds a pair only when the host has bounded its optimum by
+// the row's band, so one that does not finish is reported as too wide (and
+// re-run on the HBM tiers); a 32-lane row that does not finish goes to the
+// single-pair path, the whole wave, one pair at a time.  Up to five
+// workgroups per CU (102 registers per lane, no spills).
+template <bool B8>
+__global__ __launch_bounds__(64 * WV_WAVES, 5) void wfa_rows_kernel(WfaArgs a, WaveGeom G) {
+  extern __shared__ __align__(16) uint8_t rows_lds_pool[];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int32_t)(threadIdx.x >> 6));
+  uint8_t* base = rows_lds_pool + (size_t)wv * G.per_wave_bytes;
+  const uint32_t nwaves = gridDim.x * WV_WAVES;
+  uint32_t first[5], off[5];  // first set and first entry of each segment
+  first[0] = off[0] = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t per = j < 2 ? 2u : 4u;
+    first[j + 1] = first[j] + (a.rows_seg[j] + per - 1) / per;
+    off[j + 1] = off[j] + a.rows_seg[j];
+  }
+  const int32_t xn = a.cp.xn, oen = a.cp.oen;
+  for (uint32_t v = blockIdx.x * WV_WAVES + (uint32_t)wv; v < first[4]; v += nwaves) {
+    const int j = v < first[1] ? 0 : v < first[2] ? 1 : v < first[3] ? 2 : 3;
+    const uint32_t per = j < 2 ? 2u : 4u;
+    const uint32_t pos = off[j] + (v - first[j]) * per + (uint32_t)lane / (64u / per);
+    const uint32_t pi = pos < off[j + 1] ? a.work_list[pos] : 0xFFFFFFFFu;
+    bool wider;
+    if (j == 0) {
+      wider = xn == 2 && oen == 4 ? rows_pass<2, 4, -1, B8, 32>(a, G, base, lane, pi)
+            : xn == 2 && oen == 5 ? rows_pass<2, 5, -1, B8, 32>(a, G, base, lane, pi)
+                                  : rows_pass<1, 1, -1, B8, 32>(a, G, base, lane, pi);
+    } else if (j == 1) {
+      wider = xn == 2 && oen == 4 ? rows_pass<2, 4, 1, B8, 32>(a, G, base, lane, pi)
+            : xn == 2 && oen == 5 ? rows_pass<2, 5, 1, B8, 32>(a, G, base, lane, pi)
+                                  : rows_pass<1, 1, 1, B8, 32>(a, G, base, lane, pi);
+    } else if (j == 2) {
+      wider = xn == 2 && oen == 4 ? rows_pass<2, 4, -1, B8, 16>(a, G, base, lane, pi)
+            : xn == 2 && oen == 5 ? rows_pass<2, 5, -1, B8, 16>(a, G, base, lane, pi)
+                                  : rows_pass<1, 1, -1, B8, 16>(a, G, base, lane, pi);
+    } else {
+      wider = xn == 2 && oen == 4 ? rows_pass<2, 4, 1, B8, 16>(a, G, base, lane, pi)
+            : xn == 2 && oen == 5 ? rows_pass<2, 5, 1, B8, 16>(a, G, base, lane, pi)
+                                  : rows_pass<1, 1, 1, B8, 16>(a, G, base, lane, pi);
+    }
+    const uint32_t lead = (uint32_t)lane % (64u / per);  // 0 on the first lane of each row
+    if (j >= 2) {
+      if (wider && lead == 0) {
+        a.out[pi] = PairOut{ST_OVERFLOW_WIDTH, 0, 0, 0, 0, 0, 0, 0};
+        if (a.overflow) __hip_atomic_store(a.overflow, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      }
+    } else {
+      uint64_t f = __builtin_amdgcn_ballot_w64(wider && lead == 0);
+      while (f) {
+        const int32_t l = (int32_t)__builtin_ctzll(f);
+        f &= f - 1;
+        const uint32_t fpi = (uint32_t)readlane((int32_t)pi, l);
+        wave_sync<true>();
+        const PairDesc fpd = a.pairs[fpi];
+        wave_affine<true, B8, 64>(a, G, base, fpd, a.out + fpi, lane, nullptr);
+      }
+    }
+    wave_sync<true>();  // the wave's region is reused by its next set
   }
 }
 
@@ -1591,6 +1950,14 @@ hipError_t launch_wfa_wave(int mode, bool lds, const WfaArgs& a, const WaveGeom&
     if (lds) hipLaunchKernelGGL((wfa_wave_kernel<ENG_LINEAR, true>), dim3(grid), block, bytes, st, a, G);
     else hipLaunchKernelGGL((wfa_wave_kernel<ENG_LINEAR, false>), dim3(grid), block, 0, st, a, G);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_wfa_rows(bool b8, const WfaArgs& a, const WaveGeom& G, uint32_t grid, hipStream_t st) {
+  const dim3 block(64 * WV_WAVES);
+  const size_t bytes = (size_t)G.per_wave_bytes * WV_WAVES;
+  if (b8) hipLaunchKernelGGL((wfa_rows_kernel<true>), dim3(grid), block, bytes, st, a, G);
+  else hipLaunchKernelGGL((wfa_rows_kernel<false>), dim3(grid), block, bytes, st, a, G);
   return hipGetLastError();
 }
 

@@ -181,6 +181,73 @@ def test_caps(px, engine):
         compare(px, engine, qs, ts, costs=(2, 4, 1), cap=cap, label=f"cap{cap}")
 
 
+def rows_batch(seed, n, alphabet=b"ACGT"):
+    """Global-branch pairs around the banded-row limits: |qn - tn| from 0 to
+    60 in both directions (rows take |qn - tn| <= 48), error rates whose
+    optimum lands below and above the band bound of |qn - tn| + 15 steps."""
+    rng = random.Random(seed)
+    qs, ts = [], []
+    while len(qs) < n:
+        q = rand_seq(rng, rng.randint(1, 300), alphabet)
+        t = mutate(rng, q, rng.choice([0.0, 0.005, 0.02, 0.05, 0.12]), alphabet)
+        d = rng.randint(0, 60)
+        if rng.random() < 0.6:
+            t += rand_seq(rng, d, alphabet)
+        else:
+            q += rand_seq(rng, d, alphabet)
+        t = t[:len(q) + len(q) // 2 + 10]  # global branch
+        if q and t:
+            qs.append(q)
+            ts.append(t)
+    return qs, ts
+
+
+def rows_launches(px, qs, ts, costs, ctx=None, cap=None):
+    """(results, storage codes of the run's LDS-tier launches)."""
+    pb = px.pack_batch(qs, ts)
+    plan = px.Plan(pb, px.ENGINE_AFFINE, px.AffineCosts(*costs), cap, ctx)
+    try:
+        plan.run()
+        kinds = sorted(li["storage"] for li in plan.last_launch_info() if li["tier"] == 0)
+        return plan.results(), kinds
+    finally:
+        plan.close()
+
+
+@pytest.mark.parametrize("costs", [(4, 6, 2), (2, 4, 1), (1, 0, 1)])
+def test_banded_rows(px, costs):
+    """Banded rows (16- or 32-diagonal windows sliding with the score step,
+    four or two pairs per wave): both window directions, pairs whose optimum
+    lies inside and beyond the 16-lane band (routed to 32-lane rows) and
+    beyond the 32-lane band (finished on the one-pair path in the same
+    launch), 2-bit and byte encodings, caps inside and beyond the band, and
+    partial sets of rows; every result equals the oracle's and the one-pair-
+    per-wave path's bytes, with the routing on and with every pair in 32-lane
+    rows."""
+    off_ctx = px.Context(0, dict(lds_rows=2))
+    wide_ctx = px.Context(0, dict(lds_rows=3))
+    try:
+        for label, alphabet, n in (("2bit", b"ACGT", 803), ("bytes", b"ACGTNacgt", 301), ("tiny", b"ACGT", 3)):
+            qs, ts = rows_batch(1000 * costs[0] + 100 * costs[1] + n, n, alphabet)
+            if n < 4:  # a partial quad of pairs the rows take
+                qs, ts = map(list, zip(*[(q, t) for q, t in zip(*rows_batch(n, 40, alphabet))
+                                         if abs(len(q) - len(t)) <= 48][:n]))
+            for cap in (None, 3, 40):
+                res, kinds = rows_launches(px, qs, ts, costs, cap=cap)
+                assert any(k >= 3 for k in kinds), f"no banded-row launch: {kinds}"
+                compare(px, "affine", qs, ts, costs=costs, cap=cap, label=f"rows-{label}-cap{cap}", res=res)
+                off, off_kinds = rows_launches(px, qs, ts, costs, ctx=off_ctx, cap=cap)
+                assert all(k == 0 for k in off_kinds)
+                assert np.array_equal(res.aln, off.aln) and np.array_equal(res.runs, off.runs) \
+                    and np.array_equal(res.run_off, off.run_off)
+                wide, wide_kinds = rows_launches(px, qs, ts, costs, ctx=wide_ctx, cap=cap)
+                assert any(k >= 3 for k in wide_kinds)
+                assert np.array_equal(res.aln, wide.aln) and np.array_equal(res.runs, wide.runs)
+    finally:
+        off_ctx.close()
+        wide_ctx.close()
+
+
 def test_long_pairs_escalate_tiers(px):
     """Long and divergent pairs overflow the one-wave tier and are re-run on
     the wider tiers; results stay bit-exact."""
