@@ -150,7 +150,8 @@ struct WaveGeom {
 };
 
 // Bytes of one row region of `rw` lanes (16 or 32): sequences, then
-// (rows_steps + rw - 16) steps of rw u16 history cells.
+// (rows_steps + rw - 16) steps of rw u16 history cells (steps 0 .. the last
+// step a row can reach).
 __host__ __device__ inline uint32_t rows_region_bytes(uint32_t hist_off, uint32_t rows_steps, uint32_t rw) {
   return hist_off + (((rows_steps + rw - 16u) * rw * 2u + 15u) & ~15u);
 }

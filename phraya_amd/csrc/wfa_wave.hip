@@ -1587,6 +1587,33 @@ __device__ __forceinline__ int32_t extend_rows(const lmem_t<const uint64_t>* qw,
   return live ? p + min(n, lim) : p;
 }
 
+// Candidates of one cell (affine_step's recurrence and tie rules on P =
+// offset + 1 values, 0 = no cell): gap-open / gap-extend into the
+// query-only (ins) and target-only (del) layers, then the m layer's best of
+// mismatch, ins and del; ops as the history cell stores them.
+struct RowCell {
+  int32_t ins, del, best;
+  uint32_t ops;  // mop | iop << 2 | dop << 3
+};
+
+__device__ __forceinline__ RowCell row_cell(int32_t vx, int32_t vol, int32_t vor, int32_t vil, int32_t vdr,
+                                            int32_t qn, int32_t tn, int32_t k) {
+  const uint32_t uq = (uint32_t)qn, utk = (uint32_t)(tn + k);
+  RowCell c;
+  c.ins = (uint32_t)(vol - 1) < uq ? vol + 1 : 0;
+  uint32_t iop = 0;
+  if ((uint32_t)(vil - 1) < uq && vil + 1 > c.ins) { c.ins = vil + 1; iop = 1; }
+  c.del = (uint32_t)(vor - 1) <= utk ? vor : 0;
+  uint32_t dop = 0;
+  if ((uint32_t)(vdr - 1) <= utk && vdr > c.del) { c.del = vdr; dop = 1; }
+  c.best = (uint32_t)(vx - 1) < min(uq, utk) ? vx + 1 : 0;
+  uint32_t mop = 0;
+  if (c.ins > c.best) { c.best = c.ins; mop = 1; }
+  if (c.del > c.best) { c.best = c.del; mop = 2; }
+  c.ops = mop | (iop << 2) | (dop << 3);
+  return c;
+}
+
 // One pass of the wave's 64 / RW rows (pi = ~0u: empty row).  XN, OEN:
 // mismatch and open + extend in score steps (extend = 1); DIR: window
 // direction of every pair.  Returns true on the lanes of a row whose pair's
@@ -1635,50 +1662,42 @@ __device__ __forceinline__ bool rows_pass(const WfaArgs& a, const WaveGeom& G, u
     end_s = 0;
     live = false;
   }
-  for (int32_t s = 1;; ++s) {
-    live = live && s <= lim;
-    if (__builtin_amdgcn_ballot_w64(live) == 0) break;
-    const int32_t k = B0 + DIR * s + r;
-    const bool inr = live && (uint32_t)(k + tn) <= span;
-    const int32_t vx = row_read<DIR * XN, RW>(M[MD - XN], lane);
-    const int32_t vol = row_read<DIR * OEN - 1, RW>(M[MD - OEN], lane);
-    const int32_t vor = row_read<DIR * OEN + 1, RW>(M[MD - OEN], lane);
-    const int32_t vil = row_read<DIR - 1, RW>(I1, lane);
-    const int32_t vdr = row_read<DIR + 1, RW>(D1, lane);
-    // Candidate tests on P - 1 (the offset), as in affine_step.
-    const uint32_t uq = (uint32_t)qn, utk = (uint32_t)(tn + k);
-    int32_t ins = (uint32_t)(vol - 1) < uq ? vol + 1 : 0;
-    uint32_t iop = 0;
-    if ((uint32_t)(vil - 1) < uq && vil + 1 > ins) { ins = vil + 1; iop = 1; }
-    int32_t del = (uint32_t)(vor - 1) <= utk ? vor : 0;
-    uint32_t dop = 0;
-    if ((uint32_t)(vdr - 1) <= utk && vdr > del) { del = vdr; dop = 1; }
-    int32_t best = (uint32_t)(vx - 1) < min(uq, utk) ? vx + 1 : 0;
-    uint32_t mop = 0;
-    if (ins > best) { best = ins; mop = 1; }
-    if (del > best) { best = del; mop = 2; }
-    const int32_t mval = extend_rows<B8>(qw, tw, qn, tn, inr ? best - 1 : -1, k);
-    // Every row stores (s stays inside every region of the launch while
-    // any row is live); steps past a row's end are never read.
-    hist[(uint32_t)s * RW + (uint32_t)r] =
-        inr ? (uint16_t)(((uint32_t)best << 4) | mop | (iop << 2) | (dop << 3)) : (uint16_t)0;
-    const bool hit = inr && k == kend && (mval == qn || ins == qn + 1 || del == qn + 1);
+  // The end of a row: the first step whose end-diagonal cell reaches qn in
+  // some layer (m, then ins, then del).
+  auto end_found = [&](int32_t s, bool hit, int32_t mval, const RowCell& c) {
     const uint64_t hm = __builtin_amdgcn_ballot_w64(hit);
     if (hm) {
-      const int32_t layer = mval == qn ? 0 : ins == qn + 1 ? 1 : 2;
+      const int32_t layer = mval == qn ? 0 : c.ins == qn + 1 ? 1 : 2;
       const uint64_t h1 = __builtin_amdgcn_ballot_w64(hit && layer == 1);
       const uint64_t h2 = __builtin_amdgcn_ballot_w64(hit && layer == 2);
-      if ((hm >> rsh) & RMASK) {
+      if (live && ((hm >> rsh) & RMASK)) {
         end_s = s;
         end_layer = ((h1 >> rsh) & RMASK) ? 1 : ((h2 >> rsh) & RMASK) ? 2 : 0;
         live = false;
       }
     }
+  };
+  // Every row stores each step (s stays inside every region of the launch
+  // while any row is live); steps past a row's end are never read.
+  for (int32_t s = 1;; ++s) {
+    live = live && s <= lim;
+    if (__builtin_amdgcn_ballot_w64(live) == 0) break;
+    const int32_t k = B0 + DIR * s + r;
+    const bool inr = live && (uint32_t)(k + tn) <= span;
+    const RowCell c = row_cell(row_read<DIR * XN, RW>(M[MD - XN], lane),
+                               row_read<DIR * OEN - 1, RW>(M[MD - OEN], lane),
+                               row_read<DIR * OEN + 1, RW>(M[MD - OEN], lane),
+                               row_read<DIR - 1, RW>(I1, lane), row_read<DIR + 1, RW>(D1, lane), qn, tn, k);
+    const int32_t mval = extend_rows<B8>(qw, tw, qn, tn, inr ? c.best - 1 : -1, k);
+    const uint32_t cell = ((uint32_t)c.best << 4) | c.ops;
+    hist[(uint32_t)s * RW + (uint32_t)r] = (uint16_t)(inr ? cell : 0u);
+    const bool hit = inr & (k == kend) & ((mval == qn) | (c.ins == qn + 1) | (c.del == qn + 1));
+    end_found(s, hit, mval, c);
 #pragma unroll
     for (int j = 0; j + 1 < MD; ++j) M[j] = M[j + 1];
     M[MD - 1] = inr ? mval + 1 : 0;
-    I1 = inr ? ins : 0;
-    D1 = inr ? del : 0;
+    I1 = inr ? c.ins : 0;
+    D1 = inr ? c.del : 0;
   }
   // A row that did not reach its end: beyond the cap it fails as upstream's
   // search does, beyond the band alone it needs a wider pass.
