@@ -889,10 +889,11 @@ def run_serve(args):
     short_sv = _pct(lat_sv[rs.cls == 0], 99)
     short_pl5 = _pct(lat5[rs5.cls == 0], 99)
 
-    # --- optional: the plain path's capacity at the latency target, by
-    # lowering its offered rate until its short-read p99 meets 10 ms.
+    # --- the plain path's capacity at the latency target, by lowering its
+    # offered rate until its short-read p99 meets 10 ms (or the lowest rate
+    # tried).
     sweep = []
-    if args.serve_plain_sweep:
+    if not args.serve_no_sweep:
         for k in (10, 20, 40, 80, 160):
             rk = rate / k
             nk = max(200, int(rk * args.serve_seconds))
@@ -923,6 +924,25 @@ def run_serve(args):
     sv_rate = n / span_sv
     pl_rate = n / span_pl
     lat_sv_ms = lat_sv
+    # The target as measured: the served rate and p99s, and the best rate of
+    # the plain path among the runs that met a 10 ms short-read p99 (none:
+    # its capacity at the target is below the lowest rate tried).
+    plain_runs = [{"offered_req_per_s": rate, "pairs_per_s": pl_rate, "short_p99_ms": _pct(lat_pl[rs.cls == 0], 99)},
+                  {"offered_req_per_s": rate5, "pairs_per_s": n5 / float(done5.max()), "short_p99_ms": short_pl5}]
+    plain_runs += [{"offered_req_per_s": x["offered_req_per_s"], "pairs_per_s": x["pairs_per_s"],
+                    "short_p99_ms": x["short_p99_ms"]} for x in sweep]
+    met = [x for x in plain_runs if x["short_p99_ms"] is not None and x["short_p99_ms"] < 10.0]
+    best_plain = max((x["pairs_per_s"] for x in met), default=None)
+    lowest_tried = min(x["pairs_per_s"] for x in plain_runs)
+    # Algorithmic bytes of the served requests (packed inputs, score /
+    # distance / end, 4 B per CIGAR run) over the served span.
+    alg = 0
+    for i in range(n):
+        qn_i = int(rs.q_off[i + 1] - rs.q_off[i])
+        tn_i = int(rs.win[i])
+        nr = len(sv_runs[i][3]) if sv_runs[i] is not None else 0
+        alg += (qn_i + 3) // 4 + (tn_i + 3) // 4 + 12 + 4 * nr
+    achieved = alg / span_sv / 1e9
     out = {
         "metric": METRIC,
         "value": sv_rate,
@@ -950,17 +970,29 @@ def run_serve(args):
         "plain_batched_at_fifth_rate": {
             "offered_req_per_s": rate5, "requests": n5, "pairs_per_s": n5 / float(done5.max()),
             "batches": batches5, "latency_ms": _lat_summary(lat5, rs5.cls)},
-        "short_p99_target_ms": 10.0,
-        "served_short_p99_ms": short_sv,
-        "plain_short_p99_at_fifth_rate_ms": short_pl5,
-        "served_5x_plain_at_short_p99_target": bool(short_sv < 10.0 and short_pl5 > 10.0),
+        "latency_target": {
+            "short_p99_target_ms": 10.0,
+            "served_pairs_per_s": sv_rate,
+            "served_short_p99_ms": short_sv,
+            "served_all_p99_ms": _pct(lat_sv_ms, 99),
+            "plain_best_pairs_per_s_meeting_target": best_plain,
+            "plain_lowest_pairs_per_s_tried": lowest_tried,
+            "served_over_plain_at_target": (sv_rate / best_plain) if best_plain else None,
+            "served_over_plain_at_target_at_least": None if best_plain else sv_rate / lowest_tried,
+            "plain_runs": plain_runs,
+            "what": "short reads (the 60% class) have a 10 ms p99 target; a long read of 50 kbp at 8% "
+                    "takes 0.5-1 s of sequential score steps on its own, so the all-class p99 is "
+                    "that read's time"},
         "plain_sweep": sweep or None,
         "served_p99_over_plain_p99": _pct(lat_sv_ms, 99) / _pct(lat_pl, 99),
         "server_stats": stats,
         "status_counts": {str(k): int(v) for k, v in zip(*np.unique(st_sv, return_counts=True))},
         "parity": {"checked": len(chk), "mismatches": mism, "oracle_sample": "requests <= 5 kbp",
                    "served_vs_plain_compared": n, "served_vs_plain_mismatches": served_vs_plain_mism},
-        "roofline": None,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": alg,
+                     "what": "algorithmic bytes of every served request over the served span (many "
+                             "launches per lane and tier, so no single kernel)"},
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
@@ -984,8 +1016,8 @@ def main():
                     help="serve: offered load as a fraction of the measured server capacity")
     ap.add_argument("--serve-rate", type=float, default=0.0,
                     help="serve: offered requests/s (overrides --serve-load)")
-    ap.add_argument("--serve-plain-sweep", action="store_true",
-                    help="serve: also lower the plain path's rate until its short-read p99 meets 10 ms")
+    ap.add_argument("--serve-no-sweep", action="store_true",
+                    help="serve: skip lowering the plain path's rate until its short-read p99 meets 10 ms")
     ap.add_argument("--serve-lanes", type=int, default=3,
                     help="serve: launch groups in flight per tier")
     ap.add_argument("--tune", default="",
