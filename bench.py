@@ -30,6 +30,13 @@ WORKLOADS = {
     # name: (synth workload, engine, costs, description)
     "short150": ("short150", "affine", (4, 6, 2),
                  "10k x (150 bp read vs 170 bp window), affine WFA 4/6/2, score+CIGAR"),
+    "window300": ("window300", "affine", (2, 4, 1),
+                  "10k x (150 bp read vs 300 bp window = 2 x qlen, as extend_anchor cuts it; "
+                  "fitting branch), affine WFA with AffineCosts::default() 2/4/1 (Sensitive), "
+                  "score+CIGAR"),
+    "window300_myers": ("window300", "myers", (0, 0, 0),
+                        "10k x (150 bp read vs 300 bp window = 2 x qlen; fitting branch), Myers "
+                        "with CIGAR (Balanced / Fast, qlen <= 500)"),
     "myers150": ("myers150", "myers_dist", (0, 0, 0),
                  "1M x (150 bp read vs 180 bp window), Myers edit distance + end column"),
     "mixed": ("mixed", "affine", (4, 6, 2),
@@ -1144,7 +1151,8 @@ def main():
     if not args.no_cpu:
         log(f"cpu baseline ~{args.cpu_seconds:.0f} s")
         cpu = cpu_baseline(w, engine, costs, args.cpu_seconds,
-                           sample_n={"short150": None, "myers150": 20_000, "mixed": 200}[args.workload])
+                           sample_n={"short150": None, "window300": None, "window300_myers": None,
+                                     "myers150": 20_000, "mixed": 200}[args.workload])
 
     out = {
         "metric": METRIC,

@@ -4,6 +4,8 @@ Workloads (read = query, window = target; seed (0, 0)):
   short150   10,000 reads of 150 bp from a 5 Mbp uniform-ACGT genome, 1%
              substitutions + 0.1% indel events (length 1-3, geometric p=1/2),
              against the 170 bp window starting at the read's source position.
+  window300  10,000 reads drawn as short150's, against 300 bp windows (2 x qlen, the window the
+             executor cuts for an anchor): the fitting branch.
   myers150   1,000,000 reads of 150 bp, per-read event rate ~ U[0, 5%] (80%
              substitutions, 20% indels of length 1-3), against 180 bp windows.
   mixed      200,000 reads, lengths log-uniform in [100, 20000]; 70% class
@@ -159,6 +161,13 @@ WORKLOADS = {
                      class_a_frac=1.0, a_err_lo=0.011, a_err_hi=0.011,
                      a_indel_frac=0.001 / 0.011, b_err_lo=0, b_err_hi=0, b_indel_frac=0,
                      indel_mean=2.0, indel_max=3),
+    # Reads drawn as short150's, against the 2 x qlen = 300 bp windows extend_anchor
+    # cuts (executor.rs:551-553): the fitting branch.
+    "window300": dict(genome_seed=0x5EED0001, read_seed=0xC0FFEE01, n=10_000, len_lo=150,
+                      len_hi=150, log_uniform=0, window_scale=1.0, window_add=150,
+                      class_a_frac=1.0, a_err_lo=0.011, a_err_hi=0.011,
+                      a_indel_frac=0.001 / 0.011, b_err_lo=0, b_err_hi=0, b_indel_frac=0,
+                      indel_mean=2.0, indel_max=3),
     "myers150": dict(genome_seed=0x5EED0001, read_seed=0xC0FFEE02, n=1_000_000, len_lo=150,
                      len_hi=150, log_uniform=0, window_scale=1.0, window_add=30,
                      class_a_frac=1.0, a_err_lo=0.0, a_err_hi=0.05, a_indel_frac=0.2,
