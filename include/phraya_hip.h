@@ -236,6 +236,12 @@ uint64_t phx_format_cigars(const uint32_t* runs, const uint64_t* run_off, uint64
 /* Per-pair forms of the upstream API.  The CIGAR text is written to cigar
  * (NUL-terminated, truncated to cigar_cap); the return value is the pair
  * status or a PHX_E_* code.  cigar_len (may be NULL) gets the full length. */
+/* Name of the implementation the calling thread's last per-pair call ran,
+ * "hip-<arch>" (e.g. "hip-gfx950"), or "" before any: the device counterpart
+ * of upstream's thread-local LAST_IMPL / get_active_dispatch_target
+ * (phraya-align/src/wfa_simd.rs:25-29, :181-184), kept per thread like it.
+ * The string lives as long as the context that ran the call. */
+const char* phx_last_impl(void);
 int phx_wfa_extend(phx_ctx* ctx, const uint8_t* q, size_t qlen, const uint8_t* t, size_t tlen,
                    uint64_t seed_q, uint64_t seed_t, phx_alignment* out, char* cigar,
                    size_t cigar_cap, size_t* cigar_len);

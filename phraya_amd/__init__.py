@@ -378,6 +378,13 @@ def myers_extend(query: bytes, target: bytes, seed: SeedAnchor = SeedAnchor()) -
     return _one(ENGINE_MYERS, query, target, seed)
 
 
+def get_device_dispatch_target() -> str:
+    """Implementation the calling thread's last per-pair C-ABI call ran
+    ("hip-<arch>", "" before any; phx_last_impl): the device counterpart of
+    upstream's get_active_dispatch_target (wfa_simd.rs:25-29, :181-184)."""
+    return L.lib().phx_last_impl().decode()
+
+
 def myers_edit_distance(query: bytes, target: bytes) -> tuple[int, str]:
     """myers_edit_distance (lib.rs:94-96): global distance and CIGAR."""
     res = align_batch(ENGINE_MYERS_GLOBAL, [bytes(query)], [bytes(target)])

@@ -35,6 +35,7 @@ EXPORTS = (
     "phx_plan_algorithmic_bytes", "phx_plan_destroy", "phx_format_cigar", "phx_format_cigars",
     "phx_plan_set_tuning", "phx_plan_get_tuning", "phx_plan_autotune", "phx_plan_use_graph",
     "phx_wfa_extend", "phx_wfa_extend_affine", "phx_myers_extend", "phx_myers_edit_distance",
+    "phx_last_impl",
     "phx_serve_start", "phx_serve_submit", "phx_serve_poll", "phx_serve_stats_get",
     "phx_serve_stop", "phx_estimate_request", "phx_contig_anchors", "phx_contig_plan_create",
     "phx_contig_plan_run", "phx_contig_plan_results", "phx_contig_plan_gaps",
@@ -219,6 +220,8 @@ def lib() -> ctypes.CDLL:
     L.phx_ctx_destroy.argtypes = [P]
     L.phx_ctx_last_error.argtypes = [P]
     L.phx_ctx_last_error.restype = ctypes.c_char_p
+    L.phx_last_impl.argtypes = []
+    L.phx_last_impl.restype = ctypes.c_char_p
     L.phx_cigar_bound.argtypes = [P]
     L.phx_cigar_bound.restype = ctypes.c_uint64
     L.phx_align_batch.argtypes = [P, P, P]

@@ -1361,7 +1361,13 @@ phx_ctx* phx_ctx_create(int device, uint32_t flags, const phx_tuning* tuning) {
     if (hipGetDevice(&c->device) != hipSuccess) { delete c; return nullptr; }
   }
   hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
+  if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) {
+    c->cu_count = prop.multiProcessorCount;
+    // "hip-gfx950" from "gfx950:sramecc+:xnack-".
+    std::string arch(prop.gcnArchName);
+    arch = arch.substr(0, arch.find(':'));
+    std::snprintf(c->impl_name, sizeof c->impl_name, "hip-%s", arch.c_str());
+  }
   // The allocation-free entry point's scratch on this device (its address is
   // looked up once per device, here, with the device current).
   c->noalloc_scratch = noalloc_scratch_of(c->device);
@@ -1949,10 +1955,15 @@ uint64_t phx_format_cigars(const uint32_t* runs, const uint64_t* run_off, uint64
   return len;
 }
 
+// Implementation the calling thread's last per-pair call ran (upstream keeps
+// the same per thread in LAST_IMPL).
+static thread_local const char* t_last_impl = "";
+
 static int one_pair(phx_ctx* ctx, int engine, const uint8_t* q, size_t qlen, const uint8_t* t,
                     size_t tlen, uint64_t sq, uint64_t st, phx_costs costs, phx_alignment* out,
                     char* cigar, size_t cigar_cap, size_t* cigar_len) {
   if (!ctx || !out) return PHX_E_ARG;
+  t_last_impl = ctx->impl_name;
   static const uint8_t kEmpty = 0;
   const uint64_t qo[2] = {0, qlen}, to[2] = {0, tlen};
   phx_batch b{};
@@ -1975,6 +1986,8 @@ static int one_pair(phx_ctx* ctx, int engine, const uint8_t* q, size_t qlen, con
   if (cigar_len) *cigar_len = len;
   return out->status;
 }
+
+const char* phx_last_impl(void) { return t_last_impl; }
 
 int phx_wfa_extend(phx_ctx* ctx, const uint8_t* q, size_t qlen, const uint8_t* t, size_t tlen,
                    uint64_t seed_q, uint64_t seed_t, phx_alignment* out, char* cigar,

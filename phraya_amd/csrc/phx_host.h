@@ -642,6 +642,7 @@ struct phx_ctx {
   std::mutex mu;
   phx_tuning tun{};
   int cu_count = 256;
+  char impl_name[64] = "hip";           // "hip-<arch>" (phx_last_impl)
   uint8_t* noalloc_scratch = nullptr;   // device address of this device's allocation-free scratch
   PlanCache* plan_cache = nullptr;      // buffers of one-shot plans, reused by the next one
   std::mutex cache_mu;

@@ -31,6 +31,17 @@ def test_library_exports_every_declared_symbol():
     assert lib.phx_version().decode().startswith("phraya_hip")
 
 
+def test_last_impl_is_empty_before_any_per_pair_call():
+    """phx_last_impl is per thread (as upstream's LAST_IMPL): a thread that
+    has made no per-pair call reads ""."""
+    import threading
+    got = []
+    t = threading.Thread(target=lambda: got.append(L.lib().phx_last_impl()))
+    t.start()
+    t.join()
+    assert got == [b""]
+
+
 def test_format_cigar_host_helper():
     lib = L.lib()
     runs = np.array([(150 << 4) | 0, (20 << 4) | 2, (1 << 4) | 1, (3 << 4) | 3], dtype=np.uint32)

@@ -313,6 +313,36 @@ def test_mixed_subset(px):
     compare(px, "affine", qs, ts, costs=(4, 6, 2), label="mixed")
 
 
+def test_per_pair_entry_points_and_dispatch_name(px):
+    """The C per-pair entry points (what an unchanged extend_anchor binds)
+    equal the batched path, size the CIGAR text as documented (full length
+    returned when the buffer is short), and name the implementation they ran
+    per thread (phx_last_impl, upstream's LAST_IMPL)."""
+    import ctypes
+    from phraya_amd import _lib as L
+    lib = L.lib()
+    ctx = px.default_context()
+    qs, ts, _ = random_batch(31, 40, 150)
+    ref = px.align_batch(1, qs, ts, costs=px.AffineCosts(2, 4, 1))
+    for i, (q, t) in enumerate(zip(qs, ts)):
+        out = L.CAlignment()
+        cap = 8
+        buf = ctypes.create_string_buffer(cap)
+        n = ctypes.c_size_t(0)
+        rc = lib.phx_wfa_extend_affine(ctx.handle, q, len(q), t, len(t), 0, 0, L.Costs(2, 4, 1),
+                                       ctypes.byref(out), buf, cap, ctypes.byref(n))
+        assert rc == int(ref.aln["status"][i])
+        if rc != 0:
+            continue
+        if n.value >= cap:  # one retry with the full length, as the Rust glue does
+            buf = ctypes.create_string_buffer(n.value + 1)
+            rc = lib.phx_wfa_extend_affine(ctx.handle, q, len(q), t, len(t), 0, 0, L.Costs(2, 4, 1),
+                                           ctypes.byref(out), buf, n.value + 1, ctypes.byref(n))
+        assert buf.value.decode() == ref.cigar(i) and n.value == len(ref.cigar(i))
+        assert out.score == int(ref.aln["score"][i]) and out.target_end == int(ref.aln["target_end"][i])
+    assert px.get_device_dispatch_target().startswith("hip-gfx9")
+
+
 def test_deterministic_and_isolated(px):
     """Same batch twice gives identical bytes; a pair alone equals the same
     pair inside a mixed batch."""
