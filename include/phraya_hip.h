@@ -539,6 +539,46 @@ int phx_seed_and_chain_batch(phx_ctx* ctx, const phx_seed_index* index, uint64_t
                              const uint32_t* q_positions, const uint64_t* q_off, uint32_t nq,
                              phx_chain_results* out);
 
+/* --- Minimizer sketches ------------------------------------------------------
+ *
+ * sketch_alphabet (phraya-core/src/types.rs:1846-1867) for a batch of
+ * sequences: canonical (k, w) minimizers (canonical = 1: Alphabet::Dna, the
+ * sketch / sketch_sequence entry points, types.rs:1873-1900) or forward ones
+ * (canonical = 0: Alphabet::Protein), as MinimizerSketch's (value, position)
+ * lists (types.rs:1801-1809).  Upstream takes them from simd-minimizers 2.3;
+ * this computes the same scheme (2-bit ASCII code (b >> 1) & 3, k-mer value
+ * first base most significant, canonical = min with the reverse complement's
+ * value, 64-bit hash order, per-window leftmost or, for canonical windows
+ * with at most half T/G, rightmost minimum, consecutive repeats reported
+ * once) with its own hash, so values and positions follow that definition
+ * and not the crate's bit for bit.  Sequences are concatenated in seqs,
+ * sequence i at seq_off[i] .. seq_off[i + 1] (each under 2^32 bytes);
+ * 1 <= k <= 32, 1 <= w <= 128, k + w - 1 odd for canonical sketches (the
+ * crate's own requirement).  A sequence shorter than k + w - 1 has an empty
+ * sketch. */
+typedef struct phx_sketch_plan phx_sketch_plan;
+
+/* Resident form: create uploads the sequences and lays out the tasks; run
+ * is three launches (count per task, scan, write); results downloads:
+ * off[0 .. nseq] (sequence i's entries at off[i] .. off[i + 1]) and, when
+ * cap >= the total, values / positions.  *total is set also on
+ * PHX_E_CAPACITY.  capacity is an upper bound on the total (the number of
+ * windows). */
+phx_sketch_plan* phx_sketch_plan_create(phx_ctx* ctx, const uint8_t* seqs, const uint64_t* seq_off,
+                                        uint32_t nseq, uint32_t k, uint32_t w, int32_t canonical);
+int phx_sketch_plan_run(phx_sketch_plan* plan);
+int phx_sketch_plan_results(phx_sketch_plan* plan, uint64_t* values, uint32_t* positions, uint64_t cap,
+                            uint64_t* off, uint64_t* total);
+uint64_t phx_sketch_plan_total(phx_sketch_plan* plan);
+uint64_t phx_sketch_plan_capacity(const phx_sketch_plan* plan);
+/* Kernel time of the last run (events around its launches), milliseconds. */
+double phx_sketch_plan_last_kernel_ms(const phx_sketch_plan* plan);
+void phx_sketch_plan_destroy(phx_sketch_plan* plan);
+/* One call: create, run, results, destroy. */
+int phx_sketch_batch(phx_ctx* ctx, const uint8_t* seqs, const uint64_t* seq_off, uint32_t nseq, uint32_t k,
+                     uint32_t w, int32_t canonical, uint64_t* values, uint32_t* positions, uint64_t cap,
+                     uint64_t* off, uint64_t* total);
+
 /* --- align_read's strand handling for a batch --------------------------------
  *
  * phraya-align/src/executor.rs:573-733 after seeding and chaining: each read

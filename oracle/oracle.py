@@ -106,6 +106,8 @@ def lib():
         L.orc_seed_and_chain_timed.argtypes = [P, P, u64, u64, P, P, P, u32, P, ctypes.c_int,
                                                ctypes.c_int, P]
         L.orc_seed_and_chain_timed.restype = ctypes.c_double
+        L.orc_sketch.argtypes = [P, u64, u32, u32, ctypes.c_int, P, P]
+        L.orc_sketch.restype = i64
         _lib = L
     return _lib
 
@@ -333,3 +335,17 @@ def seed_and_chain_timed(target_mins, queries, max_occ=2**64 - 1, params=None, t
                                                 qv.ctypes.data, qp.ctypes.data, q_off.ctypes.data,
                                                 len(queries), ctypes.byref(params),
                                                 threads or hardware_threads(), passes, ctypes.byref(chk)))
+
+
+def sketch(seq, k: int = 21, w: int = 11, canonical: bool = True):
+    """Minimizer sketch (sketch_alphabet, phraya-core/src/types.rs:1846-1867)
+    of one byte sequence: (values u64, positions u32) in window order; the
+    scheme is stated in phraya_oracle.cpp."""
+    s = np.frombuffer(bytes(seq), np.uint8) if not isinstance(seq, np.ndarray) else np.ascontiguousarray(seq, np.uint8)
+    cap = max(1, len(s) - k - w + 2)
+    vals = np.zeros(cap, np.uint64)
+    pos = np.zeros(cap, np.uint32)
+    n = lib().orc_sketch(s.ctypes.data, len(s), k, w, 1 if canonical else 0, vals.ctypes.data, pos.ctypes.data)
+    if n < 0:
+        raise ValueError("orc_sketch: invalid k / w")
+    return vals[:n], pos[:n]

@@ -1325,3 +1325,82 @@ double orc_seed_and_chain_timed(const uint64_t* t_vals, const uint32_t* t_pos, u
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Minimizer sketch (sketch_alphabet, phraya-core/src/types.rs:1846-1867).
+//
+// Upstream takes it from the simd-minimizers crate (2.3, phraya-core/
+// Cargo.toml:9), whose source is not in the reference; this restates the
+// scheme that crate publishes for canonical and forward minimizers over an
+// ASCII sequence, as far as it is fixed by its description:
+//   * base code (b >> 1) & 3 (A 0, C 1, T 2, G 3; other bytes by the same
+//     rule), complement code ^ 2;
+//   * k-mer value: the 2-bit codes with the first base most significant; a
+//     canonical sketch takes the smaller of that and the reverse
+//     complement's value;
+//   * order: a 64-bit hash of the value (here the SplitMix64 finalizer);
+//   * each window of w consecutive k-mers selects its smallest hash;
+//     forward sketches take the leftmost of equal hashes; canonical ones
+//     take the leftmost when more than half of the window's l = k + w - 1
+//     bases are T or G and the rightmost otherwise (l odd), so a sequence
+//     and its reverse complement select mirrored positions;
+//   * consecutive windows selecting the same k-mer report it once.
+// Exact agreement with the crate's hash and values is not pinned (no
+// fixture of its output exists in the reference); what is pinned is the
+// window rule (brute force), the strand symmetry and, for forward sketches,
+// agreement with a second implementation.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+inline uint64_t orc_mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t orc_sketch(const uint8_t* s, uint64_t n, uint32_t k, uint32_t w, int canonical, uint64_t* vals,
+                   uint32_t* pos) {
+  if (k == 0 || k > 32 || w == 0) return -1;
+  const uint64_t l = (uint64_t)k + w - 1;
+  if (canonical && l % 2 == 0) return -1;
+  if (n < l) return 0;
+  const uint64_t nk = n - k + 1, nwin = n - l + 1;
+  std::vector<uint64_t> V(nk), H(nk);
+  for (uint64_t i = 0; i < nk; ++i) {
+    uint64_t F = 0, R = 0;
+    for (uint32_t j = 0; j < k; ++j) {
+      const uint64_t c = (s[i + j] >> 1) & 3;
+      F = (F << 2) | c;
+      R |= (c ^ 2) << (2 * j);
+    }
+    V[i] = canonical ? std::min(F, R) : F;
+    H[i] = orc_mix64(V[i]);
+  }
+  int64_t cnt = 0;
+  uint64_t last = ~0ull;
+  for (uint64_t x = 0; x < nwin; ++x) {
+    bool left = true;
+    if (canonical) {
+      uint64_t tg = 0;
+      for (uint64_t j = x; j < x + l; ++j) tg += ((s[j] >> 1) & 3) >> 1;
+      left = 2 * tg > l;
+    }
+    uint64_t best = x;
+    for (uint64_t i = x + 1; i < x + w; ++i)
+      if (H[i] < H[best] || (!left && H[i] == H[best])) best = i;
+    if (best != last) {
+      vals[cnt] = V[best];
+      pos[cnt] = (uint32_t)best;
+      ++cnt;
+      last = best;
+    }
+  }
+  return cnt;
+}
+
+}  // extern "C"

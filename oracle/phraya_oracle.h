@@ -101,6 +101,14 @@ void orc_batch_free(orc_batch* b);
 
 int orc_hardware_threads(void);
 
+/* Minimizer sketch of s[0, n) (sketch_alphabet, phraya-core/src/types.rs:
+ * 1846-1867): (value, position) of each selected k-mer in window order,
+ * canonical (DNA) or forward; vals / pos hold at least n - k - w + 2 entries.
+ * Returns the count, or -1 for k outside 1..32, w = 0, or a canonical sketch
+ * with k + w - 1 even.  The scheme is stated in phraya_oracle.cpp. */
+int64_t orc_sketch(const uint8_t* s, uint64_t n, uint32_t k, uint32_t w, int canonical, uint64_t* vals,
+                   uint32_t* pos);
+
 #ifdef __cplusplus
 }
 #endif

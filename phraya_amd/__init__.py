@@ -946,6 +946,113 @@ def seed_and_chain(target_mins, queries, max_occ: int = 2 ** 64 - 1, params=None
         ix.close()
 
 
+# --- minimizer sketches --------------------------------------------------------
+
+DEFAULT_K, DEFAULT_W = 21, 11                   # types.rs:1743-1746
+DEFAULT_K_PROTEIN, DEFAULT_W_PROTEIN = 6, 5     # types.rs:1754-1757
+
+
+def _concat(seqs):
+    arrs = [np.frombuffer(bytes(s), np.uint8) if not isinstance(s, np.ndarray) else
+            np.ascontiguousarray(s, np.uint8) for s in seqs]
+    off = np.zeros(len(arrs) + 1, np.uint64)
+    if arrs:
+        off[1:] = np.cumsum([len(a) for a in arrs])
+    buf = np.concatenate(arrs) if arrs else np.zeros(0, np.uint8)
+    return np.ascontiguousarray(buf), off
+
+
+class SketchPlan:
+    """sketch_alphabet (phraya-core/src/types.rs:1846-1867) for a batch of
+    sequences, resident on the device (phx_sketch_plan_*): canonical
+    minimizers for DNA, forward ones for protein."""
+
+    def __init__(self, seqs, k: int = DEFAULT_K, w: int = DEFAULT_W, canonical: bool = True,
+                 ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self._buf, self._off = _concat(seqs)
+        self.n = len(self._off) - 1
+        self._h = L.lib().phx_sketch_plan_create(self.ctx.handle, self._buf.ctypes.data if len(self._buf) else None,
+                                                 self._off.ctypes.data, self.n, k, w, 1 if canonical else 0)
+        if not self._h:
+            raise ValueError(f"phx_sketch_plan_create failed: {self.ctx.last_error()}")
+        _track(self, 0)
+
+    def run(self) -> None:
+        rc = L.lib().phx_sketch_plan_run(self._h)
+        if rc != L.PHX_OK:
+            raise RuntimeError(f"phx_sketch_plan_run failed ({rc}): {self.ctx.last_error()}")
+
+    def last_kernel_ms(self) -> float:
+        return float(L.lib().phx_sketch_plan_last_kernel_ms(self._h))
+
+    @property
+    def capacity(self) -> int:
+        return int(L.lib().phx_sketch_plan_capacity(self._h))
+
+    def total(self) -> int:
+        return int(L.lib().phx_sketch_plan_total(self._h))
+
+    def results(self):
+        """(values u64, positions u32, off u64[n + 1]): sequence i's
+        minimizers at off[i] .. off[i + 1], in window order."""
+        off = np.zeros(self.n + 1, np.uint64)
+        total = ctypes.c_uint64(0)
+        rc = L.lib().phx_sketch_plan_results(self._h, None, None, 0, off.ctypes.data, ctypes.byref(total))
+        vals = np.zeros(max(1, total.value), np.uint64)
+        pos = np.zeros(max(1, total.value), np.uint32)
+        if rc in (L.PHX_OK, L.PHX_E_CAPACITY):
+            rc = L.lib().phx_sketch_plan_results(self._h, vals.ctypes.data, pos.ctypes.data, len(vals),
+                                                 off.ctypes.data, ctypes.byref(total))
+        if rc != L.PHX_OK:
+            raise RuntimeError(f"phx_sketch_plan_results failed ({rc}): {self.ctx.last_error()}")
+        return vals[:total.value], pos[:total.value], off
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            L.lib().phx_sketch_plan_destroy(self._h)
+            self._h = None
+
+    def _release(self) -> None:
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def sketch_batch(seqs, k: int = DEFAULT_K, w: int = DEFAULT_W, canonical: bool = True,
+                 ctx: Optional[Context] = None):
+    """Sketches of many sequences in one device pass: a list of (n, 2) u64
+    arrays of (value, position) rows, MinimizerSketch's layout."""
+    p = SketchPlan(seqs, k, w, canonical, ctx)
+    try:
+        p.run()
+        v, q, off = p.results()
+    finally:
+        p.close()
+    out = []
+    for i in range(len(off) - 1):
+        a, b = int(off[i]), int(off[i + 1])
+        out.append(np.stack([v[a:b], q[a:b].astype(np.uint64)], axis=1) if b > a else np.zeros((0, 2), np.uint64))
+    return out
+
+
+def sketch_alphabet(seq, k: int, w: int, alphabet: str = "dna", ctx: Optional[Context] = None):
+    """sketch_alphabet (types.rs:1846-1867): canonical minimizers for "dna",
+    forward ones for "protein"."""
+    if alphabet not in ("dna", "protein"):
+        raise ValueError("alphabet must be 'dna' or 'protein'")
+    return sketch_batch([seq], k, w, alphabet == "dna", ctx)[0]
+
+
+def sketch(seq, k: int = DEFAULT_K, w: int = DEFAULT_W, ctx: Optional[Context] = None):
+    """sketch (types.rs:1873-1877): canonical DNA minimizers."""
+    return sketch_alphabet(seq, k, w, "dna", ctx)
+
+
 # --- align_read's strand handling ---------------------------------------------
 
 STRAND_FORWARD, STRAND_REVERSE = 0, 1

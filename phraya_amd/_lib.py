@@ -47,6 +47,9 @@ EXPORTS = (
     "phx_seed_index_create", "phx_seed_occurrence_cap", "phx_seed_index_destroy", "phx_chain_plan_create",
     "phx_chain_plan_run", "phx_chain_plan_results", "phx_chain_plan_last_kernel_ms", "phx_chain_plan_seeds",
     "phx_chain_plan_destroy", "phx_seed_and_chain_batch",
+    "phx_sketch_plan_create", "phx_sketch_plan_run", "phx_sketch_plan_results", "phx_sketch_plan_total",
+    "phx_sketch_plan_capacity", "phx_sketch_plan_last_kernel_ms", "phx_sketch_plan_destroy",
+    "phx_sketch_batch",
     "phx_score_alignments", "phx_reverse_complement", "phx_align_reads_bounds",
     "phx_align_reads_batch", "phx_variant_plan_create", "phx_variant_plan_run",
     "phx_variant_plan_results", "phx_variant_plan_last_kernel_ms", "phx_variant_plan_destroy",
@@ -271,6 +274,21 @@ def lib() -> ctypes.CDLL:
     L.phx_chain_plan_destroy.argtypes = [P]
     L.phx_chain_plan_destroy.restype = None
     L.phx_seed_and_chain_batch.argtypes = [P, P, ctypes.c_uint64, P, P, P, P, ctypes.c_uint32, P]
+    L.phx_sketch_plan_create.argtypes = [P, P, P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                         ctypes.c_int32]
+    L.phx_sketch_plan_create.restype = P
+    L.phx_sketch_plan_run.argtypes = [P]
+    L.phx_sketch_plan_results.argtypes = [P, P, P, ctypes.c_uint64, P, P]
+    L.phx_sketch_plan_total.argtypes = [P]
+    L.phx_sketch_plan_total.restype = ctypes.c_uint64
+    L.phx_sketch_plan_capacity.argtypes = [P]
+    L.phx_sketch_plan_capacity.restype = ctypes.c_uint64
+    L.phx_sketch_plan_last_kernel_ms.argtypes = [P]
+    L.phx_sketch_plan_last_kernel_ms.restype = ctypes.c_double
+    L.phx_sketch_plan_destroy.argtypes = [P]
+    L.phx_sketch_plan_destroy.restype = None
+    L.phx_sketch_batch.argtypes = [P, P, P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32,
+                                   P, P, ctypes.c_uint64, P, P]
     L.phx_reverse_complement.argtypes = [P, ctypes.c_uint64, P]
     L.phx_reverse_complement.restype = None
     L.phx_align_reads_bounds.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, P,

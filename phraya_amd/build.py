@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libphraya_hip.so")
 SYNTH_LIB = os.path.join(HERE, "libphx_synth.so")
 SOURCES = ["wfa_wave.hip", "myers_kernels.hip", "phx_api.hip", "phx_serve.hip", "phx_contig.hip",
-           "phx_executor.hip", "phx_variants.hip", "phx_chain.hip"]
+           "phx_executor.hip", "phx_variants.hip", "phx_chain.hip", "phx_sketch.hip"]
 ARCH = os.environ.get("PHX_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
