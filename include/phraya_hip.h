@@ -717,6 +717,92 @@ void phx_variant_plan_destroy(phx_variant_plan* plan);
  * plan creation would refuse (the reason in phx_ctx_last_error). */
 int phx_extract_variants(phx_ctx* ctx, const phx_variant_batch* batch, phx_variant_results* out);
 
+/* --- Output files: .phraya and .phraya.queries --------------------------------
+ *
+ * write_phraya (phraya-io/src/phraya.rs:67-80): the MessagePack form of
+ * PhrayaFile::new(reference_length, sample_id, timestamp, observations,
+ * CoverageTrack::new(coverage)) (phraya.rs:40-62, types.rs:626-651),
+ * zstd-compressed at `level` (upstream: 3).  The observations are the device
+ * variant records (phx_variant_results, in their order) made into
+ * VariantObservations as extract_variants_from_cigar does
+ * (phraya-align/src/executor.rs:814-1044): alleles {alt: 1} for an SNV,
+ * {'.': 1} for a deletion, inserted-base counts for an insertion; confidence
+ * 0 in a hotspot; the per-read fields from reads[record.alignment].
+ * Host code; PHX_E_ARG for inconsistent inputs, PHX_E_CAPACITY (size set)
+ * when cap is short, PHX_E_DEVICE when libzstd.so.1 cannot be loaded. */
+typedef struct phx_phraya_header {
+  uint32_t reference_length;
+  const char* sample_id;          /* NUL-terminated UTF-8 */
+  const char* timestamp;
+} phx_phraya_header;
+
+typedef struct phx_mate_info {    /* MateInfo (phraya-core/src/types.rs:2031-2050) */
+  const char* mate_id;
+  int32_t insert_size;
+  uint8_t proper_pair;
+  uint8_t is_first_in_pair;
+  uint8_t is_second_in_pair;
+  uint8_t mate_mapped;
+} phx_mate_info;
+
+typedef struct phx_obs_read {     /* extract_variants_from_cigar's per-read arguments (executor.rs:814-831) */
+  const char* cigar;              /* the alignment's CIGAR string */
+  const char* provenance;         /* e.g. "sample1:read42" */
+  const uint8_t* query;           /* the aligned (oriented) query bytes: insertion alleles */
+  uint64_t query_len;
+  double avg_base_quality;
+  double confidence;
+  int64_t insert_size_sum;        /* insert_stats, used with has_insert_stats and a mate */
+  uint32_t insert_size_count;
+  uint32_t edit_distance;
+  uint8_t mapq;
+  uint8_t strand;                 /* PHX_STRAND_FORWARD "+", PHX_STRAND_REVERSE "-" */
+  uint8_t has_insert_stats;
+  uint8_t reserved;
+  const phx_mate_info* mate;      /* NULL: mate_info None (the field is omitted) */
+} phx_obs_read;
+
+int phx_zstd_available(void);
+/* The uncompressed MessagePack payload (what zstd decodes the file to). */
+int phx_phraya_encode(const phx_phraya_header* header, const phx_variant* variants, uint64_t n_variants,
+                      const uint32_t* window_coverage, uint64_t n_window_coverage, const phx_obs_read* reads,
+                      uint32_t n_reads, const uint32_t* coverage, uint64_t coverage_len, uint8_t* out,
+                      uint64_t cap, uint64_t* size);
+int phx_phraya_write(const char* path, const phx_phraya_header* header, const phx_variant* variants,
+                     uint64_t n_variants, const uint32_t* window_coverage, uint64_t n_window_coverage,
+                     const phx_obs_read* reads, uint32_t n_reads, const uint32_t* coverage, uint64_t coverage_len,
+                     int level);
+
+/* write_queries (phraya-io/src/queries.rs:62-103): per read its
+ * (position, score ratio) placements; kept are scores >= 0.95, reads left
+ * with none are dropped, ids ascending, placements by position then score
+ * descending.  Ids must be distinct (a map key each). */
+typedef struct phx_query_hits {
+  const char* query_id;
+  const uint32_t* pos;
+  const double* score;
+  uint32_t n;
+} phx_query_hits;
+int phx_queries_encode(const phx_query_hits* queries, uint32_t n, uint8_t* out, uint64_t cap, uint64_t* size);
+int phx_queries_write(const char* path, const phx_query_hits* queries, uint32_t n, int level);
+
+/* write_cross_space_queries (queries.rs:132-167): CrossSpacePlacement
+ * (space, pos, identity) lists, identity >= 0.95 kept, ordered by space,
+ * position, identity descending. */
+typedef struct phx_placement {
+  const char* space;
+  uint32_t pos;
+  double identity;
+} phx_placement;
+typedef struct phx_cross_space_hits {
+  const char* query_id;
+  const phx_placement* placements;
+  uint32_t n;
+} phx_cross_space_hits;
+int phx_cross_space_queries_encode(const phx_cross_space_hits* queries, uint32_t n, uint8_t* out, uint64_t cap,
+                                   uint64_t* size);
+int phx_cross_space_queries_write(const char* path, const phx_cross_space_hits* queries, uint32_t n, int level);
+
 #ifdef __cplusplus
 }
 #endif

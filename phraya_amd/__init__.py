@@ -1258,3 +1258,139 @@ def extract_variants(runs, run_off, target_start, queries, target, spans, repeat
         return p.results()
     finally:
         p.close()
+
+
+# --- output files: .phraya and .phraya.queries (host code) ----------------------
+
+@dataclass
+class ReadMeta:
+    """extract_variants_from_cigar's per-read arguments (executor.rs:814-831)
+    for the observations of one alignment."""
+    cigar: str
+    provenance: str
+    query: bytes = b""              # the aligned query bytes (insertion alleles)
+    edit_distance: int = 0
+    mapq: int = 60
+    strand: int = 0                 # STRAND_FORWARD "+", STRAND_REVERSE "-"
+    avg_base_quality: float = 0.0
+    confidence: float = 1.0
+    mate: Optional[dict] = None     # MateInfo fields (types.rs:2031-2050)
+    insert_stats: Optional[tuple] = None   # (sum, count)
+
+
+def _phraya_args(reference_length, sample_id, timestamp, variants, window_coverage, reads, coverage):
+    keep = []
+    hdr = L.CPhrayaHeader(reference_length, sample_id.encode(), timestamp.encode())
+    v = np.ascontiguousarray(variants, dtype=L.VARIANT_DTYPE)
+    wc = np.ascontiguousarray(window_coverage, dtype=np.uint32)
+    cov = np.ascontiguousarray(coverage, dtype=np.uint32)
+    rs = (L.CObsRead * max(1, len(reads)))()
+    for i, r in enumerate(reads):
+        q = np.frombuffer(bytes(r.query), np.uint8) if len(r.query) else np.zeros(1, np.uint8)
+        keep.append(q)
+        mate = None
+        if r.mate is not None:
+            m = r.mate
+            mate = L.CMateInfo(m["mate_id"].encode(), m.get("insert_size", 0), int(m.get("proper_pair", False)),
+                               int(m.get("is_first_in_pair", False)), int(m.get("is_second_in_pair", False)),
+                               int(m.get("mate_mapped", False)))
+            keep.append(mate)
+        cig, prov = r.cigar.encode(), r.provenance.encode()
+        keep += [cig, prov]
+        s_, c_ = r.insert_stats if r.insert_stats is not None else (0, 0)
+        rs[i] = L.CObsRead(cig, prov, q.ctypes.data, len(r.query), r.avg_base_quality, r.confidence, s_, c_,
+                           r.edit_distance, r.mapq, r.strand, int(r.insert_stats is not None), 0,
+                           ctypes.addressof(mate) if mate is not None else None)
+    keep += [hdr, v, wc, cov, rs]
+    args = (ctypes.byref(hdr), v.ctypes.data if len(v) else None, len(v), wc.ctypes.data if len(wc) else None,
+            len(wc), rs, len(reads), cov.ctypes.data if len(cov) else None, len(cov))
+    return args, keep
+
+
+def _encoded(fn, *args) -> bytes:
+    size = ctypes.c_uint64(0)
+    rc = fn(*args, None, 0, ctypes.byref(size))
+    if rc not in (L.PHX_OK, L.PHX_E_CAPACITY):
+        raise ValueError(f"{fn.__name__} refused the input ({rc})")
+    buf = np.zeros(max(1, size.value), np.uint8)
+    rc = fn(*args, buf.ctypes.data, len(buf), ctypes.byref(size))
+    if rc != L.PHX_OK:
+        raise ValueError(f"{fn.__name__} failed ({rc})")
+    return buf[:size.value].tobytes()
+
+
+def encode_phraya(reference_length: int, sample_id: str, timestamp: str, variants, window_coverage, reads,
+                  coverage) -> bytes:
+    """The MessagePack payload of write_phraya (phraya-io/src/phraya.rs:67-80)
+    for the device variant records (VARIANT_DTYPE, window values in
+    window_coverage), one ReadMeta per alignment and the summed per-position
+    coverage (CoverageTrack::new quantizes and run-length encodes it)."""
+    args, keep = _phraya_args(reference_length, sample_id, timestamp, variants, window_coverage, reads, coverage)
+    return _encoded(L.lib().phx_phraya_encode, *args)
+
+
+def write_phraya(path: str, reference_length: int, sample_id: str, timestamp: str, variants, window_coverage,
+                 reads, coverage, level: int = 3) -> None:
+    """write_phraya: encode_phraya's payload, zstd-compressed, to path."""
+    args, keep = _phraya_args(reference_length, sample_id, timestamp, variants, window_coverage, reads, coverage)
+    rc = L.lib().phx_phraya_write(str(path).encode(), *args, level)
+    if rc != L.PHX_OK:
+        raise RuntimeError(f"phx_phraya_write failed ({rc})")
+
+
+def _query_args(index: dict):
+    keep = []
+    hits = (L.CQueryHits * max(1, len(index)))()
+    for i, (qid, lst) in enumerate(index.items()):
+        pos = np.ascontiguousarray([p for p, _ in lst], dtype=np.uint32)
+        sc = np.ascontiguousarray([s for _, s in lst], dtype=np.float64)
+        b = qid.encode()
+        keep += [pos, sc, b]
+        hits[i] = L.CQueryHits(b, pos.ctypes.data if len(pos) else None, sc.ctypes.data if len(sc) else None,
+                               len(lst))
+    keep.append(hits)
+    return (hits, len(index)), keep
+
+
+def encode_queries(index: dict) -> bytes:
+    """The MessagePack payload of write_queries (queries.rs:62-103) for
+    {query_id: [(position, score_ratio), ...]}."""
+    args, keep = _query_args(index)
+    return _encoded(L.lib().phx_queries_encode, *args)
+
+
+def write_queries(path: str, index: dict, level: int = 3) -> None:
+    args, keep = _query_args(index)
+    rc = L.lib().phx_queries_write(str(path).encode(), *args, level)
+    if rc != L.PHX_OK:
+        raise RuntimeError(f"phx_queries_write failed ({rc})")
+
+
+def _cross_args(index: dict):
+    keep = []
+    hits = (L.CCrossSpaceHits * max(1, len(index)))()
+    for i, (qid, lst) in enumerate(index.items()):
+        pl = (L.CPlacement * max(1, len(lst)))()
+        for j, (space, pos, ident) in enumerate(lst):
+            sb = space.encode()
+            keep.append(sb)
+            pl[j] = L.CPlacement(sb, pos, ident)
+        b = qid.encode()
+        keep += [pl, b]
+        hits[i] = L.CCrossSpaceHits(b, ctypes.addressof(pl), len(lst))
+    keep.append(hits)
+    return (hits, len(index)), keep
+
+
+def encode_cross_space_queries(index: dict) -> bytes:
+    """The payload of write_cross_space_queries (queries.rs:132-167) for
+    {query_id: [(space, position, identity), ...]}."""
+    args, keep = _cross_args(index)
+    return _encoded(L.lib().phx_cross_space_queries_encode, *args)
+
+
+def write_cross_space_queries(path: str, index: dict, level: int = 3) -> None:
+    args, keep = _cross_args(index)
+    rc = L.lib().phx_cross_space_queries_write(str(path).encode(), *args, level)
+    if rc != L.PHX_OK:
+        raise RuntimeError(f"phx_cross_space_queries_write failed ({rc})")

@@ -50,6 +50,8 @@ EXPORTS = (
     "phx_sketch_plan_create", "phx_sketch_plan_run", "phx_sketch_plan_results", "phx_sketch_plan_total",
     "phx_sketch_plan_capacity", "phx_sketch_plan_last_kernel_ms", "phx_sketch_plan_destroy",
     "phx_sketch_batch",
+    "phx_zstd_available", "phx_phraya_encode", "phx_phraya_write", "phx_queries_encode", "phx_queries_write",
+    "phx_cross_space_queries_encode", "phx_cross_space_queries_write",
     "phx_score_alignments", "phx_reverse_complement", "phx_align_reads_bounds",
     "phx_align_reads_batch", "phx_variant_plan_create", "phx_variant_plan_run",
     "phx_variant_plan_results", "phx_variant_plan_last_kernel_ms", "phx_variant_plan_destroy",
@@ -156,6 +158,39 @@ class CVariantBatch(ctypes.Structure):
                 ("span_off", ctypes.c_void_p), ("repeats", ctypes.c_void_p),
                 ("n_repeats", ctypes.c_uint64), ("hotspots", ctypes.c_void_p),
                 ("n_hotspots", ctypes.c_uint64), ("radius", ctypes.c_uint32)]
+
+
+class CPhrayaHeader(ctypes.Structure):
+    _fields_ = [("reference_length", ctypes.c_uint32), ("sample_id", ctypes.c_char_p),
+                ("timestamp", ctypes.c_char_p)]
+
+
+class CMateInfo(ctypes.Structure):
+    _fields_ = [("mate_id", ctypes.c_char_p), ("insert_size", ctypes.c_int32), ("proper_pair", ctypes.c_uint8),
+                ("is_first_in_pair", ctypes.c_uint8), ("is_second_in_pair", ctypes.c_uint8),
+                ("mate_mapped", ctypes.c_uint8)]
+
+
+class CObsRead(ctypes.Structure):
+    _fields_ = [("cigar", ctypes.c_char_p), ("provenance", ctypes.c_char_p), ("query", ctypes.c_void_p),
+                ("query_len", ctypes.c_uint64), ("avg_base_quality", ctypes.c_double),
+                ("confidence", ctypes.c_double), ("insert_size_sum", ctypes.c_int64),
+                ("insert_size_count", ctypes.c_uint32), ("edit_distance", ctypes.c_uint32),
+                ("mapq", ctypes.c_uint8), ("strand", ctypes.c_uint8), ("has_insert_stats", ctypes.c_uint8),
+                ("reserved", ctypes.c_uint8), ("mate", ctypes.c_void_p)]
+
+
+class CQueryHits(ctypes.Structure):
+    _fields_ = [("query_id", ctypes.c_char_p), ("pos", ctypes.c_void_p), ("score", ctypes.c_void_p),
+                ("n", ctypes.c_uint32)]
+
+
+class CPlacement(ctypes.Structure):
+    _fields_ = [("space", ctypes.c_char_p), ("pos", ctypes.c_uint32), ("identity", ctypes.c_double)]
+
+
+class CCrossSpaceHits(ctypes.Structure):
+    _fields_ = [("query_id", ctypes.c_char_p), ("placements", ctypes.c_void_p), ("n", ctypes.c_uint32)]
 
 
 class CVariantResults(ctypes.Structure):
@@ -289,6 +324,15 @@ def lib() -> ctypes.CDLL:
     L.phx_sketch_plan_destroy.restype = None
     L.phx_sketch_batch.argtypes = [P, P, P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32,
                                    P, P, ctypes.c_uint64, P, P]
+    L.phx_zstd_available.argtypes = []
+    L.phx_phraya_encode.argtypes = [P, P, ctypes.c_uint64, P, ctypes.c_uint64, P, ctypes.c_uint32, P,
+                                    ctypes.c_uint64, P, ctypes.c_uint64, P]
+    L.phx_phraya_write.argtypes = [ctypes.c_char_p, P, P, ctypes.c_uint64, P, ctypes.c_uint64, P, ctypes.c_uint32,
+                                   P, ctypes.c_uint64, ctypes.c_int]
+    L.phx_queries_encode.argtypes = [P, ctypes.c_uint32, P, ctypes.c_uint64, P]
+    L.phx_queries_write.argtypes = [ctypes.c_char_p, P, ctypes.c_uint32, ctypes.c_int]
+    L.phx_cross_space_queries_encode.argtypes = [P, ctypes.c_uint32, P, ctypes.c_uint64, P]
+    L.phx_cross_space_queries_write.argtypes = [ctypes.c_char_p, P, ctypes.c_uint32, ctypes.c_int]
     L.phx_reverse_complement.argtypes = [P, ctypes.c_uint64, P]
     L.phx_reverse_complement.restype = None
     L.phx_align_reads_bounds.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, P,

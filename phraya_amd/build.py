@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libphraya_hip.so")
 SYNTH_LIB = os.path.join(HERE, "libphx_synth.so")
 SOURCES = ["wfa_wave.hip", "myers_kernels.hip", "phx_api.hip", "phx_serve.hip", "phx_contig.hip",
-           "phx_executor.hip", "phx_variants.hip", "phx_chain.hip", "phx_sketch.hip"]
+           "phx_executor.hip", "phx_variants.hip", "phx_chain.hip", "phx_sketch.hip", "phx_io.cpp"]
 ARCH = os.environ.get("PHX_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -35,7 +35,7 @@ def build(verbose: bool = False) -> str:
     obj_dir = os.path.join(HERE, "_build")
     os.makedirs(obj_dir, exist_ok=True)
     for src in SOURCES:
-        obj = os.path.join(obj_dir, src.replace(".hip", ".o"))
+        obj = os.path.join(obj_dir, os.path.splitext(src)[0] + ".o")
         cmd = [hipcc, *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC, "-c",
                os.path.join(CSRC, src), "-o", obj]
         if verbose:
@@ -43,7 +43,7 @@ def build(verbose: bool = False) -> str:
         subprocess.run(cmd, check=True)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-ldl"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
