@@ -559,7 +559,8 @@ int phx_seed_and_chain_batch(phx_ctx* ctx, const phx_seed_index* index, uint64_t
 typedef struct phx_sketch_plan phx_sketch_plan;
 
 /* Resident form: create uploads the sequences and lays out the tasks; run
- * is three launches (count per task, scan, write); results downloads:
+ * is three launches (compute into per-task staging, scan of the counts,
+ * copy to the final offsets); results downloads:
  * off[0 .. nseq] (sequence i's entries at off[i] .. off[i + 1]) and, when
  * cap >= the total, values / positions.  *total is set also on
  * PHX_E_CAPACITY.  capacity is an upper bound on the total (the number of
@@ -573,6 +574,8 @@ uint64_t phx_sketch_plan_total(phx_sketch_plan* plan);
 uint64_t phx_sketch_plan_capacity(const phx_sketch_plan* plan);
 /* Kernel time of the last run (events around its launches), milliseconds. */
 double phx_sketch_plan_last_kernel_ms(const phx_sketch_plan* plan);
+/* One launch of the last run: 0 compute, 1 scan, 2 copy (milliseconds). */
+double phx_sketch_plan_last_launch_ms(const phx_sketch_plan* plan, uint32_t i);
 void phx_sketch_plan_destroy(phx_sketch_plan* plan);
 /* One call: create, run, results, destroy. */
 int phx_sketch_batch(phx_ctx* ctx, const uint8_t* seqs, const uint64_t* seq_off, uint32_t nseq, uint32_t k,

@@ -108,6 +108,10 @@ def lib():
         L.orc_seed_and_chain_timed.restype = ctypes.c_double
         L.orc_sketch.argtypes = [P, u64, u32, u32, ctypes.c_int, P, P]
         L.orc_sketch.restype = i64
+        L.orc_sketch_rolling.argtypes = [P, u64, u32, u32, ctypes.c_int, P, P]
+        L.orc_sketch_rolling.restype = i64
+        L.orc_sketch_timed.argtypes = [P, P, u32, u32, u32, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]
+        L.orc_sketch_timed.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -337,15 +341,29 @@ def seed_and_chain_timed(target_mins, queries, max_occ=2**64 - 1, params=None, t
                                                 threads or hardware_threads(), passes, ctypes.byref(chk)))
 
 
-def sketch(seq, k: int = 21, w: int = 11, canonical: bool = True):
+def sketch(seq, k: int = 21, w: int = 11, canonical: bool = True, rolling: bool = False):
     """Minimizer sketch (sketch_alphabet, phraya-core/src/types.rs:1846-1867)
     of one byte sequence: (values u64, positions u32) in window order; the
-    scheme is stated in phraya_oracle.cpp."""
+    scheme is stated in phraya_oracle.cpp.  rolling: the CPU baseline's
+    implementation of the same scheme."""
     s = np.frombuffer(bytes(seq), np.uint8) if not isinstance(seq, np.ndarray) else np.ascontiguousarray(seq, np.uint8)
     cap = max(1, len(s) - k - w + 2)
     vals = np.zeros(cap, np.uint64)
     pos = np.zeros(cap, np.uint32)
-    n = lib().orc_sketch(s.ctypes.data, len(s), k, w, 1 if canonical else 0, vals.ctypes.data, pos.ctypes.data)
+    fn = lib().orc_sketch_rolling if rolling else lib().orc_sketch
+    n = fn(s.ctypes.data, len(s), k, w, 1 if canonical else 0, vals.ctypes.data, pos.ctypes.data)
     if n < 0:
         raise ValueError("orc_sketch: invalid k / w")
     return vals[:n], pos[:n]
+
+
+def sketch_timed(seqs, k: int = 21, w: int = 11, canonical: bool = True, threads=None, passes: int = 1) -> float:
+    """Seconds for `passes` rounds of sketching every sequence (CPU baseline,
+    the rolling implementation, sequences split over threads)."""
+    arrs = [np.frombuffer(bytes(s), np.uint8) if not isinstance(s, np.ndarray) else np.ascontiguousarray(s, np.uint8)
+            for s in seqs]
+    buf = np.ascontiguousarray(np.concatenate(arrs + [np.zeros(1, np.uint8)]))
+    off = np.concatenate([[0], np.cumsum([len(a) for a in arrs])]).astype(np.uint64)
+    chk = ctypes.c_uint64(0)
+    return float(lib().orc_sketch_timed(buf.ctypes.data, off.ctypes.data, len(arrs), k, w, 1 if canonical else 0,
+                                        threads or hardware_threads(), passes, ctypes.byref(chk)))

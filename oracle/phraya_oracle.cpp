@@ -1404,3 +1404,99 @@ int64_t orc_sketch(const uint8_t* s, uint64_t n, uint32_t k, uint32_t w, int can
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// The same sketch computed the way a CPU implementation would (rolling
+// k-mer values, a running T/G count, one scan of w hashes per window), for
+// the CPU baseline; checked against orc_sketch by the tests.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+int64_t sketch_rolling(const uint8_t* s, uint64_t n, uint32_t k, uint32_t w, bool canonical, std::vector<uint64_t>& V,
+                       std::vector<uint64_t>& H, uint64_t* vals, uint32_t* pos) {
+  const uint64_t l = (uint64_t)k + w - 1;
+  if (n < l) return 0;
+  const uint64_t nk = n - k + 1;
+  V.resize(nk);
+  H.resize(nk);
+  const uint64_t kmask = k == 32 ? ~0ull : ((1ull << (2 * k)) - 1);
+  const uint32_t rsh = 2 * (k - 1);
+  uint64_t F = 0, R = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint64_t c = (s[i] >> 1) & 3;
+    F = ((F << 2) | c) & kmask;
+    R = (R >> 2) | ((c ^ 2) << rsh);
+    if (i + 1 >= k) {
+      const uint64_t v = canonical && R < F ? R : F;
+      V[i + 1 - k] = v;
+      H[i + 1 - k] = orc_mix64(v);
+    }
+  }
+  uint64_t tg = 0;
+  for (uint64_t j = 0; j < l; ++j) tg += ((s[j] >> 1) & 3) >> 1;
+  int64_t cnt = 0;
+  uint64_t last = ~0ull;
+  for (uint64_t x = 0;; ++x) {
+    const bool left = !canonical || 2 * tg > l;
+    uint64_t best = x, hb = H[x];
+    for (uint64_t i = x + 1; i < x + w; ++i)
+      if (H[i] < hb || (!left && H[i] == hb)) {
+        hb = H[i];
+        best = i;
+      }
+    if (best != last) {
+      vals[cnt] = V[best];
+      pos[cnt] = (uint32_t)best;
+      ++cnt;
+      last = best;
+    }
+    if (x + l >= n) break;
+    tg += (((s[x + l] >> 1) & 3) >> 1) - (((s[x] >> 1) & 3) >> 1);
+  }
+  return cnt;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t orc_sketch_rolling(const uint8_t* s, uint64_t n, uint32_t k, uint32_t w, int canonical, uint64_t* vals,
+                           uint32_t* pos) {
+  if (k == 0 || k > 32 || w == 0) return -1;
+  if (canonical && ((uint64_t)k + w - 1) % 2 == 0) return -1;
+  std::vector<uint64_t> V, H;
+  return sketch_rolling(s, n, k, w, canonical != 0, V, H, vals, pos);
+}
+
+// CPU baseline: `passes` rounds of sketching every sequence of the batch
+// (sequences split over `threads`); returns the seconds of the passes.
+double orc_sketch_timed(const uint8_t* seq, const uint64_t* off, uint32_t nseq, uint32_t k, uint32_t w, int canonical,
+                        int threads, int passes, uint64_t* checksum) {
+  const int T = std::max(1, threads);
+  std::atomic<uint64_t> sum{0};
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int r = 0; r < passes; ++r) {
+    std::vector<std::thread> pool;
+    for (int t = 0; t < T; ++t)
+      pool.emplace_back([&, t]() {
+        std::vector<uint64_t> V, H, vals;
+        std::vector<uint32_t> pos;
+        uint64_t s = 0;
+        for (uint32_t i = (uint32_t)t; i < nseq; i += (uint32_t)T) {
+          const uint64_t n = off[i + 1] - off[i];
+          vals.resize(n + 1);
+          pos.resize(n + 1);
+          const int64_t c = sketch_rolling(seq + off[i], n, k, w, canonical != 0, V, H, vals.data(), pos.data());
+          s += (uint64_t)c + (c > 0 ? vals[c - 1] : 0);
+        }
+        sum += s;
+      });
+    for (auto& th : pool) th.join();
+  }
+  const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  *checksum = sum.load();
+  return el;
+}
+
+}  // extern "C"

@@ -108,6 +108,12 @@ int orc_hardware_threads(void);
  * with k + w - 1 even.  The scheme is stated in phraya_oracle.cpp. */
 int64_t orc_sketch(const uint8_t* s, uint64_t n, uint32_t k, uint32_t w, int canonical, uint64_t* vals,
                    uint32_t* pos);
+/* The same sketch by rolling values and a running T/G count (the CPU
+ * baseline's form), and the timed batch of it. */
+int64_t orc_sketch_rolling(const uint8_t* s, uint64_t n, uint32_t k, uint32_t w, int canonical, uint64_t* vals,
+                           uint32_t* pos);
+double orc_sketch_timed(const uint8_t* seq, const uint64_t* off, uint32_t nseq, uint32_t k, uint32_t w, int canonical,
+                        int threads, int passes, uint64_t* checksum);
 
 #ifdef __cplusplus
 }

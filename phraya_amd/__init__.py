@@ -986,6 +986,10 @@ class SketchPlan:
     def last_kernel_ms(self) -> float:
         return float(L.lib().phx_sketch_plan_last_kernel_ms(self._h))
 
+    def last_launch_ms(self, i: int) -> float:
+        """0 compute, 1 scan, 2 copy."""
+        return float(L.lib().phx_sketch_plan_last_launch_ms(self._h, i))
+
     @property
     def capacity(self) -> int:
         return int(L.lib().phx_sketch_plan_capacity(self._h))

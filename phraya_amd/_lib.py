@@ -48,7 +48,8 @@ EXPORTS = (
     "phx_chain_plan_run", "phx_chain_plan_results", "phx_chain_plan_last_kernel_ms", "phx_chain_plan_seeds",
     "phx_chain_plan_destroy", "phx_seed_and_chain_batch",
     "phx_sketch_plan_create", "phx_sketch_plan_run", "phx_sketch_plan_results", "phx_sketch_plan_total",
-    "phx_sketch_plan_capacity", "phx_sketch_plan_last_kernel_ms", "phx_sketch_plan_destroy",
+    "phx_sketch_plan_capacity", "phx_sketch_plan_last_kernel_ms", "phx_sketch_plan_last_launch_ms",
+    "phx_sketch_plan_destroy",
     "phx_sketch_batch",
     "phx_zstd_available", "phx_phraya_encode", "phx_phraya_write", "phx_queries_encode", "phx_queries_write",
     "phx_cross_space_queries_encode", "phx_cross_space_queries_write",
@@ -320,6 +321,8 @@ def lib() -> ctypes.CDLL:
     L.phx_sketch_plan_capacity.restype = ctypes.c_uint64
     L.phx_sketch_plan_last_kernel_ms.argtypes = [P]
     L.phx_sketch_plan_last_kernel_ms.restype = ctypes.c_double
+    L.phx_sketch_plan_last_launch_ms.argtypes = [P, ctypes.c_uint32]
+    L.phx_sketch_plan_last_launch_ms.restype = ctypes.c_double
     L.phx_sketch_plan_destroy.argtypes = [P]
     L.phx_sketch_plan_destroy.restype = None
     L.phx_sketch_batch.argtypes = [P, P, P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32,

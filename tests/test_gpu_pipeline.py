@@ -1,9 +1,12 @@
 """The per-read path end to end on the GPU, as align_read composes it
-(phraya-align/src/executor.rs:573-788): minimizer sketches of each read and
-of its reverse complement -> seed_and_chain (phx_seed_and_chain_batch) ->
-the orientations worth extending, extended and chosen
-(phx_align_reads_batch) -> variant sites of the kept primary with the
-read's coverage windows (phx_extract_variants).  Every stage is compared
+(phraya-align/src/executor.rs:573-788): canonical (21, 11) minimizer
+sketches of each read, of its reverse complement and of the genome
+(phx_sketch_batch, as sketch() does, phraya-core/src/types.rs:1873-1877) ->
+seed_and_chain (phx_seed_and_chain_batch) -> the orientations worth
+extending, extended and chosen (phx_align_reads_batch) -> variant sites of
+the kept primary with the read's coverage windows (phx_extract_variants) ->
+the .phraya payload of those observations (phx_phraya_encode, as the align
+command writes it, phraya-cli/src/main.rs:556-639).  Every stage is compared
 with the oracle composed the same way."""
 from __future__ import annotations
 
@@ -50,9 +53,12 @@ def test_reads_from_sketch_to_variant_sites(strategy):
         reads.append(rc_ref(src) if i % 3 == 0 else src)
     reads.append(bytes(rng.choice(b"ACGT") for _ in range(150)))  # unrelated: no seeds either way
     say(f"strategy {strategy}: {len(reads)} reads")
-    gs = synth.minimizer_sketch(np.frombuffer(genome, np.uint8))
-    sk = [synth.minimizer_sketch(np.frombuffer(r, np.uint8)) for r in reads]
-    sk += [synth.minimizer_sketch(np.frombuffer(rc_ref(r), np.uint8)) for r in reads]
+    seqs = [genome] + reads + [rc_ref(r) for r in reads]
+    allsk = px.sketch_batch(seqs)
+    for s_, k_ in zip(seqs[::37], allsk[::37]):
+        v_, p_ = O.sketch(s_, 21, 11, True)
+        assert np.array_equal(k_[:, 0], v_) and np.array_equal(k_[:, 1], p_.astype(np.uint64))
+    gs, sk = allsk[0], allsk[1:]
     cap = O.seed_occurrence_cap(gs, 256)
     n = len(reads)
 
@@ -103,3 +109,20 @@ def test_reads_from_sketch_to_variant_sites(strategy):
         assert np.array_equal(v.variants[f], ov[f]), f
     say(f"{len(kept)} kept reads, {len(v.variants)} variant sites")
     assert len(v.variants) > 100
+
+    # The .phraya payload: observations with each read's fields, the
+    # genome coverage summed over every alignment window (merge_result).
+    from oracle import formats as F
+    cov = np.zeros(len(genome), np.uint32)
+    for sp in spans:
+        for a, b in sp:
+            cov[min(a, len(genome)):min(b, len(genome))] += 1
+    metas = [px.ReadMeta(cigar=g.scored.primary.cigar, provenance=f"sample1:read{i}", query=g.query,
+                         edit_distance=g.scored.primary.edit_distance, mapq=60, strand=g.strand,
+                         avg_base_quality=40.0, confidence=1.0) for i, g in enumerate(kept)]
+    doc = F.unpack(px.encode_phraya(len(genome), "worker_0", "t0", v.variants, v.coverage, metas, cov))
+    want = F.phraya_file(len(genome), "worker_0", "t0", ov, ocov,
+                         [dict(cigar=m.cigar, provenance=m.provenance, query=m.query,
+                               edit_distance=m.edit_distance, mapq=60, strand=m.strand, avg_base_quality=40.0,
+                               confidence=1.0) for m in metas], cov)
+    assert doc == want

@@ -1,7 +1,7 @@
 """Minimizer sketches on the GPU (phx_sketch_plan_*, phx_sketch_batch through
 the C ABI) against the oracle (orc_sketch), bit-exact: every value and
 position of every sequence, canonical and forward, across task boundaries
-(252 windows per task), at the 5 Mbp genome's full size, plus strand
+(253 - w windows per task), at the 5 Mbp genome's full size, plus strand
 symmetry on the device's own output."""
 from __future__ import annotations
 
@@ -16,7 +16,10 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-TASK = 252
+
+
+def task_windows(w):
+    return 253 - w
 
 
 def say(msg):
@@ -47,8 +50,9 @@ def check_batch(px, seqs, k, w, canonical, label):
 def lengths_around_tasks(k, w):
     l = k + w - 1
     out = [0, 1, l - 1, l, l + 1, 150, 300]
+    tw = task_windows(w)
     for j in (1, 2, 3, 7):
-        out += [l - 2 + TASK * j, l - 1 + TASK * j, l + TASK * j]
+        out += [l - 2 + tw * j, l - 1 + tw * j, l + tw * j]
     return out
 
 

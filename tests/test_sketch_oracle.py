@@ -136,3 +136,18 @@ def test_invalid_parameters():
         with pytest.raises(ValueError):
             O.sketch(b"ACGT" * 20, k, w, c)
     assert len(O.sketch(b"ACGT" * 7, 21, 11)[0]) == 0   # 28 < l = 31
+
+
+@pytest.mark.parametrize("k,w,canonical", [(21, 11, True), (21, 5, True), (32, 31, False), (1, 1, True),
+                                           (6, 5, False), (15, 9, True)])
+def test_rolling_baseline_equals_oracle(k, w, canonical):
+    # The CPU baseline's implementation (rolling values, running T/G count)
+    # reports exactly the oracle's sketch.
+    rng = random.Random(k + 100 * w)
+    for trial in range(20):
+        n = rng.choice([0, k + w - 2, k + w - 1, 200, 3000])
+        seq = _random_dna(rng, n, repeat=trial % 2 == 1).encode()
+        a = O.sketch(seq, k, w, canonical)
+        b = O.sketch(seq, k, w, canonical, rolling=True)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert O.sketch_timed([b"ACGT" * 100, b"", b"A" * 40], k, w, canonical, threads=2) >= 0.0
