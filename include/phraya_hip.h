@@ -121,6 +121,13 @@ typedef struct phx_tuning {
                                 per wave in 16-lane rows when an alignment within the row's band exists,
                                 else two per wave in 32-lane rows (0 or 1 = on, 2 = off: one pair per
                                 wave, 3 = every such pair in 32-lane rows) */
+  uint32_t band_permille;    /* workgroup register tiers: the band's bound on a pair's optimum is its step
+                                estimate x band_permille / 1000 (0 = 800; at most 1000); a pair whose
+                                optimum exceeds it is re-run with the whole estimate as bound, then
+                                without one.  Results do not depend on it. */
+  uint32_t regs_narrow;      /* workgroup register tiers: a pair of the 1024-thread tier whose band fits the
+                                256-thread tier's window (4,096 diagonals) runs there, four pairs per CU
+                                (0 or 1 = on, 2 = off) */
 } phx_tuning;
 
 /* Context: owns the stream, device buffers and the submission lock.

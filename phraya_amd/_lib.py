@@ -207,7 +207,8 @@ class CTuning(ctypes.Structure):
                 ("hbm_budget_mib", ctypes.c_uint64), ("block256_min_len", ctypes.c_uint32),
                 ("block1024_min_len", ctypes.c_uint32), ("heavy_permille", ctypes.c_uint32),
                 ("block_lds_ring", ctypes.c_uint32), ("block_flight", ctypes.c_uint32),
-                ("block_regs", ctypes.c_uint32), ("lds_rows", ctypes.c_uint32)]
+                ("block_regs", ctypes.c_uint32), ("lds_rows", ctypes.c_uint32),
+                ("band_permille", ctypes.c_uint32), ("regs_narrow", ctypes.c_uint32)]
 
 
 class CServeConfig(ctypes.Structure):

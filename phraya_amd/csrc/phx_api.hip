@@ -48,6 +48,7 @@ struct phx_plan {
   uint64_t input_bytes = 0;                            // packed algorithmic input bytes
   int32_t ring_slots = 2;
   DevBuf d_seq, d_pairs, d_out, d_runs, d_work, d_work2, d_ctr, d_scratch;
+  DevBuf d_bounds;  // (pair, bound) pairs of a relaxed-bound re-run
   DevBuf d_gather, d_gidx;  // compacted CIGAR runs and their source / target offsets
   DevBuf d_arena;  // HBM-tier scratch (history + ring slots); each group of a run gets its own slice
   uint32_t* ctr_host = nullptr;  // pinned overflow flags, one per group (kernels write them)
@@ -75,6 +76,8 @@ struct phx_plan {
   std::vector<uint32_t> order;                         // pending pairs, longest first
   std::vector<int32_t> est;                            // WFA: estimated score steps per pair
   std::vector<uint64_t> band_words;                    // WFA: history words within the pair's bound (0 = none)
+  bool banded = false;                                 // WFA: the cost shape has a workgroup register wavefront
+  int32_t en_steps = 1;                                //   and its gap extension in score steps
   std::vector<uint8_t> rows_ok;                        // WFA: LDS-tier pair a banded row can take
   std::vector<uint8_t> rows_cert;                      // ... whose optimum is bounded by its 16-lane band
   // Allocation-free plans: every device buffer is a carve of a fixed region
@@ -345,6 +348,8 @@ phx_tuning merge_tuning(phx_tuning d, const phx_tuning* t) {
   if (t->block_flight) d.block_flight = std::min<uint32_t>(t->block_flight, 4);
   if (t->block_regs) d.block_regs = std::min<uint32_t>(t->block_regs, 2);
   if (t->lds_rows) d.lds_rows = std::min<uint32_t>(t->lds_rows, 3);
+  if (t->band_permille) d.band_permille = std::min<uint32_t>(t->band_permille, 1000);
+  if (t->regs_narrow) d.regs_narrow = std::min<uint32_t>(t->regs_narrow, 2);
   return d;
 }
 bool has_cigar(int engine) { return engine != PHX_ENGINE_MYERS_DIST; }
@@ -667,6 +672,9 @@ int build_graph(phx_plan* P) {
   return PHX_OK;
 }
 
+int rerun_relaxed(phx_plan* P, std::vector<uint32_t>& wl, std::vector<PairOut>& outs, uint32_t* d_ctr,
+                  std::vector<uint32_t>& exact);
+
 int run_wfa(phx_plan* P) {
   phx_ctx* ctx = P->ctx;
   uint32_t* d_ctr = (uint32_t*)P->d_ctr.p;
@@ -701,10 +709,26 @@ int run_wfa(phx_plan* P) {
   auto overflowed = [&](uint32_t i) {
     return outs[i].status >= ST_OVERFLOW_HIST && outs[i].status <= ST_OVERFLOW_RUNS;
   };
-  std::vector<uint32_t> from_lds, exact;
+  // A register-wavefront pair whose band emptied before its end (status
+  // WIDTH while the band fits the group's window: the band spans at most
+  // bound + 1 diagonals around kend / 2, so only an optimum above the bound
+  // stops it) is re-run with the whole estimate as bound when that is
+  // larger and fits the 1024-thread window.
+  auto relaxable = [&](int gi, uint32_t i) {
+    const PairDesc& pd = P->pairs[i];
+    const int nt = P->group_tier[gi] == TIER_BLOCK256 ? 256 : 1024;
+    return !P->fixed() && P->group_rl[gi] == 2 && outs[i].status == ST_OVERFLOW_WIDTH && pd.bound > 0 &&
+           (uint64_t)pd.bound + 4 <= reg_window(nt) && P->est[i] > (int32_t)pd.bound &&
+           (uint64_t)P->est[i] + 4 <= reg_window(1024);
+  };
+  std::vector<uint32_t> from_lds, exact, relaxed;
   for (int gi : launched)
     for (uint32_t i : P->work[gi])
-      if (overflowed(i)) (P->group_tier[gi] == 0 ? from_lds : exact).push_back(i);
+      if (overflowed(i)) (P->group_tier[gi] == 0 ? from_lds : relaxable(gi, i) ? relaxed : exact).push_back(i);
+  if (!relaxed.empty()) {
+    const int rc = rerun_relaxed(P, relaxed, outs, d_ctr, exact);
+    if (rc != PHX_OK) return rc;
+  }
   for (int round = 0; round < 8 && (!from_lds.empty() || !exact.empty()); ++round) {
     std::vector<uint32_t>& wl = !from_lds.empty() ? from_lds : exact;
     const bool est = &wl == &from_lds;
@@ -751,6 +775,60 @@ int run_wfa(phx_plan* P) {
     }
     wl.clear();
     exact.insert(exact.end(), again.begin(), again.end());
+  }
+  return PHX_OK;
+}
+
+// The relaxed-bound round: pairs whose optimum exceeded their band bound
+// run again on the 1024-thread register tier with the whole estimate as
+// bound (set on the device copies of their descriptors for this launch
+// and restored after it); those that overflow again join `exact`.
+int rerun_relaxed(phx_plan* P, std::vector<uint32_t>& wl, std::vector<PairOut>& outs, uint32_t* d_ctr,
+                  std::vector<uint32_t>& exact) {
+  phx_ctx* ctx = P->ctx;
+  std::stable_sort(wl.begin(), wl.end(), [&](uint32_t a, uint32_t b) { return P->est[a] > P->est[b]; });
+  std::vector<uint32_t> set(2 * wl.size()), restore(2 * wl.size());
+  uint64_t words = 0;
+  for (size_t k = 0; k < wl.size(); ++k) {
+    const uint32_t i = wl[k];
+    const PairDesc& pd = P->pairs[i];
+    const bool fitting = !(pd.flags & PF_GLOBAL) && pd.tn > pd.qn + pd.qn / 2 + 10;
+    set[2 * k] = restore[2 * k] = i;
+    set[2 * k + 1] = (uint32_t)P->est[i];
+    restore[2 * k + 1] = pd.bound;
+    words = std::max(words, band_hist_words(P->est[i], pd.qn, pd.tn, P->en_steps, fitting));
+  }
+  if (plan_alloc(P, P->d_work2, (size_t)P->n * 4) != PHX_OK || plan_alloc(P, P->d_bounds, set.size() * 4) != PHX_OK)
+    return PHX_E_NOMEM;
+  HIPCHK(hipMemcpy(P->d_work2.p, wl.data(), wl.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(P->d_bounds.p, set.data(), set.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(launch_set_bounds((PairDesc*)P->d_pairs.p, (const uint32_t*)P->d_bounds.p, (uint32_t)wl.size(),
+                           ctx->stream));
+  HIPCHK(launch_zero(d_ctr, 2, ctx->stream));
+  if (clear_flags(P, 1) != PHX_OK) return PHX_E_DEVICE;
+  const uint64_t slot_bytes = (words * 4 + 255) & ~(uint64_t)255;
+  const GroupSlots rg = group_slots(P, TIER_BLOCK1024, wl, slot_bytes, P->group_budget(-1), true);
+  int rc = plan_transient(P, P->d_arena, (size_t)(rg.n * rg.per));
+  if (rc != PHX_OK) return rc;
+  P->rerun_lists.push_back(wl);
+  rc = launch_wfa_tier(P, TIER_BLOCK1024, P->rerun_lists.back(), (uint32_t*)P->d_work2.p, d_ctr, P->ctr_dev, rg,
+                       (uint8_t*)P->d_arena.p, ctx->stream, 2, TIME_BOTH);
+  if (rc != PHX_OK) return rc;
+  if (plan_event(P, &P->ev_end) != PHX_OK) return PHX_E_DEVICE;
+  HIPCHK(hipEventRecord(P->ev_end, ctx->stream));
+  P->end_ev = P->ev_end;
+  // The descriptors get their own bounds back (the upload of `restore`
+  // waits for the launch: same stream, synchronous copy).
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpy(P->d_bounds.p, restore.data(), restore.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(launch_set_bounds((PairDesc*)P->d_pairs.p, (const uint32_t*)P->d_bounds.p, (uint32_t)wl.size(),
+                           ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (read_flags(P, 1) != PHX_OK) return PHX_E_DEVICE;
+  if (((volatile uint32_t*)P->ctr_host)[0] != 0) {
+    HIPCHK(hipMemcpy(outs.data(), P->d_out.p, sizeof(PairOut) * P->n, hipMemcpyDeviceToHost));
+    for (uint32_t i : wl)
+      if (outs[i].status >= ST_OVERFLOW_HIST && outs[i].status <= ST_OVERFLOW_RUNS) exact.push_back(i);
   }
   return PHX_OK;
 }
@@ -867,7 +945,7 @@ void plan_group(phx_plan* P) {
         const uint64_t seq_b = 16 + 8ull * lds_seq_words(P->pairs[i]);
         // A bounded pair's band spans at most bound + 1 diagonals (en = 1);
         // an unbounded one spans its whole range, about 2 steps + 1.
-        const bool fits_window = P->pairs[i].bound ? (uint64_t)P->est[i] + 4 <= reg_window(nt)
+        const bool fits_window = P->pairs[i].bound ? (uint64_t)P->pairs[i].bound + 4 <= reg_window(nt)
                                                    : (uint64_t)P->est[i] * 4 / 5 < reg_window(nt) / 2;
         if (regs_ok && (P->tun.block_regs == 1 || fits_window) && RG.per_wave_bytes + seq_b <= kBlockLdsMax)
           where = 2;
@@ -876,19 +954,30 @@ void plan_group(phx_plan* P) {
           where = 1;
         parts[where].push_back(i);
       }
-      for (int r = 0; r < 3; ++r) {
-        if (parts[r].empty()) continue;
+      // Register pairs of the 1024-thread tier whose band fits the
+      // 256-thread window run there: four pairs per CU in flight instead of
+      // one, whose step barriers and latencies overlap.
+      std::vector<uint32_t> narrow;
+      if (tier == TIER_BLOCK1024 && P->tun.regs_narrow != 2) {
+        std::vector<uint32_t> keep;
+        for (uint32_t i : parts[2])
+          (P->pairs[i].bound && (uint64_t)P->pairs[i].bound + 4 <= reg_window(256) ? narrow : keep).push_back(i);
+        parts[2].swap(keep);
+      }
+      for (int r = 0; r < 4; ++r) {
+        std::vector<uint32_t>& part = r < 3 ? parts[r] : narrow;
+        if (part.empty()) continue;
         uint64_t mx = 0;
         double v = 0;
-        for (uint32_t i : parts[r]) {
+        for (uint32_t i : part) {
           // Register pairs store only their band's cells.
-          const uint64_t wi = r == 2 && P->pairs[i].bound ? P->band_words[i] : words_of(i);
+          const uint64_t wi = r >= 2 && P->pairs[i].bound ? P->band_words[i] : words_of(i);
           mx = std::max<uint64_t>(mx, wi);
           v += (double)wi;
         }
-        P->work.push_back(parts[r]);
-        P->group_tier.push_back(tier);
-        P->group_rl.push_back((uint8_t)r);
+        P->work.push_back(part);
+        P->group_tier.push_back(r < 3 ? tier : TIER_BLOCK256);
+        P->group_rl.push_back((uint8_t)(r < 3 ? r : 2));
         P->group_slot.push_back((mx * 4 + 255) & ~(uint64_t)255);
         vol.push_back(v);
       }
@@ -1018,6 +1107,28 @@ void plan_group(phx_plan* P) {
 }
 
 // Uploads the flattened work lists (group by group).
+// Band bounds of the pairs that have a step estimate (cost shapes with a
+// workgroup register wavefront): estimate x band_permille / 1000, and the
+// history words of that band.  A pair whose optimum exceeds its bound is
+// re-run with the whole estimate as bound (run_plan's re-run rounds).
+int32_t band_bound(const phx_plan* P, uint32_t i) {
+  // Plans on a fixed region keep the whole estimate (their re-runs take the
+  // ring path, and a fixed region has no room for the relaxed round).
+  const int64_t pm = P->fixed() ? 1000 : P->tun.band_permille;
+  return std::max<int32_t>(1, (int32_t)((int64_t)P->est[i] * pm / 1000));
+}
+
+void apply_bounds(phx_plan* P) {
+  if (!P->banded) return;
+  for (uint32_t i = 0; i < P->n; ++i) {
+    if (P->est[i] <= 0) continue;
+    PairDesc& pd = P->pairs[i];
+    const bool fitting = !(pd.flags & PF_GLOBAL) && pd.tn > pd.qn + pd.qn / 2 + 10;
+    pd.bound = (uint32_t)band_bound(P, i);
+    P->band_words[i] = band_hist_words((int32_t)pd.bound, pd.qn, pd.tn, P->en_steps, fitting);
+  }
+}
+
 int upload_work(phx_plan* P) {
   phx_ctx* ctx = P->ctx;
   std::vector<uint32_t> flat;
@@ -1085,13 +1196,10 @@ phx_plan* plan_setup(phx_plan* P, const std::vector<const uint8_t*>& qh,
     parallel_for(wide.size(), [&](size_t k) {
       const uint32_t i = wide[k];
       P->est[i] = estimate_steps_seq(P->engine, P->costs, qh[i], P->qlen[i], th[i], P->tlen[i]);
-      if (banded) {
-        PairDesc& pd = P->pairs[i];
-        const bool fitting = !(pd.flags & PF_GLOBAL) && pd.tn > pd.qn + pd.qn / 2 + 10;
-        pd.bound = (uint32_t)P->est[i];
-        P->band_words[i] = band_hist_words(P->est[i], pd.qn, pd.tn, en_steps, fitting);
-      }
     });
+    P->en_steps = en_steps;
+    P->banded = banded;
+    apply_bounds(P);
     plan_group(P);
   } else {
     P->work.assign(9, {});
@@ -1392,6 +1500,7 @@ phx_ctx* phx_ctx_create(int device, uint32_t flags, const phx_tuning* tuning) {
   d.heavy_permille = 50;
   d.block_lds_ring = 1024;
   d.block_flight = 1;
+  d.band_permille = 800;
   c->tun = merge_tuning(d, tuning);
   return c;
 }
@@ -1553,7 +1662,7 @@ int phx_plan_run(phx_plan* P) {
   P->last_launches = 0;
   P->launch_rec.clear();
   P->rerun_lists.clear();
-  P->rerun_lists.reserve(8);  // records point into it: no reallocation within a run
+  P->rerun_lists.reserve(10);  // records point into it: no reallocation within a run (8 rounds + 1)
   P->start_ev = nullptr;
   P->end_ev = nullptr;
   P->ms_ready = P->kernel_ms_ready = false;
@@ -1654,11 +1763,16 @@ int phx_plan_set_tuning(phx_plan* P, const phx_tuning* t) {
   phx_ctx* ctx = P->ctx;
   std::lock_guard<std::mutex> lk(ctx->mu);
   HIPCHK(hipSetDevice(ctx->device));
+  const uint32_t old_pm = P->tun.band_permille;
   P->tun = merge_tuning(P->tun, t);
   if (!is_wfa(P->engine)) return PHX_OK;
   if (P->gexec) {  // the launch sequence changes with the grouping
     (void)hipGraphExecDestroy(P->gexec);
     P->gexec = nullptr;
+  }
+  if (P->banded && P->tun.band_permille != old_pm && !P->fixed()) {
+    apply_bounds(P);
+    if (P->n) HIPCHK(hipMemcpy(P->d_pairs.p, P->pairs.data(), sizeof(PairDesc) * P->n, hipMemcpyHostToDevice));
   }
   plan_group(P);
   return upload_work(P);
@@ -1771,7 +1885,6 @@ int phx_plan_autotune(phx_plan* P, uint32_t max_trials, phx_tuning* best_out, do
 }
 
 static int fetch_out(phx_plan* P) {
-  phx_ctx* ctx = P->ctx;
   if (P->out_valid) return PHX_OK;
   P->out_h.resize(P->n);
   const int rc = d2h(P, P->out_h.data(), P->d_out.p, sizeof(PairOut) * P->n);
@@ -1879,6 +1992,7 @@ void phx_plan_destroy(phx_plan* P) {
                       &P->d_scratch, &P->d_gather, &P->d_gidx})
       dev_free(*b);
     dev_free(P->d_arena);
+    dev_free(P->d_bounds);
   }
   if (P->gexec) (void)hipGraphExecDestroy(P->gexec);
   for (hipEvent_t e : P->kev) (void)hipEventDestroy(e);

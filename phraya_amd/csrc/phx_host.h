@@ -26,6 +26,8 @@
 namespace phx {
 
 hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t st);
+// pairs[iv[2 k]].bound = iv[2 k + 1] for k < n.
+hipError_t launch_set_bounds(PairDesc* pairs, const uint32_t* iv, uint32_t n, hipStream_t st);
 hipError_t launch_myers(int nb_template, const MyersArgs& a, hipStream_t st);
 hipError_t launch_wfa_wave(int mode, bool lds, const WfaArgs& a, const WaveGeom& G,
                            uint32_t grid, hipStream_t st);

@@ -666,8 +666,8 @@ __device__ __forceinline__ bool affine_regs_any(const WaveStore<true>& S, int32_
 // ints {m at lane 0, d at lane 0, m at lane 63, i at lane 63} per (slot,
 // chunk) with slot = step % (MD + 1); entries are indexed c + 1 and the
 // entries of c = -1 and c = C stay zero, so the end chunks read "no cell"
-// without a test.  Laid out after the staged sequences.
-constexpr int kBlockEdgeSlots = 6;  // MD + 1 for every shape with a register wavefront
+// without a test.  Laid out after the staged sequences (MD + 1 = 6 slots
+// for every shape with a register wavefront).
 
 struct BlockEdges {
   lmem_t<int32_t>* tab;  // [slot][C + 2][4]
@@ -1951,6 +1951,17 @@ hipError_t launch_gather_runs(const uint32_t* runs, const uint64_t* src, const u
 __global__ void zero_u32_kernel(uint32_t* p, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
+}
+
+__global__ void set_bounds_kernel(PairDesc* pairs, const uint32_t* iv, uint32_t n) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) pairs[iv[2 * k]].bound = iv[2 * k + 1];
+}
+
+hipError_t launch_set_bounds(PairDesc* pairs, const uint32_t* iv, uint32_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(set_bounds_kernel, dim3((n + 255) / 256), dim3(256), 0, st, pairs, iv, n);
+  return hipGetLastError();
 }
 
 hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t st) {

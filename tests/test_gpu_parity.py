@@ -458,9 +458,13 @@ def test_workgroup_register_wavefront(px, costs):
     band pruning to each pair's step estimate; a pair whose optimum is far
     above its estimate (the band empties) and, with every pair forced onto
     the register path, pairs whose range outgrows the 4,096-diagonal window
-    of the 256-thread tier stop and are re-run on the ring path.  Every
-    result equals the oracle's, and the launches show the register storage
-    was used (and the re-runs)."""
+    of the 256-thread tier stop and are re-run on the ring path.  With the
+    band's bound cut to a thousandth of the estimate (band_permille = 1, set
+    at context creation and again through set_tuning on a plan) every
+    register pair's band empties and the whole set is re-run on the
+    register path with its estimate as bound.  Every result equals the
+    oracle's, and the launches show the register storage was used (and the
+    re-runs)."""
     rng = random.Random(costs[0] * 100 + costs[1])
     qs, ts = [], []
     for L, rate in ((1100, 0.01), (1500, 0.05), (2000, 0.15), (2600, 0.08), (3300, 0.02),
@@ -491,9 +495,11 @@ def test_workgroup_register_wavefront(px, costs):
     # empties before the end and the pair is re-run without a bound.
     P, D, Q, R = (rand_seq(rng, n) for n in (1500, 800, 1500, 400))
     uq, ut = [P + D + Q + R + R, qs[20]], [P + D + D + Q, ts[20]]
+    regs_pairs = {}
     for tun, pairs, label in (
             (dict(block256_min_len=1024, block1024_min_len=6000), (uq, ut), "underestimated"),
-            (dict(block256_min_len=1024, block1024_min_len=6000), (qs, ts), "tiers"),
+            (dict(block256_min_len=1024, block1024_min_len=6000, band_permille=1000), (qs, ts), "tiers"),
+            (dict(block256_min_len=1024, block1024_min_len=6000, band_permille=1), (qs, ts), "relaxed"),
             (dict(block256_min_len=1024, block1024_min_len=6000), (bq, ts), "bytes"),
             (dict(block256_min_len=1024, block1024_min_len=20000, block_regs=1, heavy_permille=1),
              (qs[:6] + wq, ts[:6] + wt), "forced")):
@@ -508,6 +514,10 @@ def test_workgroup_register_wavefront(px, costs):
         regs = [r for r in info if r["storage"] == 2]
         say(f"{label}: launches {[(r['tier'], r['storage'], r['pairs']) for r in info]}")
         assert regs, "no launch used the register wavefront"
+        regs_pairs[label] = sum(r["pairs"] for r in regs)
+        if label == "relaxed":
+            # every register pair ran twice: its band, then the estimate
+            assert regs_pairs["relaxed"] == 2 * regs_pairs["tiers"], info
         if label == "tiers":
             assert {r["tier"] for r in regs} == {2, 3}
         if label == "underestimated":
@@ -517,6 +527,18 @@ def test_workgroup_register_wavefront(px, costs):
             # launch takes both (other ring launches here hold one pair)
             assert max(r["pairs"] for r in info if r["storage"] == 0 and r["tier"] >= 2) >= 2
         compare(px, "affine", *pairs, costs=costs, label=f"block regs {label}", res=res)
+    # The bound follows set_tuning on a live plan, and back.
+    ctx = px.Context(0, dict(block256_min_len=1024, block1024_min_len=6000))
+    plan = px.Plan(px.pack_batch(qs, ts), px.ENGINE_AFFINE, px.AffineCosts(*costs), ctx=ctx)
+    for pm, runs in ((1, 2), (1000, 1)):
+        plan.set_tuning(dict(band_permille=pm))
+        plan.run()
+        res = plan.results()
+        n_regs = sum(r["pairs"] for r in plan.last_launch_info() if r["storage"] == 2)
+        assert n_regs == runs * regs_pairs["tiers"], (pm, plan.last_launch_info())
+        compare(px, "affine", qs, ts, costs=costs, label=f"block regs set_tuning {pm}", res=res)
+    plan.close()
+    ctx.close()
 
 
 @pytest.mark.parametrize("costs", [(4, 6, 2), (2, 4, 1), (1, 0, 1), (3, 5, 2), (1, 30, 1)])
