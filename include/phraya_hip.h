@@ -105,7 +105,7 @@ typedef struct phx_tuning {
   uint32_t hbm_slots;        /* HBM wave tier: pairs in flight */
   uint32_t hbm_slot_mib;     /* HBM wave tier: history slot per pair when no estimate (MiB) */
   uint64_t hbm_budget_mib;   /* device scratch budget of the HBM tiers (MiB) */
-  uint32_t block256_min_len; /* qn + tn above which a pair runs a 256-thread workgroup (0 = 8192) */
+  uint32_t block256_min_len; /* qn + tn above which a pair runs a 256-thread workgroup (0 = 2048) */
   uint32_t block1024_min_len;/* ... a 1024-thread workgroup (0 = 16384) */
   uint32_t heavy_permille;   /* per mille of each length bucket (largest estimates) run one tier wider (0 = 50) */
   uint32_t block_lds_ring;   /* workgroup tiers: LDS ring width in diagonals for pairs whose wavefront fits
@@ -125,9 +125,10 @@ typedef struct phx_tuning {
                                 estimate x band_permille / 1000 (0 = 800; at most 1000); a pair whose
                                 optimum exceeds it is re-run with the whole estimate as bound, then
                                 without one.  Results do not depend on it. */
-  uint32_t regs_narrow;      /* workgroup register tiers: a pair of the 1024-thread tier whose band fits the
-                                256-thread tier's window (4,096 diagonals) runs there, four pairs per CU
-                                (0 or 1 = on, 2 = off) */
+  uint32_t regs_narrow;      /* workgroup register tiers: a pair of the 1024-thread tier runs on the
+                                narrowest register tier whose window holds its band: 256 threads (4,096
+                                diagonals, four pairs per CU) or 512 (8,192, two per CU) (0 or 1 = on,
+                                2 = off) */
 } phx_tuning;
 
 /* Context: owns the stream, device buffers and the submission lock.
@@ -192,7 +193,8 @@ uint32_t phx_plan_last_launches(const phx_plan* plan);
  * wavefront cells written and score steps.  Instrumentation; no counterpart
  * in the reference. */
 typedef struct phx_launch_info {
-  int32_t tier;
+  int32_t tier;     /* 0 LDS wave, 1 HBM wave, 2 256-thread, 3 1024-thread, 4 512-thread workgroup,
+                       8 Myers */
   uint32_t pairs;
   uint32_t grid;
   uint32_t storage; /* workgroup tiers: 0 ring in HBM, 1 ring in LDS, 2 register wavefront;

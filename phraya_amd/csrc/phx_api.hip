@@ -396,6 +396,7 @@ GroupSlots group_slots(const phx_plan* P, int t, const std::vector<uint32_t>& wo
   // Pairs in flight: one per wave (wave tier) or per workgroup (block tiers).
   const uint64_t max_flight = t == TIER_WAVE ? P->tun.hbm_slots
                             : t == TIER_BLOCK256 ? (uint64_t)P->ctx->cu_count * 4 * P->tun.block_flight
+                            : t == TIER_BLOCK512 ? (uint64_t)P->ctx->cu_count * 2 * P->tun.block_flight
                                                  : (uint64_t)P->ctx->cu_count * P->tun.block_flight;
   g.n = std::max<uint64_t>(1, budget / g.per);
   g.n = std::min<uint64_t>(std::min<uint64_t>(g.n, max_flight), work.size());
@@ -460,7 +461,7 @@ int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint3
     slots = (uint32_t)gs.n;
     if (storage == 1) G = block_lds_geometry(ring_frames(P->engine, P->costs), gs.wcap, seq_words);
     if (storage == 2) {
-      G = block_regs_geometry(t == TIER_BLOCK256 ? 256 : 1024, seq_words);
+      G = block_regs_geometry(tier_threads(t), seq_words);
       a.band = 1;
     }
   }
@@ -528,14 +529,19 @@ int issue_groups(phx_plan* P, bool capture, std::vector<int>* launched,
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     return P->group_load[a] > P->group_load[b];
   });
-  // The wide register group runs alone on stream 0 and is launched last:
-  // its workgroups (one per CU) take the CUs the other groups leave, and
-  // the rest of them start as the other groups' workgroups finish.
+  // The wide register group runs alone on stream 0.  Its workgroups each
+  // need a whole CU, which the persistent workgroups of the other groups
+  // free only as those groups finish.  When it holds at most one pair per
+  // CU (the longest pairs of the batch, a few rounds of steps each) it is
+  // launched first, so those pairs start at once; a larger one is launched
+  // last: its workgroups take the CUs the other groups leave, and the rest
+  // start as the other groups' workgroups finish.
   const int wide = P->wide_group >= 0 && P->wide_group < ngroups && !P->work[P->wide_group].empty()
                        ? P->wide_group : -1;
   if (wide >= 0) {
     order.erase(std::find(order.begin(), order.end(), wide));
-    order.push_back(wide);
+    if (P->work[wide].size() <= (size_t)ctx->cu_count) order.insert(order.begin(), wide);
+    else order.push_back(wide);
   }
   // Scratch: one arena (grow-only), a disjoint slice per HBM group.
   std::vector<GroupSlots> gs(ngroups);
@@ -716,7 +722,7 @@ int run_wfa(phx_plan* P) {
   // larger and fits the 1024-thread window.
   auto relaxable = [&](int gi, uint32_t i) {
     const PairDesc& pd = P->pairs[i];
-    const int nt = P->group_tier[gi] == TIER_BLOCK256 ? 256 : 1024;
+    const int nt = tier_threads(P->group_tier[gi]);
     return !P->fixed() && P->group_rl[gi] == 2 && outs[i].status == ST_OVERFLOW_WIDTH && pd.bound > 0 &&
            (uint64_t)pd.bound + 4 <= reg_window(nt) && P->est[i] > (int32_t)pd.bound &&
            (uint64_t)P->est[i] + 4 <= reg_window(1024);
@@ -954,18 +960,22 @@ void plan_group(phx_plan* P) {
           where = 1;
         parts[where].push_back(i);
       }
-      // Register pairs of the 1024-thread tier whose band fits the
-      // 256-thread window run there: four pairs per CU in flight instead of
-      // one, whose step barriers and latencies overlap.
-      std::vector<uint32_t> narrow;
+      // Register pairs of the 1024-thread tier run on the narrowest register
+      // tier whose window holds their band: 256 threads (4,096 diagonals,
+      // four pairs per CU in flight) or 512 (8,192, two per CU), whose step
+      // barriers and latencies overlap, instead of one pair per CU.
+      std::vector<uint32_t> narrow, mid;
       if (tier == TIER_BLOCK1024 && P->tun.regs_narrow != 2) {
         std::vector<uint32_t> keep;
-        for (uint32_t i : parts[2])
-          (P->pairs[i].bound && (uint64_t)P->pairs[i].bound + 4 <= reg_window(256) ? narrow : keep).push_back(i);
+        for (uint32_t i : parts[2]) {
+          const uint64_t b = P->pairs[i].bound;
+          (b && b + 4 <= reg_window(256) ? narrow : b && b + 4 <= reg_window(512) ? mid : keep).push_back(i);
+        }
         parts[2].swap(keep);
+        std::stable_sort(mid.begin(), mid.end(), [&](uint32_t a, uint32_t b) { return P->est[a] > P->est[b]; });
       }
-      for (int r = 0; r < 4; ++r) {
-        std::vector<uint32_t>& part = r < 3 ? parts[r] : narrow;
+      for (int r = 0; r < 5; ++r) {
+        std::vector<uint32_t>& part = r < 3 ? parts[r] : r == 3 ? narrow : mid;
         if (part.empty()) continue;
         uint64_t mx = 0;
         double v = 0;
@@ -976,7 +986,7 @@ void plan_group(phx_plan* P) {
           v += (double)wi;
         }
         P->work.push_back(part);
-        P->group_tier.push_back(r < 3 ? tier : TIER_BLOCK256);
+        P->group_tier.push_back(r < 3 ? tier : r == 3 ? TIER_BLOCK256 : TIER_BLOCK512);
         P->group_rl.push_back((uint8_t)(r < 3 ? r : 2));
         P->group_slot.push_back((mx * 4 + 255) & ~(uint64_t)255);
         vol.push_back(v);
@@ -990,6 +1000,37 @@ void plan_group(phx_plan* P) {
   // leave CUs idle when the smaller ones finish first).  Its history slots
   // are sized for its largest pair, so it is given its pairs in flight (one
   // per CU) from the budget before the other groups share the rest.
+  // Likewise the 512-thread register groups of the length buckets become
+  // one group, longest estimate first.
+  {
+    std::vector<uint32_t> merged;
+    uint64_t slot = 0;
+    double v = 0;
+    size_t first = SIZE_MAX;
+    std::vector<size_t> drop;
+    for (size_t g = 0; g < P->work.size(); ++g) {
+      if (P->group_tier[g] != TIER_BLOCK512) continue;
+      merged.insert(merged.end(), P->work[g].begin(), P->work[g].end());
+      slot = std::max(slot, P->group_slot[g]);
+      v += vol[g];
+      if (first == SIZE_MAX) first = g;
+      else drop.push_back(g);
+    }
+    if (!drop.empty()) {
+      std::stable_sort(merged.begin(), merged.end(), [&](uint32_t a, uint32_t b) { return P->est[a] > P->est[b]; });
+      P->work[first] = std::move(merged);
+      P->group_slot[first] = slot;
+      vol[first] = v;
+      for (size_t k = drop.size(); k-- > 0;) {
+        const size_t g = drop[k];
+        P->work.erase(P->work.begin() + g);
+        P->group_tier.erase(P->group_tier.begin() + g);
+        P->group_rl.erase(P->group_rl.begin() + g);
+        P->group_slot.erase(P->group_slot.begin() + g);
+        vol.erase(vol.begin() + g);  // (row-group segments index only the LDS groups in front)
+      }
+    }
+  }
   int wide = -1;
   P->wide_group = -1;
   {
@@ -1039,6 +1080,7 @@ void plan_group(phx_plan* P) {
   auto tier_flight = [&](int t) -> uint64_t {
     return t == TIER_WAVE ? P->tun.hbm_slots
          : t == TIER_BLOCK256 ? (uint64_t)ctx->cu_count * 4 * P->tun.block_flight
+         : t == TIER_BLOCK512 ? (uint64_t)ctx->cu_count * 2 * P->tun.block_flight
                               : (uint64_t)ctx->cu_count * P->tun.block_flight;
   };
   std::vector<double> ask(P->work.size(), 0.0);
@@ -1078,7 +1120,7 @@ void plan_group(phx_plan* P) {
   P->group_load.assign(P->work.size(), 0.0);
   for (size_t g = 0; g < P->work.size(); ++g) {
     const int t = P->group_tier[g];
-    const double threads = t == TIER_BLOCK1024 ? 1024 : t == TIER_BLOCK256 ? 256 : 64;
+    const double threads = t >= TIER_BLOCK256 ? tier_threads(t) : 64;
     const double flight = t == TIER_LDS ? (double)P->tun.wave_slots
                         : (double)tier_flight(t);
     double sum = 0, longest = 0;
@@ -1495,7 +1537,7 @@ phx_ctx* phx_ctx_create(int device, uint32_t flags, const phx_tuning* tuning) {
   d.hbm_budget_mib = 65536;
   if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b > 0)
     d.hbm_budget_mib = std::max<uint64_t>(1024, (uint64_t)(0.6 * (double)free_b) >> 20);
-  d.block256_min_len = 8192;
+  d.block256_min_len = 2048;
   d.block1024_min_len = 16384;
   d.heavy_permille = 50;
   d.block_lds_ring = 1024;
@@ -1856,7 +1898,7 @@ int phx_plan_autotune(phx_plan* P, uint32_t max_trials, phx_tuning* best_out, do
   // Stage 2: pairs in flight of the workgroup tiers (a second workgroup per
   // CU hides memory latency when a group has pairs to spare), then of the
   // HBM wave tier.
-  if (pairs_in(TIER_BLOCK256) + pairs_in(TIER_BLOCK1024) > 2 * (size_t)cu) {
+  if (pairs_in(TIER_BLOCK256) + pairs_in(TIER_BLOCK512) + pairs_in(TIER_BLOCK1024) > 2 * (size_t)cu) {
     phx_tuning t = best;
     t.block_flight = 2;
     if ((rc = consider(t)) != PHX_OK) return rc;

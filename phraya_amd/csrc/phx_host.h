@@ -57,8 +57,12 @@ enum Tier : int {
   TIER_LDS = 0,      // wave per pair, everything in LDS
   TIER_WAVE = 1,     // wave per pair, ring and history in an HBM slot
   TIER_BLOCK256 = 2, // 256-thread workgroup per pair, HBM slot
-  TIER_BLOCK1024 = 3 // 1024-thread workgroup per pair, HBM slot
+  TIER_BLOCK1024 = 3,// 1024-thread workgroup per pair, HBM slot
+  TIER_BLOCK512 = 4  // 512-thread workgroup per pair, register wavefront only (two per CU)
 };
+
+// Threads per pair of a workgroup tier.
+inline int tier_threads(int t) { return t == TIER_BLOCK256 ? 256 : t == TIER_BLOCK512 ? 512 : 1024; }
 
 inline const uint8_t* acgt_table() {
   static const std::array<uint8_t, 256> t = [] {
@@ -564,8 +568,7 @@ inline hipError_t launch_tier(int tier, int mode, const WfaArgs& a, const WaveGe
     const uint32_t wgs = std::max<uint32_t>(1, (slots + kWaveWaves - 1) / kWaveWaves);
     return launch_wfa_wave(mode, tier == TIER_LDS, a, G, wgs, st);
   }
-  return launch_wfa_block(mode, tier == TIER_BLOCK256 ? 256 : 1024, storage, a, G,
-                          std::max<uint32_t>(1, slots), st);
+  return launch_wfa_block(mode, tier_threads(tier), storage, a, G, std::max<uint32_t>(1, slots), st);
 }
 
 // LDS layout of a block-tier workgroup with the ring in LDS: int32 ring
@@ -585,7 +588,7 @@ constexpr uint32_t kBlockLdsMax = 64u * 1024u;  // dynamic LDS a block-tier work
 // 256-thread tier 4,096; chunk-end tables of kRegEdgeSlots steps after the
 // staged sequences.
 constexpr uint32_t kRegEdgeSlots = 6;
-inline uint32_t reg_chunks(int tier_threads) { return tier_threads == 256 ? 4u * 16u : 16u * 16u; }
+inline uint32_t reg_chunks(int tier_threads) { return (uint32_t)tier_threads / 64u * 16u; }
 inline uint32_t reg_window(int tier_threads) { return 64u * reg_chunks(tier_threads); }
 inline WaveGeom block_regs_geometry(int tier_threads, uint32_t seq_words) {
   WaveGeom G{};

@@ -1991,21 +1991,24 @@ hipError_t launch_wfa_rows(bool b8, const WfaArgs& a, const WaveGeom& G, uint32_
   return hipGetLastError();
 }
 
-// nt = threads per pair: 256 or 1024; storage 0: ring in HBM, 1: ring and
-// sequences in LDS, 2: register wavefront, sequences and chunk ends in LDS
-// (G gives the layout; dynamic LDS = G.per_wave_bytes).
+// nt = threads per pair: 256, 512 (register wavefront only) or 1024;
+// storage 0: ring in HBM, 1: ring and sequences in LDS, 2: register
+// wavefront, sequences and chunk ends in LDS (G gives the layout; dynamic
+// LDS = G.per_wave_bytes).
 template <int MODE>
 hipError_t launch_block_mode(int nt, int storage, const WfaArgs& a, const WaveGeom& G, uint32_t grid,
                              hipStream_t st) {
   const bool rl = storage != 0;
   const size_t lds = rl ? G.per_wave_bytes : 0;
   if (storage == 2 && MODE == ENG_AFFINE) {
-    // 16 chunks per wave: 16,384 diagonals for the wide tier, 4,096 for the
-    // narrow one.
+    // 16 chunks per wave: 16,384 diagonals for 1024 threads, 8,192 for 512,
+    // 4,096 for 256.
     if (nt == 1024) hipLaunchKernelGGL((wfa_block_kernel<MODE, 1024, true, 16>), dim3(grid), dim3(1024), lds, st, a, G);
+    else if (nt == 512) hipLaunchKernelGGL((wfa_block_kernel<MODE, 512, true, 16>), dim3(grid), dim3(512), lds, st, a, G);
     else hipLaunchKernelGGL((wfa_block_kernel<MODE, 256, true, 16>), dim3(grid), dim3(256), lds, st, a, G);
     return hipGetLastError();
   }
+  if (nt == 512) return hipErrorInvalidValue;  // register wavefront only
   if (nt == 1024) {
     if (rl) hipLaunchKernelGGL((wfa_block_kernel<MODE, 1024, true>), dim3(grid), dim3(1024), lds, st, a, G);
     else hipLaunchKernelGGL((wfa_block_kernel<MODE, 1024, false>), dim3(grid), dim3(1024), 0, st, a, G);

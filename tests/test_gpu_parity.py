@@ -458,7 +458,9 @@ def test_workgroup_register_wavefront(px, costs):
     band pruning to each pair's step estimate; a pair whose optimum is far
     above its estimate (the band empties) and, with every pair forced onto
     the register path, pairs whose range outgrows the 4,096-diagonal window
-    of the 256-thread tier stop and are re-run on the ring path.  With the
+    of the 256-thread tier stop and are re-run on the ring path.  By default
+    the 1024-thread tier's register pairs run on the narrowest register tier
+    whose window holds their band (256 or 512 threads).  With the
     band's bound cut to a thousandth of the estimate (band_permille = 1, set
     at context creation and again through set_tuning on a plan) every
     register pair's band empties and the whole set is re-run on the
@@ -498,7 +500,9 @@ def test_workgroup_register_wavefront(px, costs):
     regs_pairs = {}
     for tun, pairs, label in (
             (dict(block256_min_len=1024, block1024_min_len=6000), (uq, ut), "underestimated"),
-            (dict(block256_min_len=1024, block1024_min_len=6000, band_permille=1000), (qs, ts), "tiers"),
+            (dict(block256_min_len=1024, block1024_min_len=6000, band_permille=1000, regs_narrow=2), (qs, ts),
+             "tiers"),
+            (dict(block256_min_len=1024, block1024_min_len=6000, band_permille=1000), (qs, ts), "narrowest"),
             (dict(block256_min_len=1024, block1024_min_len=6000, band_permille=1), (qs, ts), "relaxed"),
             (dict(block256_min_len=1024, block1024_min_len=6000), (bq, ts), "bytes"),
             (dict(block256_min_len=1024, block1024_min_len=20000, block_regs=1, heavy_permille=1),
@@ -520,6 +524,10 @@ def test_workgroup_register_wavefront(px, costs):
             assert regs_pairs["relaxed"] == 2 * regs_pairs["tiers"], info
         if label == "tiers":
             assert {r["tier"] for r in regs} == {2, 3}
+        if label == "narrowest":
+            # the 1024-thread tier's pairs run on the 256- or 512-thread one
+            assert 4 in {r["tier"] for r in regs} and 3 not in {r["tier"] for r in regs}, info
+            assert regs_pairs["narrowest"] == regs_pairs["tiers"]
         if label == "underestimated":
             assert any(r["storage"] == 0 and r["tier"] >= 2 for r in info), info
         if label == "forced":
