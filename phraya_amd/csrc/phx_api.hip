@@ -953,7 +953,9 @@ void plan_group(phx_plan* P) {
         // an unbounded one spans its whole range, about 2 steps + 1.
         const bool fits_window = P->pairs[i].bound ? (uint64_t)P->pairs[i].bound + 4 <= reg_window(nt)
                                                    : (uint64_t)P->est[i] * 4 / 5 < reg_window(nt) / 2;
-        if (regs_ok && (P->tun.block_regs == 1 || fits_window) && RG.per_wave_bytes + seq_b <= kBlockLdsMax)
+        // Register frames hold offset + 1 in 16 bits: queries up to 65,534 bases.
+        if (regs_ok && (P->tun.block_regs == 1 || fits_window) && P->qlen[i] <= kRegMaxQuery &&
+            RG.per_wave_bytes + seq_b <= kBlockLdsMax)
           where = 2;
         else if (tier >= TIER_BLOCK256 && W > 1 && ring_bytes + 1024 <= kBlockLdsMax &&
                  width <= W && ring_bytes + seq_b <= kBlockLdsMax)

@@ -589,6 +589,9 @@ constexpr uint32_t kBlockLdsMax = 64u * 1024u;  // dynamic LDS a block-tier work
 // staged sequences.
 constexpr uint32_t kRegEdgeSlots = 6;
 inline uint32_t reg_chunks(int tier_threads) { return (uint32_t)tier_threads / 64u * 16u; }
+// Longest query the register wavefront takes: its frames pack offset + 1
+// (at most qn + 1) into 16-bit halves.
+constexpr uint64_t kRegMaxQuery = 65534;
 inline uint32_t reg_window(int tier_threads) { return 64u * reg_chunks(tier_threads); }
 inline WaveGeom block_regs_geometry(int tier_threads, uint32_t seq_words) {
   WaveGeom G{};

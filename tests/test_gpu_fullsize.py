@@ -227,3 +227,33 @@ def test_caps_on_workgroup_tiers(cap):
         qs.append(mutate(rng, t[:L], rng.choice([0.02, 0.08])))
         ts.append(t)
     compare(px, "affine", qs, ts, costs=(4, 6, 2), cap=cap, label=f"long cap {cap}")
+
+
+def test_queries_past_the_register_frames():
+    """The register wavefront keeps offset + 1 in 16-bit frame halves, so a
+    query longer than 65,534 bases must run on a ring tier even when its band
+    fits the register window and its sequences fit in LDS (a 70 kbp pair at
+    0.2% does both).  Pairs on either side of the limit, every one checked
+    optimal, and none past it on a register launch."""
+    import random
+    import phraya_amd as px
+    from test_gpu_parity import mutate, rand_seq
+    rng = random.Random(65535)
+    qs, ts = [], []
+    for L in (60000, 65534, 65535, 65536, 70000, 72000):
+        q = rand_seq(rng, L)
+        qs.append(q)
+        ts.append(mutate(rng, q, 0.002))
+    pb = px.pack_batch(qs, ts)
+    plan = px.Plan(pb, px.ENGINE_AFFINE, px.AffineCosts(4, 6, 2))
+    plan.run()
+    res = plan.results()
+    for li in plan.last_launch_info():
+        say(f"register frames: launch tier {li['tier']} storage {li['storage']} pairs {li['pairs']}")
+    regs = sum(li["pairs"] for li in plan.last_launch_info() if li["tier"] >= 2 and li["storage"] == 2)
+    plan.close()
+    assert regs <= 2, f"{regs} pairs on register launches; only the two of <= 65,534 bases may be"
+    o = lambda xs: np.concatenate([[0], np.cumsum([len(x) for x in xs])]).astype(np.uint64)
+    cat = lambda xs: np.frombuffer(b"".join(xs), np.uint8)
+    check_all("register frames", cat(qs), o(qs), cat(ts), o(ts), res.aln, res.runs, res.run_off,
+              (4, 6, 2), chunks=1)
