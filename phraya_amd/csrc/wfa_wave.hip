@@ -348,6 +348,14 @@ __device__ __forceinline__ int32_t from_below(int32_t v) {
 __device__ __forceinline__ int32_t from_above(int32_t v) {
   return __builtin_amdgcn_mov_dpp(v, 0x130, 0xF, 0xF, true);  // wave_shl:1
 }
+// The same shifts with the end lane taking `edge` instead of 0 (the DPP
+// move leaves a lane with no source lane unwritten, so it keeps `edge`).
+__device__ __forceinline__ int32_t below_or(int32_t edge, int32_t v) {
+  return __builtin_amdgcn_update_dpp(edge, v, 0x138, 0xF, 0xF, false);  // wave_shr:1
+}
+__device__ __forceinline__ int32_t above_or(int32_t edge, int32_t v) {
+  return __builtin_amdgcn_update_dpp(edge, v, 0x130, 0xF, 0xF, false);  // wave_shl:1
+}
 
 // Neighbours over a window of CH chunks of 64 diagonals (chunk c, lane L =
 // diagonal kb + 64 c + L): the lane below / above, across chunk edges too;
@@ -803,15 +811,22 @@ __device__ void affine_block_regs(const WaveStore<false, true>& S, const BlockEd
     // Table rows of steps s - OEN and s - EN (s >= 1; a step before 0 maps
     // to a row no step has written yet, which reads as no cell) and of s.
     const int32_t ro = ((s + SL - OEN) % SL) * CE, re = ((s + SL - EN) % SL) * CE, rw = (s % SL) * CE;
-    // This wave's chunks c = w + W j that the step range reaches.  A wave
-    // with none runs one chunk outside the range (no cell changes: every
-    // lane is out of range), so the loop always runs: a loop that may run
-    // zero times makes the compiler copy every frame vector each step.
-    // With a bound the chunks next to the band are run too, with no cell in
-    // range: a chunk the band has left publishes "no cell" ends for the
-    // chunk beside it (its earlier ends would otherwise stay in the table).
-    const int32_t c_lo = max(((lo - kb) >> 6) - (band.bound > 0), 0);
-    const int32_t c_hi = min(((hi - kb) >> 6) + (band.bound > 0), C - 1);
+    // This wave's chunks c = w + W j that the step range reaches (both
+    // ends lie in the window, checked above).  A wave with none runs one
+    // chunk outside the range (no cell changes: every lane is out of
+    // range), so the loop always runs: a loop that may run zero times makes
+    // the compiler copy every frame vector each step.
+    const int32_t c_lo = (lo - kb) >> 6, c_hi = (hi - kb) >> 6;
+    // With a bound the band also shrinks, and a chunk it has left would
+    // keep its earlier ends in the table: the chunks next to the range get
+    // "no cell" ends in this step's row (entries c_lo and c_hi + 2, inside
+    // the row: entries 0 and C + 1 are the row's permanent zero entries).
+    // Their frames need nothing: a chunk's steps inside the band are one
+    // interval (lo(s) falls then rises, hi(s) rises then falls), so a
+    // chunk the band reaches again later does not exist, and one it has not
+    // reached yet still holds zero frames.
+    if (band.bound > 0 && w == 0 && lane < 8)
+      tab[(rw + (lane < 4 ? c_lo : c_hi + 2)) * 4 + (lane & 3)] = 0;
     int32_t j = c_lo <= w ? 0 : (c_lo - w + W - 1) / W;
     int32_t j_end = c_hi < w ? -1 : (c_hi - w) / W;
     if (j > j_end) {
@@ -828,15 +843,13 @@ __device__ void affine_block_regs(const WaveStore<false, true>& S, const BlockEd
       const uint32_t idr = FID[j];
       const int32_t om = frame_m<MD>(m, OEN);
       const int32_t ie = (int32_t)(idr & 0xFFFFu), de = (int32_t)(idr >> 16);
-      int32_t vol = from_below(om), vor = from_above(om), vil = from_below(ie), vdr = from_above(de);
       // Chunk ends: the neighbouring chunks' end lanes, published earlier
-      // (entry c + 1 of the row; c - 1 and c + 1 are entries c and c + 2).
+      // (entry c + 1 of the row; c - 1 and c + 1 are entries c and c + 2),
+      // taken by lane 0 / lane 63 in the DPP shifts.
       const int32_t eb_m = tab[(ro + c) * 4 + 2], ea_m = tab[(ro + c + 2) * 4 + 0];
       const int32_t eb_i = tab[(re + c) * 4 + 3], ea_d = tab[(re + c + 2) * 4 + 1];
-      vol = lane == 0 ? eb_m : vol;
-      vil = lane == 0 ? eb_i : vil;
-      vor = lane == 63 ? ea_m : vor;
-      vdr = lane == 63 ? ea_d : vdr;
+      const int32_t vol = below_or(eb_m, om), vor = above_or(ea_m, om);
+      const int32_t vil = below_or(eb_i, ie), vdr = above_or(ea_d, de);
       const int32_t k = kb + 64 * c + lane;
       const uint32_t kl = (uint32_t)(k - lo);
       const bool inr = kl <= (uint32_t)(width - 1);
