@@ -51,6 +51,12 @@ struct phx_plan {
   DevBuf d_bounds;  // (pair, bound) pairs of a relaxed-bound re-run
   DevBuf d_gather, d_gidx;  // compacted CIGAR runs and their source / target offsets
   DevBuf d_arena;  // HBM-tier scratch (history + ring slots); each group of a run gets its own slice
+  // Row stage (banded-row groups): RowDesc records (one per list position of
+  // every row group, in list order), their pair indices, then the staged
+  // sequence words; group_rdesc[g] = first record of group g (~0: none).
+  DevBuf d_rstage;
+  std::vector<uint64_t> group_rdesc;
+  uint64_t rstage_recs = 0, rstage_words = 0;          // its capacity (0 records: no row stage)
   uint32_t* ctr_host = nullptr;  // pinned overflow flags, one per group (kernels write them)
   uint32_t* ctr_dev = nullptr;   // the same flags' device address
   std::vector<std::vector<uint32_t>> work;             // initial work per launch group / Myers group
@@ -135,7 +141,7 @@ struct PlanCache {
   bool busy = false;
   uint8_t* pin = nullptr;  // pinned staging for the plan's uploads and downloads
   size_t pin_bytes = 0;
-  DevBuf seq, pairs, out, runs, work, work2, ctr, scratch, gather, gidx, arena;
+  DevBuf seq, pairs, out, runs, work, work2, ctr, scratch, gather, gidx, arena, rstage;
   uint32_t* ctr_host = nullptr;
   uint32_t* ctr_dev = nullptr;
   std::vector<hipEvent_t> kev;
@@ -160,6 +166,7 @@ void swap_cache(PlanCache& C, phx_plan* P) {
   std::swap(C.gather, P->d_gather);
   std::swap(C.gidx, P->d_gidx);
   std::swap(C.arena, P->d_arena);
+  std::swap(C.rstage, P->d_rstage);
   std::swap(C.ctr_host, P->ctr_host);
   std::swap(C.ctr_dev, P->ctr_dev);
   std::swap(C.kev, P->kev);
@@ -189,7 +196,7 @@ void cache_return(phx_plan* P) {
   phx_ctx* ctx = P->ctx;
   std::lock_guard<std::mutex> lk(ctx->cache_mu);
   for (DevBuf* b : {&P->d_seq, &P->d_pairs, &P->d_out, &P->d_runs, &P->d_work, &P->d_work2, &P->d_ctr,
-                    &P->d_scratch, &P->d_gather, &P->d_gidx, &P->d_arena})
+                    &P->d_scratch, &P->d_gather, &P->d_gidx, &P->d_arena, &P->d_rstage})
     if (b->bytes > kCacheKeepBytes) dev_free(*b);
   swap_cache(*ctx->plan_cache, P);
   ctx->plan_cache->busy = false;
@@ -417,7 +424,8 @@ enum LaunchTiming { TIME_BOTH, TIME_NONE };
 int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint32_t* d_list,
                     uint32_t* d_counters, uint32_t* overflow, const GroupSlots& gs, uint8_t* slice,
                     hipStream_t st, int storage = 0, LaunchTiming timing = TIME_BOTH,
-                    uint32_t seq_words = 0, uint32_t row_steps = 0, const uint32_t* rows_seg = nullptr) {
+                    uint32_t seq_words = 0, uint32_t row_steps = 0, const uint32_t* rows_seg = nullptr,
+                    const RowDesc* rows_desc = nullptr, const uint64_t* rows_seq = nullptr) {
   const bool rl = storage != 0;
   phx_ctx* ctx = P->ctx;
   WfaArgs a{};
@@ -449,6 +457,8 @@ int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint3
         a.rows_seg[j] = seg[j];
         sets += (seg[j] + (j < 2 ? 1u : 3u)) / (j < 2 ? 2u : 4u);
       }
+      a.rows_desc = rows_desc;
+      a.rows_seq = rows_seq;
     }
     slots = lds_tier_waves(sets, P->tun.wave_slots);
     a.static_order = 1;  // the list is stride-balanced for exactly these waves
@@ -495,6 +505,8 @@ int launch_wfa_tier(phx_plan* P, int t, const std::vector<uint32_t>& work, uint3
 // groups on one stream run one after another), join, counter readback.
 // capture: the streams are being captured into a graph; no timing events are
 // recorded and each kernel node is returned in *knodes.
+uint64_t rstage_seq_off(const phx_plan* P);
+
 int issue_groups(phx_plan* P, bool capture, std::vector<int>* launched,
                  std::vector<hipGraphNode_t>* knodes) {
   phx_ctx* ctx = P->ctx;
@@ -583,11 +595,15 @@ int issue_groups(phx_plan* P, bool capture, std::vector<int>* launched,
       HIPCHK(hipStreamWaitEvent(st, ctx->join[0], 0));
       used_stream[si] = true;
     }
+    const bool staged = gi < (int)P->group_rdesc.size() && P->group_rdesc[gi] != ~0ull;
+    uint8_t* const rbase = (uint8_t*)P->d_rstage.p;
     int rc = launch_wfa_tier(P, tier[gi], wl, (uint32_t*)P->d_work.p + group_off[gi],
                              d_ctr + 2 * gi, P->ctr_dev + gi, gs[gi],
                              (uint8_t*)P->d_arena.p + slice_off[gi], st, P->group_rl[gi],
                              timed ? TIME_BOTH : TIME_NONE, P->group_seqw[gi], P->group_rows[gi],
-                             P->group_rowseg[gi].data());
+                             P->group_rowseg[gi].data(),
+                             staged ? (const RowDesc*)rbase + P->group_rdesc[gi] : nullptr,
+                             staged ? (const uint64_t*)(rbase + rstage_seq_off(P)) : nullptr);
     if (rc != PHX_OK) return rc;
     if (capture) {
       hipStreamCaptureStatus cs;
@@ -1173,6 +1189,82 @@ void apply_bounds(phx_plan* P) {
   }
 }
 
+// Row stage capacity: a record and the staged words of every pair a banded
+// row may take (whatever the grouping); none when a pair's run slots lie
+// past a record's 32-bit offsets (its launches then read PairDesc).
+void row_stage_caps(phx_plan* P) {
+  P->rstage_recs = P->rstage_words = 0;
+  if (P->rows_ok.size() != P->n) return;
+  uint64_t recs = 0, words = 0;
+  for (uint32_t i = 0; i < P->n; ++i) {
+    if (!P->rows_ok[i]) continue;
+    const PairDesc& pd = P->pairs[i];
+    if (pd.run_off + pd.run_cap > UINT32_MAX) return;
+    const bool b8 = (pd.flags & PF_BYTES) != 0;
+    ++recs;
+    words += words_for(pd.qn, b8) + words_for(pd.tn, b8);
+  }
+  if (words > UINT32_MAX) return;
+  P->rstage_recs = recs;
+  P->rstage_words = words;
+}
+
+// d_rstage layout: records, pair indices (16-byte aligned), staged words.
+uint64_t rstage_idx_off(const phx_plan* P) { return P->rstage_recs * sizeof(RowDesc); }
+uint64_t rstage_seq_off(const phx_plan* P) { return rstage_idx_off(P) + ((P->rstage_recs * 4 + 15) & ~15ull); }
+uint64_t rstage_bytes(const phx_plan* P) {
+  return P->rstage_recs ? rstage_seq_off(P) + P->rstage_words * 8 : 0;
+}
+
+// Records and pair indices of the current banded-row groups, in list order
+// (group_rdesc: each group's first record).
+void row_records(phx_plan* P, std::vector<RowDesc>& rd, std::vector<uint32_t>& idx) {
+  P->group_rdesc.assign(P->work.size(), ~0ull);
+  rd.clear();
+  idx.clear();
+  if (!P->rstage_recs) return;
+  uint64_t words = 0;
+  for (size_t g = 0; g < P->work.size(); ++g) {
+    if (P->group_tier[g] != TIER_LDS || P->group_rl[g] < kRowsStorage || P->work[g].empty()) continue;
+    P->group_rdesc[g] = rd.size();
+    for (uint32_t i : P->work[g]) {
+      const PairDesc& pd = P->pairs[i];
+      const bool b8 = (pd.flags & PF_BYTES) != 0;
+      rd.push_back(RowDesc{(uint32_t)words, (uint32_t)pd.run_off, pd.run_cap, (uint16_t)pd.qn, (uint16_t)pd.tn});
+      idx.push_back(i);
+      words += words_for(pd.qn, b8) + words_for(pd.tn, b8);
+    }
+  }
+  if (rd.size() > P->rstage_recs || words > P->rstage_words) {  // (row groups hold only rows_ok pairs)
+    P->group_rdesc.assign(P->work.size(), ~0ull);
+    rd.clear();
+    idx.clear();
+  }
+}
+
+// Copies the row pairs' sequences into the stage (device descriptors and
+// records already uploaded), on the context's stream ahead of every run.
+int stage_rows(phx_plan* P, uint32_t nrec) {
+  phx_ctx* ctx = P->ctx;
+  uint8_t* base = (uint8_t*)P->d_rstage.p;
+  HIPCHK(launch_stage_rows((const PairDesc*)P->d_pairs.p, (const uint32_t*)(base + rstage_idx_off(P)),
+                           (const RowDesc*)base, (uint64_t*)(base + rstage_seq_off(P)), nrec, ctx->stream));
+  return PHX_OK;
+}
+
+// After a regrouping: new records for the new lists, and their stage.
+int restage_rows(phx_plan* P) {
+  phx_ctx* ctx = P->ctx;
+  std::vector<RowDesc> rd;
+  std::vector<uint32_t> idx;
+  row_records(P, rd, idx);
+  if (rd.empty()) return PHX_OK;
+  uint8_t* base = (uint8_t*)P->d_rstage.p;
+  HIPCHK(hipMemcpy(base, rd.data(), rd.size() * sizeof(RowDesc), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(base + rstage_idx_off(P), idx.data(), idx.size() * 4, hipMemcpyHostToDevice));
+  return stage_rows(P, (uint32_t)rd.size());
+}
+
 int upload_work(phx_plan* P) {
   phx_ctx* ctx = P->ctx;
   std::vector<uint32_t> flat;
@@ -1268,6 +1360,13 @@ phx_plan* plan_setup(phx_plan* P, const std::vector<const uint8_t*>& qh,
   if (plan_alloc(P, P->d_runs, P->runs_total * 4 + 4) != PHX_OK) return fail_plan("runs");
   if (plan_alloc(P, P->d_work, (size_t)std::max<uint32_t>(n, 1) * 4) != PHX_OK) return fail_plan("work");
   if (plan_alloc(P, P->d_ctr, 2 * kMaxGroups * sizeof(uint32_t)) != PHX_OK) return fail_plan("counters");
+  row_stage_caps(P);
+  // The stage is an optimisation: a fixed region with no room for it keeps
+  // the batch (its row launches read PairDesc).
+  if (P->fixed() && P->rstage_recs &&
+      ((P->fixed_used + 255) & ~(uint64_t)255) + rstage_bytes(P) + (64u << 10) > P->fixed_cap)
+    P->rstage_recs = P->rstage_words = 0;
+  if (P->rstage_recs && plan_alloc(P, P->d_rstage, rstage_bytes(P)) != PHX_OK) return fail_plan("row stage");
   if (P->fixed()) {
     // Re-run lists and the overflow flags (words of the region, read back
     // after each run) are carved up front: a run allocates nothing.
@@ -1295,16 +1394,30 @@ phx_plan* plan_setup(phx_plan* P, const std::vector<const uint8_t*>& qh,
   // orders after), else straight from pageable memory.
   size_t wl_words = 0;
   for (auto& w : P->work) wl_words += w.size();
+  // Row stage records of the banded-row groups (their sequences are copied
+  // into the stage by a kernel once the descriptors are on the device).
+  std::vector<RowDesc> rd;
+  std::vector<uint32_t> ridx;
+  row_records(P, rd, ridx);
   const size_t seq_b = words * 8, pd_b = sizeof(PairDesc) * (size_t)n, wl_b = wl_words * 4;
-  if (uint8_t* pin = staging(P, seq_b + pd_b + wl_b)) {
+  const size_t rd_b = rd.size() * sizeof(RowDesc), ri_b = ridx.size() * 4;
+  uint8_t* const rbase = (uint8_t*)P->d_rstage.p;
+  if (uint8_t* pin = staging(P, seq_b + pd_b + wl_b + rd_b + ri_b)) {
     if (seq_b) std::memcpy(pin, packed->data(), seq_b);
     if (pd_b) std::memcpy(pin + seq_b, P->pairs.data(), pd_b);
     uint32_t* wl = (uint32_t*)(pin + seq_b + pd_b);
     for (auto& w : P->work) wl = std::copy(w.begin(), w.end(), wl);
+    uint8_t* const pr = pin + seq_b + pd_b + wl_b;
+    if (rd_b) std::memcpy(pr, rd.data(), rd_b);
+    if (ri_b) std::memcpy(pr + rd_b, ridx.data(), ri_b);
     if ((seq_b && hipMemcpyAsync(P->d_seq.p, pin, seq_b, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) ||
         (pd_b && hipMemcpyAsync(P->d_pairs.p, pin + seq_b, pd_b, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) ||
-        (wl_b && hipMemcpyAsync(P->d_work.p, pin + seq_b + pd_b, wl_b, hipMemcpyHostToDevice, ctx->stream) != hipSuccess))
+        (wl_b && hipMemcpyAsync(P->d_work.p, pin + seq_b + pd_b, wl_b, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) ||
+        (rd_b && hipMemcpyAsync(rbase, pr, rd_b, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) ||
+        (ri_b && hipMemcpyAsync(rbase + rstage_idx_off(P), pr + rd_b, ri_b, hipMemcpyHostToDevice,
+                                ctx->stream) != hipSuccess))
       return fail_plan("upload", PHX_E_DEVICE);
+    if (!rd.empty() && stage_rows(P, (uint32_t)rd.size()) != PHX_OK) return fail_plan("row stage", PHX_E_DEVICE);
     if (!P->fixed() && !P->ctr_host &&
         (hipHostMalloc((void**)&P->ctr_host, kMaxGroups * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
          hipHostGetDevicePointer((void**)&P->ctr_dev, P->ctr_host, 0) != hipSuccess))
@@ -1322,6 +1435,11 @@ phx_plan* plan_setup(phx_plan* P, const std::vector<const uint8_t*>& qh,
     return fail_plan("pinned counters");
   // Work lists are static between tunings: upload once.
   if (upload_work(P) != PHX_OK) return fail_plan("work upload", PHX_E_DEVICE);
+  if (!rd.empty() &&
+      (hipMemcpy(rbase, rd.data(), rd_b, hipMemcpyHostToDevice) != hipSuccess ||
+       hipMemcpy(rbase + rstage_idx_off(P), ridx.data(), ri_b, hipMemcpyHostToDevice) != hipSuccess ||
+       stage_rows(P, (uint32_t)rd.size()) != PHX_OK))
+    return fail_plan("row stage", PHX_E_DEVICE);
   return P;
 }
 
@@ -1554,7 +1672,7 @@ void phx_ctx_destroy(phx_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   if (PlanCache* C = c->plan_cache) {
     for (DevBuf* b : {&C->seq, &C->pairs, &C->out, &C->runs, &C->work, &C->work2, &C->ctr, &C->scratch,
-                      &C->gather, &C->gidx, &C->arena})
+                      &C->gather, &C->gidx, &C->arena, &C->rstage})
       dev_free(*b);
     if (C->ctr_host) (void)hipHostFree(C->ctr_host);
     if (C->pin) (void)hipHostFree(C->pin);
@@ -1819,7 +1937,8 @@ int phx_plan_set_tuning(phx_plan* P, const phx_tuning* t) {
     if (P->n) HIPCHK(hipMemcpy(P->d_pairs.p, P->pairs.data(), sizeof(PairDesc) * P->n, hipMemcpyHostToDevice));
   }
   plan_group(P);
-  return upload_work(P);
+  const int rc = upload_work(P);
+  return rc != PHX_OK ? rc : restage_rows(P);
 }
 
 int phx_plan_get_tuning(const phx_plan* P, phx_tuning* t) {
@@ -2033,7 +2152,7 @@ void phx_plan_destroy(phx_plan* P) {
   if (P->cached) cache_return(P);  // leaves the plan with nothing to free
   if (!P->fixed()) {  // a fixed plan's buffers are carves of the scratch array
     for (DevBuf* b : {&P->d_seq, &P->d_pairs, &P->d_out, &P->d_runs, &P->d_work, &P->d_work2, &P->d_ctr,
-                      &P->d_scratch, &P->d_gather, &P->d_gidx})
+                      &P->d_scratch, &P->d_gather, &P->d_gidx, &P->d_rstage})
       dev_free(*b);
     dev_free(P->d_arena);
     dev_free(P->d_bounds);

@@ -106,6 +106,18 @@ __host__ __device__ inline bool rows_shape(int32_t xn, int32_t oen, int32_t en) 
 }
 
 // Kernel arguments of one WFA tier launch.
+// A banded-row pair as its launch reads it: the plan copies each row pair's
+// query and target, shifted to bit 0, into a staging arena in the order of
+// the launch list (so the pairs of one wave sit together), and keeps this
+// 16-byte record per list position (queries and targets of at most 2,048
+// bases: banded rows hold qn + tn <= 2048).
+struct RowDesc {
+  uint32_t q_word;  // first staged word of the query; the target's follows its words
+  uint32_t run_off, run_cap;
+  uint16_t qn, tn;
+};
+static_assert(sizeof(RowDesc) == 16, "RowDesc is one 16-byte load");
+
 struct WfaArgs {
   const PairDesc* pairs;
   const uint32_t* work_list;
@@ -126,6 +138,11 @@ struct WfaArgs {
   // in this order: 32-lane rows with qn <= tn, 32-lane rows with qn > tn,
   // 16-lane rows with qn <= tn, 16-lane rows with qn > tn.
   uint32_t rows_seg[4];
+  // Banded-row launches of a plan with a row stage: the group's records,
+  // parallel to work_list, and the staging arena; null: the launch reads
+  // PairDesc (pairs[work_list[pos]]).
+  const RowDesc* rows_desc;
+  const uint64_t* rows_seq;
 };
 
 // LDS layout of one wave of the wave-resident WFA tier (byte offsets inside

@@ -34,6 +34,8 @@ hipError_t launch_wfa_wave(int mode, bool lds, const WfaArgs& a, const WaveGeom&
 hipError_t launch_wfa_block(int mode, int nt, int storage, const WfaArgs& a, const WaveGeom& G,
                             uint32_t grid, hipStream_t st);
 hipError_t launch_wfa_rows(bool b8, const WfaArgs& a, const WaveGeom& G, uint32_t grid, hipStream_t st);
+hipError_t launch_stage_rows(const PairDesc* pairs, const uint32_t* idx, const RowDesc* rd, uint64_t* stage,
+                             uint32_t n, hipStream_t st);
 hipError_t launch_gather_runs(const uint32_t* runs, const uint64_t* src, const uint64_t* off,
                               uint32_t* dst, uint32_t n, hipStream_t st);
 

@@ -1633,7 +1633,7 @@ __device__ __forceinline__ RowCell row_cell(int32_t vx, int32_t vol, int32_t vor
 // optimum lies beyond its band (nothing stored for it).
 template <int XN, int OEN, int DIR, bool B8, int RW>
 __device__ __forceinline__ bool rows_pass(const WfaArgs& a, const WaveGeom& G, uint8_t* base, int lane,
-                                          uint32_t pi) {
+                                          uint32_t pi, uint32_t pos) {
   constexpr int MD = XN > OEN ? XN : OEN;
   constexpr int32_t B0 = DIR < 0 ? 0 : 1 - RW;
   constexpr uint64_t RMASK = RW == 64 ? ~0ull : (1ull << RW) - 1;
@@ -1641,7 +1641,19 @@ __device__ __forceinline__ bool rows_pass(const WfaArgs& a, const WaveGeom& G, u
   const uint32_t rsh = (uint32_t)lane & (uint32_t)(64 - RW);  // the row's bits in a ballot
   const bool has = pi != 0xFFFFFFFFu;
   PairDesc pd{};
-  if (has) pd = a.pairs[pi];
+  if (has) {
+    if (a.rows_desc) {  // staged at bit 0 in list order (shifts 0)
+      const RowDesc rd = a.rows_desc[pos];
+      pd.q = a.rows_seq + rd.q_word;
+      pd.t = pd.q + seq_words<B8>(rd.qn);
+      pd.qn = rd.qn;
+      pd.tn = rd.tn;
+      pd.run_off = rd.run_off;
+      pd.run_cap = rd.run_cap;
+    } else {
+      pd = a.pairs[pi];
+    }
+  }
   const int32_t qn = pd.qn, tn = pd.tn;
   uint8_t* rb = base + (uint32_t)(lane / RW) * rows_region_bytes(G.rows_hist_off, G.rows_steps, RW);
   lmem_t<uint64_t>* seq = (lmem_t<uint64_t>*)rb;
@@ -1864,21 +1876,21 @@ __global__ __launch_bounds__(64 * WV_WAVES, 5) void wfa_rows_kernel(WfaArgs a, W
     const uint32_t pi = pos < off[j + 1] ? a.work_list[pos] : 0xFFFFFFFFu;
     bool wider;
     if (j == 0) {
-      wider = xn == 2 && oen == 4 ? rows_pass<2, 4, -1, B8, 32>(a, G, base, lane, pi)
-            : xn == 2 && oen == 5 ? rows_pass<2, 5, -1, B8, 32>(a, G, base, lane, pi)
-                                  : rows_pass<1, 1, -1, B8, 32>(a, G, base, lane, pi);
+      wider = xn == 2 && oen == 4 ? rows_pass<2, 4, -1, B8, 32>(a, G, base, lane, pi, pos)
+            : xn == 2 && oen == 5 ? rows_pass<2, 5, -1, B8, 32>(a, G, base, lane, pi, pos)
+                                  : rows_pass<1, 1, -1, B8, 32>(a, G, base, lane, pi, pos);
     } else if (j == 1) {
-      wider = xn == 2 && oen == 4 ? rows_pass<2, 4, 1, B8, 32>(a, G, base, lane, pi)
-            : xn == 2 && oen == 5 ? rows_pass<2, 5, 1, B8, 32>(a, G, base, lane, pi)
-                                  : rows_pass<1, 1, 1, B8, 32>(a, G, base, lane, pi);
+      wider = xn == 2 && oen == 4 ? rows_pass<2, 4, 1, B8, 32>(a, G, base, lane, pi, pos)
+            : xn == 2 && oen == 5 ? rows_pass<2, 5, 1, B8, 32>(a, G, base, lane, pi, pos)
+                                  : rows_pass<1, 1, 1, B8, 32>(a, G, base, lane, pi, pos);
     } else if (j == 2) {
-      wider = xn == 2 && oen == 4 ? rows_pass<2, 4, -1, B8, 16>(a, G, base, lane, pi)
-            : xn == 2 && oen == 5 ? rows_pass<2, 5, -1, B8, 16>(a, G, base, lane, pi)
-                                  : rows_pass<1, 1, -1, B8, 16>(a, G, base, lane, pi);
+      wider = xn == 2 && oen == 4 ? rows_pass<2, 4, -1, B8, 16>(a, G, base, lane, pi, pos)
+            : xn == 2 && oen == 5 ? rows_pass<2, 5, -1, B8, 16>(a, G, base, lane, pi, pos)
+                                  : rows_pass<1, 1, -1, B8, 16>(a, G, base, lane, pi, pos);
     } else {
-      wider = xn == 2 && oen == 4 ? rows_pass<2, 4, 1, B8, 16>(a, G, base, lane, pi)
-            : xn == 2 && oen == 5 ? rows_pass<2, 5, 1, B8, 16>(a, G, base, lane, pi)
-                                  : rows_pass<1, 1, 1, B8, 16>(a, G, base, lane, pi);
+      wider = xn == 2 && oen == 4 ? rows_pass<2, 4, 1, B8, 16>(a, G, base, lane, pi, pos)
+            : xn == 2 && oen == 5 ? rows_pass<2, 5, 1, B8, 16>(a, G, base, lane, pi, pos)
+                                  : rows_pass<1, 1, 1, B8, 16>(a, G, base, lane, pi, pos);
     }
     const uint32_t lead = (uint32_t)lane % (64u / per);  // 0 on the first lane of each row
     if (j >= 2) {
@@ -1958,6 +1970,45 @@ hipError_t launch_gather_runs(const uint32_t* runs, const uint64_t* src, const u
                               uint32_t* dst, uint32_t n, hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(gather_runs_kernel, dim3((n + 3) / 4), dim3(256), 0, st, runs, src, off, dst, n);
+  return hipGetLastError();
+}
+
+// Row stage: for staged pair k (list position order), its query and target
+// from wherever its descriptor points, shifted to bit 0 exactly as the row
+// launch stages them into LDS (word j = bits of words j and j + 1 of the
+// source for j + 1 < n, the last word 0), to the staging arena from word
+// rd[k].q_word on (query, then target); a wave per pair.
+__global__ __launch_bounds__(256) void stage_rows_kernel(const PairDesc* __restrict__ pairs,
+                                                         const uint32_t* __restrict__ idx,
+                                                         const RowDesc* __restrict__ rd, uint64_t* stage,
+                                                         uint32_t n) {
+  const uint32_t k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63;
+  if (k >= n) return;
+  const PairDesc pd = pairs[idx[k]];
+  const bool b8 = (pd.flags & PF_BYTES) != 0;
+  uint64_t* dst = stage + rd[k].q_word;
+  for (int side = 0; side < 2; ++side) {
+    const uint64_t* src = side ? pd.t : pd.q;
+    const uint32_t len = (uint32_t)(side ? pd.tn : pd.qn);
+    const uint32_t b = (side ? pd.t_shift : pd.q_shift) * (b8 ? 8u : 2u);
+    const uint32_t nw = b8 ? seq_words<true>((int32_t)len) : seq_words<false>((int32_t)len);
+    if (side) dst += b8 ? seq_words<true>(pd.qn) : seq_words<false>(pd.qn);
+    for (uint32_t j = lane; j < nw; j += 64) {
+      uint64_t v = 0;
+      if (j + 1 < nw) {
+        const uint64_t lo = src[j];
+        v = b == 0 ? lo : (lo >> b) | (src[j + 1] << (64 - b));
+      }
+      dst[j] = v;
+    }
+  }
+}
+
+hipError_t launch_stage_rows(const PairDesc* pairs, const uint32_t* idx, const RowDesc* rd, uint64_t* stage,
+                             uint32_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(stage_rows_kernel, dim3((n + 3) / 4), dim3(256), 0, st, pairs, idx, rd, stage, n);
   return hipGetLastError();
 }
 
