@@ -105,7 +105,9 @@ typedef struct phx_tuning {
   uint32_t hbm_slots;        /* HBM wave tier: pairs in flight */
   uint32_t hbm_slot_mib;     /* HBM wave tier: history slot per pair when no estimate (MiB) */
   uint64_t hbm_budget_mib;   /* device scratch budget of the HBM tiers (MiB) */
-  uint32_t block256_min_len; /* qn + tn above which a pair runs a 256-thread workgroup (0 = 2048) */
+  uint32_t block256_min_len; /* qn + tn above which a pair runs a 256-thread workgroup (0 = 2048 for
+                                affine costs of 4/6/2 or 2/4/1 shape, which have a register
+                                wavefront on the workgroup tiers; 8192 otherwise) */
   uint32_t block1024_min_len;/* ... a 1024-thread workgroup (0 = 16384) */
   uint32_t heavy_permille;   /* per mille of each length bucket (largest estimates) run one tier wider (0 = 50) */
   uint32_t block_lds_ring;   /* workgroup tiers: LDS ring width in diagonals for pairs whose wavefront fits

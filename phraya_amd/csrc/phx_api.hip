@@ -880,11 +880,15 @@ void plan_group(phx_plan* P) {
   }
   // Tier of each bucket from its upper length edge and the tuning.
   static const uint64_t kBucketEdge[6] = {2048, 4096, 8192, 16384, 32768, ~0ull};
+  // 256-thread edge 0 = by cost shape: with a workgroup register wavefront
+  // pairs past 2,048 take it (several pairs per CU, bands in registers);
+  // otherwise pairs up to 8,192 stay on the HBM wave tier (a wave each).
+  const uint64_t edge256 = P->tun.block256_min_len ? P->tun.block256_min_len : P->banded ? 2048 : 8192;
   int kBucketTier[6];
   for (int b = 0; b < 6; ++b)
     kBucketTier[b] = b == 0 ? TIER_LDS
                    : kBucketEdge[b] > P->tun.block1024_min_len ? TIER_BLOCK1024
-                   : kBucketEdge[b] > P->tun.block256_min_len ? TIER_BLOCK256 : TIER_WAVE;
+                   : kBucketEdge[b] > edge256 ? TIER_BLOCK256 : TIER_WAVE;
   P->work.clear();
   P->group_tier.clear();
   P->group_slot.clear();
@@ -1657,7 +1661,7 @@ phx_ctx* phx_ctx_create(int device, uint32_t flags, const phx_tuning* tuning) {
   d.hbm_budget_mib = 65536;
   if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b > 0)
     d.hbm_budget_mib = std::max<uint64_t>(1024, (uint64_t)(0.6 * (double)free_b) >> 20);
-  d.block256_min_len = 2048;
+  d.block256_min_len = 0;  // by cost shape (plan_group)
   d.block1024_min_len = 16384;
   d.heavy_permille = 50;
   d.block_lds_ring = 1024;
