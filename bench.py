@@ -126,6 +126,7 @@ TIER_NAMES = {0: "phx::wfa_wave_kernel<1, true> (LDS-resident wave per pair)",
               2: "phx::wfa_block_kernel<1, 256, *> (256-thread workgroup per pair)",
               3: "phx::wfa_block_kernel<1, 1024, *> (1024-thread workgroup per pair)",
               4: "phx::wfa_block_kernel<1, 512, true, 16> (512-thread workgroup per pair, register wavefront)",
+              5: "phx::wfa_block_kernel<1, 64, true, 16> (one-wave workgroup per pair, register wavefront)",
               8: "phx::myers_kernel<*> (lane per pair)"}
 
 

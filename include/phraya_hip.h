@@ -127,10 +127,10 @@ typedef struct phx_tuning {
                                 estimate x band_permille / 1000 (0 = 800; at most 1000); a pair whose
                                 optimum exceeds it is re-run with the whole estimate as bound, then
                                 without one.  Results do not depend on it. */
-  uint32_t regs_narrow;      /* workgroup register tiers: a pair of the 1024-thread tier runs on the
-                                narrowest register tier whose window holds its band: 256 threads (4,096
-                                diagonals, four pairs per CU) or 512 (8,192, two per CU) (0 or 1 = on,
-                                2 = off) */
+  uint32_t regs_narrow;      /* workgroup register tiers: a pair runs on the narrowest register tier
+                                whose window holds its band: one wave (1,024 diagonals, 16 pairs per
+                                CU), 256 threads (4,096, four per CU) or 512 (8,192, two per CU)
+                                (0 or 1 = on, 2 = off, 3 = on without the one-wave tier) */
 } phx_tuning;
 
 /* Context: owns the stream, device buffers and the submission lock.
@@ -195,8 +195,8 @@ uint32_t phx_plan_last_launches(const phx_plan* plan);
  * wavefront cells written and score steps.  Instrumentation; no counterpart
  * in the reference. */
 typedef struct phx_launch_info {
-  int32_t tier;     /* 0 LDS wave, 1 HBM wave, 2 256-thread, 3 1024-thread, 4 512-thread workgroup,
-                       8 Myers */
+  int32_t tier;     /* 0 LDS wave, 1 HBM wave, 2 256-thread, 3 1024-thread, 4 512-thread,
+                       5 one-wave workgroup, 8 Myers */
   uint32_t pairs;
   uint32_t grid;
   uint32_t storage; /* workgroup tiers: 0 ring in HBM, 1 ring in LDS, 2 register wavefront;

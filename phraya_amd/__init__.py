@@ -265,7 +265,7 @@ class Plan:
 
     def last_launch_info(self) -> list[dict]:
         """Per launch of the last run: tier (0 LDS wave, 1 HBM wave, 2 256-thread,
-        3 1024-thread and 4 512-thread workgroup, 8 Myers), storage of the workgroup tiers
+        3 1024-thread, 4 512-thread and 5 one-wave workgroup, 8 Myers), storage of the workgroup tiers
         (0 HBM ring, 1 LDS ring, 2 register wavefront) and of the LDS tier (0 a pair
         per wave, 3-4 banded rows), grid, pairs, device ms, and the
         algorithmic bytes, cells and steps of its pairs."""

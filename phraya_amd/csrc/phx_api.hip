@@ -356,7 +356,7 @@ phx_tuning merge_tuning(phx_tuning d, const phx_tuning* t) {
   if (t->block_regs) d.block_regs = std::min<uint32_t>(t->block_regs, 2);
   if (t->lds_rows) d.lds_rows = std::min<uint32_t>(t->lds_rows, 3);
   if (t->band_permille) d.band_permille = std::min<uint32_t>(t->band_permille, 1000);
-  if (t->regs_narrow) d.regs_narrow = std::min<uint32_t>(t->regs_narrow, 2);
+  if (t->regs_narrow) d.regs_narrow = std::min<uint32_t>(t->regs_narrow, 3);
   return d;
 }
 bool has_cigar(int engine) { return engine != PHX_ENGINE_MYERS_DIST; }
@@ -402,9 +402,7 @@ GroupSlots group_slots(const phx_plan* P, int t, const std::vector<uint32_t>& wo
   g.per = g.hist + ((g.ring_words * 4 + 255) & ~(uint64_t)255);
   // Pairs in flight: one per wave (wave tier) or per workgroup (block tiers).
   const uint64_t max_flight = t == TIER_WAVE ? P->tun.hbm_slots
-                            : t == TIER_BLOCK256 ? (uint64_t)P->ctx->cu_count * 4 * P->tun.block_flight
-                            : t == TIER_BLOCK512 ? (uint64_t)P->ctx->cu_count * 2 * P->tun.block_flight
-                                                 : (uint64_t)P->ctx->cu_count * P->tun.block_flight;
+                                              : (uint64_t)P->ctx->cu_count * tier_per_cu(t) * P->tun.block_flight;
   g.n = std::max<uint64_t>(1, budget / g.per);
   g.n = std::min<uint64_t>(std::min<uint64_t>(g.n, max_flight), work.size());
   // Whole workgroups of waves; fixed plans round down where that keeps one
@@ -982,22 +980,28 @@ void plan_group(phx_plan* P) {
           where = 1;
         parts[where].push_back(i);
       }
-      // Register pairs of the 1024-thread tier run on the narrowest register
-      // tier whose window holds their band: 256 threads (4,096 diagonals,
-      // four pairs per CU in flight) or 512 (8,192, two per CU), whose step
-      // barriers and latencies overlap, instead of one pair per CU.
-      std::vector<uint32_t> narrow, mid;
-      if (tier == TIER_BLOCK1024 && P->tun.regs_narrow != 2) {
+      // Register pairs run on the narrowest register tier whose window holds
+      // their band: one wave (1,024 diagonals, 16 pairs per CU in flight,
+      // no cross-wave barrier), 256 threads (4,096 diagonals, four per CU)
+      // or 512 (8,192, two per CU), whose step barriers and latencies
+      // overlap, instead of one pair per CU.
+      std::vector<uint32_t> narrow, mid, tiny;
+      if (P->tun.regs_narrow != 2) {
         std::vector<uint32_t> keep;
+        const bool wave_tier = P->tun.regs_narrow != 3;
         for (uint32_t i : parts[2]) {
           const uint64_t b = P->pairs[i].bound;
-          (b && b + 4 <= reg_window(256) ? narrow : b && b + 4 <= reg_window(512) ? mid : keep).push_back(i);
+          (b && wave_tier && b + 4 <= reg_window(64)                     ? tiny
+           : tier == TIER_BLOCK1024 && b && b + 4 <= reg_window(256)     ? narrow
+           : tier == TIER_BLOCK1024 && b && b + 4 <= reg_window(512)     ? mid
+                                                                         : keep).push_back(i);
         }
         parts[2].swap(keep);
-        std::stable_sort(mid.begin(), mid.end(), [&](uint32_t a, uint32_t b) { return P->est[a] > P->est[b]; });
+        for (auto* l : {&mid, &tiny})
+          std::stable_sort(l->begin(), l->end(), [&](uint32_t a, uint32_t b) { return P->est[a] > P->est[b]; });
       }
-      for (int r = 0; r < 5; ++r) {
-        std::vector<uint32_t>& part = r < 3 ? parts[r] : r == 3 ? narrow : mid;
+      for (int r = 0; r < 6; ++r) {
+        std::vector<uint32_t>& part = r < 3 ? parts[r] : r == 3 ? narrow : r == 4 ? mid : tiny;
         if (part.empty()) continue;
         uint64_t mx = 0;
         double v = 0;
@@ -1008,7 +1012,7 @@ void plan_group(phx_plan* P) {
           v += (double)wi;
         }
         P->work.push_back(part);
-        P->group_tier.push_back(r < 3 ? tier : r == 3 ? TIER_BLOCK256 : TIER_BLOCK512);
+        P->group_tier.push_back(r < 3 ? tier : r == 3 ? TIER_BLOCK256 : r == 4 ? TIER_BLOCK512 : TIER_BLOCK64);
         P->group_rl.push_back((uint8_t)(r < 3 ? r : 2));
         P->group_slot.push_back((mx * 4 + 255) & ~(uint64_t)255);
         vol.push_back(v);
@@ -1022,16 +1026,16 @@ void plan_group(phx_plan* P) {
   // leave CUs idle when the smaller ones finish first).  Its history slots
   // are sized for its largest pair, so it is given its pairs in flight (one
   // per CU) from the budget before the other groups share the rest.
-  // Likewise the 512-thread register groups of the length buckets become
-  // one group, longest estimate first.
-  {
+  // Likewise the 512-thread and the one-wave register groups of the length
+  // buckets become one group each, longest estimate first.
+  for (const int mt : {TIER_BLOCK512, TIER_BLOCK64}) {
     std::vector<uint32_t> merged;
     uint64_t slot = 0;
     double v = 0;
     size_t first = SIZE_MAX;
     std::vector<size_t> drop;
     for (size_t g = 0; g < P->work.size(); ++g) {
-      if (P->group_tier[g] != TIER_BLOCK512) continue;
+      if (P->group_tier[g] != mt) continue;
       merged.insert(merged.end(), P->work[g].begin(), P->work[g].end());
       slot = std::max(slot, P->group_slot[g]);
       v += vol[g];
@@ -1100,10 +1104,7 @@ void plan_group(phx_plan* P) {
   // when the asks exceed the budget they are scaled down together.
   const uint64_t total = P->tun.hbm_budget_mib << 20;
   auto tier_flight = [&](int t) -> uint64_t {
-    return t == TIER_WAVE ? P->tun.hbm_slots
-         : t == TIER_BLOCK256 ? (uint64_t)ctx->cu_count * 4 * P->tun.block_flight
-         : t == TIER_BLOCK512 ? (uint64_t)ctx->cu_count * 2 * P->tun.block_flight
-                              : (uint64_t)ctx->cu_count * P->tun.block_flight;
+    return t == TIER_WAVE ? P->tun.hbm_slots : (uint64_t)ctx->cu_count * tier_per_cu(t) * P->tun.block_flight;
   };
   std::vector<double> ask(P->work.size(), 0.0);
   double ask_sum = 0;
@@ -2023,7 +2024,8 @@ int phx_plan_autotune(phx_plan* P, uint32_t max_trials, phx_tuning* best_out, do
   // Stage 2: pairs in flight of the workgroup tiers (a second workgroup per
   // CU hides memory latency when a group has pairs to spare), then of the
   // HBM wave tier.
-  if (pairs_in(TIER_BLOCK256) + pairs_in(TIER_BLOCK512) + pairs_in(TIER_BLOCK1024) > 2 * (size_t)cu) {
+  if (pairs_in(TIER_BLOCK256) + pairs_in(TIER_BLOCK512) + pairs_in(TIER_BLOCK1024) + pairs_in(TIER_BLOCK64) >
+      2 * (size_t)cu) {
     phx_tuning t = best;
     t.block_flight = 2;
     if ((rc = consider(t)) != PHX_OK) return rc;

@@ -60,11 +60,19 @@ enum Tier : int {
   TIER_WAVE = 1,     // wave per pair, ring and history in an HBM slot
   TIER_BLOCK256 = 2, // 256-thread workgroup per pair, HBM slot
   TIER_BLOCK1024 = 3,// 1024-thread workgroup per pair, HBM slot
-  TIER_BLOCK512 = 4  // 512-thread workgroup per pair, register wavefront only (two per CU)
+  TIER_BLOCK512 = 4, // 512-thread workgroup per pair, register wavefront only (two per CU)
+  TIER_BLOCK64 = 5   // one-wave workgroup per pair, register wavefront only (16 per CU)
 };
 
 // Threads per pair of a workgroup tier.
-inline int tier_threads(int t) { return t == TIER_BLOCK256 ? 256 : t == TIER_BLOCK512 ? 512 : 1024; }
+inline int tier_threads(int t) {
+  return t == TIER_BLOCK64 ? 64 : t == TIER_BLOCK256 ? 256 : t == TIER_BLOCK512 ? 512 : 1024;
+}
+// Workgroups of a workgroup tier a CU holds at once (register tiers: 121
+// VGPRs per lane, so four waves per SIMD).
+inline uint64_t tier_per_cu(int t) {
+  return t == TIER_BLOCK64 ? 16 : t == TIER_BLOCK256 ? 4 : t == TIER_BLOCK512 ? 2 : 1;
+}
 
 inline const uint8_t* acgt_table() {
   static const std::array<uint8_t, 256> t = [] {

@@ -235,8 +235,10 @@ __device__ __forceinline__ WaveStore<LDS, RL> make_store(const WfaArgs& a, const
       hbytes = pd.hist_bytes;
       S.wmask = (1u << pd.ring_log2) - 1;
     } else {
+      // (wave kernels hold WV_WAVES pairs per workgroup; a workgroup kernel,
+      // RL included the one-wave register tier, one)
       const uint32_t wv = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(threadIdx.x >> 6));
-      const uint64_t gw = NT == 64 ? (uint64_t)blockIdx.x * WV_WAVES + wv : (uint64_t)blockIdx.x;
+      const uint64_t gw = NT == 64 && !RL ? (uint64_t)blockIdx.x * WV_WAVES + wv : (uint64_t)blockIdx.x;
       S.ring = (typename S_t::ring_t*)(a.ring_g + gw * a.ring_stride);
       h = a.hist + gw * a.hist_stride;
       hbytes = a.hist_stride;
@@ -2055,7 +2057,7 @@ hipError_t launch_wfa_rows(bool b8, const WfaArgs& a, const WaveGeom& G, uint32_
   return hipGetLastError();
 }
 
-// nt = threads per pair: 256, 512 (register wavefront only) or 1024;
+// nt = threads per pair: 64 or 512 (register wavefront only), 256 or 1024;
 // storage 0: ring in HBM, 1: ring and sequences in LDS, 2: register
 // wavefront, sequences and chunk ends in LDS (G gives the layout; dynamic
 // LDS = G.per_wave_bytes).
@@ -2066,13 +2068,15 @@ hipError_t launch_block_mode(int nt, int storage, const WfaArgs& a, const WaveGe
   const size_t lds = rl ? G.per_wave_bytes : 0;
   if (storage == 2 && MODE == ENG_AFFINE) {
     // 16 chunks per wave: 16,384 diagonals for 1024 threads, 8,192 for 512,
-    // 4,096 for 256.
+    // 4,096 for 256, 1,024 for one wave.
     if (nt == 1024) hipLaunchKernelGGL((wfa_block_kernel<MODE, 1024, true, 16>), dim3(grid), dim3(1024), lds, st, a, G);
     else if (nt == 512) hipLaunchKernelGGL((wfa_block_kernel<MODE, 512, true, 16>), dim3(grid), dim3(512), lds, st, a, G);
-    else hipLaunchKernelGGL((wfa_block_kernel<MODE, 256, true, 16>), dim3(grid), dim3(256), lds, st, a, G);
+    else if (nt == 256) hipLaunchKernelGGL((wfa_block_kernel<MODE, 256, true, 16>), dim3(grid), dim3(256), lds, st, a, G);
+    else if (nt == 64) hipLaunchKernelGGL((wfa_block_kernel<MODE, 64, true, 16>), dim3(grid), dim3(64), lds, st, a, G);
+    else return hipErrorInvalidValue;
     return hipGetLastError();
   }
-  if (nt == 512) return hipErrorInvalidValue;  // register wavefront only
+  if (nt == 512 || nt == 64) return hipErrorInvalidValue;  // register wavefront only
   if (nt == 1024) {
     if (rl) hipLaunchKernelGGL((wfa_block_kernel<MODE, 1024, true>), dim3(grid), dim3(1024), lds, st, a, G);
     else hipLaunchKernelGGL((wfa_block_kernel<MODE, 1024, false>), dim3(grid), dim3(1024), 0, st, a, G);
