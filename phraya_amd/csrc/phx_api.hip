@@ -1365,7 +1365,10 @@ phx_plan* plan_setup(phx_plan* P, const std::vector<const uint8_t*>& qh,
   if (plan_alloc(P, P->d_runs, P->runs_total * 4 + 4) != PHX_OK) return fail_plan("runs");
   if (plan_alloc(P, P->d_work, (size_t)std::max<uint32_t>(n, 1) * 4) != PHX_OK) return fail_plan("work");
   if (plan_alloc(P, P->d_ctr, 2 * kMaxGroups * sizeof(uint32_t)) != PHX_OK) return fail_plan("counters");
-  row_stage_caps(P);
+  // The row stage pays off over repeated runs of one plan; a one-shot plan
+  // (it borrows the context's cached buffers) runs once, so it skips the
+  // extra upload and copy kernel and its row launches read PairDesc.
+  if (!P->cached) row_stage_caps(P);
   // The stage is an optimisation: a fixed region with no room for it keeps
   // the batch (its row launches read PairDesc).
   if (P->fixed() && P->rstage_recs &&
