@@ -92,6 +92,27 @@ def test_mixed_all_200k_pairs():
     say(f"mixed: all {w.n} pairs bit-checked and optimal ({cells / 1e12:.2f} T DP cells)")
 
 
+def test_mixed_default_costs_other_seed():
+    """The same mixed-length regime under AffineCosts::default() (2/4/1,
+    whose score step is 1 and whose register wavefront keeps five m frames)
+    on another read seed: 100,000 pairs, every one checked optimal, and the
+    one-wave, 256-, 512- and 1024-thread register tiers all in the run."""
+    import phraya_amd as px
+    from phraya_amd import synth
+    w = synth.workload("mixed", n=100_000, seed=12)
+    pb = px.PackedBatch(w.q_seq, w.q_off, w.t_seq, w.t_off, np.zeros(w.n, np.uint64),
+                        np.zeros(w.n, np.uint64))
+    plan = px.Plan(pb, px.ENGINE_AFFINE, px.AffineCosts(2, 4, 1))
+    plan.run()
+    res = plan.results()
+    regs = {li["tier"] for li in plan.last_launch_info() if li["storage"] == 2}
+    plan.close()
+    say(f"mixed 2/4/1: register tiers {sorted(regs)}")
+    assert {2, 3, 4, 5} <= regs, f"register tiers {sorted(regs)}"
+    check_all("mixed 2/4/1", w.q_seq, w.q_off, w.t_seq, w.t_off, res.aln, res.runs, res.run_off, (2, 4, 1),
+              chunks=4)
+
+
 def test_myers150_all_1m_pairs():
     """C2 in full: every one of the 1,000,000 distance + end-column results
     equals the oracle's forward-only Myers (myers_extend without backtrace)."""
