@@ -864,8 +864,22 @@ void plan_group(phx_plan* P) {
   // (the rest, slot = its largest estimate), so a few outliers do not
   // shrink the slots in flight of the whole bucket.  A pair that outgrows
   // its slot still completes (exact re-run).
+  // A global-branch LDS-tier pair that no banded row takes and whose
+  // estimated history (or step count) the LDS tier cannot hold goes to the
+  // workgroup tiers (register shapes only; the estimate runs ~1.4x high, so
+  // a pair near the limit may have fitted).  Banded-row pairs stay (a row
+  // holds up to |kend| + 31 steps, which the estimate's constant terms
+  // overstate for short pairs), and so do fitting pairs (the estimate
+  // prices the whole length difference, which the fitting end does not pay).
   auto lds_ok = [&](uint32_t i) {
-    return P->qlen[i] + P->tlen[i] <= kWaveMaxLen && P->ring_slots <= kWaveRingMax;
+    if (P->qlen[i] + P->tlen[i] > kWaveMaxLen || P->ring_slots > kWaveRingMax) return false;
+    if (!P->banded || P->est.size() != P->n || P->est[i] <= 0) return true;
+    if (P->rows_ok.size() == P->n && P->rows_ok[i]) return true;
+    const PairDesc& pd = P->pairs[i];
+    if (!(pd.flags & PF_GLOBAL) && pd.tn > pd.qn + pd.qn / 2 + 10) return true;
+    const int64_t e = P->est[i];
+    const uint32_t cells = lds_geometry(0, lds_seq_words(pd), P->tun.wave_lds_kib).cells_cap;
+    return e < (int64_t)kWaveSteps && exact_hist_words((int32_t)e, pd.qn, pd.tn) <= cells;
   };
   auto words_of = [&](uint32_t i) {
     return exact_hist_words(P->est[i], P->pairs[i].qn, P->pairs[i].tn);
@@ -1334,6 +1348,17 @@ phx_plan* plan_setup(phx_plan* P, const std::vector<const uint8_t*>& qh,
                           (int64_t)std::abs((int64_t)P->qlen[i] - (int64_t)P->tlen[i]) + 15;
       });
     }
+    // With a workgroup register wavefront, the global-branch LDS-tier pairs
+    // that no banded row takes get an estimate too: plan_group
+    // sends those whose estimated history outgrows the LDS tier to the
+    // register tiers instead of finding that out by a failed attempt and a
+    // serial re-run.
+    if (banded)
+      for (uint32_t i : P->order) {
+        const PairDesc& pd = P->pairs[i];
+        const bool fitting = !(pd.flags & PF_GLOBAL) && pd.tn > pd.qn + pd.qn / 2 + 10;
+        if (lds_ok(i) && !P->rows_ok[i] && !fitting && pd.qn && pd.tn) wide.push_back(i);
+      }
     parallel_for(wide.size(), [&](size_t k) {
       const uint32_t i = wide[k];
       P->est[i] = estimate_steps_seq(P->engine, P->costs, qh[i], P->qlen[i], th[i], P->tlen[i]);
